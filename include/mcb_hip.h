@@ -1,0 +1,188 @@
+/*
+ * mcb_hip.h - C ABI of the gfx950 device library (libmcb_hip.so).
+ *
+ * Two forms of the de Pina support-vector hot path live behind this boundary.
+ *
+ * 1. The reference-compatible form.  The five extern "C" entry points of gpu/common.cuh:18-27
+ *    keep their names and signatures, and the C++ class gpu_struct (gpu/gpu_struct.cuh:15-84)
+ *    becomes the opaque handle mcb_gpu with one function per method.  The trees, the coordinate
+ *    map and the parity output keep the compressed_trees chunk layout of
+ *    include/compressed_trees.h:16-26 (per chunk row: tree_rows u32[chunk*(n+1)],
+ *    tree_cols u32[chunk*n], edge_offset i32[chunk*n], precompute_value u32[chunk*n]).
+ *    A host driver written against the reference (mcb/mcb_cycle_deg_two_remove.cpp) can call it
+ *    through a thin shim.
+ *
+ * 2. The native form (mcb_solver_*).  Trees, candidates and every support vector stay resident in
+ *    HBM; each phase runs (a) tree-path parity fused with (b) the minimum-rank odd-candidate
+ *    search, the cycle-vector construction, and (c) the GF(2) support-vector update, without a
+ *    host round trip.  Its inputs are the arrays of libmcb_host.so (include/mcb_host.h).
+ *
+ * Status-returning functions return MCB_OK (0) or a negative MCB_E_* code from mcb_host.h;
+ * mcb_hip_last_error() gives the calling thread's last message.  The five reference names keep
+ * the reference's behaviour on a runtime error: print and exit (include/utils.h:36-47).
+ */
+#ifndef MCB_HIP_H
+#define MCB_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "mcb_host.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------------------------
+ * Reference entry points (gpu/common.cuh:18-27, gpu/common.cu:9-67)
+ * ------------------------------------------------------------------------------------------- */
+
+/* gpu/common.cu:9-12 - selects the current device and caches its properties. */
+void init_cuda(void);
+
+/* gpu/common.cu:14-30 - min(end - start, number of compute units). */
+size_t configure_grid(int start, int end);
+
+/* gpu/common.cu:32-40 - chunk*nodes uint32 of pinned host memory (not zeroed). */
+unsigned *allocate_pinned_memory(int chunk, int nodes);
+
+/* gpu/common.cu:42-46 */
+void free_pinned_memory(unsigned *ptr);
+
+/* gpu/common.cu:48-67 - largest chunk of roots whose trees fit in device memory next to the
+ * coordinate map and the support vector, with nstream chunk slots. */
+size_t calculate_chunk_size(size_t num_nodes, size_t num_edges, size_t size_vector, size_t nstream);
+
+/* ---------------------------------------------------------------------------------------------
+ * gpu_struct as an opaque handle (gpu/gpu_struct.cuh:15-84, gpu/gpu_struct.cu)
+ * ------------------------------------------------------------------------------------------- */
+
+typedef struct mcb_gpu mcb_gpu;
+
+/* gpu_struct::gpu_struct (gpu_struct.cuh:44-59): num_edges = 2m', num_non_tree_edges = N,
+ * size_vector = W, original_nodes = n', fvs_size = F, chunk_size roots per chunk, nstreams =
+ * number of chunks (the reference passes ceil(F/chunk_size)).  Every chunk gets a device slot when
+ * they fit in device memory; otherwise min(32, chunks) slots are allocated and trees must be
+ * streamed per phase (multiple_transfer). */
+int mcb_gpu_create(int num_edges, int num_non_tree_edges, int size_vector, int original_nodes, int fvs_size,
+                   int chunk_size, int nstreams, mcb_gpu **out);
+
+/* gpu_struct::initialize_memory (gpu_struct.cu:77-102): uploads the coordinate map (num_edges
+ * int32, -1 for global tree edges) and, when every chunk has a slot, every chunk's tree_cols and
+ * edge_offset.  tree_rows is accepted for layout compatibility; the kernels only need the parent
+ * position of each position, which must precede it (tree_cols[0] = 0xFFFFFFFF for the root). */
+int mcb_gpu_initialize_memory(mcb_gpu *g, const int32_t *non_tree_edges_array, const uint32_t *const *tree_rows,
+                              const uint32_t *const *tree_cols, const int32_t *const *edge_offset, int num_rows);
+
+/* gpu_struct::copy_support_vector (gpu_struct.cu:104-114): S (size_vector uint64 words) to device.
+ * *ms (optional) receives the elapsed time in milliseconds. */
+int mcb_gpu_copy_support_vector(mcb_gpu *g, const uint64_t *elements, float *ms);
+
+/* gpu_struct::process_shortest_path (gpu_struct.cu:169-193): for every chunk, parity of every
+ * tree position for the current S (0/1 per position, the root 0), copied into
+ * precompute_value[chunk].  With multiple_transfer the chunk's trees are sent first. */
+int mcb_gpu_process_shortest_path(mcb_gpu *g, const uint32_t *const *tree_rows, const uint32_t *const *tree_cols,
+                                  const int32_t *const *edge_offset, uint32_t *const *precompute_value, int num_rows,
+                                  int multiple_transfer, float *ms);
+
+/* gpu_struct::fetch (gpu_struct.cu:116-130): copies every resident chunk's parity to the host. */
+int mcb_gpu_fetch(mcb_gpu *g, uint32_t *const *precompute_value, int num_rows, float *ms);
+
+/* gpu_struct::calculate_memory (gpu_struct.cu:52-75), in MB. */
+int mcb_gpu_calculate_memory(mcb_gpu *g, double *total_mb, double *static_mb, double *variable_mb);
+
+/* gpu_struct::clear_memory (gpu_struct.cu:19-28) and destruction. */
+void mcb_gpu_destroy(mcb_gpu *g);
+
+/* ---------------------------------------------------------------------------------------------
+ * Native device-resident phase loop
+ * ------------------------------------------------------------------------------------------- */
+
+typedef struct mcb_solver mcb_solver;
+
+/* Device-ready problem (the arrays of include/mcb_host.h, all in host memory; copied at create). */
+typedef struct {
+    int32_t num_roots;               /* F */
+    int32_t num_non_tree;            /* N */
+    int64_t num_candidates;          /* K */
+    int64_t num_positions;           /* P = tree_offset[F] */
+    const int64_t *tree_offset;      /* [F+1] */
+    const uint32_t *parent_pos;      /* [P] parent position within the root's layout, root 0xFFFFFFFF */
+    const int32_t *pos_coord;        /* [P] coordinate of the edge to the parent or -1 */
+    const uint32_t *cand_root;       /* [K] rank order */
+    const uint32_t *cand_pos_u;      /* [K] */
+    const uint32_t *cand_pos_v;      /* [K] */
+    const int32_t *cand_coord;       /* [K] */
+    const int64_t *root_cand_offset; /* [F+1] */
+    const uint32_t *root_cand_rank;  /* [K] ascending within each root */
+} mcb_solver_desc;
+
+typedef struct {
+    int32_t store_cycles;     /* 1: keep every phase's cycle vector (N*W words) for mcb_solver_cycle_vectors */
+    int32_t waves_per_block;  /* parity/search kernel; 0 = automatic */
+    int32_t max_blocks;       /* 0 = automatic (resident blocks on every compute unit) */
+    int32_t timing;           /* 1: time each kernel launch with HIP events on the solver's stream */
+    int32_t update_mode;      /* 0: per-phase dot + XOR kernels (reference schedule) */
+} mcb_solver_options;
+
+void mcb_solver_options_init(mcb_solver_options *opt);
+
+int mcb_solver_create(const mcb_solver_desc *desc, const mcb_solver_options *opt, mcb_solver **out);
+
+/* Sets every support vector back to its unit vector and the phase counter to 0. */
+int mcb_solver_reset(mcb_solver *s);
+
+/* Enqueues the next `phases` phases on the solver's stream (asynchronous). */
+int mcb_solver_run(mcb_solver *s, int32_t phases);
+
+/* Waits for the stream; returns MCB_E_NOT_FOUND if a phase found no odd candidate. */
+int mcb_solver_sync(mcb_solver *s);
+
+/* Phases enqueued so far. */
+int32_t mcb_solver_phase(const mcb_solver *s);
+
+/* Rank of the candidate chosen in each of the first `count` phases. */
+int mcb_solver_selected(mcb_solver *s, int32_t *out, int32_t count);
+
+/* Support vectors as used in their phases, row-major N x W uint64 (row e = S_e). */
+int mcb_solver_support_vectors(mcb_solver *s, uint64_t *out);
+
+/* Cycle vectors of the first `count` phases, row-major count x W (needs store_cycles). */
+int mcb_solver_cycle_vectors(mcb_solver *s, uint64_t *out, int32_t count);
+
+/* Parity of every tree position for a given support vector S (W words), without searching:
+ * out[p] = XOR of S bits along the path root -> position p (kernel (a) alone). */
+int mcb_solver_parity(mcb_solver *s, const uint64_t *S, uint8_t *out);
+
+/* Kernel ids for timing. */
+enum { MCB_K_PARITY_SEARCH = 0, MCB_K_CYCLE = 1, MCB_K_DOT = 2, MCB_K_XOR = 3, MCB_K_COUNT = 4 };
+
+/* Accumulated event time (ms) and launch count of one kernel since the last reset (timing=1). */
+int mcb_solver_kernel_time(mcb_solver *s, int kernel, double *total_ms, int64_t *launches);
+
+/* Device bytes the solver holds. */
+int64_t mcb_solver_device_bytes(const mcb_solver *s);
+
+void mcb_solver_destroy(mcb_solver *s);
+
+/* ---------------------------------------------------------------------------------------------
+ * Utilities
+ * ------------------------------------------------------------------------------------------- */
+
+/* Number of visible devices (0 when no GPU); never aborts. */
+int mcb_hip_device_count(void);
+
+/* Name and compute-unit count of the current device. */
+int mcb_hip_device_info(char *name, int name_len, int *compute_units, int64_t *total_mem);
+
+/* Copy kernel of `bytes` bytes between two device buffers, timed with events: the measured
+ * bandwidth ceiling used beside the 8 TB/s HBM peak.  Returns GB/s in *gbps. */
+int mcb_hip_copy_bandwidth(int64_t bytes, int iters, double *gbps);
+
+const char *mcb_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCB_HIP_H */

@@ -1,0 +1,144 @@
+/*
+ * mcb_host.h - C ABI of the host-side preprocessing library (libmcb_host.so).
+ *
+ * The host library turns an undirected integer-weighted graph into the inputs of the
+ * de Pina support-vector phase loop: the reduced multigraph, the feedback-vertex-set
+ * roots, the spanning-tree coordinates of the cycle space, one shortest-path tree per
+ * root laid out in BFS-level order, and the candidate cycles in their global rank order.
+ * It replaces the host half of the reference's driver
+ *   mcb/mcb_cycle_deg_two_remove.cpp:68-267   (GPU driver: read, reduce, FVS, trees, candidates)
+ *   mcb/mcb_cycle.cpp:76-203                  (CPU baseline: same without degree-two removal)
+ * No function here touches a GPU; every call is thread-safe on distinct problems.
+ *
+ * All functions return MCB_OK (0) or a negative MCB_E* code; mcb_host_last_error()
+ * gives a message for the calling thread.
+ */
+#ifndef MCB_HOST_H
+#define MCB_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MCB_OK = 0,
+    MCB_E_IO = -1,          /* file missing or unreadable (FileReader.h:29-32 exits 1) */
+    MCB_E_FORMAT = -2,      /* unsupported Matrix-Market banner/size line (FileReader.h:34-50) */
+    MCB_E_INVALID = -3,     /* bad argument */
+    MCB_E_GRAPH = -4,       /* graph the path cannot handle (disconnected, no vertex of degree > 2 ...) */
+    MCB_E_NOMEM = -5,
+    MCB_E_DEVICE = -6,      /* HIP runtime error (device library only) */
+    MCB_E_STATE = -7,       /* call out of order */
+    MCB_E_NOT_FOUND = -8    /* no odd candidate in a phase (cannot happen for a valid candidate set) */
+};
+
+typedef struct mcb_problem mcb_problem;
+
+/* Options of mcb_problem_build. */
+typedef struct {
+    int reduce_degree_two;  /* 1: contract degree-two chains and root the spanning tree at the first
+                               vertex of degree > 2 (mcb_cycle_deg_two_remove.cpp, mcb_non_cpu_rd.cpp);
+                               0: keep the graph and root it at vertex 0 (mcb_cycle.cpp:110-113) */
+    int prune_trees;        /* 1: keep in each root's position layout only the root-to-endpoint paths
+                               of that root's candidates (exact: parity is needed only at endpoints) */
+    int keep_vertex_trees;  /* 1: also keep, per root, the parent edge of every vertex (F*n' ints),
+                               the input of the CPU phase loop and of the reference tree layout */
+    int num_threads;        /* OpenMP threads for the per-root work (<=0: library default) */
+    const int32_t *fvs_order; /* optional: vertex ids of the reduced graph in the order the reverse-delete
+                               pass tries them (NULL: the reference order, degree asc then id desc) */
+    int fvs_order_len;
+} mcb_build_options;
+
+void mcb_build_options_init(mcb_build_options *opt);
+
+/* Scalar facts about a built problem. */
+typedef struct {
+    int32_t input_nodes;        /* n of the input file */
+    int32_t input_edges;        /* m of the input file (undirected) */
+    int32_t single_cycle;       /* 1 if every vertex has degree two (mcb_cycle_deg_two_remove.cpp:104-113) */
+    int32_t nodes_removed;      /* degree-two vertices contracted away */
+    int32_t source_vertex;      /* spanning-tree root (an id of the reduced graph, see O3) */
+    int32_t nodes;              /* n' of the reduced multigraph */
+    int32_t directed_edges;     /* 2m' (both directions of every edge) */
+    int32_t num_non_tree;       /* N = m' - n' + 1, the cycle-space dimension */
+    int32_t words;              /* W = ceil(N / 64) */
+    int32_t fvs_size;           /* F, number of roots */
+    int64_t num_candidates;     /* K */
+    int64_t positions;          /* total positions over all root layouts */
+    int32_t max_positions;      /* largest single root layout */
+    int32_t max_depth;          /* deepest level over all root layouts */
+    int64_t input_total_weight; /* sum of input edge weights (int64) */
+} mcb_problem_info;
+
+/* Arrays a problem exposes (mcb_problem_array). Element type in brackets, length after it. */
+enum {
+    MCB_A_ROWS = 0,         /* [u32, 2m']  reduced multigraph, CSR order by (row, col) */
+    MCB_A_COLS,             /* [u32, 2m'] */
+    MCB_A_WEIGHTS,          /* [i32, 2m'] */
+    MCB_A_REVERSE,          /* [u32, 2m']  offset of the opposite direction */
+    MCB_A_ROW_OFFSETS,      /* [u32, n'+1] */
+    MCB_A_FVS_RANK,         /* [i32, n']   root index of a vertex, -1 if not a root (FVS.cpp:216-227) */
+    MCB_A_ROOTS,            /* [u32, F]    vertex of each root index */
+    MCB_A_NT_EDGES,         /* [u32, N]    edge offset (row<col direction) of each coordinate */
+    MCB_A_NT_MAP,           /* [i32, 2m']  coordinate of an edge offset, both directions, -1 for tree edges */
+    MCB_A_CAND_ROOT,        /* [u32, K]    root index of each candidate, in rank order */
+    MCB_A_CAND_EDGE,        /* [u32, K]    non-tree edge offset of each candidate */
+    MCB_A_CAND_WEIGHT,      /* [i32, K]    cycle weight d(u)+d(v)+w(e) */
+    MCB_A_CAND_POS_U,       /* [u32, K]    position of rows[edge] in the root's layout */
+    MCB_A_CAND_POS_V,       /* [u32, K]    position of cols[edge] in the root's layout */
+    MCB_A_CAND_COORD,       /* [i32, K]    coordinate of the candidate's own edge or -1 */
+    MCB_A_TREE_OFFSET,      /* [i64, F+1]  first position of each root's layout */
+    MCB_A_PARENT_POS,       /* [u32, P]    parent position (root: 0xFFFFFFFF), relative to the root's layout */
+    MCB_A_POS_COORD,        /* [i32, P]    coordinate of the edge to the parent, -1 if a tree edge or root */
+    MCB_A_ROOT_CAND_OFFSET, /* [i64, F+1]  candidates of each root, ranks ascending */
+    MCB_A_ROOT_CAND_RANK,   /* [u32, K] */
+    MCB_A_VERTEX_PARENT,    /* [i32, F*n'] parent edge of each vertex per root (-1 at the root); keep_vertex_trees */
+    MCB_A_POS_VERTEX,       /* [u32, P]    vertex at each position (kept with keep_vertex_trees) */
+    MCB_A_COUNT
+};
+
+int mcb_problem_build_mtx(const char *path, const mcb_build_options *opt, mcb_problem **out);
+
+/* Edges as 0-based endpoint pairs; each undirected edge once. */
+int mcb_problem_build_edges(int32_t nodes, int32_t edges, const int32_t *u, const int32_t *v,
+                            const int32_t *w, const mcb_build_options *opt, mcb_problem **out);
+
+int mcb_problem_get_info(const mcb_problem *p, mcb_problem_info *info);
+
+/* Pointer to an internal array (valid until mcb_problem_free) and its element count. */
+int mcb_problem_array(const mcb_problem *p, int which, const void **data, int64_t *count);
+
+/* The reference's per-root tree layout for root index r, exactly as
+ * dijkstra::fill_tree_edges (include/Dijkstra.h:151-203) writes it: rows (level offsets, n'+1),
+ * cols (parent position, root 0xFFFFFFFF), edge_offset (edge id, root -1), nodes_index (vertex -> position).
+ * Recomputes that root's shortest-path tree. */
+int mcb_problem_reference_layout(const mcb_problem *p, int32_t r, uint32_t *rows, uint32_t *cols,
+                                 int32_t *edge_offset, uint32_t *nodes_index);
+
+void mcb_problem_free(mcb_problem *p);
+
+/* Reverse-delete feedback vertex set of the reduced multigraph (FVS.cpp:166-227), given the order in
+ * which vertices are tried (NULL: degree asc, id desc).  Writes rank per vertex or -1; returns F or <0. */
+int mcb_fvs_reverse_delete(int32_t nodes, int32_t directed_edges, const uint32_t *row_offsets,
+                           const uint32_t *cols, const int32_t *order, int32_t *fvs_rank);
+
+/* 1 if the multigraph induced on vertices with in_set[v]==0 has a cycle (parallel edges count),
+ * as FVS::test_fvs (FVS.cpp:145-164); 0 if it is a forest. */
+int mcb_induced_has_cycle(int32_t nodes, const uint32_t *row_offsets, const uint32_t *cols,
+                          const uint8_t *in_set);
+
+/* Biconnected components of a simple undirected graph (edge list, 0-based).  Writes, per edge, its
+ * component id (components numbered in order of completion); returns the count or <0. */
+int mcb_biconnected_components(int32_t nodes, int32_t edges, const int32_t *u, const int32_t *v,
+                               int32_t *edge_component);
+
+const char *mcb_host_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCB_HOST_H */

@@ -1,0 +1,1105 @@
+// gfx950 kernels and device runtime of the de Pina support-vector phase loop.
+//
+// Data layout in HBM (native form)
+//   trees      parent_pos u32[P], pos_coord i32[P]: every root's shortest-path tree in BFS-level
+//              order (parent before child), one contiguous segment per root (tree_off i64[F+1]).
+//              pos_coord holds the cycle-space coordinate of the edge to the parent (-1: none),
+//              so the parity pass needs no gather through the 2m'-entry coordinate map.
+//   candidates rank-indexed (cand_*) for the cycle kernel, and root-major copies ordered by rank
+//              (rc_*) for the search, 16 B per candidate.
+//   support    word-major matrix Sw u64[W][Npad]: Sw[w][j] = word w of S_j without its own bit j
+//              (bit j of S_j is always 1).  A phase-e update touches words 0..e/64 of columns
+//              j > e, coalesced across j.  S_e itself is also kept contiguous (double buffer) for
+//              the parity kernel.
+//
+// Kernels per phase e
+//   k_parity_search  (a)+(b): one wave per root; parity of 64 consecutive positions per step,
+//                    kept as bits in LDS; S_e in LDS; then the root's candidates in rank order
+//                    until the first odd one -> atomicMin into best[e].  Roots are visited in
+//                    order of their lowest candidate rank, so a wave stops as soon as its next
+//                    root cannot beat the best rank found so far.
+//   k_cycle          the chosen candidate's cycle vector C_e (walk of the two tree paths with the
+//                    root's parent array staged in LDS) and the list of its nonzero words below e.
+//   k_dot            p_j = <C_e, S_j> for j > e, reading only C_e's nonzero words.
+//   k_xor            S_j ^= S_e for the j with p_j = 1, one thread per (word, j); also writes S_{e+1}
+//                    contiguously for the next phase.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "mcb_hip.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(call)                                                                            \
+    do {                                                                                         \
+        hipError_t _e = (call);                                                                  \
+        if (_e != hipSuccess)                                                                    \
+            return fail(MCB_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(_e));       \
+    } while (0)
+
+// The reference names abort on a runtime error like CudaError (include/utils.h:36-47).
+#define HIP_OR_EXIT(call)                                                                        \
+    do {                                                                                         \
+        hipError_t _e = (call);                                                                  \
+        if (_e != hipSuccess) {                                                                  \
+            std::fprintf(stderr, "Error :%s : %s : line No = %d\n", hipGetErrorString(_e),       \
+                         __FILE__, __LINE__);                                                    \
+            std::exit(static_cast<int>(_e));                                                     \
+        }                                                                                        \
+    } while (0)
+
+constexpr uint32_t kNoParent = 0xFFFFFFFFu;
+constexpr uint32_t kNoRank = 0xFFFFFFFFu;
+constexpr int kWave = 64;
+constexpr int kLdsLimit = 64 * 1024;
+constexpr int kUnroll = 4;
+
+__device__ __forceinline__ uint32_t bit_of(const uint64_t *s, int32_t c) {
+    return c >= 0 ? static_cast<uint32_t>((s[c >> 6] >> (c & 63)) & 1ull) : 0u;
+}
+
+__device__ __forceinline__ uint32_t pbit(const uint64_t *p, uint32_t pos) {
+    return static_cast<uint32_t>((p[pos >> 6] >> (pos & 63)) & 1ull);
+}
+
+// Parity of 64 consecutive positions [wb, wb+64) of one tree, given their parent position and
+// the S bit of their parent edge.  Parents before wb are final in P (LDS bits); parents inside
+// the window are resolved level by level with ballots (in BFS-level order a window rarely spans
+// more than two levels).  Returns this lane's parity; lane 0 stores the 64-bit word.
+__device__ __forceinline__ uint32_t window_parity(uint64_t *P, uint32_t wb, uint32_t par, uint32_t b, int lane) {
+    const bool has_parent = par != kNoParent;
+    const bool inwin = has_parent && par >= wb;
+    uint32_t val = (has_parent && !inwin) ? (b ^ pbit(P, par)) : 0u;
+    bool resolved = !inwin;
+    uint64_t pending = __ballot(!resolved);
+    while (pending) {
+        const uint64_t rm = __ballot(resolved);
+        const uint64_t pm = __ballot(val);
+        if (!resolved) {
+            const uint32_t k = par - wb;
+            if ((rm >> k) & 1ull) {
+                val = b ^ static_cast<uint32_t>((pm >> k) & 1ull);
+                resolved = true;
+            }
+        }
+        pending = __ballot(!resolved);
+    }
+    const uint64_t word = __ballot(val);
+    if (lane == 0) P[wb >> 6] = word;
+    __builtin_amdgcn_wave_barrier();
+    return val;
+}
+
+// Parity of every position of one tree segment into the wave's LDS bit array P.  The next
+// kUnroll windows' parents and coordinates are loaded before any of them is resolved.
+template <typename WriteFn>
+__device__ __forceinline__ void tree_parity(const uint32_t *pp, const int32_t *pc, uint32_t npos, const uint64_t *S,
+                                            uint64_t *P, int lane, WriteFn write) {
+    for (uint32_t base = 0; base < npos; base += kWave * kUnroll) {
+        uint32_t par[kUnroll];
+        int32_t co[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint32_t pos = base + u * kWave + lane;
+            const bool v = pos < npos;
+            par[u] = v ? pp[pos] : kNoParent;
+            co[u] = v ? pc[pos] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint32_t wb = base + u * kWave;
+            if (wb >= npos) break;
+            const uint32_t val = window_parity(P, wb, par[u], bit_of(S, co[u]), lane);
+            write(wb + lane, val);
+        }
+    }
+}
+
+struct SearchArgs {
+    const int64_t *tree_off;
+    const uint32_t *parent_pos;
+    const int32_t *pos_coord;
+    const uint32_t *root_order;     // roots by ascending lowest candidate rank
+    const uint32_t *root_min_rank;  // indexed by root
+    const int64_t *rc_off;          // per-root candidate segment
+    const uint32_t *rc_pu, *rc_pv, *rc_rank;
+    const int32_t *rc_coord;
+    const uint64_t *S;              // contiguous S_e
+    uint32_t *best;                 // &best[e]
+    uint8_t *parity_out;            // parity-only mode: one byte per position
+    int32_t F, W, p_words, total_waves;
+};
+
+// (a)+(b).  SEARCH = false computes the parity of every position of every root into parity_out.
+template <bool S_IN_LDS, bool SEARCH>
+__global__ __launch_bounds__(256) void k_parity_search(SearchArgs a) {
+    extern __shared__ uint64_t lds[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wib = threadIdx.x / kWave;
+    const int wpb = blockDim.x / kWave;
+    const uint64_t *Sx = a.S;
+    if (S_IN_LDS) {
+        for (int i = threadIdx.x; i < a.W; i += blockDim.x) lds[i] = a.S[i];
+        __syncthreads();
+        Sx = lds;
+    }
+    uint64_t *P = lds + (S_IN_LDS ? a.W : 0) + static_cast<size_t>(wib) * a.p_words;
+    for (int idx = blockIdx.x * wpb + wib; idx < a.F; idx += a.total_waves) {
+        const uint32_t r = SEARCH ? a.root_order[idx] : static_cast<uint32_t>(idx);
+        uint32_t bestk = kNoRank;
+        if (SEARCH) {
+            bestk = __hip_atomic_load(a.best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (a.root_min_rank[r] >= bestk) break;  // later roots start even higher
+        }
+        const int64_t off = a.tree_off[r];
+        const uint32_t npos = static_cast<uint32_t>(a.tree_off[r + 1] - off);
+        if (SEARCH) {
+            tree_parity(a.parent_pos + off, a.pos_coord + off, npos, Sx, P, lane, [](uint32_t, uint32_t) {});
+        } else {
+            uint8_t *out = a.parity_out + off;
+            tree_parity(a.parent_pos + off, a.pos_coord + off, npos, Sx, P, lane, [&](uint32_t pos, uint32_t v) {
+                if (pos < npos) out[pos] = static_cast<uint8_t>(v);
+            });
+            continue;
+        }
+        // Candidates of this root, ranks ascending: the first odd one is this root's best.
+        const int64_t cb = a.rc_off[r], ce = a.rc_off[r + 1];
+        for (int64_t k0 = cb; k0 < ce; k0 += kWave) {
+            const int64_t k = k0 + lane;
+            const bool v = k < ce;
+            const uint32_t rk = v ? a.rc_rank[k] : kNoRank;
+            const bool live = rk < bestk;
+            uint32_t odd = 0;
+            if (live) odd = bit_of(Sx, a.rc_coord[k]) ^ pbit(P, a.rc_pu[k]) ^ pbit(P, a.rc_pv[k]);
+            const uint64_t m = __ballot(odd != 0);
+            if (m) {
+                const int first = __ffsll(static_cast<unsigned long long>(m)) - 1;
+                const uint32_t win = __shfl(rk, first);
+                if (lane == 0) atomicMin(a.best, win);
+                break;
+            }
+            if (__ballot(v && !live)) break;  // the rest of the list ranks no better
+            if (((k0 - cb) & (kWave * 8 - 1)) == 0)
+                bestk = __hip_atomic_load(a.best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+struct CycleArgs {
+    const uint32_t *best;  // &best[e]
+    const uint32_t *cand_root, *cand_pu, *cand_pv;
+    const int32_t *cand_coord;
+    const int64_t *tree_off;
+    const uint32_t *parent_pos;
+    const int32_t *pos_coord;
+    uint64_t *C;        // W words
+    uint32_t *D;        // nonzero words of C below e
+    uint32_t *Dcount;
+    uint64_t *C_store;  // optional row e of the stored cycle vectors
+    uint32_t *status;
+    int32_t e, W, parent_cap;
+};
+
+constexpr int kPathCap = 512;
+
+// Cycle vector of the chosen candidate: coordinates of the edges on root -> u, root -> v and the
+// candidate's own edge (the two paths are disjoint below the root, so setting bits equals XOR).
+__global__ __launch_bounds__(256) void k_cycle(CycleArgs a) {
+    extern __shared__ uint64_t lds[];
+    uint64_t *Cl = lds;                                         // W words
+    uint32_t *par_l = reinterpret_cast<uint32_t *>(lds + a.W);  // parent_cap entries
+    __shared__ uint32_t path[2 * kPathCap];
+    __shared__ uint32_t cur[2];
+    __shared__ int32_t cnt[2];
+    __shared__ int32_t more;
+    __shared__ uint32_t dcount;
+    const int tid = threadIdx.x;
+    const uint32_t rank = *a.best;
+    for (int i = tid; i < a.W; i += blockDim.x) Cl[i] = 0;
+    if (tid == 0) dcount = 0;
+    if (rank == kNoRank) {
+        __syncthreads();
+        for (int i = tid; i < a.W; i += blockDim.x) {
+            a.C[i] = 0;
+            if (a.C_store) a.C_store[i] = 0;
+        }
+        if (tid == 0) {
+            *a.Dcount = 0;
+            atomicOr(a.status, 1u);
+        }
+        return;
+    }
+    const uint32_t r = a.cand_root[rank];
+    const int64_t off = a.tree_off[r];
+    const uint32_t npos = static_cast<uint32_t>(a.tree_off[r + 1] - off);
+    const uint32_t *pp = a.parent_pos + off;
+    const int32_t *pc = a.pos_coord + off;
+    const bool cached = npos <= static_cast<uint32_t>(a.parent_cap);
+    if (cached)
+        for (uint32_t i = tid; i < npos; i += blockDim.x) par_l[i] = pp[i];
+    if (tid < 2) cur[tid] = tid == 0 ? a.cand_pu[rank] : a.cand_pv[rank];
+    __syncthreads();
+    if (tid == 0) {
+        const int32_t c = a.cand_coord[rank];
+        if (c >= 0) atomicOr(reinterpret_cast<unsigned long long *>(&Cl[c >> 6]), 1ull << (c & 63));
+    }
+    for (;;) {
+        if (tid < 2) {
+            uint32_t pos = cur[tid];
+            int32_t n = 0;
+            while (pos != 0 && n < kPathCap) {
+                path[tid * kPathCap + n++] = pos;
+                pos = cached ? par_l[pos] : pp[pos];
+            }
+            cur[tid] = pos;
+            cnt[tid] = n;
+        }
+        __syncthreads();
+        const int32_t n0 = cnt[0], n1 = cnt[1];
+        for (int i = tid; i < n0 + n1; i += blockDim.x) {
+            const uint32_t pos = i < n0 ? path[i] : path[kPathCap + (i - n0)];
+            const int32_t c = pc[pos];
+            if (c >= 0) atomicOr(reinterpret_cast<unsigned long long *>(&Cl[c >> 6]), 1ull << (c & 63));
+        }
+        if (tid == 0) more = (cur[0] != 0 || cur[1] != 0);
+        __syncthreads();
+        if (!more) break;
+    }
+    const int32_t dlim = a.e > 0 ? ((a.e - 1) >> 6) + 1 : 0;
+    for (int i = tid; i < a.W; i += blockDim.x) {
+        const uint64_t v = Cl[i];
+        a.C[i] = v;
+        if (a.C_store) a.C_store[i] = v;
+        if (i < dlim && v != 0) a.D[atomicAdd(&dcount, 1u)] = static_cast<uint32_t>(i);
+    }
+    __syncthreads();
+    if (tid == 0) *a.Dcount = dcount;
+}
+
+// p_j = <C_e, S_j> over words below e (S_j has no other bits there) plus C_e's bit j (S_j's own).
+__global__ __launch_bounds__(256) void k_dot(const uint64_t *__restrict__ C, const uint32_t *__restrict__ D,
+                                             const uint32_t *__restrict__ Dcount, const uint64_t *__restrict__ Sw,
+                                             int64_t npad, int32_t e, int32_t N, uint8_t *__restrict__ flag) {
+    const int64_t j = static_cast<int64_t>(e) + 1 + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    const uint32_t nd = *Dcount;
+    uint32_t p = static_cast<uint32_t>((C[j >> 6] >> (j & 63)) & 1ull);
+    for (uint32_t t = 0; t < nd; ++t) {
+        const uint32_t w = D[t];
+        p ^= static_cast<uint32_t>(__popcll(C[w] & Sw[static_cast<int64_t>(w) * npad + j]) & 1);
+    }
+    flag[j] = static_cast<uint8_t>(p);
+}
+
+// S_j ^= S_e for flagged j > e, word w = blockIdx.y.  The column j = e+1 (final after this phase)
+// is also written contiguously, with its own bit, as S_{e+1} for the next phase.
+__global__ __launch_bounds__(256) void k_xor(uint64_t *__restrict__ Sw, int64_t npad, const uint64_t *__restrict__ Se,
+                                             uint64_t *__restrict__ Snext, const uint8_t *__restrict__ flag,
+                                             int32_t e, int32_t N, int64_t jbase) {
+    const int64_t j = jbase + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int32_t w = blockIdx.y;
+    if (j <= e || j >= N) return;
+    const uint64_t se = Se[w];
+    uint64_t *ptr = Sw + static_cast<int64_t>(w) * npad + j;
+    if (j == static_cast<int64_t>(e) + 1) {
+        uint64_t v = *ptr;
+        if (flag[j] && se) {
+            v ^= se;
+            *ptr = v;
+        }
+        if (w == ((e + 1) >> 6)) v |= 1ull << ((e + 1) & 63);
+        Snext[w] = v;
+    } else if (se && flag[j]) {
+        *ptr ^= se;
+    }
+}
+
+// Reference-compatible parity of one chunk: every position of every valid root slot, 0/1 as
+// uint32 (gpu/kernel_init_edges.cu:32-75 followed by gpu/kernel_multi_search.cu:62-108).
+__global__ __launch_bounds__(256) void k_compat_parity(const int32_t *__restrict__ nt, const int32_t *__restrict__ eo,
+                                                       const uint32_t *__restrict__ cols, uint32_t *__restrict__ out,
+                                                       const uint64_t *__restrict__ S, int32_t n, int32_t roots,
+                                                       int32_t p_words) {
+    extern __shared__ uint64_t lds[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wib = threadIdx.x / kWave;
+    const int wpb = blockDim.x / kWave;
+    uint64_t *P = lds + static_cast<size_t>(wib) * p_words;
+    const int slot = blockIdx.x * wpb + wib;
+    if (slot >= roots) return;
+    const int32_t *e = eo + static_cast<int64_t>(slot) * n;
+    const uint32_t *c = cols + static_cast<int64_t>(slot) * n;
+    uint32_t *o = out + static_cast<int64_t>(slot) * n;
+    for (uint32_t base = 0; base < static_cast<uint32_t>(n); base += kWave) {
+        const uint32_t pos = base + lane;
+        const bool v = pos < static_cast<uint32_t>(n);
+        const int32_t edge = v ? e[pos] : -1;
+        const uint32_t par = v ? c[pos] : kNoParent;
+        const int32_t coord = edge >= 0 ? nt[edge] : -1;
+        const uint32_t val = window_parity(P, base, par, bit_of(S, coord), lane);
+        if (v) o[pos] = val;
+    }
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_copy_u64(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst, int64_t n) {
+    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Device properties shared by both forms
+// ---------------------------------------------------------------------------------------------
+
+struct DeviceProps {
+    bool init = false;
+    int device = 0;
+    hipDeviceProp_t prop{};
+};
+DeviceProps g_dev;
+
+int ensure_device() {
+    if (g_dev.init) return MCB_OK;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(MCB_E_DEVICE, "no HIP device");
+    HIP_TRY(hipGetDevice(&g_dev.device));
+    HIP_TRY(hipGetDeviceProperties(&g_dev.prop, g_dev.device));
+    g_dev.init = true;
+    return MCB_OK;
+}
+
+template <typename T>
+int dev_alloc(T **p, size_t count, int64_t &total) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    HIP_TRY(hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T)));
+    total += static_cast<int64_t>(count * sizeof(T));
+    return MCB_OK;
+}
+
+template <typename T>
+int upload(T **p, const T *src, size_t count, int64_t &total) {
+    int rc = dev_alloc(p, count, total);
+    if (rc) return rc;
+    if (count && src) HIP_TRY(hipMemcpy(*p, src, count * sizeof(T), hipMemcpyHostToDevice));
+    return MCB_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// Reference-compatible form
+// ---------------------------------------------------------------------------------------------
+
+struct mcb_gpu {
+    int num_edges = 0, num_non_tree = 0, size_vector = 0, nodes = 0, fvs = 0, chunk = 0;
+    int num_chunks = 0, nslots = 0, nstreams = 0;
+    bool all_resident = false;
+    int32_t *d_nt = nullptr;
+    uint64_t *d_S = nullptr;
+    std::vector<int32_t *> d_eo;
+    std::vector<uint32_t *> d_cols, d_out;
+    std::vector<hipStream_t> streams;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool initialized = false;
+    int64_t bytes = 0;
+};
+
+namespace {
+
+int compat_roots_in_chunk(const mcb_gpu *g, int i) { return std::min(g->chunk, g->fvs - i * g->chunk); }
+
+int compat_check_layout(const mcb_gpu *g, const uint32_t *cols, int roots) {
+    for (int s = 0; s < roots; ++s) {
+        const uint32_t *c = cols + static_cast<size_t>(s) * g->nodes;
+        if (c[0] != kNoParent) return fail(MCB_E_INVALID, "tree_cols: position 0 must be the root (parent -1)");
+        for (int i = 1; i < g->nodes; ++i)
+            if (c[i] >= static_cast<uint32_t>(i))
+                return fail(MCB_E_INVALID, "tree_cols: a parent position must precede its child");
+    }
+    return MCB_OK;
+}
+
+int compat_launch(mcb_gpu *g, int slot, int roots, hipStream_t st) {
+    const int p_words = (g->nodes + 63) / 64;
+    int wpb = 4;
+    while (wpb > 1 && wpb * p_words * 8 > kLdsLimit) --wpb;
+    if (p_words * 8 > kLdsLimit) return fail(MCB_E_INVALID, "tree too large for the LDS parity buffer");
+    const int blocks = (roots + wpb - 1) / wpb;
+    hipLaunchKernelGGL(k_compat_parity, dim3(blocks), dim3(wpb * kWave), wpb * p_words * 8, st, g->d_nt,
+                       g->d_eo[slot], g->d_cols[slot], g->d_out[slot], g->d_S, g->nodes, roots, p_words);
+    HIP_TRY(hipGetLastError());
+    return MCB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mcb_hip_last_error(void) { return g_err.c_str(); }
+
+int mcb_hip_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+int mcb_hip_device_info(char *name, int name_len, int *compute_units, int64_t *total_mem) {
+    int rc = ensure_device();
+    if (rc) return rc;
+    if (name && name_len > 0) {
+        std::snprintf(name, static_cast<size_t>(name_len), "%s (%s)", g_dev.prop.name, g_dev.prop.gcnArchName);
+    }
+    if (compute_units) *compute_units = g_dev.prop.multiProcessorCount;
+    if (total_mem) *total_mem = static_cast<int64_t>(g_dev.prop.totalGlobalMem);
+    return MCB_OK;
+}
+
+void init_cuda(void) {
+    HIP_OR_EXIT(hipGetDevice(&g_dev.device));
+    HIP_OR_EXIT(hipGetDeviceProperties(&g_dev.prop, g_dev.device));
+    g_dev.init = true;
+}
+
+size_t configure_grid(int start, int end) {
+    if (!g_dev.init) init_cuda();
+    const int cus = g_dev.prop.multiProcessorCount;
+    return (end - start) < cus ? static_cast<size_t>(end - start) : static_cast<size_t>(cus);
+}
+
+unsigned *allocate_pinned_memory(int chunk, int nodes) {
+    unsigned *p = nullptr;
+    HIP_OR_EXIT(hipHostMalloc(reinterpret_cast<void **>(&p),
+                              sizeof(unsigned) * static_cast<size_t>(chunk) * static_cast<size_t>(nodes),
+                              hipHostMallocDefault));
+    return p;
+}
+
+void free_pinned_memory(unsigned *p) { HIP_OR_EXIT(hipHostFree(p)); }
+
+size_t calculate_chunk_size(size_t num_nodes, size_t num_edges, size_t size_vector, size_t nstream) {
+    if (!g_dev.init) init_cuda();
+    const size_t global_bytes = g_dev.prop.totalGlobalMem;
+    const size_t static_bytes = 4 * num_edges + 8 * size_vector;
+    if (static_bytes >= global_bytes || nstream == 0) return 0;
+    const size_t elems = (global_bytes - static_bytes) / 4;
+    return elems / (nstream * (num_nodes * 4 + 1));
+}
+
+int mcb_gpu_create(int num_edges, int num_non_tree_edges, int size_vector, int original_nodes, int fvs_size,
+                   int chunk_size, int nstreams, mcb_gpu **out) {
+    if (!out || num_edges <= 0 || num_non_tree_edges <= 0 || size_vector <= 0 || original_nodes <= 0 ||
+        fvs_size <= 0 || chunk_size <= 0 || nstreams <= 0)
+        return fail(MCB_E_INVALID, "invalid arguments");
+    *out = nullptr;
+    int rc = ensure_device();
+    if (rc) return rc;
+    mcb_gpu *g = new (std::nothrow) mcb_gpu();
+    if (!g) return fail(MCB_E_NOMEM, "out of memory");
+    g->num_edges = num_edges;
+    g->num_non_tree = num_non_tree_edges;
+    g->size_vector = size_vector;
+    g->nodes = original_nodes;
+    g->fvs = fvs_size;
+    g->chunk = chunk_size;
+    g->num_chunks = (fvs_size + chunk_size - 1) / chunk_size;
+    const size_t per_slot = static_cast<size_t>(chunk_size) * original_nodes * 12;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    const size_t reserve = static_cast<size_t>(1) << 30;
+    g->all_resident = static_cast<size_t>(g->num_chunks) * per_slot + reserve < free_b;
+    g->nslots = g->all_resident ? g->num_chunks : std::min(32, g->num_chunks);
+    g->nstreams = std::min(32, std::max(1, std::min(nstreams, g->num_chunks)));
+    auto cleanup = [&](int code) {
+        mcb_gpu_destroy(g);
+        return code;
+    };
+    if ((rc = dev_alloc(&g->d_nt, num_edges, g->bytes))) return cleanup(rc);
+    if ((rc = dev_alloc(&g->d_S, size_vector, g->bytes))) return cleanup(rc);
+    g->d_eo.assign(g->nslots, nullptr);
+    g->d_cols.assign(g->nslots, nullptr);
+    g->d_out.assign(g->nslots, nullptr);
+    const size_t cn = static_cast<size_t>(chunk_size) * original_nodes;
+    for (int i = 0; i < g->nslots; ++i) {
+        if ((rc = dev_alloc(&g->d_eo[i], cn, g->bytes))) return cleanup(rc);
+        if ((rc = dev_alloc(&g->d_cols[i], cn, g->bytes))) return cleanup(rc);
+        if ((rc = dev_alloc(&g->d_out[i], cn, g->bytes))) return cleanup(rc);
+    }
+    g->streams.assign(g->nstreams, nullptr);
+    for (int i = 0; i < g->nstreams; ++i)
+        if (hipStreamCreate(&g->streams[i]) != hipSuccess) return cleanup(fail(MCB_E_DEVICE, "hipStreamCreate"));
+    if (hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess)
+        return cleanup(fail(MCB_E_DEVICE, "hipEventCreate"));
+    *out = g;
+    return MCB_OK;
+}
+
+int mcb_gpu_initialize_memory(mcb_gpu *g, const int32_t *nt, const uint32_t *const *tree_rows,
+                              const uint32_t *const *tree_cols, const int32_t *const *edge_offset, int num_rows) {
+    (void)tree_rows;
+    if (!g || !nt || !tree_cols || !edge_offset || num_rows != g->num_chunks)
+        return fail(MCB_E_INVALID, "invalid arguments");
+    HIP_TRY(hipMemcpy(g->d_nt, nt, sizeof(int32_t) * g->num_edges, hipMemcpyHostToDevice));
+    for (int i = 0; i < g->num_chunks; ++i) {
+        const int roots = compat_roots_in_chunk(g, i);
+        int rc = compat_check_layout(g, tree_cols[i], roots);
+        if (rc) return rc;
+        if (i < g->nslots && g->all_resident) {
+            const size_t bytes = sizeof(int32_t) * static_cast<size_t>(roots) * g->nodes;
+            HIP_TRY(hipMemcpy(g->d_eo[i], edge_offset[i], bytes, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(g->d_cols[i], tree_cols[i], bytes, hipMemcpyHostToDevice));
+        }
+    }
+    g->initialized = true;
+    return MCB_OK;
+}
+
+int mcb_gpu_copy_support_vector(mcb_gpu *g, const uint64_t *elements, float *ms) {
+    if (!g || !elements) return fail(MCB_E_INVALID, "invalid arguments");
+    HIP_TRY(hipEventRecord(g->ev0, nullptr));
+    HIP_TRY(hipMemcpy(g->d_S, elements, sizeof(uint64_t) * g->size_vector, hipMemcpyHostToDevice));
+    HIP_TRY(hipEventRecord(g->ev1, nullptr));
+    HIP_TRY(hipEventSynchronize(g->ev1));
+    if (ms) HIP_TRY(hipEventElapsedTime(ms, g->ev0, g->ev1));
+    return MCB_OK;
+}
+
+int mcb_gpu_process_shortest_path(mcb_gpu *g, const uint32_t *const *tree_rows, const uint32_t *const *tree_cols,
+                                  const int32_t *const *edge_offset, uint32_t *const *precompute_value, int num_rows,
+                                  int multiple_transfer, float *ms) {
+    (void)tree_rows;
+    if (!g || !precompute_value || num_rows != g->num_chunks) return fail(MCB_E_INVALID, "invalid arguments");
+    if (!g->initialized) return fail(MCB_E_STATE, "initialize_memory has not been called");
+    if (multiple_transfer && (!tree_cols || !edge_offset)) return fail(MCB_E_INVALID, "trees needed for transfer");
+    if (!multiple_transfer && !g->all_resident)
+        return fail(MCB_E_STATE, "trees do not fit in device memory; use multiple_transfer");
+    HIP_TRY(hipEventRecord(g->ev0, nullptr));
+    for (int i = 0; i < g->num_chunks; ++i) {
+        const int slot = g->all_resident ? i : i % g->nslots;
+        hipStream_t st = g->streams[i % g->nstreams];
+        const int roots = compat_roots_in_chunk(g, i);
+        const size_t bytes = sizeof(uint32_t) * static_cast<size_t>(roots) * g->nodes;
+        if (multiple_transfer) {
+            HIP_TRY(hipMemcpyAsync(g->d_eo[slot], edge_offset[i], bytes, hipMemcpyHostToDevice, st));
+            HIP_TRY(hipMemcpyAsync(g->d_cols[slot], tree_cols[i], bytes, hipMemcpyHostToDevice, st));
+        }
+        int rc = compat_launch(g, slot, roots, st);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(precompute_value[i], g->d_out[slot], bytes, hipMemcpyDeviceToHost, st));
+        if (!g->all_resident && (i + 1) % g->nslots == 0) HIP_TRY(hipDeviceSynchronize());
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipEventRecord(g->ev1, nullptr));
+    HIP_TRY(hipEventSynchronize(g->ev1));
+    if (ms) HIP_TRY(hipEventElapsedTime(ms, g->ev0, g->ev1));
+    return MCB_OK;
+}
+
+int mcb_gpu_fetch(mcb_gpu *g, uint32_t *const *precompute_value, int num_rows, float *ms) {
+    if (!g || !precompute_value || num_rows != g->num_chunks) return fail(MCB_E_INVALID, "invalid arguments");
+    HIP_TRY(hipEventRecord(g->ev0, nullptr));
+    for (int i = 0; i < g->num_chunks && i < g->nslots; ++i) {
+        const size_t bytes = sizeof(uint32_t) * static_cast<size_t>(compat_roots_in_chunk(g, i)) * g->nodes;
+        HIP_TRY(hipMemcpy(precompute_value[i], g->d_out[i], bytes, hipMemcpyDeviceToHost));
+    }
+    HIP_TRY(hipEventRecord(g->ev1, nullptr));
+    HIP_TRY(hipEventSynchronize(g->ev1));
+    if (ms) HIP_TRY(hipEventElapsedTime(ms, g->ev0, g->ev1));
+    return MCB_OK;
+}
+
+int mcb_gpu_calculate_memory(mcb_gpu *g, double *total_mb, double *static_mb, double *variable_mb) {
+    if (!g) return fail(MCB_E_INVALID, "invalid arguments");
+    const double mb = 1024.0 * 1024.0;
+    const double st = (4.0 * g->num_edges + 8.0 * g->size_vector) / mb;
+    const double var = static_cast<double>(g->bytes) / mb - st;
+    if (total_mb) *total_mb = st + var;
+    if (static_mb) *static_mb = st;
+    if (variable_mb) *variable_mb = var;
+    return MCB_OK;
+}
+
+void mcb_gpu_destroy(mcb_gpu *g) {
+    if (!g) return;
+    (void)hipFree(g->d_nt);
+    (void)hipFree(g->d_S);
+    for (auto p : g->d_eo) (void)hipFree(p);
+    for (auto p : g->d_cols) (void)hipFree(p);
+    for (auto p : g->d_out) (void)hipFree(p);
+    for (auto s : g->streams)
+        if (s) (void)hipStreamDestroy(s);
+    if (g->ev0) (void)hipEventDestroy(g->ev0);
+    if (g->ev1) (void)hipEventDestroy(g->ev1);
+    delete g;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// Native form
+// ---------------------------------------------------------------------------------------------
+
+struct mcb_solver {
+    mcb_solver_options opt{};
+    int32_t F = 0, N = 0, W = 0, max_pos = 0, p_words = 0;
+    int64_t K = 0, Ptot = 0, npad = 0;
+    int32_t phase = 0;
+    // trees and candidates
+    int64_t *tree_off = nullptr;
+    uint32_t *parent_pos = nullptr;
+    int32_t *pos_coord = nullptr;
+    uint32_t *root_order = nullptr, *root_min_rank = nullptr;
+    int64_t *rc_off = nullptr;
+    uint32_t *rc_pu = nullptr, *rc_pv = nullptr, *rc_rank = nullptr;
+    int32_t *rc_coord = nullptr;
+    uint32_t *cand_root = nullptr, *cand_pu = nullptr, *cand_pv = nullptr;
+    int32_t *cand_coord = nullptr;
+    // phase state
+    uint64_t *Sw = nullptr, *Scur = nullptr, *C = nullptr, *C_store = nullptr;
+    uint32_t *D = nullptr, *Dcount = nullptr, *best = nullptr, *status = nullptr;
+    uint8_t *flag = nullptr;
+    // launch shapes
+    int wpb = 4, blocks = 1, lds_bytes = 0, parent_cap = 0, cycle_lds = 0;
+    bool s_in_lds = true;
+    hipStream_t stream = nullptr;
+    int64_t bytes = 0;
+    // timing
+    double kt_ms[MCB_K_COUNT] = {0, 0, 0, 0};
+    int64_t kt_n[MCB_K_COUNT] = {0, 0, 0, 0};
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_kind;
+    size_t ev_used = 0;
+};
+
+namespace {
+
+int harvest_events(mcb_solver *s) {
+    if (s->ev_used == 0) return MCB_OK;
+    HIP_TRY(hipEventSynchronize(s->ev_pool[s->ev_used - 1]));
+    for (size_t i = 0; i + 1 < s->ev_used; i += 2) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, s->ev_pool[i], s->ev_pool[i + 1]));
+        s->kt_ms[s->ev_kind[i / 2]] += ms;
+        s->kt_n[s->ev_kind[i / 2]] += 1;
+    }
+    s->ev_used = 0;
+    return MCB_OK;
+}
+
+int timed_begin(mcb_solver *s, int kind) {
+    if (!s->opt.timing) return MCB_OK;
+    if (s->ev_used + 2 > s->ev_pool.size()) {
+        int rc = harvest_events(s);
+        if (rc) return rc;
+    }
+    s->ev_kind[s->ev_used / 2] = kind;
+    HIP_TRY(hipEventRecord(s->ev_pool[s->ev_used++], s->stream));
+    return MCB_OK;
+}
+
+int timed_end(mcb_solver *s) {
+    if (!s->opt.timing) return MCB_OK;
+    HIP_TRY(hipEventRecord(s->ev_pool[s->ev_used++], s->stream));
+    return MCB_OK;
+}
+
+SearchArgs search_args(const mcb_solver *s, int32_t e) {
+    SearchArgs a{};
+    a.tree_off = s->tree_off;
+    a.parent_pos = s->parent_pos;
+    a.pos_coord = s->pos_coord;
+    a.root_order = s->root_order;
+    a.root_min_rank = s->root_min_rank;
+    a.rc_off = s->rc_off;
+    a.rc_pu = s->rc_pu;
+    a.rc_pv = s->rc_pv;
+    a.rc_rank = s->rc_rank;
+    a.rc_coord = s->rc_coord;
+    a.S = s->Scur + static_cast<int64_t>(e & 1) * s->W;
+    a.best = s->best + e;
+    a.parity_out = nullptr;
+    a.F = s->F;
+    a.W = s->W;
+    a.p_words = s->p_words;
+    a.total_waves = s->blocks * s->wpb;
+    return a;
+}
+
+int launch_phase(mcb_solver *s, int32_t e) {
+    hipStream_t st = s->stream;
+    int rc;
+    // (a)+(b)
+    if ((rc = timed_begin(s, MCB_K_PARITY_SEARCH))) return rc;
+    SearchArgs a = search_args(s, e);
+    if (s->s_in_lds)
+        hipLaunchKernelGGL((k_parity_search<true, true>), dim3(s->blocks), dim3(s->wpb * kWave), s->lds_bytes, st, a);
+    else
+        hipLaunchKernelGGL((k_parity_search<false, true>), dim3(s->blocks), dim3(s->wpb * kWave), s->lds_bytes, st, a);
+    HIP_TRY(hipGetLastError());
+    if ((rc = timed_end(s))) return rc;
+    // cycle vector
+    if ((rc = timed_begin(s, MCB_K_CYCLE))) return rc;
+    CycleArgs c{};
+    c.best = s->best + e;
+    c.cand_root = s->cand_root;
+    c.cand_pu = s->cand_pu;
+    c.cand_pv = s->cand_pv;
+    c.cand_coord = s->cand_coord;
+    c.tree_off = s->tree_off;
+    c.parent_pos = s->parent_pos;
+    c.pos_coord = s->pos_coord;
+    c.C = s->C;
+    c.D = s->D;
+    c.Dcount = s->Dcount;
+    c.C_store = s->C_store ? s->C_store + static_cast<int64_t>(e) * s->W : nullptr;
+    c.status = s->status;
+    c.e = e;
+    c.W = s->W;
+    c.parent_cap = s->parent_cap;
+    hipLaunchKernelGGL(k_cycle, dim3(1), dim3(256), s->cycle_lds, st, c);
+    HIP_TRY(hipGetLastError());
+    if ((rc = timed_end(s))) return rc;
+    if (e + 1 >= s->N) return MCB_OK;
+    // (c) update of S_j, j > e
+    const int64_t rows = s->N - (e + 1);
+    if ((rc = timed_begin(s, MCB_K_DOT))) return rc;
+    hipLaunchKernelGGL(k_dot, dim3(static_cast<unsigned>((rows + 255) / 256)), dim3(256), 0, st, s->C, s->D, s->Dcount,
+                       s->Sw, s->npad, e, s->N, s->flag);
+    HIP_TRY(hipGetLastError());
+    if ((rc = timed_end(s))) return rc;
+    if ((rc = timed_begin(s, MCB_K_XOR))) return rc;
+    const int64_t jbase = (static_cast<int64_t>(e) + 1) & ~static_cast<int64_t>(63);
+    const int64_t span = s->N - jbase;
+    const unsigned wy = static_cast<unsigned>(((e + 1) >> 6) + 1);
+    hipLaunchKernelGGL(k_xor, dim3(static_cast<unsigned>((span + 255) / 256), wy), dim3(256), 0, st, s->Sw, s->npad,
+                       s->Scur + static_cast<int64_t>(e & 1) * s->W, s->Scur + static_cast<int64_t>((e + 1) & 1) * s->W,
+                       s->flag, e, s->N, jbase);
+    HIP_TRY(hipGetLastError());
+    if ((rc = timed_end(s))) return rc;
+    return MCB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void mcb_solver_options_init(mcb_solver_options *opt) {
+    if (!opt) return;
+    std::memset(opt, 0, sizeof(*opt));
+}
+
+void mcb_solver_destroy(mcb_solver *s) {
+    if (!s) return;
+    void *ptrs[] = {s->tree_off, s->parent_pos, s->pos_coord, s->root_order, s->root_min_rank, s->rc_off, s->rc_pu,
+                    s->rc_pv, s->rc_rank, s->rc_coord, s->cand_root, s->cand_pu, s->cand_pv, s->cand_coord, s->Sw,
+                    s->Scur, s->C, s->C_store, s->D, s->Dcount, s->best, s->status, s->flag};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, mcb_solver **out) {
+    if (!d || !out || d->num_roots <= 0 || d->num_non_tree <= 0 || d->num_candidates <= 0 || d->num_positions <= 0 ||
+        !d->tree_offset || !d->parent_pos || !d->pos_coord || !d->cand_root || !d->cand_pos_u || !d->cand_pos_v ||
+        !d->cand_coord || !d->root_cand_offset || !d->root_cand_rank)
+        return fail(MCB_E_INVALID, "invalid arguments");
+    if (d->num_candidates >= static_cast<int64_t>(kNoRank)) return fail(MCB_E_INVALID, "too many candidates");
+    *out = nullptr;
+    int rc = ensure_device();
+    if (rc) return rc;
+    mcb_solver *s = new (std::nothrow) mcb_solver();
+    if (!s) return fail(MCB_E_NOMEM, "out of memory");
+    auto cleanup = [&](int code) {
+        mcb_solver_destroy(s);
+        return code;
+    };
+    if (opt) s->opt = *opt;
+    s->F = d->num_roots;
+    s->N = d->num_non_tree;
+    s->K = d->num_candidates;
+    s->Ptot = d->num_positions;
+    s->W = (s->N + 63) / 64;
+    s->npad = (static_cast<int64_t>(s->N) + 63) & ~static_cast<int64_t>(63);
+    const int32_t F = s->F;
+    const int64_t K = s->K;
+    // Validate the layout on the host (parent before child, positions and ranks in range).
+    if (d->tree_offset[0] != 0 || d->tree_offset[F] != s->Ptot) return cleanup(fail(MCB_E_INVALID, "tree_offset"));
+    if (d->root_cand_offset[0] != 0 || d->root_cand_offset[F] != K)
+        return cleanup(fail(MCB_E_INVALID, "root_cand_offset"));
+    for (int32_t r = 0; r < F; ++r) {
+        const int64_t a = d->tree_offset[r], b = d->tree_offset[r + 1];
+        if (b <= a || b - a > 0x7FFFFFFF) return cleanup(fail(MCB_E_INVALID, "empty or oversized tree"));
+        s->max_pos = std::max<int32_t>(s->max_pos, static_cast<int32_t>(b - a));
+        if (d->parent_pos[a] != kNoParent) return cleanup(fail(MCB_E_INVALID, "position 0 must be the root"));
+        for (int64_t i = a + 1; i < b; ++i) {
+            if (d->parent_pos[i] >= static_cast<uint64_t>(i - a))
+                return cleanup(fail(MCB_E_INVALID, "parent position must precede its child"));
+            if (d->pos_coord[i] >= s->N) return cleanup(fail(MCB_E_INVALID, "coordinate out of range"));
+        }
+    }
+    for (int64_t k = 0; k < K; ++k) {
+        const uint32_t r = d->cand_root[k];
+        if (r >= static_cast<uint32_t>(F)) return cleanup(fail(MCB_E_INVALID, "candidate root out of range"));
+        const int64_t np = d->tree_offset[r + 1] - d->tree_offset[r];
+        if (d->cand_pos_u[k] >= np || d->cand_pos_v[k] >= np || d->cand_coord[k] >= s->N)
+            return cleanup(fail(MCB_E_INVALID, "candidate position out of range"));
+    }
+    s->p_words = (s->max_pos + 63) / 64;
+    // Root-major candidate copies and the root visiting order (lowest rank first).
+    std::vector<uint32_t> rc_pu(K), rc_pv(K), rc_rank(K), min_rank(F, kNoRank), order(F);
+    std::vector<int32_t> rc_coord(K);
+    for (int32_t r = 0; r < F; ++r) {
+        const int64_t a = d->root_cand_offset[r], b = d->root_cand_offset[r + 1];
+        for (int64_t i = a; i < b; ++i) {
+            const uint32_t k = d->root_cand_rank[i];
+            if (k >= K || d->cand_root[k] != static_cast<uint32_t>(r) || (i > a && k <= d->root_cand_rank[i - 1]))
+                return cleanup(fail(MCB_E_INVALID, "root_cand_rank must list each root's ranks ascending"));
+            rc_pu[i] = d->cand_pos_u[k];
+            rc_pv[i] = d->cand_pos_v[k];
+            rc_rank[i] = k;
+            rc_coord[i] = d->cand_coord[k];
+        }
+        if (b > a) min_rank[r] = d->root_cand_rank[a];
+    }
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return min_rank[x] < min_rank[y]; });
+
+    int64_t &B = s->bytes;
+    if ((rc = upload(&s->tree_off, d->tree_offset, F + 1, B))) return cleanup(rc);
+    if ((rc = upload(&s->parent_pos, d->parent_pos, s->Ptot, B))) return cleanup(rc);
+    if ((rc = upload(&s->pos_coord, d->pos_coord, s->Ptot, B))) return cleanup(rc);
+    if ((rc = upload(&s->root_order, order.data(), F, B))) return cleanup(rc);
+    if ((rc = upload(&s->root_min_rank, min_rank.data(), F, B))) return cleanup(rc);
+    if ((rc = upload(&s->rc_off, d->root_cand_offset, F + 1, B))) return cleanup(rc);
+    if ((rc = upload(&s->rc_pu, rc_pu.data(), K, B))) return cleanup(rc);
+    if ((rc = upload(&s->rc_pv, rc_pv.data(), K, B))) return cleanup(rc);
+    if ((rc = upload(&s->rc_rank, rc_rank.data(), K, B))) return cleanup(rc);
+    if ((rc = upload(&s->rc_coord, rc_coord.data(), K, B))) return cleanup(rc);
+    if ((rc = upload(&s->cand_root, d->cand_root, K, B))) return cleanup(rc);
+    if ((rc = upload(&s->cand_pu, d->cand_pos_u, K, B))) return cleanup(rc);
+    if ((rc = upload(&s->cand_pv, d->cand_pos_v, K, B))) return cleanup(rc);
+    if ((rc = upload(&s->cand_coord, d->cand_coord, K, B))) return cleanup(rc);
+    if ((rc = dev_alloc(&s->Sw, static_cast<size_t>(s->W) * s->npad, B))) return cleanup(rc);
+    if ((rc = dev_alloc(&s->Scur, 2 * static_cast<size_t>(s->W), B))) return cleanup(rc);
+    if ((rc = dev_alloc(&s->C, s->W, B))) return cleanup(rc);
+    if (s->opt.store_cycles && (rc = dev_alloc(&s->C_store, static_cast<size_t>(s->N) * s->W, B))) return cleanup(rc);
+    if ((rc = dev_alloc(&s->D, s->W, B))) return cleanup(rc);
+    if ((rc = dev_alloc(&s->Dcount, 1, B))) return cleanup(rc);
+    if ((rc = dev_alloc(&s->best, s->N, B))) return cleanup(rc);
+    if ((rc = dev_alloc(&s->status, 1, B))) return cleanup(rc);
+    if ((rc = dev_alloc(&s->flag, s->npad, B))) return cleanup(rc);
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(MCB_E_DEVICE, "hipStreamCreate"));
+
+    // Launch shape of the parity/search kernel: S_e and one bit per position per wave in LDS.
+    const int p_bytes = s->p_words * 8;
+    if (p_bytes > kLdsLimit) return cleanup(fail(MCB_E_INVALID, "a tree layout exceeds the LDS parity buffer"));
+    int wpb = s->opt.waves_per_block > 0 ? std::min(4, s->opt.waves_per_block) : 4;
+    s->s_in_lds = static_cast<int64_t>(s->W) * 8 + p_bytes <= kLdsLimit;
+    const int s_bytes = s->s_in_lds ? s->W * 8 : 0;
+    while (wpb > 1 && s_bytes + wpb * p_bytes > kLdsLimit) --wpb;
+    s->wpb = wpb;
+    s->lds_bytes = s_bytes + wpb * p_bytes;
+    int per_cu = 0;
+    if (s->s_in_lds)
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_parity_search<true, true>, wpb * kWave,
+                                                             s->lds_bytes));
+    else
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_parity_search<false, true>, wpb * kWave,
+                                                             s->lds_bytes));
+    per_cu = std::max(1, per_cu);
+    int64_t blocks = static_cast<int64_t>(per_cu) * g_dev.prop.multiProcessorCount;
+    if (s->opt.max_blocks > 0) blocks = std::min<int64_t>(blocks, s->opt.max_blocks);
+    blocks = std::min<int64_t>(blocks, (F + wpb - 1) / wpb);
+    s->blocks = static_cast<int>(std::max<int64_t>(1, blocks));
+    // Cycle kernel LDS: C_e plus the chosen root's parent array when it fits.
+    const int cycle_static = 8 * 1024;  // path buffers and counters of k_cycle
+    if (static_cast<int64_t>(s->W) * 8 > kLdsLimit - cycle_static - 4096)
+        return cleanup(fail(MCB_E_INVALID, "cycle-space dimension too large for the LDS cycle buffer"));
+    s->parent_cap = (kLdsLimit - cycle_static - s->W * 8) / 4;
+    s->cycle_lds = s->W * 8 + std::min(s->parent_cap, s->max_pos) * 4;
+    if (s->opt.timing) {
+        s->ev_pool.resize(4096);
+        s->ev_kind.resize(2048);
+        for (auto &e : s->ev_pool)
+            if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(MCB_E_DEVICE, "hipEventCreate"));
+    }
+    rc = mcb_solver_reset(s);
+    if (rc) return cleanup(rc);
+    *out = s;
+    return MCB_OK;
+}
+
+int mcb_solver_reset(mcb_solver *s) {
+    if (!s) return fail(MCB_E_INVALID, "invalid arguments");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemsetAsync(s->Sw, 0, sizeof(uint64_t) * static_cast<size_t>(s->W) * s->npad, s->stream));
+    std::vector<uint64_t> s0(2 * static_cast<size_t>(s->W), 0);
+    s0[0] = 1;  // S_0 = e_0
+    HIP_TRY(hipMemcpyAsync(s->Scur, s0.data(), sizeof(uint64_t) * s0.size(), hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<unsigned>((s->N + 255) / 256)), dim3(256), 0, s->stream, s->best,
+                       kNoRank, static_cast<int64_t>(s->N));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemsetAsync(s->status, 0, sizeof(uint32_t), s->stream));
+    HIP_TRY(hipMemsetAsync(s->flag, 0, static_cast<size_t>(s->npad), s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->phase = 0;
+    s->ev_used = 0;
+    for (int i = 0; i < MCB_K_COUNT; ++i) {
+        s->kt_ms[i] = 0;
+        s->kt_n[i] = 0;
+    }
+    return MCB_OK;
+}
+
+int mcb_solver_run(mcb_solver *s, int32_t phases) {
+    if (!s || phases < 0) return fail(MCB_E_INVALID, "invalid arguments");
+    const int32_t end = std::min(s->N, s->phase + phases);
+    for (int32_t e = s->phase; e < end; ++e) {
+        int rc = launch_phase(s, e);
+        if (rc) return rc;
+        s->phase = e + 1;
+    }
+    return MCB_OK;
+}
+
+int mcb_solver_sync(mcb_solver *s) {
+    if (!s) return fail(MCB_E_INVALID, "invalid arguments");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    int rc = harvest_events(s);
+    if (rc) return rc;
+    uint32_t st = 0;
+    HIP_TRY(hipMemcpy(&st, s->status, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (st) return fail(MCB_E_NOT_FOUND, "a phase found no odd candidate");
+    return MCB_OK;
+}
+
+int32_t mcb_solver_phase(const mcb_solver *s) { return s ? s->phase : -1; }
+
+int mcb_solver_selected(mcb_solver *s, int32_t *out, int32_t count) {
+    if (!s || !out || count < 0 || count > s->N) return fail(MCB_E_INVALID, "invalid arguments");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(out, s->best, sizeof(uint32_t) * count, hipMemcpyDeviceToHost));
+    return MCB_OK;
+}
+
+int mcb_solver_support_vectors(mcb_solver *s, uint64_t *out) {
+    if (!s || !out) return fail(MCB_E_INVALID, "invalid arguments");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    std::vector<uint64_t> sw(static_cast<size_t>(s->W) * s->npad);
+    HIP_TRY(hipMemcpy(sw.data(), s->Sw, sizeof(uint64_t) * sw.size(), hipMemcpyDeviceToHost));
+    for (int32_t j = 0; j < s->N; ++j) {
+        uint64_t *row = out + static_cast<size_t>(j) * s->W;
+        for (int32_t w = 0; w < s->W; ++w) row[w] = sw[static_cast<size_t>(w) * s->npad + j];
+        row[j >> 6] |= 1ull << (j & 63);
+    }
+    return MCB_OK;
+}
+
+int mcb_solver_cycle_vectors(mcb_solver *s, uint64_t *out, int32_t count) {
+    if (!s || !out || count < 0 || count > s->N) return fail(MCB_E_INVALID, "invalid arguments");
+    if (!s->C_store) return fail(MCB_E_STATE, "solver was created without store_cycles");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(out, s->C_store, sizeof(uint64_t) * static_cast<size_t>(count) * s->W, hipMemcpyDeviceToHost));
+    return MCB_OK;
+}
+
+int mcb_solver_parity(mcb_solver *s, const uint64_t *S, uint8_t *out) {
+    if (!s || !S || !out) return fail(MCB_E_INVALID, "invalid arguments");
+    uint64_t *dS = nullptr;
+    uint8_t *dP = nullptr;
+    int64_t dummy = 0;
+    int rc = upload(&dS, S, s->W, dummy);
+    if (rc) return rc;
+    rc = dev_alloc(&dP, static_cast<size_t>(s->Ptot), dummy);
+    if (rc) {
+        (void)hipFree(dS);
+        return rc;
+    }
+    SearchArgs a = search_args(s, 0);
+    a.S = dS;
+    a.parity_out = dP;
+    a.best = nullptr;
+    if (s->s_in_lds)
+        hipLaunchKernelGGL((k_parity_search<true, false>), dim3(s->blocks), dim3(s->wpb * kWave), s->lds_bytes,
+                           s->stream, a);
+    else
+        hipLaunchKernelGGL((k_parity_search<false, false>), dim3(s->blocks), dim3(s->wpb * kWave), s->lds_bytes,
+                           s->stream, a);
+    hipError_t e1 = hipGetLastError();
+    hipError_t e2 = hipStreamSynchronize(s->stream);
+    hipError_t e3 = hipMemcpy(out, dP, static_cast<size_t>(s->Ptot), hipMemcpyDeviceToHost);
+    (void)hipFree(dS);
+    (void)hipFree(dP);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return fail(MCB_E_DEVICE, "parity kernel failed");
+    return MCB_OK;
+}
+
+int mcb_solver_kernel_time(mcb_solver *s, int kernel, double *total_ms, int64_t *launches) {
+    if (!s || kernel < 0 || kernel >= MCB_K_COUNT) return fail(MCB_E_INVALID, "invalid arguments");
+    int rc = harvest_events(s);
+    if (rc) return rc;
+    if (total_ms) *total_ms = s->kt_ms[kernel];
+    if (launches) *launches = s->kt_n[kernel];
+    return MCB_OK;
+}
+
+int64_t mcb_solver_device_bytes(const mcb_solver *s) { return s ? s->bytes : -1; }
+
+int mcb_hip_copy_bandwidth(int64_t bytes, int iters, double *gbps) {
+    if (bytes <= 0 || iters <= 0 || !gbps) return fail(MCB_E_INVALID, "invalid arguments");
+    int rc = ensure_device();
+    if (rc) return rc;
+    const int64_t n = bytes / 8;
+    uint64_t *a = nullptr, *b = nullptr;
+    HIP_TRY(hipMalloc(&a, n * 8));
+    if (hipMalloc(&b, n * 8) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(MCB_E_DEVICE, "hipMalloc");
+    }
+    (void)hipMemset(a, 1, n * 8);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const unsigned grid = static_cast<unsigned>(g_dev.prop.multiProcessorCount) * 8;
+    hipLaunchKernelGGL(k_copy_u64, dim3(grid), dim3(256), 0, nullptr, a, b, n);
+    (void)hipEventRecord(e0, nullptr);
+    for (int i = 0; i < iters; ++i) hipLaunchKernelGGL(k_copy_u64, dim3(grid), dim3(256), 0, nullptr, a, b, n);
+    (void)hipEventRecord(e1, nullptr);
+    hipError_t err = hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (err != hipSuccess) return fail(MCB_E_DEVICE, "copy kernel failed");
+    *gbps = 2.0 * static_cast<double>(n * 8) * iters / (ms * 1e-3) / 1e9;
+    return MCB_OK;
+}
+
+}  // extern "C"

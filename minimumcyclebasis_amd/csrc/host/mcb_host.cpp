@@ -1,0 +1,1094 @@
+// Host-side preprocessing for the de Pina support-vector phase loop.
+//
+// Every step whose output order decides which cycle a phase selects is reproduced with the
+// same standard-library containers and the same insertion/sort sequences as the reference,
+// because libstdc++'s std::sort (unstable), std::priority_queue pop order among equal keys and
+// std::unordered_map iteration order are all functions of exactly those sequences:
+//   O1 CSR edge order          include/CsrGraph.h:113-174, include/CsrGraphMulti.h:96-168
+//   O2 relabel order           include/CsrGraphMulti.h:229-300
+//   O3 spanning-tree root      include/CsrGraphDegreeTwoRemoval.cpp:103-112
+//   O4 coordinate order        include/CsrGraphMultiSpanningTree.cpp:11-135
+//   O5 FVS reverse-delete      include/FVS.cpp:166-227
+//   O6 Dijkstra pop order      include/Dijkstra.h:78-127
+//   O7 SP non-tree edge order  include/Dijkstra.h:129-149
+//   O8 candidate rank          mcb/mcb_cycle_deg_two_remove.cpp:244-267, include/cycle_searcher.h:45-53
+// The recursive DFS passes are run with explicit stacks (same visit order, no stack-depth limit),
+// the FVS acyclicity test is an incremental union-find (same accept/reject sequence, O(m a(n))
+// instead of O(n m)), and the candidate filter is a top-down pass per tree (same accepted set as
+// the per-edge LCA walk of include/Dijkstra.h:205-254).
+
+#include "mcb_host.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <numeric>
+#include <queue>
+#include <string>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Graph containers
+// ---------------------------------------------------------------------------------------------
+
+// Simple input graph in CSR form, both directions stored (csr_graph, include/CsrGraph.h:12-218).
+struct InputGraph {
+    int32_t n = 0;
+    std::vector<uint32_t> rows, cols, ro, deg;
+    std::vector<int32_t> w;
+};
+
+// Reduced multigraph with the opposite-direction link of every edge (csr_multi_graph).
+struct Multigraph {
+    int32_t n = 0;
+    std::vector<uint32_t> rows, cols, ro, deg, rev;
+    std::vector<int32_t> w;
+};
+
+struct RowColLess {
+    const uint32_t *r, *c;
+    bool operator()(uint32_t a, uint32_t b) const {
+        if (r[a] == r[b]) return c[a] < c[b];
+        return r[a] < r[b];
+    }
+};
+
+// Sorts edges by (row, col) with std::sort over the insertion sequence, then builds row offsets
+// and degrees (include/CsrGraph.h:113-174).
+void finish_input_csr(InputGraph &g) {
+    const size_t m2 = g.rows.size();
+    std::vector<uint32_t> idx(m2);
+    std::iota(idx.begin(), idx.end(), 0u);
+    std::sort(idx.begin(), idx.end(), RowColLess{g.rows.data(), g.cols.data()});
+    std::vector<uint32_t> r(m2), c(m2);
+    std::vector<int32_t> w(m2);
+    for (size_t i = 0; i < m2; ++i) {
+        r[i] = g.rows[idx[i]];
+        c[i] = g.cols[idx[i]];
+        w[i] = g.w[idx[i]];
+    }
+    g.rows.swap(r);
+    g.cols.swap(c);
+    g.w.swap(w);
+    g.ro.assign(g.n + 1, 0);
+    g.deg.assign(g.n, 0);
+    for (size_t i = 0; i < m2; ++i) g.ro[g.rows[i]]++;
+    uint32_t prev = 0;
+    for (int32_t i = 0; i <= g.n; ++i) {
+        uint32_t cur = g.ro[i];
+        g.ro[i] = prev;
+        prev += cur;
+    }
+    for (int32_t i = 0; i < g.n; ++i) g.deg[i] = g.ro[i + 1] - g.ro[i];
+}
+
+// Appends an undirected edge as two directed edges linked to each other
+// (csr_multi_graph::insert, include/CsrGraphMulti.h:75-90).
+void multi_insert(Multigraph &g, uint32_t a, uint32_t b, int32_t wt) {
+    const uint32_t k = static_cast<uint32_t>(g.rows.size());
+    g.rows.push_back(a);
+    g.cols.push_back(b);
+    g.w.push_back(wt);
+    g.rev.push_back(k + 1);
+    g.rows.push_back(b);
+    g.cols.push_back(a);
+    g.w.push_back(wt);
+    g.rev.push_back(k);
+}
+
+// Sort by (row, col) with std::sort, carrying the opposite-direction links
+// (include/CsrGraphMulti.h:96-168).
+void finish_multi_csr(Multigraph &g) {
+    const size_t m2 = g.rows.size();
+    std::vector<uint32_t> idx(m2);
+    std::iota(idx.begin(), idx.end(), 0u);
+    std::sort(idx.begin(), idx.end(), RowColLess{g.rows.data(), g.cols.data()});
+    std::vector<uint32_t> newpos(m2);
+    for (size_t i = 0; i < m2; ++i) newpos[idx[i]] = static_cast<uint32_t>(i);
+    std::vector<uint32_t> r(m2), c(m2), rv(m2);
+    std::vector<int32_t> w(m2);
+    for (size_t i = 0; i < m2; ++i) {
+        r[i] = g.rows[idx[i]];
+        c[i] = g.cols[idx[i]];
+        w[i] = g.w[idx[i]];
+        rv[i] = newpos[g.rev[idx[i]]];
+    }
+    g.rows.swap(r);
+    g.cols.swap(c);
+    g.w.swap(w);
+    g.rev.swap(rv);
+    g.ro.assign(g.n + 1, 0);
+    g.deg.assign(g.n, 0);
+    for (size_t i = 0; i < m2; ++i) g.ro[g.rows[i]]++;
+    uint32_t prev = 0;
+    for (int32_t i = 0; i <= g.n; ++i) {
+        uint32_t cur = g.ro[i];
+        g.ro[i] = prev;
+        prev += cur;
+    }
+    for (int32_t i = 0; i < g.n; ++i) g.deg[i] = g.ro[i + 1] - g.ro[i];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Degree-two chain contraction (include/CsrGraphDegreeTwoRemoval.cpp:12-126)
+// ---------------------------------------------------------------------------------------------
+
+struct ChainEdge {
+    uint32_t a, b;
+    uint32_t weight;
+};
+
+// Depth-first pass from the first vertex of degree > 2.  An edge is marked for removal when it
+// touches a degree-two vertex; a chain is closed when the walk leaves its last degree-two vertex.
+// Post-order matters: the chain bookkeeping for the edge (row -> child) runs after the child's
+// subtree is finished, exactly where the recursive reference resumes.
+int mark_degree_two_chains(const InputGraph &g, std::vector<uint32_t> &removal,
+                           std::vector<std::vector<uint32_t>> &chains, int32_t &src) {
+    const int32_t n = g.n;
+    src = -1;
+    for (int32_t i = 0; i < n; ++i)
+        if (g.deg[i] > 2) {
+            src = i;
+            break;
+        }
+    if (src < 0) return fail(MCB_E_GRAPH, "no vertex of degree > 2 to start the chain walk from");
+
+    std::vector<uint8_t> visited(n, 0);
+    std::vector<int64_t> parent(n, -1);
+    bool have_temp = false;
+    std::vector<uint32_t> temp;
+
+    auto chain_step = [&](uint32_t row, uint32_t column, uint32_t offset) -> int {
+        const bool dc = g.deg[column] == 2, dr = g.deg[row] == 2;
+        if (dc && dr) {
+            if (!have_temp) return fail(MCB_E_GRAPH, "degree-two chain walk entered a chain mid-way");
+            temp.push_back(offset);
+            removal.push_back(offset);
+        } else if (dr) {
+            temp.assign(1, offset);
+            have_temp = true;
+            removal.push_back(offset);
+        } else if (dc) {
+            if (!have_temp) return fail(MCB_E_GRAPH, "degree-two chain walk closed a chain it never opened");
+            temp.push_back(offset);
+            chains.push_back(temp);
+            temp.clear();
+            have_temp = false;
+            removal.push_back(offset);
+        }
+        return MCB_OK;
+    };
+
+    struct Frame {
+        uint32_t row, off;
+        bool returning;
+    };
+    std::vector<Frame> stack;
+    visited[src] = 1;
+    stack.push_back({static_cast<uint32_t>(src), g.ro[src], false});
+    while (!stack.empty()) {
+        Frame &f = stack.back();
+        const uint32_t row = f.row;
+        if (f.returning) {
+            f.returning = false;
+            const uint32_t off = f.off++;
+            int rc = chain_step(row, g.cols[off], off);
+            if (rc) return rc;
+            continue;
+        }
+        if (f.off >= g.ro[row + 1]) {
+            stack.pop_back();
+            continue;
+        }
+        const uint32_t off = f.off;
+        const uint32_t column = g.cols[off];
+        if (!visited[column]) {
+            visited[column] = 1;
+            parent[column] = row;
+            f.returning = true;
+            stack.push_back({column, g.ro[column], false});
+            continue;
+        }
+        f.off++;
+        if (static_cast<int64_t>(column) == parent[row]) {
+            if (g.deg[column] == 2 || g.deg[row] == 2) removal.push_back(off);
+            continue;
+        }
+        if (g.deg[column] == 2) {
+            removal.push_back(off);
+            continue;
+        }
+        int rc = chain_step(row, column, off);
+        if (rc) return rc;
+    }
+    return MCB_OK;
+}
+
+// Relabel by first appearance and rebuild as a multigraph, new chain edges first
+// (csr_multi_graph::get_modified_graph, include/CsrGraphMulti.h:229-300).
+void build_reduced(const InputGraph &g, const std::vector<uint32_t> *removal,
+                   const std::vector<ChainEdge> *new_edges, Multigraph &out) {
+    const size_t m2 = g.rows.size();
+    std::vector<uint8_t> filter(m2, 0);
+    if (removal)
+        for (uint32_t off : *removal) filter[off] = 1;
+    std::vector<int64_t> label(g.n, -1);
+    int64_t count = 0;
+    for (size_t i = 0; i < m2; ++i) {
+        if (filter[i]) continue;
+        if (label[g.rows[i]] < 0) label[g.rows[i]] = count++;
+        if (label[g.cols[i]] < 0) label[g.cols[i]] = count++;
+    }
+    if (new_edges)
+        for (const ChainEdge &e : *new_edges) {
+            if (label[e.a] < 0) label[e.a] = count++;
+            if (label[e.b] < 0) label[e.b] = count++;
+        }
+    out = Multigraph();
+    out.n = static_cast<int32_t>(count);
+    if (new_edges)
+        for (const ChainEdge &e : *new_edges)
+            multi_insert(out, static_cast<uint32_t>(label[e.a]), static_cast<uint32_t>(label[e.b]),
+                         static_cast<int32_t>(e.weight));
+    for (size_t i = 0; i < m2; ++i)
+        if (!filter[i] && g.rows[i] < g.cols[i])
+            multi_insert(out, static_cast<uint32_t>(label[g.rows[i]]), static_cast<uint32_t>(label[g.cols[i]]),
+                         g.w[i]);
+    finish_multi_csr(out);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Spanning tree and cycle-space coordinates (include/CsrGraphMultiSpanningTree.cpp:11-135)
+// ---------------------------------------------------------------------------------------------
+
+int spanning_tree_coordinates(const Multigraph &g, int32_t src, std::vector<uint32_t> &nt_edges) {
+    const int32_t n = g.n;
+    if (src < 0 || src >= n) return fail(MCB_E_GRAPH, "spanning-tree root is not a vertex of the reduced graph");
+    std::vector<uint8_t> visited(n, 0), is_tree(g.rows.size(), 0);
+    std::vector<int64_t> parent(n, -1);
+    int64_t tree_size = 0;
+    struct Frame {
+        uint32_t row, off;
+    };
+    std::vector<Frame> stack;
+    visited[src] = 1;
+    stack.push_back({static_cast<uint32_t>(src), g.ro[src]});
+    while (!stack.empty()) {
+        Frame &f = stack.back();
+        const uint32_t row = f.row;
+        if (f.off >= g.ro[row + 1]) {
+            stack.pop_back();
+            continue;
+        }
+        const uint32_t off = f.off++;
+        const uint32_t column = g.cols[off];
+        if (!visited[column]) {
+            visited[column] = 1;
+            ++tree_size;
+            parent[column] = row;
+            is_tree[off] = 1;
+            stack.push_back({column, g.ro[column]});
+            continue;
+        }
+        const uint32_t rv = g.rev[off];
+        if (static_cast<int64_t>(column) == parent[row]) {
+            if (is_tree[rv] == 1) {
+                is_tree[off] = 1;
+            } else if (is_tree[rv] == 2) {
+                is_tree[off] = 2;
+            } else {
+                is_tree[off] = 2;
+                is_tree[rv] = 2;
+                nt_edges.push_back(row < column ? off : rv);
+            }
+        } else if (is_tree[off] == 0) {
+            is_tree[off] = 2;
+            is_tree[rv] = 2;
+            nt_edges.push_back(row < column ? off : rv);
+        }
+    }
+    if (tree_size != n - 1) return fail(MCB_E_GRAPH, "reduced graph is not connected");
+    return MCB_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Feedback vertex set (include/FVS.cpp)
+// ---------------------------------------------------------------------------------------------
+
+struct UnionFind {
+    std::vector<int32_t> p;
+    explicit UnionFind(int32_t n) : p(n) { std::iota(p.begin(), p.end(), 0); }
+    int32_t find(int32_t x) {
+        while (p[x] != x) {
+            p[x] = p[p[x]];
+            x = p[x];
+        }
+        return x;
+    }
+};
+
+// The reference greedy (FVS::MGA) picks the vertex of least W/degree; its pruning step marks only
+// the picked vertex dead (FVS.cpp:41-56), so W and degrees never change and the pick order is
+// (degree desc, id asc).  The list is then reversed and each vertex is dropped from the set when
+// the remaining graph stays acyclic (FVS.cpp:194-213).  Dropping v keeps the forest acyclic iff
+// no two of v's edges into non-set vertices land in the same forest component (a parallel edge
+// lands twice in one component); the forest only grows, so one union-find pass decides it.
+int fvs_reverse_delete_impl(int32_t n, const uint32_t *ro, const uint32_t *cols, const int32_t *order,
+                            int32_t *fvs_rank) {
+    std::vector<int32_t> ord;
+    if (order) {
+        ord.assign(order, order + n);
+        std::vector<uint8_t> seen(n, 0);
+        for (int32_t v : ord) {
+            if (v < 0 || v >= n || seen[v]) return fail(MCB_E_INVALID, "FVS order is not a permutation of the vertices");
+            seen[v] = 1;
+        }
+    } else {
+        ord.resize(n);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) {
+            uint32_t da = ro[a + 1] - ro[a], db = ro[b + 1] - ro[b];
+            if (da != db) return da < db;
+            return a > b;
+        });
+    }
+    std::vector<uint8_t> in_set(n, 1);
+    UnionFind uf(n);
+    std::vector<int32_t> comps;
+    for (int32_t v : ord) {
+        comps.clear();
+        bool cycle = false;
+        for (uint32_t e = ro[v]; e < ro[v + 1]; ++e) {
+            uint32_t c = cols[e];
+            if (c == static_cast<uint32_t>(v)) {  // a self-loop is a cycle on its own
+                cycle = true;
+                break;
+            }
+            if (in_set[c]) continue;
+            comps.push_back(uf.find(static_cast<int32_t>(c)));
+        }
+        if (!cycle) {
+            std::sort(comps.begin(), comps.end());
+            cycle = std::adjacent_find(comps.begin(), comps.end()) != comps.end();
+        }
+        if (cycle) continue;  // v stays in the set
+        in_set[v] = 0;
+        for (int32_t r : comps) uf.p[uf.find(r)] = v;
+    }
+    int32_t count = 0;
+    for (int32_t i = 0; i < n; ++i) fvs_rank[i] = in_set[i] ? count++ : -1;
+    return count;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Per-root shortest-path trees and candidates
+// ---------------------------------------------------------------------------------------------
+
+struct HeapLess {  // include/Dijkstra.h:37-43: the heap's top is the least distance
+    bool operator()(const std::pair<int, int> &a, const std::pair<int, int> &b) const {
+        return a.second > b.second;
+    }
+};
+
+struct SpTree {
+    std::vector<int32_t> dist, level, pedge;
+    std::vector<uint8_t> in_tree;
+    std::vector<uint32_t> order;  // vertices in finalisation order, root excluded
+};
+
+// dijkstra::dijkstra_sp (include/Dijkstra.h:78-127): lazy-deletion binary heap, strict '<' relax.
+void dijkstra(const Multigraph &g, uint32_t src, SpTree &t) {
+    const int32_t n = g.n;
+    t.dist.assign(n, -1);
+    t.level.assign(n, -1);
+    t.pedge.assign(n, -1);
+    t.in_tree.assign(n, 0);
+    t.order.clear();
+    t.dist[src] = 0;
+    t.level[src] = 0;
+    std::priority_queue<std::pair<int, int>, std::vector<std::pair<int, int>>, HeapLess> pq;
+    pq.push(std::make_pair(static_cast<int>(src), 0));
+    while (!pq.empty()) {
+        std::pair<int, int> val = pq.top();
+        pq.pop();
+        const uint32_t u = static_cast<uint32_t>(val.first);
+        if (t.in_tree[u]) continue;
+        if (u != src) t.order.push_back(u);
+        t.in_tree[u] = 1;
+        for (uint32_t off = g.ro[u]; off < g.ro[u + 1]; ++off) {
+            const uint32_t c = g.cols[off];
+            if (t.in_tree[c]) continue;
+            const int32_t cand = t.dist[u] + g.w[off];
+            if (t.dist[c] == -1 || cand < t.dist[c]) {
+                t.dist[c] = cand;
+                pq.push(std::make_pair(static_cast<int>(c), cand));
+                t.pedge[c] = static_cast<int32_t>(off);
+                t.level[c] = t.level[u] + 1;
+            }
+        }
+    }
+}
+
+struct RootOutput {
+    // candidates in the order the reference collects them for this root (map iteration order)
+    std::vector<uint32_t> cand_edge;
+    std::vector<int32_t> cand_weight;
+    std::vector<uint32_t> cand_pu, cand_pv;
+    // position layout
+    std::vector<uint32_t> parent_pos;
+    std::vector<int32_t> coord;
+    std::vector<uint32_t> vertex;  // only with keep_vertex_trees
+    int32_t depth = 0;
+};
+
+struct Workspace {
+    SpTree t;
+    std::vector<uint8_t> blocked, keep, is_tree_edge;
+    std::vector<uint32_t> branch, pos_of, nt_list, level_count, level_start;
+    std::vector<uint32_t> cand_list;  // edge offsets accepted, in SP non-tree order
+    std::vector<int32_t> cand_w;
+};
+
+}  // namespace
+
+struct mcb_problem {
+    mcb_problem_info info{};
+    Multigraph g;
+    std::vector<int32_t> fvs_rank;
+    std::vector<uint32_t> roots, nt_edges;
+    std::vector<int32_t> nt_map;
+    std::vector<uint32_t> cand_root, cand_edge, cand_pu, cand_pv;
+    std::vector<int32_t> cand_weight, cand_coord;
+    std::vector<int64_t> tree_offset, root_cand_offset;
+    std::vector<uint32_t> parent_pos, root_cand_rank;
+    std::vector<int32_t> pos_coord, vertex_parent;
+    std::vector<uint32_t> pos_vertex;
+};
+
+namespace {
+
+// One root: Dijkstra, SP non-tree edges (include/Dijkstra.h:129-149), the candidate filter of
+// include/Dijkstra.h:205-254 and the grouping of include/cycle_searcher.h:45-53.
+void process_root(const mcb_problem &P, uint32_t src, bool prune, Workspace &ws, RootOutput &out,
+                  int32_t *vertex_parent_row) {
+    const Multigraph &g = P.g;
+    const int32_t n = g.n;
+    SpTree &t = ws.t;
+    dijkstra(g, src, t);
+    if (vertex_parent_row)
+        for (int32_t v = 0; v < n; ++v) vertex_parent_row[v] = t.pedge[v];
+
+    // SP non-tree edges: first direction met in offset order.
+    const size_t m2 = g.rows.size();
+    ws.is_tree_edge.assign(m2, 0);
+    for (uint32_t v : t.order) ws.is_tree_edge[t.pedge[v]] = 1;
+    ws.nt_list.clear();
+    for (size_t i = 0; i < m2; ++i) {
+        if (ws.is_tree_edge[i] == 1) continue;
+        const uint8_t r = ws.is_tree_edge[g.rev[i]];
+        if (r == 1) {
+            ws.is_tree_edge[i] = 1;
+        } else if (r == 2) {
+            ws.is_tree_edge[i] = 2;
+        } else {
+            ws.is_tree_edge[i] = 2;
+            ws.nt_list.push_back(static_cast<uint32_t>(i));
+        }
+    }
+
+    // Top-down: blocked[x] = the path root..x holds a root vertex of smaller id than src;
+    // branch[x] = the child of src whose subtree holds x.
+    ws.blocked.assign(n, 0);
+    ws.branch.assign(n, 0xFFFFFFFFu);
+    for (uint32_t v : t.order) {
+        const uint32_t p = g.rows[t.pedge[v]];
+        ws.blocked[v] = ws.blocked[p] || (P.fvs_rank[v] >= 0 && v < src);
+        ws.branch[v] = (p == src) ? v : ws.branch[p];
+    }
+
+    // Accept (u,v): the tree paths from u and v meet first at src and avoid smaller roots.
+    std::unordered_map<unsigned long long, uint32_t> groups_of;
+    std::vector<std::vector<uint32_t>> groups;
+    for (uint32_t e : ws.nt_list) {
+        const uint32_t u = g.rows[e], v = g.cols[e];
+        if (ws.blocked[u] || ws.blocked[v]) continue;
+        if (!(u == src || v == src || ws.branch[u] != ws.branch[v])) continue;
+        const unsigned long long key =
+            (static_cast<unsigned long long>(std::min(u, v)) << 32) | std::max(u, v);
+        auto it = groups_of.find(key);
+        if (it == groups_of.end()) {
+            groups_of.insert(std::make_pair(key, static_cast<uint32_t>(groups.size())));
+            groups.emplace_back(1, e);
+        } else {
+            groups[it->second].push_back(e);
+        }
+    }
+    out.cand_edge.clear();
+    out.cand_weight.clear();
+    for (auto &kv : groups_of)
+        for (uint32_t e : groups[kv.second]) {
+            out.cand_edge.push_back(e);
+            out.cand_weight.push_back(t.dist[g.rows[e]] + t.dist[g.cols[e]] + g.w[e]);
+        }
+
+    // Which vertices get a position: all of them, or the closure of the candidate endpoints
+    // under the parent relation (every ancestor of a kept vertex is kept).
+    ws.keep.assign(n, prune ? 0 : 1);
+    ws.keep[src] = 1;
+    if (prune) {
+        for (uint32_t e : out.cand_edge) {
+            for (uint32_t x : {g.rows[e], g.cols[e]}) {
+                while (!ws.keep[x]) {
+                    ws.keep[x] = 1;
+                    x = g.rows[t.pedge[x]];
+                }
+            }
+        }
+    }
+    // BFS-level layout: level buckets, finalisation order inside a level (parents come first).
+    int32_t maxlev = 0;
+    for (uint32_t v : t.order)
+        if (ws.keep[v]) maxlev = std::max(maxlev, t.level[v]);
+    ws.level_count.assign(maxlev + 2, 0);
+    ws.level_count[0] = 1;
+    for (uint32_t v : t.order)
+        if (ws.keep[v]) ws.level_count[t.level[v]]++;
+    ws.level_start.assign(maxlev + 2, 0);
+    for (int32_t l = 1; l <= maxlev + 1; ++l) ws.level_start[l] = ws.level_start[l - 1] + ws.level_count[l - 1];
+    const uint32_t npos = ws.level_start[maxlev + 1];
+    ws.pos_of.assign(n, 0xFFFFFFFFu);
+    ws.pos_of[src] = 0;
+    ws.level_start[0] = 1;
+    for (uint32_t v : t.order)
+        if (ws.keep[v]) ws.pos_of[v] = ws.level_start[t.level[v]]++;
+    out.parent_pos.assign(npos, 0xFFFFFFFFu);
+    out.coord.assign(npos, -1);
+    for (uint32_t v : t.order) {
+        if (!ws.keep[v]) continue;
+        const uint32_t pos = ws.pos_of[v];
+        const uint32_t e = static_cast<uint32_t>(t.pedge[v]);
+        out.parent_pos[pos] = ws.pos_of[g.rows[e]];
+        out.coord[pos] = P.nt_map[e];
+    }
+    out.depth = maxlev;
+    if (vertex_parent_row) {
+        out.vertex.assign(npos, 0);
+        out.vertex[0] = src;
+        for (uint32_t v : t.order)
+            if (ws.keep[v]) out.vertex[ws.pos_of[v]] = v;
+    }
+    out.cand_pu.resize(out.cand_edge.size());
+    out.cand_pv.resize(out.cand_edge.size());
+    for (size_t k = 0; k < out.cand_edge.size(); ++k) {
+        out.cand_pu[k] = ws.pos_of[g.rows[out.cand_edge[k]]];
+        out.cand_pv[k] = ws.pos_of[g.cols[out.cand_edge[k]]];
+    }
+}
+
+int build_from_input(InputGraph &in, int64_t input_weight_sum, const mcb_build_options &opt, mcb_problem **out) {
+    std::unique_ptr<mcb_problem> P(new (std::nothrow) mcb_problem());
+    if (!P) return fail(MCB_E_NOMEM, "out of memory");
+    finish_input_csr(in);
+    mcb_problem_info &info = P->info;
+    info.input_nodes = in.n;
+    info.input_edges = static_cast<int32_t>(in.rows.size() / 2);
+    info.input_total_weight = input_weight_sum;
+    for (uint32_t i = 0; i < in.rows.size(); ++i)
+        if (in.rows[i] == in.cols[i]) return fail(MCB_E_GRAPH, "input graph has a self-loop");
+
+    int32_t deg2 = 0;
+    for (int32_t i = 0; i < in.n; ++i) deg2 += (in.deg[i] == 2);
+    if (deg2 == in.n) {  // the whole graph is one cycle (mcb_cycle_deg_two_remove.cpp:104-113)
+        info.single_cycle = 1;
+        info.nodes_removed = in.n - 1;
+        info.num_non_tree = 1;
+        info.words = 1;
+        info.fvs_size = 1;
+        info.num_candidates = 1;
+        info.nodes = in.n;
+        info.directed_edges = static_cast<int32_t>(in.rows.size());
+        *out = P.release();
+        return MCB_OK;
+    }
+
+    int32_t src = 0;
+    if (opt.reduce_degree_two) {
+        std::vector<uint32_t> removal;
+        std::vector<std::vector<uint32_t>> chains;
+        int rc = mark_degree_two_chains(in, removal, chains, src);
+        if (rc) return rc;
+        std::vector<ChainEdge> new_edges;
+        int32_t removed = 0;
+        for (const auto &ch : chains) {
+            uint32_t total = 0;
+            for (uint32_t off : ch) total += static_cast<uint32_t>(in.w[off]);
+            new_edges.push_back({in.rows[ch.back()], in.cols[ch.front()], total});
+            removed += static_cast<int32_t>(ch.size()) - 1;
+        }
+        info.nodes_removed = removed;
+        build_reduced(in, &removal, &new_edges, P->g);
+    } else {
+        build_reduced(in, nullptr, nullptr, P->g);
+    }
+    in = InputGraph();  // release the input graph early
+
+    Multigraph &g = P->g;
+    info.source_vertex = src;
+    info.nodes = g.n;
+    info.directed_edges = static_cast<int32_t>(g.rows.size());
+
+    // FVS and root numbering by vertex id.
+    P->fvs_rank.assign(g.n, -1);
+    const int32_t *order = nullptr;
+    if (opt.fvs_order) {
+        if (opt.fvs_order_len != g.n) return fail(MCB_E_INVALID, "FVS order length differs from the reduced vertex count");
+        order = opt.fvs_order;
+    }
+    int F = fvs_reverse_delete_impl(g.n, g.ro.data(), g.cols.data(), order, P->fvs_rank.data());
+    if (F < 0) return F;
+    info.fvs_size = F;
+    P->roots.resize(F);
+    for (int32_t v = 0; v < g.n; ++v)
+        if (P->fvs_rank[v] >= 0) P->roots[P->fvs_rank[v]] = static_cast<uint32_t>(v);
+
+    // Coordinates.
+    int rc = spanning_tree_coordinates(g, src, P->nt_edges);
+    if (rc) return rc;
+    const int32_t N = static_cast<int32_t>(P->nt_edges.size());
+    info.num_non_tree = N;
+    info.words = (N + 63) / 64;
+    P->nt_map.assign(g.rows.size(), -1);
+    for (int32_t i = 0; i < N; ++i) P->nt_map[P->nt_edges[i]] = i;
+    for (size_t i = 0; i < g.rows.size(); ++i)
+        if (P->nt_map[i] < 0 && P->nt_map[g.rev[i]] >= 0) P->nt_map[i] = P->nt_map[g.rev[i]];
+
+    // Per-root trees and candidates.
+    std::vector<RootOutput> per_root(F);
+    if (opt.keep_vertex_trees) P->vertex_parent.assign(static_cast<size_t>(F) * g.n, -1);
+    int threads = opt.num_threads > 0 ? opt.num_threads : omp_get_max_threads();
+    int err = 0;
+#pragma omp parallel num_threads(threads)
+    {
+        Workspace ws;
+#pragma omp for schedule(dynamic, 4)
+        for (int32_t r = 0; r < F; ++r) {
+            try {
+                process_root(*P, P->roots[r], opt.prune_trees != 0, ws, per_root[r],
+                             opt.keep_vertex_trees ? P->vertex_parent.data() + static_cast<size_t>(r) * g.n : nullptr);
+            } catch (const std::bad_alloc &) {
+#pragma omp atomic write
+                err = 1;
+            }
+        }
+    }
+    if (err) return fail(MCB_E_NOMEM, "out of memory while building shortest-path trees");
+
+    // Collect candidates root by root (roots ascend by vertex id) and rank them by weight with
+    // the same std::sort over the same sequence (mcb_cycle_deg_two_remove.cpp:247-258).
+    int64_t K = 0, positions = 0;
+    int32_t max_pos = 0, max_depth = 0;
+    for (const RootOutput &o : per_root) {
+        K += static_cast<int64_t>(o.cand_edge.size());
+        positions += static_cast<int64_t>(o.parent_pos.size());
+        max_pos = std::max<int32_t>(max_pos, static_cast<int32_t>(o.parent_pos.size()));
+        max_depth = std::max(max_depth, o.depth);
+    }
+    if (K > 0xFFFFFFFFll - 1) return fail(MCB_E_GRAPH, "more than 2^32-1 candidate cycles");
+    info.num_candidates = K;
+    info.positions = positions;
+    info.max_positions = max_pos;
+    info.max_depth = max_depth;
+
+    std::vector<uint32_t> c_root(K), c_edge(K), c_pu(K), c_pv(K);
+    std::vector<int32_t> c_w(K);
+    {
+        int64_t k = 0;
+        for (int32_t r = 0; r < F; ++r) {
+            const RootOutput &o = per_root[r];
+            for (size_t i = 0; i < o.cand_edge.size(); ++i, ++k) {
+                c_root[k] = static_cast<uint32_t>(r);
+                c_edge[k] = o.cand_edge[i];
+                c_w[k] = o.cand_weight[i];
+                c_pu[k] = o.cand_pu[i];
+                c_pv[k] = o.cand_pv[i];
+            }
+        }
+    }
+    std::vector<uint32_t> rank_order(K);
+    std::iota(rank_order.begin(), rank_order.end(), 0u);
+    std::sort(rank_order.begin(), rank_order.end(),
+              [&](uint32_t a, uint32_t b) { return c_w[a] < c_w[b]; });
+    P->cand_root.resize(K);
+    P->cand_edge.resize(K);
+    P->cand_weight.resize(K);
+    P->cand_pu.resize(K);
+    P->cand_pv.resize(K);
+    P->cand_coord.resize(K);
+    for (int64_t k = 0; k < K; ++k) {
+        const uint32_t s = rank_order[k];
+        P->cand_root[k] = c_root[s];
+        P->cand_edge[k] = c_edge[s];
+        P->cand_weight[k] = c_w[s];
+        P->cand_pu[k] = c_pu[s];
+        P->cand_pv[k] = c_pv[s];
+        P->cand_coord[k] = P->nt_map[c_edge[s]];
+    }
+    // Per-root candidate lists, ranks ascending.
+    P->root_cand_offset.assign(F + 1, 0);
+    for (int64_t k = 0; k < K; ++k) P->root_cand_offset[P->cand_root[k] + 1]++;
+    for (int32_t r = 0; r < F; ++r) P->root_cand_offset[r + 1] += P->root_cand_offset[r];
+    P->root_cand_rank.resize(K);
+    {
+        std::vector<int64_t> fill(P->root_cand_offset.begin(), P->root_cand_offset.end() - 1);
+        for (int64_t k = 0; k < K; ++k) P->root_cand_rank[fill[P->cand_root[k]]++] = static_cast<uint32_t>(k);
+    }
+    // Concatenated position layouts.
+    P->tree_offset.assign(F + 1, 0);
+    for (int32_t r = 0; r < F; ++r)
+        P->tree_offset[r + 1] = P->tree_offset[r] + static_cast<int64_t>(per_root[r].parent_pos.size());
+    P->parent_pos.resize(positions);
+    P->pos_coord.resize(positions);
+    if (opt.keep_vertex_trees) P->pos_vertex.resize(positions);
+    for (int32_t r = 0; r < F; ++r) {
+        std::copy(per_root[r].parent_pos.begin(), per_root[r].parent_pos.end(),
+                  P->parent_pos.begin() + P->tree_offset[r]);
+        std::copy(per_root[r].coord.begin(), per_root[r].coord.end(), P->pos_coord.begin() + P->tree_offset[r]);
+        if (opt.keep_vertex_trees)
+            std::copy(per_root[r].vertex.begin(), per_root[r].vertex.end(), P->pos_vertex.begin() + P->tree_offset[r]);
+        RootOutput().parent_pos.swap(per_root[r].parent_pos);
+        per_root[r] = RootOutput();
+    }
+    *out = P.release();
+    return MCB_OK;
+}
+
+bool read_line(FILE *f, std::string &line) {
+    line.clear();
+    int ch;
+    while ((ch = fgetc(f)) != EOF) {
+        if (ch == '\n') return true;
+        line.push_back(static_cast<char>(ch));
+    }
+    return !line.empty();
+}
+
+std::string lower(std::string s) {
+    for (char &c : s) c = static_cast<char>(tolower(static_cast<unsigned char>(c)));
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+void mcb_build_options_init(mcb_build_options *opt) {
+    if (!opt) return;
+    std::memset(opt, 0, sizeof(*opt));
+    opt->reduce_degree_two = 1;
+    opt->prune_trees = 0;
+    opt->keep_vertex_trees = 0;
+    opt->num_threads = 0;
+}
+
+const char *mcb_host_last_error(void) { return g_err.c_str(); }
+
+int mcb_problem_build_edges(int32_t nodes, int32_t edges, const int32_t *u, const int32_t *v, const int32_t *w,
+                            const mcb_build_options *opt, mcb_problem **out) {
+    if (!out || nodes <= 0 || edges < 0 || (edges > 0 && (!u || !v || !w)))
+        return fail(MCB_E_INVALID, "invalid arguments");
+    *out = nullptr;
+    mcb_build_options def;
+    mcb_build_options_init(&def);
+    if (!opt) opt = &def;
+    try {
+        InputGraph g;
+        g.n = nodes;
+        g.rows.reserve(2 * static_cast<size_t>(edges));
+        g.cols.reserve(2 * static_cast<size_t>(edges));
+        g.w.reserve(2 * static_cast<size_t>(edges));
+        int64_t sum = 0;
+        for (int32_t i = 0; i < edges; ++i) {
+            if (u[i] < 0 || u[i] >= nodes || v[i] < 0 || v[i] >= nodes)
+                return fail(MCB_E_INVALID, "edge endpoint out of range");
+            g.rows.push_back(static_cast<uint32_t>(u[i]));
+            g.cols.push_back(static_cast<uint32_t>(v[i]));
+            g.w.push_back(w[i]);
+            g.rows.push_back(static_cast<uint32_t>(v[i]));
+            g.cols.push_back(static_cast<uint32_t>(u[i]));
+            g.w.push_back(w[i]);
+            sum += w[i];
+        }
+        return build_from_input(g, sum, *opt, out);
+    } catch (const std::bad_alloc &) {
+        return fail(MCB_E_NOMEM, "out of memory");
+    }
+}
+
+// Matrix-Market reader with the acceptance rules of include/FileReader.h:27-67 (coordinate,
+// integer or real, symmetric or general; entries "u v w", 1-based).  Values must be integers.
+int mcb_problem_build_mtx(const char *path, const mcb_build_options *opt, mcb_problem **out) {
+    if (!path || !out) return fail(MCB_E_INVALID, "invalid arguments");
+    *out = nullptr;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return fail(MCB_E_IO, std::string("cannot open ") + path);
+    std::string line;
+    if (!read_line(f, line)) {
+        std::fclose(f);
+        return fail(MCB_E_FORMAT, "empty file");
+    }
+    char b0[64] = {0}, b1[64] = {0}, b2[64] = {0}, b3[64] = {0}, b4[64] = {0};
+    if (std::sscanf(line.c_str(), "%63s %63s %63s %63s %63s", b0, b1, b2, b3, b4) != 5 ||
+        std::string(b0) != "%%MatrixMarket") {
+        std::fclose(f);
+        return fail(MCB_E_FORMAT, "Could not process Matrix Market banner.");
+    }
+    const std::string obj = lower(b1), fmt = lower(b2), field = lower(b3), sym = lower(b4);
+    if (!(obj == "matrix" && fmt == "coordinate" && (field == "integer" || field == "real") &&
+          (sym == "symmetric" || sym == "general"))) {
+        std::fclose(f);
+        return fail(MCB_E_FORMAT, "Sorry, this application does not support this mtx file.");
+    }
+    long long M = 0, Ncol = 0, nnz = 0;
+    bool got = false;
+    while (read_line(f, line)) {
+        size_t p = line.find_first_not_of(" \t\r");
+        if (p == std::string::npos || line[p] == '%') continue;
+        if (std::sscanf(line.c_str(), "%lld %lld %lld", &M, &Ncol, &nnz) == 3) got = true;
+        break;
+    }
+    if (!got || M <= 0 || nnz < 0 || M > 0x7FFFFFFF || nnz > 0x3FFFFFFF) {
+        std::fclose(f);
+        return fail(MCB_E_FORMAT, "Couldn't find all 3 parameters");
+    }
+    std::vector<int32_t> uu(nnz), vv(nnz), ww(nnz);
+    for (long long i = 0; i < nnz; ++i) {
+        long long a, b;
+        double wt;
+        if (std::fscanf(f, "%lld %lld %lf", &a, &b, &wt) != 3) {
+            std::fclose(f);
+            return fail(MCB_E_FORMAT, "truncated edge list");
+        }
+        if (wt != static_cast<double>(static_cast<int32_t>(wt))) {
+            std::fclose(f);
+            return fail(MCB_E_FORMAT, "edge weight is not a 32-bit integer");
+        }
+        uu[i] = static_cast<int32_t>(a - 1);
+        vv[i] = static_cast<int32_t>(b - 1);
+        ww[i] = static_cast<int32_t>(wt);
+    }
+    std::fclose(f);
+    return mcb_problem_build_edges(static_cast<int32_t>(M), static_cast<int32_t>(nnz), uu.data(), vv.data(),
+                                   ww.data(), opt, out);
+}
+
+int mcb_problem_get_info(const mcb_problem *p, mcb_problem_info *info) {
+    if (!p || !info) return fail(MCB_E_INVALID, "invalid arguments");
+    *info = p->info;
+    return MCB_OK;
+}
+
+int mcb_problem_array(const mcb_problem *p, int which, const void **data, int64_t *count) {
+    if (!p || !data || !count) return fail(MCB_E_INVALID, "invalid arguments");
+#define MCB_ARR(id, vec)                                      \
+    case id:                                                  \
+        *data = (p->vec).data();                              \
+        *count = static_cast<int64_t>((p->vec).size());       \
+        return MCB_OK;
+    switch (which) {
+        MCB_ARR(MCB_A_ROWS, g.rows)
+        MCB_ARR(MCB_A_COLS, g.cols)
+        MCB_ARR(MCB_A_WEIGHTS, g.w)
+        MCB_ARR(MCB_A_REVERSE, g.rev)
+        MCB_ARR(MCB_A_ROW_OFFSETS, g.ro)
+        MCB_ARR(MCB_A_FVS_RANK, fvs_rank)
+        MCB_ARR(MCB_A_ROOTS, roots)
+        MCB_ARR(MCB_A_NT_EDGES, nt_edges)
+        MCB_ARR(MCB_A_NT_MAP, nt_map)
+        MCB_ARR(MCB_A_CAND_ROOT, cand_root)
+        MCB_ARR(MCB_A_CAND_EDGE, cand_edge)
+        MCB_ARR(MCB_A_CAND_WEIGHT, cand_weight)
+        MCB_ARR(MCB_A_CAND_POS_U, cand_pu)
+        MCB_ARR(MCB_A_CAND_POS_V, cand_pv)
+        MCB_ARR(MCB_A_CAND_COORD, cand_coord)
+        MCB_ARR(MCB_A_TREE_OFFSET, tree_offset)
+        MCB_ARR(MCB_A_PARENT_POS, parent_pos)
+        MCB_ARR(MCB_A_POS_COORD, pos_coord)
+        MCB_ARR(MCB_A_ROOT_CAND_OFFSET, root_cand_offset)
+        MCB_ARR(MCB_A_ROOT_CAND_RANK, root_cand_rank)
+        MCB_ARR(MCB_A_VERTEX_PARENT, vertex_parent)
+        MCB_ARR(MCB_A_POS_VERTEX, pos_vertex)
+        default:
+            return fail(MCB_E_INVALID, "unknown array id");
+    }
+#undef MCB_ARR
+}
+
+namespace {
+struct LevelSorter {  // edge_sorter (include/Dijkstra.h:7-21)
+    int edge;
+    int level;
+};
+}  // namespace
+
+int mcb_problem_reference_layout(const mcb_problem *p, int32_t r, uint32_t *rows, uint32_t *cols, int32_t *edge_offset,
+                                 uint32_t *nodes_index) {
+    if (!p || !rows || !cols || !edge_offset || !nodes_index) return fail(MCB_E_INVALID, "invalid arguments");
+    if (p->info.single_cycle || r < 0 || r >= p->info.fvs_size) return fail(MCB_E_INVALID, "root index out of range");
+    const Multigraph &g = p->g;
+    const int32_t n = g.n;
+    const uint32_t src = p->roots[r];
+    SpTree t;
+    dijkstra(g, src, t);
+    std::vector<LevelSorter> edges;
+    edges.reserve(n);
+    edges.push_back({-1, 0});
+    for (uint32_t v : t.order) edges.push_back({t.pedge[v], t.level[v]});
+    std::sort(edges.begin(), edges.end(),
+              [](const LevelSorter &a, const LevelSorter &b) { return a.level < b.level; });
+    std::memset(rows, 0, sizeof(uint32_t) * (n + 1));
+    for (size_t i = 0; i < edges.size(); ++i) {
+        rows[edges[i].level]++;
+        if (edges[i].edge == -1) {
+            nodes_index[src] = static_cast<uint32_t>(i);
+            cols[i] = 0xFFFFFFFFu;
+            edge_offset[i] = -1;
+        } else {
+            const uint32_t row = g.rows[edges[i].edge], col = g.cols[edges[i].edge];
+            nodes_index[col] = static_cast<uint32_t>(i);
+            cols[i] = nodes_index[row];
+            edge_offset[i] = edges[i].edge;
+        }
+    }
+    uint32_t prev = 0;
+    for (int32_t i = 0; i <= n; ++i) {
+        uint32_t cur = rows[i];
+        rows[i] = prev;
+        prev += cur;
+    }
+    return MCB_OK;
+}
+
+void mcb_problem_free(mcb_problem *p) { delete p; }
+
+int mcb_fvs_reverse_delete(int32_t nodes, int32_t directed_edges, const uint32_t *row_offsets, const uint32_t *cols,
+                           const int32_t *order, int32_t *fvs_rank) {
+    if (nodes <= 0 || directed_edges < 0 || !row_offsets || !cols || !fvs_rank)
+        return fail(MCB_E_INVALID, "invalid arguments");
+    if (row_offsets[nodes] != static_cast<uint32_t>(directed_edges))
+        return fail(MCB_E_INVALID, "row offsets do not match the edge count");
+    try {
+        return fvs_reverse_delete_impl(nodes, row_offsets, cols, order, fvs_rank);
+    } catch (const std::bad_alloc &) {
+        return fail(MCB_E_NOMEM, "out of memory");
+    }
+}
+
+int mcb_induced_has_cycle(int32_t nodes, const uint32_t *row_offsets, const uint32_t *cols, const uint8_t *in_set) {
+    if (nodes <= 0 || !row_offsets || !cols || !in_set) return fail(MCB_E_INVALID, "invalid arguments");
+    // Count each undirected edge once via its lower-offset direction: an edge joining two vertices
+    // already connected closes a cycle.  Parallel edges appear as two offsets per direction.
+    UnionFind uf(nodes);
+    for (int32_t v = 0; v < nodes; ++v) {
+        if (in_set[v]) continue;
+        for (uint32_t e = row_offsets[v]; e < row_offsets[v + 1]; ++e) {
+            const uint32_t c = cols[e];
+            if (in_set[c]) continue;
+            if (c == static_cast<uint32_t>(v)) return 1;
+            if (c < static_cast<uint32_t>(v)) continue;  // the (c -> v) direction was counted at c
+            const int32_t a = uf.find(v), b = uf.find(static_cast<int32_t>(c));
+            if (a == b) return 1;
+            uf.p[a] = b;
+        }
+    }
+    return 0;
+}
+
+int mcb_biconnected_components(int32_t nodes, int32_t edges, const int32_t *u, const int32_t *v,
+                               int32_t *edge_component) {
+    if (nodes <= 0 || edges < 0 || (edges > 0 && (!u || !v || !edge_component)))
+        return fail(MCB_E_INVALID, "invalid arguments");
+    // Iterative Hopcroft-Tarjan with an edge stack.
+    std::vector<uint32_t> ro(nodes + 1, 0), adj_to(2 * static_cast<size_t>(edges)), adj_id(2 * static_cast<size_t>(edges));
+    for (int32_t i = 0; i < edges; ++i) {
+        if (u[i] < 0 || u[i] >= nodes || v[i] < 0 || v[i] >= nodes) return fail(MCB_E_INVALID, "endpoint out of range");
+        ro[u[i] + 1]++;
+        ro[v[i] + 1]++;
+    }
+    for (int32_t i = 0; i < nodes; ++i) ro[i + 1] += ro[i];
+    {
+        std::vector<uint32_t> fill(ro.begin(), ro.end() - 1);
+        for (int32_t i = 0; i < edges; ++i) {
+            adj_to[fill[u[i]]] = static_cast<uint32_t>(v[i]);
+            adj_id[fill[u[i]]++] = static_cast<uint32_t>(i);
+            adj_to[fill[v[i]]] = static_cast<uint32_t>(u[i]);
+            adj_id[fill[v[i]]++] = static_cast<uint32_t>(i);
+        }
+    }
+    std::vector<int64_t> disc(nodes, -1), low(nodes, 0);
+    std::vector<uint32_t> estack;
+    for (int32_t i = 0; i < edges; ++i) edge_component[i] = -1;
+    struct Frame {
+        uint32_t vtx, it;
+        int64_t pedge;
+    };
+    std::vector<Frame> st;
+    int64_t time = 0;
+    int32_t comp = 0;
+    for (int32_t s = 0; s < nodes; ++s) {
+        if (disc[s] >= 0) continue;
+        disc[s] = low[s] = time++;
+        st.push_back({static_cast<uint32_t>(s), ro[s], -1});
+        while (!st.empty()) {
+            Frame &f = st.back();
+            const uint32_t x = f.vtx;
+            if (f.it < ro[x + 1]) {
+                const uint32_t k = f.it++;
+                const uint32_t y = adj_to[k], id = adj_id[k];
+                if (static_cast<int64_t>(id) == f.pedge) continue;
+                if (disc[y] < 0) {
+                    estack.push_back(id);
+                    disc[y] = low[y] = time++;
+                    st.push_back({y, ro[y], static_cast<int64_t>(id)});
+                } else if (disc[y] < disc[x]) {
+                    estack.push_back(id);
+                    low[x] = std::min(low[x], disc[y]);
+                }
+                continue;
+            }
+            const int64_t pe = f.pedge;
+            st.pop_back();
+            if (st.empty()) break;
+            const uint32_t p = st.back().vtx;
+            low[p] = std::min(low[p], low[x]);
+            if (low[x] >= disc[p]) {
+                while (!estack.empty()) {
+                    const uint32_t id = estack.back();
+                    estack.pop_back();
+                    edge_component[id] = comp;
+                    if (static_cast<int64_t>(id) == pe) break;
+                }
+                ++comp;
+            }
+        }
+    }
+    return comp;
+}
+
+}  // extern "C"

@@ -1,0 +1,236 @@
+"""Python front end of the gfx950 device library (``libmcb_hip.so``).
+
+* ``Solver``    - the native device-resident phase loop over a ``host.Problem``.
+* ``GpuStruct`` - the reference-compatible chunked parity interface (gpu_struct,
+  gpu/gpu_struct.cuh:15-84) over numpy chunk arrays in the compressed_trees layout.
+* ``minimum_cycle_basis`` - end to end: preprocess on the host, run every phase on the GPU.
+
+There is no CPU fallback: without the HIP library or a GPU every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._hipsig import SolverDesc, SolverOptions
+from .host import Problem
+
+KERNELS = ("parity_search", "cycle", "dot", "xor")
+
+
+def device_count() -> int:
+    return int(_lib.hip().mcb_hip_device_count())
+
+
+def device_info() -> dict:
+    buf = ctypes.create_string_buffer(256)
+    cus = ctypes.c_int()
+    mem = ctypes.c_int64()
+    _lib.check_hip(_lib.hip().mcb_hip_device_info(buf, 256, ctypes.byref(cus), ctypes.byref(mem)))
+    return {"name": buf.value.decode(), "compute_units": cus.value, "total_mem": mem.value}
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, iters: int = 20) -> float:
+    """GB/s of a plain device copy kernel (read + write bytes)."""
+    g = ctypes.c_double()
+    _lib.check_hip(_lib.hip().mcb_hip_copy_bandwidth(int(nbytes), int(iters), ctypes.byref(g)))
+    return g.value
+
+
+class Solver:
+    """Device-resident de Pina phase loop for one preprocessed problem."""
+
+    def __init__(self, problem: Problem, store_cycles: bool = False, timing: bool = False,
+                 waves_per_block: int = 0, max_blocks: int = 0):
+        info = problem.info
+        if info.single_cycle:
+            raise ValueError("a single-cycle graph has no phase loop")
+        self.problem = problem
+        self.N = info.num_non_tree
+        self.W = info.words
+        self.F = info.fvs_size
+        self.K = info.num_candidates
+        self.P = info.positions
+        self._keep = [problem.array(n) for n in ("tree_offset", "parent_pos", "pos_coord", "cand_root", "cand_pos_u",
+                                                 "cand_pos_v", "cand_coord", "root_cand_offset", "root_cand_rank")]
+        d = SolverDesc()
+        d.num_roots = self.F
+        d.num_non_tree = self.N
+        d.num_candidates = self.K
+        d.num_positions = self.P
+        (d.tree_offset, d.parent_pos, d.pos_coord, d.cand_root, d.cand_pos_u, d.cand_pos_v, d.cand_coord,
+         d.root_cand_offset, d.root_cand_rank) = [a.ctypes.data for a in self._keep]
+        o = SolverOptions()
+        _lib.hip().mcb_solver_options_init(ctypes.byref(o))
+        o.store_cycles = int(bool(store_cycles))
+        o.timing = int(bool(timing))
+        o.waves_per_block = int(waves_per_block)
+        o.max_blocks = int(max_blocks)
+        h = ctypes.c_void_p()
+        _lib.check_hip(_lib.hip().mcb_solver_create(ctypes.byref(d), ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+        self.store_cycles = bool(store_cycles)
+
+    def reset(self) -> None:
+        _lib.check_hip(_lib.hip().mcb_solver_reset(self._h))
+
+    def run(self, phases: int | None = None) -> None:
+        """Enqueue the next ``phases`` phases (all remaining when None)."""
+        n = self.N if phases is None else int(phases)
+        _lib.check_hip(_lib.hip().mcb_solver_run(self._h, n))
+
+    def sync(self) -> None:
+        _lib.check_hip(_lib.hip().mcb_solver_sync(self._h))
+
+    @property
+    def phase(self) -> int:
+        return int(_lib.hip().mcb_solver_phase(self._h))
+
+    def selected(self, count: int | None = None) -> np.ndarray:
+        count = self.phase if count is None else int(count)
+        out = np.zeros(count, dtype=np.int32)
+        _lib.check_hip(_lib.hip().mcb_solver_selected(self._h, out.ctypes.data, count))
+        return out
+
+    def support_vectors(self) -> np.ndarray:
+        out = np.zeros((self.N, self.W), dtype=np.uint64)
+        _lib.check_hip(_lib.hip().mcb_solver_support_vectors(self._h, out.ctypes.data))
+        return out
+
+    def cycle_vectors(self, count: int | None = None) -> np.ndarray:
+        count = self.phase if count is None else int(count)
+        out = np.zeros((count, self.W), dtype=np.uint64)
+        _lib.check_hip(_lib.hip().mcb_solver_cycle_vectors(self._h, out.ctypes.data, count))
+        return out
+
+    def parity(self, S: np.ndarray) -> np.ndarray:
+        """Kernel (a) alone: parity of every position of every root layout for S."""
+        S = np.ascontiguousarray(S, dtype=np.uint64)
+        if S.size != self.W:
+            raise ValueError("S must have W words")
+        out = np.zeros(self.P, dtype=np.uint8)
+        _lib.check_hip(_lib.hip().mcb_solver_parity(self._h, S.ctypes.data, out.ctypes.data))
+        return out
+
+    def kernel_time(self, kernel: str) -> tuple[float, int]:
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        _lib.check_hip(_lib.hip().mcb_solver_kernel_time(self._h, KERNELS.index(kernel), ctypes.byref(ms),
+                                                         ctypes.byref(n)))
+        return ms.value, n.value
+
+    @property
+    def device_bytes(self) -> int:
+        return int(_lib.hip().mcb_solver_device_bytes(self._h))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.hip().mcb_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuStruct:
+    """gpu_struct (gpu/gpu_struct.cuh:15-84) over the compressed_trees chunk layout.
+
+    ``tree_rows[i]``, ``tree_cols[i]``, ``edge_offset[i]`` and ``precompute_value[i]`` are the
+    arrays of chunk row i (roots i*chunk .. i*chunk+chunk-1), each ``chunk * n`` entries
+    (``chunk * (n + 1)`` for tree_rows), as include/compressed_trees.h:16-26 allocates them.
+    """
+
+    def __init__(self, num_edges: int, num_non_tree_edges: int, size_vector: int, original_nodes: int,
+                 fvs_size: int, chunk_size: int, nstreams: int):
+        h = ctypes.c_void_p()
+        _lib.check_hip(_lib.hip().mcb_gpu_create(num_edges, num_non_tree_edges, size_vector, original_nodes,
+                                                 fvs_size, chunk_size, nstreams, ctypes.byref(h)))
+        self._h = h
+        self.num_chunks = (fvs_size + chunk_size - 1) // chunk_size
+
+    @staticmethod
+    def _ptrs(arrays):
+        arr = (ctypes.c_void_p * len(arrays))(*[a.ctypes.data for a in arrays])
+        return arr
+
+    def initialize_memory(self, non_tree_edges: np.ndarray, tree_rows, tree_cols, edge_offset) -> None:
+        nt = np.ascontiguousarray(non_tree_edges, dtype=np.int32)
+        self._trees = (list(tree_rows), list(tree_cols), list(edge_offset))
+        _lib.check_hip(_lib.hip().mcb_gpu_initialize_memory(self._h, nt.ctypes.data, self._ptrs(tree_rows),
+                                                            self._ptrs(tree_cols), self._ptrs(edge_offset),
+                                                            len(tree_cols)))
+
+    def copy_support_vector(self, S: np.ndarray) -> float:
+        S = np.ascontiguousarray(S, dtype=np.uint64)
+        ms = ctypes.c_float()
+        _lib.check_hip(_lib.hip().mcb_gpu_copy_support_vector(self._h, S.ctypes.data, ctypes.byref(ms)))
+        return ms.value
+
+    def process_shortest_path(self, precompute_value, multiple_transfer: bool = False) -> float:
+        rows, cols, eo = self._trees
+        ms = ctypes.c_float()
+        _lib.check_hip(_lib.hip().mcb_gpu_process_shortest_path(
+            self._h, self._ptrs(rows), self._ptrs(cols), self._ptrs(eo), self._ptrs(precompute_value),
+            len(precompute_value), int(bool(multiple_transfer)), ctypes.byref(ms)))
+        return ms.value
+
+    def fetch(self, precompute_value) -> float:
+        ms = ctypes.c_float()
+        _lib.check_hip(_lib.hip().mcb_gpu_fetch(self._h, self._ptrs(precompute_value), len(precompute_value),
+                                                ctypes.byref(ms)))
+        return ms.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.hip().mcb_gpu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class MCBResult:
+    total_weight: int            # exact (int64)
+    total_weight_int32: int      # as the reference's int accumulator reports it
+    num_cycles: int
+    selected: np.ndarray         # candidate rank chosen in each phase
+    cycles: tuple                # (root vertex, u, v, weight) arrays in phase order, reduced-graph ids
+
+
+def minimum_cycle_basis(source, reduce_degree_two: bool = True, prune_trees: bool = True, num_threads: int = 0,
+                        fvs_order=None) -> MCBResult:
+    """Minimum cycle basis weight and cycles of a graph given as a .mtx path or (n, u, v, w)."""
+    if isinstance(source, Problem):
+        prob = source
+    elif isinstance(source, (str, bytes)) or hasattr(source, "__fspath__"):
+        prob = Problem.from_mtx(source, reduce_degree_two=reduce_degree_two, prune_trees=prune_trees,
+                                num_threads=num_threads, fvs_order=fvs_order)
+    else:
+        n, u, v, w = source
+        prob = Problem.from_edges(n, u, v, w, reduce_degree_two=reduce_degree_two, prune_trees=prune_trees,
+                                  num_threads=num_threads, fvs_order=fvs_order)
+    info = prob.info
+    if info.single_cycle:
+        tw = int(info.input_total_weight)
+        return MCBResult(tw, int(np.int32(np.int64(tw) & 0xFFFFFFFF).astype(np.int32)) if tw else 0, 1,
+                         np.zeros(1, dtype=np.int32), ())
+    s = Solver(prob)
+    s.run()
+    s.sync()
+    sel = s.selected()
+    s.close()
+    r, a, b, w = prob.candidate_table()
+    ww = w[sel].astype(np.int64)
+    total = int(ww.sum())
+    return MCBResult(total, int(np.int64(total).astype(np.int32)), int(sel.size), sel,
+                     (r[sel], a[sel], b[sel], w[sel]))

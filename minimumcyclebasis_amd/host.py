@@ -1,0 +1,197 @@
+"""Python view of a preprocessed MCB problem (``libmcb_host.so``).
+
+``Problem`` owns one native ``mcb_problem``: the reduced multigraph, the FVS roots, the
+cycle-space coordinates, the per-root shortest-path tree layouts and the ranked candidate
+cycles.  It mirrors the host half of the reference driver
+(mcb/mcb_cycle_deg_two_remove.cpp:68-267, mcb/mcb_cycle.cpp:76-203).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+# Array ids (include/mcb_host.h) with their numpy dtypes.
+ARRAYS = {
+    "rows": (0, np.uint32),
+    "cols": (1, np.uint32),
+    "weights": (2, np.int32),
+    "reverse": (3, np.uint32),
+    "row_offsets": (4, np.uint32),
+    "fvs_rank": (5, np.int32),
+    "roots": (6, np.uint32),
+    "nt_edges": (7, np.uint32),
+    "nt_map": (8, np.int32),
+    "cand_root": (9, np.uint32),
+    "cand_edge": (10, np.uint32),
+    "cand_weight": (11, np.int32),
+    "cand_pos_u": (12, np.uint32),
+    "cand_pos_v": (13, np.uint32),
+    "cand_coord": (14, np.int32),
+    "tree_offset": (15, np.int64),
+    "parent_pos": (16, np.uint32),
+    "pos_coord": (17, np.int32),
+    "root_cand_offset": (18, np.int64),
+    "root_cand_rank": (19, np.uint32),
+    "vertex_parent": (20, np.int32),
+    "pos_vertex": (21, np.uint32),
+}
+
+
+@dataclass
+class Info:
+    input_nodes: int
+    input_edges: int
+    single_cycle: bool
+    nodes_removed: int
+    source_vertex: int
+    nodes: int
+    directed_edges: int
+    num_non_tree: int
+    words: int
+    fvs_size: int
+    num_candidates: int
+    positions: int
+    max_positions: int
+    max_depth: int
+    input_total_weight: int
+
+
+def _options(reduce_degree_two: bool, prune_trees: bool, keep_vertex_trees: bool, num_threads: int,
+             fvs_order=None):
+    opt = _lib.BuildOptions()
+    _lib.host().mcb_build_options_init(ctypes.byref(opt))
+    opt.reduce_degree_two = int(bool(reduce_degree_two))
+    opt.prune_trees = int(bool(prune_trees))
+    opt.keep_vertex_trees = int(bool(keep_vertex_trees))
+    opt.num_threads = int(num_threads)
+    keep = None
+    if fvs_order is not None:
+        keep = np.ascontiguousarray(fvs_order, dtype=np.int32)
+        opt.fvs_order = keep.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        opt.fvs_order_len = int(keep.size)
+    return opt, keep
+
+
+class Problem:
+    """A preprocessed graph ready for the phase loop.
+
+    Build from a Matrix-Market file (``Problem.from_mtx``) or an edge list
+    (``Problem.from_edges``, 0-based endpoints, each undirected edge once).
+    """
+
+    def __init__(self, handle: ctypes.c_void_p):
+        self._h = handle
+        info = _lib.ProblemInfo()
+        _lib.check_host(_lib.host().mcb_problem_get_info(self._h, ctypes.byref(info)))
+        self.info = Info(**{f: getattr(info, f) for f, _ in _lib.ProblemInfo._fields_})
+        self.info.single_cycle = bool(self.info.single_cycle)
+        self._cache: dict = {}
+
+    @classmethod
+    def from_mtx(cls, path: str, reduce_degree_two: bool = True, prune_trees: bool = False,
+                 keep_vertex_trees: bool = False, num_threads: int = 0, fvs_order=None) -> "Problem":
+        opt, _keep = _options(reduce_degree_two, prune_trees, keep_vertex_trees, num_threads, fvs_order)
+        h = ctypes.c_void_p()
+        _lib.check_host(_lib.host().mcb_problem_build_mtx(str(path).encode(), ctypes.byref(opt), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_edges(cls, n: int, u, v, w, reduce_degree_two: bool = True, prune_trees: bool = False,
+                   keep_vertex_trees: bool = False, num_threads: int = 0, fvs_order=None) -> "Problem":
+        u = np.ascontiguousarray(u, dtype=np.int32)
+        v = np.ascontiguousarray(v, dtype=np.int32)
+        w = np.ascontiguousarray(w, dtype=np.int32)
+        if not (u.shape == v.shape == w.shape):
+            raise ValueError("u, v, w must have the same length")
+        opt, _keep = _options(reduce_degree_two, prune_trees, keep_vertex_trees, num_threads, fvs_order)
+        h = ctypes.c_void_p()
+        _lib.check_host(_lib.host().mcb_problem_build_edges(
+            int(n), int(u.size), u.ctypes.data, v.ctypes.data, w.ctypes.data, ctypes.byref(opt), ctypes.byref(h)))
+        return cls(h)
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    def array(self, name: str) -> np.ndarray:
+        """A read-only numpy view of an internal array (valid while this object lives)."""
+        if name in self._cache:
+            return self._cache[name]
+        which, dtype = ARRAYS[name]
+        ptr = ctypes.c_void_p()
+        cnt = ctypes.c_int64()
+        _lib.check_host(_lib.host().mcb_problem_array(self._h, which, ctypes.byref(ptr), ctypes.byref(cnt)))
+        n = int(cnt.value)
+        if n == 0 or not ptr.value:
+            arr = np.zeros(0, dtype=dtype)
+        else:
+            buf = (ctypes.c_char * (n * np.dtype(dtype).itemsize)).from_address(ptr.value)
+            arr = np.frombuffer(buf, dtype=dtype, count=n)
+        arr.flags.writeable = False
+        self._cache[name] = arr
+        return arr
+
+    def reference_layout(self, r: int):
+        """(rows, cols, edge_offset, nodes_index) of root index ``r`` in the reference's own
+        BFS-level layout (include/Dijkstra.h:151-203)."""
+        n = self.info.nodes
+        rows = np.zeros(n + 1, dtype=np.uint32)
+        cols = np.zeros(n, dtype=np.uint32)
+        eo = np.zeros(n, dtype=np.int32)
+        ni = np.zeros(n, dtype=np.uint32)
+        _lib.check_host(_lib.host().mcb_problem_reference_layout(
+            self._h, int(r), rows.ctypes.data, cols.ctypes.data, eo.ctypes.data, ni.ctypes.data))
+        return rows, cols, eo, ni
+
+    def candidate_table(self):
+        """Candidates in rank order: (root vertex, u, v, weight) with 0-based reduced-graph ids."""
+        roots = self.array("roots")
+        e = self.array("cand_edge")
+        return (roots[self.array("cand_root")], self.array("rows")[e], self.array("cols")[e],
+                self.array("cand_weight"))
+
+    def close(self) -> None:
+        if self._h:
+            self._cache.clear()
+            _lib.host().mcb_problem_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fvs_reverse_delete(row_offsets, cols, order=None) -> np.ndarray:
+    """Reverse-delete FVS of a CSR multigraph; returns the root rank per vertex (-1 = not in F)."""
+    ro = np.ascontiguousarray(row_offsets, dtype=np.uint32)
+    c = np.ascontiguousarray(cols, dtype=np.uint32)
+    n = ro.size - 1
+    out = np.zeros(n, dtype=np.int32)
+    o = None if order is None else np.ascontiguousarray(order, dtype=np.int32)
+    _lib.check_host(_lib.host().mcb_fvs_reverse_delete(
+        n, int(c.size), ro.ctypes.data, c.ctypes.data, None if o is None else o.ctypes.data, out.ctypes.data))
+    return out
+
+
+def induced_has_cycle(row_offsets, cols, in_set) -> bool:
+    ro = np.ascontiguousarray(row_offsets, dtype=np.uint32)
+    c = np.ascontiguousarray(cols, dtype=np.uint32)
+    s = np.ascontiguousarray(in_set, dtype=np.uint8)
+    return bool(_lib.check_host(_lib.host().mcb_induced_has_cycle(ro.size - 1, ro.ctypes.data, c.ctypes.data,
+                                                                  s.ctypes.data)))
+
+
+def biconnected_components(n: int, u, v) -> np.ndarray:
+    """Component id per edge (simple graph, 0-based endpoints)."""
+    u = np.ascontiguousarray(u, dtype=np.int32)
+    v = np.ascontiguousarray(v, dtype=np.int32)
+    out = np.zeros(u.size, dtype=np.int32)
+    _lib.check_host(_lib.host().mcb_biconnected_components(int(n), int(u.size), u.ctypes.data, v.ctypes.data,
+                                                           out.ctypes.data))
+    return out

@@ -1,0 +1,272 @@
+"""Writes the golden fixtures under tests/golden/ by running the reference's own CPU executables.
+
+Run in a container that has the reference tree (the executables are built from its sources by
+``make -C oracle ref`` into oracle/_ref/).  The fixtures hold only inputs (generator specs and,
+for tiny graphs, the edge lists) and the reference's outputs:
+
+* ``trace_*.json.gz``  - mcb/Test_deg_two_remove.cpp (degree-two contraction) and mcb/Test.cpp
+  (no contraction, spanning tree at vertex 0) on small graphs: reduced graph size, coordinate
+  edges (Test.cpp:132-140), the ranked candidate list "List Cycles Pre Isometric"
+  (Test.cpp:189-221), and per phase the support vector S_e, the chosen cycle, its cycle vector
+  C_e and every product <C_e, S_j> (Test.cpp:239-300), plus "Total Weight".  Bit vectors are
+  printed MSB-first per 64-bit word, words ascending (include/bit_vector.h:97-116,143-149).
+* ``csv_*.json``       - the stats CSV lines of mcb_cpu (mcb/mcb_non_cpu_rd.cpp) and
+  mcb_cpu_baseline (mcb/mcb_cycle.cpp) on larger graphs: Fvs_size, Initial_Cycles,
+  Final_cycles, Total_Weight (include/stats.h:256-338).
+
+    python tests/golden/make_golden.py [--only NAME] [--skip-large]
+"""
+from __future__ import annotations
+
+import argparse
+import gzip
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from minimumcyclebasis_amd import generators as G  # noqa: E402
+
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+# name -> (generator, kwargs); all seeded.
+TINY = {
+    "k4": ("explicit", {"n": 4, "edges": [[1, 0, 1], [2, 0, 2], [3, 0, 3], [2, 1, 4], [3, 1, 5], [3, 2, 6]]}),
+    "petersen": ("explicit", {"n": 10, "edges": [[1, 0, 3], [2, 1, 1], [3, 2, 4], [4, 3, 1], [4, 0, 5],
+                                                 [5, 0, 9], [6, 1, 2], [7, 2, 6], [8, 3, 5], [9, 4, 3],
+                                                 [7, 5, 5], [8, 6, 8], [9, 7, 9], [5, 8, 7], [6, 9, 9]]}),
+    "theta3": ("explicit", {"n": 8, "edges": [[2, 0, 1], [2, 1, 2], [3, 0, 3], [3, 1, 1], [4, 0, 2], [4, 1, 2],
+                                              [5, 0, 4], [5, 6, 1], [6, 7, 1], [7, 1, 1]]}),
+    "grid3x3": ("grid", {"rows": 3, "cols": 3, "wmax": 100, "seed": 1}),
+    "grid4x4": ("grid", {"rows": 4, "cols": 4, "wmax": 100, "seed": 1}),
+    "grid5x5_ties": ("grid", {"rows": 5, "cols": 5, "wmax": 2, "seed": 3}),
+    "grid6x5": ("grid", {"rows": 6, "cols": 5, "wmax": 100, "seed": 7}),
+    "k5_ties": ("complete", {"n": 5, "wmax": 3, "seed": 2}),
+    "k6_ties": ("complete", {"n": 6, "wmax": 2, "seed": 5}),
+    "er12": ("erdos_renyi_lbc", {"n": 12, "m": 24, "wmax": 5, "seed": 11}),
+    "delaunay20": ("delaunay", {"n": 20, "seed": 4}),
+    "delaunay40": ("delaunay", {"n": 40, "seed": 5}),
+    "biconn20": ("random_biconnected", {"n": 20, "avg_degree": 2.6, "wmax": 3, "seed": 9}),
+    "biconn30": ("random_biconnected", {"n": 30, "avg_degree": 3.0, "wmax": 50, "seed": 10}),
+    "chunglu60": ("chung_lu_lbc", {"n": 60, "gamma": 2.3, "avg_degree": 5.0, "wmax": 20, "seed": 12}),
+}
+
+MEDIUM = {
+    "grid12x12": ("grid", {"rows": 12, "cols": 12, "wmax": 100, "seed": 2}),
+    "grid16x16": ("grid", {"rows": 16, "cols": 16, "wmax": 100, "seed": 1}),
+    "grid14x14_ties": ("grid", {"rows": 14, "cols": 14, "wmax": 3, "seed": 4}),
+    "er150": ("erdos_renyi_lbc", {"n": 150, "m": 450, "wmax": 1000, "seed": 1}),
+    "er120_ties": ("erdos_renyi_lbc", {"n": 120, "m": 360, "wmax": 4, "seed": 2}),
+    "delaunay150": ("delaunay", {"n": 150, "seed": 1}),
+    "biconn200": ("random_biconnected", {"n": 200, "avg_degree": 3.4, "wmax": 1000, "seed": 3}),
+    "chunglu300": ("chung_lu_lbc", {"n": 300, "gamma": 2.3, "avg_degree": 6.0, "wmax": 10000, "seed": 1}),
+}
+
+LARGE = {
+    "grid64x64": ("grid", {"rows": 64, "cols": 64, "wmax": 100, "seed": 1}),
+    "grid32x32": ("grid", {"rows": 32, "cols": 32, "wmax": 100, "seed": 1}),
+    "delaunay2000": ("delaunay", {"n": 2000, "seed": 1}),
+    "er2000": ("erdos_renyi_lbc", {"n": 2000, "m": 6000, "wmax": 1000, "seed": 1}),
+    "chunglu2000": ("chung_lu_lbc", {"n": 2000, "gamma": 2.3, "avg_degree": 8.0, "wmax": 10000, "seed": 1}),
+}
+
+
+def make_graph(gen: str, kw: dict):
+    if gen == "explicit":
+        e = np.asarray(kw["edges"], dtype=np.int64)
+        return G._finish(kw["n"], e[:, 0], e[:, 1], e[:, 2])
+    return getattr(G, gen)(**kw)
+
+
+def edge_digest(n, u, v, w) -> str:
+    import hashlib
+
+    h = hashlib.sha256()
+    h.update(np.asarray([n], dtype=np.int64).tobytes())
+    for a in (u, v, w):
+        h.update(np.ascontiguousarray(a, dtype=np.int64).tobytes())
+    return h.hexdigest()
+
+
+def _bits_to_words(tokens):
+    return [int(t, 2) for t in tokens]
+
+
+def _is_bits(tok: str) -> bool:
+    return len(tok) == 64 and set(tok) <= {"0", "1"}
+
+
+def parse_trace(stream):
+    """Parses the stdout of Test / Test_deg_two_remove (run with one thread)."""
+    out = {"nt_map": [], "candidates": [], "phases": [], "reduced_edges": [], "non_tree_edges": []}
+    section = None
+    phase = None
+    for raw in stream:
+        line = raw.rstrip("\n")
+        if line.startswith("====="):
+            if section in ("graph", "tree", "nt"):
+                section = None
+            continue
+        if line.startswith("Number of nodes = "):
+            a, b = line.replace("Number of nodes = ", "").split(",edges = ")
+            out["nodes"], out["edges"] = int(a), int(b)
+            section = "graph"
+            continue
+        if line.startswith("Printing Spanning Tree Edges"):
+            section = "tree"
+            continue
+        if line.startswith("Printing Non-Tree Edges,count = "):
+            out["num_non_tree"] = int(line.split("= ")[1])
+            section = "nt"
+            continue
+        if line.startswith("Number of initial cycles = "):
+            out["num_candidates"] = int(line.split("= ")[1])
+            section = None
+            continue
+        if line.startswith("List Cycles Pre Isometric"):
+            section = "cands"
+            continue
+        if line.startswith("List Cycles Post Isometric"):
+            section = "post"
+            continue
+        if line.startswith("Number of non_tree_edges = "):
+            out["N"] = int(line.split("= ")[1])
+            section = "phases"
+            continue
+        if line.startswith("Total Weight = "):
+            out["total_weight"] = int(line.split("= ")[1])
+            section = "done"
+            continue
+        if section == "graph":
+            a, rest = line.split(" ", 1)
+            b, wt = rest.split(" - ")
+            out["reduced_edges"].append([int(a), int(b), int(wt)])
+        elif section == "nt":
+            a, b = line.split()
+            out["non_tree_edges"].append([int(a), int(b)])
+        elif section is None and " : " in line and " - " in line:
+            i, rest = line.split(" : ")
+            a, b = rest.split(" - ")
+            out["nt_map"].append([int(i), int(a), int(b)])
+        elif section == "cands" and "-(" in line:
+            root, rest = line.split("-(", 1)
+            edge, wt = rest.split(") : ")
+            a, b = edge.split(" - ")
+            out["candidates"].append([int(root), int(a), int(b), int(wt)])
+        elif section == "phases":
+            if phase is not None and phase["state"] == "chosen":
+                if line.startswith("Root is "):
+                    phase["chosen"] = [int(line.split()[-1])]
+                elif line.startswith("Edge is "):
+                    a, b = line.replace("Edge is ", "").split(" - ")
+                    phase["chosen"] += [int(a), int(b)]
+                elif line.startswith("Total weight = "):
+                    phase["chosen"].append(int(line.split("= ")[1]))
+                elif line.startswith("Ci "):
+                    phase["C"] = _bits_to_words(line[3:].split())
+                    phase["state"] = "products"
+                    if phase["e"] == out["N"] - 1:
+                        section = "final_list"
+                continue
+            toks = line.split()
+            if not toks:
+                continue
+            if phase is not None and phase["state"] == "products" and len(phase["products"]) < out["N"] - 1 - phase["e"]:
+                if not (toks[0] in ("0", "1") and all(_is_bits(t) for t in toks[1:])):
+                    raise ValueError(f"unexpected product line: {line!r}")
+                phase["products"].append(int(toks[0]))
+                continue
+            if not all(_is_bits(t) for t in toks):
+                raise ValueError(f"unexpected support-vector line: {line!r}")
+            phase = {"e": len(out["phases"]), "S": _bits_to_words(toks), "products": [], "state": "chosen"}
+            out["phases"].append(phase)
+    for ph in out["phases"]:
+        ph.pop("state", None)
+    return out
+
+
+def run_trace(exe: str, mtx: str, keep_vectors: bool):
+    with tempfile.TemporaryDirectory() as td:
+        proc = subprocess.Popen([exe, mtx, os.path.join(td, "out"), "1"], stdout=subprocess.PIPE,
+                                stderr=subprocess.DEVNULL, text=True, cwd=td)
+        res = parse_trace(proc.stdout)
+        rc = proc.wait()
+    if rc != 0:
+        raise RuntimeError(f"{exe} failed with {rc}")
+    if not keep_vectors:
+        for ph in res["phases"]:
+            ph.pop("S", None)
+            ph.pop("C", None)
+            ph.pop("products", None)
+    return res
+
+
+def run_csv(exe: str, mtx: str, threads: int = 8):
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "stats.csv")
+        env = dict(os.environ)
+        subprocess.run(["bash", "-c", f"ulimit -s unlimited && exec {exe} {mtx} {out} {threads}"], check=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd=td, env=env)
+        with open(out) as f:
+            f.readline()
+            vals = [x.strip() for x in f.readline().split(",")]
+    # Total_Nodes, Total_Edges, New_Edges, Node_removed, Fvs_size, Initial_Cycles, Final_cycles, Total_Weight, ...
+    return {"total_nodes": int(vals[0]), "total_edges": int(vals[1]), "new_edges": int(vals[2]),
+            "nodes_removed": int(vals[3]), "fvs_size": int(vals[4]), "initial_cycles": int(vals[5]),
+            "final_cycles": int(vals[6]), "total_weight": int(vals[7]),
+            "precompute_s": float(vals[11]), "total_s": float(vals[15])}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--only", default=None)
+    ap.add_argument("--skip-large", action="store_true")
+    args = ap.parse_args()
+    subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True, stdout=subprocess.DEVNULL)
+    test_rd = os.path.join(REF_DIR, "Test_deg_two_remove")
+    test_plain = os.path.join(REF_DIR, "Test")
+    with tempfile.TemporaryDirectory() as td:
+        for tier, table, keep in (("tiny", TINY, True), ("medium", MEDIUM, False)):
+            for name, (gen, kw) in table.items():
+                if args.only and name != args.only:
+                    continue
+                n, u, v, w = make_graph(gen, kw)
+                mtx = os.path.join(td, name + ".mtx")
+                G.write_mtx(mtx, n, u, v, w)
+                fx = {"name": name, "tier": tier, "generator": gen, "kwargs": kw,
+                      "edge_digest": edge_digest(n, u, v, w)}
+                if tier == "tiny":
+                    fx["graph"] = {"n": n, "u": u.tolist(), "v": v.tolist(), "w": w.tolist()}
+                fx["reduced"] = run_trace(test_rd, mtx, keep)
+                fx["plain"] = run_trace(test_plain, mtx, keep)
+                path = os.path.join(HERE, f"trace_{name}.json.gz")
+                with gzip.open(path, "wt") as f:
+                    json.dump(fx, f, separators=(",", ":"))
+                print(f"{name}: N={fx['reduced']['N']} K={fx['reduced']['num_candidates']} "
+                      f"W={fx['reduced']['total_weight']} plain W={fx['plain']['total_weight']}", flush=True)
+        if args.skip_large:
+            return
+        for name, (gen, kw) in LARGE.items():
+            if args.only and name != args.only:
+                continue
+            n, u, v, w = make_graph(gen, kw)
+            mtx = os.path.join(td, name + ".mtx")
+            G.write_mtx(mtx, n, u, v, w)
+            fx = {"name": name, "tier": "large", "generator": gen, "kwargs": kw,
+                  "edge_digest": edge_digest(n, u, v, w), "threads": 8,
+                  "mcb_cpu": run_csv(os.path.join(REF_DIR, "mcb_cpu"), mtx),
+                  "mcb_cpu_baseline": run_csv(os.path.join(REF_DIR, "mcb_cpu_baseline"), mtx)}
+            with open(os.path.join(HERE, f"csv_{name}.json"), "w") as f:
+                json.dump(fx, f, indent=1)
+            print(f"{name}: {fx['mcb_cpu']} / {fx['mcb_cpu_baseline']}", flush=True)
+
+
+if __name__ == "__main__":
+    main()
