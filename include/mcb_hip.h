@@ -163,6 +163,14 @@ int mcb_solver_kernel_time(mcb_solver *s, int kernel, double *total_ms, int64_t 
 /* Device bytes the solver holds. */
 int64_t mcb_solver_device_bytes(const mcb_solver *s);
 
+/* Work done by the parity/search kernel since the last reset: tree positions streamed and
+ * candidates read (roots whose best candidate cannot beat the phase's best are skipped). */
+int mcb_solver_work_counters(mcb_solver *s, int64_t *positions, int64_t *candidates);
+
+/* Launch shape chosen for the parity/search kernel. */
+int mcb_solver_launch_shape(const mcb_solver *s, int32_t *blocks, int32_t *waves_per_block, int32_t *lds_bytes,
+                            int32_t *s_in_lds);
+
 void mcb_solver_destroy(mcb_solver *s);
 
 /* ---------------------------------------------------------------------------------------------

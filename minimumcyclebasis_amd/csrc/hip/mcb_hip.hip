@@ -143,6 +143,7 @@ struct SearchArgs {
     const uint64_t *S;              // contiguous S_e
     uint32_t *best;                 // &best[e]
     uint8_t *parity_out;            // parity-only mode: one byte per position
+    uint64_t *wave_stats;           // optional: per wave, positions streamed and candidates read
     int32_t F, W, p_words, total_waves;
 };
 
@@ -160,7 +161,9 @@ __global__ __launch_bounds__(256) void k_parity_search(SearchArgs a) {
         Sx = lds;
     }
     uint64_t *P = lds + (S_IN_LDS ? a.W : 0) + static_cast<size_t>(wib) * a.p_words;
-    for (int idx = blockIdx.x * wpb + wib; idx < a.F; idx += a.total_waves) {
+    uint64_t n_pos = 0, n_cand = 0;
+    const int gw = blockIdx.x * wpb + wib;
+    for (int idx = gw; idx < a.F; idx += a.total_waves) {
         const uint32_t r = SEARCH ? a.root_order[idx] : static_cast<uint32_t>(idx);
         uint32_t bestk = kNoRank;
         if (SEARCH) {
@@ -169,6 +172,7 @@ __global__ __launch_bounds__(256) void k_parity_search(SearchArgs a) {
         }
         const int64_t off = a.tree_off[r];
         const uint32_t npos = static_cast<uint32_t>(a.tree_off[r + 1] - off);
+        n_pos += npos;
         if (SEARCH) {
             tree_parity(a.parent_pos + off, a.pos_coord + off, npos, Sx, P, lane, [](uint32_t, uint32_t) {});
         } else {
@@ -183,6 +187,7 @@ __global__ __launch_bounds__(256) void k_parity_search(SearchArgs a) {
         for (int64_t k0 = cb; k0 < ce; k0 += kWave) {
             const int64_t k = k0 + lane;
             const bool v = k < ce;
+            n_cand += static_cast<uint64_t>(ce - k0 < kWave ? ce - k0 : kWave);
             const uint32_t rk = v ? a.rc_rank[k] : kNoRank;
             const bool live = rk < bestk;
             uint32_t odd = 0;
@@ -199,6 +204,10 @@ __global__ __launch_bounds__(256) void k_parity_search(SearchArgs a) {
                 bestk = __hip_atomic_load(a.best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         __builtin_amdgcn_wave_barrier();
+    }
+    if (a.wave_stats && lane == 0) {  // each wave owns its slot; launches are stream-ordered
+        a.wave_stats[2 * gw] += n_pos;
+        a.wave_stats[2 * gw + 1] += n_cand;
     }
 }
 
@@ -431,13 +440,18 @@ namespace {
 
 int compat_roots_in_chunk(const mcb_gpu *g, int i) { return std::min(g->chunk, g->fvs - i * g->chunk); }
 
-int compat_check_layout(const mcb_gpu *g, const uint32_t *cols, int roots) {
+// The kernels index LDS by parent position and the coordinate map by edge offset, so both are
+// range-checked on the host before a chunk is used.
+int compat_check_layout(const mcb_gpu *g, const uint32_t *cols, const int32_t *eo, int roots) {
     for (int s = 0; s < roots; ++s) {
         const uint32_t *c = cols + static_cast<size_t>(s) * g->nodes;
+        const int32_t *e = eo + static_cast<size_t>(s) * g->nodes;
         if (c[0] != kNoParent) return fail(MCB_E_INVALID, "tree_cols: position 0 must be the root (parent -1)");
         for (int i = 1; i < g->nodes; ++i)
             if (c[i] >= static_cast<uint32_t>(i))
                 return fail(MCB_E_INVALID, "tree_cols: a parent position must precede its child");
+        for (int i = 0; i < g->nodes; ++i)
+            if (e[i] < -1 || e[i] >= g->num_edges) return fail(MCB_E_INVALID, "edge_offset out of range");
     }
     return MCB_OK;
 }
@@ -561,10 +575,14 @@ int mcb_gpu_initialize_memory(mcb_gpu *g, const int32_t *nt, const uint32_t *con
     (void)tree_rows;
     if (!g || !nt || !tree_cols || !edge_offset || num_rows != g->num_chunks)
         return fail(MCB_E_INVALID, "invalid arguments");
+    if (static_cast<int64_t>(g->num_non_tree) > 64 * static_cast<int64_t>(g->size_vector))
+        return fail(MCB_E_INVALID, "size_vector too small for num_non_tree_edges");
+    for (int i = 0; i < g->num_edges; ++i)
+        if (nt[i] < -1 || nt[i] >= g->num_non_tree) return fail(MCB_E_INVALID, "non-tree map entry out of range");
     HIP_TRY(hipMemcpy(g->d_nt, nt, sizeof(int32_t) * g->num_edges, hipMemcpyHostToDevice));
     for (int i = 0; i < g->num_chunks; ++i) {
         const int roots = compat_roots_in_chunk(g, i);
-        int rc = compat_check_layout(g, tree_cols[i], roots);
+        int rc = compat_check_layout(g, tree_cols[i], edge_offset[i], roots);
         if (rc) return rc;
         if (i < g->nslots && g->all_resident) {
             const size_t bytes = sizeof(int32_t) * static_cast<size_t>(roots) * g->nodes;
@@ -602,6 +620,8 @@ int mcb_gpu_process_shortest_path(mcb_gpu *g, const uint32_t *const *tree_rows, 
         const int roots = compat_roots_in_chunk(g, i);
         const size_t bytes = sizeof(uint32_t) * static_cast<size_t>(roots) * g->nodes;
         if (multiple_transfer) {
+            int rc = compat_check_layout(g, tree_cols[i], edge_offset[i], roots);
+            if (rc) return rc;
             HIP_TRY(hipMemcpyAsync(g->d_eo[slot], edge_offset[i], bytes, hipMemcpyHostToDevice, st));
             HIP_TRY(hipMemcpyAsync(g->d_cols[slot], tree_cols[i], bytes, hipMemcpyHostToDevice, st));
         }
@@ -680,6 +700,7 @@ struct mcb_solver {
     uint64_t *Sw = nullptr, *Scur = nullptr, *C = nullptr, *C_store = nullptr;
     uint32_t *D = nullptr, *Dcount = nullptr, *best = nullptr, *status = nullptr;
     uint8_t *flag = nullptr;
+    uint64_t *wave_stats = nullptr;  // 2 counters per wave of the parity/search kernel
     // launch shapes
     int wpb = 4, blocks = 1, lds_bytes = 0, parent_cap = 0, cycle_lds = 0;
     bool s_in_lds = true;
@@ -740,6 +761,7 @@ SearchArgs search_args(const mcb_solver *s, int32_t e) {
     a.S = s->Scur + static_cast<int64_t>(e & 1) * s->W;
     a.best = s->best + e;
     a.parity_out = nullptr;
+    a.wave_stats = s->wave_stats;
     a.F = s->F;
     a.W = s->W;
     a.p_words = s->p_words;
@@ -814,7 +836,7 @@ void mcb_solver_destroy(mcb_solver *s) {
     if (!s) return;
     void *ptrs[] = {s->tree_off, s->parent_pos, s->pos_coord, s->root_order, s->root_min_rank, s->rc_off, s->rc_pu,
                     s->rc_pv, s->rc_rank, s->rc_coord, s->cand_root, s->cand_pu, s->cand_pv, s->cand_coord, s->Sw,
-                    s->Scur, s->C, s->C_store, s->D, s->Dcount, s->best, s->status, s->flag};
+                    s->Scur, s->C, s->C_store, s->D, s->Dcount, s->best, s->status, s->flag, s->wave_stats};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
@@ -936,6 +958,7 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
     if (s->opt.max_blocks > 0) blocks = std::min<int64_t>(blocks, s->opt.max_blocks);
     blocks = std::min<int64_t>(blocks, (F + wpb - 1) / wpb);
     s->blocks = static_cast<int>(std::max<int64_t>(1, blocks));
+    if ((rc = dev_alloc(&s->wave_stats, 2 * static_cast<size_t>(s->blocks) * s->wpb, B))) return cleanup(rc);
     // Cycle kernel LDS: C_e plus the chosen root's parent array when it fits.
     const int cycle_static = 8 * 1024;  // path buffers and counters of k_cycle
     if (static_cast<int64_t>(s->W) * 8 > kLdsLimit - cycle_static - 4096)
@@ -966,6 +989,7 @@ int mcb_solver_reset(mcb_solver *s) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemsetAsync(s->status, 0, sizeof(uint32_t), s->stream));
     HIP_TRY(hipMemsetAsync(s->flag, 0, static_cast<size_t>(s->npad), s->stream));
+    HIP_TRY(hipMemsetAsync(s->wave_stats, 0, sizeof(uint64_t) * 2 * s->blocks * s->wpb, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->phase = 0;
     s->ev_used = 0;
@@ -1044,6 +1068,7 @@ int mcb_solver_parity(mcb_solver *s, const uint64_t *S, uint8_t *out) {
     a.S = dS;
     a.parity_out = dP;
     a.best = nullptr;
+    a.wave_stats = nullptr;
     if (s->s_in_lds)
         hipLaunchKernelGGL((k_parity_search<true, false>), dim3(s->blocks), dim3(s->wpb * kWave), s->lds_bytes,
                            s->stream, a);
@@ -1069,6 +1094,31 @@ int mcb_solver_kernel_time(mcb_solver *s, int kernel, double *total_ms, int64_t 
 }
 
 int64_t mcb_solver_device_bytes(const mcb_solver *s) { return s ? s->bytes : -1; }
+
+int mcb_solver_work_counters(mcb_solver *s, int64_t *positions, int64_t *candidates) {
+    if (!s) return fail(MCB_E_INVALID, "invalid arguments");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    std::vector<uint64_t> st(2 * static_cast<size_t>(s->blocks) * s->wpb);
+    HIP_TRY(hipMemcpy(st.data(), s->wave_stats, sizeof(uint64_t) * st.size(), hipMemcpyDeviceToHost));
+    uint64_t p = 0, c = 0;
+    for (size_t i = 0; i < st.size(); i += 2) {
+        p += st[i];
+        c += st[i + 1];
+    }
+    if (positions) *positions = static_cast<int64_t>(p);
+    if (candidates) *candidates = static_cast<int64_t>(c);
+    return MCB_OK;
+}
+
+int mcb_solver_launch_shape(const mcb_solver *s, int32_t *blocks, int32_t *waves_per_block, int32_t *lds_bytes,
+                            int32_t *s_in_lds) {
+    if (!s) return fail(MCB_E_INVALID, "invalid arguments");
+    if (blocks) *blocks = s->blocks;
+    if (waves_per_block) *waves_per_block = s->wpb;
+    if (lds_bytes) *lds_bytes = s->lds_bytes;
+    if (s_in_lds) *s_in_lds = s->s_in_lds ? 1 : 0;
+    return MCB_OK;
+}
 
 int mcb_hip_copy_bandwidth(int64_t bytes, int iters, double *gbps) {
     if (bytes <= 0 || iters <= 0 || !gbps) return fail(MCB_E_INVALID, "invalid arguments");

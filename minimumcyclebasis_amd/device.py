@@ -126,6 +126,18 @@ class Solver:
     def device_bytes(self) -> int:
         return int(_lib.hip().mcb_solver_device_bytes(self._h))
 
+    def work_counters(self) -> tuple[int, int]:
+        """(tree positions streamed, candidates read) by the parity/search kernel since reset."""
+        p = ctypes.c_int64()
+        c = ctypes.c_int64()
+        _lib.check_hip(_lib.hip().mcb_solver_work_counters(self._h, ctypes.byref(p), ctypes.byref(c)))
+        return p.value, c.value
+
+    def launch_shape(self) -> dict:
+        v = [ctypes.c_int32() for _ in range(4)]
+        _lib.check_hip(_lib.hip().mcb_solver_launch_shape(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("blocks", "waves_per_block", "lds_bytes", "s_in_lds"), (x.value for x in v)))
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             _lib.hip().mcb_solver_destroy(self._h)

@@ -1,0 +1,79 @@
+"""Full-size GPU runs: the 64x64 grid against the reference's own stats CSV (Total_Weight,
+Final_cycles) and the oracle's phase sequence; the Erdos-Renyi 20000/60000 LBC workload through
+size-independent properties and a phase-prefix comparison with the oracle."""
+import numpy as np
+import pytest
+
+from helpers import graph_of, load_csv
+from minimumcyclebasis_amd import generators as G
+from minimumcyclebasis_amd.device import Solver, minimum_cycle_basis
+from minimumcyclebasis_amd.host import Problem
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("prune", [False, True])
+def test_grid64_against_reference_csv(prune):
+    fx = load_csv("grid64x64")
+    n, u, v, w = graph_of(fx)
+    print(f"case grid64x64 prune={prune}", flush=True)
+    res = minimum_cycle_basis((n, u, v, w), prune_trees=prune)
+    assert res.total_weight == fx["mcb_cpu"]["total_weight"] == fx["mcb_cpu_baseline"]["total_weight"]
+    assert res.num_cycles == fx["mcb_cpu"]["final_cycles"]
+    resp = minimum_cycle_basis((n, u, v, w), reduce_degree_two=False, prune_trees=prune)
+    assert resp.total_weight == fx["mcb_cpu_baseline"]["total_weight"]
+
+
+def test_grid64_sequence_against_oracle():
+    fx = load_csv("grid64x64")
+    n, u, v, w = graph_of(fx)
+    p = Problem.from_edges(n, u, v, w, keep_vertex_trees=True, prune_trees=True)
+    ref = pyoracle.run(p, want_s=True)
+    s = Solver(p)
+    s.run()
+    s.sync()
+    assert np.array_equal(s.selected(), ref["selected"])
+    assert np.array_equal(s.support_vectors(), ref["S"])
+
+
+@pytest.mark.parametrize("name", ["delaunay2000", "er2000", "chunglu2000"])
+def test_medium_graphs_against_reference_csv(name):
+    fx = load_csv(name)
+    n, u, v, w = graph_of(fx)
+    print(f"case {name}", flush=True)
+    res = minimum_cycle_basis((n, u, v, w))
+    assert res.total_weight == fx["mcb_cpu"]["total_weight"]
+    assert res.num_cycles == fx["mcb_cpu"]["final_cycles"]
+
+
+def test_er20000_properties_and_prefix():
+    n, u, v, w = G.erdos_renyi_lbc(20000, 60000, 1000, seed=1)
+    print("case er20000 full", flush=True)
+    p_full = Problem.from_edges(n, u, v, w, prune_trees=False)
+    p_pr = Problem.from_edges(n, u, v, w, prune_trees=True)
+    N = p_full.info.num_non_tree
+    s1 = Solver(p_full)
+    s1.run()
+    s1.sync()
+    sel1 = s1.selected()
+    s1.close()
+    s2 = Solver(p_pr)
+    s2.run()
+    s2.sync()
+    sel2 = s2.selected()
+    S = s2.support_vectors()
+    s2.close()
+    assert sel1.size == N and np.unique(sel1).size == N
+    assert np.array_equal(sel1, sel2)
+    # cycle weights are non-decreasing in rank; every chosen rank is a valid candidate
+    assert sel1.max() < p_full.info.num_candidates
+    # S_e is a unit vector plus lower coordinates only (triangular support)
+    for e in (0, 1, N // 3, N - 1):
+        row = S[e]
+        bits = np.flatnonzero(np.unpackbits(row.view(np.uint8), bitorder="little"))
+        assert bits.max() == e
+    # the first phases against the oracle
+    p_o = Problem.from_edges(n, u, v, w, keep_vertex_trees=True)
+    ref = pyoracle.run(p_o, phases=12)
+    assert np.array_equal(sel1[:12], ref["selected"])

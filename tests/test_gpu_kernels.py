@@ -1,0 +1,130 @@
+"""GPU parity tests: every kernel through the C ABI against the CPU oracle, bit-exact.
+
+(a)  tree-path parity, native layout (Solver.parity) and the reference chunk layout (GpuStruct),
+     against oracle_parity (include/work_per_thread.h:154-210 restated);
+(a)+(b)+cycle+(c)  the whole phase loop: chosen rank per phase, every support vector S_e and every
+     cycle vector C_e against oracle_run (mcb/mcb_cycle.cpp:226-305 restated)."""
+import numpy as np
+import pytest
+
+from helpers import load_trace, problem_of, trace_names
+from minimumcyclebasis_amd.device import GpuStruct, Solver
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+MODES = ("reduced", "plain")
+PARITY_CASES = ["k4", "theta3", "grid5x5_ties", "delaunay40", "chunglu60", "grid16x16", "er150", "chunglu300",
+                "biconn200", "delaunay150"]
+
+
+def random_support(rng, N, W, density):
+    S = np.zeros(W, dtype=np.uint64)
+    bits = np.flatnonzero(rng.random(N) < density)
+    for b in bits:
+        S[b // 64] |= np.uint64(1) << np.uint64(b % 64)
+    return S
+
+
+@pytest.mark.parametrize("prune", [False, True])
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("name", PARITY_CASES)
+def test_parity_native_layout(name, mode, prune):
+    fx = load_trace(name)
+    p = problem_of(fx, mode, keep_vertex_trees=True, prune_trees=prune)
+    print(f"case {name} {mode} prune={prune} F={p.info.fvs_size} P={p.info.positions}", flush=True)
+    s = Solver(p)
+    rng = np.random.default_rng(7)
+    to, pv = p.array("tree_offset"), p.array("pos_vertex")
+    root_of_pos = np.repeat(np.arange(p.info.fvs_size), np.diff(to))
+    for density in (0.0, 0.05, 0.5, 1.0):
+        S = random_support(rng, p.info.num_non_tree, p.info.words, density)
+        got = s.parity(S)
+        want = pyoracle.parity(p, S)
+        assert np.array_equal(got, want[root_of_pos, pv]), f"density {density}"
+    s.close()
+
+
+@pytest.mark.parametrize("multiple_transfer", [False, True])
+@pytest.mark.parametrize("chunk", [3, 720])
+@pytest.mark.parametrize("name", ["grid4x4", "delaunay40", "chunglu60", "er150", "grid16x16"])
+def test_parity_reference_layout(name, chunk, multiple_transfer):
+    """gpu_struct drop-in: compressed_trees chunk arrays in, per-position parity out."""
+    fx = load_trace(name)
+    p = problem_of(fx, "reduced", keep_vertex_trees=True)
+    n, F, N, W = p.info.nodes, p.info.fvs_size, p.info.num_non_tree, p.info.words
+    print(f"case {name} chunk={chunk} mt={multiple_transfer} F={F} n={n}", flush=True)
+    rows_c, cols_c, eo_c, ni_c, out_c = [], [], [], [], []
+    nchunks = (F + chunk - 1) // chunk
+    for i in range(nchunks):
+        R = np.zeros(chunk * (n + 1), dtype=np.uint32)
+        Cc = np.zeros(chunk * n, dtype=np.uint32)
+        E = np.full(chunk * n, -7, dtype=np.int32)  # padded slots hold garbage, as pinned memory does
+        NI = np.zeros(chunk * n, dtype=np.uint32)
+        for s_ in range(chunk):
+            r = i * chunk + s_
+            if r >= F:
+                break
+            rr, cc, ee, nn = p.reference_layout(r)
+            R[s_ * (n + 1):(s_ + 1) * (n + 1)] = rr
+            Cc[s_ * n:(s_ + 1) * n] = cc
+            E[s_ * n:(s_ + 1) * n] = ee
+            NI[s_ * n:(s_ + 1) * n] = nn
+        rows_c.append(R)
+        cols_c.append(Cc)
+        eo_c.append(E)
+        ni_c.append(NI)
+        out_c.append(np.full(chunk * n, 9, dtype=np.uint32))
+    g = GpuStruct(p.info.directed_edges, N, W, n, F, chunk, nchunks)
+    g.initialize_memory(p.array("nt_map"), rows_c, cols_c, eo_c)
+    rng = np.random.default_rng(3)
+    for density in (0.1, 0.6):
+        S = random_support(rng, N, W, density)
+        g.copy_support_vector(S)
+        g.process_shortest_path(out_c, multiple_transfer=multiple_transfer)
+        want = pyoracle.parity(p, S)
+        for r in range(F):
+            i, s_ = divmod(r, chunk)
+            nodes_index = ni_c[i][s_ * n:(s_ + 1) * n]
+            got = out_c[i][s_ * n:(s_ + 1) * n][nodes_index]
+            assert np.array_equal(got, want[r].astype(np.uint32)), f"root {r}"
+    g.close()
+
+
+PHASE_CASES = trace_names()
+
+
+@pytest.mark.parametrize("prune", [False, True])
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("name", PHASE_CASES)
+def test_phase_loop_matches_oracle(name, mode, prune):
+    fx = load_trace(name)
+    p = problem_of(fx, mode, keep_vertex_trees=True, prune_trees=prune)
+    print(f"case {name} {mode} prune={prune} N={p.info.num_non_tree} K={p.info.num_candidates}", flush=True)
+    ref = pyoracle.run(p, want_s=True, want_c=True)
+    s = Solver(p, store_cycles=True)
+    s.run()
+    s.sync()
+    assert np.array_equal(s.selected(), ref["selected"])
+    assert np.array_equal(s.support_vectors(), ref["S"])
+    assert np.array_equal(s.cycle_vectors(), ref["C"])
+    # the chosen candidates reproduce the reference's own chosen cycles and total
+    r, a, b, w = p.candidate_table()
+    sel = s.selected()
+    chosen = np.stack([r[sel] + 1, a[sel] + 1, b[sel] + 1, w[sel]], axis=1).astype(np.int64).tolist()
+    assert chosen == [ph["chosen"] for ph in fx[mode]["phases"]]
+    s.close()
+
+
+def test_rerun_after_reset():
+    fx = load_trace("grid16x16")
+    p = problem_of(fx, "reduced", keep_vertex_trees=True)
+    s = Solver(p)
+    s.run()
+    s.sync()
+    first = s.selected().copy()
+    s.reset()
+    s.run(50)
+    s.run()
+    s.sync()
+    assert np.array_equal(s.selected(), first)
