@@ -68,66 +68,194 @@ constexpr uint32_t kNoParent = 0xFFFFFFFFu;
 constexpr uint32_t kNoRank = 0xFFFFFFFFu;
 constexpr int kWave = 64;
 constexpr int kLdsLimit = 64 * 1024;
-constexpr int kUnroll = 4;
+#ifndef MCB_UNROLL
+#define MCB_UNROLL 8
+#endif
+#ifndef MCB_BLOCK_WAVES
+#define MCB_BLOCK_WAVES 8
+#endif
+#ifndef MCB_BIG_WINDOWS
+#define MCB_BIG_WINDOWS 32
+#endif
+constexpr int kUnroll = MCB_UNROLL;           // tree windows per pipelined load group
+constexpr int kBlockWaves = MCB_BLOCK_WAVES;  // waves per block of the parity/search kernel
+constexpr uint32_t kBigPositions = 64u * MCB_BIG_WINDOWS;  // larger trees are shared by a whole block
 
+// The loads below are unconditional (a clamped address, the value selected afterwards): a load
+// under a divergent branch makes the compiler wait for every outstanding load at the join, which
+// would serialise the prefetch pipeline of the tree kernels.
 __device__ __forceinline__ uint32_t bit_of(const uint64_t *s, int32_t c) {
-    return c >= 0 ? static_cast<uint32_t>((s[c >> 6] >> (c & 63)) & 1ull) : 0u;
+    const uint32_t cc = c >= 0 ? static_cast<uint32_t>(c) : 0u;
+    const uint32_t bit = static_cast<uint32_t>((s[cc >> 6] >> (cc & 63)) & 1ull);
+    return c >= 0 ? bit : 0u;
+}
+
+// Parents and coordinates of kUnroll windows, loaded raw (clamped address, no select): the
+// values stay in flight until the group is processed.  Buffers alternate between two groups so
+// that no register holding an in-flight load is copied (a copy would wait for the load).
+struct WindowGroup {
+    uint32_t par[kUnroll];
+    int32_t co[kUnroll];
+};
+
+__device__ __forceinline__ void load_group(WindowGroup &g, const uint32_t *pp, const int32_t *pc, uint32_t npos,
+                                           uint32_t first_pos, uint32_t stride) {
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+        const uint32_t pos = first_pos + u * stride;
+        const uint32_t at = pos < npos ? pos : 0u;
+        g.par[u] = pp[at];
+        g.co[u] = pc[at];
+    }
 }
 
 __device__ __forceinline__ uint32_t pbit(const uint64_t *p, uint32_t pos) {
     return static_cast<uint32_t>((p[pos >> 6] >> (pos & 63)) & 1ull);
 }
 
-// Parity of 64 consecutive positions [wb, wb+64) of one tree, given their parent position and
-// the S bit of their parent edge.  Parents before wb are final in P (LDS bits); parents inside
-// the window are resolved level by level with ballots (in BFS-level order a window rarely spans
-// more than two levels).  Returns this lane's parity; lane 0 stores the 64-bit word.
-__device__ __forceinline__ uint32_t window_parity(uint64_t *P, uint32_t wb, uint32_t par, uint32_t b, int lane) {
-    const bool has_parent = par != kNoParent;
-    const bool inwin = has_parent && par >= wb;
-    uint32_t val = (has_parent && !inwin) ? (b ^ pbit(P, par)) : 0u;
-    bool resolved = !inwin;
+// Lanes of one window whose parent lies in the same window (the window straddles a level
+// boundary): settle them by repeated ballots; each round settles at least the lowest pending lane.
+__device__ __forceinline__ uint32_t settle_in_window(uint32_t val, bool pending_lane, uint32_t p, uint32_t wb,
+                                                     uint32_t sb) {
+    bool resolved = !pending_lane;
     uint64_t pending = __ballot(!resolved);
     while (pending) {
         const uint64_t rm = __ballot(resolved);
         const uint64_t pm = __ballot(val);
         if (!resolved) {
-            const uint32_t k = par - wb;
+            const uint32_t k = p - wb;
             if ((rm >> k) & 1ull) {
-                val = b ^ static_cast<uint32_t>((pm >> k) & 1ull);
+                val = sb ^ static_cast<uint32_t>((pm >> k) & 1ull);
                 resolved = true;
             }
         }
         pending = __ballot(!resolved);
     }
+    return val;
+}
+
+// Parity of every position of one tree segment into the wave's LDS bit array P (bit p = parity
+// of position p).  Positions are taken 64 at a time (a window, one lane each); the parents and
+// coordinates of the next kUnroll windows are loaded while the current kUnroll windows are
+// resolved.  A parent before the window is read from P (already final); a parent inside the
+// window only occurs where the window straddles a level boundary and is settled by ballots.
+// One window of tree_parity: par/co are the raw loads for position wb + lane.
+template <typename WriteFn>
+__device__ __forceinline__ void small_window(uint32_t wb, uint32_t npos, uint32_t rawp, int32_t rawc,
+                                             const uint64_t *S, uint64_t *P, int lane, WriteFn &write) {
+    const bool v = wb + lane < npos;
+    const uint32_t p = v ? rawp : kNoParent;
+    const uint32_t sb = bit_of(S, v ? rawc : -1);
+    const bool before = p < wb;
+    const uint32_t pb = pbit(P, before ? p : 0u);
+    uint32_t val = sb ^ (before ? pb : 0u);
+    const bool inwin = (p != kNoParent) && (p >= wb);
+    if (__ballot(inwin)) val = settle_in_window(val, inwin, p, wb, sb);
+    const uint64_t word = __ballot(val);
+    if (lane == 0) P[wb / kWave] = word;
+    write(wb + lane, val);
+}
+
+template <typename WriteFn>
+__device__ __forceinline__ void small_group(const WindowGroup &g, uint32_t base, uint32_t npos, const uint64_t *S,
+                                            uint64_t *P, int lane, WriteFn &write) {
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+        const uint32_t wb = base + u * kWave;
+        if (wb >= npos) break;
+        small_window(wb, npos, g.par[u], g.co[u], S, P, lane, write);
+    }
+}
+
+// Parity of every position of one tree segment into the wave's LDS bit array P (bit p = parity
+// of position p).  Positions are taken 64 at a time (a window, one lane each); the parents and
+// coordinates of the next kUnroll windows are loaded while the current kUnroll windows are
+// resolved.  A parent before the window is read from P (already final); a parent inside the
+// window only occurs where the window straddles a level boundary and is settled by ballots.
+template <typename WriteFn>
+__device__ __forceinline__ void tree_parity(const uint32_t *pp, const int32_t *pc, uint32_t npos, const uint64_t *S,
+                                            uint64_t *P, int lane, WriteFn write) {
+    constexpr uint32_t G = kWave * kUnroll;
+    WindowGroup A, B;
+    load_group(A, pp, pc, npos, lane, kWave);
+    for (uint32_t base = 0; base < npos; base += 2 * G) {
+        load_group(B, pp, pc, npos, base + G + lane, kWave);
+        small_group(A, base, npos, S, P, lane, write);
+        if (base + G >= npos) break;
+        load_group(A, pp, pc, npos, base + 2 * G + lane, kWave);
+        small_group(B, base + G, npos, S, P, lane, write);
+    }
+}
+
+// Parity of one big root's tree by all waves of a workgroup.  Window w (positions 64w..64w+63)
+// belongs to wave w mod nw; each wave walks its windows in increasing order with the next group of
+// loads in flight.  A parent in an earlier window is read from LDS once that window's ready stamp
+// equals this root's generation; every dependency points to a lower window and the lowest unfinished
+// window never waits on another wave, so the waits always end.
+// Parity of one window in a reference-layout tree (used by the chunked drop-in kernel): parents
+// before wb are final in P; parents inside the window are settled with ballots.
+__device__ __forceinline__ uint32_t window_parity(uint64_t *P, uint32_t wb, uint32_t par, uint32_t b, int lane) {
+    const bool before = par < wb;
+    const uint32_t pb = pbit(P, before ? par : 0u);
+    uint32_t val = b ^ (before ? pb : 0u);
+    const bool inwin = (par != kNoParent) && (par >= wb);
+    if (__ballot(inwin)) val = settle_in_window(val, inwin, par, wb, b);
     const uint64_t word = __ballot(val);
     if (lane == 0) P[wb >> 6] = word;
     __builtin_amdgcn_wave_barrier();
     return val;
 }
 
-// Parity of every position of one tree segment into the wave's LDS bit array P.  The next
-// kUnroll windows' parents and coordinates are loaded before any of them is resolved.
 template <typename WriteFn>
-__device__ __forceinline__ void tree_parity(const uint32_t *pp, const int32_t *pc, uint32_t npos, const uint64_t *S,
-                                            uint64_t *P, int lane, WriteFn write) {
-    for (uint32_t base = 0; base < npos; base += kWave * kUnroll) {
-        uint32_t par[kUnroll];
-        int32_t co[kUnroll];
+__device__ __forceinline__ void coop_group(const WindowGroup &g, uint32_t w0, uint32_t nwin, uint32_t npos,
+                                           const uint64_t *S, uint64_t *P, uint32_t *ready, uint32_t gen, int nw,
+                                           int lane, WriteFn &write) {
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-            const uint32_t pos = base + u * kWave + lane;
-            const bool v = pos < npos;
-            par[u] = v ? pp[pos] : kNoParent;
-            co[u] = v ? pc[pos] : -1;
+    for (int u = 0; u < kUnroll; ++u) {
+        const uint32_t w = w0 + u * nw;
+        if (w >= nwin) break;
+        const uint32_t wb = w * kWave;
+        const bool v = wb + lane < npos;
+        const uint32_t p = v ? g.par[u] : kNoParent;
+        const uint32_t sb = bit_of(S, v ? g.co[u] : -1);
+        const bool before = p < wb;
+        const uint32_t pw = before ? p / kWave : 0u;
+        // wait until the parent's window is final (windows of this wave are final already)
+        while (before && __hip_atomic_load(ready + pw, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != gen)
+            __builtin_amdgcn_s_sleep(1);
+        const uint32_t pb = pbit(P, before ? p : 0u);
+        uint32_t val = sb ^ (before ? pb : 0u);
+        const bool inwin = (p != kNoParent) && (p >= wb);
+        if (__ballot(inwin)) val = settle_in_window(val, inwin, p, wb, sb);
+        const uint64_t word = __ballot(val);
+        if (lane == 0) {
+            P[w] = word;
+            __hip_atomic_store(ready + w, gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
-#pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-            const uint32_t wb = base + u * kWave;
-            if (wb >= npos) break;
-            const uint32_t val = window_parity(P, wb, par[u], bit_of(S, co[u]), lane);
-            write(wb + lane, val);
-        }
+        write(wb + lane, val);
+    }
+}
+
+// Parity of one big root's tree by all waves of a workgroup.  Window w (positions 64w..64w+63)
+// belongs to wave w mod nw; each wave walks its windows in increasing order with the next group of
+// loads in flight.  A parent in an earlier window is read from LDS once that window's ready stamp
+// equals this root's generation; every dependency points to a lower window and the lowest unfinished
+// window never waits on another wave, so the waits always end.
+template <typename WriteFn>
+__device__ __forceinline__ void tree_parity_coop(const uint32_t *pp, const int32_t *pc, uint32_t npos,
+                                                 const uint64_t *S, uint64_t *P, uint32_t *ready, uint32_t gen,
+                                                 int wid, int nw, int lane, WriteFn write) {
+    const uint32_t nwin = (npos + kWave - 1) / kWave;
+    const uint32_t stride = static_cast<uint32_t>(nw) * kWave;  // positions between a wave's windows
+    const uint32_t gstep = kUnroll * static_cast<uint32_t>(nw);   // windows between a wave's groups
+    WindowGroup A, B;
+    load_group(A, pp, pc, npos, wid * kWave + lane, stride);
+    for (uint32_t w0 = wid; w0 < nwin; w0 += 2 * gstep) {
+        load_group(B, pp, pc, npos, (w0 + gstep) * kWave + lane, stride);
+        coop_group(A, w0, nwin, npos, S, P, ready, gen, nw, lane, write);
+        if (w0 + gstep >= nwin) break;
+        load_group(A, pp, pc, npos, (w0 + 2 * gstep) * kWave + lane, stride);
+        coop_group(B, w0 + gstep, nwin, npos, S, P, ready, gen, nw, lane, write);
     }
 }
 
@@ -135,8 +263,9 @@ struct SearchArgs {
     const int64_t *tree_off;
     const uint32_t *parent_pos;
     const int32_t *pos_coord;
-    const uint32_t *root_order;     // roots by ascending lowest candidate rank
-    const uint32_t *root_min_rank;  // indexed by root
+    const uint32_t *big_list;       // roots with more than kBigPositions positions, largest first
+    const uint32_t *small_list;     // the other roots, largest first
+    const uint32_t *root_min_rank;  // lowest candidate rank of each root
     const int64_t *rc_off;          // per-root candidate segment
     const uint32_t *rc_pu, *rc_pv, *rc_rank;
     const int32_t *rc_coord;
@@ -144,68 +273,112 @@ struct SearchArgs {
     uint32_t *best;                 // &best[e]
     uint8_t *parity_out;            // parity-only mode: one byte per position
     uint64_t *wave_stats;           // optional: per wave, positions streamed and candidates read
-    int32_t F, W, p_words, total_waves;
+    int32_t n_big, n_small, W, p_words_big, p_words_small;
 };
 
-// (a)+(b).  SEARCH = false computes the parity of every position of every root into parity_out.
-template <bool S_IN_LDS, bool SEARCH>
-__global__ __launch_bounds__(256) void k_parity_search(SearchArgs a) {
-    extern __shared__ uint64_t lds[];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wib = threadIdx.x / kWave;
-    const int wpb = blockDim.x / kWave;
-    const uint64_t *Sx = a.S;
-    if (S_IN_LDS) {
-        for (int i = threadIdx.x; i < a.W; i += blockDim.x) lds[i] = a.S[i];
-        __syncthreads();
-        Sx = lds;
+// Scans candidate windows k0, k0 + stride, ... of one root (ranks ascending within the root) and
+// lowers best to the rank of the first odd one; stops early once the ranks cannot win.
+__device__ __forceinline__ void scan_candidates(const SearchArgs &a, const uint64_t *Sx, const uint64_t *P,
+                                                int64_t first, int64_t ce, int64_t stride, uint32_t bestk, int lane,
+                                                uint64_t &n_cand) {
+    for (int64_t k0 = first; k0 < ce; k0 += stride) {
+        const int64_t k = k0 + lane;
+        const bool v = k < ce;
+        n_cand += static_cast<uint64_t>(ce - k0 < kWave ? ce - k0 : kWave);
+        const uint32_t rk = v ? a.rc_rank[k] : kNoRank;
+        const bool live = rk < bestk;
+        uint32_t odd = 0;
+        if (live) odd = bit_of(Sx, a.rc_coord[k]) ^ pbit(P, a.rc_pu[k]) ^ pbit(P, a.rc_pv[k]);
+        const uint64_t m = __ballot(odd != 0);
+        if (m) {
+            const int firstlane = __ffsll(static_cast<unsigned long long>(m)) - 1;
+            const uint32_t win = __shfl(rk, firstlane);
+            if (lane == 0) atomicMin(a.best, win);
+            return;
+        }
+        if (__ballot(v && !live)) return;
+        bestk = __hip_atomic_load(a.best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    uint64_t *P = lds + (S_IN_LDS ? a.W : 0) + static_cast<size_t>(wib) * a.p_words;
+}
+
+// (a)+(b) for one phase.  Block b takes big roots b, b + grid, ... (all waves of the block on one
+// tree), then wave w of the grid takes small roots w, w + waves, ...; both lists are sorted by size,
+// largest first, so the static round-robin spreads the large trees evenly (a shared work counter
+// costs more than it balances).  A root whose lowest candidate rank is not below the best rank found
+// so far is skipped.  SEARCH = false computes the parity of every position of every root into
+// parity_out instead.
+template <bool S_IN_LDS, bool SEARCH>
+__global__ __launch_bounds__(kBlockWaves * kWave) void k_parity_search(SearchArgs a) {
+    extern __shared__ uint64_t lds[];
+    __shared__ uint32_t skip;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    const uint64_t *Sx = a.S;
+    uint64_t *Pb = lds + (S_IN_LDS ? a.W : 0);
+    uint32_t *ready = reinterpret_cast<uint32_t *>(Pb + a.p_words_big);
+    uint64_t *Ps = Pb + a.p_words_big + (a.p_words_big + 1) / 2 + static_cast<size_t>(wid) * a.p_words_small;
+    if (S_IN_LDS)
+        for (int i = threadIdx.x; i < a.W; i += blockDim.x) lds[i] = a.S[i];
+    for (int i = threadIdx.x; i < a.p_words_big; i += blockDim.x) ready[i] = 0;
+    __syncthreads();
+    if (S_IN_LDS) Sx = lds;
     uint64_t n_pos = 0, n_cand = 0;
-    const int gw = blockIdx.x * wpb + wib;
-    for (int idx = gw; idx < a.F; idx += a.total_waves) {
-        const uint32_t r = SEARCH ? a.root_order[idx] : static_cast<uint32_t>(idx);
+    uint32_t gen = 0;
+    for (uint32_t it = blockIdx.x; it < static_cast<uint32_t>(a.n_big); it += gridDim.x) {
+        if (threadIdx.x == 0)
+            skip = SEARCH ? (a.root_min_rank[a.big_list[it]] >=
+                             __hip_atomic_load(a.best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                          : 0u;
+        __syncthreads();
+        const uint32_t sk = skip;
+        __syncthreads();
+        if (sk) continue;
+        const uint32_t r = a.big_list[it];
+        const int64_t off = a.tree_off[r];
+        const uint32_t npos = static_cast<uint32_t>(a.tree_off[r + 1] - off);
+        ++gen;
+        if (lane == 0 && wid == 0) n_pos += npos;
+        if (SEARCH) {
+            tree_parity_coop(a.parent_pos + off, a.pos_coord + off, npos, Sx, Pb, ready, gen, wid, nw, lane,
+                             [](uint32_t, uint32_t) {});
+        } else {
+            uint8_t *out = a.parity_out + off;
+            tree_parity_coop(a.parent_pos + off, a.pos_coord + off, npos, Sx, Pb, ready, gen, wid, nw, lane,
+                             [&](uint32_t pos, uint32_t v) {
+                                 if (pos < npos) out[pos] = static_cast<uint8_t>(v);
+                             });
+        }
+        __syncthreads();
+        if (SEARCH) {
+            const uint32_t bestk = __hip_atomic_load(a.best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            scan_candidates(a, Sx, Pb, a.rc_off[r] + static_cast<int64_t>(wid) * kWave, a.rc_off[r + 1],
+                            static_cast<int64_t>(nw) * kWave, bestk, lane, n_cand);
+        }
+        __syncthreads();
+    }
+    for (uint32_t it = blockIdx.x * nw + wid; it < static_cast<uint32_t>(a.n_small); it += gridDim.x * nw) {
+        const uint32_t r = a.small_list[it];
         uint32_t bestk = kNoRank;
         if (SEARCH) {
             bestk = __hip_atomic_load(a.best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (a.root_min_rank[r] >= bestk) break;  // later roots start even higher
+            if (a.root_min_rank[r] >= bestk) continue;
         }
         const int64_t off = a.tree_off[r];
         const uint32_t npos = static_cast<uint32_t>(a.tree_off[r + 1] - off);
-        n_pos += npos;
+        if (lane == 0) n_pos += npos;
         if (SEARCH) {
-            tree_parity(a.parent_pos + off, a.pos_coord + off, npos, Sx, P, lane, [](uint32_t, uint32_t) {});
+            tree_parity(a.parent_pos + off, a.pos_coord + off, npos, Sx, Ps, lane, [](uint32_t, uint32_t) {});
+            scan_candidates(a, Sx, Ps, a.rc_off[r], a.rc_off[r + 1], kWave, bestk, lane, n_cand);
         } else {
             uint8_t *out = a.parity_out + off;
-            tree_parity(a.parent_pos + off, a.pos_coord + off, npos, Sx, P, lane, [&](uint32_t pos, uint32_t v) {
+            tree_parity(a.parent_pos + off, a.pos_coord + off, npos, Sx, Ps, lane, [&](uint32_t pos, uint32_t v) {
                 if (pos < npos) out[pos] = static_cast<uint8_t>(v);
             });
-            continue;
         }
-        // Candidates of this root, ranks ascending: the first odd one is this root's best.
-        const int64_t cb = a.rc_off[r], ce = a.rc_off[r + 1];
-        for (int64_t k0 = cb; k0 < ce; k0 += kWave) {
-            const int64_t k = k0 + lane;
-            const bool v = k < ce;
-            n_cand += static_cast<uint64_t>(ce - k0 < kWave ? ce - k0 : kWave);
-            const uint32_t rk = v ? a.rc_rank[k] : kNoRank;
-            const bool live = rk < bestk;
-            uint32_t odd = 0;
-            if (live) odd = bit_of(Sx, a.rc_coord[k]) ^ pbit(P, a.rc_pu[k]) ^ pbit(P, a.rc_pv[k]);
-            const uint64_t m = __ballot(odd != 0);
-            if (m) {
-                const int first = __ffsll(static_cast<unsigned long long>(m)) - 1;
-                const uint32_t win = __shfl(rk, first);
-                if (lane == 0) atomicMin(a.best, win);
-                break;
-            }
-            if (__ballot(v && !live)) break;  // the rest of the list ranks no better
-            if (((k0 - cb) & (kWave * 8 - 1)) == 0)
-                bestk = __hip_atomic_load(a.best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        __builtin_amdgcn_wave_barrier();
     }
     if (a.wave_stats && lane == 0) {  // each wave owns its slot; launches are stream-ordered
+        const size_t gw = static_cast<size_t>(blockIdx.x) * nw + wid;
         a.wave_stats[2 * gw] += n_pos;
         a.wave_stats[2 * gw + 1] += n_cand;
     }
@@ -683,14 +856,14 @@ void mcb_gpu_destroy(mcb_gpu *g) {
 
 struct mcb_solver {
     mcb_solver_options opt{};
-    int32_t F = 0, N = 0, W = 0, max_pos = 0, p_words = 0;
+    int32_t F = 0, N = 0, W = 0, max_pos = 0, p_words_big = 0, p_words_small = 0, n_big = 0, n_small = 0;
     int64_t K = 0, Ptot = 0, npad = 0;
     int32_t phase = 0;
     // trees and candidates
     int64_t *tree_off = nullptr;
     uint32_t *parent_pos = nullptr;
     int32_t *pos_coord = nullptr;
-    uint32_t *root_order = nullptr, *root_min_rank = nullptr;
+    uint32_t *big_list = nullptr, *small_list = nullptr, *root_min_rank = nullptr;
     int64_t *rc_off = nullptr;
     uint32_t *rc_pu = nullptr, *rc_pv = nullptr, *rc_rank = nullptr;
     int32_t *rc_coord = nullptr;
@@ -751,7 +924,8 @@ SearchArgs search_args(const mcb_solver *s, int32_t e) {
     a.tree_off = s->tree_off;
     a.parent_pos = s->parent_pos;
     a.pos_coord = s->pos_coord;
-    a.root_order = s->root_order;
+    a.big_list = s->big_list;
+    a.small_list = s->small_list;
     a.root_min_rank = s->root_min_rank;
     a.rc_off = s->rc_off;
     a.rc_pu = s->rc_pu;
@@ -762,10 +936,11 @@ SearchArgs search_args(const mcb_solver *s, int32_t e) {
     a.best = s->best + e;
     a.parity_out = nullptr;
     a.wave_stats = s->wave_stats;
-    a.F = s->F;
+    a.n_big = s->n_big;
+    a.n_small = s->n_small;
     a.W = s->W;
-    a.p_words = s->p_words;
-    a.total_waves = s->blocks * s->wpb;
+    a.p_words_big = s->p_words_big;
+    a.p_words_small = s->p_words_small;
     return a;
 }
 
@@ -834,7 +1009,8 @@ void mcb_solver_options_init(mcb_solver_options *opt) {
 
 void mcb_solver_destroy(mcb_solver *s) {
     if (!s) return;
-    void *ptrs[] = {s->tree_off, s->parent_pos, s->pos_coord, s->root_order, s->root_min_rank, s->rc_off, s->rc_pu,
+    void *ptrs[] = {s->tree_off, s->parent_pos, s->pos_coord, s->big_list, s->small_list, s->root_min_rank,
+                    s->rc_off, s->rc_pu,
                     s->rc_pv, s->rc_rank, s->rc_coord, s->cand_root, s->cand_pu, s->cand_pv, s->cand_coord, s->Sw,
                     s->Scur, s->C, s->C_store, s->D, s->Dcount, s->best, s->status, s->flag, s->wave_stats};
     for (void *p : ptrs)
@@ -890,8 +1066,7 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
         if (d->cand_pos_u[k] >= np || d->cand_pos_v[k] >= np || d->cand_coord[k] >= s->N)
             return cleanup(fail(MCB_E_INVALID, "candidate position out of range"));
     }
-    s->p_words = (s->max_pos + 63) / 64;
-    // Root-major candidate copies and the root visiting order (lowest rank first).
+    // Root-major candidate copies, lowest rank per root, and the two work lists (largest first).
     std::vector<uint32_t> rc_pu(K), rc_pv(K), rc_rank(K), min_rank(F, kNoRank), order(F);
     std::vector<int32_t> rc_coord(K);
     for (int32_t r = 0; r < F; ++r) {
@@ -908,13 +1083,29 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
         if (b > a) min_rank[r] = d->root_cand_rank[a];
     }
     std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return min_rank[x] < min_rank[y]; });
+    auto npos_of = [&](uint32_t r) { return d->tree_offset[r + 1] - d->tree_offset[r]; };
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return npos_of(x) > npos_of(y); });
+    std::vector<uint32_t> big, small;
+    int64_t max_small = 1;
+    for (uint32_t r : order) {
+        if (npos_of(r) > static_cast<int64_t>(kBigPositions)) {
+            big.push_back(r);
+        } else {
+            small.push_back(r);
+            max_small = std::max(max_small, npos_of(r));
+        }
+    }
+    s->n_big = static_cast<int32_t>(big.size());
+    s->n_small = static_cast<int32_t>(small.size());
+    s->p_words_big = big.empty() ? 0 : static_cast<int32_t>((s->max_pos + 63) / 64);
+    s->p_words_small = small.empty() ? 0 : static_cast<int32_t>((max_small + 63) / 64);
 
     int64_t &B = s->bytes;
     if ((rc = upload(&s->tree_off, d->tree_offset, F + 1, B))) return cleanup(rc);
     if ((rc = upload(&s->parent_pos, d->parent_pos, s->Ptot, B))) return cleanup(rc);
     if ((rc = upload(&s->pos_coord, d->pos_coord, s->Ptot, B))) return cleanup(rc);
-    if ((rc = upload(&s->root_order, order.data(), F, B))) return cleanup(rc);
+    if ((rc = upload(&s->big_list, big.data(), big.size(), B))) return cleanup(rc);
+    if ((rc = upload(&s->small_list, small.data(), small.size(), B))) return cleanup(rc);
     if ((rc = upload(&s->root_min_rank, min_rank.data(), F, B))) return cleanup(rc);
     if ((rc = upload(&s->rc_off, d->root_cand_offset, F + 1, B))) return cleanup(rc);
     if ((rc = upload(&s->rc_pu, rc_pu.data(), K, B))) return cleanup(rc);
@@ -937,26 +1128,25 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(MCB_E_DEVICE, "hipStreamCreate"));
 
-    // Launch shape of the parity/search kernel: S_e and one bit per position per wave in LDS.
-    const int p_bytes = s->p_words * 8;
-    if (p_bytes > kLdsLimit) return cleanup(fail(MCB_E_INVALID, "a tree layout exceeds the LDS parity buffer"));
-    int wpb = s->opt.waves_per_block > 0 ? std::min(4, s->opt.waves_per_block) : 4;
-    s->s_in_lds = static_cast<int64_t>(s->W) * 8 + p_bytes <= kLdsLimit;
-    const int s_bytes = s->s_in_lds ? s->W * 8 : 0;
-    while (wpb > 1 && s_bytes + wpb * p_bytes > kLdsLimit) --wpb;
-    s->wpb = wpb;
-    s->lds_bytes = s_bytes + wpb * p_bytes;
+    // Launch shape of the parity/search kernel: S_e, one bit per position of the block's big tree
+    // plus its ready stamps, and one bit per position of each wave's small tree, all in LDS.
+    const int64_t tree_bytes = 8ll * s->p_words_big + 4ll * (s->p_words_big + 1) / 2 * 2 +
+                               8ll * kBlockWaves * s->p_words_small;
+    if (tree_bytes > kLdsLimit) return cleanup(fail(MCB_E_INVALID, "a tree layout exceeds the LDS parity buffer"));
+    s->wpb = kBlockWaves;
+    s->s_in_lds = static_cast<int64_t>(s->W) * 8 + tree_bytes <= kLdsLimit;
+    s->lds_bytes = static_cast<int>((s->s_in_lds ? static_cast<int64_t>(s->W) * 8 : 0) + tree_bytes);
     int per_cu = 0;
     if (s->s_in_lds)
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_parity_search<true, true>, wpb * kWave,
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_parity_search<true, true>, kBlockWaves * kWave,
                                                              s->lds_bytes));
     else
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_parity_search<false, true>, wpb * kWave,
-                                                             s->lds_bytes));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_parity_search<false, true>,
+                                                             kBlockWaves * kWave, s->lds_bytes));
     per_cu = std::max(1, per_cu);
     int64_t blocks = static_cast<int64_t>(per_cu) * g_dev.prop.multiProcessorCount;
     if (s->opt.max_blocks > 0) blocks = std::min<int64_t>(blocks, s->opt.max_blocks);
-    blocks = std::min<int64_t>(blocks, (F + wpb - 1) / wpb);
+    blocks = std::min<int64_t>(blocks, s->n_big + (s->n_small + kBlockWaves - 1) / kBlockWaves);
     s->blocks = static_cast<int>(std::max<int64_t>(1, blocks));
     if ((rc = dev_alloc(&s->wave_stats, 2 * static_cast<size_t>(s->blocks) * s->wpb, B))) return cleanup(rc);
     // Cycle kernel LDS: C_e plus the chosen root's parent array when it fits.
