@@ -154,7 +154,9 @@ int mcb_solver_cycle_vectors(mcb_solver *s, uint64_t *out, int32_t count);
  * out[p] = XOR of S bits along the path root -> position p (kernel (a) alone). */
 int mcb_solver_parity(mcb_solver *s, const uint64_t *S, uint8_t *out);
 
-/* Kernel ids for timing. */
+/* Kernel ids for timing.  A phase is one launch of the phase kernel (update of the previous phase,
+ * parity + search, cycle vector): its time is reported under MCB_K_PARITY_SEARCH; the other ids
+ * are kept for layout stability and report zero launches. */
 enum { MCB_K_PARITY_SEARCH = 0, MCB_K_CYCLE = 1, MCB_K_DOT = 2, MCB_K_XOR = 3, MCB_K_COUNT = 4 };
 
 /* Accumulated event time (ms) and launch count of one kernel since the last reset (timing=1). */

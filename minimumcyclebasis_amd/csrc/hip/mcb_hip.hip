@@ -5,24 +5,27 @@
 //              order (parent before child), one contiguous segment per root (tree_off i64[F+1]).
 //              pos_coord holds the cycle-space coordinate of the edge to the parent (-1: none),
 //              so the parity pass needs no gather through the 2m'-entry coordinate map.
-//   candidates rank-indexed (cand_*) for the cycle kernel, and root-major copies ordered by rank
+//              pos_pre/pos_end u32[P] hold DFS pre-order intervals (ancestor tests for C_e).
+//   candidates rank-indexed (cand_*) for the cycle vector, and root-major copies ordered by rank
 //              (rc_*) for the search, 16 B per candidate.
 //   support    word-major matrix Sw u64[W][Npad]: Sw[w][j] = word w of S_j without its own bit j
 //              (bit j of S_j is always 1).  A phase-e update touches words 0..e/64 of columns
-//              j > e, coalesced across j.  S_e itself is also kept contiguous (double buffer) for
-//              the parity kernel.
+//              j > e, coalesced across j.  S_e itself is also kept contiguous for the parity pass.
 //
-// Kernels per phase e
-//   k_parity_search  (a)+(b): one wave per root; parity of 64 consecutive positions per step,
-//                    kept as bits in LDS; S_e in LDS; then the root's candidates in rank order
-//                    until the first odd one -> atomicMin into best[e].  Roots are visited in
-//                    order of their lowest candidate rank, so a wave stops as soon as its next
-//                    root cannot beat the best rank found so far.
-//   k_cycle          the chosen candidate's cycle vector C_e (walk of the two tree paths with the
-//                    root's parent array staged in LDS) and the list of its nonzero words below e.
-//   k_dot            p_j = <C_e, S_j> for j > e, reading only C_e's nonzero words.
-//   k_xor            S_j ^= S_e for the j with p_j = 1, one thread per (word, j); also writes S_{e+1}
-//                    contiguously for the next phase.
+// One kernel per phase e (k_phase)
+//   prologue   the update (c), one thread per column j > e: the dot product of phase e-1
+//              (p_j = <C_{e-1}, S_j>, from C_{e-1}'s nonzero words below e-1) and the XOR of phase
+//              e-2 (S_j ^= S_{e-2} where phase e-2's p_j was 1, over S_{e-2}'s nonzero words).
+//              Both steps of a column run in one thread, so no column is read while it is written.
+//   body       (a)+(b): tree-path parity of every root with S_e in LDS, kept as one bit per position
+//              in LDS, then the root's candidates in rank order until the first odd one ->
+//              atomicMin into best[e].  Trees larger than 2048 positions are shared by the 8 waves of
+//              a block; smaller ones take one wave each.
+//   epilogue   the last block to finish builds C_e for the chosen candidate (ancestor test against
+//              DFS pre-order intervals over the root's tree), C_e's nonzero words below e, and
+//              column e+1 of (c), which makes S_{e+1} final and writes it contiguously.
+// S_e rotates through four contiguous buffers and C_e through two, so a phase reads earlier
+// phases' outputs while writing its own.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -259,26 +262,53 @@ __device__ __forceinline__ void tree_parity_coop(const uint32_t *pp, const int32
     }
 }
 
-struct SearchArgs {
+struct PhaseArgs {
+    // trees, in BFS-level order per root, with DFS pre-order intervals for ancestor tests
     const int64_t *tree_off;
     const uint32_t *parent_pos;
     const int32_t *pos_coord;
-    const uint32_t *big_list;       // roots with more than kBigPositions positions, largest first
-    const uint32_t *small_list;     // the other roots, largest first
-    const uint32_t *root_min_rank;  // lowest candidate rank of each root
-    const int64_t *rc_off;          // per-root candidate segment
+    const uint32_t *pos_pre, *pos_end;  // x is an ancestor-or-self of y iff pre[x] <= pre[y] < end[x]
+    const uint32_t *big_list;           // roots with more than kBigPositions positions, largest first
+    const uint32_t *small_list;         // the other roots, largest first
+    const uint32_t *root_min_rank;      // lowest candidate rank of each root
+    // candidates: root-major (search) and rank-indexed (cycle vector)
+    const int64_t *rc_off;
     const uint32_t *rc_pu, *rc_pv, *rc_rank;
     const int32_t *rc_coord;
-    const uint64_t *S;              // contiguous S_e
-    uint32_t *best;                 // &best[e]
-    uint8_t *parity_out;            // parity-only mode: one byte per position
-    uint64_t *wave_stats;           // optional: per wave, positions streamed and candidates read
-    int32_t n_big, n_small, W, p_words_big, p_words_small;
+    const uint32_t *cand_root, *cand_pu, *cand_pv;
+    const int32_t *cand_coord;
+    // support vectors: word-major matrix without the diagonal, plus contiguous copies in a ring of
+    // four (S_{e-2}, S_{e-1}, S_e, S_{e+1}) with the lists of their nonzero words
+    uint64_t *Sw;
+    int64_t npad;
+    const uint64_t *S;                     // S_e
+    const uint64_t *S_m1;                  // S_{e-1}
+    const uint64_t *S_m2;                  // S_{e-2}
+    uint64_t *S_next;                      // S_{e+1}, written by the epilogue
+    const uint32_t *E_m2, *Ecount_m2;      // nonzero words of S_{e-2}
+    uint32_t *E_next, *Ecount_next;        // nonzero words of S_{e+1}, written by the epilogue
+    // cycle vectors: C_{e-1} and its nonzero words below e-1 (read), C_e (written)
+    const uint64_t *C_prev;
+    const uint32_t *D_prev, *Dcount_prev;
+    const uint32_t *c_prev;                // <C_{e-1}, S_{e-2}>
+    uint64_t *C;
+    uint32_t *D, *Dcount;
+    uint32_t *c_out;                       // <C_e, S_{e-1}>, written by the epilogue
+    uint64_t *C_store;                     // optional: row e of the stored cycle vectors
+    uint8_t *flag;           // per column j: p_j of the latest phase whose dot products are done
+    uint32_t *best;          // &best[e]
+    uint32_t *done;          // &done[e]: blocks finished
+    uint32_t *status;
+    uint8_t *parity_out;     // parity-only mode: one byte per position
+    uint64_t *wave_stats;    // optional: per wave, positions streamed and candidates read
+    int32_t e, N, W, n_big, n_small, p_words_big, p_words_small;
+    int32_t upd_tiles;  // prologue tiles: 64 columns from e+1 each (0 in phase 0)
 };
+
 
 // Scans candidate windows k0, k0 + stride, ... of one root (ranks ascending within the root) and
 // lowers best to the rank of the first odd one; stops early once the ranks cannot win.
-__device__ __forceinline__ void scan_candidates(const SearchArgs &a, const uint64_t *Sx, const uint64_t *P,
+__device__ __forceinline__ void scan_candidates(const PhaseArgs &a, const uint64_t *Sx, const uint64_t *P,
                                                 int64_t first, int64_t ce, int64_t stride, uint32_t bestk, int lane,
                                                 uint64_t &n_cand) {
     for (int64_t k0 = first; k0 < ce; k0 += stride) {
@@ -301,19 +331,183 @@ __device__ __forceinline__ void scan_candidates(const SearchArgs &a, const uint6
     }
 }
 
-// (a)+(b) for one phase.  Block b takes big roots b, b + grid, ... (all waves of the block on one
-// tree), then wave w of the grid takes small roots w, w + waves, ...; both lists are sorted by size,
-// largest first, so the static round-robin spreads the large trees evenly (a shared work counter
-// costs more than it balances).  A root whose lowest candidate rank is not below the best rank found
-// so far is skipped.  SEARCH = false computes the parity of every position of every root into
-// parity_out instead.
+// The support-vector update (c) of phase p, S_j ^= S_p for every j > p with <C_p, S_j> = 1, is
+// split across two later kernels so that each column is only ever touched by one thread:
+//   phase p+1's prologue computes f_j = <C_p, S_j> for j > p+1 (flag[j]);
+//   phase p+2's prologue applies S_j ^= S_p where f_j = 1 (j > p+2);
+//   column p+1 (resp. p+2) is finished by phase p's (resp. p+1's) epilogue instead.
+// In phase e's prologue the thread of column j > e therefore first reads f_j of phase e-2, computes
+// <C_{e-1}, S_j> on the column as it still is, corrects it for the pending XOR of S_{e-2}
+// (<C_{e-1}, S_j ^ S_{e-2}> = <C_{e-1}, S_j> ^ <C_{e-1}, S_{e-2}>), stores it as f_j of phase e-1,
+// and then applies the pending XOR.  The dot product needs only C_{e-1}'s nonzero words below e-1
+// (S_j has no other bits there besides its own bit j, which meets C_{e-1}[j]); the XOR needs only
+// S_{e-2}'s nonzero words.  Loads are issued in batches before the stores.
+__device__ __forceinline__ void column_update(const PhaseArgs &a, int64_t j) {
+    constexpr int kB = 8;
+    const uint32_t f_old = a.e >= 2 ? a.flag[j] : 0u;
+    uint32_t p = static_cast<uint32_t>((a.C_prev[j >> 6] >> (j & 63)) & 1ull) ^ (f_old & *a.c_prev);
+    const uint32_t nd = *a.Dcount_prev;
+#pragma unroll 4
+    for (uint32_t q = 0; q < nd; ++q) {
+        const uint32_t w = a.D_prev[q];
+        p ^= static_cast<uint32_t>(__popcll(a.C_prev[w] & a.Sw[static_cast<int64_t>(w) * a.npad + j]) & 1);
+    }
+    a.flag[j] = static_cast<uint8_t>(p);
+    if (!f_old) return;
+    const uint32_t ne = *a.Ecount_m2;
+    for (uint32_t q0 = 0; q0 < ne; q0 += kB) {
+        uint32_t w[kB];
+        uint64_t se[kB], v[kB];
+#pragma unroll
+        for (int k = 0; k < kB; ++k) {
+            const uint32_t q = q0 + k;
+            w[k] = a.E_m2[q < ne ? q : 0];
+            se[k] = q < ne ? a.S_m2[w[k]] : 0ull;
+        }
+#pragma unroll
+        for (int k = 0; k < kB; ++k) v[k] = a.Sw[static_cast<int64_t>(w[k]) * a.npad + j];
+#pragma unroll
+        for (int k = 0; k < kB; ++k)
+            if (se[k]) a.Sw[static_cast<int64_t>(w[k]) * a.npad + j] = v[k] ^ se[k];
+    }
+}
+
+// Epilogue of phase e, run by the last block to finish:
+//   C_e of the chosen candidate: coordinates of the edges on root -> u, root -> v and of the
+//   candidate's own edge (the two paths meet only at the root, so setting bits equals XOR), found by
+//   testing every position of the root's tree against the DFS intervals of u and v;
+//   the list of C_e's nonzero words below e, and <C_e, S_{e-1}> for the next prologue;
+//   column e+1: the pending XOR of S_{e-1}, then phase e's own update, which makes S_{e+1} final;
+//   S_{e+1} and the list of its nonzero words, contiguous, for phase e+1.
+// LDS: C_e (W words) then the word list (W uint32).
+__device__ void phase_epilogue(const PhaseArgs &a, uint64_t *lds) {
+    constexpr int kB = 8;
+    __shared__ uint32_t cnt;
+    __shared__ uint32_t par;
+    __shared__ uint32_t cpar;
+    const int tid = threadIdx.x;
+    uint64_t *Cl = lds;
+    uint32_t *Dl = reinterpret_cast<uint32_t *>(lds + a.W);
+    const uint32_t rank = __hip_atomic_load(a.best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int i = tid; i < a.W; i += blockDim.x) Cl[i] = 0;
+    if (tid == 0) {
+        cnt = 0;
+        par = 0;
+        cpar = 0;
+    }
+    __syncthreads();
+    if (rank != kNoRank) {
+        const uint32_t r = a.cand_root[rank];
+        const int64_t off = a.tree_off[r];
+        const uint32_t npos = static_cast<uint32_t>(a.tree_off[r + 1] - off);
+        const uint32_t pu = a.pos_pre[off + a.cand_pu[rank]];
+        const uint32_t pv = a.pos_pre[off + a.cand_pv[rank]];
+        if (tid == 0) {
+            const int32_t c = a.cand_coord[rank];
+            if (c >= 0) atomicOr(reinterpret_cast<unsigned long long *>(&Cl[c >> 6]), 1ull << (c & 63));
+        }
+        for (uint32_t x0 = 1 + tid; x0 < npos; x0 += kB * blockDim.x) {
+            uint32_t lo[kB], hi[kB];
+            int32_t c[kB];
+#pragma unroll
+            for (int k = 0; k < kB; ++k) {
+                const uint32_t x = x0 + k * blockDim.x;
+                const uint32_t at = x < npos ? x : 0u;
+                lo[k] = a.pos_pre[off + at];
+                hi[k] = a.pos_end[off + at];
+                c[k] = x < npos ? a.pos_coord[off + at] : -1;
+            }
+#pragma unroll
+            for (int k = 0; k < kB; ++k) {
+                const bool on = (lo[k] <= pu && pu < hi[k]) || (lo[k] <= pv && pv < hi[k]);
+                if (on && c[k] >= 0)
+                    atomicOr(reinterpret_cast<unsigned long long *>(&Cl[c[k] >> 6]), 1ull << (c[k] & 63));
+            }
+        }
+    } else if (tid == 0) {
+        atomicOr(a.status, 1u);
+    }
+    __syncthreads();
+    const int32_t dlim = a.e > 0 ? ((a.e - 1) >> 6) + 1 : 0;
+    for (int i = tid; i < a.W; i += blockDim.x) {
+        const uint64_t v = Cl[i];
+        a.C[i] = v;
+        if (a.C_store) a.C_store[i] = v;
+        if (i < dlim && v != 0) {
+            const uint32_t slot = atomicAdd(&cnt, 1u);
+            Dl[slot] = static_cast<uint32_t>(i);
+            a.D[slot] = static_cast<uint32_t>(i);
+        }
+    }
+    __syncthreads();
+    const uint32_t nd = cnt;
+    const int64_t j = static_cast<int64_t>(a.e) + 1;
+    const bool have_col = j < a.N;
+    // column e+1 after the pending XOR of S_{e-1} (its flag was set by this phase's prologue)
+    const uint32_t f_old = (a.e >= 1 && have_col) ? a.flag[j] : 0u;
+    uint32_t part = 0, cp = 0;
+    for (uint32_t q = tid; q < nd; q += blockDim.x) {
+        const uint32_t w = Dl[q];
+        if (have_col) {
+            uint64_t col = a.Sw[static_cast<int64_t>(w) * a.npad + j];
+            if (f_old) col ^= a.S_m1[w];
+            part ^= static_cast<uint32_t>(__popcll(Cl[w] & col) & 1);
+        }
+        if (a.e >= 1) cp ^= static_cast<uint32_t>(__popcll(Cl[w] & a.S_m1[w]) & 1);
+    }
+    if (part) atomicXor(&par, 1u);
+    if (cp) atomicXor(&cpar, 1u);
+    __syncthreads();
+    if (tid == 0) {
+        *a.Dcount = nd;
+        *a.c_out = cpar;
+        cnt = 0;
+    }
+    if (!have_col) return;
+    const uint32_t p = par ^ static_cast<uint32_t>((Cl[j >> 6] >> (j & 63)) & 1ull);
+    const int32_t wl = static_cast<int32_t>(j >> 6) + 1;
+    const int32_t we = a.e >> 6;
+    const int32_t wm1 = a.e >= 1 ? ((a.e - 1) >> 6) : -1;
+    __syncthreads();
+    for (int32_t w = tid; w < wl; w += blockDim.x) {
+        uint64_t *cell = a.Sw + static_cast<int64_t>(w) * a.npad + j;
+        const uint64_t old = *cell;
+        uint64_t v = old;
+        if (f_old && w <= wm1) v ^= a.S_m1[w];
+        if (p && w <= we) v ^= a.S[w];
+        if (v != old) *cell = v;
+        if (w == static_cast<int32_t>(j >> 6)) v |= 1ull << (j & 63);
+        a.S_next[w] = v;
+        if (v) a.E_next[atomicAdd(&cnt, 1u)] = static_cast<uint32_t>(w);
+    }
+    __syncthreads();
+    if (tid == 0) *a.Ecount_next = cnt;
+}
+
+// One phase e: the column updates of (c) (dot products of phase e-1, XOR of phase e-2) on columns
+// > e, then (a)+(b), then the epilogue in the last block.
+// (a)+(b): block b takes big roots b, b + grid, ... (all waves of the block on one tree), then wave
+// w of the grid takes small roots w, w + waves, ...; both lists are sorted by size, largest first, so
+// the static round-robin spreads the large trees evenly (a shared work counter costs more than it
+// balances).  A root whose lowest candidate rank is not below the best rank found so far is skipped.
+// SEARCH = false computes the parity of every position of every root into parity_out instead.
 template <bool S_IN_LDS, bool SEARCH>
-__global__ __launch_bounds__(kBlockWaves * kWave) void k_parity_search(SearchArgs a) {
+__global__ __launch_bounds__(kBlockWaves * kWave) void k_phase(PhaseArgs a) {
     extern __shared__ uint64_t lds[];
     __shared__ uint32_t skip;
+    __shared__ uint32_t last;
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
     const int nw = blockDim.x / kWave;
+    if (SEARCH && a.upd_tiles > 0) {
+        // 64-column tiles of the update, handed out from the last wave of the grid backwards
+        const int waves = static_cast<int>(gridDim.x) * nw;
+        const int gw = static_cast<int>(blockIdx.x) * nw + wid;
+        for (int t = waves - 1 - gw; t < a.upd_tiles; t += waves) {
+            const int64_t j = static_cast<int64_t>(a.e) + 1 + static_cast<int64_t>(t) * kWave + lane;
+            if (j < a.N) column_update(a, j);
+        }
+    }
     const uint64_t *Sx = a.S;
     uint64_t *Pb = lds + (S_IN_LDS ? a.W : 0);
     uint32_t *ready = reinterpret_cast<uint32_t *>(Pb + a.p_words_big);
@@ -382,135 +576,17 @@ __global__ __launch_bounds__(kBlockWaves * kWave) void k_parity_search(SearchArg
         a.wave_stats[2 * gw] += n_pos;
         a.wave_stats[2 * gw + 1] += n_cand;
     }
-}
-
-struct CycleArgs {
-    const uint32_t *best;  // &best[e]
-    const uint32_t *cand_root, *cand_pu, *cand_pv;
-    const int32_t *cand_coord;
-    const int64_t *tree_off;
-    const uint32_t *parent_pos;
-    const int32_t *pos_coord;
-    uint64_t *C;        // W words
-    uint32_t *D;        // nonzero words of C below e
-    uint32_t *Dcount;
-    uint64_t *C_store;  // optional row e of the stored cycle vectors
-    uint32_t *status;
-    int32_t e, W, parent_cap;
-};
-
-constexpr int kPathCap = 512;
-
-// Cycle vector of the chosen candidate: coordinates of the edges on root -> u, root -> v and the
-// candidate's own edge (the two paths are disjoint below the root, so setting bits equals XOR).
-__global__ __launch_bounds__(256) void k_cycle(CycleArgs a) {
-    extern __shared__ uint64_t lds[];
-    uint64_t *Cl = lds;                                         // W words
-    uint32_t *par_l = reinterpret_cast<uint32_t *>(lds + a.W);  // parent_cap entries
-    __shared__ uint32_t path[2 * kPathCap];
-    __shared__ uint32_t cur[2];
-    __shared__ int32_t cnt[2];
-    __shared__ int32_t more;
-    __shared__ uint32_t dcount;
-    const int tid = threadIdx.x;
-    const uint32_t rank = *a.best;
-    for (int i = tid; i < a.W; i += blockDim.x) Cl[i] = 0;
-    if (tid == 0) dcount = 0;
-    if (rank == kNoRank) {
-        __syncthreads();
-        for (int i = tid; i < a.W; i += blockDim.x) {
-            a.C[i] = 0;
-            if (a.C_store) a.C_store[i] = 0;
-        }
-        if (tid == 0) {
-            *a.Dcount = 0;
-            atomicOr(a.status, 1u);
-        }
-        return;
-    }
-    const uint32_t r = a.cand_root[rank];
-    const int64_t off = a.tree_off[r];
-    const uint32_t npos = static_cast<uint32_t>(a.tree_off[r + 1] - off);
-    const uint32_t *pp = a.parent_pos + off;
-    const int32_t *pc = a.pos_coord + off;
-    const bool cached = npos <= static_cast<uint32_t>(a.parent_cap);
-    if (cached)
-        for (uint32_t i = tid; i < npos; i += blockDim.x) par_l[i] = pp[i];
-    if (tid < 2) cur[tid] = tid == 0 ? a.cand_pu[rank] : a.cand_pv[rank];
+    if (!SEARCH) return;
+    // The last block to finish sees every other block's result for this phase.
     __syncthreads();
-    if (tid == 0) {
-        const int32_t c = a.cand_coord[rank];
-        if (c >= 0) atomicOr(reinterpret_cast<unsigned long long *>(&Cl[c >> 6]), 1ull << (c & 63));
-    }
-    for (;;) {
-        if (tid < 2) {
-            uint32_t pos = cur[tid];
-            int32_t n = 0;
-            while (pos != 0 && n < kPathCap) {
-                path[tid * kPathCap + n++] = pos;
-                pos = cached ? par_l[pos] : pp[pos];
-            }
-            cur[tid] = pos;
-            cnt[tid] = n;
-        }
-        __syncthreads();
-        const int32_t n0 = cnt[0], n1 = cnt[1];
-        for (int i = tid; i < n0 + n1; i += blockDim.x) {
-            const uint32_t pos = i < n0 ? path[i] : path[kPathCap + (i - n0)];
-            const int32_t c = pc[pos];
-            if (c >= 0) atomicOr(reinterpret_cast<unsigned long long *>(&Cl[c >> 6]), 1ull << (c & 63));
-        }
-        if (tid == 0) more = (cur[0] != 0 || cur[1] != 0);
-        __syncthreads();
-        if (!more) break;
-    }
-    const int32_t dlim = a.e > 0 ? ((a.e - 1) >> 6) + 1 : 0;
-    for (int i = tid; i < a.W; i += blockDim.x) {
-        const uint64_t v = Cl[i];
-        a.C[i] = v;
-        if (a.C_store) a.C_store[i] = v;
-        if (i < dlim && v != 0) a.D[atomicAdd(&dcount, 1u)] = static_cast<uint32_t>(i);
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(a.done, 1u) == gridDim.x - 1;
     }
     __syncthreads();
-    if (tid == 0) *a.Dcount = dcount;
-}
-
-// p_j = <C_e, S_j> over words below e (S_j has no other bits there) plus C_e's bit j (S_j's own).
-__global__ __launch_bounds__(256) void k_dot(const uint64_t *__restrict__ C, const uint32_t *__restrict__ D,
-                                             const uint32_t *__restrict__ Dcount, const uint64_t *__restrict__ Sw,
-                                             int64_t npad, int32_t e, int32_t N, uint8_t *__restrict__ flag) {
-    const int64_t j = static_cast<int64_t>(e) + 1 + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j >= N) return;
-    const uint32_t nd = *Dcount;
-    uint32_t p = static_cast<uint32_t>((C[j >> 6] >> (j & 63)) & 1ull);
-    for (uint32_t t = 0; t < nd; ++t) {
-        const uint32_t w = D[t];
-        p ^= static_cast<uint32_t>(__popcll(C[w] & Sw[static_cast<int64_t>(w) * npad + j]) & 1);
-    }
-    flag[j] = static_cast<uint8_t>(p);
-}
-
-// S_j ^= S_e for flagged j > e, word w = blockIdx.y.  The column j = e+1 (final after this phase)
-// is also written contiguously, with its own bit, as S_{e+1} for the next phase.
-__global__ __launch_bounds__(256) void k_xor(uint64_t *__restrict__ Sw, int64_t npad, const uint64_t *__restrict__ Se,
-                                             uint64_t *__restrict__ Snext, const uint8_t *__restrict__ flag,
-                                             int32_t e, int32_t N, int64_t jbase) {
-    const int64_t j = jbase + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int32_t w = blockIdx.y;
-    if (j <= e || j >= N) return;
-    const uint64_t se = Se[w];
-    uint64_t *ptr = Sw + static_cast<int64_t>(w) * npad + j;
-    if (j == static_cast<int64_t>(e) + 1) {
-        uint64_t v = *ptr;
-        if (flag[j] && se) {
-            v ^= se;
-            *ptr = v;
-        }
-        if (w == ((e + 1) >> 6)) v |= 1ull << ((e + 1) & 63);
-        Snext[w] = v;
-    } else if (se && flag[j]) {
-        *ptr ^= se;
-    }
+    if (!last) return;
+    __threadfence();
+    phase_epilogue(a, lds);
 }
 
 // Reference-compatible parity of one chunk: every position of every valid root slot, 0/1 as
@@ -861,7 +937,7 @@ struct mcb_solver {
     int32_t phase = 0;
     // trees and candidates
     int64_t *tree_off = nullptr;
-    uint32_t *parent_pos = nullptr;
+    uint32_t *parent_pos = nullptr, *pos_pre = nullptr, *pos_end = nullptr;
     int32_t *pos_coord = nullptr;
     uint32_t *big_list = nullptr, *small_list = nullptr, *root_min_rank = nullptr;
     int64_t *rc_off = nullptr;
@@ -869,14 +945,15 @@ struct mcb_solver {
     int32_t *rc_coord = nullptr;
     uint32_t *cand_root = nullptr, *cand_pu = nullptr, *cand_pv = nullptr;
     int32_t *cand_coord = nullptr;
-    // phase state
+    // phase state: S_e contiguous in a ring of three, C_e and its word list in a ring of two
     uint64_t *Sw = nullptr, *Scur = nullptr, *C = nullptr, *C_store = nullptr;
-    uint32_t *D = nullptr, *Dcount = nullptr, *best = nullptr, *status = nullptr;
+    uint32_t *D = nullptr, *Dcount = nullptr, *best = nullptr, *done = nullptr, *status = nullptr;
+    uint32_t *E = nullptr, *Ecount = nullptr;  // nonzero words of S_e, ring of four
+    uint32_t *cdot = nullptr;                 // <C_e, S_{e-1}>, ring of two
     uint8_t *flag = nullptr;
-    uint64_t *wave_stats = nullptr;  // 2 counters per wave of the parity/search kernel
-    // launch shapes
-    int wpb = 4, blocks = 1, lds_bytes = 0, parent_cap = 0, cycle_lds = 0;
-    bool s_in_lds = true;
+    uint64_t *wave_stats = nullptr;  // 2 counters per wave of the phase kernel
+    // launch shape
+    int wpb = kBlockWaves, blocks = 1, lds_bytes = 0;
     hipStream_t stream = nullptr;
     int64_t bytes = 0;
     // timing
@@ -919,11 +996,13 @@ int timed_end(mcb_solver *s) {
     return MCB_OK;
 }
 
-SearchArgs search_args(const mcb_solver *s, int32_t e) {
-    SearchArgs a{};
+PhaseArgs phase_args(const mcb_solver *s, int32_t e) {
+    PhaseArgs a{};
     a.tree_off = s->tree_off;
     a.parent_pos = s->parent_pos;
     a.pos_coord = s->pos_coord;
+    a.pos_pre = s->pos_pre;
+    a.pos_end = s->pos_end;
     a.big_list = s->big_list;
     a.small_list = s->small_list;
     a.root_min_rank = s->root_min_rank;
@@ -932,70 +1011,83 @@ SearchArgs search_args(const mcb_solver *s, int32_t e) {
     a.rc_pv = s->rc_pv;
     a.rc_rank = s->rc_rank;
     a.rc_coord = s->rc_coord;
-    a.S = s->Scur + static_cast<int64_t>(e & 1) * s->W;
+    a.cand_root = s->cand_root;
+    a.cand_pu = s->cand_pu;
+    a.cand_pv = s->cand_pv;
+    a.cand_coord = s->cand_coord;
+    a.Sw = s->Sw;
+    a.npad = s->npad;
+    const int64_t W = s->W;
+    a.S = s->Scur + (e % 4) * W;
+    a.S_m1 = s->Scur + ((e + 3) % 4) * W;
+    a.S_m2 = s->Scur + ((e + 2) % 4) * W;
+    a.S_next = s->Scur + ((e + 1) % 4) * W;
+    a.E_m2 = s->E + ((e + 2) % 4) * W;
+    a.Ecount_m2 = s->Ecount + (e + 2) % 4;
+    a.E_next = s->E + ((e + 1) % 4) * W;
+    a.Ecount_next = s->Ecount + (e + 1) % 4;
+    a.C_prev = s->C + ((e + 1) & 1) * W;
+    a.D_prev = s->D + ((e + 1) & 1) * W;
+    a.Dcount_prev = s->Dcount + ((e + 1) & 1);
+    a.c_prev = s->cdot + ((e + 1) & 1);
+    a.C = s->C + (e & 1) * W;
+    a.D = s->D + (e & 1) * W;
+    a.Dcount = s->Dcount + (e & 1);
+    a.c_out = s->cdot + (e & 1);
+    a.C_store = s->C_store ? s->C_store + static_cast<int64_t>(e) * W : nullptr;
+    a.flag = s->flag;
     a.best = s->best + e;
+    a.done = s->done + e;
+    a.status = s->status;
     a.parity_out = nullptr;
     a.wave_stats = s->wave_stats;
+    a.e = e;
+    a.N = s->N;
+    a.W = s->W;
     a.n_big = s->n_big;
     a.n_small = s->n_small;
-    a.W = s->W;
     a.p_words_big = s->p_words_big;
     a.p_words_small = s->p_words_small;
+    if (e >= 1 && e + 1 < s->N) a.upd_tiles = (s->N - (e + 1) + kWave - 1) / kWave;
     return a;
 }
 
 int launch_phase(mcb_solver *s, int32_t e) {
-    hipStream_t st = s->stream;
-    int rc;
-    // (a)+(b)
-    if ((rc = timed_begin(s, MCB_K_PARITY_SEARCH))) return rc;
-    SearchArgs a = search_args(s, e);
-    if (s->s_in_lds)
-        hipLaunchKernelGGL((k_parity_search<true, true>), dim3(s->blocks), dim3(s->wpb * kWave), s->lds_bytes, st, a);
-    else
-        hipLaunchKernelGGL((k_parity_search<false, true>), dim3(s->blocks), dim3(s->wpb * kWave), s->lds_bytes, st, a);
+    int rc = timed_begin(s, MCB_K_PARITY_SEARCH);
+    if (rc) return rc;
+    const PhaseArgs a = phase_args(s, e);
+    hipLaunchKernelGGL((k_phase<true, true>), dim3(s->blocks), dim3(s->wpb * kWave), s->lds_bytes, s->stream, a);
     HIP_TRY(hipGetLastError());
-    if ((rc = timed_end(s))) return rc;
-    // cycle vector
-    if ((rc = timed_begin(s, MCB_K_CYCLE))) return rc;
-    CycleArgs c{};
-    c.best = s->best + e;
-    c.cand_root = s->cand_root;
-    c.cand_pu = s->cand_pu;
-    c.cand_pv = s->cand_pv;
-    c.cand_coord = s->cand_coord;
-    c.tree_off = s->tree_off;
-    c.parent_pos = s->parent_pos;
-    c.pos_coord = s->pos_coord;
-    c.C = s->C;
-    c.D = s->D;
-    c.Dcount = s->Dcount;
-    c.C_store = s->C_store ? s->C_store + static_cast<int64_t>(e) * s->W : nullptr;
-    c.status = s->status;
-    c.e = e;
-    c.W = s->W;
-    c.parent_cap = s->parent_cap;
-    hipLaunchKernelGGL(k_cycle, dim3(1), dim3(256), s->cycle_lds, st, c);
-    HIP_TRY(hipGetLastError());
-    if ((rc = timed_end(s))) return rc;
-    if (e + 1 >= s->N) return MCB_OK;
-    // (c) update of S_j, j > e
-    const int64_t rows = s->N - (e + 1);
-    if ((rc = timed_begin(s, MCB_K_DOT))) return rc;
-    hipLaunchKernelGGL(k_dot, dim3(static_cast<unsigned>((rows + 255) / 256)), dim3(256), 0, st, s->C, s->D, s->Dcount,
-                       s->Sw, s->npad, e, s->N, s->flag);
-    HIP_TRY(hipGetLastError());
-    if ((rc = timed_end(s))) return rc;
-    if ((rc = timed_begin(s, MCB_K_XOR))) return rc;
-    const int64_t jbase = (static_cast<int64_t>(e) + 1) & ~static_cast<int64_t>(63);
-    const int64_t span = s->N - jbase;
-    const unsigned wy = static_cast<unsigned>(((e + 1) >> 6) + 1);
-    hipLaunchKernelGGL(k_xor, dim3(static_cast<unsigned>((span + 255) / 256), wy), dim3(256), 0, st, s->Sw, s->npad,
-                       s->Scur + static_cast<int64_t>(e & 1) * s->W, s->Scur + static_cast<int64_t>((e + 1) & 1) * s->W,
-                       s->flag, e, s->N, jbase);
-    HIP_TRY(hipGetLastError());
-    if ((rc = timed_end(s))) return rc;
-    return MCB_OK;
+    return timed_end(s);
+}
+
+// DFS pre-order intervals of every tree layout: pre[x] is x's index in a pre-order walk of its
+// root's tree (children in position order) and end[x] = pre[x] + subtree size, so that x is an
+// ancestor-or-self of y exactly when pre[x] <= pre[y] < end[x].
+void preorder_intervals(const int64_t *tree_offset, const uint32_t *parent_pos, int32_t F, std::vector<uint32_t> &pre,
+                        std::vector<uint32_t> &end) {
+    const int64_t P = tree_offset[F];
+    pre.assign(P, 0);
+    end.assign(P, 0);
+    std::vector<uint32_t> size, next;
+    for (int32_t r = 0; r < F; ++r) {
+        const int64_t a = tree_offset[r];
+        const uint32_t n = static_cast<uint32_t>(tree_offset[r + 1] - a);
+        const uint32_t *pp = parent_pos + a;
+        size.assign(n, 1);
+        for (uint32_t x = n; x-- > 1;) size[pp[x]] += size[x];
+        next.assign(n, 0);
+        uint32_t *pr = pre.data() + a, *en = end.data() + a;
+        pr[0] = 0;
+        next[0] = 1;
+        for (uint32_t x = 1; x < n; ++x) {  // parents precede children
+            const uint32_t p = pp[x];
+            pr[x] = next[p];
+            next[p] += size[x];
+            next[x] = pr[x] + 1;
+        }
+        for (uint32_t x = 0; x < n; ++x) en[x] = pr[x] + size[x];
+    }
 }
 
 }  // namespace
@@ -1009,10 +1101,10 @@ void mcb_solver_options_init(mcb_solver_options *opt) {
 
 void mcb_solver_destroy(mcb_solver *s) {
     if (!s) return;
-    void *ptrs[] = {s->tree_off, s->parent_pos, s->pos_coord, s->big_list, s->small_list, s->root_min_rank,
-                    s->rc_off, s->rc_pu,
-                    s->rc_pv, s->rc_rank, s->rc_coord, s->cand_root, s->cand_pu, s->cand_pv, s->cand_coord, s->Sw,
-                    s->Scur, s->C, s->C_store, s->D, s->Dcount, s->best, s->status, s->flag, s->wave_stats};
+    void *ptrs[] = {s->tree_off, s->parent_pos, s->pos_pre, s->pos_end, s->pos_coord, s->big_list, s->small_list,
+                    s->root_min_rank, s->rc_off, s->rc_pu, s->rc_pv, s->rc_rank, s->rc_coord, s->cand_root,
+                    s->cand_pu, s->cand_pv, s->cand_coord, s->Sw, s->Scur, s->C, s->C_store, s->D, s->Dcount,
+                    s->best, s->done, s->status, s->wave_stats, s->flag, s->E, s->Ecount, s->cdot};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
@@ -1099,11 +1191,15 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
     s->n_small = static_cast<int32_t>(small.size());
     s->p_words_big = big.empty() ? 0 : static_cast<int32_t>((s->max_pos + 63) / 64);
     s->p_words_small = small.empty() ? 0 : static_cast<int32_t>((max_small + 63) / 64);
+    std::vector<uint32_t> pre, end;
+    preorder_intervals(d->tree_offset, d->parent_pos, F, pre, end);
 
     int64_t &B = s->bytes;
     if ((rc = upload(&s->tree_off, d->tree_offset, F + 1, B))) return cleanup(rc);
     if ((rc = upload(&s->parent_pos, d->parent_pos, s->Ptot, B))) return cleanup(rc);
     if ((rc = upload(&s->pos_coord, d->pos_coord, s->Ptot, B))) return cleanup(rc);
+    if ((rc = upload(&s->pos_pre, pre.data(), s->Ptot, B))) return cleanup(rc);
+    if ((rc = upload(&s->pos_end, end.data(), s->Ptot, B))) return cleanup(rc);
     if ((rc = upload(&s->big_list, big.data(), big.size(), B))) return cleanup(rc);
     if ((rc = upload(&s->small_list, small.data(), small.size(), B))) return cleanup(rc);
     if ((rc = upload(&s->root_min_rank, min_rank.data(), F, B))) return cleanup(rc);
@@ -1117,44 +1213,39 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
     if ((rc = upload(&s->cand_pv, d->cand_pos_v, K, B))) return cleanup(rc);
     if ((rc = upload(&s->cand_coord, d->cand_coord, K, B))) return cleanup(rc);
     if ((rc = dev_alloc(&s->Sw, static_cast<size_t>(s->W) * s->npad, B))) return cleanup(rc);
-    if ((rc = dev_alloc(&s->Scur, 2 * static_cast<size_t>(s->W), B))) return cleanup(rc);
-    if ((rc = dev_alloc(&s->C, s->W, B))) return cleanup(rc);
+    if ((rc = dev_alloc(&s->Scur, 4 * static_cast<size_t>(s->W), B))) return cleanup(rc);
+    if ((rc = dev_alloc(&s->C, 2 * static_cast<size_t>(s->W), B))) return cleanup(rc);
     if (s->opt.store_cycles && (rc = dev_alloc(&s->C_store, static_cast<size_t>(s->N) * s->W, B))) return cleanup(rc);
-    if ((rc = dev_alloc(&s->D, s->W, B))) return cleanup(rc);
-    if ((rc = dev_alloc(&s->Dcount, 1, B))) return cleanup(rc);
+    if ((rc = dev_alloc(&s->D, 2 * static_cast<size_t>(s->W), B))) return cleanup(rc);
+    if ((rc = dev_alloc(&s->Dcount, 2, B))) return cleanup(rc);
     if ((rc = dev_alloc(&s->best, s->N, B))) return cleanup(rc);
-    if ((rc = dev_alloc(&s->status, 1, B))) return cleanup(rc);
+    if ((rc = dev_alloc(&s->done, s->N, B))) return cleanup(rc);
     if ((rc = dev_alloc(&s->flag, s->npad, B))) return cleanup(rc);
+    if ((rc = dev_alloc(&s->E, 4 * static_cast<size_t>(s->W), B))) return cleanup(rc);
+    if ((rc = dev_alloc(&s->Ecount, 4, B))) return cleanup(rc);
+    if ((rc = dev_alloc(&s->cdot, 2, B))) return cleanup(rc);
+    if ((rc = dev_alloc(&s->status, 1, B))) return cleanup(rc);
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(MCB_E_DEVICE, "hipStreamCreate"));
 
-    // Launch shape of the parity/search kernel: S_e, one bit per position of the block's big tree
-    // plus its ready stamps, and one bit per position of each wave's small tree, all in LDS.
-    const int64_t tree_bytes = 8ll * s->p_words_big + 4ll * (s->p_words_big + 1) / 2 * 2 +
+    // LDS of the phase kernel: S_e, one bit per position of the block's big tree plus its ready
+    // stamps, and one bit per position of each wave's small tree; the epilogue reuses it for C_e and
+    // its word list (12 bytes per word of C_e).
+    const int64_t tree_bytes = 8ll * s->p_words_big + 8ll * ((s->p_words_big + 1) / 2) +
                                8ll * kBlockWaves * s->p_words_small;
-    if (tree_bytes > kLdsLimit) return cleanup(fail(MCB_E_INVALID, "a tree layout exceeds the LDS parity buffer"));
-    s->wpb = kBlockWaves;
-    s->s_in_lds = static_cast<int64_t>(s->W) * 8 + tree_bytes <= kLdsLimit;
-    s->lds_bytes = static_cast<int>((s->s_in_lds ? static_cast<int64_t>(s->W) * 8 : 0) + tree_bytes);
+    const int64_t need = std::max<int64_t>(8ll * s->W + tree_bytes, 12ll * s->W);
+    if (need > kLdsLimit)
+        return cleanup(fail(MCB_E_INVALID, "support vector and tree layout exceed the LDS budget"));
+    s->lds_bytes = static_cast<int>(need);
     int per_cu = 0;
-    if (s->s_in_lds)
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_parity_search<true, true>, kBlockWaves * kWave,
-                                                             s->lds_bytes));
-    else
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_parity_search<false, true>,
-                                                             kBlockWaves * kWave, s->lds_bytes));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_phase<true, true>, kBlockWaves * kWave,
+                                                         s->lds_bytes));
     per_cu = std::max(1, per_cu);
     int64_t blocks = static_cast<int64_t>(per_cu) * g_dev.prop.multiProcessorCount;
     if (s->opt.max_blocks > 0) blocks = std::min<int64_t>(blocks, s->opt.max_blocks);
     blocks = std::min<int64_t>(blocks, s->n_big + (s->n_small + kBlockWaves - 1) / kBlockWaves);
     s->blocks = static_cast<int>(std::max<int64_t>(1, blocks));
     if ((rc = dev_alloc(&s->wave_stats, 2 * static_cast<size_t>(s->blocks) * s->wpb, B))) return cleanup(rc);
-    // Cycle kernel LDS: C_e plus the chosen root's parent array when it fits.
-    const int cycle_static = 8 * 1024;  // path buffers and counters of k_cycle
-    if (static_cast<int64_t>(s->W) * 8 > kLdsLimit - cycle_static - 4096)
-        return cleanup(fail(MCB_E_INVALID, "cycle-space dimension too large for the LDS cycle buffer"));
-    s->parent_cap = (kLdsLimit - cycle_static - s->W * 8) / 4;
-    s->cycle_lds = s->W * 8 + std::min(s->parent_cap, s->max_pos) * 4;
     if (s->opt.timing) {
         s->ev_pool.resize(4096);
         s->ev_kind.resize(2048);
@@ -1171,14 +1262,22 @@ int mcb_solver_reset(mcb_solver *s) {
     if (!s) return fail(MCB_E_INVALID, "invalid arguments");
     HIP_TRY(hipStreamSynchronize(s->stream));
     HIP_TRY(hipMemsetAsync(s->Sw, 0, sizeof(uint64_t) * static_cast<size_t>(s->W) * s->npad, s->stream));
-    std::vector<uint64_t> s0(2 * static_cast<size_t>(s->W), 0);
-    s0[0] = 1;  // S_0 = e_0
+    std::vector<uint64_t> s0(4 * static_cast<size_t>(s->W), 0);
+    s0[0] = 1;  // S_0 = e_0 in slot 0 of the ring
     HIP_TRY(hipMemcpyAsync(s->Scur, s0.data(), sizeof(uint64_t) * s0.size(), hipMemcpyHostToDevice, s->stream));
     hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<unsigned>((s->N + 255) / 256)), dim3(256), 0, s->stream, s->best,
                        kNoRank, static_cast<int64_t>(s->N));
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemsetAsync(s->status, 0, sizeof(uint32_t), s->stream));
+    HIP_TRY(hipMemsetAsync(s->done, 0, sizeof(uint32_t) * static_cast<size_t>(s->N), s->stream));
     HIP_TRY(hipMemsetAsync(s->flag, 0, static_cast<size_t>(s->npad), s->stream));
+    HIP_TRY(hipMemsetAsync(s->cdot, 0, 2 * sizeof(uint32_t), s->stream));
+    // S_0 = e_0: one nonzero word, word 0, in slot 0 of the ring
+    HIP_TRY(hipMemsetAsync(s->E, 0, 4 * sizeof(uint32_t) * s->W, s->stream));
+    const uint32_t ecount0[4] = {1, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(s->Ecount, ecount0, sizeof(ecount0), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemsetAsync(s->Dcount, 0, 2 * sizeof(uint32_t), s->stream));
+    HIP_TRY(hipMemsetAsync(s->C, 0, 2 * sizeof(uint64_t) * s->W, s->stream));
+    HIP_TRY(hipMemsetAsync(s->status, 0, sizeof(uint32_t), s->stream));
     HIP_TRY(hipMemsetAsync(s->wave_stats, 0, sizeof(uint64_t) * 2 * s->blocks * s->wpb, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->phase = 0;
@@ -1254,17 +1353,12 @@ int mcb_solver_parity(mcb_solver *s, const uint64_t *S, uint8_t *out) {
         (void)hipFree(dS);
         return rc;
     }
-    SearchArgs a = search_args(s, 0);
+    PhaseArgs a = phase_args(s, 0);
     a.S = dS;
     a.parity_out = dP;
-    a.best = nullptr;
     a.wave_stats = nullptr;
-    if (s->s_in_lds)
-        hipLaunchKernelGGL((k_parity_search<true, false>), dim3(s->blocks), dim3(s->wpb * kWave), s->lds_bytes,
-                           s->stream, a);
-    else
-        hipLaunchKernelGGL((k_parity_search<false, false>), dim3(s->blocks), dim3(s->wpb * kWave), s->lds_bytes,
-                           s->stream, a);
+    a.upd_tiles = 0;
+    hipLaunchKernelGGL((k_phase<true, false>), dim3(s->blocks), dim3(s->wpb * kWave), s->lds_bytes, s->stream, a);
     hipError_t e1 = hipGetLastError();
     hipError_t e2 = hipStreamSynchronize(s->stream);
     hipError_t e3 = hipMemcpy(out, dP, static_cast<size_t>(s->Ptot), hipMemcpyDeviceToHost);
@@ -1306,7 +1400,7 @@ int mcb_solver_launch_shape(const mcb_solver *s, int32_t *blocks, int32_t *waves
     if (blocks) *blocks = s->blocks;
     if (waves_per_block) *waves_per_block = s->wpb;
     if (lds_bytes) *lds_bytes = s->lds_bytes;
-    if (s_in_lds) *s_in_lds = s->s_in_lds ? 1 : 0;
+    if (s_in_lds) *s_in_lds = 1;
     return MCB_OK;
 }
 
