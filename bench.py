@@ -39,10 +39,15 @@ WORKLOADS = {
 
 
 def host_cores() -> int:
+    """Host threads for the CPU legs: the affinity mask, capped by OMP_NUM_THREADS when set."""
     try:
-        return len(os.sched_getaffinity(0))
+        n = len(os.sched_getaffinity(0))
     except AttributeError:
-        return os.cpu_count() or 1
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
 
 
 def cpu_baseline(n, u, v, w, budget_s: float, threads: int) -> dict:
@@ -54,16 +59,31 @@ def cpu_baseline(n, u, v, w, budget_s: float, threads: int) -> dict:
 
     p = Problem.from_edges(n, u, v, w, keep_vertex_trees=True, num_threads=threads)
     N = p.info.num_non_tree
-    t0 = time.perf_counter()
-    pyoracle.run(p, phases=1, num_threads=threads)
-    t1 = time.perf_counter() - t0
-    X = int(max(1, min(N, budget_s / max(t1, 1e-6))))
-    t0 = time.perf_counter()
-    pyoracle.run(p, phases=X, num_threads=threads)
-    tx = time.perf_counter() - t0
+    X = 1
+    while True:  # each run restarts at phase 0; grow X until one run takes about budget_s
+        t0 = time.perf_counter()
+        pyoracle.run(p, phases=X, num_threads=threads)
+        tx = time.perf_counter() - t0
+        if tx >= 0.5 * budget_s or X >= N:
+            break
+        X = int(min(N, max(2 * X, X * budget_s / max(tx, 1e-6))))
     p.close()
     return {"value": tx * N / X, "unit": "s per full phase loop (extrapolated)", "cores": threads, "kind": "port",
             "sample": f"first {X} of {N} phases timed ({tx:.2f} s), scaled by N/{X}"}
+
+
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this kernel and
+    workload (counter values scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def main() -> int:
@@ -75,14 +95,24 @@ def main() -> int:
     ap.add_argument("--no-prune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--phases", type=int, default=0,
+                    help="profiling aid: run only the first PHASES phases per step (0 = the full loop)")
+    ap.add_argument("--copy-kernel", type=int, default=0, metavar="BYTES",
+                    help="calibration aid: run only the device copy kernel over BYTES bytes and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the phase loop of one graph runs on one device "
               "(multi-GPU sharding by root is a design note, not part of this build)", file=sys.stderr)
         return 2
 
-    from minimumcyclebasis_amd.device import Solver, device_info
+    from minimumcyclebasis_amd.device import Solver, copy_bandwidth, device_info
     from minimumcyclebasis_amd.host import Problem
+
+    if args.copy_kernel:
+        gbs = copy_bandwidth(args.copy_kernel, iters=max(1, args.steps))
+        print(json.dumps({"copy_kernel_bytes": args.copy_kernel, "iters": max(1, args.steps), "gbps": round(gbs, 1)}),
+              flush=True)
+        return 0
 
     desc, gen = WORKLOADS[args.workload]
     n, u, v, w = gen()
@@ -98,39 +128,43 @@ def main() -> int:
     solver = Solver(prob, timing=True)
     upload_s = time.perf_counter() - t0
     dev = device_info()
+    phases = args.phases if 0 < args.phases < info.num_non_tree else None
 
     for i in range(args.warmup):
         solver.reset()
-        solver.run()
+        solver.run(phases)
         solver.sync()
         print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
 
-    kt = {k: [0.0, 0] for k in ("parity_search", "cycle", "dot", "xor")}
-    pos = cand = 0
+    k_ms = 0.0
+    k_n = 0
+    pos = cand = upd = 0
     t_total = 0.0
     sel = None
     for i in range(args.steps):
         solver.reset()
         t0 = time.perf_counter()
-        solver.run()
+        solver.run(phases)
         solver.sync()
         dt = time.perf_counter() - t0
         t_total += dt
-        for k in kt:
-            ms, cnt = solver.kernel_time(k)
-            kt[k][0] += ms
-            kt[k][1] += cnt
-        p_, c_ = solver.work_counters()
+        ms, cnt = solver.kernel_time("parity_search")
+        k_ms += ms
+        k_n += cnt
+        p_, c_, u_ = solver.work_counters()
         pos += p_
         cand += c_
+        upd += u_
         sel = solver.selected()
         print(f"[bench] step {i}: {dt:.3f} s", file=sys.stderr, flush=True)
     ms_per_step = 1000.0 * t_total / args.steps
     total_weight = int(prob.array("cand_weight")[sel].astype(np.int64).sum())
 
-    ks_ms, ks_n = kt["parity_search"]
-    avg_ms = ks_ms / max(ks_n, 1)
-    bytes_per_launch = (8.0 * pos + 16.0 * cand) / max(ks_n, 1)
+    # Algorithmic bytes of one phase-kernel launch: 8 per tree position streamed (u32 parent position
+    # + i32 coordinate), 16 per candidate read (u_pos, v_pos, rank, coordinate), and the update's
+    # support-word reads and read-modify-writes as the kernel counts them.
+    avg_ms = k_ms / max(k_n, 1)
+    bytes_per_launch = (8.0 * pos + 16.0 * cand + upd) / max(k_n, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
 
     out = {
@@ -162,16 +196,20 @@ def main() -> int:
         "total_weight": total_weight,
         "host_preprocess_s": round(host_s, 3),
         "upload_s": round(upload_s, 3),
-        "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in kt.items()},
+        "phases_per_step": k_n // max(args.steps, 1),
+        "kernel_ms_per_step": round(k_ms / args.steps, 3),
         "roofline": {
-            "kernel": "k_parity_search",
+            "kernel": "k_phase",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic("k_phase", args.workload) if not args.no_prune else None,
             "bytes_per_launch": round(bytes_per_launch, 1),
+            "bytes_split": {"tree_positions": round(8.0 * pos / max(k_n, 1), 1),
+                            "candidates": round(16.0 * cand / max(k_n, 1), 1),
+                            "support_update": round(upd / max(k_n, 1), 1)},
             "avg_launch_us": round(avg_ms * 1000.0, 3),
         },
     }

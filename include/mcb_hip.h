@@ -165,9 +165,11 @@ int mcb_solver_kernel_time(mcb_solver *s, int kernel, double *total_ms, int64_t 
 /* Device bytes the solver holds. */
 int64_t mcb_solver_device_bytes(const mcb_solver *s);
 
-/* Work done by the parity/search kernel since the last reset: tree positions streamed and
- * candidates read (roots whose best candidate cannot beat the phase's best are skipped). */
-int mcb_solver_work_counters(mcb_solver *s, int64_t *positions, int64_t *candidates);
+/* Work done by the phase kernel since the last reset: tree positions streamed and candidates read
+ * by the parity/search (roots whose best candidate cannot beat the phase's best are skipped), and
+ * the algorithmic bytes of the support-vector update (8 per support word read for a dot product,
+ * 16 per word XORed, 2 per column flag). */
+int mcb_solver_work_counters(mcb_solver *s, int64_t *positions, int64_t *candidates, int64_t *update_bytes);
 
 /* Launch shape chosen for the parity/search kernel. */
 int mcb_solver_launch_shape(const mcb_solver *s, int32_t *blocks, int32_t *waves_per_block, int32_t *lds_bytes,

@@ -63,7 +63,7 @@ SIGNATURES = {
     "mcb_solver_parity": (ctypes.c_int, [P, P, P]),
     "mcb_solver_kernel_time": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]),
     "mcb_solver_device_bytes": (i64, [P]),
-    "mcb_solver_work_counters": (ctypes.c_int, [P, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
+    "mcb_solver_work_counters": (ctypes.c_int, [P, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i64)]),
     "mcb_solver_launch_shape": (ctypes.c_int, [P, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32),
                                                ctypes.POINTER(i32)]),
     "mcb_solver_destroy": (None, [P]),

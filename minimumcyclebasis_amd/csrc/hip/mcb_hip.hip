@@ -300,7 +300,7 @@ struct PhaseArgs {
     uint32_t *done;          // &done[e]: blocks finished
     uint32_t *status;
     uint8_t *parity_out;     // parity-only mode: one byte per position
-    uint64_t *wave_stats;    // optional: per wave, positions streamed and candidates read
+    uint64_t *wave_stats;    // optional: per wave, positions streamed, candidates read, update bytes
     int32_t e, N, W, n_big, n_small, p_words_big, p_words_small;
     int32_t upd_tiles;  // prologue tiles: 64 columns from e+1 each (0 in phase 0)
 };
@@ -342,7 +342,8 @@ __device__ __forceinline__ void scan_candidates(const PhaseArgs &a, const uint64
 // and then applies the pending XOR.  The dot product needs only C_{e-1}'s nonzero words below e-1
 // (S_j has no other bits there besides its own bit j, which meets C_{e-1}[j]); the XOR needs only
 // S_{e-2}'s nonzero words.  Loads are issued in batches before the stores.
-__device__ __forceinline__ void column_update(const PhaseArgs &a, int64_t j) {
+// Returns whether the column took the pending XOR.
+__device__ __forceinline__ uint32_t column_update(const PhaseArgs &a, int64_t j) {
     constexpr int kB = 8;
     const uint32_t f_old = a.e >= 2 ? a.flag[j] : 0u;
     uint32_t p = static_cast<uint32_t>((a.C_prev[j >> 6] >> (j & 63)) & 1ull) ^ (f_old & *a.c_prev);
@@ -353,7 +354,7 @@ __device__ __forceinline__ void column_update(const PhaseArgs &a, int64_t j) {
         p ^= static_cast<uint32_t>(__popcll(a.C_prev[w] & a.Sw[static_cast<int64_t>(w) * a.npad + j]) & 1);
     }
     a.flag[j] = static_cast<uint8_t>(p);
-    if (!f_old) return;
+    if (!f_old) return 0u;
     const uint32_t ne = *a.Ecount_m2;
     for (uint32_t q0 = 0; q0 < ne; q0 += kB) {
         uint32_t w[kB];
@@ -370,6 +371,7 @@ __device__ __forceinline__ void column_update(const PhaseArgs &a, int64_t j) {
         for (int k = 0; k < kB; ++k)
             if (se[k]) a.Sw[static_cast<int64_t>(w[k]) * a.npad + j] = v[k] ^ se[k];
     }
+    return 1u;
 }
 
 // Epilogue of phase e, run by the last block to finish:
@@ -499,13 +501,19 @@ __global__ __launch_bounds__(kBlockWaves * kWave) void k_phase(PhaseArgs a) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
     const int nw = blockDim.x / kWave;
+    uint64_t n_upd = 0;  // algorithmic bytes of the update: 8 per dot word, 16 per XORed word, 2 per flag
     if (SEARCH && a.upd_tiles > 0) {
         // 64-column tiles of the update, handed out from the last wave of the grid backwards
         const int waves = static_cast<int>(gridDim.x) * nw;
         const int gw = static_cast<int>(blockIdx.x) * nw + wid;
         for (int t = waves - 1 - gw; t < a.upd_tiles; t += waves) {
             const int64_t j = static_cast<int64_t>(a.e) + 1 + static_cast<int64_t>(t) * kWave + lane;
-            if (j < a.N) column_update(a, j);
+            const uint32_t x = j < a.N ? column_update(a, j) : 0u;
+            if (a.wave_stats) {
+                const uint64_t cols = __popcll(__ballot(j < a.N));
+                const uint64_t xors = __popcll(__ballot(x != 0));
+                n_upd += cols * (2ull + 8ull * *a.Dcount_prev) + (xors ? xors * 16ull * *a.Ecount_m2 : 0ull);
+            }
         }
     }
     const uint64_t *Sx = a.S;
@@ -573,8 +581,9 @@ __global__ __launch_bounds__(kBlockWaves * kWave) void k_phase(PhaseArgs a) {
     }
     if (a.wave_stats && lane == 0) {  // each wave owns its slot; launches are stream-ordered
         const size_t gw = static_cast<size_t>(blockIdx.x) * nw + wid;
-        a.wave_stats[2 * gw] += n_pos;
-        a.wave_stats[2 * gw + 1] += n_cand;
+        a.wave_stats[3 * gw] += n_pos;
+        a.wave_stats[3 * gw + 1] += n_cand;
+        a.wave_stats[3 * gw + 2] += n_upd;
     }
     if (!SEARCH) return;
     // The last block to finish sees every other block's result for this phase.
@@ -733,7 +742,11 @@ int mcb_hip_device_info(char *name, int name_len, int *compute_units, int64_t *t
     int rc = ensure_device();
     if (rc) return rc;
     if (name && name_len > 0) {
-        std::snprintf(name, static_cast<size_t>(name_len), "%s (%s)", g_dev.prop.name, g_dev.prop.gcnArchName);
+        if (g_dev.prop.name[0])
+            std::snprintf(name, static_cast<size_t>(name_len), "%s (%s)", g_dev.prop.name, g_dev.prop.gcnArchName);
+        else
+            std::snprintf(name, static_cast<size_t>(name_len), "%s, %d CUs", g_dev.prop.gcnArchName,
+                          g_dev.prop.multiProcessorCount);
     }
     if (compute_units) *compute_units = g_dev.prop.multiProcessorCount;
     if (total_mem) *total_mem = static_cast<int64_t>(g_dev.prop.totalGlobalMem);
@@ -951,7 +964,7 @@ struct mcb_solver {
     uint32_t *E = nullptr, *Ecount = nullptr;  // nonzero words of S_e, ring of four
     uint32_t *cdot = nullptr;                 // <C_e, S_{e-1}>, ring of two
     uint8_t *flag = nullptr;
-    uint64_t *wave_stats = nullptr;  // 2 counters per wave of the phase kernel
+    uint64_t *wave_stats = nullptr;  // 3 counters per wave of the phase kernel
     // launch shape
     int wpb = kBlockWaves, blocks = 1, lds_bytes = 0;
     hipStream_t stream = nullptr;
@@ -1245,7 +1258,7 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
     if (s->opt.max_blocks > 0) blocks = std::min<int64_t>(blocks, s->opt.max_blocks);
     blocks = std::min<int64_t>(blocks, s->n_big + (s->n_small + kBlockWaves - 1) / kBlockWaves);
     s->blocks = static_cast<int>(std::max<int64_t>(1, blocks));
-    if ((rc = dev_alloc(&s->wave_stats, 2 * static_cast<size_t>(s->blocks) * s->wpb, B))) return cleanup(rc);
+    if ((rc = dev_alloc(&s->wave_stats, 3 * static_cast<size_t>(s->blocks) * s->wpb, B))) return cleanup(rc);
     if (s->opt.timing) {
         s->ev_pool.resize(4096);
         s->ev_kind.resize(2048);
@@ -1278,7 +1291,7 @@ int mcb_solver_reset(mcb_solver *s) {
     HIP_TRY(hipMemsetAsync(s->Dcount, 0, 2 * sizeof(uint32_t), s->stream));
     HIP_TRY(hipMemsetAsync(s->C, 0, 2 * sizeof(uint64_t) * s->W, s->stream));
     HIP_TRY(hipMemsetAsync(s->status, 0, sizeof(uint32_t), s->stream));
-    HIP_TRY(hipMemsetAsync(s->wave_stats, 0, sizeof(uint64_t) * 2 * s->blocks * s->wpb, s->stream));
+    HIP_TRY(hipMemsetAsync(s->wave_stats, 0, sizeof(uint64_t) * 3 * s->blocks * s->wpb, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->phase = 0;
     s->ev_used = 0;
@@ -1379,18 +1392,20 @@ int mcb_solver_kernel_time(mcb_solver *s, int kernel, double *total_ms, int64_t 
 
 int64_t mcb_solver_device_bytes(const mcb_solver *s) { return s ? s->bytes : -1; }
 
-int mcb_solver_work_counters(mcb_solver *s, int64_t *positions, int64_t *candidates) {
+int mcb_solver_work_counters(mcb_solver *s, int64_t *positions, int64_t *candidates, int64_t *update_bytes) {
     if (!s) return fail(MCB_E_INVALID, "invalid arguments");
     HIP_TRY(hipStreamSynchronize(s->stream));
-    std::vector<uint64_t> st(2 * static_cast<size_t>(s->blocks) * s->wpb);
+    std::vector<uint64_t> st(3 * static_cast<size_t>(s->blocks) * s->wpb);
     HIP_TRY(hipMemcpy(st.data(), s->wave_stats, sizeof(uint64_t) * st.size(), hipMemcpyDeviceToHost));
-    uint64_t p = 0, c = 0;
-    for (size_t i = 0; i < st.size(); i += 2) {
+    uint64_t p = 0, c = 0, u = 0;
+    for (size_t i = 0; i < st.size(); i += 3) {
         p += st[i];
         c += st[i + 1];
+        u += st[i + 2];
     }
     if (positions) *positions = static_cast<int64_t>(p);
     if (candidates) *candidates = static_cast<int64_t>(c);
+    if (update_bytes) *update_bytes = static_cast<int64_t>(u);
     return MCB_OK;
 }
 

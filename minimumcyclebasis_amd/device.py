@@ -126,12 +126,13 @@ class Solver:
     def device_bytes(self) -> int:
         return int(_lib.hip().mcb_solver_device_bytes(self._h))
 
-    def work_counters(self) -> tuple[int, int]:
-        """(tree positions streamed, candidates read) by the parity/search kernel since reset."""
+    def work_counters(self) -> tuple[int, int, int]:
+        """(tree positions streamed, candidates read, support-update bytes) by the phase kernel since reset."""
         p = ctypes.c_int64()
         c = ctypes.c_int64()
-        _lib.check_hip(_lib.hip().mcb_solver_work_counters(self._h, ctypes.byref(p), ctypes.byref(c)))
-        return p.value, c.value
+        u = ctypes.c_int64()
+        _lib.check_hip(_lib.hip().mcb_solver_work_counters(self._h, ctypes.byref(p), ctypes.byref(c), ctypes.byref(u)))
+        return p.value, c.value, u.value
 
     def launch_shape(self) -> dict:
         v = [ctypes.c_int32() for _ in range(4)]
