@@ -93,6 +93,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="er20000", choices=sorted(WORKLOADS))
     ap.add_argument("--no-prune", action="store_true")
+    ap.add_argument("--block-phases", type=int, default=0,
+                    help="phases per pass over the trees (0 automatic, 1 one kernel per phase)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--phases", type=int, default=0,
@@ -125,7 +127,7 @@ def main() -> int:
           f"F={info.fvs_size} K={info.num_candidates} positions={info.positions} host={host_s:.2f}s",
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    solver = Solver(prob, timing=True)
+    solver = Solver(prob, timing=True, block_phases=args.block_phases)
     upload_s = time.perf_counter() - t0
     dev = device_info()
     phases = args.phases if 0 < args.phases < info.num_non_tree else None
@@ -136,8 +138,9 @@ def main() -> int:
         solver.sync()
         print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
 
-    k_ms = 0.0
-    k_n = 0
+    from minimumcyclebasis_amd.device import KERNELS
+
+    kt = {k: [0.0, 0] for k in KERNELS}
     pos = cand = upd = 0
     t_total = 0.0
     sel = None
@@ -148,9 +151,10 @@ def main() -> int:
         solver.sync()
         dt = time.perf_counter() - t0
         t_total += dt
-        ms, cnt = solver.kernel_time("parity_search")
-        k_ms += ms
-        k_n += cnt
+        for k in KERNELS:
+            ms, cnt = solver.kernel_time(k)
+            kt[k][0] += ms
+            kt[k][1] += cnt
         p_, c_, u_ = solver.work_counters()
         pos += p_
         cand += c_
@@ -160,11 +164,19 @@ def main() -> int:
     ms_per_step = 1000.0 * t_total / args.steps
     total_weight = int(prob.array("cand_weight")[sel].astype(np.int64).sum())
 
-    # Algorithmic bytes of one phase-kernel launch: 8 per tree position streamed (u32 parent position
-    # + i32 coordinate), 16 per candidate read (u_pos, v_pos, rank, coordinate), and the update's
-    # support-word reads and read-modify-writes as the kernel counts them.
+    # Algorithmic bytes of one launch of the dominant kernel.  One kernel per phase: 8 per tree
+    # position streamed (u32 parent position + i32 coordinate), 16 per candidate read (u_pos, v_pos,
+    # rank, coordinate) and the update's support-word reads and read-modify-writes as the kernel
+    # counts them.  Blocked: the tree pass streams every position (8 B) and every candidate (16 B
+    # read, 8 B candidate word written); the update is a kernel of its own, reported apart.
+    shape = solver.launch_shape()
+    dominant = "k_phase" if shape["block_phases"] == 1 else "k_block_q"
+    k_ms, k_n = kt["parity_search"]
     avg_ms = k_ms / max(k_n, 1)
-    bytes_per_launch = (8.0 * pos + 16.0 * cand + upd) / max(k_n, 1)
+    blocked = shape["block_phases"] > 1
+    cand_bytes = 24.0 if blocked else 16.0
+    dom_bytes = 8.0 * pos + cand_bytes * cand + (0 if blocked else upd)
+    bytes_per_launch = dom_bytes / max(k_n, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
 
     out = {
@@ -196,20 +208,22 @@ def main() -> int:
         "total_weight": total_weight,
         "host_preprocess_s": round(host_s, 3),
         "upload_s": round(upload_s, 3),
-        "phases_per_step": k_n // max(args.steps, 1),
-        "kernel_ms_per_step": round(k_ms / args.steps, 3),
+        "block_phases": shape["block_phases"],
+        "launches_per_step": k_n // max(args.steps, 1),
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in kt.items() if v[1]},
+        "update_bytes_per_step": upd // max(args.steps, 1),
         "roofline": {
-            "kernel": "k_phase",
+            "kernel": dominant,
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic("k_phase", args.workload) if not args.no_prune else None,
+            "traffic": measured_traffic(dominant, args.workload) if not args.no_prune else None,
             "bytes_per_launch": round(bytes_per_launch, 1),
             "bytes_split": {"tree_positions": round(8.0 * pos / max(k_n, 1), 1),
-                            "candidates": round(16.0 * cand / max(k_n, 1), 1),
-                            "support_update": round(upd / max(k_n, 1), 1)},
+                            "candidates": round(cand_bytes * cand / max(k_n, 1), 1),
+                            "support_update": 0.0 if blocked else round(upd / max(k_n, 1), 1)},
             "avg_launch_us": round(avg_ms * 1000.0, 3),
         },
     }

@@ -122,7 +122,9 @@ typedef struct {
     int32_t waves_per_block;  /* parity/search kernel; 0 = automatic */
     int32_t max_blocks;       /* 0 = automatic (resident blocks on every compute unit) */
     int32_t timing;           /* 1: time each kernel launch with HIP events on the solver's stream */
-    int32_t update_mode;      /* 0: per-phase dot + XOR kernels (reference schedule) */
+    int32_t block_phases;     /* phases resolved per pass over the trees: 0 = automatic (64 when the
+                                 largest tree's 64-bit path words fit one workgroup's LDS, else 1),
+                                 1 = one kernel per phase, 2..64 = blocked with that many */
 } mcb_solver_options;
 
 void mcb_solver_options_init(mcb_solver_options *opt);
@@ -154,10 +156,20 @@ int mcb_solver_cycle_vectors(mcb_solver *s, uint64_t *out, int32_t count);
  * out[p] = XOR of S bits along the path root -> position p (kernel (a) alone). */
 int mcb_solver_parity(mcb_solver *s, const uint64_t *S, uint8_t *out);
 
-/* Kernel ids for timing.  A phase is one launch of the phase kernel (update of the previous phase,
- * parity + search, cycle vector): its time is reported under MCB_K_PARITY_SEARCH; the other ids
- * are kept for layout stability and report zero launches. */
-enum { MCB_K_PARITY_SEARCH = 0, MCB_K_CYCLE = 1, MCB_K_DOT = 2, MCB_K_XOR = 3, MCB_K_COUNT = 4 };
+/* Kernel ids for timing.  One kernel per phase: every launch of the phase kernel (update of the
+ * previous phase, parity + search, cycle vector) counts under MCB_K_PARITY_SEARCH.  Blocked: per
+ * block of phases, the pass over the trees (path words and candidate words) counts under
+ * MCB_K_PARITY_SEARCH, the greedy basis and phase assignment under MCB_K_SELECT, the cycle and final
+ * support vectors under MCB_K_CYCLE, the update of later support vectors under MCB_K_UPDATE and the
+ * packing of the block's vectors under MCB_K_TRANSPOSE. */
+enum {
+    MCB_K_PARITY_SEARCH = 0,
+    MCB_K_SELECT = 1,
+    MCB_K_CYCLE = 2,
+    MCB_K_UPDATE = 3,
+    MCB_K_TRANSPOSE = 4,
+    MCB_K_COUNT = 5
+};
 
 /* Accumulated event time (ms) and launch count of one kernel since the last reset (timing=1). */
 int mcb_solver_kernel_time(mcb_solver *s, int kernel, double *total_ms, int64_t *launches);
@@ -171,9 +183,10 @@ int64_t mcb_solver_device_bytes(const mcb_solver *s);
  * 16 per word XORed, 2 per column flag). */
 int mcb_solver_work_counters(mcb_solver *s, int64_t *positions, int64_t *candidates, int64_t *update_bytes);
 
-/* Launch shape chosen for the parity/search kernel. */
+/* Launch shape of the dominant kernel (the phase kernel, or the blocked form's tree pass) and the
+ * number of phases per block in use. */
 int mcb_solver_launch_shape(const mcb_solver *s, int32_t *blocks, int32_t *waves_per_block, int32_t *lds_bytes,
-                            int32_t *s_in_lds);
+                            int32_t *block_phases);
 
 void mcb_solver_destroy(mcb_solver *s);
 

@@ -31,7 +31,7 @@ class SolverOptions(ctypes.Structure):
         ("waves_per_block", i32),
         ("max_blocks", i32),
         ("timing", i32),
-        ("update_mode", i32),
+        ("block_phases", i32),
     ]
 
 

@@ -637,6 +637,575 @@ __global__ void k_copy_u64(const uint64_t *__restrict__ src, uint64_t *__restric
 }
 
 // ---------------------------------------------------------------------------------------------
+// Blocked phase loop: up to 64 phases resolved per pass over the trees
+//
+// Parity is linear in S.  At the start of a block of kk phases e..e+kk-1 the vectors
+// T_l = S_{e+l} (all updates of earlier phases applied) are packed into one word per coordinate,
+// Tt[c] bit l = T_l[c].  One pass over every tree gives each candidate k the word
+// Qr[k] bit l = <C_k, T_l>.  The final vector of phase e+i is S_{e+i} = sum over l of M[i][l] T_l
+// (M lower unitriangular, row i = mask m_i), so candidate k is odd in phase e+i exactly when
+// popcount(m_i & Qr[k]) is odd.  The block's winners are then the first kk candidates, in rank
+// order, whose words are linearly independent over GF(2), and the phase a winner serves is the
+// lowest set bit of its word after elimination by the winners before it.  That is a greedy basis,
+// computed per rank chunk in parallel and merged chunk group by chunk group (the greedy basis of a
+// concatenation is the greedy basis of the concatenated greedy bases).  The later vectors
+// j >= e+kk then take the whole block's update at once: b_ji = <C_{e+i}, S_j> xor
+// sum_{l<i} b_jl <C_{e+i}, S_{e+l}>, S_j ^= sum_i b_ji S_{e+i}.
+// ---------------------------------------------------------------------------------------------
+
+constexpr int kBlockK = 64;      // phases per block (word width)
+constexpr int kQWaves = 16;      // waves per workgroup of the tree pass
+constexpr int kChunk = 512;      // ranks per first-level greedy basis
+constexpr int kGroup = 16;       // bases merged per group
+constexpr int kMaxLevels = 8;    // kChunk * kGroup^(kMaxLevels-1) covers 2^32 ranks
+constexpr int kTile = 32;        // support words staged in LDS per update step
+
+struct BlockArgs {
+    const int64_t *tree_off;
+    const uint32_t *parent_pos;
+    const int32_t *pos_coord;
+    const uint32_t *pos_pre, *pos_end;
+    const uint32_t *big_list, *small_list;
+    const int64_t *rc_off;
+    const uint32_t *rc_pu, *rc_pv, *rc_rank;
+    const int32_t *rc_coord;
+    const uint32_t *cand_root, *cand_pu, *cand_pv;
+    const int32_t *cand_coord;
+    uint64_t *Sw;
+    int64_t npad;
+    uint64_t *Tt;        // [npad] bit l of Tt[c] = bit c of T_l
+    uint64_t *Qr;        // [K]    bit l of Qr[k] = <C_k, T_l>
+    uint32_t *sum_cnt;   // [slots]      greedy bases: size
+    uint32_t *sum_rank;  // [slots][64]  ranks, ascending
+    uint64_t *sum_q;     // [slots][64]  their words
+    uint32_t *grp_count; // [groups]     members finished, per merge group
+    int64_t lvl_slot[kMaxLevels];  // first slot of each level
+    int64_t lvl_n[kMaxLevels];     // bases on each level
+    int64_t lvl_grp[kMaxLevels];   // first group counter of the groups merging level L into L+1
+    int32_t levels;
+    uint32_t *sel;       // [N] rank chosen in each phase
+    uint64_t *blk_q;     // [64] word of phase i's winner
+    uint64_t *blk_m;     // [64] mask m_i
+    uint64_t *blk_g;     // [64] bit l of row i = <C_{e+i}, S_{e+l}>, l < i
+    uint64_t *C_blk;     // [64][W] cycle vectors of the block
+    uint32_t *D_blk;     // [64][W] nonzero words of each below e
+    uint32_t *Dcnt_blk;  // [64]
+    uint64_t *Sf_blk;    // [64][W] final support vectors of the block
+    uint64_t *C_store;   // optional [N][W]
+    uint32_t *status;
+    unsigned long long *upd_bytes;  // accumulated algorithmic bytes of the update
+    int64_t K;
+    int32_t e, kk, N, W, n_big, n_small, big_words, small_words;
+};
+
+// Transposes the block's vectors: wave w builds Tt[64w .. 64w+63] from word w of columns
+// e .. e+kk-1 (bit b of that word in lane l is bit l of Tt[64w+b]); own bits are added.
+__global__ __launch_bounds__(256) void k_block_transpose(BlockArgs a) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t w = static_cast<int64_t>(blockIdx.x) * 4 + threadIdx.x / kWave;
+    const int64_t wl = (static_cast<int64_t>(a.e) + a.kk + kWave - 1) / kWave;
+    if (w >= wl) return;
+    const uint64_t x = lane < a.kk ? a.Sw[w * a.npad + a.e + lane] : 0ull;
+    uint64_t mine = 0;
+    for (int b = 0; b < kWave; ++b) {
+        const uint64_t m = __ballot(static_cast<int>((x >> b) & 1ull));
+        if (lane == b) mine = m;
+    }
+    const int64_t c = w * kWave + lane;
+    if (c >= a.e && c < static_cast<int64_t>(a.e) + a.kk) mine |= 1ull << (c - a.e);
+    a.Tt[c] = mine;
+}
+
+// Settles the lanes of one window whose parent lies in the same window: a lane takes its parent's
+// word once the parent is final; each round finalises at least the lowest pending lane.
+__device__ __forceinline__ uint64_t settle_words(uint64_t val, bool pending, uint32_t p, uint32_t wb, int lane) {
+    const uint64_t own = val;
+    bool done = !pending;
+    while (__ballot(!done)) {
+        const int src = pending ? static_cast<int>(p - wb) : lane;
+        const uint64_t pv = __shfl(val, src);
+        const int pd = __shfl(static_cast<int>(done), src);
+        if (!done && pd) {
+            val = own ^ pv;
+            done = true;
+        }
+    }
+    return val;
+}
+
+// Path words of kUnroll windows of one wave's tree: word[pos] = word[parent] ^ Tt[coordinate].
+__device__ __forceinline__ void word_group(const WindowGroup &g, uint32_t base, uint32_t npos, const uint64_t *Tt,
+                                           uint64_t *Wd, int lane) {
+    uint64_t t[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) t[u] = Tt[g.co[u] >= 0 ? g.co[u] : 0];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+        const uint32_t wb = base + u * kWave;
+        if (wb >= npos) break;
+        const bool v = wb + lane < npos;
+        const uint32_t p = v ? g.par[u] : kNoParent;
+        const uint64_t tv = (v && g.co[u] >= 0) ? t[u] : 0ull;
+        const bool before = p < wb;
+        uint64_t val = tv ^ (before ? Wd[p] : 0ull);
+        const bool inwin = (p != kNoParent) && (p >= wb);
+        if (__ballot(inwin)) val = settle_words(val, inwin, p, wb, lane);
+        if (v) Wd[wb + lane] = val;
+    }
+}
+
+__device__ __forceinline__ void tree_words(const uint32_t *pp, const int32_t *pc, uint32_t npos, const uint64_t *Tt,
+                                           uint64_t *Wd, int lane) {
+    constexpr uint32_t G = kWave * kUnroll;
+    WindowGroup A, B;
+    load_group(A, pp, pc, npos, lane, kWave);
+    for (uint32_t base = 0; base < npos; base += 2 * G) {
+        load_group(B, pp, pc, npos, base + G + lane, kWave);
+        word_group(A, base, npos, Tt, Wd, lane);
+        if (base + G >= npos) break;
+        load_group(A, pp, pc, npos, base + 2 * G + lane, kWave);
+        word_group(B, base + G, npos, Tt, Wd, lane);
+    }
+}
+
+// The same for a big tree shared by all waves of the workgroup: window w belongs to wave w mod nw
+// and is published with a ready stamp (this root's generation) once its words are in LDS.  A wave
+// waits only for lower windows, and the lowest unfinished window never waits, so every wait ends.
+__device__ __forceinline__ void coop_word_group(const WindowGroup &g, uint32_t w0, uint32_t nwin, uint32_t npos,
+                                                const uint64_t *Tt, uint64_t *Wd, uint32_t *ready, uint32_t gen,
+                                                int nw, int lane) {
+    uint64_t t[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) t[u] = Tt[g.co[u] >= 0 ? g.co[u] : 0];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+        const uint32_t w = w0 + u * nw;
+        if (w >= nwin) break;
+        const uint32_t wb = w * kWave;
+        const bool v = wb + lane < npos;
+        const uint32_t p = v ? g.par[u] : kNoParent;
+        const uint64_t tv = (v && g.co[u] >= 0) ? t[u] : 0ull;
+        const bool before = p < wb;
+        const uint32_t pw = before ? p / kWave : 0u;
+        while (before && __hip_atomic_load(ready + pw, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != gen)
+            __builtin_amdgcn_s_sleep(1);
+        uint64_t val = tv ^ (before ? Wd[p] : 0ull);
+        const bool inwin = (p != kNoParent) && (p >= wb);
+        if (__ballot(inwin)) val = settle_words(val, inwin, p, wb, lane);
+        if (v) Wd[wb + lane] = val;
+        if (lane == 0) __hip_atomic_store(ready + w, gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+__device__ __forceinline__ void tree_words_coop(const uint32_t *pp, const int32_t *pc, uint32_t npos,
+                                                const uint64_t *Tt, uint64_t *Wd, uint32_t *ready, uint32_t gen,
+                                                int wid, int nw, int lane) {
+    const uint32_t nwin = (npos + kWave - 1) / kWave;
+    const uint32_t stride = static_cast<uint32_t>(nw) * kWave;
+    const uint32_t gstep = kUnroll * static_cast<uint32_t>(nw);
+    WindowGroup A, B;
+    load_group(A, pp, pc, npos, wid * kWave + lane, stride);
+    for (uint32_t w0 = wid; w0 < nwin; w0 += 2 * gstep) {
+        load_group(B, pp, pc, npos, (w0 + gstep) * kWave + lane, stride);
+        coop_word_group(A, w0, nwin, npos, Tt, Wd, ready, gen, nw, lane);
+        if (w0 + gstep >= nwin) break;
+        load_group(A, pp, pc, npos, (w0 + 2 * gstep) * kWave + lane, stride);
+        coop_word_group(B, w0 + gstep, nwin, npos, Tt, Wd, ready, gen, nw, lane);
+    }
+}
+
+// Words of root r's candidates: Qr[rank] = word[u] ^ word[v] ^ Tt[own coordinate].
+__device__ __forceinline__ void root_words(const BlockArgs &a, uint32_t r, const uint64_t *Wd, int64_t first,
+                                           int64_t stride, int lane) {
+    constexpr int kB = 4;
+    const int64_t ce = a.rc_off[r + 1];
+    for (int64_t k0 = a.rc_off[r] + first; k0 < ce; k0 += kB * stride) {
+        uint32_t pu[kB], pv[kB], rk[kB];
+        int32_t c[kB];
+#pragma unroll
+        for (int b = 0; b < kB; ++b) {
+            const int64_t k = k0 + b * stride + lane;
+            const int64_t at = k < ce ? k : a.rc_off[r];
+            pu[b] = a.rc_pu[at];
+            pv[b] = a.rc_pv[at];
+            rk[b] = a.rc_rank[at];
+            c[b] = a.rc_coord[at];
+        }
+#pragma unroll
+        for (int b = 0; b < kB; ++b) {
+            const int64_t k = k0 + b * stride + lane;
+            if (k < ce) {
+                uint64_t q = Wd[pu[b]] ^ Wd[pv[b]];
+                if (c[b] >= 0) q ^= a.Tt[c[b]];
+                a.Qr[rk[b]] = q;
+            }
+        }
+    }
+}
+
+// Pass over every tree and every candidate of the block.  Big roots (more than small_words
+// positions) take a whole workgroup each, the others one wave each; both lists are handed out
+// round-robin, largest first.
+__global__ __launch_bounds__(kQWaves * kWave) void k_block_q(BlockArgs a) {
+    extern __shared__ uint64_t lds[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    uint64_t *Wbig = lds;
+    uint32_t *ready = reinterpret_cast<uint32_t *>(lds + a.big_words);
+    const uint32_t nwin_big = static_cast<uint32_t>(a.big_words) / kWave;
+    for (uint32_t i = threadIdx.x; i < nwin_big; i += blockDim.x) ready[i] = 0;
+    __syncthreads();
+    uint32_t gen = 0;
+    for (uint32_t it = blockIdx.x; it < static_cast<uint32_t>(a.n_big); it += gridDim.x) {
+        const uint32_t r = a.big_list[it];
+        const int64_t off = a.tree_off[r];
+        const uint32_t npos = static_cast<uint32_t>(a.tree_off[r + 1] - off);
+        ++gen;
+        tree_words_coop(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wbig, ready, gen, wid, nw, lane);
+        __syncthreads();
+        root_words(a, r, Wbig, static_cast<int64_t>(wid) * kWave, static_cast<int64_t>(nw) * kWave, lane);
+        __syncthreads();
+    }
+    // the big-tree region is free from here on; each wave takes its own slice for small roots
+    uint64_t *Ws = lds + static_cast<size_t>(wid) * a.small_words;
+    for (uint32_t it = blockIdx.x * nw + wid; it < static_cast<uint32_t>(a.n_small); it += gridDim.x * nw) {
+        const uint32_t r = a.small_list[it];
+        const int64_t off = a.tree_off[r];
+        const uint32_t npos = static_cast<uint32_t>(a.tree_off[r + 1] - off);
+        tree_words(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Ws, lane);
+        root_words(a, r, Ws, 0, kWave, lane);
+    }
+}
+
+// Greedy insertion of one batch (lane i: word q and rank, ranks ascending with the lane) into a
+// wave's basis, where lane l holds the pivot whose lowest set bit is l.  Each batch vector that is
+// independent of the basis and of the batch vectors before it is appended to (out_rank, out_q);
+// cnt (wave-uniform) stops at kk.
+__device__ __forceinline__ void basis_insert(uint64_t &piv, uint32_t &cnt, uint64_t q, uint32_t rank, int kk,
+                                             uint32_t *out_rank, uint64_t *out_q, int lane) {
+    uint64_t v = q;
+    while (cnt < static_cast<uint32_t>(kk)) {
+        for (;;) {
+            const int lb = v ? __ffsll(static_cast<unsigned long long>(v)) - 1 : 0;
+            const uint64_t p = __shfl(piv, lb);
+            const bool red = v != 0 && p != 0;
+            if (!__ballot(red)) break;
+            if (red) v ^= p;
+        }
+        const uint64_t cand = __ballot(v != 0);
+        if (!cand) break;
+        const int f = __ffsll(static_cast<unsigned long long>(cand)) - 1;
+        const uint64_t vf = __shfl(v, f);
+        const int lbf = __ffsll(static_cast<unsigned long long>(vf)) - 1;
+        if (lane == lbf) piv = vf;
+        const uint64_t qf = __shfl(q, f);
+        const uint32_t rf = __shfl(rank, f);
+        if (lane == 0) {
+            out_rank[cnt] = rf;
+            out_q[cnt] = qf;
+        }
+        ++cnt;
+        if (lane == f) v = 0;
+    }
+}
+
+// The block's greedy basis (top summary) in rank order -> phase of each winner, masks m_i, the
+// products <C_{e+i}, S_{e+l}> and the chosen ranks.  One wave.
+__device__ void block_finalize(const BlockArgs &a, int lane) {
+    const int kk = a.kk;
+    const int64_t top = a.lvl_slot[a.levels - 1];
+    const uint32_t n = a.sum_cnt[top];
+    bool ok = n == static_cast<uint32_t>(kk);
+    if (ok) {
+        const uint64_t q = lane < kk ? a.sum_q[top * kWave + lane] : 0ull;
+        const uint32_t rk = lane < kk ? a.sum_rank[top * kWave + lane] : 0u;
+        // phase of each winner: lowest set bit at which it enters the basis built in rank order
+        uint64_t piv = 0, v = q;
+        int phase_of = -1;
+        for (int f = 0; f < kk; ++f) {
+            for (;;) {
+                const int lb = v ? __ffsll(static_cast<unsigned long long>(v)) - 1 : 0;
+                const uint64_t p = __shfl(piv, lb);
+                const bool red = lane >= f && v != 0 && p != 0;
+                if (!__ballot(red)) break;
+                if (red) v ^= p;
+            }
+            const uint64_t vf = __shfl(v, f);
+            if (vf == 0) {
+                ok = false;
+                break;
+            }
+            const int lbf = __ffsll(static_cast<unsigned long long>(vf)) - 1;
+            if (lane == lbf) piv = vf;
+            if (lane == f) phase_of = lbf;
+        }
+        if (ok) {
+            uint64_t qi = 0;
+            uint32_t ri = 0;
+            for (int i = 0; i < kk; ++i) {
+                const uint64_t b = __ballot(phase_of == i);
+                const int f = b ? __ffsll(static_cast<unsigned long long>(b)) - 1 : 0;
+                const uint64_t qf = __shfl(q, f);
+                const uint32_t rf = __shfl(rk, f);
+                if (lane == i) {
+                    qi = qf;
+                    ri = rf;
+                }
+            }
+            // masks: after phase i, every later mask with an odd product with the winner takes m_i
+            uint64_t m = lane < kk ? (1ull << lane) : 0ull;
+            for (int i = 0; i < kk; ++i) {
+                const uint64_t qw = __shfl(qi, i);
+                const uint64_t mi = __shfl(m, i);
+                if (lane > i && lane < kk && (__popcll(m & qw) & 1)) m ^= mi;
+            }
+            uint64_t g = 0;
+            for (int l = 0; l < kk; ++l) {
+                const uint64_t ml = __shfl(m, l);
+                if (l < lane) g |= static_cast<uint64_t>(__popcll(ml & qi) & 1) << l;
+            }
+            const bool odd = lane >= kk || (__popcll(m & qi) & 1);
+            if (__ballot(!odd)) ok = false;
+            if (lane < kk) {
+                a.sel[a.e + lane] = ri;
+                a.blk_q[lane] = qi;
+                a.blk_m[lane] = m;
+                a.blk_g[lane] = g;
+            }
+        }
+    }
+    if (!ok) {
+        if (lane == 0) atomicOr(a.status, 1u);
+        if (lane < kk) a.sel[a.e + lane] = kNoRank;
+    }
+    // every group has finished: clear the counters for the next block
+    for (int64_t i = lane; i < a.lvl_grp[a.levels - 1]; i += kWave) a.grp_count[i] = 0;
+}
+
+// Greedy basis of the block.  Wave c takes ranks [c*kChunk, (c+1)*kChunk); the last wave to finish
+// in each group of kGroup merges the group's bases into the next level, and the wave that builds
+// the single top basis finalises the block.
+__global__ __launch_bounds__(256) void k_block_basis(BlockArgs a) {
+    constexpr int kBatches = kChunk / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * 4 + threadIdx.x / kWave;
+    if (c >= a.lvl_n[0]) return;
+    const int64_t r0 = c * kChunk;
+    const int64_t r1 = r0 + kChunk < a.K ? r0 + kChunk : a.K;
+    uint64_t q[kBatches];
+#pragma unroll
+    for (int u = 0; u < kBatches; ++u) {
+        const int64_t k = r0 + u * kWave + lane;
+        q[u] = a.Qr[k < r1 ? k : r0];
+        if (k >= r1) q[u] = 0;
+    }
+    uint64_t piv = 0;
+    uint32_t cnt = 0;
+    int64_t slot = a.lvl_slot[0] + c;
+#pragma unroll
+    for (int u = 0; u < kBatches; ++u) {
+        if (cnt >= static_cast<uint32_t>(a.kk) || !__ballot(q[u] != 0)) continue;
+        basis_insert(piv, cnt, q[u], static_cast<uint32_t>(r0 + u * kWave + lane), a.kk, a.sum_rank + slot * kWave,
+                     a.sum_q + slot * kWave, lane);
+    }
+    if (lane == 0) a.sum_cnt[slot] = cnt;
+    int64_t idx = c;
+    for (int L = 0; L + 1 < a.levels; ++L) {
+        const int64_t g = idx / kGroup;
+        const int64_t rest = a.lvl_n[L] - g * kGroup;
+        const uint32_t members = static_cast<uint32_t>(rest < kGroup ? rest : kGroup);
+        uint32_t last = 0;
+        if (lane == 0) {
+            __threadfence();
+            last = atomicAdd(a.grp_count + a.lvl_grp[L] + g, 1u) == members - 1 ? 1u : 0u;
+        }
+        last = __shfl(last, 0);
+        if (!last) return;
+        __threadfence();
+        piv = 0;
+        cnt = 0;
+        slot = a.lvl_slot[L + 1] + g;
+        for (uint32_t m = 0; m < members && cnt < static_cast<uint32_t>(a.kk); ++m) {
+            const int64_t s = a.lvl_slot[L] + g * kGroup + m;
+            const uint32_t n = __hip_atomic_load(a.sum_cnt + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (n == 0) continue;
+            const uint64_t qq = lane < static_cast<int>(n) ? a.sum_q[s * kWave + lane] : 0ull;
+            const uint32_t rr = lane < static_cast<int>(n) ? a.sum_rank[s * kWave + lane] : 0u;
+            basis_insert(piv, cnt, qq, rr, a.kk, a.sum_rank + slot * kWave, a.sum_q + slot * kWave, lane);
+        }
+        if (lane == 0) a.sum_cnt[slot] = cnt;
+        idx = g;
+    }
+    __threadfence();
+    block_finalize(a, lane);
+}
+
+// C of one chosen candidate into LDS (W words, cleared by the caller): the coordinates of the edges
+// on root -> u and root -> v, found by testing every position of the root's tree against the DFS
+// intervals of u and v, and the candidate's own edge.
+__device__ void cycle_into_lds(const int64_t *tree_off, const uint32_t *pos_pre, const uint32_t *pos_end,
+                               const int32_t *pos_coord, uint32_t root, uint32_t cu, uint32_t cv, int32_t ccoord,
+                               uint64_t *Cl) {
+    constexpr int kB = 8;
+    const int tid = threadIdx.x;
+    const int64_t off = tree_off[root];
+    const uint32_t npos = static_cast<uint32_t>(tree_off[root + 1] - off);
+    const uint32_t pu = pos_pre[off + cu];
+    const uint32_t pv = pos_pre[off + cv];
+    if (tid == 0 && ccoord >= 0)
+        atomicOr(reinterpret_cast<unsigned long long *>(&Cl[ccoord >> 6]), 1ull << (ccoord & 63));
+    for (uint32_t x0 = 1 + tid; x0 < npos; x0 += kB * blockDim.x) {
+        uint32_t lo[kB], hi[kB];
+        int32_t c[kB];
+#pragma unroll
+        for (int k = 0; k < kB; ++k) {
+            const uint32_t x = x0 + k * blockDim.x;
+            const uint32_t at = x < npos ? x : 0u;
+            lo[k] = pos_pre[off + at];
+            hi[k] = pos_end[off + at];
+            c[k] = x < npos ? pos_coord[off + at] : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < kB; ++k) {
+            const bool on = (lo[k] <= pu && pu < hi[k]) || (lo[k] <= pv && pv < hi[k]);
+            if (on && c[k] >= 0)
+                atomicOr(reinterpret_cast<unsigned long long *>(&Cl[c[k] >> 6]), 1ull << (c[k] & 63));
+        }
+    }
+}
+
+// Workgroup i: C_{e+i} of phase e+i's winner (and its nonzero words below e), and the final
+// support vector S_{e+i} = sum over l of m_i[l] T_l.
+__global__ __launch_bounds__(256) void k_block_cycles(BlockArgs a) {
+    extern __shared__ uint64_t lds[];
+    __shared__ uint32_t cnt;
+    const int i = blockIdx.x;
+    const int tid = threadIdx.x;
+    uint64_t *Cl = lds;
+    for (int w = tid; w < a.W; w += blockDim.x) Cl[w] = 0;
+    if (tid == 0) cnt = 0;
+    __syncthreads();
+    const uint32_t rank = a.sel[a.e + i];
+    if (rank < a.K)
+        cycle_into_lds(a.tree_off, a.pos_pre, a.pos_end, a.pos_coord, a.cand_root[rank], a.cand_pu[rank],
+                       a.cand_pv[rank], a.cand_coord[rank], Cl);
+    __syncthreads();
+    const int32_t dlim = (a.e + kWave - 1) / kWave;
+    uint64_t *Cout = a.C_blk + static_cast<int64_t>(i) * a.W;
+    uint32_t *Dout = a.D_blk + static_cast<int64_t>(i) * a.W;
+    for (int w = tid; w < a.W; w += blockDim.x) {
+        const uint64_t v = Cl[w];
+        Cout[w] = v;
+        if (a.C_store) a.C_store[(static_cast<int64_t>(a.e) + i) * a.W + w] = v;
+        if (w < dlim && v) Dout[atomicAdd(&cnt, 1u)] = static_cast<uint32_t>(w);
+    }
+    const uint64_t m = a.blk_m[i];
+    const int32_t wl = (a.e + a.kk + kWave - 1) / kWave;
+    for (int w = tid; w < wl; w += blockDim.x) {
+        uint64_t v = 0;
+        for (int l = 0; l < a.kk; ++l) {
+            if (!((m >> l) & 1ull)) continue;
+            const int64_t j = static_cast<int64_t>(a.e) + l;
+            uint64_t t = a.Sw[static_cast<int64_t>(w) * a.npad + j];
+            if ((j >> 6) == w) t |= 1ull << (j & 63);
+            v ^= t;
+        }
+        a.Sf_blk[static_cast<int64_t>(i) * a.W + w] = v;
+    }
+    __syncthreads();
+    if (tid == 0) a.Dcnt_blk[i] = cnt;
+}
+
+// Thread j: column e+i < e+kk receives its final vector S_{e+i}; column j >= e+kk takes the
+// block's update: d_i = <C_{e+i}, S_j>, b_i = d_i ^ sum_{l<i} b_l <C_{e+i}, S_{e+l}>,
+// S_j ^= sum_i b_i S_{e+i}.  S_j has support below e besides its own bit, so the dot products
+// read only the words where some C_{e+i} is nonzero below e, plus bit j of each C.
+// LDS: G rows [64] | word masks U [dlim] | word list [dlim] | S tile [64][kTile].
+__global__ __launch_bounds__(256) void k_block_update(BlockArgs a) {
+    extern __shared__ uint64_t lds[];
+    __shared__ uint32_t nlist;
+    const int tid = threadIdx.x;
+    const int kk = a.kk;
+    const int32_t dlim = (a.e + kWave - 1) / kWave;
+    const int32_t wl = (a.e + kk + kWave - 1) / kWave;
+    uint64_t *G = lds;
+    uint64_t *U = G + kBlockK;
+    uint32_t *wlist = reinterpret_cast<uint32_t *>(U + dlim);
+    uint64_t *T = U + dlim + (dlim + 1) / 2;
+    for (int x = tid; x < kBlockK; x += blockDim.x) G[x] = x < kk ? a.blk_g[x] : 0ull;
+    for (int x = tid; x < dlim; x += blockDim.x) U[x] = 0;
+    if (tid == 0) nlist = 0;
+    __syncthreads();
+    for (int i = 0; i < kk; ++i) {
+        const uint32_t nd = a.Dcnt_blk[i];
+        const uint32_t *Di = a.D_blk + static_cast<int64_t>(i) * a.W;
+        for (uint32_t q = tid; q < nd; q += blockDim.x)
+            atomicOr(reinterpret_cast<unsigned long long *>(&U[Di[q]]), 1ull << i);
+    }
+    __syncthreads();
+    for (int x = tid; x < dlim; x += blockDim.x)
+        if (U[x]) wlist[atomicAdd(&nlist, 1u)] = static_cast<uint32_t>(x);
+    __syncthreads();
+    const uint32_t nl = nlist;
+    const int64_t j = static_cast<int64_t>(a.e) + static_cast<int64_t>(blockIdx.x) * blockDim.x + tid;
+    const bool own_col = j < static_cast<int64_t>(a.e) + kk;
+    const bool upd_col = !own_col && j < a.N;
+    uint64_t b = 0;
+    uint64_t nbytes = 0;
+    if (own_col) {
+        const int i = static_cast<int>(j - a.e);
+        const uint64_t *Sf = a.Sf_blk + static_cast<int64_t>(i) * a.W;
+        for (int32_t w = 0; w < wl; ++w) {
+            uint64_t v = Sf[w];
+            if ((j >> 6) == w) v &= ~(1ull << (j & 63));
+            a.Sw[static_cast<int64_t>(w) * a.npad + j] = v;
+        }
+    } else if (upd_col) {
+        uint64_t d = 0;
+        for (int i = 0; i < kk; ++i)
+            d |= ((a.C_blk[static_cast<int64_t>(i) * a.W + (j >> 6)] >> (j & 63)) & 1ull) << i;
+        for (uint32_t x = 0; x < nl; ++x) {
+            const uint32_t w = wlist[x];
+            const uint64_t s = a.Sw[static_cast<int64_t>(w) * a.npad + j];
+            uint64_t mm = U[w];
+            while (mm) {
+                const int i = __ffsll(static_cast<unsigned long long>(mm)) - 1;
+                mm &= mm - 1;
+                d ^= static_cast<uint64_t>(__popcll(a.C_blk[static_cast<int64_t>(i) * a.W + w] & s) & 1) << i;
+            }
+        }
+        for (int i = 0; i < kk; ++i) {
+            const uint64_t bi = ((d >> i) ^ static_cast<uint64_t>(__popcll(b & G[i]))) & 1ull;
+            b |= bi << i;
+        }
+        nbytes = 2 + 8ull * nl;
+    }
+    for (int32_t w0 = 0; w0 < wl; w0 += kTile) {
+        __syncthreads();
+        for (int x = tid; x < kk * kTile; x += blockDim.x) {
+            const int i = x / kTile, t = x % kTile;
+            T[x] = w0 + t < wl ? a.Sf_blk[static_cast<int64_t>(i) * a.W + w0 + t] : 0ull;
+        }
+        __syncthreads();
+        if (b) {
+            const int tn = wl - w0 < kTile ? wl - w0 : kTile;
+            for (int t = 0; t < tn; ++t) {
+                uint64_t x = 0;
+                for (int i = 0; i < kk; ++i) x ^= T[i * kTile + t] & (0ull - ((b >> i) & 1ull));
+                if (x) {
+                    a.Sw[static_cast<int64_t>(w0 + t) * a.npad + j] ^= x;
+                    nbytes += 16;
+                }
+            }
+        }
+    }
+    // one atomic per wave
+    for (int o = kWave / 2; o > 0; o >>= 1) nbytes += __shfl_down(nbytes, o);
+    if ((tid & (kWave - 1)) == 0 && nbytes) atomicAdd(a.upd_bytes, static_cast<unsigned long long>(nbytes));
+}
+
+// ---------------------------------------------------------------------------------------------
 // Device properties shared by both forms
 // ---------------------------------------------------------------------------------------------
 
@@ -969,9 +1538,20 @@ struct mcb_solver {
     int wpb = kBlockWaves, blocks = 1, lds_bytes = 0;
     hipStream_t stream = nullptr;
     int64_t bytes = 0;
+    // blocked form (block_phases > 1)
+    int32_t bk = 1;                                   // phases per block; 1 = one kernel per phase
+    int32_t q_big_words = 0, q_small_words = 0, q_blocks = 0, q_lds = 0, n_qbig = 0, n_qsmall = 0;
+    uint32_t *q_big_list = nullptr, *q_small_list = nullptr;
+    uint64_t *Tt = nullptr, *Qr = nullptr, *sum_q = nullptr, *blk_q = nullptr, *blk_m = nullptr, *blk_g = nullptr;
+    uint64_t *C_blk = nullptr, *Sf_blk = nullptr;
+    uint32_t *sum_cnt = nullptr, *sum_rank = nullptr, *grp_count = nullptr, *D_blk = nullptr, *Dcnt_blk = nullptr;
+    unsigned long long *upd_bytes = nullptr;
+    int32_t levels = 0;
+    int64_t lvl_slot[kMaxLevels] = {}, lvl_n[kMaxLevels] = {}, lvl_grp[kMaxLevels] = {};
+    int64_t blk_positions = 0, blk_candidates = 0;  // work of the tree passes since reset
     // timing
-    double kt_ms[MCB_K_COUNT] = {0, 0, 0, 0};
-    int64_t kt_n[MCB_K_COUNT] = {0, 0, 0, 0};
+    double kt_ms[MCB_K_COUNT] = {};
+    int64_t kt_n[MCB_K_COUNT] = {};
     std::vector<hipEvent_t> ev_pool;
     std::vector<int> ev_kind;
     size_t ev_used = 0;
@@ -1065,6 +1645,89 @@ PhaseArgs phase_args(const mcb_solver *s, int32_t e) {
     return a;
 }
 
+BlockArgs block_args(const mcb_solver *s, int32_t e, int32_t kk) {
+    BlockArgs a{};
+    a.tree_off = s->tree_off;
+    a.parent_pos = s->parent_pos;
+    a.pos_coord = s->pos_coord;
+    a.pos_pre = s->pos_pre;
+    a.pos_end = s->pos_end;
+    a.big_list = s->q_big_list;
+    a.small_list = s->q_small_list;
+    a.rc_off = s->rc_off;
+    a.rc_pu = s->rc_pu;
+    a.rc_pv = s->rc_pv;
+    a.rc_rank = s->rc_rank;
+    a.rc_coord = s->rc_coord;
+    a.cand_root = s->cand_root;
+    a.cand_pu = s->cand_pu;
+    a.cand_pv = s->cand_pv;
+    a.cand_coord = s->cand_coord;
+    a.Sw = s->Sw;
+    a.npad = s->npad;
+    a.Tt = s->Tt;
+    a.Qr = s->Qr;
+    a.sum_cnt = s->sum_cnt;
+    a.sum_rank = s->sum_rank;
+    a.sum_q = s->sum_q;
+    a.grp_count = s->grp_count;
+    for (int L = 0; L < kMaxLevels; ++L) {
+        a.lvl_slot[L] = s->lvl_slot[L];
+        a.lvl_n[L] = s->lvl_n[L];
+        a.lvl_grp[L] = s->lvl_grp[L];
+    }
+    a.levels = s->levels;
+    a.sel = s->best;
+    a.blk_q = s->blk_q;
+    a.blk_m = s->blk_m;
+    a.blk_g = s->blk_g;
+    a.C_blk = s->C_blk;
+    a.D_blk = s->D_blk;
+    a.Dcnt_blk = s->Dcnt_blk;
+    a.Sf_blk = s->Sf_blk;
+    a.C_store = s->C_store;
+    a.status = s->status;
+    a.upd_bytes = s->upd_bytes;
+    a.K = s->K;
+    a.e = e;
+    a.kk = kk;
+    a.N = s->N;
+    a.W = s->W;
+    a.n_big = s->n_qbig;
+    a.n_small = s->n_qsmall;
+    a.big_words = s->q_big_words;
+    a.small_words = s->q_small_words;
+    return a;
+}
+
+// One block of kk phases starting at e: transpose, tree pass, greedy basis, cycle vectors, update.
+int launch_block(mcb_solver *s, int32_t e, int32_t kk) {
+    const BlockArgs a = block_args(s, e, kk);
+    const int64_t wl = (static_cast<int64_t>(e) + kk + kWave - 1) / kWave;
+    int rc = timed_begin(s, MCB_K_TRANSPOSE);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_block_transpose, dim3(static_cast<unsigned>((wl + 3) / 4)), dim3(256), 0, s->stream, a);
+    HIP_TRY(hipGetLastError());
+    if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_PARITY_SEARCH))) return rc;
+    hipLaunchKernelGGL(k_block_q, dim3(s->q_blocks), dim3(kQWaves * kWave), s->q_lds, s->stream, a);
+    HIP_TRY(hipGetLastError());
+    if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_SELECT))) return rc;
+    hipLaunchKernelGGL(k_block_basis, dim3(static_cast<unsigned>((s->lvl_n[0] + 3) / 4)), dim3(256), 0, s->stream, a);
+    HIP_TRY(hipGetLastError());
+    if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_CYCLE))) return rc;
+    hipLaunchKernelGGL(k_block_cycles, dim3(kk), dim3(256), sizeof(uint64_t) * s->W, s->stream, a);
+    HIP_TRY(hipGetLastError());
+    if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_UPDATE))) return rc;
+    const int32_t dlim = (e + kWave - 1) / kWave;
+    const size_t ulds = sizeof(uint64_t) * (kBlockK + dlim + (dlim + 1) / 2 + static_cast<size_t>(kBlockK) * kTile);
+    const int64_t cols = s->N - e;
+    hipLaunchKernelGGL(k_block_update, dim3(static_cast<unsigned>((cols + 255) / 256)), dim3(256), ulds, s->stream, a);
+    HIP_TRY(hipGetLastError());
+    s->blk_positions += s->Ptot;
+    s->blk_candidates += s->K;
+    return timed_end(s);
+}
+
 int launch_phase(mcb_solver *s, int32_t e) {
     int rc = timed_begin(s, MCB_K_PARITY_SEARCH);
     if (rc) return rc;
@@ -1103,6 +1766,83 @@ void preorder_intervals(const int64_t *tree_offset, const uint32_t *parent_pos, 
     }
 }
 
+// Buffers and launch shape of the blocked form.  Trees with more than 1024 positions take a whole
+// workgroup of the tree pass (their path words must fit the LDS of one workgroup); smaller ones take
+// one wave.  Chooses the per-phase form instead when block_phases asks for it or when the largest
+// tree does not fit.
+template <typename NposFn>
+int setup_blocked(mcb_solver *s, const std::vector<uint32_t> &order, NposFn npos_of) {
+    const int want = s->opt.block_phases;
+    if (want < 0 || want > kBlockK) return fail(MCB_E_INVALID, "block_phases must be 0..64");
+    if (want == 1) return MCB_OK;
+    constexpr int64_t kSmallMax = 1024;
+    std::vector<uint32_t> big, small;
+    int64_t max_big = 0, max_small = 1;
+    for (uint32_t r : order) {
+        const int64_t np = npos_of(r);
+        if (np > kSmallMax) {
+            big.push_back(r);
+            max_big = std::max(max_big, np);
+        } else {
+            small.push_back(r);
+            max_small = std::max(max_small, np);
+        }
+    }
+    const int64_t big_words = (max_big + kWave - 1) / kWave * kWave;
+    const int64_t small_words = (max_small + kWave - 1) / kWave * kWave;
+    const int64_t lds = std::max<int64_t>(8 * big_words + 4 * (big_words / kWave), 8 * kQWaves * small_words);
+    const int64_t lds_max = std::min<int64_t>(g_dev.prop.sharedMemPerBlock, 160 * 1024);
+    if (lds > lds_max) {
+        if (want > 1) return fail(MCB_E_INVALID, "a tree is too large for the blocked form's LDS budget");
+        return MCB_OK;  // automatic choice: fall back to one kernel per phase
+    }
+    s->bk = want > 1 ? want : kBlockK;
+    s->q_big_words = static_cast<int32_t>(big_words);
+    s->q_small_words = static_cast<int32_t>(small_words);
+    s->q_lds = static_cast<int32_t>(lds);
+    s->n_qbig = static_cast<int32_t>(big.size());
+    s->n_qsmall = static_cast<int32_t>(small.size());
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(k_block_q), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                s->q_lds));
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_block_q, kQWaves * kWave, s->q_lds));
+    per_cu = std::max(1, per_cu);
+    s->q_blocks = per_cu * g_dev.prop.multiProcessorCount;
+    // greedy-basis levels: kChunk ranks per first-level basis, kGroup bases per merge
+    int64_t n = (s->K + kChunk - 1) / kChunk, slots = 0, groups = 0;
+    s->levels = 0;
+    for (;;) {
+        if (s->levels >= kMaxLevels) return fail(MCB_E_INVALID, "too many candidates for the basis levels");
+        s->lvl_slot[s->levels] = slots;
+        s->lvl_n[s->levels] = n;
+        s->lvl_grp[s->levels] = groups;
+        slots += n;
+        ++s->levels;
+        if (n == 1) break;
+        n = (n + kGroup - 1) / kGroup;
+        groups += n;
+    }
+    int64_t &B = s->bytes;
+    int rc;
+    if ((rc = upload(&s->q_big_list, big.data(), big.size(), B))) return rc;
+    if ((rc = upload(&s->q_small_list, small.data(), small.size(), B))) return rc;
+    if ((rc = dev_alloc(&s->Tt, static_cast<size_t>(s->npad), B))) return rc;
+    if ((rc = dev_alloc(&s->Qr, static_cast<size_t>(s->K), B))) return rc;
+    if ((rc = dev_alloc(&s->sum_cnt, static_cast<size_t>(slots), B))) return rc;
+    if ((rc = dev_alloc(&s->sum_rank, static_cast<size_t>(slots) * kWave, B))) return rc;
+    if ((rc = dev_alloc(&s->sum_q, static_cast<size_t>(slots) * kWave, B))) return rc;
+    if ((rc = dev_alloc(&s->grp_count, static_cast<size_t>(std::max<int64_t>(groups, 1)), B))) return rc;
+    if ((rc = dev_alloc(&s->blk_q, kBlockK, B))) return rc;
+    if ((rc = dev_alloc(&s->blk_m, kBlockK, B))) return rc;
+    if ((rc = dev_alloc(&s->blk_g, kBlockK, B))) return rc;
+    if ((rc = dev_alloc(&s->C_blk, static_cast<size_t>(kBlockK) * s->W, B))) return rc;
+    if ((rc = dev_alloc(&s->Sf_blk, static_cast<size_t>(kBlockK) * s->W, B))) return rc;
+    if ((rc = dev_alloc(&s->D_blk, static_cast<size_t>(kBlockK) * s->W, B))) return rc;
+    if ((rc = dev_alloc(&s->Dcnt_blk, kBlockK, B))) return rc;
+    if ((rc = dev_alloc(&s->upd_bytes, 1, B))) return rc;
+    return MCB_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1117,7 +1857,9 @@ void mcb_solver_destroy(mcb_solver *s) {
     void *ptrs[] = {s->tree_off, s->parent_pos, s->pos_pre, s->pos_end, s->pos_coord, s->big_list, s->small_list,
                     s->root_min_rank, s->rc_off, s->rc_pu, s->rc_pv, s->rc_rank, s->rc_coord, s->cand_root,
                     s->cand_pu, s->cand_pv, s->cand_coord, s->Sw, s->Scur, s->C, s->C_store, s->D, s->Dcount,
-                    s->best, s->done, s->status, s->wave_stats, s->flag, s->E, s->Ecount, s->cdot};
+                    s->best, s->done, s->status, s->wave_stats, s->flag, s->E, s->Ecount, s->cdot,
+                    s->q_big_list, s->q_small_list, s->Tt, s->Qr, s->sum_q, s->blk_q, s->blk_m, s->blk_g, s->C_blk,
+                    s->Sf_blk, s->sum_cnt, s->sum_rank, s->grp_count, s->D_blk, s->Dcnt_blk, s->upd_bytes};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
@@ -1259,6 +2001,7 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
     blocks = std::min<int64_t>(blocks, s->n_big + (s->n_small + kBlockWaves - 1) / kBlockWaves);
     s->blocks = static_cast<int>(std::max<int64_t>(1, blocks));
     if ((rc = dev_alloc(&s->wave_stats, 3 * static_cast<size_t>(s->blocks) * s->wpb, B))) return cleanup(rc);
+    if ((rc = setup_blocked(s, order, npos_of))) return cleanup(rc);
     if (s->opt.timing) {
         s->ev_pool.resize(4096);
         s->ev_kind.resize(2048);
@@ -1292,8 +2035,16 @@ int mcb_solver_reset(mcb_solver *s) {
     HIP_TRY(hipMemsetAsync(s->C, 0, 2 * sizeof(uint64_t) * s->W, s->stream));
     HIP_TRY(hipMemsetAsync(s->status, 0, sizeof(uint32_t), s->stream));
     HIP_TRY(hipMemsetAsync(s->wave_stats, 0, sizeof(uint64_t) * 3 * s->blocks * s->wpb, s->stream));
+    if (s->bk > 1) {
+        HIP_TRY(hipMemsetAsync(s->Tt, 0, sizeof(uint64_t) * static_cast<size_t>(s->npad), s->stream));
+        HIP_TRY(hipMemsetAsync(s->grp_count, 0, sizeof(uint32_t) * std::max<int64_t>(s->lvl_grp[s->levels - 1], 1),
+                               s->stream));
+        HIP_TRY(hipMemsetAsync(s->upd_bytes, 0, sizeof(unsigned long long), s->stream));
+    }
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->phase = 0;
+    s->blk_positions = 0;
+    s->blk_candidates = 0;
     s->ev_used = 0;
     for (int i = 0; i < MCB_K_COUNT; ++i) {
         s->kt_ms[i] = 0;
@@ -1305,10 +2056,12 @@ int mcb_solver_reset(mcb_solver *s) {
 int mcb_solver_run(mcb_solver *s, int32_t phases) {
     if (!s || phases < 0) return fail(MCB_E_INVALID, "invalid arguments");
     const int32_t end = std::min(s->N, s->phase + phases);
-    for (int32_t e = s->phase; e < end; ++e) {
-        int rc = launch_phase(s, e);
+    while (s->phase < end) {
+        const int32_t e = s->phase;
+        const int32_t kk = std::min(s->bk, end - e);
+        int rc = s->bk > 1 ? launch_block(s, e, kk) : launch_phase(s, e);
         if (rc) return rc;
-        s->phase = e + 1;
+        s->phase = e + kk;
     }
     return MCB_OK;
 }
@@ -1397,11 +2150,16 @@ int mcb_solver_work_counters(mcb_solver *s, int64_t *positions, int64_t *candida
     HIP_TRY(hipStreamSynchronize(s->stream));
     std::vector<uint64_t> st(3 * static_cast<size_t>(s->blocks) * s->wpb);
     HIP_TRY(hipMemcpy(st.data(), s->wave_stats, sizeof(uint64_t) * st.size(), hipMemcpyDeviceToHost));
-    uint64_t p = 0, c = 0, u = 0;
+    uint64_t p = static_cast<uint64_t>(s->blk_positions), c = static_cast<uint64_t>(s->blk_candidates), u = 0;
     for (size_t i = 0; i < st.size(); i += 3) {
         p += st[i];
         c += st[i + 1];
         u += st[i + 2];
+    }
+    if (s->bk > 1) {
+        unsigned long long ub = 0;
+        HIP_TRY(hipMemcpy(&ub, s->upd_bytes, sizeof(ub), hipMemcpyDeviceToHost));
+        u += ub;
     }
     if (positions) *positions = static_cast<int64_t>(p);
     if (candidates) *candidates = static_cast<int64_t>(c);
@@ -1410,12 +2168,13 @@ int mcb_solver_work_counters(mcb_solver *s, int64_t *positions, int64_t *candida
 }
 
 int mcb_solver_launch_shape(const mcb_solver *s, int32_t *blocks, int32_t *waves_per_block, int32_t *lds_bytes,
-                            int32_t *s_in_lds) {
+                            int32_t *block_phases) {
     if (!s) return fail(MCB_E_INVALID, "invalid arguments");
-    if (blocks) *blocks = s->blocks;
-    if (waves_per_block) *waves_per_block = s->wpb;
-    if (lds_bytes) *lds_bytes = s->lds_bytes;
-    if (s_in_lds) *s_in_lds = 1;
+    const bool blk = s->bk > 1;
+    if (blocks) *blocks = blk ? s->q_blocks : s->blocks;
+    if (waves_per_block) *waves_per_block = blk ? kQWaves : s->wpb;
+    if (lds_bytes) *lds_bytes = blk ? s->q_lds : s->lds_bytes;
+    if (block_phases) *block_phases = s->bk;
     return MCB_OK;
 }
 

@@ -18,7 +18,7 @@ from . import _lib
 from ._hipsig import SolverDesc, SolverOptions
 from .host import Problem
 
-KERNELS = ("parity_search", "cycle", "dot", "xor")
+KERNELS = ("parity_search", "select", "cycle", "update", "transpose")
 
 
 def device_count() -> int:
@@ -44,7 +44,8 @@ class Solver:
     """Device-resident de Pina phase loop for one preprocessed problem."""
 
     def __init__(self, problem: Problem, store_cycles: bool = False, timing: bool = False,
-                 waves_per_block: int = 0, max_blocks: int = 0):
+                 waves_per_block: int = 0, max_blocks: int = 0, block_phases: int = 0):
+        """block_phases: phases per pass over the trees (0 automatic, 1 one kernel per phase, up to 64)."""
         info = problem.info
         if info.single_cycle:
             raise ValueError("a single-cycle graph has no phase loop")
@@ -69,6 +70,7 @@ class Solver:
         o.timing = int(bool(timing))
         o.waves_per_block = int(waves_per_block)
         o.max_blocks = int(max_blocks)
+        o.block_phases = int(block_phases)
         h = ctypes.c_void_p()
         _lib.check_hip(_lib.hip().mcb_solver_create(ctypes.byref(d), ctypes.byref(o), ctypes.byref(h)))
         self._h = h
@@ -137,7 +139,7 @@ class Solver:
     def launch_shape(self) -> dict:
         v = [ctypes.c_int32() for _ in range(4)]
         _lib.check_hip(_lib.hip().mcb_solver_launch_shape(self._h, *[ctypes.byref(x) for x in v]))
-        return dict(zip(("blocks", "waves_per_block", "lds_bytes", "s_in_lds"), (x.value for x in v)))
+        return dict(zip(("blocks", "waves_per_block", "lds_bytes", "block_phases"), (x.value for x in v)))
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -25,12 +25,14 @@ def test_grid64_against_reference_csv(prune):
     assert resp.total_weight == fx["mcb_cpu_baseline"]["total_weight"]
 
 
-def test_grid64_sequence_against_oracle():
+@pytest.mark.parametrize("block", [1, 0])
+def test_grid64_sequence_against_oracle(block):
     fx = load_csv("grid64x64")
     n, u, v, w = graph_of(fx)
+    print(f"case grid64x64 sequence block={block}", flush=True)
     p = Problem.from_edges(n, u, v, w, keep_vertex_trees=True, prune_trees=True)
     ref = pyoracle.run(p, want_s=True)
-    s = Solver(p)
+    s = Solver(p, block_phases=block)
     s.run()
     s.sync()
     assert np.array_equal(s.selected(), ref["selected"])
@@ -64,8 +66,16 @@ def test_er20000_properties_and_prefix():
     sel2 = s2.selected()
     S = s2.support_vectors()
     s2.close()
+    print("case er20000 one kernel per phase", flush=True)
+    s3 = Solver(p_pr, block_phases=1)
+    s3.run()
+    s3.sync()
+    sel3 = s3.selected()
+    assert np.array_equal(s3.support_vectors(), S)
+    s3.close()
     assert sel1.size == N and np.unique(sel1).size == N
     assert np.array_equal(sel1, sel2)
+    assert np.array_equal(sel1, sel3)
     # cycle weights are non-decreasing in rank; every chosen rank is a valid candidate
     assert sel1.max() < p_full.info.num_candidates
     # S_e is a unit vector plus lower coordinates only (triangular support)

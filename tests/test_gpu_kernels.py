@@ -94,15 +94,23 @@ def test_parity_reference_layout(name, chunk, multiple_transfer):
 PHASE_CASES = trace_names()
 
 
+# block_phases: 1 = one kernel per phase; 0 = automatic (64 phases per tree pass); 7 and 33 give
+# partial words and blocks that end mid-word
+BLOCKS = (1, 0, 7, 33)
+
+
+@pytest.mark.parametrize("block", BLOCKS)
 @pytest.mark.parametrize("prune", [False, True])
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("name", PHASE_CASES)
-def test_phase_loop_matches_oracle(name, mode, prune):
+def test_phase_loop_matches_oracle(name, mode, prune, block):
     fx = load_trace(name)
     p = problem_of(fx, mode, keep_vertex_trees=True, prune_trees=prune)
-    print(f"case {name} {mode} prune={prune} N={p.info.num_non_tree} K={p.info.num_candidates}", flush=True)
+    print(f"case {name} {mode} prune={prune} block={block} N={p.info.num_non_tree} K={p.info.num_candidates}",
+          flush=True)
     ref = pyoracle.run(p, want_s=True, want_c=True)
-    s = Solver(p, store_cycles=True)
+    s = Solver(p, store_cycles=True, block_phases=block)
+    assert s.launch_shape()["block_phases"] == (64 if block == 0 else block)
     s.run()
     s.sync()
     assert np.array_equal(s.selected(), ref["selected"])
@@ -116,10 +124,12 @@ def test_phase_loop_matches_oracle(name, mode, prune):
     s.close()
 
 
-def test_rerun_after_reset():
+@pytest.mark.parametrize("block", BLOCKS)
+def test_rerun_after_reset(block):
     fx = load_trace("grid16x16")
     p = problem_of(fx, "reduced", keep_vertex_trees=True)
-    s = Solver(p)
+    print(f"case rerun grid16x16 block={block}", flush=True)
+    s = Solver(p, block_phases=block)
     s.run()
     s.sync()
     first = s.selected().copy()
