@@ -93,6 +93,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="er20000", choices=sorted(WORKLOADS))
     ap.add_argument("--no-prune", action="store_true")
+    ap.add_argument("--breakdown", action="store_true",
+                    help="time every kernel (HIP events between all launches), not only the dominant one")
     ap.add_argument("--block-phases", type=int, default=0,
                     help="phases per pass over the trees (0 automatic, 1 one kernel per phase)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -127,7 +129,7 @@ def main() -> int:
           f"F={info.fvs_size} K={info.num_candidates} positions={info.positions} host={host_s:.2f}s",
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    solver = Solver(prob, timing=True, block_phases=args.block_phases)
+    solver = Solver(prob, timing=1 if args.breakdown else 2, block_phases=args.block_phases)
     upload_s = time.perf_counter() - t0
     dev = device_info()
     phases = args.phases if 0 < args.phases < info.num_non_tree else None
@@ -167,14 +169,15 @@ def main() -> int:
     # Algorithmic bytes of one launch of the dominant kernel.  One kernel per phase: 8 per tree
     # position streamed (u32 parent position + i32 coordinate), 16 per candidate read (u_pos, v_pos,
     # rank, coordinate) and the update's support-word reads and read-modify-writes as the kernel
-    # counts them.  Blocked: the tree pass streams every position (8 B) and every candidate (16 B
-    # read, 8 B candidate word written); the update is a kernel of its own, reported apart.
+    # counts them.  Blocked: the tree pass streams every position (8 B) and every candidate (16 B);
+    # its few independent candidate words are written sparsely and the update is a kernel of its
+    # own, reported apart.
     shape = solver.launch_shape()
     dominant = "k_phase" if shape["block_phases"] == 1 else "k_block_q"
     k_ms, k_n = kt["parity_search"]
     avg_ms = k_ms / max(k_n, 1)
     blocked = shape["block_phases"] > 1
-    cand_bytes = 24.0 if blocked else 16.0
+    cand_bytes = 16.0
     dom_bytes = 8.0 * pos + cand_bytes * cand + (0 if blocked else upd)
     bytes_per_launch = dom_bytes / max(k_n, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0

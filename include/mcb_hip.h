@@ -121,7 +121,8 @@ typedef struct {
     int32_t store_cycles;     /* 1: keep every phase's cycle vector (N*W words) for mcb_solver_cycle_vectors */
     int32_t waves_per_block;  /* parity/search kernel; 0 = automatic */
     int32_t max_blocks;       /* 0 = automatic (resident blocks on every compute unit) */
-    int32_t timing;           /* 1: time each kernel launch with HIP events on the solver's stream */
+    int32_t timing;           /* HIP events on the solver's stream around kernel launches: 0 none,
+                                 1 every kernel, 2 only MCB_K_PARITY_SEARCH (the dominant kernel) */
     int32_t block_phases;     /* phases resolved per pass over the trees: 0 = automatic (64 when the
                                  largest tree's 64-bit path words fit one workgroup's LDS, else 1),
                                  1 = one kernel per phase, 2..64 = blocked with that many */

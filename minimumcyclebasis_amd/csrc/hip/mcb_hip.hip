@@ -674,14 +674,13 @@ struct BlockArgs {
     uint64_t *Sw;
     int64_t npad;
     uint64_t *Tt;        // [npad] bit l of Tt[c] = bit c of T_l
-    uint64_t *Qr;        // [K]    bit l of Qr[k] = <C_k, T_l>
+    uint64_t *Qr;        // [K]    bit l of Qr[k] = <C_k, T_l>, valid where qbits is set
+    uint64_t *qbits;     // [ceil(K/64)] ranks whose word was emitted by the tree pass
     uint32_t *sum_cnt;   // [slots]      greedy bases: size
     uint32_t *sum_rank;  // [slots][64]  ranks, ascending
     uint64_t *sum_q;     // [slots][64]  their words
-    uint32_t *grp_count; // [groups]     members finished, per merge group
     int64_t lvl_slot[kMaxLevels];  // first slot of each level
     int64_t lvl_n[kMaxLevels];     // bases on each level
-    int64_t lvl_grp[kMaxLevels];   // first group counter of the groups merging level L into L+1
     int32_t levels;
     uint32_t *sel;       // [N] rank chosen in each phase
     uint64_t *blk_q;     // [64] word of phase i's winner
@@ -692,6 +691,12 @@ struct BlockArgs {
     uint32_t *Dcnt_blk;  // [64]
     uint64_t *Sf_blk;    // [64][W] final support vectors of the block
     uint64_t *C_store;   // optional [N][W]
+    uint64_t *Ct;        // [npad] bit i of Ct[c] = bit c of C_{e+i}
+    uint32_t *pw_list;   // [W] words below e where some C_{e+i} is nonzero
+    uint32_t *pw_off;    // [W+1] first pair of each listed word (prefix sums)
+    uint32_t *pw_cnt;    // [1] listed words
+    uint32_t *pair_i;    // [64*W] phase i of each (word, C_{e+i}) pair
+    uint64_t *pair_c;    // [64*W] that word of C_{e+i}
     uint32_t *status;
     unsigned long long *upd_bytes;  // accumulated algorithmic bytes of the update
     int64_t K;
@@ -714,6 +719,14 @@ __global__ __launch_bounds__(256) void k_block_transpose(BlockArgs a) {
     const int64_t c = w * kWave + lane;
     if (c >= a.e && c < static_cast<int64_t>(a.e) + a.kk) mine |= 1ull << (c - a.e);
     a.Tt[c] = mine;
+}
+
+// Value of a 64-bit register in lane l (l wave-uniform).
+__device__ __forceinline__ uint64_t rl64(uint64_t x, int l) {
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(x)), l));
+    const uint32_t hi =
+        static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(x >> 32)), l));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
 // Settles the lanes of one window whose parent lies in the same window: a lane takes its parent's
@@ -814,18 +827,64 @@ __device__ __forceinline__ void tree_words_coop(const uint32_t *pp, const int32_
     }
 }
 
-// Words of root r's candidates: Qr[rank] = word[u] ^ word[v] ^ Tt[own coordinate].
-__device__ __forceinline__ void root_words(const BlockArgs &a, uint32_t r, const uint64_t *Wd, int64_t first,
-                                           int64_t stride, int lane) {
+// Greedy insertion of one batch (lane i: word q, ranks ascending with the lane; q = 0 for no entry)
+// into a wave basis (lane b: pivot whose lowest set bit is b; P: the pivot columns).  The batch is
+// first reduced by the existing pivots lane by lane; the lanes left nonzero are resolved column by
+// column, the first lane with a column's bit becoming its pivot and the bit being cleared from the
+// lanes after it.  The lanes that become pivots are exactly the batch vectors independent of the
+// basis and of the batch vectors before them; emit(lane) is called in the lane that becomes one,
+// in rank order.  Returns the number of new pivots (wave-uniform).
+template <typename EmitFn>
+__device__ __forceinline__ int basis_reduce_insert(uint64_t &piv, uint64_t &P, uint64_t q, int lane, EmitFn emit) {
+    uint64_t v = q;
+    while (__ballot((v & P) != 0)) {
+        const uint64_t h = v & P;
+        const int col = h ? __ffsll(static_cast<unsigned long long>(h)) - 1 : 0;
+        const uint64_t pc = __shfl(piv, col);
+        if (h) v ^= pc;
+    }
+    int added = 0;
+    uint64_t nz = __ballot(v != 0);
+    while (nz) {
+        uint64_t orv = v;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) orv |= __shfl_xor(orv, o);
+        const int col = __ffsll(static_cast<unsigned long long>(orv)) - 1;
+        const uint64_t m = __ballot((v >> col) & 1ull);
+        const int p = __ffsll(static_cast<unsigned long long>(m)) - 1;
+        const uint64_t pvv = rl64(v, p);
+        if (lane == col) piv = pvv;
+        P |= 1ull << col;
+        ++added;
+        if (lane > p && ((v >> col) & 1ull)) v ^= pvv;
+        if (lane == p) {
+            emit();
+            v = 0;
+        }
+        nz = __ballot(v != 0);
+    }
+    return added;
+}
+
+// Candidate words of one rank-ascending run of a root's candidates, kept only where they are
+// independent of the run's earlier words: word = path word[u] ^ path word[v] ^ Tt[own coordinate].
+// The wave keeps the run's greedy basis (lane b: pivot with lowest set bit b; P: its columns).  A
+// batch is first reduced by the existing pivots lane by lane; the lanes left nonzero are resolved
+// column by column, the first lane with a column's bit becoming its pivot.  Each new pivot's rank
+// and original word go to Qr/qbits.  A word dependent on earlier words of the same run is dependent
+// on earlier candidates overall, so it cannot be in the block's greedy basis and is dropped.
+__device__ __forceinline__ void run_words(const BlockArgs &a, const uint64_t *Wd, int64_t k_begin, int64_t k_end,
+                                          int lane) {
     constexpr int kB = 4;
-    const int64_t ce = a.rc_off[r + 1];
-    for (int64_t k0 = a.rc_off[r] + first; k0 < ce; k0 += kB * stride) {
+    uint64_t piv = 0, P = 0;
+    int npiv = 0;
+    for (int64_t k0 = k_begin; k0 < k_end && npiv < a.kk; k0 += kB * kWave) {
         uint32_t pu[kB], pv[kB], rk[kB];
         int32_t c[kB];
 #pragma unroll
         for (int b = 0; b < kB; ++b) {
-            const int64_t k = k0 + b * stride + lane;
-            const int64_t at = k < ce ? k : a.rc_off[r];
+            const int64_t k = k0 + b * kWave + lane;
+            const int64_t at = k < k_end ? k : k_begin;
             pu[b] = a.rc_pu[at];
             pv[b] = a.rc_pv[at];
             rk[b] = a.rc_rank[at];
@@ -833,12 +892,16 @@ __device__ __forceinline__ void root_words(const BlockArgs &a, uint32_t r, const
         }
 #pragma unroll
         for (int b = 0; b < kB; ++b) {
-            const int64_t k = k0 + b * stride + lane;
-            if (k < ce) {
-                uint64_t q = Wd[pu[b]] ^ Wd[pv[b]];
+            const int64_t k = k0 + b * kWave + lane;
+            uint64_t q = 0;
+            if (k < k_end) {
+                q = Wd[pu[b]] ^ Wd[pv[b]];
                 if (c[b] >= 0) q ^= a.Tt[c[b]];
-                a.Qr[rk[b]] = q;
             }
+            npiv += basis_reduce_insert(piv, P, q, lane, [&]() {
+                a.Qr[rk[b]] = q;
+                atomicOr(reinterpret_cast<unsigned long long *>(a.qbits + (rk[b] >> 6)), 1ull << (rk[b] & 63));
+            });
         }
     }
 }
@@ -864,7 +927,11 @@ __global__ __launch_bounds__(kQWaves * kWave) void k_block_q(BlockArgs a) {
         ++gen;
         tree_words_coop(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wbig, ready, gen, wid, nw, lane);
         __syncthreads();
-        root_words(a, r, Wbig, static_cast<int64_t>(wid) * kWave, static_cast<int64_t>(nw) * kWave, lane);
+        // each wave takes one contiguous run of the root's candidates
+        const int64_t cb = a.rc_off[r], ce = a.rc_off[r + 1];
+        const int64_t seg = (ce - cb + nw - 1) / nw;
+        const int64_t s0 = cb + seg * wid, s1 = s0 + seg < ce ? s0 + seg : ce;
+        if (s0 < s1) run_words(a, Wbig, s0, s1, lane);
         __syncthreads();
     }
     // the big-tree region is free from here on; each wave takes its own slice for small roots
@@ -874,39 +941,61 @@ __global__ __launch_bounds__(kQWaves * kWave) void k_block_q(BlockArgs a) {
         const int64_t off = a.tree_off[r];
         const uint32_t npos = static_cast<uint32_t>(a.tree_off[r + 1] - off);
         tree_words(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Ws, lane);
-        root_words(a, r, Ws, 0, kWave, lane);
+        run_words(a, Ws, a.rc_off[r], a.rc_off[r + 1], lane);
     }
 }
 
-// Greedy insertion of one batch (lane i: word q and rank, ranks ascending with the lane) into a
-// wave's basis, where lane l holds the pivot whose lowest set bit is l.  Each batch vector that is
-// independent of the basis and of the batch vectors before it is appended to (out_rank, out_q);
-// cnt (wave-uniform) stops at kk.
-__device__ __forceinline__ void basis_insert(uint64_t &piv, uint32_t &cnt, uint64_t q, uint32_t rank, int kk,
-                                             uint32_t *out_rank, uint64_t *out_q, int lane) {
+// Inserts one batch (lane i: word q, ranks ascending with the lane; q = 0 for no entry) into a
+// wave's basis held as piv (lane b: the pivot whose lowest set bit is b, or 0), column by column:
+// a column that has a pivot is cleared from every lane; otherwise the first lane with that bit
+// becomes its pivot and the bit is cleared from the lanes after it.  The lanes that become pivots
+// are exactly the batch vectors independent of the basis and of the batch vectors before them, and
+// the column at which a vector becomes a pivot is the lowest set bit it has after elimination by
+// everything before it in rank order.  Returns the mask of new pivot lanes; *col (if given) receives,
+// in each such lane, its column.
+__device__ __forceinline__ uint64_t basis_columns(uint64_t &piv, uint64_t q, int kk, int lane, int *col) {
     uint64_t v = q;
-    while (cnt < static_cast<uint32_t>(kk)) {
-        for (;;) {
-            const int lb = v ? __ffsll(static_cast<unsigned long long>(v)) - 1 : 0;
-            const uint64_t p = __shfl(piv, lb);
-            const bool red = v != 0 && p != 0;
-            if (!__ballot(red)) break;
-            if (red) v ^= p;
+    uint64_t newp = 0;
+    for (int b = 0; b < kk; ++b) {
+        const uint64_t pb = rl64(piv, b);
+        const bool has = (v >> b) & 1ull;
+        if (pb) {
+            if (has) v ^= pb;
+        } else {
+            const uint64_t m = __ballot(has);
+            if (m) {
+                const int p = __ffsll(static_cast<unsigned long long>(m)) - 1;
+                const uint64_t pv = rl64(v, p);
+                if (lane == b) piv = pv;
+                if (lane > p && has) v ^= pv;
+                if (lane == p) {
+                    v = 0;
+                    if (col) *col = b;
+                }
+                newp |= 1ull << p;
+            }
         }
-        const uint64_t cand = __ballot(v != 0);
-        if (!cand) break;
-        const int f = __ffsll(static_cast<unsigned long long>(cand)) - 1;
-        const uint64_t vf = __shfl(v, f);
-        const int lbf = __ffsll(static_cast<unsigned long long>(vf)) - 1;
-        if (lane == lbf) piv = vf;
-        const uint64_t qf = __shfl(q, f);
-        const uint32_t rf = __shfl(rank, f);
+    }
+    return newp;
+}
+
+// Greedy insertion of one batch into a wave basis; the new independent vectors (original word and
+// rank) are appended in rank order to (out_rank, out_q); cnt (wave-uniform) stops at kk.
+__device__ __forceinline__ void basis_insert(uint64_t &piv, uint64_t &P, uint32_t &cnt, uint64_t q, uint32_t rank,
+                                             uint32_t *out_rank, uint64_t *out_q, int lane) {
+    uint64_t mine = 0;
+    basis_reduce_insert(piv, P, q, lane, [&]() { mine = 1; });
+    uint64_t newp = __ballot(mine != 0);
+    while (newp) {
+        const int p = __ffsll(static_cast<unsigned long long>(newp)) - 1;
+        newp &= newp - 1;
+        const uint64_t qp = rl64(q, p);
+        const uint32_t rp = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rank), p));
         if (lane == 0) {
-            out_rank[cnt] = rf;
-            out_q[cnt] = qf;
+            out_rank[cnt] = rp;
+            out_q[cnt] = qp;
         }
         ++cnt;
-        if (lane == f) v = 0;
     }
 }
 
@@ -920,34 +1009,19 @@ __device__ void block_finalize(const BlockArgs &a, int lane) {
     if (ok) {
         const uint64_t q = lane < kk ? a.sum_q[top * kWave + lane] : 0ull;
         const uint32_t rk = lane < kk ? a.sum_rank[top * kWave + lane] : 0u;
-        // phase of each winner: lowest set bit at which it enters the basis built in rank order
-        uint64_t piv = 0, v = q;
+        // phase of each winner: the column at which it becomes a pivot in rank order
+        uint64_t piv = 0;
         int phase_of = -1;
-        for (int f = 0; f < kk; ++f) {
-            for (;;) {
-                const int lb = v ? __ffsll(static_cast<unsigned long long>(v)) - 1 : 0;
-                const uint64_t p = __shfl(piv, lb);
-                const bool red = lane >= f && v != 0 && p != 0;
-                if (!__ballot(red)) break;
-                if (red) v ^= p;
-            }
-            const uint64_t vf = __shfl(v, f);
-            if (vf == 0) {
-                ok = false;
-                break;
-            }
-            const int lbf = __ffsll(static_cast<unsigned long long>(vf)) - 1;
-            if (lane == lbf) piv = vf;
-            if (lane == f) phase_of = lbf;
-        }
+        const uint64_t all = basis_columns(piv, q, kk, lane, &phase_of);
+        if (all != (kk == kWave ? ~0ull : ((1ull << kk) - 1))) ok = false;
         if (ok) {
             uint64_t qi = 0;
             uint32_t ri = 0;
             for (int i = 0; i < kk; ++i) {
                 const uint64_t b = __ballot(phase_of == i);
                 const int f = b ? __ffsll(static_cast<unsigned long long>(b)) - 1 : 0;
-                const uint64_t qf = __shfl(q, f);
-                const uint32_t rf = __shfl(rk, f);
+                const uint64_t qf = rl64(q, f);
+                const uint32_t rf = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rk), f));
                 if (lane == i) {
                     qi = qf;
                     ri = rf;
@@ -956,13 +1030,13 @@ __device__ void block_finalize(const BlockArgs &a, int lane) {
             // masks: after phase i, every later mask with an odd product with the winner takes m_i
             uint64_t m = lane < kk ? (1ull << lane) : 0ull;
             for (int i = 0; i < kk; ++i) {
-                const uint64_t qw = __shfl(qi, i);
-                const uint64_t mi = __shfl(m, i);
+                const uint64_t qw = rl64(qi, i);
+                const uint64_t mi = rl64(m, i);
                 if (lane > i && lane < kk && (__popcll(m & qw) & 1)) m ^= mi;
             }
             uint64_t g = 0;
             for (int l = 0; l < kk; ++l) {
-                const uint64_t ml = __shfl(m, l);
+                const uint64_t ml = rl64(m, l);
                 if (l < lane) g |= static_cast<uint64_t>(__popcll(ml & qi) & 1) << l;
             }
             const bool odd = lane >= kk || (__popcll(m & qi) & 1);
@@ -979,13 +1053,10 @@ __device__ void block_finalize(const BlockArgs &a, int lane) {
         if (lane == 0) atomicOr(a.status, 1u);
         if (lane < kk) a.sel[a.e + lane] = kNoRank;
     }
-    // every group has finished: clear the counters for the next block
-    for (int64_t i = lane; i < a.lvl_grp[a.levels - 1]; i += kWave) a.grp_count[i] = 0;
 }
 
-// Greedy basis of the block.  Wave c takes ranks [c*kChunk, (c+1)*kChunk); the last wave to finish
-// in each group of kGroup merges the group's bases into the next level, and the wave that builds
-// the single top basis finalises the block.
+// First level of the block's greedy basis: wave c, ranks [c*kChunk, (c+1)*kChunk) -> slot c; only
+// the words the tree pass emitted take part (the others cannot be in the basis).
 __global__ __launch_bounds__(256) void k_block_basis(BlockArgs a) {
     constexpr int kBatches = kChunk / kWave;
     const int lane = threadIdx.x & (kWave - 1);
@@ -993,52 +1064,68 @@ __global__ __launch_bounds__(256) void k_block_basis(BlockArgs a) {
     if (c >= a.lvl_n[0]) return;
     const int64_t r0 = c * kChunk;
     const int64_t r1 = r0 + kChunk < a.K ? r0 + kChunk : a.K;
-    uint64_t q[kBatches];
+    uint64_t bits[kBatches], q[kBatches];
 #pragma unroll
     for (int u = 0; u < kBatches; ++u) {
-        const int64_t k = r0 + u * kWave + lane;
-        q[u] = a.Qr[k < r1 ? k : r0];
-        if (k >= r1) q[u] = 0;
+        const int64_t wd = (r0 >> 6) + u;
+        bits[u] = r0 + u * kWave < r1 ? a.qbits[wd] : 0ull;
     }
-    uint64_t piv = 0;
+#pragma unroll
+    for (int u = 0; u < kBatches; ++u) {
+        const bool on = (bits[u] >> lane) & 1ull;
+        q[u] = on ? a.Qr[r0 + u * kWave + lane] : 0ull;
+    }
+    if (lane < kBatches && r0 + lane * kWave < r1) a.qbits[(r0 >> 6) + lane] = 0;  // ready for the next block
+    uint64_t piv = 0, P = 0;
     uint32_t cnt = 0;
-    int64_t slot = a.lvl_slot[0] + c;
+    const int64_t slot = a.lvl_slot[0] + c;
 #pragma unroll
     for (int u = 0; u < kBatches; ++u) {
         if (cnt >= static_cast<uint32_t>(a.kk) || !__ballot(q[u] != 0)) continue;
-        basis_insert(piv, cnt, q[u], static_cast<uint32_t>(r0 + u * kWave + lane), a.kk, a.sum_rank + slot * kWave,
+        basis_insert(piv, P, cnt, q[u], static_cast<uint32_t>(r0 + u * kWave + lane), a.sum_rank + slot * kWave,
                      a.sum_q + slot * kWave, lane);
     }
     if (lane == 0) a.sum_cnt[slot] = cnt;
-    int64_t idx = c;
-    for (int L = 0; L + 1 < a.levels; ++L) {
-        const int64_t g = idx / kGroup;
-        const int64_t rest = a.lvl_n[L] - g * kGroup;
-        const uint32_t members = static_cast<uint32_t>(rest < kGroup ? rest : kGroup);
-        uint32_t last = 0;
-        if (lane == 0) {
-            __threadfence();
-            last = atomicAdd(a.grp_count + a.lvl_grp[L] + g, 1u) == members - 1 ? 1u : 0u;
-        }
-        last = __shfl(last, 0);
-        if (!last) return;
+    if (a.levels == 1) {
         __threadfence();
-        piv = 0;
-        cnt = 0;
-        slot = a.lvl_slot[L + 1] + g;
-        for (uint32_t m = 0; m < members && cnt < static_cast<uint32_t>(a.kk); ++m) {
-            const int64_t s = a.lvl_slot[L] + g * kGroup + m;
-            const uint32_t n = __hip_atomic_load(a.sum_cnt + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (n == 0) continue;
-            const uint64_t qq = lane < static_cast<int>(n) ? a.sum_q[s * kWave + lane] : 0ull;
-            const uint32_t rr = lane < static_cast<int>(n) ? a.sum_rank[s * kWave + lane] : 0u;
-            basis_insert(piv, cnt, qq, rr, a.kk, a.sum_rank + slot * kWave, a.sum_q + slot * kWave, lane);
-        }
-        if (lane == 0) a.sum_cnt[slot] = cnt;
-        idx = g;
+        block_finalize(a, lane);
     }
-    __threadfence();
-    block_finalize(a, lane);
+}
+
+// One merge level: wave g merges bases g*kGroup .. g*kGroup+kGroup-1 of level L (in rank order) into
+// slot g of level L+1; at the top level the single wave also finalises the block.  The bases of
+// level L were written by the previous launch.
+__global__ __launch_bounds__(256) void k_block_merge(BlockArgs a, int L) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * 4 + threadIdx.x / kWave;
+    if (g >= a.lvl_n[L + 1]) return;
+    const int64_t rest = a.lvl_n[L] - g * kGroup;
+    const int members = static_cast<int>(rest < kGroup ? rest : kGroup);
+    const int64_t s0 = a.lvl_slot[L] + g * kGroup;
+    const uint32_t my_n = lane < members ? a.sum_cnt[s0 + lane] : 0u;
+    uint64_t qq[kGroup];
+    uint32_t rr[kGroup];
+#pragma unroll
+    for (int m = 0; m < kGroup; ++m) {
+        const int64_t s = s0 + (m < members ? m : 0);
+        qq[m] = a.sum_q[s * kWave + lane];
+        rr[m] = a.sum_rank[s * kWave + lane];
+    }
+    uint64_t piv = 0, P = 0;
+    uint32_t cnt = 0;
+    const int64_t slot = a.lvl_slot[L + 1] + g;
+#pragma unroll
+    for (int m = 0; m < kGroup; ++m) {
+        const uint32_t n = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(my_n), m));
+        if (m >= members || n == 0 || cnt >= static_cast<uint32_t>(a.kk)) continue;
+        basis_insert(piv, P, cnt, lane < static_cast<int>(n) ? qq[m] : 0ull, rr[m], a.sum_rank + slot * kWave,
+                     a.sum_q + slot * kWave, lane);
+    }
+    if (lane == 0) a.sum_cnt[slot] = cnt;
+    if (L + 2 == a.levels) {
+        __threadfence();
+        block_finalize(a, lane);
+    }
 }
 
 // C of one chosen candidate into LDS (W words, cleared by the caller): the coordinates of the edges
@@ -1117,37 +1204,109 @@ __global__ __launch_bounds__(256) void k_block_cycles(BlockArgs a) {
     if (tid == 0) a.Dcnt_blk[i] = cnt;
 }
 
-// Thread j: column e+i < e+kk receives its final vector S_{e+i}; column j >= e+kk takes the
-// block's update: d_i = <C_{e+i}, S_j>, b_i = d_i ^ sum_{l<i} b_l <C_{e+i}, S_{e+l}>,
-// S_j ^= sum_i b_i S_{e+i}.  S_j has support below e besides its own bit, so the dot products
-// read only the words where some C_{e+i} is nonzero below e, plus bit j of each C.
-// LDS: G rows [64] | word masks U [dlim] | word list [dlim] | S tile [64][kTile].
-__global__ __launch_bounds__(256) void k_block_update(BlockArgs a) {
+// Inputs of the update that every column shares.  Workgroup 0 lists the words below e where some
+// C_{e+i} is nonzero, each with its (i, word of C_{e+i}) pairs; the other workgroups transpose the
+// block's cycle vectors (one wave per word: Ct[64w + b] bit i = bit b of word w of C_{e+i}).
+__global__ __launch_bounds__(256) void k_block_pairs(BlockArgs a) {
     extern __shared__ uint64_t lds[];
-    __shared__ uint32_t nlist;
+    __shared__ uint32_t nlist, npairs;
     const int tid = threadIdx.x;
-    const int kk = a.kk;
+    if (blockIdx.x > 0) {  // transpose
+        const int lane = tid & (kWave - 1);
+        const int64_t w = static_cast<int64_t>(blockIdx.x - 1) * 4 + tid / kWave;
+        if (w >= a.W) return;
+        const uint64_t x = lane < a.kk ? a.C_blk[static_cast<int64_t>(lane) * a.W + w] : 0ull;
+        uint64_t mine = 0;
+        for (int b = 0; b < kWave; ++b) {
+            const uint64_t m = __ballot(static_cast<int>((x >> b) & 1ull));
+            if (lane == b) mine = m;
+        }
+        a.Ct[w * kWave + lane] = mine;
+        return;
+    }
+    // U[w]: bit i set when C_{e+i} is nonzero at word w (w below e); then the listed words in
+    // ascending order with the prefix sums of their pair counts, so that the pairs of consecutive
+    // listed words are contiguous.
     const int32_t dlim = (a.e + kWave - 1) / kWave;
-    const int32_t wl = (a.e + kk + kWave - 1) / kWave;
-    uint64_t *G = lds;
-    uint64_t *U = G + kBlockK;
-    uint32_t *wlist = reinterpret_cast<uint32_t *>(U + dlim);
-    uint64_t *T = U + dlim + (dlim + 1) / 2;
-    for (int x = tid; x < kBlockK; x += blockDim.x) G[x] = x < kk ? a.blk_g[x] : 0ull;
+    uint64_t *U = lds;
+    uint32_t *part = reinterpret_cast<uint32_t *>(U + dlim);  // [2 * blockDim.x]: words, pairs per thread
     for (int x = tid; x < dlim; x += blockDim.x) U[x] = 0;
-    if (tid == 0) nlist = 0;
     __syncthreads();
-    for (int i = 0; i < kk; ++i) {
+    for (int i = 0; i < a.kk; ++i) {
         const uint32_t nd = a.Dcnt_blk[i];
         const uint32_t *Di = a.D_blk + static_cast<int64_t>(i) * a.W;
         for (uint32_t q = tid; q < nd; q += blockDim.x)
             atomicOr(reinterpret_cast<unsigned long long *>(&U[Di[q]]), 1ull << i);
     }
     __syncthreads();
-    for (int x = tid; x < dlim; x += blockDim.x)
-        if (U[x]) wlist[atomicAdd(&nlist, 1u)] = static_cast<uint32_t>(x);
+    const int per = (dlim + static_cast<int>(blockDim.x) - 1) / static_cast<int>(blockDim.x);
+    const int w0 = tid * per, w1 = w0 + per < dlim ? w0 + per : dlim;
+    uint32_t nw = 0, np = 0;
+    for (int w = w0; w < w1; ++w)
+        if (U[w]) {
+            ++nw;
+            np += static_cast<uint32_t>(__popcll(U[w]));
+        }
+    part[tid] = nw;
+    part[blockDim.x + tid] = np;
     __syncthreads();
-    const uint32_t nl = nlist;
+    if (tid == 0) {
+        uint32_t sw = 0, sp = 0;
+        for (unsigned t = 0; t < blockDim.x; ++t) {
+            const uint32_t a1 = part[t], a2 = part[blockDim.x + t];
+            part[t] = sw;
+            part[blockDim.x + t] = sp;
+            sw += a1;
+            sp += a2;
+        }
+        nlist = sw;
+        npairs = sp;
+    }
+    __syncthreads();
+    uint32_t x = part[tid], off = part[blockDim.x + tid];
+    for (int w = w0; w < w1; ++w) {
+        uint64_t m = U[w];
+        if (!m) continue;
+        a.pw_list[x] = static_cast<uint32_t>(w);
+        a.pw_off[x] = off;
+        ++x;
+        while (m) {
+            const int i = __ffsll(static_cast<unsigned long long>(m)) - 1;
+            m &= m - 1;
+            a.pair_i[off] = static_cast<uint32_t>(i);
+            a.pair_c[off] = a.C_blk[static_cast<int64_t>(i) * a.W + w];
+            ++off;
+        }
+    }
+    if (tid == 0) {
+        a.pw_off[nlist] = npairs;
+        *a.pw_cnt = nlist;
+    }
+}
+
+// Thread j: column e+i < e+kk receives its final vector S_{e+i}; column j >= e+kk takes the
+// block's update: d_i = <C_{e+i}, S_j>, b_i = d_i ^ sum_{l<i} b_l <C_{e+i}, S_{e+l}>,
+// S_j ^= sum_i b_i S_{e+i}.  S_j has support below e besides its own bit, so the dot products
+// read only the words where some C_{e+i} is nonzero below e, plus bit j of each C.
+// The word list and its pairs are staged in LDS in segments of at most kSegWords words and
+// kSegPairs pairs.  LDS: G rows [64] | S tile [64][kTile] | pair words [kSegPairs] | pair phases
+// [kSegPairs] | listed words [kSegWords] | their pair offsets [kSegWords + 1].
+constexpr int kSegWords = 1024;
+constexpr int kSegPairs = 2048;
+__global__ __launch_bounds__(256) void k_block_update(BlockArgs a) {
+    extern __shared__ uint64_t lds[];
+    __shared__ uint32_t seg_end;
+    constexpr int kB = 8;
+    const int tid = threadIdx.x;
+    const int kk = a.kk;
+    const int32_t wl = (a.e + kk + kWave - 1) / kWave;
+    uint64_t *G = lds;
+    uint64_t *T = lds + kBlockK;
+    uint64_t *Pc = T + kBlockK * kTile;
+    uint32_t *Pi = reinterpret_cast<uint32_t *>(Pc + kSegPairs);
+    uint32_t *Lw = Pi + kSegPairs;
+    uint32_t *Lo = Lw + kSegWords;
+    for (int x = tid; x < kBlockK; x += blockDim.x) G[x] = x < kk ? a.blk_g[x] : 0ull;
     const int64_t j = static_cast<int64_t>(a.e) + static_cast<int64_t>(blockIdx.x) * blockDim.x + tid;
     const bool own_col = j < static_cast<int64_t>(a.e) + kk;
     const bool upd_col = !own_col && j < a.N;
@@ -1161,20 +1320,54 @@ __global__ __launch_bounds__(256) void k_block_update(BlockArgs a) {
             if ((j >> 6) == w) v &= ~(1ull << (j & 63));
             a.Sw[static_cast<int64_t>(w) * a.npad + j] = v;
         }
-    } else if (upd_col) {
-        uint64_t d = 0;
-        for (int i = 0; i < kk; ++i)
-            d |= ((a.C_blk[static_cast<int64_t>(i) * a.W + (j >> 6)] >> (j & 63)) & 1ull) << i;
-        for (uint32_t x = 0; x < nl; ++x) {
-            const uint32_t w = wlist[x];
-            const uint64_t s = a.Sw[static_cast<int64_t>(w) * a.npad + j];
-            uint64_t mm = U[w];
-            while (mm) {
-                const int i = __ffsll(static_cast<unsigned long long>(mm)) - 1;
-                mm &= mm - 1;
-                d ^= static_cast<uint64_t>(__popcll(a.C_blk[static_cast<int64_t>(i) * a.W + w] & s) & 1) << i;
+    }
+    uint64_t d = upd_col ? a.Ct[j] : 0ull;
+    const uint32_t nl = *a.pw_cnt;
+    for (uint32_t s0 = 0; s0 < nl;) {
+        __syncthreads();
+        if (tid == 0) {  // the longest run of listed words from s0 whose pairs fit
+            const uint32_t p0 = a.pw_off[s0];
+            uint32_t lo = s0 + 1, hi = s0 + kSegWords < nl ? s0 + kSegWords : nl;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) / 2;
+                if (a.pw_off[mid] - p0 <= static_cast<uint32_t>(kSegPairs)) lo = mid;
+                else hi = mid - 1;
+            }
+            seg_end = lo;
+        }
+        __syncthreads();
+        const uint32_t s1 = seg_end;
+        const uint32_t p0 = a.pw_off[s0], p1 = a.pw_off[s1];
+        for (uint32_t x = tid; x < s1 - s0; x += blockDim.x) {
+            Lw[x] = a.pw_list[s0 + x];
+            Lo[x] = a.pw_off[s0 + x] - p0;
+        }
+        if (tid == 0) Lo[s1 - s0] = p1 - p0;
+        for (uint32_t q = tid; q < p1 - p0; q += blockDim.x) {
+            Pc[q] = a.pair_c[p0 + q];
+            Pi[q] = a.pair_i[p0 + q];
+        }
+        __syncthreads();
+        if (upd_col) {
+            for (uint32_t x0 = 0; x0 < s1 - s0; x0 += kB) {
+                uint64_t sv[kB];
+#pragma unroll
+                for (int k = 0; k < kB; ++k) {
+                    const uint32_t x = x0 + k < s1 - s0 ? x0 + k : x0;
+                    sv[k] = a.Sw[static_cast<int64_t>(Lw[x]) * a.npad + j];
+                }
+#pragma unroll
+                for (int k = 0; k < kB; ++k) {
+                    if (x0 + k >= s1 - s0) break;
+                    for (uint32_t q = Lo[x0 + k]; q < Lo[x0 + k + 1]; ++q)
+                        d ^= static_cast<uint64_t>(__popcll(Pc[q] & sv[k]) & 1) << Pi[q];
+                }
             }
         }
+        s0 = s1;
+    }
+    __syncthreads();
+    if (upd_col) {
         for (int i = 0; i < kk; ++i) {
             const uint64_t bi = ((d >> i) ^ static_cast<uint64_t>(__popcll(b & G[i]))) & 1ull;
             b |= bi << i;
@@ -1183,16 +1376,17 @@ __global__ __launch_bounds__(256) void k_block_update(BlockArgs a) {
     }
     for (int32_t w0 = 0; w0 < wl; w0 += kTile) {
         __syncthreads();
-        for (int x = tid; x < kk * kTile; x += blockDim.x) {
+        for (int x = tid; x < kBlockK * kTile; x += blockDim.x) {
             const int i = x / kTile, t = x % kTile;
-            T[x] = w0 + t < wl ? a.Sf_blk[static_cast<int64_t>(i) * a.W + w0 + t] : 0ull;
+            T[x] = (i < kk && w0 + t < wl) ? a.Sf_blk[static_cast<int64_t>(i) * a.W + w0 + t] : 0ull;
         }
         __syncthreads();
         if (b) {
             const int tn = wl - w0 < kTile ? wl - w0 : kTile;
             for (int t = 0; t < tn; ++t) {
                 uint64_t x = 0;
-                for (int i = 0; i < kk; ++i) x ^= T[i * kTile + t] & (0ull - ((b >> i) & 1ull));
+#pragma unroll
+                for (int i = 0; i < kBlockK; ++i) x ^= T[i * kTile + t] & (0ull - ((b >> i) & 1ull));
                 if (x) {
                     a.Sw[static_cast<int64_t>(w0 + t) * a.npad + j] ^= x;
                     nbytes += 16;
@@ -1542,12 +1736,13 @@ struct mcb_solver {
     int32_t bk = 1;                                   // phases per block; 1 = one kernel per phase
     int32_t q_big_words = 0, q_small_words = 0, q_blocks = 0, q_lds = 0, n_qbig = 0, n_qsmall = 0;
     uint32_t *q_big_list = nullptr, *q_small_list = nullptr;
-    uint64_t *Tt = nullptr, *Qr = nullptr, *sum_q = nullptr, *blk_q = nullptr, *blk_m = nullptr, *blk_g = nullptr;
-    uint64_t *C_blk = nullptr, *Sf_blk = nullptr;
-    uint32_t *sum_cnt = nullptr, *sum_rank = nullptr, *grp_count = nullptr, *D_blk = nullptr, *Dcnt_blk = nullptr;
+    uint64_t *Tt = nullptr, *Qr = nullptr, *qbits = nullptr, *sum_q = nullptr, *blk_q = nullptr, *blk_m = nullptr, *blk_g = nullptr;
+    uint64_t *C_blk = nullptr, *Sf_blk = nullptr, *Ct = nullptr, *pair_c = nullptr;
+    uint32_t *pw_list = nullptr, *pw_off = nullptr, *pw_cnt = nullptr, *pair_i = nullptr;
+    uint32_t *sum_cnt = nullptr, *sum_rank = nullptr, *D_blk = nullptr, *Dcnt_blk = nullptr;
     unsigned long long *upd_bytes = nullptr;
     int32_t levels = 0;
-    int64_t lvl_slot[kMaxLevels] = {}, lvl_n[kMaxLevels] = {}, lvl_grp[kMaxLevels] = {};
+    int64_t lvl_slot[kMaxLevels] = {}, lvl_n[kMaxLevels] = {};
     int64_t blk_positions = 0, blk_candidates = 0;  // work of the tree passes since reset
     // timing
     double kt_ms[MCB_K_COUNT] = {};
@@ -1555,6 +1750,7 @@ struct mcb_solver {
     std::vector<hipEvent_t> ev_pool;
     std::vector<int> ev_kind;
     size_t ev_used = 0;
+    bool timing_open = false;
 };
 
 namespace {
@@ -1573,7 +1769,8 @@ int harvest_events(mcb_solver *s) {
 }
 
 int timed_begin(mcb_solver *s, int kind) {
-    if (!s->opt.timing) return MCB_OK;
+    s->timing_open = s->opt.timing == 1 || (s->opt.timing == 2 && kind == MCB_K_PARITY_SEARCH);
+    if (!s->timing_open) return MCB_OK;
     if (s->ev_used + 2 > s->ev_pool.size()) {
         int rc = harvest_events(s);
         if (rc) return rc;
@@ -1584,7 +1781,8 @@ int timed_begin(mcb_solver *s, int kind) {
 }
 
 int timed_end(mcb_solver *s) {
-    if (!s->opt.timing) return MCB_OK;
+    if (!s->timing_open) return MCB_OK;
+    s->timing_open = false;
     HIP_TRY(hipEventRecord(s->ev_pool[s->ev_used++], s->stream));
     return MCB_OK;
 }
@@ -1667,14 +1865,13 @@ BlockArgs block_args(const mcb_solver *s, int32_t e, int32_t kk) {
     a.npad = s->npad;
     a.Tt = s->Tt;
     a.Qr = s->Qr;
+    a.qbits = s->qbits;
     a.sum_cnt = s->sum_cnt;
     a.sum_rank = s->sum_rank;
     a.sum_q = s->sum_q;
-    a.grp_count = s->grp_count;
     for (int L = 0; L < kMaxLevels; ++L) {
         a.lvl_slot[L] = s->lvl_slot[L];
         a.lvl_n[L] = s->lvl_n[L];
-        a.lvl_grp[L] = s->lvl_grp[L];
     }
     a.levels = s->levels;
     a.sel = s->best;
@@ -1686,6 +1883,12 @@ BlockArgs block_args(const mcb_solver *s, int32_t e, int32_t kk) {
     a.Dcnt_blk = s->Dcnt_blk;
     a.Sf_blk = s->Sf_blk;
     a.C_store = s->C_store;
+    a.Ct = s->Ct;
+    a.pw_list = s->pw_list;
+    a.pw_off = s->pw_off;
+    a.pw_cnt = s->pw_cnt;
+    a.pair_i = s->pair_i;
+    a.pair_c = s->pair_c;
     a.status = s->status;
     a.upd_bytes = s->upd_bytes;
     a.K = s->K;
@@ -1714,12 +1917,21 @@ int launch_block(mcb_solver *s, int32_t e, int32_t kk) {
     if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_SELECT))) return rc;
     hipLaunchKernelGGL(k_block_basis, dim3(static_cast<unsigned>((s->lvl_n[0] + 3) / 4)), dim3(256), 0, s->stream, a);
     HIP_TRY(hipGetLastError());
+    for (int L = 0; L + 1 < s->levels; ++L) {
+        hipLaunchKernelGGL(k_block_merge, dim3(static_cast<unsigned>((s->lvl_n[L + 1] + 3) / 4)), dim3(256), 0,
+                           s->stream, a, L);
+        HIP_TRY(hipGetLastError());
+    }
     if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_CYCLE))) return rc;
     hipLaunchKernelGGL(k_block_cycles, dim3(kk), dim3(256), sizeof(uint64_t) * s->W, s->stream, a);
     HIP_TRY(hipGetLastError());
-    if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_UPDATE))) return rc;
     const int32_t dlim = (e + kWave - 1) / kWave;
-    const size_t ulds = sizeof(uint64_t) * (kBlockK + dlim + (dlim + 1) / 2 + static_cast<size_t>(kBlockK) * kTile);
+    hipLaunchKernelGGL(k_block_pairs, dim3(static_cast<unsigned>(1 + (s->W + 3) / 4)), dim3(256),
+                       sizeof(uint64_t) * dlim + 2 * 256 * sizeof(uint32_t), s->stream, a);
+    HIP_TRY(hipGetLastError());
+    if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_UPDATE))) return rc;
+    const size_t ulds = sizeof(uint64_t) * (kBlockK + static_cast<size_t>(kBlockK) * kTile + kSegPairs) +
+                        sizeof(uint32_t) * (kSegPairs + 2 * kSegWords + 1);
     const int64_t cols = s->N - e;
     hipLaunchKernelGGL(k_block_update, dim3(static_cast<unsigned>((cols + 255) / 256)), dim3(256), ulds, s->stream, a);
     HIP_TRY(hipGetLastError());
@@ -1809,18 +2021,16 @@ int setup_blocked(mcb_solver *s, const std::vector<uint32_t> &order, NposFn npos
     per_cu = std::max(1, per_cu);
     s->q_blocks = per_cu * g_dev.prop.multiProcessorCount;
     // greedy-basis levels: kChunk ranks per first-level basis, kGroup bases per merge
-    int64_t n = (s->K + kChunk - 1) / kChunk, slots = 0, groups = 0;
+    int64_t n = (s->K + kChunk - 1) / kChunk, slots = 0;
     s->levels = 0;
     for (;;) {
         if (s->levels >= kMaxLevels) return fail(MCB_E_INVALID, "too many candidates for the basis levels");
         s->lvl_slot[s->levels] = slots;
         s->lvl_n[s->levels] = n;
-        s->lvl_grp[s->levels] = groups;
         slots += n;
         ++s->levels;
         if (n == 1) break;
         n = (n + kGroup - 1) / kGroup;
-        groups += n;
     }
     int64_t &B = s->bytes;
     int rc;
@@ -1828,10 +2038,10 @@ int setup_blocked(mcb_solver *s, const std::vector<uint32_t> &order, NposFn npos
     if ((rc = upload(&s->q_small_list, small.data(), small.size(), B))) return rc;
     if ((rc = dev_alloc(&s->Tt, static_cast<size_t>(s->npad), B))) return rc;
     if ((rc = dev_alloc(&s->Qr, static_cast<size_t>(s->K), B))) return rc;
+    if ((rc = dev_alloc(&s->qbits, static_cast<size_t>((s->K + 63) / 64), B))) return rc;
     if ((rc = dev_alloc(&s->sum_cnt, static_cast<size_t>(slots), B))) return rc;
     if ((rc = dev_alloc(&s->sum_rank, static_cast<size_t>(slots) * kWave, B))) return rc;
     if ((rc = dev_alloc(&s->sum_q, static_cast<size_t>(slots) * kWave, B))) return rc;
-    if ((rc = dev_alloc(&s->grp_count, static_cast<size_t>(std::max<int64_t>(groups, 1)), B))) return rc;
     if ((rc = dev_alloc(&s->blk_q, kBlockK, B))) return rc;
     if ((rc = dev_alloc(&s->blk_m, kBlockK, B))) return rc;
     if ((rc = dev_alloc(&s->blk_g, kBlockK, B))) return rc;
@@ -1840,6 +2050,12 @@ int setup_blocked(mcb_solver *s, const std::vector<uint32_t> &order, NposFn npos
     if ((rc = dev_alloc(&s->D_blk, static_cast<size_t>(kBlockK) * s->W, B))) return rc;
     if ((rc = dev_alloc(&s->Dcnt_blk, kBlockK, B))) return rc;
     if ((rc = dev_alloc(&s->upd_bytes, 1, B))) return rc;
+    if ((rc = dev_alloc(&s->Ct, static_cast<size_t>(s->npad), B))) return rc;
+    if ((rc = dev_alloc(&s->pw_list, static_cast<size_t>(s->W), B))) return rc;
+    if ((rc = dev_alloc(&s->pw_off, static_cast<size_t>(s->W) + 1, B))) return rc;
+    if ((rc = dev_alloc(&s->pw_cnt, 1, B))) return rc;
+    if ((rc = dev_alloc(&s->pair_i, static_cast<size_t>(kBlockK) * s->W, B))) return rc;
+    if ((rc = dev_alloc(&s->pair_c, static_cast<size_t>(kBlockK) * s->W, B))) return rc;
     return MCB_OK;
 }
 
@@ -1858,8 +2074,9 @@ void mcb_solver_destroy(mcb_solver *s) {
                     s->root_min_rank, s->rc_off, s->rc_pu, s->rc_pv, s->rc_rank, s->rc_coord, s->cand_root,
                     s->cand_pu, s->cand_pv, s->cand_coord, s->Sw, s->Scur, s->C, s->C_store, s->D, s->Dcount,
                     s->best, s->done, s->status, s->wave_stats, s->flag, s->E, s->Ecount, s->cdot,
-                    s->q_big_list, s->q_small_list, s->Tt, s->Qr, s->sum_q, s->blk_q, s->blk_m, s->blk_g, s->C_blk,
-                    s->Sf_blk, s->sum_cnt, s->sum_rank, s->grp_count, s->D_blk, s->Dcnt_blk, s->upd_bytes};
+                    s->q_big_list, s->q_small_list, s->Tt, s->Qr, s->qbits, s->sum_q, s->blk_q, s->blk_m, s->blk_g, s->C_blk,
+                    s->Sf_blk, s->sum_cnt, s->sum_rank, s->D_blk, s->Dcnt_blk, s->upd_bytes,
+                    s->Ct, s->pair_c, s->pw_list, s->pw_off, s->pw_cnt, s->pair_i};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
@@ -2037,8 +2254,7 @@ int mcb_solver_reset(mcb_solver *s) {
     HIP_TRY(hipMemsetAsync(s->wave_stats, 0, sizeof(uint64_t) * 3 * s->blocks * s->wpb, s->stream));
     if (s->bk > 1) {
         HIP_TRY(hipMemsetAsync(s->Tt, 0, sizeof(uint64_t) * static_cast<size_t>(s->npad), s->stream));
-        HIP_TRY(hipMemsetAsync(s->grp_count, 0, sizeof(uint32_t) * std::max<int64_t>(s->lvl_grp[s->levels - 1], 1),
-                               s->stream));
+        HIP_TRY(hipMemsetAsync(s->qbits, 0, sizeof(uint64_t) * static_cast<size_t>((s->K + 63) / 64), s->stream));
         HIP_TRY(hipMemsetAsync(s->upd_bytes, 0, sizeof(unsigned long long), s->stream));
     }
     HIP_TRY(hipStreamSynchronize(s->stream));

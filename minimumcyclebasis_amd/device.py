@@ -43,9 +43,10 @@ def copy_bandwidth(nbytes: int = 1 << 30, iters: int = 20) -> float:
 class Solver:
     """Device-resident de Pina phase loop for one preprocessed problem."""
 
-    def __init__(self, problem: Problem, store_cycles: bool = False, timing: bool = False,
+    def __init__(self, problem: Problem, store_cycles: bool = False, timing: int | bool = False,
                  waves_per_block: int = 0, max_blocks: int = 0, block_phases: int = 0):
-        """block_phases: phases per pass over the trees (0 automatic, 1 one kernel per phase, up to 64)."""
+        """timing: 0/False none, 1/True every kernel, 2 only the dominant kernel (HIP events).
+        block_phases: phases per pass over the trees (0 automatic, 1 one kernel per phase, up to 64)."""
         info = problem.info
         if info.single_cycle:
             raise ValueError("a single-cycle graph has no phase loop")
@@ -67,7 +68,7 @@ class Solver:
         o = SolverOptions()
         _lib.hip().mcb_solver_options_init(ctypes.byref(o))
         o.store_cycles = int(bool(store_cycles))
-        o.timing = int(bool(timing))
+        o.timing = int(timing) if isinstance(timing, int) and not isinstance(timing, bool) else int(bool(timing))
         o.waves_per_block = int(waves_per_block)
         o.max_blocks = int(max_blocks)
         o.block_phases = int(block_phases)
