@@ -655,8 +655,8 @@ __global__ void k_copy_u64(const uint64_t *__restrict__ src, uint64_t *__restric
 
 constexpr int kBlockK = 64;      // phases per block (word width)
 constexpr int kQWaves = 16;      // waves per workgroup of the tree pass
-constexpr int kChunk = 512;      // ranks per first-level greedy basis
-constexpr int kGroup = 16;       // bases merged per group
+constexpr int kChunk = 4096;     // ranks per first-level greedy basis (one candidate-bitmap word per lane)
+constexpr int kGroup = 64;       // bases merged per group
 constexpr int kMaxLevels = 8;    // kChunk * kGroup^(kMaxLevels-1) covers 2^32 ranks
 constexpr int kTile = 32;        // support words staged in LDS per update step
 
@@ -692,6 +692,7 @@ struct BlockArgs {
     uint64_t *Sf_blk;    // [64][W] final support vectors of the block
     uint64_t *C_store;   // optional [N][W]
     uint64_t *Ct;        // [npad] bit i of Ct[c] = bit c of C_{e+i}
+    uint64_t *Bc;        // [npad] update coefficients of each later column
     uint32_t *pw_list;   // [W] words below e where some C_{e+i} is nonzero
     uint32_t *pw_off;    // [W+1] first pair of each listed word (prefix sums)
     uint32_t *pw_cnt;    // [1] listed words
@@ -828,39 +829,37 @@ __device__ __forceinline__ void tree_words_coop(const uint32_t *pp, const int32_
 }
 
 // Greedy insertion of one batch (lane i: word q, ranks ascending with the lane; q = 0 for no entry)
-// into a wave basis (lane b: pivot whose lowest set bit is b; P: the pivot columns).  The batch is
-// first reduced by the existing pivots lane by lane; the lanes left nonzero are resolved column by
-// column, the first lane with a column's bit becoming its pivot and the bit being cleared from the
-// lanes after it.  The lanes that become pivots are exactly the batch vectors independent of the
-// basis and of the batch vectors before them; emit(lane) is called in the lane that becomes one,
-// in rank order.  Returns the number of new pivots (wave-uniform).
+// into a wave basis kept in reduced echelon form: lane b holds the pivot of column b (P: the pivot
+// columns), and every pivot is zero in every other pivot column.  The batch is reduced by the
+// pivots (one XOR per pivot column the word has); then, while a lane is nonzero, the first such
+// lane becomes the pivot of its lowest set bit, that bit is cleared from the other pivots and from
+// the later lanes.  The lanes that become pivots are exactly the batch vectors independent of the
+// basis and of the batch vectors before them; emit() is called in each, in rank order.  Returns
+// the number of new pivots (wave-uniform).
 template <typename EmitFn>
 __device__ __forceinline__ int basis_reduce_insert(uint64_t &piv, uint64_t &P, uint64_t q, int lane, EmitFn emit) {
     uint64_t v = q;
-    while (__ballot((v & P) != 0)) {
-        const uint64_t h = v & P;
-        const int col = h ? __ffsll(static_cast<unsigned long long>(h)) - 1 : 0;
-        const uint64_t pc = __shfl(piv, col);
-        if (h) v ^= pc;
+    if (__ballot((v & P) != 0)) {
+        uint64_t r = 0;
+        for (uint64_t pm = P; pm; pm &= pm - 1) {
+            const int col = __ffsll(static_cast<unsigned long long>(pm)) - 1;
+            const uint64_t pc = rl64(piv, col);
+            if ((v >> col) & 1ull) r ^= pc;
+        }
+        v ^= r;
     }
     int added = 0;
     uint64_t nz = __ballot(v != 0);
     while (nz) {
-        uint64_t orv = v;
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) orv |= __shfl_xor(orv, o);
-        const int col = __ffsll(static_cast<unsigned long long>(orv)) - 1;
-        const uint64_t m = __ballot((v >> col) & 1ull);
-        const int p = __ffsll(static_cast<unsigned long long>(m)) - 1;
+        const int p = __ffsll(static_cast<unsigned long long>(nz)) - 1;
         const uint64_t pvv = rl64(v, p);
+        const int col = __ffsll(static_cast<unsigned long long>(pvv)) - 1;
         if (lane == col) piv = pvv;
+        else if ((piv >> col) & 1ull) piv ^= pvv;
         P |= 1ull << col;
         ++added;
-        if (lane > p && ((v >> col) & 1ull)) v ^= pvv;
-        if (lane == p) {
-            emit();
-            v = 0;
-        }
+        if (lane == p) emit();
+        if ((v >> col) & 1ull) v ^= pvv;
         nz = __ballot(v != 0);
     }
     return added;
@@ -999,6 +998,65 @@ __device__ __forceinline__ void basis_insert(uint64_t &piv, uint64_t &P, uint32_
     }
 }
 
+// basis_insert with the reduction by existing pivots done through a table in LDS: tab[g][x] is
+// the XOR of the pivots of the columns 8g+t for the set bits t of x (zero for columns without a
+// pivot), so a word reduces with eight lookups.  The table is rebuilt, when pivots were added since
+// it was built, before a batch that has bits in pivot columns.  tab: 8 x 256 words for this wave.
+__device__ __forceinline__ void basis_insert_tab(uint64_t &piv, uint64_t &P, uint32_t &cnt, bool &stale,
+                                                 uint64_t *tab, uint64_t q, uint32_t rank, uint32_t *out_rank,
+                                                 uint64_t *out_q, int lane) {
+    uint64_t v = q;
+    if (__ballot((v & P) != 0)) {
+        if (stale) {
+            for (int g = 0; g < 8; ++g) {
+                uint64_t pt[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) pt[t] = rl64(piv, 8 * g + t);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int x = lane + kWave * k;
+                    uint64_t e = 0;
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) e ^= pt[t] & (0ull - static_cast<uint64_t>((x >> t) & 1));
+                    tab[g * 256 + x] = e;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            stale = false;
+        }
+        uint64_t r = 0;
+#pragma unroll
+        for (int g = 0; g < 8; ++g) r ^= tab[g * 256 + static_cast<int>((v >> (8 * g)) & 255ull)];
+        v ^= r;
+    }
+    uint64_t mine = 0;
+    uint64_t nz = __ballot(v != 0);
+    while (nz) {
+        const int p = __ffsll(static_cast<unsigned long long>(nz)) - 1;
+        const uint64_t pvv = rl64(v, p);
+        const int col = __ffsll(static_cast<unsigned long long>(pvv)) - 1;
+        if (lane == col) piv = pvv;
+        else if ((piv >> col) & 1ull) piv ^= pvv;
+        P |= 1ull << col;
+        stale = true;
+        if (lane == p) mine = 1;
+        if ((v >> col) & 1ull) v ^= pvv;
+        nz = __ballot(v != 0);
+    }
+    uint64_t newp = __ballot(mine != 0);
+    while (newp) {
+        const int p = __ffsll(static_cast<unsigned long long>(newp)) - 1;
+        newp &= newp - 1;
+        const uint64_t qp = rl64(q, p);
+        const uint32_t rp = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rank), p));
+        if (lane == 0) {
+            out_rank[cnt] = rp;
+            out_q[cnt] = qp;
+        }
+        ++cnt;
+    }
+}
+
 // The block's greedy basis (top summary) in rank order -> phase of each winner, masks m_i, the
 // products <C_{e+i}, S_{e+l}> and the chosen ranks.  One wave.
 __device__ void block_finalize(const BlockArgs &a, int lane) {
@@ -1056,34 +1114,53 @@ __device__ void block_finalize(const BlockArgs &a, int lane) {
 }
 
 // First level of the block's greedy basis: wave c, ranks [c*kChunk, (c+1)*kChunk) -> slot c; only
-// the words the tree pass emitted take part (the others cannot be in the basis).
+// the words the tree pass emitted take part (the others cannot be in the basis).  Lane l reads
+// bitmap word l of the chunk; a prefix sum of the popcounts lists the emitted ranks in rank order,
+// which are inserted 64 at a time.
 __global__ __launch_bounds__(256) void k_block_basis(BlockArgs a) {
-    constexpr int kBatches = kChunk / kWave;
+    __shared__ uint64_t s_words[4][kWave];
+    __shared__ uint32_t s_offs[4][kWave];
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * 4 + threadIdx.x / kWave;
+    const int wv = threadIdx.x / kWave;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * 4 + wv;
     if (c >= a.lvl_n[0]) return;
     const int64_t r0 = c * kChunk;
-    const int64_t r1 = r0 + kChunk < a.K ? r0 + kChunk : a.K;
-    uint64_t bits[kBatches], q[kBatches];
+    const int64_t wbase = r0 >> 6;
+    const int64_t nwords = (a.K + kWave - 1) / kWave;
+    const bool valid = wbase + lane < nwords;
+    const uint64_t word = valid ? a.qbits[wbase + lane] : 0ull;
+    if (word) a.qbits[wbase + lane] = 0;  // ready for the next block
+    const uint32_t own = static_cast<uint32_t>(__popcll(word));
+    uint32_t incl = own;
 #pragma unroll
-    for (int u = 0; u < kBatches; ++u) {
-        const int64_t wd = (r0 >> 6) + u;
-        bits[u] = r0 + u * kWave < r1 ? a.qbits[wd] : 0ull;
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
     }
-#pragma unroll
-    for (int u = 0; u < kBatches; ++u) {
-        const bool on = (bits[u] >> lane) & 1ull;
-        q[u] = on ? a.Qr[r0 + u * kWave + lane] : 0ull;
-    }
-    if (lane < kBatches && r0 + lane * kWave < r1) a.qbits[(r0 >> 6) + lane] = 0;  // ready for the next block
+    const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), kWave - 1));
+    s_words[wv][lane] = word;
+    s_offs[wv][lane] = incl - own;
+    __builtin_amdgcn_wave_barrier();
     uint64_t piv = 0, P = 0;
     uint32_t cnt = 0;
     const int64_t slot = a.lvl_slot[0] + c;
-#pragma unroll
-    for (int u = 0; u < kBatches; ++u) {
-        if (cnt >= static_cast<uint32_t>(a.kk) || !__ballot(q[u] != 0)) continue;
-        basis_insert(piv, P, cnt, q[u], static_cast<uint32_t>(r0 + u * kWave + lane), a.sum_rank + slot * kWave,
-                     a.sum_q + slot * kWave, lane);
+    for (uint32_t b0 = 0; b0 < total && cnt < static_cast<uint32_t>(a.kk); b0 += kWave) {
+        const uint32_t t = b0 + lane;
+        uint64_t q = 0;
+        uint32_t rank = 0;
+        if (t < total) {
+            int lo = 0, hi = kWave - 1;  // the last lane whose first entry is at or before t owns entry t
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_offs[wv][mid] <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            uint64_t w = s_words[wv][lo];
+            for (uint32_t k = t - s_offs[wv][lo]; k > 0; --k) w &= w - 1;
+            rank = static_cast<uint32_t>(r0 + lo * kWave + (__ffsll(static_cast<unsigned long long>(w)) - 1));
+            q = a.Qr[rank];
+        }
+        basis_insert(piv, P, cnt, q, rank, a.sum_rank + slot * kWave, a.sum_q + slot * kWave, lane);
     }
     if (lane == 0) a.sum_cnt[slot] = cnt;
     if (a.levels == 1) {
@@ -1092,34 +1169,66 @@ __global__ __launch_bounds__(256) void k_block_basis(BlockArgs a) {
     }
 }
 
-// One merge level: wave g merges bases g*kGroup .. g*kGroup+kGroup-1 of level L (in rank order) into
-// slot g of level L+1; at the top level the single wave also finalises the block.  The bases of
-// level L were written by the previous launch.
-__global__ __launch_bounds__(256) void k_block_merge(BlockArgs a, int L) {
+// One merge level: wave g merges bases g*kGroup .. g*kGroup+kGroup-1 of level L into slot g of level
+// L+1.  Their entries, concatenated in order, are in rank order; a prefix sum of the bases' sizes
+// locates entry t, and the entries are inserted 64 at a time with the next batch's loads in
+// flight.  At the top level the single wave also finalises the block.  The bases of level L were
+// written by the previous launch.
+constexpr int kMergeWaves = 2;  // waves per workgroup of k_block_merge (16 KB of LDS table each)
+__global__ __launch_bounds__(kMergeWaves * kWave) void k_block_merge(BlockArgs a, int L) {
+    __shared__ uint32_t s_offs[kMergeWaves][kWave];
+    __shared__ uint64_t s_tab[kMergeWaves][8 * 256];
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t g = static_cast<int64_t>(blockIdx.x) * 4 + threadIdx.x / kWave;
+    const int wv = threadIdx.x / kWave;
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * kMergeWaves + wv;
     if (g >= a.lvl_n[L + 1]) return;
     const int64_t rest = a.lvl_n[L] - g * kGroup;
     const int members = static_cast<int>(rest < kGroup ? rest : kGroup);
     const int64_t s0 = a.lvl_slot[L] + g * kGroup;
-    const uint32_t my_n = lane < members ? a.sum_cnt[s0 + lane] : 0u;
-    uint64_t qq[kGroup];
-    uint32_t rr[kGroup];
+    const uint32_t own = lane < members ? a.sum_cnt[s0 + lane] : 0u;
+    uint32_t incl = own;
 #pragma unroll
-    for (int m = 0; m < kGroup; ++m) {
-        const int64_t s = s0 + (m < members ? m : 0);
-        qq[m] = a.sum_q[s * kWave + lane];
-        rr[m] = a.sum_rank[s * kWave + lane];
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
     }
+    const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), kWave - 1));
+    s_offs[wv][lane] = incl - own;
+    __builtin_amdgcn_wave_barrier();
+    // entry t of the concatenation: base "lo" (the last base starting at or before t), position t - start
+    auto locate = [&](uint32_t t) -> int64_t {
+        int lo = 0, hi = kWave - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_offs[wv][mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        return (s0 + lo) * kWave + (t - s_offs[wv][lo]);
+    };
     uint64_t piv = 0, P = 0;
     uint32_t cnt = 0;
+    bool stale = true;
     const int64_t slot = a.lvl_slot[L + 1] + g;
-#pragma unroll
-    for (int m = 0; m < kGroup; ++m) {
-        const uint32_t n = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(my_n), m));
-        if (m >= members || n == 0 || cnt >= static_cast<uint32_t>(a.kk)) continue;
-        basis_insert(piv, P, cnt, lane < static_cast<int>(n) ? qq[m] : 0ull, rr[m], a.sum_rank + slot * kWave,
-                     a.sum_q + slot * kWave, lane);
+    uint64_t qn = 0;
+    uint32_t rn = 0;
+    if (lane < static_cast<int>(total)) {
+        const int64_t at = locate(lane);
+        qn = a.sum_q[at];
+        rn = a.sum_rank[at];
+    }
+    for (uint32_t b0 = 0; b0 < total && cnt < static_cast<uint32_t>(a.kk); b0 += kWave) {
+        const uint64_t q = qn;
+        const uint32_t r = rn;
+        const uint32_t tn = b0 + kWave + lane;
+        qn = 0;
+        rn = 0;
+        if (tn < total) {
+            const int64_t at = locate(tn);
+            qn = a.sum_q[at];
+            rn = a.sum_rank[at];
+        }
+        basis_insert_tab(piv, P, cnt, stale, s_tab[wv], q, r, a.sum_rank + slot * kWave, a.sum_q + slot * kWave,
+                         lane);
     }
     if (lane == 0) a.sum_cnt[slot] = cnt;
     if (L + 2 == a.levels) {
@@ -1284,44 +1393,30 @@ __global__ __launch_bounds__(256) void k_block_pairs(BlockArgs a) {
     }
 }
 
-// Thread j: column e+i < e+kk receives its final vector S_{e+i}; column j >= e+kk takes the
-// block's update: d_i = <C_{e+i}, S_j>, b_i = d_i ^ sum_{l<i} b_l <C_{e+i}, S_{e+l}>,
-// S_j ^= sum_i b_i S_{e+i}.  S_j has support below e besides its own bit, so the dot products
-// read only the words where some C_{e+i} is nonzero below e, plus bit j of each C.
-// The word list and its pairs are staged in LDS in segments of at most kSegWords words and
-// kSegPairs pairs.  LDS: G rows [64] | S tile [64][kTile] | pair words [kSegPairs] | pair phases
-// [kSegPairs] | listed words [kSegWords] | their pair offsets [kSegWords + 1].
+// The block's update of a later column j >= e+kk is S_j ^= sum_i b_i S_{e+i} with
+// b_i = <C_{e+i}, S_j> ^ sum_{l<i} b_l <C_{e+i}, S_{e+l}>.  S_j has support below e besides its own
+// bit, so the products read only the listed words (where some C_{e+i} is nonzero below e) plus bit j
+// of each C (Ct[j]).  k_block_dot: thread j computes b into Bc[j].  The list and its pairs are
+// staged in LDS in segments of at most kSegWords words and kSegPairs pairs.
+// LDS: G rows [64] | pair words [kSegPairs] | pair phases [kSegPairs] | listed words [kSegWords] |
+// their pair offsets [kSegWords + 1].
 constexpr int kSegWords = 1024;
 constexpr int kSegPairs = 2048;
-__global__ __launch_bounds__(256) void k_block_update(BlockArgs a) {
+__global__ __launch_bounds__(256) void k_block_dot(BlockArgs a) {
     extern __shared__ uint64_t lds[];
     __shared__ uint32_t seg_end;
     constexpr int kB = 8;
     const int tid = threadIdx.x;
     const int kk = a.kk;
-    const int32_t wl = (a.e + kk + kWave - 1) / kWave;
     uint64_t *G = lds;
-    uint64_t *T = lds + kBlockK;
-    uint64_t *Pc = T + kBlockK * kTile;
+    uint64_t *Pc = G + kBlockK;
     uint32_t *Pi = reinterpret_cast<uint32_t *>(Pc + kSegPairs);
     uint32_t *Lw = Pi + kSegPairs;
     uint32_t *Lo = Lw + kSegWords;
     for (int x = tid; x < kBlockK; x += blockDim.x) G[x] = x < kk ? a.blk_g[x] : 0ull;
-    const int64_t j = static_cast<int64_t>(a.e) + static_cast<int64_t>(blockIdx.x) * blockDim.x + tid;
-    const bool own_col = j < static_cast<int64_t>(a.e) + kk;
-    const bool upd_col = !own_col && j < a.N;
-    uint64_t b = 0;
-    uint64_t nbytes = 0;
-    if (own_col) {
-        const int i = static_cast<int>(j - a.e);
-        const uint64_t *Sf = a.Sf_blk + static_cast<int64_t>(i) * a.W;
-        for (int32_t w = 0; w < wl; ++w) {
-            uint64_t v = Sf[w];
-            if ((j >> 6) == w) v &= ~(1ull << (j & 63));
-            a.Sw[static_cast<int64_t>(w) * a.npad + j] = v;
-        }
-    }
-    uint64_t d = upd_col ? a.Ct[j] : 0ull;
+    const int64_t j = static_cast<int64_t>(a.e) + kk + static_cast<int64_t>(blockIdx.x) * blockDim.x + tid;
+    const bool col = j < a.N;
+    uint64_t d = col ? a.Ct[j] : 0ull;
     const uint32_t nl = *a.pw_cnt;
     for (uint32_t s0 = 0; s0 < nl;) {
         __syncthreads();
@@ -1348,7 +1443,7 @@ __global__ __launch_bounds__(256) void k_block_update(BlockArgs a) {
             Pi[q] = a.pair_i[p0 + q];
         }
         __syncthreads();
-        if (upd_col) {
+        if (col) {
             for (uint32_t x0 = 0; x0 < s1 - s0; x0 += kB) {
                 uint64_t sv[kB];
 #pragma unroll
@@ -1367,34 +1462,75 @@ __global__ __launch_bounds__(256) void k_block_update(BlockArgs a) {
         s0 = s1;
     }
     __syncthreads();
-    if (upd_col) {
+    uint64_t nbytes = 0;
+    if (col) {
+        uint64_t b = 0;
         for (int i = 0; i < kk; ++i) {
             const uint64_t bi = ((d >> i) ^ static_cast<uint64_t>(__popcll(b & G[i]))) & 1ull;
             b |= bi << i;
         }
+        a.Bc[j] = b;
         nbytes = 2 + 8ull * nl;
     }
-    for (int32_t w0 = 0; w0 < wl; w0 += kTile) {
-        __syncthreads();
-        for (int x = tid; x < kBlockK * kTile; x += blockDim.x) {
-            const int i = x / kTile, t = x % kTile;
-            T[x] = (i < kk && w0 + t < wl) ? a.Sf_blk[static_cast<int64_t>(i) * a.W + w0 + t] : 0ull;
+    for (int o = kWave / 2; o > 0; o >>= 1) nbytes += __shfl_down(nbytes, o);
+    if ((tid & (kWave - 1)) == 0 && nbytes) atomicAdd(a.upd_bytes, static_cast<unsigned long long>(nbytes));
+}
+
+// Workgroup (x, y): columns e + 256x .. +255, words kTile*y .. +kTile-1.  Column e+i < e+kk receives
+// its final vector S_{e+i}; a later column j with Bc[j] != 0 takes S_j ^= sum_i b_i S_{e+i}.
+// LDS: the block's final vectors on the workgroup's words [64][kTile].
+__global__ __launch_bounds__(256) void k_block_xor(BlockArgs a) {
+    extern __shared__ uint64_t lds[];
+    constexpr int kW = 8;
+    const int tid = threadIdx.x;
+    const int kk = a.kk;
+    const int32_t wl = (a.e + kk + kWave - 1) / kWave;
+    const int32_t w0 = static_cast<int32_t>(blockIdx.y) * kTile;
+    const int tn = wl - w0 < kTile ? wl - w0 : kTile;
+    uint64_t *T = lds;
+    for (int x = tid; x < kBlockK * kTile; x += blockDim.x) {
+        const int i = x / kTile, t = x % kTile;
+        T[x] = (i < kk && t < tn) ? a.Sf_blk[static_cast<int64_t>(i) * a.W + w0 + t] : 0ull;
+    }
+    __syncthreads();
+    const int64_t j = static_cast<int64_t>(a.e) + static_cast<int64_t>(blockIdx.x) * blockDim.x + tid;
+    uint64_t nbytes = 0;
+    if (j < static_cast<int64_t>(a.e) + kk) {
+        const int i = static_cast<int>(j - a.e);
+        for (int t = 0; t < tn; ++t) {
+            uint64_t v = T[i * kTile + t];
+            if ((j >> 6) == w0 + t) v &= ~(1ull << (j & 63));
+            a.Sw[static_cast<int64_t>(w0 + t) * a.npad + j] = v;
         }
-        __syncthreads();
+    } else if (j < a.N) {
+        const uint64_t b = a.Bc[j];
         if (b) {
-            const int tn = wl - w0 < kTile ? wl - w0 : kTile;
-            for (int t = 0; t < tn; ++t) {
-                uint64_t x = 0;
+            for (int t0 = 0; t0 < tn; t0 += kW) {
+                uint64_t xs[kW];
 #pragma unroll
-                for (int i = 0; i < kBlockK; ++i) x ^= T[i * kTile + t] & (0ull - ((b >> i) & 1ull));
-                if (x) {
-                    a.Sw[static_cast<int64_t>(w0 + t) * a.npad + j] ^= x;
-                    nbytes += 16;
+                for (int u = 0; u < kW; ++u) xs[u] = 0;
+#pragma unroll 4
+                for (int i = 0; i < kBlockK; ++i) {
+                    const uint64_t m = 0ull - ((b >> i) & 1ull);
+                    const uint64_t *row = T + i * kTile + t0;
+#pragma unroll
+                    for (int u = 0; u < kW; ++u) xs[u] ^= row[u] & m;
                 }
+                uint64_t sv[kW];
+#pragma unroll
+                for (int u = 0; u < kW; ++u) {
+                    if (t0 + u >= tn) xs[u] = 0;
+                    sv[u] = xs[u] ? a.Sw[static_cast<int64_t>(w0 + t0 + u) * a.npad + j] : 0ull;
+                }
+#pragma unroll
+                for (int u = 0; u < kW; ++u)
+                    if (xs[u]) {
+                        a.Sw[static_cast<int64_t>(w0 + t0 + u) * a.npad + j] = sv[u] ^ xs[u];
+                        nbytes += 16;
+                    }
             }
         }
     }
-    // one atomic per wave
     for (int o = kWave / 2; o > 0; o >>= 1) nbytes += __shfl_down(nbytes, o);
     if ((tid & (kWave - 1)) == 0 && nbytes) atomicAdd(a.upd_bytes, static_cast<unsigned long long>(nbytes));
 }
@@ -1737,7 +1873,7 @@ struct mcb_solver {
     int32_t q_big_words = 0, q_small_words = 0, q_blocks = 0, q_lds = 0, n_qbig = 0, n_qsmall = 0;
     uint32_t *q_big_list = nullptr, *q_small_list = nullptr;
     uint64_t *Tt = nullptr, *Qr = nullptr, *qbits = nullptr, *sum_q = nullptr, *blk_q = nullptr, *blk_m = nullptr, *blk_g = nullptr;
-    uint64_t *C_blk = nullptr, *Sf_blk = nullptr, *Ct = nullptr, *pair_c = nullptr;
+    uint64_t *C_blk = nullptr, *Sf_blk = nullptr, *Ct = nullptr, *Bc = nullptr, *pair_c = nullptr;
     uint32_t *pw_list = nullptr, *pw_off = nullptr, *pw_cnt = nullptr, *pair_i = nullptr;
     uint32_t *sum_cnt = nullptr, *sum_rank = nullptr, *D_blk = nullptr, *Dcnt_blk = nullptr;
     unsigned long long *upd_bytes = nullptr;
@@ -1884,6 +2020,7 @@ BlockArgs block_args(const mcb_solver *s, int32_t e, int32_t kk) {
     a.Sf_blk = s->Sf_blk;
     a.C_store = s->C_store;
     a.Ct = s->Ct;
+    a.Bc = s->Bc;
     a.pw_list = s->pw_list;
     a.pw_off = s->pw_off;
     a.pw_cnt = s->pw_cnt;
@@ -1918,8 +2055,8 @@ int launch_block(mcb_solver *s, int32_t e, int32_t kk) {
     hipLaunchKernelGGL(k_block_basis, dim3(static_cast<unsigned>((s->lvl_n[0] + 3) / 4)), dim3(256), 0, s->stream, a);
     HIP_TRY(hipGetLastError());
     for (int L = 0; L + 1 < s->levels; ++L) {
-        hipLaunchKernelGGL(k_block_merge, dim3(static_cast<unsigned>((s->lvl_n[L + 1] + 3) / 4)), dim3(256), 0,
-                           s->stream, a, L);
+        hipLaunchKernelGGL(k_block_merge, dim3(static_cast<unsigned>((s->lvl_n[L + 1] + kMergeWaves - 1) / kMergeWaves)),
+                           dim3(kMergeWaves * kWave), 0, s->stream, a, L);
         HIP_TRY(hipGetLastError());
     }
     if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_CYCLE))) return rc;
@@ -1930,10 +2067,15 @@ int launch_block(mcb_solver *s, int32_t e, int32_t kk) {
                        sizeof(uint64_t) * dlim + 2 * 256 * sizeof(uint32_t), s->stream, a);
     HIP_TRY(hipGetLastError());
     if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_UPDATE))) return rc;
-    const size_t ulds = sizeof(uint64_t) * (kBlockK + static_cast<size_t>(kBlockK) * kTile + kSegPairs) +
-                        sizeof(uint32_t) * (kSegPairs + 2 * kSegWords + 1);
+    const int64_t later = s->N - e - kk;
+    if (later > 0) {
+        const size_t dlds = sizeof(uint64_t) * (kBlockK + kSegPairs) + sizeof(uint32_t) * (kSegPairs + 2 * kSegWords + 1);
+        hipLaunchKernelGGL(k_block_dot, dim3(static_cast<unsigned>((later + 255) / 256)), dim3(256), dlds, s->stream, a);
+        HIP_TRY(hipGetLastError());
+    }
     const int64_t cols = s->N - e;
-    hipLaunchKernelGGL(k_block_update, dim3(static_cast<unsigned>((cols + 255) / 256)), dim3(256), ulds, s->stream, a);
+    hipLaunchKernelGGL(k_block_xor, dim3(static_cast<unsigned>((cols + 255) / 256), static_cast<unsigned>((wl + kTile - 1) / kTile)),
+                       dim3(256), sizeof(uint64_t) * kBlockK * kTile, s->stream, a);
     HIP_TRY(hipGetLastError());
     s->blk_positions += s->Ptot;
     s->blk_candidates += s->K;
@@ -2051,6 +2193,7 @@ int setup_blocked(mcb_solver *s, const std::vector<uint32_t> &order, NposFn npos
     if ((rc = dev_alloc(&s->Dcnt_blk, kBlockK, B))) return rc;
     if ((rc = dev_alloc(&s->upd_bytes, 1, B))) return rc;
     if ((rc = dev_alloc(&s->Ct, static_cast<size_t>(s->npad), B))) return rc;
+    if ((rc = dev_alloc(&s->Bc, static_cast<size_t>(s->npad), B))) return rc;
     if ((rc = dev_alloc(&s->pw_list, static_cast<size_t>(s->W), B))) return rc;
     if ((rc = dev_alloc(&s->pw_off, static_cast<size_t>(s->W) + 1, B))) return rc;
     if ((rc = dev_alloc(&s->pw_cnt, 1, B))) return rc;
@@ -2076,7 +2219,7 @@ void mcb_solver_destroy(mcb_solver *s) {
                     s->best, s->done, s->status, s->wave_stats, s->flag, s->E, s->Ecount, s->cdot,
                     s->q_big_list, s->q_small_list, s->Tt, s->Qr, s->qbits, s->sum_q, s->blk_q, s->blk_m, s->blk_g, s->C_blk,
                     s->Sf_blk, s->sum_cnt, s->sum_rank, s->D_blk, s->Dcnt_blk, s->upd_bytes,
-                    s->Ct, s->pair_c, s->pw_list, s->pw_off, s->pw_cnt, s->pair_i};
+                    s->Ct, s->Bc, s->pair_c, s->pw_list, s->pw_off, s->pw_cnt, s->pair_i};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
