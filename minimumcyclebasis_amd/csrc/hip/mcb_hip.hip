@@ -84,6 +84,23 @@ constexpr int kUnroll = MCB_UNROLL;           // tree windows per pipelined load
 constexpr int kBlockWaves = MCB_BLOCK_WAVES;  // waves per block of the parity/search kernel
 constexpr uint32_t kBigPositions = 64u * MCB_BIG_WINDOWS;  // larger trees are shared by a whole block
 
+// Index of the calling wave within its workgroup, in a scalar register: the compiler cannot prove
+// threadIdx.x / 64 wave-uniform, and loop bounds derived from it would make everything they carry
+// divergent.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave); }
+
+// A value every lane holds, moved to scalar registers (a vector load from a uniform address is
+// otherwise taken as divergent, and so is everything computed from it).
+__device__ __forceinline__ int64_t uniform64(int64_t x) {
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(x))));
+    const uint32_t hi =
+        static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(static_cast<uint64_t>(x) >> 32))));
+    return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+__device__ __forceinline__ uint32_t uniform32(uint32_t x) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x)));
+}
+
 // The loads below are unconditional (a clamped address, the value selected afterwards): a load
 // under a divergent branch makes the compiler wait for every outstanding load at the join, which
 // would serialise the prefetch pipeline of the tree kernels.
@@ -499,7 +516,7 @@ __global__ __launch_bounds__(kBlockWaves * kWave) void k_phase(PhaseArgs a) {
     __shared__ uint32_t skip;
     __shared__ uint32_t last;
     const int lane = threadIdx.x & (kWave - 1);
-    const int wid = threadIdx.x / kWave;
+    const int wid = wave_id();
     const int nw = blockDim.x / kWave;
     uint64_t n_upd = 0;  // algorithmic bytes of the update: 8 per dot word, 16 per XORed word, 2 per flag
     if (SEARCH && a.upd_tiles > 0) {
@@ -606,7 +623,7 @@ __global__ __launch_bounds__(256) void k_compat_parity(const int32_t *__restrict
                                                        int32_t p_words) {
     extern __shared__ uint64_t lds[];
     const int lane = threadIdx.x & (kWave - 1);
-    const int wib = threadIdx.x / kWave;
+    const int wib = wave_id();
     const int wpb = blockDim.x / kWave;
     uint64_t *P = lds + static_cast<size_t>(wib) * p_words;
     const int slot = blockIdx.x * wpb + wib;
@@ -708,7 +725,7 @@ struct BlockArgs {
 // e .. e+kk-1 (bit b of that word in lane l is bit l of Tt[64w+b]); own bits are added.
 __global__ __launch_bounds__(256) void k_block_transpose(BlockArgs a) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t w = static_cast<int64_t>(blockIdx.x) * 4 + threadIdx.x / kWave;
+    const int64_t w = static_cast<int64_t>(blockIdx.x) * 4 + wave_id();
     const int64_t wl = (static_cast<int64_t>(a.e) + a.kk + kWave - 1) / kWave;
     if (w >= wl) return;
     const uint64_t x = lane < a.kk ? a.Sw[w * a.npad + a.e + lane] : 0ull;
@@ -839,6 +856,7 @@ __device__ __forceinline__ void tree_words_coop(const uint32_t *pp, const int32_
 template <typename EmitFn>
 __device__ __forceinline__ int basis_reduce_insert(uint64_t &piv, uint64_t &P, uint64_t q, int lane, EmitFn emit) {
     uint64_t v = q;
+    P = static_cast<uint64_t>(uniform64(static_cast<int64_t>(P)));
     if (__ballot((v & P) != 0)) {
         uint64_t r = 0;
         for (uint64_t pm = P; pm; pm &= pm - 1) {
@@ -853,10 +871,10 @@ __device__ __forceinline__ int basis_reduce_insert(uint64_t &piv, uint64_t &P, u
     while (nz) {
         const int p = __ffsll(static_cast<unsigned long long>(nz)) - 1;
         const uint64_t pvv = rl64(v, p);
-        const int col = __ffsll(static_cast<unsigned long long>(pvv)) - 1;
+        const int col = __builtin_amdgcn_readfirstlane(__ffsll(static_cast<unsigned long long>(pvv)) - 1);
         if (lane == col) piv = pvv;
         else if ((piv >> col) & 1ull) piv ^= pvv;
-        P |= 1ull << col;
+        P = static_cast<uint64_t>(uniform64(static_cast<int64_t>(P | (1ull << col))));
         ++added;
         if (lane == p) emit();
         if ((v >> col) & 1ull) v ^= pvv;
@@ -911,7 +929,7 @@ __device__ __forceinline__ void run_words(const BlockArgs &a, const uint64_t *Wd
 __global__ __launch_bounds__(kQWaves * kWave) void k_block_q(BlockArgs a) {
     extern __shared__ uint64_t lds[];
     const int lane = threadIdx.x & (kWave - 1);
-    const int wid = threadIdx.x / kWave;
+    const int wid = wave_id();
     const int nw = blockDim.x / kWave;
     uint64_t *Wbig = lds;
     uint32_t *ready = reinterpret_cast<uint32_t *>(lds + a.big_words);
@@ -920,14 +938,14 @@ __global__ __launch_bounds__(kQWaves * kWave) void k_block_q(BlockArgs a) {
     __syncthreads();
     uint32_t gen = 0;
     for (uint32_t it = blockIdx.x; it < static_cast<uint32_t>(a.n_big); it += gridDim.x) {
-        const uint32_t r = a.big_list[it];
-        const int64_t off = a.tree_off[r];
-        const uint32_t npos = static_cast<uint32_t>(a.tree_off[r + 1] - off);
+        const uint32_t r = uniform32(a.big_list[it]);
+        const int64_t off = uniform64(a.tree_off[r]);
+        const uint32_t npos = static_cast<uint32_t>(uniform64(a.tree_off[r + 1]) - off);
         ++gen;
         tree_words_coop(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wbig, ready, gen, wid, nw, lane);
         __syncthreads();
         // each wave takes one contiguous run of the root's candidates
-        const int64_t cb = a.rc_off[r], ce = a.rc_off[r + 1];
+        const int64_t cb = uniform64(a.rc_off[r]), ce = uniform64(a.rc_off[r + 1]);
         const int64_t seg = (ce - cb + nw - 1) / nw;
         const int64_t s0 = cb + seg * wid, s1 = s0 + seg < ce ? s0 + seg : ce;
         if (s0 < s1) run_words(a, Wbig, s0, s1, lane);
@@ -936,11 +954,11 @@ __global__ __launch_bounds__(kQWaves * kWave) void k_block_q(BlockArgs a) {
     // the big-tree region is free from here on; each wave takes its own slice for small roots
     uint64_t *Ws = lds + static_cast<size_t>(wid) * a.small_words;
     for (uint32_t it = blockIdx.x * nw + wid; it < static_cast<uint32_t>(a.n_small); it += gridDim.x * nw) {
-        const uint32_t r = a.small_list[it];
-        const int64_t off = a.tree_off[r];
-        const uint32_t npos = static_cast<uint32_t>(a.tree_off[r + 1] - off);
+        const uint32_t r = uniform32(a.small_list[it]);
+        const int64_t off = uniform64(a.tree_off[r]);
+        const uint32_t npos = static_cast<uint32_t>(uniform64(a.tree_off[r + 1]) - off);
         tree_words(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Ws, lane);
-        run_words(a, Ws, a.rc_off[r], a.rc_off[r + 1], lane);
+        run_words(a, Ws, uniform64(a.rc_off[r]), uniform64(a.rc_off[r + 1]), lane);
     }
 }
 
@@ -1121,7 +1139,7 @@ __global__ __launch_bounds__(256) void k_block_basis(BlockArgs a) {
     __shared__ uint64_t s_words[4][kWave];
     __shared__ uint32_t s_offs[4][kWave];
     const int lane = threadIdx.x & (kWave - 1);
-    const int wv = threadIdx.x / kWave;
+    const int wv = wave_id();
     const int64_t c = static_cast<int64_t>(blockIdx.x) * 4 + wv;
     if (c >= a.lvl_n[0]) return;
     const int64_t r0 = c * kChunk;
@@ -1179,7 +1197,7 @@ __global__ __launch_bounds__(kMergeWaves * kWave) void k_block_merge(BlockArgs a
     __shared__ uint32_t s_offs[kMergeWaves][kWave];
     __shared__ uint64_t s_tab[kMergeWaves][8 * 256];
     const int lane = threadIdx.x & (kWave - 1);
-    const int wv = threadIdx.x / kWave;
+    const int wv = wave_id();
     const int64_t g = static_cast<int64_t>(blockIdx.x) * kMergeWaves + wv;
     if (g >= a.lvl_n[L + 1]) return;
     const int64_t rest = a.lvl_n[L] - g * kGroup;
