@@ -683,6 +683,7 @@ struct BlockArgs {
     const int32_t *pos_coord;
     const uint32_t *pos_pre, *pos_end;
     const uint32_t *big_list, *small_list;
+    const uint8_t *big_narrow;  // per big tree: 1 if its levels average fewer than 128 positions
     const int64_t *rc_off;
     const uint32_t *rc_pu, *rc_pv, *rc_rank;
     const int32_t *rc_coord;
@@ -942,7 +943,13 @@ __global__ __launch_bounds__(kQWaves * kWave) void k_block_q(BlockArgs a) {
         const int64_t off = uniform64(a.tree_off[r]);
         const uint32_t npos = static_cast<uint32_t>(uniform64(a.tree_off[r + 1]) - off);
         ++gen;
-        tree_words_coop(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wbig, ready, gen, wid, nw, lane);
+        if (a.big_narrow[it]) {
+            // narrow, deep tree: windows mostly wait on the window before them, so one wave walks it
+            // alone (no handoffs between waves); the others wait at the barrier
+            if (wid == 0) tree_words(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wbig, lane);
+        } else {
+            tree_words_coop(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wbig, ready, gen, wid, nw, lane);
+        }
         __syncthreads();
         // each wave takes one contiguous run of the root's candidates
         const int64_t cb = uniform64(a.rc_off[r]), ce = uniform64(a.rc_off[r + 1]);
@@ -1890,6 +1897,7 @@ struct mcb_solver {
     int32_t bk = 1;                                   // phases per block; 1 = one kernel per phase
     int32_t q_big_words = 0, q_small_words = 0, q_blocks = 0, q_lds = 0, n_qbig = 0, n_qsmall = 0;
     uint32_t *q_big_list = nullptr, *q_small_list = nullptr;
+    uint8_t *q_big_narrow = nullptr;
     uint64_t *Tt = nullptr, *Qr = nullptr, *qbits = nullptr, *sum_q = nullptr, *blk_q = nullptr, *blk_m = nullptr, *blk_g = nullptr;
     uint64_t *C_blk = nullptr, *Sf_blk = nullptr, *Ct = nullptr, *Bc = nullptr, *pair_c = nullptr;
     uint32_t *pw_list = nullptr, *pw_off = nullptr, *pw_cnt = nullptr, *pair_i = nullptr;
@@ -2005,6 +2013,7 @@ BlockArgs block_args(const mcb_solver *s, int32_t e, int32_t kk) {
     a.pos_pre = s->pos_pre;
     a.pos_end = s->pos_end;
     a.big_list = s->q_big_list;
+    a.big_narrow = s->q_big_narrow;
     a.small_list = s->q_small_list;
     a.rc_off = s->rc_off;
     a.rc_pu = s->rc_pu;
@@ -2138,16 +2147,22 @@ void preorder_intervals(const int64_t *tree_offset, const uint32_t *parent_pos, 
     }
 }
 
-// Buffers and launch shape of the blocked form.  Trees with more than 1024 positions take a whole
-// workgroup of the tree pass (their path words must fit the LDS of one workgroup); smaller ones take
-// one wave.  Chooses the per-phase form instead when block_phases asks for it or when the largest
+// Buffers and launch shape of the blocked form.  Big trees take a whole workgroup of the tree pass
+// (their path words must fit the LDS of one workgroup); small ones take one wave.  Chooses the per-phase form instead when block_phases asks for it or when the largest
 // tree does not fit.
 template <typename NposFn>
-int setup_blocked(mcb_solver *s, const std::vector<uint32_t> &order, NposFn npos_of) {
+int setup_blocked(mcb_solver *s, const std::vector<uint32_t> &order, NposFn npos_of, const int64_t *tree_offset,
+                  const uint32_t *parent_pos) {
     const int want = s->opt.block_phases;
     if (want < 0 || want > kBlockK) return fail(MCB_E_INVALID, "block_phases must be 0..64");
     if (want == 1) return MCB_OK;
-    constexpr int64_t kSmallMax = 1024;
+    // Small trees take one wave and a slice of 1/kQWaves of the workgroup's LDS; the threshold is the
+    // largest multiple of 64 (at most 1024) whose slices fit the space the largest tree needs anyway.
+    int64_t largest = 0;
+    for (uint32_t r : order) largest = std::max<int64_t>(largest, npos_of(r));
+    const int64_t largest_bytes = 8 * ((largest + kWave - 1) / kWave * kWave);
+    int64_t kSmallMax = std::min<int64_t>(1024, largest_bytes / (8 * kQWaves) / kWave * kWave);
+    kSmallMax = std::max<int64_t>(kSmallMax, kWave);
     std::vector<uint32_t> big, small;
     int64_t max_big = 0, max_small = 1;
     for (uint32_t r : order) {
@@ -2158,6 +2173,22 @@ int setup_blocked(mcb_solver *s, const std::vector<uint32_t> &order, NposFn npos
         } else {
             small.push_back(r);
             max_small = std::max(max_small, np);
+        }
+    }
+    // a big tree whose levels average fewer than 128 positions is walked by one wave
+    std::vector<uint8_t> narrow(big.size(), 0);
+    {
+        std::vector<uint32_t> level;
+        for (size_t i = 0; i < big.size(); ++i) {
+            const uint32_t r = big[i];
+            const int64_t a0 = tree_offset[r], n = tree_offset[r + 1] - a0;
+            level.assign(static_cast<size_t>(n), 0);
+            uint32_t depth = 0;
+            for (int64_t x = 1; x < n; ++x) {
+                level[x] = level[parent_pos[a0 + x]] + 1;
+                depth = std::max(depth, level[x]);
+            }
+            narrow[i] = n < 128ll * (depth + 1) ? 1 : 0;
         }
     }
     const int64_t big_words = (max_big + kWave - 1) / kWave * kWave;
@@ -2195,6 +2226,7 @@ int setup_blocked(mcb_solver *s, const std::vector<uint32_t> &order, NposFn npos
     int64_t &B = s->bytes;
     int rc;
     if ((rc = upload(&s->q_big_list, big.data(), big.size(), B))) return rc;
+    if ((rc = upload(&s->q_big_narrow, narrow.data(), narrow.size(), B))) return rc;
     if ((rc = upload(&s->q_small_list, small.data(), small.size(), B))) return rc;
     if ((rc = dev_alloc(&s->Tt, static_cast<size_t>(s->npad), B))) return rc;
     if ((rc = dev_alloc(&s->Qr, static_cast<size_t>(s->K), B))) return rc;
@@ -2235,7 +2267,7 @@ void mcb_solver_destroy(mcb_solver *s) {
                     s->root_min_rank, s->rc_off, s->rc_pu, s->rc_pv, s->rc_rank, s->rc_coord, s->cand_root,
                     s->cand_pu, s->cand_pv, s->cand_coord, s->Sw, s->Scur, s->C, s->C_store, s->D, s->Dcount,
                     s->best, s->done, s->status, s->wave_stats, s->flag, s->E, s->Ecount, s->cdot,
-                    s->q_big_list, s->q_small_list, s->Tt, s->Qr, s->qbits, s->sum_q, s->blk_q, s->blk_m, s->blk_g, s->C_blk,
+                    s->q_big_list, s->q_big_narrow, s->q_small_list, s->Tt, s->Qr, s->qbits, s->sum_q, s->blk_q, s->blk_m, s->blk_g, s->C_blk,
                     s->Sf_blk, s->sum_cnt, s->sum_rank, s->D_blk, s->Dcnt_blk, s->upd_bytes,
                     s->Ct, s->Bc, s->pair_c, s->pw_list, s->pw_off, s->pw_cnt, s->pair_i};
     for (void *p : ptrs)
@@ -2379,7 +2411,7 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
     blocks = std::min<int64_t>(blocks, s->n_big + (s->n_small + kBlockWaves - 1) / kBlockWaves);
     s->blocks = static_cast<int>(std::max<int64_t>(1, blocks));
     if ((rc = dev_alloc(&s->wave_stats, 3 * static_cast<size_t>(s->blocks) * s->wpb, B))) return cleanup(rc);
-    if ((rc = setup_blocked(s, order, npos_of))) return cleanup(rc);
+    if ((rc = setup_blocked(s, order, npos_of, d->tree_offset, d->parent_pos))) return cleanup(rc);
     if (s->opt.timing) {
         s->ev_pool.resize(4096);
         s->ev_kind.resize(2048);
