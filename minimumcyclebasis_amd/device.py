@@ -17,6 +17,7 @@ import numpy as np
 from . import _lib
 from ._hipsig import SolverDesc, SolverOptions
 from .host import Problem
+from .stats import RunStats, stats_of
 
 KERNELS = ("parity_search", "select", "cycle", "update", "transpose")
 
@@ -221,11 +222,16 @@ class MCBResult:
     num_cycles: int
     selected: np.ndarray         # candidate rank chosen in each phase
     cycles: tuple                # (root vertex, u, v, weight) arrays in phase order, reduced-graph ids
+    stats: RunStats | None = None  # the reference's stats CSV row (minimumcyclebasis_amd.stats)
 
 
 def minimum_cycle_basis(source, reduce_degree_two: bool = True, prune_trees: bool = True, num_threads: int = 0,
-                        fvs_order=None) -> MCBResult:
-    """Minimum cycle basis weight and cycles of a graph given as a .mtx path or (n, u, v, w)."""
+                        fvs_order=None, block_phases: int = 0) -> MCBResult:
+    """Minimum cycle basis weight and cycles of a graph given as a .mtx path or (n, u, v, w).  The result's
+    ``stats`` is the run's row for the reference's stats CSV (``stats.append_stats_csv``)."""
+    import time
+
+    t0 = time.perf_counter()
     if isinstance(source, Problem):
         prob = source
     elif isinstance(source, (str, bytes)) or hasattr(source, "__fspath__"):
@@ -236,17 +242,55 @@ def minimum_cycle_basis(source, reduce_degree_two: bool = True, prune_trees: boo
         prob = Problem.from_edges(n, u, v, w, reduce_degree_two=reduce_degree_two, prune_trees=prune_trees,
                                   num_threads=num_threads, fvs_order=fvs_order)
     info = prob.info
+    t_host = time.perf_counter() - t0
     if info.single_cycle:
         tw = int(info.input_total_weight)
         return MCBResult(tw, int(np.int32(np.int64(tw) & 0xFFFFFFFF).astype(np.int32)) if tw else 0, 1,
-                         np.zeros(1, dtype=np.int32), ())
-    s = Solver(prob)
+                         np.zeros(1, dtype=np.int32), (), stats_of(prob, tw, construction_trees_s=t_host))
+    t1 = time.perf_counter()
+    s = Solver(prob, timing=2, block_phases=block_phases)
     s.run()
     s.sync()
+    t_loop = time.perf_counter() - t1
     sel = s.selected()
+    dev_ms = sum(s.kernel_time(k)[0] for k in KERNELS)
+    mem_mb = s.device_bytes / 2**20
     s.close()
     r, a, b, w = prob.candidate_table()
     ww = w[sel].astype(np.int64)
     total = int(ww.sum())
+    st = stats_of(prob, total, construction_trees_s=t_host, inspection_s=t_loop, precompute_s=dev_ms / 1000.0,
+                  total_memory_mb=mem_mb, static_memory_mb=mem_mb)
     return MCBResult(total, int(np.int64(total).astype(np.int32)), int(sel.size), sel,
-                     (r[sel], a[sel], b[sel], w[sel]))
+                     (r[sel], a[sel], b[sel], w[sel]), st)
+
+
+@dataclass
+class ComponentsResult:
+    total_weight: int            # exact (int64), sum over the components
+    num_cycles: int
+    components: list             # (components.Component, MCBResult) per biconnected component with a cycle
+
+
+def minimum_cycle_basis_components(source, reduce_degree_two: bool = True, prune_trees: bool = True,
+                                   num_threads: int = 0, block_phases: int = 0) -> ComponentsResult:
+    """Minimum cycle basis of a graph solved one biconnected component at a time, in process (the reference runs
+    main.sh: bicc decomposition, relabelling, one MCB run per component).  ``source`` is a .mtx path or
+    (n, u, v, w).  Each component's result carries its stats row (``MCBResult.stats``)."""
+    from . import generators
+    from .components import split_components
+
+    if isinstance(source, (str, bytes)) or hasattr(source, "__fspath__"):
+        n, u, v, w = generators.read_mtx(source)
+    else:
+        n, u, v, w = source
+    parts = []
+    total = 0
+    cycles = 0
+    for comp in split_components(n, u, v, w):
+        res = minimum_cycle_basis((comp.n, comp.u, comp.v, comp.w), reduce_degree_two=reduce_degree_two,
+                                  prune_trees=prune_trees, num_threads=num_threads, block_phases=block_phases)
+        parts.append((comp, res))
+        total += res.total_weight
+        cycles += res.num_cycles
+    return ComponentsResult(total, cycles, parts)

@@ -140,7 +140,7 @@ def random_biconnected(n: int, avg_degree: float = 5.0, wmax: int = 1000, seed: 
     perm = rng.permutation(n).astype(np.int64)
     a = perm
     b = np.roll(perm, -1)
-    target = max(n, int(round(n * avg_degree / 2)))
+    target = min(max(n, int(round(n * avg_degree / 2))), n * (n - 1) // 2)  # a simple graph has at most n(n-1)/2
     hi, lo = _canonical_pairs(a, b)
     key = hi * n + lo
     while key.size < target:

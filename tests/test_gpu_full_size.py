@@ -87,3 +87,17 @@ def test_er20000_properties_and_prefix():
     p_o = Problem.from_edges(n, u, v, w, keep_vertex_trees=True)
     ref = pyoracle.run(p_o, phases=12)
     assert np.array_equal(sel1[:12], ref["selected"])
+
+
+def test_components_driver_matches_whole_graph():
+    from minimumcyclebasis_amd.device import minimum_cycle_basis_components
+
+    n, u, v, w, _ = G.batched_components(k=40, nmin=20, nmax=400, seed=5)
+    print("case batched_components k=40", flush=True)
+    whole = minimum_cycle_basis((n, u, v, w))
+    parts = minimum_cycle_basis_components((n, u, v, w))
+    assert parts.total_weight == whole.total_weight
+    assert parts.num_cycles == whole.num_cycles == len(u) - n + 1
+    assert len(parts.components) == 40
+    for comp, res in parts.components:
+        assert res.stats.final_cycles == comp.cycle_space_dim

@@ -194,3 +194,39 @@ def test_generators_deterministic(gen, args):
         assert np.array_equal(x, y)
     assert a[0] == b[0]
     assert not all(np.array_equal(x, y) for x, y in zip(a[1:], c[1:])) or a[0] != c[0]
+
+
+# --- stats CSV (include/stats.h:256-338) --------------------------------------------------------
+
+@pytest.mark.parametrize("name", ["grid32x32", "er2000", "delaunay2000"])
+def test_stats_row_matches_reference_csv(name, tmp_path):
+    from helpers import graph_of, load_csv
+    from minimumcyclebasis_amd.stats import CPU_COLUMNS, GPU_COLUMNS, append_stats_csv, read_stats_csv, stats_of
+
+    fx = load_csv(name)
+    n, u, v, w = graph_of(fx)
+    ref = fx["mcb_cpu"]
+    p = Problem.from_edges(n, u, v, w)
+    st = stats_of(p, ref["total_weight"], construction_trees_s=1.5, inspection_s=2.25)
+    for gpu in (False, True):
+        path = tmp_path / f"stats_{gpu}.csv"
+        append_stats_csv(str(path), st, gpu=gpu)
+        append_stats_csv(str(path), st, gpu=gpu)  # a second run appends a row, not a header
+        rows = read_stats_csv(str(path))
+        assert len(rows) == 2
+        row = rows[0]
+        assert list(row) == list(GPU_COLUMNS if gpu else CPU_COLUMNS)
+        got = [row["Total_Nodes"], row["Total_Edges"], row["New_Edges"],
+               row["Nodes_removed" if gpu else "Node_removed"], row["Fvs_size"], row["Initial_Cycles"],
+               row["Final_cycles"], row["Total_Weight"]]
+        want = [ref[k] for k in ("total_nodes", "total_edges", "new_edges", "nodes_removed", "fvs_size",
+                                 "initial_cycles", "final_cycles", "total_weight")]
+        assert got == want
+        assert row["Total_time(s)"] == pytest.approx(3.75)
+
+
+def test_stats_int32_weight_wraps_like_the_reference():
+    from minimumcyclebasis_amd.stats import RunStats
+
+    st = RunStats(1, 2, 2, 0, 1, 1, 1, 2**31 + 5)
+    assert int(st.row(False).split(",")[7]) == -(2**31) + 5
