@@ -192,6 +192,54 @@ int mcb_solver_launch_shape(const mcb_solver *s, int32_t *blocks, int32_t *waves
 void mcb_solver_destroy(mcb_solver *s);
 
 /* ---------------------------------------------------------------------------------------------
+ * Batched phase loop over many independent problems
+ *
+ * Replaces main.sh:1-30, which runs the MCB driver once per biconnected component written by
+ * bicc/bicc_decomposition (one process, one device set-up and one phase loop each).  Here the
+ * problems (normally the components of one graph, each preprocessed into an mcb_solver_desc) share
+ * one device arena.  Step s resolves phases 64s .. 64s+63 of every problem that has them, with one
+ * launch per kernel for all of them, so the number of launches follows the largest problem instead
+ * of the sum over problems.  Each problem's selections are exactly those of its own mcb_solver.
+ * ------------------------------------------------------------------------------------------- */
+
+typedef struct mcb_batch mcb_batch;
+
+/* descs[0..count-1] are validated like mcb_solver_create's; of opt, store_cycles and timing are
+ * used.  Fails with MCB_E_INVALID when a tree does not fit the blocked form's LDS budget. */
+int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solver_options *opt, mcb_batch **out);
+
+/* Back to the unit support vectors, step 0. */
+int mcb_batch_reset(mcb_batch *b);
+
+/* Steps in the whole loop (ceil of the largest phase count / 64), and the next step to run. */
+int32_t mcb_batch_steps(const mcb_batch *b);
+int32_t mcb_batch_step(const mcb_batch *b);
+
+/* Enqueues the next `steps` steps (fewer at the end). */
+int mcb_batch_run(mcb_batch *b, int32_t steps);
+
+/* Waits; MCB_E_NOT_FOUND if some phase of some problem found no odd candidate. */
+int mcb_batch_sync(mcb_batch *b);
+
+/* Candidate rank (the problem's own numbering) chosen in phases 0..count-1 of problem `problem`
+ * (index into descs); -1 for a phase not run yet. */
+int mcb_batch_selected(mcb_batch *b, int32_t problem, int32_t *out, int32_t count);
+
+/* Cycle vectors (count x W words of that problem) when created with store_cycles. */
+int mcb_batch_cycle_vectors(mcb_batch *b, int32_t problem, uint64_t *out, int32_t count);
+
+/* As mcb_solver_kernel_time, mcb_solver_device_bytes and mcb_solver_work_counters. */
+int mcb_batch_kernel_time(mcb_batch *b, int kernel, double *total_ms, int64_t *launches);
+int64_t mcb_batch_device_bytes(const mcb_batch *b);
+int mcb_batch_work_counters(mcb_batch *b, int64_t *positions, int64_t *candidates, int64_t *update_bytes);
+
+/* Launch shape of the tree pass and the number of steps. */
+int mcb_batch_launch_shape(const mcb_batch *b, int32_t *blocks, int32_t *waves_per_block, int32_t *lds_bytes,
+                           int32_t *steps);
+
+void mcb_batch_destroy(mcb_batch *b);
+
+/* ---------------------------------------------------------------------------------------------
  * Utilities
  * ------------------------------------------------------------------------------------------- */
 

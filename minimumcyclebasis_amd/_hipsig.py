@@ -67,6 +67,20 @@ SIGNATURES = {
     "mcb_solver_launch_shape": (ctypes.c_int, [P, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32),
                                                ctypes.POINTER(i32)]),
     "mcb_solver_destroy": (None, [P]),
+    "mcb_batch_create": (ctypes.c_int, [P, i32, ctypes.POINTER(SolverOptions), ctypes.POINTER(P)]),
+    "mcb_batch_reset": (ctypes.c_int, [P]),
+    "mcb_batch_steps": (i32, [P]),
+    "mcb_batch_step": (i32, [P]),
+    "mcb_batch_run": (ctypes.c_int, [P, i32]),
+    "mcb_batch_sync": (ctypes.c_int, [P]),
+    "mcb_batch_selected": (ctypes.c_int, [P, i32, P, i32]),
+    "mcb_batch_cycle_vectors": (ctypes.c_int, [P, i32, P, i32]),
+    "mcb_batch_kernel_time": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]),
+    "mcb_batch_device_bytes": (i64, [P]),
+    "mcb_batch_work_counters": (ctypes.c_int, [P, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i64)]),
+    "mcb_batch_launch_shape": (ctypes.c_int, [P, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                              ctypes.POINTER(i32)]),
+    "mcb_batch_destroy": (None, [P]),
     "mcb_hip_device_count": (ctypes.c_int, []),
     "mcb_hip_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                            ctypes.POINTER(i64)]),

@@ -722,22 +722,73 @@ struct BlockArgs {
     int32_t e, kk, N, W, n_big, n_small, big_words, small_words;
 };
 
+// One block of kk phases of one problem in that problem's own numbering: its support matrix
+// Sw [W][npad], its phase-indexed selection and the block's work arrays.  Tt, Ct and Bc start at
+// the problem's first coordinate, which is coordinate cb of the tree and candidate arrays (0 for a
+// single problem; the start of the problem's range in a batch).
+struct BlkView {
+    uint64_t *Sw;
+    int64_t npad, cb;
+    int32_t e, kk, N, W;
+    uint32_t *sel;
+    uint64_t *blk_q, *blk_m, *blk_g;
+    uint64_t *C_blk, *Sf_blk;
+    uint32_t *D_blk, *Dcnt_blk;
+    uint64_t *C_store;
+    uint64_t *Tt, *Ct, *Bc;
+    uint32_t *pw_list, *pw_off, *pw_cnt, *pair_i;
+    uint64_t *pair_c;
+};
+
+__device__ __forceinline__ BlkView view_of(const BlockArgs &a) {
+    BlkView v;
+    v.Sw = a.Sw;
+    v.npad = a.npad;
+    v.cb = 0;
+    v.e = a.e;
+    v.kk = a.kk;
+    v.N = a.N;
+    v.W = a.W;
+    v.sel = a.sel;
+    v.blk_q = a.blk_q;
+    v.blk_m = a.blk_m;
+    v.blk_g = a.blk_g;
+    v.C_blk = a.C_blk;
+    v.Sf_blk = a.Sf_blk;
+    v.D_blk = a.D_blk;
+    v.Dcnt_blk = a.Dcnt_blk;
+    v.C_store = a.C_store;
+    v.Tt = a.Tt;
+    v.Ct = a.Ct;
+    v.Bc = a.Bc;
+    v.pw_list = a.pw_list;
+    v.pw_off = a.pw_off;
+    v.pw_cnt = a.pw_cnt;
+    v.pair_i = a.pair_i;
+    v.pair_c = a.pair_c;
+    return v;
+}
+
 // Transposes the block's vectors: wave w builds Tt[64w .. 64w+63] from word w of columns
 // e .. e+kk-1 (bit b of that word in lane l is bit l of Tt[64w+b]); own bits are added.
-__global__ __launch_bounds__(256) void k_block_transpose(BlockArgs a) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t w = static_cast<int64_t>(blockIdx.x) * 4 + wave_id();
-    const int64_t wl = (static_cast<int64_t>(a.e) + a.kk + kWave - 1) / kWave;
-    if (w >= wl) return;
-    const uint64_t x = lane < a.kk ? a.Sw[w * a.npad + a.e + lane] : 0ull;
+__device__ __forceinline__ void transpose_body(const BlkView &v, int64_t w, int lane) {
+    const uint64_t x = lane < v.kk ? v.Sw[w * v.npad + v.e + lane] : 0ull;
     uint64_t mine = 0;
     for (int b = 0; b < kWave; ++b) {
         const uint64_t m = __ballot(static_cast<int>((x >> b) & 1ull));
         if (lane == b) mine = m;
     }
     const int64_t c = w * kWave + lane;
-    if (c >= a.e && c < static_cast<int64_t>(a.e) + a.kk) mine |= 1ull << (c - a.e);
-    a.Tt[c] = mine;
+    if (c >= v.e && c < static_cast<int64_t>(v.e) + v.kk) mine |= 1ull << (c - v.e);
+    v.Tt[c] = mine;
+}
+
+__global__ __launch_bounds__(256) void k_block_transpose(BlockArgs a) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t w = static_cast<int64_t>(blockIdx.x) * 4 + wave_id();
+    const int64_t wl = (static_cast<int64_t>(a.e) + a.kk + kWave - 1) / kWave;
+    if (w >= wl) return;
+    transpose_body(view_of(a), w, lane);
 }
 
 // Value of a 64-bit register in lane l (l wave-uniform).
@@ -891,12 +942,13 @@ __device__ __forceinline__ int basis_reduce_insert(uint64_t &piv, uint64_t &P, u
 // column by column, the first lane with a column's bit becoming its pivot.  Each new pivot's rank
 // and original word go to Qr/qbits.  A word dependent on earlier words of the same run is dependent
 // on earlier candidates overall, so it cannot be in the block's greedy basis and is dropped.
-__device__ __forceinline__ void run_words(const BlockArgs &a, const uint64_t *Wd, int64_t k_begin, int64_t k_end,
+template <typename A>
+__device__ __forceinline__ void run_words(const A &a, const uint64_t *Wd, int64_t k_begin, int64_t k_end, int kk,
                                           int lane) {
     constexpr int kB = 4;
     uint64_t piv = 0, P = 0;
     int npiv = 0;
-    for (int64_t k0 = k_begin; k0 < k_end && npiv < a.kk; k0 += kB * kWave) {
+    for (int64_t k0 = k_begin; k0 < k_end && npiv < kk; k0 += kB * kWave) {
         uint32_t pu[kB], pv[kB], rk[kB];
         int32_t c[kB];
 #pragma unroll
@@ -926,8 +978,9 @@ __device__ __forceinline__ void run_words(const BlockArgs &a, const uint64_t *Wd
 
 // Pass over every tree and every candidate of the block.  Big roots (more than small_words
 // positions) take a whole workgroup each, the others one wave each; both lists are handed out
-// round-robin, largest first.
-__global__ __launch_bounds__(kQWaves * kWave) void k_block_q(BlockArgs a) {
+// round-robin in list order (largest first).  kk_of(r) is the number of phases in root r's block.
+template <typename A, typename KkFn>
+__device__ __forceinline__ void tree_pass(const A &a, uint32_t n_big, uint32_t n_small, KkFn kk_of) {
     extern __shared__ uint64_t lds[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = wave_id();
@@ -938,7 +991,7 @@ __global__ __launch_bounds__(kQWaves * kWave) void k_block_q(BlockArgs a) {
     for (uint32_t i = threadIdx.x; i < nwin_big; i += blockDim.x) ready[i] = 0;
     __syncthreads();
     uint32_t gen = 0;
-    for (uint32_t it = blockIdx.x; it < static_cast<uint32_t>(a.n_big); it += gridDim.x) {
+    for (uint32_t it = blockIdx.x; it < n_big; it += gridDim.x) {
         const uint32_t r = uniform32(a.big_list[it]);
         const int64_t off = uniform64(a.tree_off[r]);
         const uint32_t npos = static_cast<uint32_t>(uniform64(a.tree_off[r + 1]) - off);
@@ -955,18 +1008,22 @@ __global__ __launch_bounds__(kQWaves * kWave) void k_block_q(BlockArgs a) {
         const int64_t cb = uniform64(a.rc_off[r]), ce = uniform64(a.rc_off[r + 1]);
         const int64_t seg = (ce - cb + nw - 1) / nw;
         const int64_t s0 = cb + seg * wid, s1 = s0 + seg < ce ? s0 + seg : ce;
-        if (s0 < s1) run_words(a, Wbig, s0, s1, lane);
+        if (s0 < s1) run_words(a, Wbig, s0, s1, kk_of(r), lane);
         __syncthreads();
     }
     // the big-tree region is free from here on; each wave takes its own slice for small roots
     uint64_t *Ws = lds + static_cast<size_t>(wid) * a.small_words;
-    for (uint32_t it = blockIdx.x * nw + wid; it < static_cast<uint32_t>(a.n_small); it += gridDim.x * nw) {
+    for (uint32_t it = blockIdx.x * nw + wid; it < n_small; it += gridDim.x * nw) {
         const uint32_t r = uniform32(a.small_list[it]);
         const int64_t off = uniform64(a.tree_off[r]);
         const uint32_t npos = static_cast<uint32_t>(uniform64(a.tree_off[r + 1]) - off);
         tree_words(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Ws, lane);
-        run_words(a, Ws, uniform64(a.rc_off[r]), uniform64(a.rc_off[r + 1]), lane);
+        run_words(a, Ws, uniform64(a.rc_off[r]), uniform64(a.rc_off[r + 1]), kk_of(r), lane);
     }
+}
+
+__global__ __launch_bounds__(kQWaves * kWave) void k_block_q(BlockArgs a) {
+    tree_pass(a, static_cast<uint32_t>(a.n_big), static_cast<uint32_t>(a.n_small), [&](uint32_t) { return a.kk; });
 }
 
 // Inserts one batch (lane i: word q, ranks ascending with the lane; q = 0 for no entry) into a
@@ -1082,16 +1139,16 @@ __device__ __forceinline__ void basis_insert_tab(uint64_t &piv, uint64_t &P, uin
     }
 }
 
-// The block's greedy basis (top summary) in rank order -> phase of each winner, masks m_i, the
-// products <C_{e+i}, S_{e+l}> and the chosen ranks.  One wave.
-__device__ void block_finalize(const BlockArgs &a, int lane) {
-    const int kk = a.kk;
-    const int64_t top = a.lvl_slot[a.levels - 1];
-    const uint32_t n = a.sum_cnt[top];
+// The block's greedy basis (slot top) in rank order -> phase of each winner, masks m_i, the
+// products <C_{e+i}, S_{e+l}> and the chosen ranks (sel[i], i < kk).  One wave.
+__device__ void finalize_core(int kk, const uint32_t *sum_cnt, const uint32_t *sum_rank, const uint64_t *sum_q,
+                              int64_t top, uint32_t *sel, uint64_t *blk_q, uint64_t *blk_m, uint64_t *blk_g,
+                              uint32_t *status, int lane) {
+    const uint32_t n = sum_cnt[top];
     bool ok = n == static_cast<uint32_t>(kk);
     if (ok) {
-        const uint64_t q = lane < kk ? a.sum_q[top * kWave + lane] : 0ull;
-        const uint32_t rk = lane < kk ? a.sum_rank[top * kWave + lane] : 0u;
+        const uint64_t q = lane < kk ? sum_q[top * kWave + lane] : 0ull;
+        const uint32_t rk = lane < kk ? sum_rank[top * kWave + lane] : 0u;
         // phase of each winner: the column at which it becomes a pivot in rank order
         uint64_t piv = 0;
         int phase_of = -1;
@@ -1125,36 +1182,36 @@ __device__ void block_finalize(const BlockArgs &a, int lane) {
             const bool odd = lane >= kk || (__popcll(m & qi) & 1);
             if (__ballot(!odd)) ok = false;
             if (lane < kk) {
-                a.sel[a.e + lane] = ri;
-                a.blk_q[lane] = qi;
-                a.blk_m[lane] = m;
-                a.blk_g[lane] = g;
+                sel[lane] = ri;
+                blk_q[lane] = qi;
+                blk_m[lane] = m;
+                blk_g[lane] = g;
             }
         }
     }
     if (!ok) {
-        if (lane == 0) atomicOr(a.status, 1u);
-        if (lane < kk) a.sel[a.e + lane] = kNoRank;
+        if (lane == 0) atomicOr(status, 1u);
+        if (lane < kk) sel[lane] = kNoRank;
     }
 }
 
-// First level of the block's greedy basis: wave c, ranks [c*kChunk, (c+1)*kChunk) -> slot c; only
-// the words the tree pass emitted take part (the others cannot be in the basis).  Lane l reads
-// bitmap word l of the chunk; a prefix sum of the popcounts lists the emitted ranks in rank order,
-// which are inserted 64 at a time.
-__global__ __launch_bounds__(256) void k_block_basis(BlockArgs a) {
-    __shared__ uint64_t s_words[4][kWave];
-    __shared__ uint32_t s_offs[4][kWave];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wv = wave_id();
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * 4 + wv;
-    if (c >= a.lvl_n[0]) return;
-    const int64_t r0 = c * kChunk;
+__device__ __forceinline__ void block_finalize(const BlockArgs &a, int lane) {
+    finalize_core(a.kk, a.sum_cnt, a.sum_rank, a.sum_q, a.lvl_slot[a.levels - 1], a.sel + a.e, a.blk_q, a.blk_m,
+                  a.blk_g, a.status, lane);
+}
+
+// First level of the block's greedy basis for the ranks [r0, r0 + 64 * nwords) (r0 a multiple of
+// 64, nwords <= 64) into one slot; only the words the tree pass emitted take part (the others cannot
+// be in the basis), and the bitmap words read are cleared for the next block.  Lane l reads bitmap
+// word l; a prefix sum of the popcounts lists the emitted ranks in rank order, which are inserted 64
+// at a time.
+__device__ __forceinline__ void basis_chunk(uint64_t *qbits, const uint64_t *Qr, int64_t r0, int nwords, int64_t slot,
+                                            int kk, uint32_t *sum_cnt, uint32_t *sum_rank, uint64_t *sum_q,
+                                            uint64_t *s_words, uint32_t *s_offs, int lane) {
     const int64_t wbase = r0 >> 6;
-    const int64_t nwords = (a.K + kWave - 1) / kWave;
-    const bool valid = wbase + lane < nwords;
-    const uint64_t word = valid ? a.qbits[wbase + lane] : 0ull;
-    if (word) a.qbits[wbase + lane] = 0;  // ready for the next block
+    const bool valid = lane < nwords;
+    const uint64_t word = valid ? qbits[wbase + lane] : 0ull;
+    if (word) qbits[wbase + lane] = 0;  // ready for the next block
     const uint32_t own = static_cast<uint32_t>(__popcll(word));
     uint32_t incl = own;
 #pragma unroll
@@ -1163,13 +1220,12 @@ __global__ __launch_bounds__(256) void k_block_basis(BlockArgs a) {
         if (lane >= o) incl += t;
     }
     const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), kWave - 1));
-    s_words[wv][lane] = word;
-    s_offs[wv][lane] = incl - own;
+    s_words[lane] = word;
+    s_offs[lane] = incl - own;
     __builtin_amdgcn_wave_barrier();
     uint64_t piv = 0, P = 0;
     uint32_t cnt = 0;
-    const int64_t slot = a.lvl_slot[0] + c;
-    for (uint32_t b0 = 0; b0 < total && cnt < static_cast<uint32_t>(a.kk); b0 += kWave) {
+    for (uint32_t b0 = 0; b0 < total && cnt < static_cast<uint32_t>(kk); b0 += kWave) {
         const uint32_t t = b0 + lane;
         uint64_t q = 0;
         uint32_t rank = 0;
@@ -1177,27 +1233,92 @@ __global__ __launch_bounds__(256) void k_block_basis(BlockArgs a) {
             int lo = 0, hi = kWave - 1;  // the last lane whose first entry is at or before t owns entry t
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
-                if (s_offs[wv][mid] <= t) lo = mid;
+                if (s_offs[mid] <= t) lo = mid;
                 else hi = mid - 1;
             }
-            uint64_t w = s_words[wv][lo];
-            for (uint32_t k = t - s_offs[wv][lo]; k > 0; --k) w &= w - 1;
+            uint64_t w = s_words[lo];
+            for (uint32_t k = t - s_offs[lo]; k > 0; --k) w &= w - 1;
             rank = static_cast<uint32_t>(r0 + lo * kWave + (__ffsll(static_cast<unsigned long long>(w)) - 1));
-            q = a.Qr[rank];
+            q = Qr[rank];
         }
-        basis_insert(piv, P, cnt, q, rank, a.sum_rank + slot * kWave, a.sum_q + slot * kWave, lane);
+        basis_insert(piv, P, cnt, q, rank, sum_rank + slot * kWave, sum_q + slot * kWave, lane);
     }
-    if (lane == 0) a.sum_cnt[slot] = cnt;
+    if (lane == 0) sum_cnt[slot] = cnt;
+}
+
+// Wave c: ranks [c*kChunk, (c+1)*kChunk) -> slot c of the first level.
+__global__ __launch_bounds__(256) void k_block_basis(BlockArgs a) {
+    __shared__ uint64_t s_words[4][kWave];
+    __shared__ uint32_t s_offs[4][kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = wave_id();
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * 4 + wv;
+    if (c >= a.lvl_n[0]) return;
+    const int64_t r0 = c * kChunk;
+    const int64_t nwords = (a.K + kWave - 1) / kWave - (r0 >> 6);
+    basis_chunk(a.qbits, a.Qr, r0, static_cast<int>(nwords < kWave ? nwords : kWave), a.lvl_slot[0] + c, a.kk,
+                a.sum_cnt, a.sum_rank, a.sum_q, s_words[wv], s_offs[wv], lane);
     if (a.levels == 1) {
         __threadfence();
         block_finalize(a, lane);
     }
 }
 
+// Merges the `members` bases in slots s0 .. s0+members-1 into slot `slot`.  Their entries,
+// concatenated in order, are in rank order; a prefix sum of the bases' sizes locates entry t, and the
+// entries are inserted 64 at a time with the next batch's loads in flight.  One wave; s_offs: 64
+// words, tab: the wave's 8 x 256 reduction table.
+__device__ __forceinline__ void merge_group(int64_t s0, int members, int64_t slot, int kk, uint32_t *sum_cnt,
+                                            uint32_t *sum_rank, uint64_t *sum_q, uint32_t *s_offs, uint64_t *tab,
+                                            int lane) {
+    const uint32_t own = lane < members ? sum_cnt[s0 + lane] : 0u;
+    uint32_t incl = own;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), kWave - 1));
+    s_offs[lane] = incl - own;
+    __builtin_amdgcn_wave_barrier();
+    // entry t of the concatenation: base "lo" (the last base starting at or before t), position t - start
+    auto locate = [&](uint32_t t) -> int64_t {
+        int lo = 0, hi = kWave - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_offs[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        return (s0 + lo) * kWave + (t - s_offs[lo]);
+    };
+    uint64_t piv = 0, P = 0;
+    uint32_t cnt = 0;
+    bool stale = true;
+    uint64_t qn = 0;
+    uint32_t rn = 0;
+    if (lane < static_cast<int>(total)) {
+        const int64_t at = locate(lane);
+        qn = sum_q[at];
+        rn = sum_rank[at];
+    }
+    for (uint32_t b0 = 0; b0 < total && cnt < static_cast<uint32_t>(kk); b0 += kWave) {
+        const uint64_t q = qn;
+        const uint32_t r = rn;
+        const uint32_t tn = b0 + kWave + lane;
+        qn = 0;
+        rn = 0;
+        if (tn < total) {
+            const int64_t at = locate(tn);
+            qn = sum_q[at];
+            rn = sum_rank[at];
+        }
+        basis_insert_tab(piv, P, cnt, stale, tab, q, r, sum_rank + slot * kWave, sum_q + slot * kWave, lane);
+    }
+    if (lane == 0) sum_cnt[slot] = cnt;
+}
+
 // One merge level: wave g merges bases g*kGroup .. g*kGroup+kGroup-1 of level L into slot g of level
-// L+1.  Their entries, concatenated in order, are in rank order; a prefix sum of the bases' sizes
-// locates entry t, and the entries are inserted 64 at a time with the next batch's loads in
-// flight.  At the top level the single wave also finalises the block.  The bases of level L were
+// L+1.  At the top level the single wave also finalises the block.  The bases of level L were
 // written by the previous launch.
 constexpr int kMergeWaves = 2;  // waves per workgroup of k_block_merge (16 KB of LDS table each)
 __global__ __launch_bounds__(kMergeWaves * kWave) void k_block_merge(BlockArgs a, int L) {
@@ -1209,53 +1330,8 @@ __global__ __launch_bounds__(kMergeWaves * kWave) void k_block_merge(BlockArgs a
     if (g >= a.lvl_n[L + 1]) return;
     const int64_t rest = a.lvl_n[L] - g * kGroup;
     const int members = static_cast<int>(rest < kGroup ? rest : kGroup);
-    const int64_t s0 = a.lvl_slot[L] + g * kGroup;
-    const uint32_t own = lane < members ? a.sum_cnt[s0 + lane] : 0u;
-    uint32_t incl = own;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const uint32_t t = __shfl_up(incl, o);
-        if (lane >= o) incl += t;
-    }
-    const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), kWave - 1));
-    s_offs[wv][lane] = incl - own;
-    __builtin_amdgcn_wave_barrier();
-    // entry t of the concatenation: base "lo" (the last base starting at or before t), position t - start
-    auto locate = [&](uint32_t t) -> int64_t {
-        int lo = 0, hi = kWave - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_offs[wv][mid] <= t) lo = mid;
-            else hi = mid - 1;
-        }
-        return (s0 + lo) * kWave + (t - s_offs[wv][lo]);
-    };
-    uint64_t piv = 0, P = 0;
-    uint32_t cnt = 0;
-    bool stale = true;
-    const int64_t slot = a.lvl_slot[L + 1] + g;
-    uint64_t qn = 0;
-    uint32_t rn = 0;
-    if (lane < static_cast<int>(total)) {
-        const int64_t at = locate(lane);
-        qn = a.sum_q[at];
-        rn = a.sum_rank[at];
-    }
-    for (uint32_t b0 = 0; b0 < total && cnt < static_cast<uint32_t>(a.kk); b0 += kWave) {
-        const uint64_t q = qn;
-        const uint32_t r = rn;
-        const uint32_t tn = b0 + kWave + lane;
-        qn = 0;
-        rn = 0;
-        if (tn < total) {
-            const int64_t at = locate(tn);
-            qn = a.sum_q[at];
-            rn = a.sum_rank[at];
-        }
-        basis_insert_tab(piv, P, cnt, stale, s_tab[wv], q, r, a.sum_rank + slot * kWave, a.sum_q + slot * kWave,
-                         lane);
-    }
-    if (lane == 0) a.sum_cnt[slot] = cnt;
+    merge_group(a.lvl_slot[L] + g * kGroup, members, a.lvl_slot[L + 1] + g, a.kk, a.sum_cnt, a.sum_rank, a.sum_q,
+                s_offs[wv], s_tab[wv], lane);
     if (L + 2 == a.levels) {
         __threadfence();
         block_finalize(a, lane);
@@ -1264,18 +1340,20 @@ __global__ __launch_bounds__(kMergeWaves * kWave) void k_block_merge(BlockArgs a
 
 // C of one chosen candidate into LDS (W words, cleared by the caller): the coordinates of the edges
 // on root -> u and root -> v, found by testing every position of the root's tree against the DFS
-// intervals of u and v, and the candidate's own edge.
+// intervals of u and v, and the candidate's own edge.  Coordinates are counted from cb.
 __device__ void cycle_into_lds(const int64_t *tree_off, const uint32_t *pos_pre, const uint32_t *pos_end,
                                const int32_t *pos_coord, uint32_t root, uint32_t cu, uint32_t cv, int32_t ccoord,
-                               uint64_t *Cl) {
+                               int64_t cb, uint64_t *Cl) {
     constexpr int kB = 8;
     const int tid = threadIdx.x;
     const int64_t off = tree_off[root];
     const uint32_t npos = static_cast<uint32_t>(tree_off[root + 1] - off);
     const uint32_t pu = pos_pre[off + cu];
     const uint32_t pv = pos_pre[off + cv];
-    if (tid == 0 && ccoord >= 0)
-        atomicOr(reinterpret_cast<unsigned long long *>(&Cl[ccoord >> 6]), 1ull << (ccoord & 63));
+    if (tid == 0 && ccoord >= 0) {
+        const int64_t c = ccoord - cb;
+        atomicOr(reinterpret_cast<unsigned long long *>(&Cl[c >> 6]), 1ull << (c & 63));
+    }
     for (uint32_t x0 = 1 + tid; x0 < npos; x0 += kB * blockDim.x) {
         uint32_t lo[kB], hi[kB];
         int32_t c[kB];
@@ -1290,85 +1368,85 @@ __device__ void cycle_into_lds(const int64_t *tree_off, const uint32_t *pos_pre,
 #pragma unroll
         for (int k = 0; k < kB; ++k) {
             const bool on = (lo[k] <= pu && pu < hi[k]) || (lo[k] <= pv && pv < hi[k]);
-            if (on && c[k] >= 0)
-                atomicOr(reinterpret_cast<unsigned long long *>(&Cl[c[k] >> 6]), 1ull << (c[k] & 63));
+            if (on && c[k] >= 0) {
+                const int64_t cl = c[k] - cb;
+                atomicOr(reinterpret_cast<unsigned long long *>(&Cl[cl >> 6]), 1ull << (cl & 63));
+            }
         }
     }
 }
 
-// Workgroup i: C_{e+i} of phase e+i's winner (and its nonzero words below e), and the final
-// support vector S_{e+i} = sum over l of m_i[l] T_l.
+// C_{e+i} of phase e+i's winner (and its nonzero words below e), and the final support vector
+// S_{e+i} = sum over l of m_i[l] T_l.  One workgroup; Cl: W words of LDS, cnt: one LDS counter.
+template <typename A>
+__device__ __forceinline__ void cycles_body(const A &a, const BlkView &v, int i, uint64_t *Cl, uint32_t *cnt) {
+    const int tid = threadIdx.x;
+    for (int w = tid; w < v.W; w += blockDim.x) Cl[w] = 0;
+    if (tid == 0) *cnt = 0;
+    __syncthreads();
+    const uint32_t rank = v.sel[v.e + i];
+    if (rank < a.K)
+        cycle_into_lds(a.tree_off, a.pos_pre, a.pos_end, a.pos_coord, a.cand_root[rank], a.cand_pu[rank],
+                       a.cand_pv[rank], a.cand_coord[rank], v.cb, Cl);
+    __syncthreads();
+    const int32_t dlim = (v.e + kWave - 1) / kWave;
+    uint64_t *Cout = v.C_blk + static_cast<int64_t>(i) * v.W;
+    uint32_t *Dout = v.D_blk + static_cast<int64_t>(i) * v.W;
+    for (int w = tid; w < v.W; w += blockDim.x) {
+        const uint64_t x = Cl[w];
+        Cout[w] = x;
+        if (v.C_store) v.C_store[(static_cast<int64_t>(v.e) + i) * v.W + w] = x;
+        if (w < dlim && x) Dout[atomicAdd(cnt, 1u)] = static_cast<uint32_t>(w);
+    }
+    const uint64_t m = v.blk_m[i];
+    const int32_t wl = (v.e + v.kk + kWave - 1) / kWave;
+    for (int w = tid; w < wl; w += blockDim.x) {
+        uint64_t x = 0;
+        for (int l = 0; l < v.kk; ++l) {
+            if (!((m >> l) & 1ull)) continue;
+            const int64_t j = static_cast<int64_t>(v.e) + l;
+            uint64_t t = v.Sw[static_cast<int64_t>(w) * v.npad + j];
+            if ((j >> 6) == w) t |= 1ull << (j & 63);
+            x ^= t;
+        }
+        v.Sf_blk[static_cast<int64_t>(i) * v.W + w] = x;
+    }
+    __syncthreads();
+    if (tid == 0) v.Dcnt_blk[i] = *cnt;
+}
+
+// Workgroup i: phase e+i of the block.
 __global__ __launch_bounds__(256) void k_block_cycles(BlockArgs a) {
     extern __shared__ uint64_t lds[];
     __shared__ uint32_t cnt;
-    const int i = blockIdx.x;
-    const int tid = threadIdx.x;
-    uint64_t *Cl = lds;
-    for (int w = tid; w < a.W; w += blockDim.x) Cl[w] = 0;
-    if (tid == 0) cnt = 0;
-    __syncthreads();
-    const uint32_t rank = a.sel[a.e + i];
-    if (rank < a.K)
-        cycle_into_lds(a.tree_off, a.pos_pre, a.pos_end, a.pos_coord, a.cand_root[rank], a.cand_pu[rank],
-                       a.cand_pv[rank], a.cand_coord[rank], Cl);
-    __syncthreads();
-    const int32_t dlim = (a.e + kWave - 1) / kWave;
-    uint64_t *Cout = a.C_blk + static_cast<int64_t>(i) * a.W;
-    uint32_t *Dout = a.D_blk + static_cast<int64_t>(i) * a.W;
-    for (int w = tid; w < a.W; w += blockDim.x) {
-        const uint64_t v = Cl[w];
-        Cout[w] = v;
-        if (a.C_store) a.C_store[(static_cast<int64_t>(a.e) + i) * a.W + w] = v;
-        if (w < dlim && v) Dout[atomicAdd(&cnt, 1u)] = static_cast<uint32_t>(w);
-    }
-    const uint64_t m = a.blk_m[i];
-    const int32_t wl = (a.e + a.kk + kWave - 1) / kWave;
-    for (int w = tid; w < wl; w += blockDim.x) {
-        uint64_t v = 0;
-        for (int l = 0; l < a.kk; ++l) {
-            if (!((m >> l) & 1ull)) continue;
-            const int64_t j = static_cast<int64_t>(a.e) + l;
-            uint64_t t = a.Sw[static_cast<int64_t>(w) * a.npad + j];
-            if ((j >> 6) == w) t |= 1ull << (j & 63);
-            v ^= t;
-        }
-        a.Sf_blk[static_cast<int64_t>(i) * a.W + w] = v;
-    }
-    __syncthreads();
-    if (tid == 0) a.Dcnt_blk[i] = cnt;
+    cycles_body(a, view_of(a), blockIdx.x, lds, &cnt);
 }
 
-// Inputs of the update that every column shares.  Workgroup 0 lists the words below e where some
-// C_{e+i} is nonzero, each with its (i, word of C_{e+i}) pairs; the other workgroups transpose the
-// block's cycle vectors (one wave per word: Ct[64w + b] bit i = bit b of word w of C_{e+i}).
-__global__ __launch_bounds__(256) void k_block_pairs(BlockArgs a) {
-    extern __shared__ uint64_t lds[];
-    __shared__ uint32_t nlist, npairs;
-    const int tid = threadIdx.x;
-    if (blockIdx.x > 0) {  // transpose
-        const int lane = tid & (kWave - 1);
-        const int64_t w = static_cast<int64_t>(blockIdx.x - 1) * 4 + tid / kWave;
-        if (w >= a.W) return;
-        const uint64_t x = lane < a.kk ? a.C_blk[static_cast<int64_t>(lane) * a.W + w] : 0ull;
-        uint64_t mine = 0;
-        for (int b = 0; b < kWave; ++b) {
-            const uint64_t m = __ballot(static_cast<int>((x >> b) & 1ull));
-            if (lane == b) mine = m;
-        }
-        a.Ct[w * kWave + lane] = mine;
-        return;
+// Ct[64w + b] bit i = bit b of word w of C_{e+i}: one wave per word.
+__device__ __forceinline__ void pairs_transpose(const BlkView &v, int64_t w, int lane) {
+    const uint64_t x = lane < v.kk ? v.C_blk[static_cast<int64_t>(lane) * v.W + w] : 0ull;
+    uint64_t mine = 0;
+    for (int b = 0; b < kWave; ++b) {
+        const uint64_t m = __ballot(static_cast<int>((x >> b) & 1ull));
+        if (lane == b) mine = m;
     }
-    // U[w]: bit i set when C_{e+i} is nonzero at word w (w below e); then the listed words in
-    // ascending order with the prefix sums of their pair counts, so that the pairs of consecutive
-    // listed words are contiguous.
-    const int32_t dlim = (a.e + kWave - 1) / kWave;
+    v.Ct[w * kWave + lane] = mine;
+}
+
+// The words below e where some C_{e+i} is nonzero, in ascending order, each with its (i, word of
+// C_{e+i}) pairs and the prefix sums of the pair counts, so that the pairs of consecutive listed
+// words are contiguous.  One workgroup; lds: ceil(e/64) words + 2 * blockDim.x u32.
+__device__ __forceinline__ void pairs_list(const BlkView &v, uint64_t *lds, uint32_t *nlist, uint32_t *npairs) {
+    const int tid = threadIdx.x;
+    // U[w]: bit i set when C_{e+i} is nonzero at word w (w below e)
+    const int32_t dlim = (v.e + kWave - 1) / kWave;
     uint64_t *U = lds;
     uint32_t *part = reinterpret_cast<uint32_t *>(U + dlim);  // [2 * blockDim.x]: words, pairs per thread
     for (int x = tid; x < dlim; x += blockDim.x) U[x] = 0;
     __syncthreads();
-    for (int i = 0; i < a.kk; ++i) {
-        const uint32_t nd = a.Dcnt_blk[i];
-        const uint32_t *Di = a.D_blk + static_cast<int64_t>(i) * a.W;
+    for (int i = 0; i < v.kk; ++i) {
+        const uint32_t nd = v.Dcnt_blk[i];
+        const uint32_t *Di = v.D_blk + static_cast<int64_t>(i) * v.W;
         for (uint32_t q = tid; q < nd; q += blockDim.x)
             atomicOr(reinterpret_cast<unsigned long long *>(&U[Di[q]]), 1ull << i);
     }
@@ -1393,79 +1471,92 @@ __global__ __launch_bounds__(256) void k_block_pairs(BlockArgs a) {
             sw += a1;
             sp += a2;
         }
-        nlist = sw;
-        npairs = sp;
+        *nlist = sw;
+        *npairs = sp;
     }
     __syncthreads();
     uint32_t x = part[tid], off = part[blockDim.x + tid];
     for (int w = w0; w < w1; ++w) {
         uint64_t m = U[w];
         if (!m) continue;
-        a.pw_list[x] = static_cast<uint32_t>(w);
-        a.pw_off[x] = off;
+        v.pw_list[x] = static_cast<uint32_t>(w);
+        v.pw_off[x] = off;
         ++x;
         while (m) {
             const int i = __ffsll(static_cast<unsigned long long>(m)) - 1;
             m &= m - 1;
-            a.pair_i[off] = static_cast<uint32_t>(i);
-            a.pair_c[off] = a.C_blk[static_cast<int64_t>(i) * a.W + w];
+            v.pair_i[off] = static_cast<uint32_t>(i);
+            v.pair_c[off] = v.C_blk[static_cast<int64_t>(i) * v.W + w];
             ++off;
         }
     }
     if (tid == 0) {
-        a.pw_off[nlist] = npairs;
-        *a.pw_cnt = nlist;
+        v.pw_off[*nlist] = *npairs;
+        *v.pw_cnt = *nlist;
     }
+}
+
+// Inputs of the update that every column shares: workgroup 0 lists the words, the others
+// transpose the block's cycle vectors (4 words each).
+__global__ __launch_bounds__(256) void k_block_pairs(BlockArgs a) {
+    extern __shared__ uint64_t lds[];
+    __shared__ uint32_t nlist, npairs;
+    const BlkView v = view_of(a);
+    if (blockIdx.x > 0) {
+        const int64_t w = static_cast<int64_t>(blockIdx.x - 1) * 4 + wave_id();
+        if (w < v.W) pairs_transpose(v, w, threadIdx.x & (kWave - 1));
+        return;
+    }
+    pairs_list(v, lds, &nlist, &npairs);
 }
 
 // The block's update of a later column j >= e+kk is S_j ^= sum_i b_i S_{e+i} with
 // b_i = <C_{e+i}, S_j> ^ sum_{l<i} b_l <C_{e+i}, S_{e+l}>.  S_j has support below e besides its own
 // bit, so the products read only the listed words (where some C_{e+i} is nonzero below e) plus bit j
-// of each C (Ct[j]).  k_block_dot: thread j computes b into Bc[j].  The list and its pairs are
-// staged in LDS in segments of at most kSegWords words and kSegPairs pairs.
+// of each C (Ct[j]).  Column tile t: thread j = e + kk + 256 t + tid computes b into Bc[j].  The list
+// and its pairs are staged in LDS in segments of at most kSegWords words and kSegPairs pairs.
 // LDS: G rows [64] | pair words [kSegPairs] | pair phases [kSegPairs] | listed words [kSegWords] |
 // their pair offsets [kSegWords + 1].
 constexpr int kSegWords = 1024;
 constexpr int kSegPairs = 2048;
-__global__ __launch_bounds__(256) void k_block_dot(BlockArgs a) {
-    extern __shared__ uint64_t lds[];
-    __shared__ uint32_t seg_end;
+__device__ __forceinline__ void dot_body(const BlkView &v, int64_t tile, uint64_t *lds, uint32_t *seg_end,
+                                         unsigned long long *upd_bytes) {
     constexpr int kB = 8;
     const int tid = threadIdx.x;
-    const int kk = a.kk;
+    const int kk = v.kk;
     uint64_t *G = lds;
     uint64_t *Pc = G + kBlockK;
     uint32_t *Pi = reinterpret_cast<uint32_t *>(Pc + kSegPairs);
     uint32_t *Lw = Pi + kSegPairs;
     uint32_t *Lo = Lw + kSegWords;
-    for (int x = tid; x < kBlockK; x += blockDim.x) G[x] = x < kk ? a.blk_g[x] : 0ull;
-    const int64_t j = static_cast<int64_t>(a.e) + kk + static_cast<int64_t>(blockIdx.x) * blockDim.x + tid;
-    const bool col = j < a.N;
-    uint64_t d = col ? a.Ct[j] : 0ull;
-    const uint32_t nl = *a.pw_cnt;
+    for (int x = tid; x < kBlockK; x += blockDim.x) G[x] = x < kk ? v.blk_g[x] : 0ull;
+    const int64_t j = static_cast<int64_t>(v.e) + kk + tile * blockDim.x + tid;
+    const bool col = j < v.N;
+    uint64_t d = col ? v.Ct[j] : 0ull;
+    const uint32_t nl = *v.pw_cnt;
     for (uint32_t s0 = 0; s0 < nl;) {
         __syncthreads();
         if (tid == 0) {  // the longest run of listed words from s0 whose pairs fit
-            const uint32_t p0 = a.pw_off[s0];
+            const uint32_t p0 = v.pw_off[s0];
             uint32_t lo = s0 + 1, hi = s0 + kSegWords < nl ? s0 + kSegWords : nl;
             while (lo < hi) {
                 const uint32_t mid = (lo + hi + 1) / 2;
-                if (a.pw_off[mid] - p0 <= static_cast<uint32_t>(kSegPairs)) lo = mid;
+                if (v.pw_off[mid] - p0 <= static_cast<uint32_t>(kSegPairs)) lo = mid;
                 else hi = mid - 1;
             }
-            seg_end = lo;
+            *seg_end = lo;
         }
         __syncthreads();
-        const uint32_t s1 = seg_end;
-        const uint32_t p0 = a.pw_off[s0], p1 = a.pw_off[s1];
+        const uint32_t s1 = *seg_end;
+        const uint32_t p0 = v.pw_off[s0], p1 = v.pw_off[s1];
         for (uint32_t x = tid; x < s1 - s0; x += blockDim.x) {
-            Lw[x] = a.pw_list[s0 + x];
-            Lo[x] = a.pw_off[s0 + x] - p0;
+            Lw[x] = v.pw_list[s0 + x];
+            Lo[x] = v.pw_off[s0 + x] - p0;
         }
         if (tid == 0) Lo[s1 - s0] = p1 - p0;
         for (uint32_t q = tid; q < p1 - p0; q += blockDim.x) {
-            Pc[q] = a.pair_c[p0 + q];
-            Pi[q] = a.pair_i[p0 + q];
+            Pc[q] = v.pair_c[p0 + q];
+            Pi[q] = v.pair_i[p0 + q];
         }
         __syncthreads();
         if (col) {
@@ -1474,7 +1565,7 @@ __global__ __launch_bounds__(256) void k_block_dot(BlockArgs a) {
 #pragma unroll
                 for (int k = 0; k < kB; ++k) {
                     const uint32_t x = x0 + k < s1 - s0 ? x0 + k : x0;
-                    sv[k] = a.Sw[static_cast<int64_t>(Lw[x]) * a.npad + j];
+                    sv[k] = v.Sw[static_cast<int64_t>(Lw[x]) * v.npad + j];
                 }
 #pragma unroll
                 for (int k = 0; k < kB; ++k) {
@@ -1494,41 +1585,47 @@ __global__ __launch_bounds__(256) void k_block_dot(BlockArgs a) {
             const uint64_t bi = ((d >> i) ^ static_cast<uint64_t>(__popcll(b & G[i]))) & 1ull;
             b |= bi << i;
         }
-        a.Bc[j] = b;
+        v.Bc[j] = b;
         nbytes = 2 + 8ull * nl;
     }
     for (int o = kWave / 2; o > 0; o >>= 1) nbytes += __shfl_down(nbytes, o);
-    if ((tid & (kWave - 1)) == 0 && nbytes) atomicAdd(a.upd_bytes, static_cast<unsigned long long>(nbytes));
+    if ((tid & (kWave - 1)) == 0 && nbytes) atomicAdd(upd_bytes, static_cast<unsigned long long>(nbytes));
 }
 
-// Workgroup (x, y): columns e + 256x .. +255, words kTile*y .. +kTile-1.  Column e+i < e+kk receives
-// its final vector S_{e+i}; a later column j with Bc[j] != 0 takes S_j ^= sum_i b_i S_{e+i}.
-// LDS: the block's final vectors on the workgroup's words [64][kTile].
-__global__ __launch_bounds__(256) void k_block_xor(BlockArgs a) {
+__global__ __launch_bounds__(256) void k_block_dot(BlockArgs a) {
     extern __shared__ uint64_t lds[];
+    __shared__ uint32_t seg_end;
+    dot_body(view_of(a), blockIdx.x, lds, &seg_end, a.upd_bytes);
+}
+
+// Tile (x, y): columns e + 256x .. +255, words kTile*y .. +kTile-1.  Column e+i < e+kk receives its
+// final vector S_{e+i}; a later column j with Bc[j] != 0 takes S_j ^= sum_i b_i S_{e+i}.
+// LDS: the block's final vectors on the tile's words [64][kTile].
+__device__ __forceinline__ void xor_body(const BlkView &v, int64_t xt, int32_t yt, uint64_t *lds,
+                                         unsigned long long *upd_bytes) {
     constexpr int kW = 8;
     const int tid = threadIdx.x;
-    const int kk = a.kk;
-    const int32_t wl = (a.e + kk + kWave - 1) / kWave;
-    const int32_t w0 = static_cast<int32_t>(blockIdx.y) * kTile;
+    const int kk = v.kk;
+    const int32_t wl = (v.e + kk + kWave - 1) / kWave;
+    const int32_t w0 = yt * kTile;
     const int tn = wl - w0 < kTile ? wl - w0 : kTile;
     uint64_t *T = lds;
     for (int x = tid; x < kBlockK * kTile; x += blockDim.x) {
         const int i = x / kTile, t = x % kTile;
-        T[x] = (i < kk && t < tn) ? a.Sf_blk[static_cast<int64_t>(i) * a.W + w0 + t] : 0ull;
+        T[x] = (i < kk && t < tn) ? v.Sf_blk[static_cast<int64_t>(i) * v.W + w0 + t] : 0ull;
     }
     __syncthreads();
-    const int64_t j = static_cast<int64_t>(a.e) + static_cast<int64_t>(blockIdx.x) * blockDim.x + tid;
+    const int64_t j = static_cast<int64_t>(v.e) + xt * blockDim.x + tid;
     uint64_t nbytes = 0;
-    if (j < static_cast<int64_t>(a.e) + kk) {
-        const int i = static_cast<int>(j - a.e);
+    if (j < static_cast<int64_t>(v.e) + kk) {
+        const int i = static_cast<int>(j - v.e);
         for (int t = 0; t < tn; ++t) {
-            uint64_t v = T[i * kTile + t];
-            if ((j >> 6) == w0 + t) v &= ~(1ull << (j & 63));
-            a.Sw[static_cast<int64_t>(w0 + t) * a.npad + j] = v;
+            uint64_t x = T[i * kTile + t];
+            if ((j >> 6) == w0 + t) x &= ~(1ull << (j & 63));
+            v.Sw[static_cast<int64_t>(w0 + t) * v.npad + j] = x;
         }
-    } else if (j < a.N) {
-        const uint64_t b = a.Bc[j];
+    } else if (j < v.N) {
+        const uint64_t b = v.Bc[j];
         if (b) {
             for (int t0 = 0; t0 < tn; t0 += kW) {
                 uint64_t xs[kW];
@@ -1545,19 +1642,250 @@ __global__ __launch_bounds__(256) void k_block_xor(BlockArgs a) {
 #pragma unroll
                 for (int u = 0; u < kW; ++u) {
                     if (t0 + u >= tn) xs[u] = 0;
-                    sv[u] = xs[u] ? a.Sw[static_cast<int64_t>(w0 + t0 + u) * a.npad + j] : 0ull;
+                    sv[u] = xs[u] ? v.Sw[static_cast<int64_t>(w0 + t0 + u) * v.npad + j] : 0ull;
                 }
 #pragma unroll
                 for (int u = 0; u < kW; ++u)
                     if (xs[u]) {
-                        a.Sw[static_cast<int64_t>(w0 + t0 + u) * a.npad + j] = sv[u] ^ xs[u];
+                        v.Sw[static_cast<int64_t>(w0 + t0 + u) * v.npad + j] = sv[u] ^ xs[u];
                         nbytes += 16;
                     }
             }
         }
     }
     for (int o = kWave / 2; o > 0; o >>= 1) nbytes += __shfl_down(nbytes, o);
-    if ((tid & (kWave - 1)) == 0 && nbytes) atomicAdd(a.upd_bytes, static_cast<unsigned long long>(nbytes));
+    if ((tid & (kWave - 1)) == 0 && nbytes) atomicAdd(upd_bytes, static_cast<unsigned long long>(nbytes));
+}
+
+__global__ __launch_bounds__(256) void k_block_xor(BlockArgs a) {
+    extern __shared__ uint64_t lds[];
+    xor_body(view_of(a), blockIdx.x, static_cast<int32_t>(blockIdx.y), lds, a.upd_bytes);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Batched blocked form: many independent problems (the biconnected components of one graph) in one
+// ragged arena, advanced block by block together.  Step s runs block s (phases 64s .. 64s+63) of
+// every problem that has that many phases, with one launch per kernel for all of them.  The
+// problems are numbered by phase count, descending, so those still running at step s are a prefix.
+// Tree positions keep their per-tree numbering; coordinates and ranks are shifted into disjoint
+// ranges (coordinate base cb, rank base rb, both multiples of 64) so that Tt, Ct, Bc, Qr and the
+// candidate bitmap are shared arrays.  A problem's candidates are odd only for its own support
+// vectors, so running them side by side changes no problem's result.
+// ---------------------------------------------------------------------------------------------
+
+struct CompInfo {
+    int64_t cb;     // first coordinate (Tt, Ct, Bc, sel)
+    int64_t rb;     // first rank (Qr, candidate bitmap, candidate arrays)
+    int64_t sb;     // first word of the support matrix [W][npad]
+    int64_t cblk;   // first word of C_blk, Sf_blk, D_blk, pair_i, pair_c ([64][W])
+    int64_t pw;     // first entry of pw_list
+    int64_t pwoff;  // first entry of pw_off
+    int64_t store;  // first word of C_store ([N][W])
+    int64_t npad;
+    int64_t top;    // slot of the top-level basis
+    int32_t N, W, levels, pad_;
+};
+
+struct ChunkInfo {  // first-level basis: ranks [r0, r0 + 64 * words) into slot
+    int64_t r0, slot;
+    int32_t comp, words;
+};
+
+struct MergeGroup {  // `members` bases from slot src on into slot dst
+    int64_t src, dst;
+    int32_t comp, members;
+};
+
+struct BatchArgs {
+    const int64_t *tree_off;
+    const uint32_t *parent_pos;
+    const int32_t *pos_coord;  // shifted by the problem's cb
+    const uint32_t *pos_pre, *pos_end;
+    const uint32_t *big_list, *small_list;  // problem-major, each problem's trees largest first
+    const uint8_t *big_narrow;
+    const int64_t *rc_off;
+    const uint32_t *rc_pu, *rc_pv, *rc_rank;  // ranks shifted by rb
+    const int32_t *rc_coord;
+    const uint32_t *cand_root, *cand_pu, *cand_pv;  // indexed by shifted rank
+    const int32_t *cand_coord;
+    const uint32_t *root_comp;
+    const CompInfo *comp;
+    const ChunkInfo *chunks;
+    const MergeGroup *groups;  // this launch's level
+    const int32_t *pairs_cum;  // [problems + 1] workgroups of k_batch_pairs before each problem
+    const int32_t *dot_cum;    // this step's [active + 1]
+    const int32_t *xor_cum;    // this step's [active + 1]
+    uint64_t *Sw, *Tt, *Qr, *qbits, *sum_q;
+    uint32_t *sum_cnt, *sum_rank, *sel;
+    uint64_t *blk_q, *blk_m, *blk_g, *C_blk, *Sf_blk, *C_store, *Ct, *Bc, *pair_c;
+    uint32_t *D_blk, *Dcnt_blk, *pw_list, *pw_off, *pw_cnt, *pair_i;
+    uint32_t *status;
+    unsigned long long *upd_bytes;
+    int64_t K;  // rank bound
+    int32_t step, active, n_items, n_big, n_small, big_words, small_words, xor_wt;
+};
+
+__device__ __forceinline__ int32_t batch_kk(int32_t N, int32_t step) {
+    const int32_t left = N - step * kBlockK;
+    return left < kBlockK ? left : kBlockK;
+}
+
+__device__ __forceinline__ BlkView batch_view(const BatchArgs &a, int g) {
+    const CompInfo &c = a.comp[g];
+    BlkView v;
+    v.Sw = a.Sw + uniform64(c.sb);
+    v.npad = uniform64(c.npad);
+    v.cb = uniform64(c.cb);
+    v.N = static_cast<int32_t>(uniform32(static_cast<uint32_t>(c.N)));
+    v.W = static_cast<int32_t>(uniform32(static_cast<uint32_t>(c.W)));
+    v.e = a.step * kBlockK;
+    v.kk = batch_kk(v.N, a.step);
+    v.sel = a.sel + v.cb;
+    v.blk_q = a.blk_q + static_cast<int64_t>(g) * kBlockK;
+    v.blk_m = a.blk_m + static_cast<int64_t>(g) * kBlockK;
+    v.blk_g = a.blk_g + static_cast<int64_t>(g) * kBlockK;
+    const int64_t cblk = uniform64(c.cblk);
+    v.C_blk = a.C_blk + cblk;
+    v.Sf_blk = a.Sf_blk + cblk;
+    v.D_blk = a.D_blk + cblk;
+    v.pair_i = a.pair_i + cblk;
+    v.pair_c = a.pair_c + cblk;
+    v.Dcnt_blk = a.Dcnt_blk + static_cast<int64_t>(g) * kBlockK;
+    v.C_store = a.C_store ? a.C_store + uniform64(c.store) : nullptr;
+    v.Tt = a.Tt + v.cb;
+    v.Ct = a.Ct + v.cb;
+    v.Bc = a.Bc + v.cb;
+    v.pw_list = a.pw_list + uniform64(c.pw);
+    v.pw_off = a.pw_off + uniform64(c.pwoff);
+    v.pw_cnt = a.pw_cnt + g;
+    return v;
+}
+
+// Largest g in [0, n) with cum[g] <= x (cum ascending from cum[0] = 0, x < cum[n]): the segment
+// that holds item x.  Empty segments are skipped because the largest such g is taken.
+__device__ __forceinline__ int find_segment(const int32_t *cum, int n, int32_t x) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (static_cast<int32_t>(uniform32(static_cast<uint32_t>(cum[mid]))) <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Wave x: word x % (step + 1) of problem x / (step + 1) (every running problem has step + 1 words
+// below the block's end).
+__global__ __launch_bounds__(256) void k_batch_transpose(BatchArgs a) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t x = static_cast<int64_t>(blockIdx.x) * 4 + wave_id();
+    const int64_t wl = a.step + 1;
+    if (x >= static_cast<int64_t>(a.active) * wl) return;
+    const int g = static_cast<int>(x / wl);
+    transpose_body(batch_view(a, g), x - g * wl, lane);
+}
+
+__global__ __launch_bounds__(kQWaves * kWave) void k_batch_q(BatchArgs a) {
+    tree_pass(a, static_cast<uint32_t>(a.n_big), static_cast<uint32_t>(a.n_small), [&](uint32_t r) {
+        const uint32_t g = uniform32(a.root_comp[r]);
+        return batch_kk(static_cast<int32_t>(uniform32(static_cast<uint32_t>(a.comp[g].N))), a.step);
+    });
+}
+
+__device__ __forceinline__ void batch_finalize(const BatchArgs &a, int g, int lane) {
+    const CompInfo &c = a.comp[g];
+    const int32_t kk = batch_kk(static_cast<int32_t>(uniform32(static_cast<uint32_t>(c.N))), a.step);
+    finalize_core(kk, a.sum_cnt, a.sum_rank, a.sum_q, uniform64(c.top),
+                  a.sel + uniform64(c.cb) + static_cast<int64_t>(a.step) * kBlockK,
+                  a.blk_q + static_cast<int64_t>(g) * kBlockK, a.blk_m + static_cast<int64_t>(g) * kBlockK,
+                  a.blk_g + static_cast<int64_t>(g) * kBlockK, a.status, lane);
+}
+
+// Wave x: first-level chunk x (of the running problems' chunks, problem-major).
+__global__ __launch_bounds__(256) void k_batch_basis(BatchArgs a) {
+    __shared__ uint64_t s_words[4][kWave];
+    __shared__ uint32_t s_offs[4][kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = wave_id();
+    const int64_t x = static_cast<int64_t>(blockIdx.x) * 4 + wv;
+    if (x >= a.n_items) return;
+    const ChunkInfo &ch = a.chunks[x];
+    const int g = static_cast<int>(uniform32(static_cast<uint32_t>(ch.comp)));
+    const CompInfo &c = a.comp[g];
+    const int32_t kk = batch_kk(static_cast<int32_t>(uniform32(static_cast<uint32_t>(c.N))), a.step);
+    basis_chunk(a.qbits, a.Qr, uniform64(ch.r0), static_cast<int>(uniform32(static_cast<uint32_t>(ch.words))),
+                uniform64(ch.slot), kk, a.sum_cnt, a.sum_rank, a.sum_q, s_words[wv], s_offs[wv], lane);
+    if (uniform32(static_cast<uint32_t>(c.levels)) == 1u) {
+        __threadfence();
+        batch_finalize(a, g, lane);
+    }
+}
+
+// Wave x: merge group x of level L (of the running problems that have a level L+1); a problem's top
+// group also finalises its block.
+__global__ __launch_bounds__(kMergeWaves * kWave) void k_batch_merge(BatchArgs a, int L) {
+    __shared__ uint32_t s_offs[kMergeWaves][kWave];
+    __shared__ uint64_t s_tab[kMergeWaves][8 * 256];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = wave_id();
+    const int64_t x = static_cast<int64_t>(blockIdx.x) * kMergeWaves + wv;
+    if (x >= a.n_items) return;
+    const MergeGroup &mg = a.groups[x];
+    const int g = static_cast<int>(uniform32(static_cast<uint32_t>(mg.comp)));
+    const CompInfo &c = a.comp[g];
+    const int32_t kk = batch_kk(static_cast<int32_t>(uniform32(static_cast<uint32_t>(c.N))), a.step);
+    merge_group(uniform64(mg.src), static_cast<int>(uniform32(static_cast<uint32_t>(mg.members))), uniform64(mg.dst),
+                kk, a.sum_cnt, a.sum_rank, a.sum_q, s_offs[wv], s_tab[wv], lane);
+    if (static_cast<int>(uniform32(static_cast<uint32_t>(c.levels))) == L + 2) {
+        __threadfence();
+        batch_finalize(a, g, lane);
+    }
+}
+
+// Workgroup 64 g + i: phase 64*step + i of problem g.
+__global__ __launch_bounds__(256) void k_batch_cycles(BatchArgs a) {
+    extern __shared__ uint64_t lds[];
+    __shared__ uint32_t cnt;
+    const int g = static_cast<int>(blockIdx.x / kBlockK), i = static_cast<int>(blockIdx.x % kBlockK);
+    const BlkView v = batch_view(a, g);
+    if (i >= v.kk) return;
+    cycles_body(a, v, i, lds, &cnt);
+}
+
+// Problem g owns workgroups pairs_cum[g] .. pairs_cum[g+1]-1: the first lists its words, the others
+// transpose its cycle vectors.
+__global__ __launch_bounds__(256) void k_batch_pairs(BatchArgs a) {
+    extern __shared__ uint64_t lds[];
+    __shared__ uint32_t nlist, npairs;
+    const int32_t x = static_cast<int32_t>(blockIdx.x);
+    const int g = find_segment(a.pairs_cum, a.active, x);
+    const BlkView v = batch_view(a, g);
+    const int32_t b = x - a.pairs_cum[g];
+    if (b > 0) {
+        const int64_t w = static_cast<int64_t>(b - 1) * 4 + wave_id();
+        if (w < v.W) pairs_transpose(v, w, threadIdx.x & (kWave - 1));
+        return;
+    }
+    pairs_list(v, lds, &nlist, &npairs);
+}
+
+__global__ __launch_bounds__(256) void k_batch_dot(BatchArgs a) {
+    extern __shared__ uint64_t lds[];
+    __shared__ uint32_t seg_end;
+    const int32_t x = static_cast<int32_t>(blockIdx.x);
+    const int g = find_segment(a.dot_cum, a.active, x);
+    dot_body(batch_view(a, g), x - a.dot_cum[g], lds, &seg_end, a.upd_bytes);
+}
+
+// Problem g's tiles: ceil((N - 64 step) / 256) column tiles times xor_wt word tiles, columns fastest.
+__global__ __launch_bounds__(256) void k_batch_xor(BatchArgs a) {
+    extern __shared__ uint64_t lds[];
+    const int32_t x = static_cast<int32_t>(blockIdx.x);
+    const int g = find_segment(a.xor_cum, a.active, x);
+    const BlkView v = batch_view(a, g);
+    const int32_t t = x - a.xor_cum[g];
+    const int32_t ct = (v.N - v.e + 255) / 256;
+    const int32_t yt = static_cast<int32_t>(uniform32(static_cast<uint32_t>(t / ct)));
+    xor_body(v, t - yt * ct, yt, lds, a.upd_bytes);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1917,7 +2245,8 @@ struct mcb_solver {
 
 namespace {
 
-int harvest_events(mcb_solver *s) {
+template <typename S>
+int harvest_events(S *s) {
     if (s->ev_used == 0) return MCB_OK;
     HIP_TRY(hipEventSynchronize(s->ev_pool[s->ev_used - 1]));
     for (size_t i = 0; i + 1 < s->ev_used; i += 2) {
@@ -1930,7 +2259,8 @@ int harvest_events(mcb_solver *s) {
     return MCB_OK;
 }
 
-int timed_begin(mcb_solver *s, int kind) {
+template <typename S>
+int timed_begin(S *s, int kind) {
     s->timing_open = s->opt.timing == 1 || (s->opt.timing == 2 && kind == MCB_K_PARITY_SEARCH);
     if (!s->timing_open) return MCB_OK;
     if (s->ev_used + 2 > s->ev_pool.size()) {
@@ -1942,7 +2272,8 @@ int timed_begin(mcb_solver *s, int kind) {
     return MCB_OK;
 }
 
-int timed_end(mcb_solver *s) {
+template <typename S>
+int timed_end(S *s) {
     if (!s->timing_open) return MCB_OK;
     s->timing_open = false;
     HIP_TRY(hipEventRecord(s->ev_pool[s->ev_used++], s->stream));
@@ -2118,6 +2449,50 @@ int launch_phase(mcb_solver *s, int32_t e) {
     return timed_end(s);
 }
 
+// Checks one problem's layout on the host: parent before child, positions, coordinates and ranks
+// in range, each root's candidate ranks listed ascending.  *max_pos: the largest tree.
+int validate_desc(const mcb_solver_desc *d, int32_t *max_pos) {
+    if (!d || d->num_roots <= 0 || d->num_non_tree <= 0 || d->num_candidates <= 0 || d->num_positions <= 0 ||
+        !d->tree_offset || !d->parent_pos || !d->pos_coord || !d->cand_root || !d->cand_pos_u || !d->cand_pos_v ||
+        !d->cand_coord || !d->root_cand_offset || !d->root_cand_rank)
+        return fail(MCB_E_INVALID, "invalid arguments");
+    if (d->num_candidates >= static_cast<int64_t>(kNoRank)) return fail(MCB_E_INVALID, "too many candidates");
+    const int32_t F = d->num_roots, N = d->num_non_tree;
+    const int64_t K = d->num_candidates;
+    if (d->tree_offset[0] != 0 || d->tree_offset[F] != d->num_positions) return fail(MCB_E_INVALID, "tree_offset");
+    if (d->root_cand_offset[0] != 0 || d->root_cand_offset[F] != K) return fail(MCB_E_INVALID, "root_cand_offset");
+    int32_t mp = 0;
+    for (int32_t r = 0; r < F; ++r) {
+        const int64_t a = d->tree_offset[r], b = d->tree_offset[r + 1];
+        if (b <= a || b - a > 0x7FFFFFFF) return fail(MCB_E_INVALID, "empty or oversized tree");
+        mp = std::max<int32_t>(mp, static_cast<int32_t>(b - a));
+        if (d->parent_pos[a] != kNoParent) return fail(MCB_E_INVALID, "position 0 must be the root");
+        for (int64_t i = a + 1; i < b; ++i) {
+            if (d->parent_pos[i] >= static_cast<uint64_t>(i - a))
+                return fail(MCB_E_INVALID, "parent position must precede its child");
+            if (d->pos_coord[i] >= N) return fail(MCB_E_INVALID, "coordinate out of range");
+        }
+    }
+    for (int64_t k = 0; k < K; ++k) {
+        const uint32_t r = d->cand_root[k];
+        if (r >= static_cast<uint32_t>(F)) return fail(MCB_E_INVALID, "candidate root out of range");
+        const int64_t np = d->tree_offset[r + 1] - d->tree_offset[r];
+        if (d->cand_pos_u[k] >= np || d->cand_pos_v[k] >= np || d->cand_coord[k] >= N)
+            return fail(MCB_E_INVALID, "candidate position out of range");
+    }
+    for (int32_t r = 0; r < F; ++r) {
+        const int64_t a = d->root_cand_offset[r], b = d->root_cand_offset[r + 1];
+        if (b < a) return fail(MCB_E_INVALID, "root_cand_offset");
+        for (int64_t i = a; i < b; ++i) {
+            const uint32_t k = d->root_cand_rank[i];
+            if (k >= K || d->cand_root[k] != static_cast<uint32_t>(r) || (i > a && k <= d->root_cand_rank[i - 1]))
+                return fail(MCB_E_INVALID, "root_cand_rank must list each root's ranks ascending");
+        }
+    }
+    *max_pos = mp;
+    return MCB_OK;
+}
+
 // DFS pre-order intervals of every tree layout: pre[x] is x's index in a pre-order walk of its
 // root's tree (children in position order) and end[x] = pre[x] + subtree size, so that x is an
 // ancestor-or-self of y exactly when pre[x] <= pre[y] < end[x].
@@ -2278,13 +2653,12 @@ void mcb_solver_destroy(mcb_solver *s) {
 }
 
 int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, mcb_solver **out) {
-    if (!d || !out || d->num_roots <= 0 || d->num_non_tree <= 0 || d->num_candidates <= 0 || d->num_positions <= 0 ||
-        !d->tree_offset || !d->parent_pos || !d->pos_coord || !d->cand_root || !d->cand_pos_u || !d->cand_pos_v ||
-        !d->cand_coord || !d->root_cand_offset || !d->root_cand_rank)
-        return fail(MCB_E_INVALID, "invalid arguments");
-    if (d->num_candidates >= static_cast<int64_t>(kNoRank)) return fail(MCB_E_INVALID, "too many candidates");
+    if (!out) return fail(MCB_E_INVALID, "invalid arguments");
     *out = nullptr;
-    int rc = ensure_device();
+    int32_t max_pos = 0;
+    int rc = validate_desc(d, &max_pos);
+    if (rc) return rc;
+    rc = ensure_device();
     if (rc) return rc;
     mcb_solver *s = new (std::nothrow) mcb_solver();
     if (!s) return fail(MCB_E_NOMEM, "out of memory");
@@ -2301,28 +2675,7 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
     s->npad = (static_cast<int64_t>(s->N) + 63) & ~static_cast<int64_t>(63);
     const int32_t F = s->F;
     const int64_t K = s->K;
-    // Validate the layout on the host (parent before child, positions and ranks in range).
-    if (d->tree_offset[0] != 0 || d->tree_offset[F] != s->Ptot) return cleanup(fail(MCB_E_INVALID, "tree_offset"));
-    if (d->root_cand_offset[0] != 0 || d->root_cand_offset[F] != K)
-        return cleanup(fail(MCB_E_INVALID, "root_cand_offset"));
-    for (int32_t r = 0; r < F; ++r) {
-        const int64_t a = d->tree_offset[r], b = d->tree_offset[r + 1];
-        if (b <= a || b - a > 0x7FFFFFFF) return cleanup(fail(MCB_E_INVALID, "empty or oversized tree"));
-        s->max_pos = std::max<int32_t>(s->max_pos, static_cast<int32_t>(b - a));
-        if (d->parent_pos[a] != kNoParent) return cleanup(fail(MCB_E_INVALID, "position 0 must be the root"));
-        for (int64_t i = a + 1; i < b; ++i) {
-            if (d->parent_pos[i] >= static_cast<uint64_t>(i - a))
-                return cleanup(fail(MCB_E_INVALID, "parent position must precede its child"));
-            if (d->pos_coord[i] >= s->N) return cleanup(fail(MCB_E_INVALID, "coordinate out of range"));
-        }
-    }
-    for (int64_t k = 0; k < K; ++k) {
-        const uint32_t r = d->cand_root[k];
-        if (r >= static_cast<uint32_t>(F)) return cleanup(fail(MCB_E_INVALID, "candidate root out of range"));
-        const int64_t np = d->tree_offset[r + 1] - d->tree_offset[r];
-        if (d->cand_pos_u[k] >= np || d->cand_pos_v[k] >= np || d->cand_coord[k] >= s->N)
-            return cleanup(fail(MCB_E_INVALID, "candidate position out of range"));
-    }
+    s->max_pos = max_pos;
     // Root-major candidate copies, lowest rank per root, and the two work lists (largest first).
     std::vector<uint32_t> rc_pu(K), rc_pv(K), rc_rank(K), min_rank(F, kNoRank), order(F);
     std::vector<int32_t> rc_coord(K);
@@ -2330,8 +2683,6 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
         const int64_t a = d->root_cand_offset[r], b = d->root_cand_offset[r + 1];
         for (int64_t i = a; i < b; ++i) {
             const uint32_t k = d->root_cand_rank[i];
-            if (k >= K || d->cand_root[k] != static_cast<uint32_t>(r) || (i > a && k <= d->root_cand_rank[i - 1]))
-                return cleanup(fail(MCB_E_INVALID, "root_cand_rank must list each root's ranks ascending"));
             rc_pu[i] = d->cand_pos_u[k];
             rc_pv[i] = d->cand_pos_v[k];
             rc_rank[i] = k;
@@ -2616,6 +2967,592 @@ int mcb_hip_copy_bandwidth(int64_t bytes, int iters, double *gbps) {
     (void)hipFree(b);
     if (err != hipSuccess) return fail(MCB_E_DEVICE, "copy kernel failed");
     *gbps = 2.0 * static_cast<double>(n * 8) * iters / (ms * 1e-3) / 1e9;
+    return MCB_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// Batched form (host): the arena, its static launch tables, and one launch per kernel per step
+// ---------------------------------------------------------------------------------------------
+
+struct mcb_batch {
+    mcb_solver_options opt{};
+    int32_t G = 0, steps = 0, step = 0, merge_levels = 0;
+    int64_t K = 0;                  // rank bound (ranks of all problems, each range padded to 64)
+    std::vector<int32_t> order;     // problem at each arena position (phase count descending)
+    std::vector<int32_t> slot_of;   // arena position of each problem
+    std::vector<CompInfo> comps;    // by arena position
+    std::vector<int64_t> rbase;     // rank base by arena position
+    std::vector<int32_t> active;    // problems still running at each step
+    std::vector<int32_t> big_cum, small_cum, chunk_cum, pairs_cum;  // [G + 1] prefix counts
+    std::vector<std::vector<int32_t>> grp_cum;                      // per merge level: [G + 1]
+    std::vector<int64_t> grp_off;                                   // per merge level: first group
+    std::vector<int64_t> step_off;                                  // per step: first entry of its tables
+    std::vector<int32_t> dot_cum, xor_cum;
+    std::vector<int64_t> pos_cum, cand_cum;                         // [G + 1] tree positions, candidates
+    // device arrays
+    int64_t *tree_off = nullptr, *rc_off = nullptr;
+    uint32_t *parent_pos = nullptr, *pos_pre = nullptr, *pos_end = nullptr, *big_list = nullptr,
+             *small_list = nullptr, *rc_pu = nullptr, *rc_pv = nullptr, *rc_rank = nullptr, *cand_root = nullptr,
+             *cand_pu = nullptr, *cand_pv = nullptr, *root_comp = nullptr;
+    int32_t *pos_coord = nullptr, *rc_coord = nullptr, *cand_coord = nullptr;
+    uint8_t *big_narrow = nullptr;
+    CompInfo *d_comp = nullptr;
+    ChunkInfo *d_chunks = nullptr;
+    MergeGroup *d_groups = nullptr;
+    int32_t *d_pairs_cum = nullptr, *d_dot_cum = nullptr, *d_xor_cum = nullptr;
+    uint64_t *Sw = nullptr, *Tt = nullptr, *Qr = nullptr, *qbits = nullptr, *sum_q = nullptr, *blk_q = nullptr,
+             *blk_m = nullptr, *blk_g = nullptr, *C_blk = nullptr, *Sf_blk = nullptr, *C_store = nullptr,
+             *Ct = nullptr, *Bc = nullptr, *pair_c = nullptr;
+    uint32_t *sum_cnt = nullptr, *sum_rank = nullptr, *sel = nullptr, *D_blk = nullptr, *Dcnt_blk = nullptr,
+             *pw_list = nullptr, *pw_off = nullptr, *pw_cnt = nullptr, *pair_i = nullptr, *status = nullptr;
+    unsigned long long *upd_bytes = nullptr;
+    int64_t coords = 0, sw_words = 0, blk_words = 0, store_words = 0, slots = 0;
+    std::vector<void *> allocs;
+    int32_t q_blocks = 0, q_lds = 0, big_words = 0, small_words = 0;
+    hipStream_t stream = nullptr;
+    int64_t bytes = 0;
+    int64_t blk_positions = 0, blk_candidates = 0;
+    // timing
+    double kt_ms[MCB_K_COUNT] = {};
+    int64_t kt_n[MCB_K_COUNT] = {};
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_kind;
+    size_t ev_used = 0;
+    bool timing_open = false;
+};
+
+namespace {
+
+template <typename T>
+int bat_alloc(mcb_batch *b, T **p, size_t n) {
+    int rc = dev_alloc(p, n, b->bytes);
+    if (!rc) b->allocs.push_back(*p);
+    return rc;
+}
+
+template <typename T>
+int bat_upload(mcb_batch *b, T **p, const std::vector<T> &v) {
+    int rc = upload(p, v.data(), v.size(), b->bytes);
+    if (!rc) b->allocs.push_back(*p);
+    return rc;
+}
+
+BatchArgs batch_args(const mcb_batch *b, int32_t s) {
+    BatchArgs a{};
+    a.tree_off = b->tree_off;
+    a.parent_pos = b->parent_pos;
+    a.pos_coord = b->pos_coord;
+    a.pos_pre = b->pos_pre;
+    a.pos_end = b->pos_end;
+    a.big_list = b->big_list;
+    a.small_list = b->small_list;
+    a.big_narrow = b->big_narrow;
+    a.rc_off = b->rc_off;
+    a.rc_pu = b->rc_pu;
+    a.rc_pv = b->rc_pv;
+    a.rc_rank = b->rc_rank;
+    a.rc_coord = b->rc_coord;
+    a.cand_root = b->cand_root;
+    a.cand_pu = b->cand_pu;
+    a.cand_pv = b->cand_pv;
+    a.cand_coord = b->cand_coord;
+    a.root_comp = b->root_comp;
+    a.comp = b->d_comp;
+    a.chunks = b->d_chunks;
+    a.groups = b->d_groups;
+    a.pairs_cum = b->d_pairs_cum;
+    a.dot_cum = b->d_dot_cum + b->step_off[s];
+    a.xor_cum = b->d_xor_cum + b->step_off[s];
+    a.Sw = b->Sw;
+    a.Tt = b->Tt;
+    a.Qr = b->Qr;
+    a.qbits = b->qbits;
+    a.sum_q = b->sum_q;
+    a.sum_cnt = b->sum_cnt;
+    a.sum_rank = b->sum_rank;
+    a.sel = b->sel;
+    a.blk_q = b->blk_q;
+    a.blk_m = b->blk_m;
+    a.blk_g = b->blk_g;
+    a.C_blk = b->C_blk;
+    a.Sf_blk = b->Sf_blk;
+    a.C_store = b->C_store;
+    a.Ct = b->Ct;
+    a.Bc = b->Bc;
+    a.pair_c = b->pair_c;
+    a.D_blk = b->D_blk;
+    a.Dcnt_blk = b->Dcnt_blk;
+    a.pw_list = b->pw_list;
+    a.pw_off = b->pw_off;
+    a.pw_cnt = b->pw_cnt;
+    a.pair_i = b->pair_i;
+    a.status = b->status;
+    a.upd_bytes = b->upd_bytes;
+    a.K = b->K;
+    a.step = s;
+    a.active = b->active[s];
+    a.n_big = b->big_cum[a.active];
+    a.n_small = b->small_cum[a.active];
+    a.big_words = b->big_words;
+    a.small_words = b->small_words;
+    a.xor_wt = (s + 1 + kTile - 1) / kTile;
+    return a;
+}
+
+// Step s: block s of every problem with more than 64 s phases.
+int launch_batch_step(mcb_batch *b, int32_t s) {
+    BatchArgs a = batch_args(b, s);
+    const int32_t A = a.active;
+    const int64_t off = b->step_off[s];
+    int rc = timed_begin(b, MCB_K_TRANSPOSE);
+    if (rc) return rc;
+    const int64_t tw = static_cast<int64_t>(A) * (s + 1);
+    hipLaunchKernelGGL(k_batch_transpose, dim3(static_cast<unsigned>((tw + 3) / 4)), dim3(256), 0, b->stream, a);
+    HIP_TRY(hipGetLastError());
+    if ((rc = timed_end(b)) || (rc = timed_begin(b, MCB_K_PARITY_SEARCH))) return rc;
+    hipLaunchKernelGGL(k_batch_q, dim3(b->q_blocks), dim3(kQWaves * kWave), b->q_lds, b->stream, a);
+    HIP_TRY(hipGetLastError());
+    if ((rc = timed_end(b)) || (rc = timed_begin(b, MCB_K_SELECT))) return rc;
+    a.n_items = b->chunk_cum[A];
+    hipLaunchKernelGGL(k_batch_basis, dim3(static_cast<unsigned>((a.n_items + 3) / 4)), dim3(256), 0, b->stream, a);
+    HIP_TRY(hipGetLastError());
+    for (int L = 0; L < b->merge_levels; ++L) {
+        const int32_t n = b->grp_cum[L][A];
+        if (n == 0) break;  // no running problem has a level L+1, nor any higher one
+        a.groups = b->d_groups + b->grp_off[L];
+        a.n_items = n;
+        hipLaunchKernelGGL(k_batch_merge, dim3(static_cast<unsigned>((n + kMergeWaves - 1) / kMergeWaves)),
+                           dim3(kMergeWaves * kWave), 0, b->stream, a, L);
+        HIP_TRY(hipGetLastError());
+    }
+    if ((rc = timed_end(b)) || (rc = timed_begin(b, MCB_K_CYCLE))) return rc;
+    // problems are in descending phase count, so problem 0 has the most words
+    hipLaunchKernelGGL(k_batch_cycles, dim3(static_cast<unsigned>(A) * kBlockK), dim3(256),
+                       sizeof(uint64_t) * b->comps[0].W, b->stream, a);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_batch_pairs, dim3(static_cast<unsigned>(b->pairs_cum[A])), dim3(256),
+                       sizeof(uint64_t) * s + 2 * 256 * sizeof(uint32_t), b->stream, a);
+    HIP_TRY(hipGetLastError());
+    if ((rc = timed_end(b)) || (rc = timed_begin(b, MCB_K_UPDATE))) return rc;
+    const int32_t nd = b->dot_cum[off + A];
+    if (nd > 0) {
+        const size_t dlds = sizeof(uint64_t) * (kBlockK + kSegPairs) + sizeof(uint32_t) * (kSegPairs + 2 * kSegWords + 1);
+        hipLaunchKernelGGL(k_batch_dot, dim3(static_cast<unsigned>(nd)), dim3(256), dlds, b->stream, a);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_batch_xor, dim3(static_cast<unsigned>(b->xor_cum[off + A])), dim3(256),
+                       sizeof(uint64_t) * kBlockK * kTile, b->stream, a);
+    HIP_TRY(hipGetLastError());
+    b->blk_positions += b->pos_cum[A];
+    b->blk_candidates += b->cand_cum[A];
+    return timed_end(b);
+}
+
+}  // namespace
+
+extern "C" {
+
+void mcb_batch_destroy(mcb_batch *b) {
+    if (!b) return;
+    for (void *p : b->allocs)
+        if (p) (void)hipFree(p);
+    for (hipEvent_t e : b->ev_pool) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int mcb_batch_reset(mcb_batch *b) {
+    if (!b) return fail(MCB_E_INVALID, "invalid arguments");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipMemsetAsync(b->Sw, 0, sizeof(uint64_t) * static_cast<size_t>(b->sw_words), b->stream));
+    HIP_TRY(hipMemsetAsync(b->Tt, 0, sizeof(uint64_t) * static_cast<size_t>(b->coords), b->stream));
+    HIP_TRY(hipMemsetAsync(b->qbits, 0, sizeof(uint64_t) * static_cast<size_t>(b->K / 64), b->stream));
+    HIP_TRY(hipMemsetAsync(b->upd_bytes, 0, sizeof(unsigned long long), b->stream));
+    HIP_TRY(hipMemsetAsync(b->status, 0, sizeof(uint32_t), b->stream));
+    hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<unsigned>((b->coords + 255) / 256)), dim3(256), 0, b->stream,
+                       b->sel, kNoRank, b->coords);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->step = 0;
+    b->blk_positions = 0;
+    b->blk_candidates = 0;
+    b->ev_used = 0;
+    for (int i = 0; i < MCB_K_COUNT; ++i) {
+        b->kt_ms[i] = 0;
+        b->kt_n[i] = 0;
+    }
+    return MCB_OK;
+}
+
+int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solver_options *opt, mcb_batch **out) {
+    if (!descs || count <= 0 || !out) return fail(MCB_E_INVALID, "invalid arguments");
+    *out = nullptr;
+    int32_t largest = 0;
+    for (int32_t i = 0; i < count; ++i) {
+        int32_t mp = 0;
+        if (validate_desc(&descs[i], &mp)) return fail(MCB_E_INVALID, "problem " + std::to_string(i) + ": " + g_err);
+        largest = std::max(largest, mp);
+    }
+    int rc = ensure_device();
+    if (rc) return rc;
+    mcb_batch *b = new (std::nothrow) mcb_batch();
+    if (!b) return fail(MCB_E_NOMEM, "out of memory");
+    auto cleanup = [&](int code) {
+        mcb_batch_destroy(b);
+        return code;
+    };
+    if (opt) b->opt = *opt;
+    else mcb_solver_options_init(&b->opt);
+    const int32_t G = count;
+    b->G = G;
+    b->order.resize(G);
+    std::iota(b->order.begin(), b->order.end(), 0);
+    std::stable_sort(b->order.begin(), b->order.end(),
+                     [&](int32_t x, int32_t y) { return descs[x].num_non_tree > descs[y].num_non_tree; });
+    b->slot_of.resize(G);
+    for (int32_t g = 0; g < G; ++g) b->slot_of[b->order[g]] = g;
+
+    // Bases of every problem's ranges, and the tree classification (the threshold is the largest
+    // multiple of 64, at most 1024, whose one-wave slices fit the LDS the largest tree needs).
+    const int64_t largest_bytes = 8 * ((static_cast<int64_t>(largest) + kWave - 1) / kWave * kWave);
+    const int64_t small_max =
+        std::max<int64_t>(kWave, std::min<int64_t>(1024, largest_bytes / (8 * kQWaves) / kWave * kWave));
+    int64_t F_tot = 0, P_tot = 0, K_tot = 0, rank_tot = 0, coords = 0, sw = 0, blk = 0, pw = 0, pwoff = 0, store = 0,
+            slots = 0;
+    b->comps.resize(G);
+    b->rbase.resize(G);
+    std::vector<int64_t> root_base(G), pos_base(G), cand_base(G), slot_base(G);
+    std::vector<std::vector<int64_t>> lvl_n(G), lvl_slot(G);
+    int32_t max_levels = 0;
+    for (int32_t g = 0; g < G; ++g) {
+        const mcb_solver_desc &d = descs[b->order[g]];
+        CompInfo &c = b->comps[g];
+        c.N = d.num_non_tree;
+        c.W = (c.N + 63) / 64;
+        c.npad = static_cast<int64_t>(c.W) * 64;
+        c.cb = coords;
+        c.rb = rank_tot;
+        c.sb = sw;
+        c.cblk = blk;
+        c.pw = pw;
+        c.pwoff = pwoff;
+        c.store = store;
+        b->rbase[g] = rank_tot;
+        root_base[g] = F_tot;
+        pos_base[g] = P_tot;
+        cand_base[g] = K_tot;
+        slot_base[g] = slots;
+        int64_t n = (d.num_candidates + kChunk - 1) / kChunk;
+        for (;;) {
+            lvl_slot[g].push_back(slots);
+            lvl_n[g].push_back(n);
+            slots += n;
+            if (n == 1) break;
+            n = (n + kGroup - 1) / kGroup;
+        }
+        c.levels = static_cast<int32_t>(lvl_n[g].size());
+        if (c.levels > kMaxLevels) return cleanup(fail(MCB_E_INVALID, "too many candidates for the basis levels"));
+        max_levels = std::max(max_levels, c.levels);
+        c.top = lvl_slot[g].back();
+        F_tot += d.num_roots;
+        P_tot += d.num_positions;
+        K_tot += d.num_candidates;
+        rank_tot += (d.num_candidates + 63) / 64 * 64;
+        coords += c.npad;
+        sw += static_cast<int64_t>(c.W) * c.npad;
+        blk += static_cast<int64_t>(kBlockK) * c.W;
+        pw += c.W;
+        pwoff += c.W + 1;
+        store += static_cast<int64_t>(c.N) * c.W;
+    }
+    if (rank_tot >= static_cast<int64_t>(kNoRank) || coords > 0x7FFFFFFF || F_tot > 0x7FFFFFFF)
+        return cleanup(fail(MCB_E_INVALID, "the batch is too large for 32-bit ranks, coordinates or roots"));
+    b->K = rank_tot;
+    b->coords = coords;
+    b->sw_words = sw;
+    b->blk_words = blk;
+    b->store_words = store;
+    b->slots = slots;
+    b->merge_levels = max_levels - 1;
+    b->steps = (b->comps[0].N + kBlockK - 1) / kBlockK;
+
+    // Arena arrays: trees and candidates, problem after problem in arena order, coordinates and ranks
+    // shifted into each problem's range.
+    std::vector<int64_t> tree_off(F_tot + 1), rc_off(F_tot + 1);
+    std::vector<uint32_t> parent_pos(P_tot), rc_pu(K_tot), rc_pv(K_tot), rc_rank(K_tot), cand_root(rank_tot, 0),
+        cand_pu(rank_tot, 0), cand_pv(rank_tot, 0), root_comp(F_tot);
+    std::vector<int32_t> pos_coord(P_tot), rc_coord(K_tot), cand_coord(rank_tot, -1);
+    std::vector<uint32_t> big, small;
+    std::vector<uint8_t> narrow;
+    b->big_cum.assign(G + 1, 0);
+    b->small_cum.assign(G + 1, 0);
+    b->pos_cum.assign(G + 1, 0);
+    b->cand_cum.assign(G + 1, 0);
+    int64_t max_big = 0, max_small = 1;
+    std::vector<uint32_t> roots, level;
+    for (int32_t g = 0; g < G; ++g) {
+        const mcb_solver_desc &d = descs[b->order[g]];
+        const CompInfo &c = b->comps[g];
+        const int64_t R = root_base[g], PB = pos_base[g], KB = cand_base[g], rb = c.rb, cb = c.cb;
+        for (int32_t r = 0; r < d.num_roots; ++r) {
+            tree_off[R + r] = PB + d.tree_offset[r];
+            rc_off[R + r] = KB + d.root_cand_offset[r];
+            root_comp[R + r] = static_cast<uint32_t>(g);
+        }
+        for (int64_t i = 0; i < d.num_positions; ++i) {
+            parent_pos[PB + i] = d.parent_pos[i];
+            pos_coord[PB + i] = d.pos_coord[i] >= 0 ? static_cast<int32_t>(cb + d.pos_coord[i]) : -1;
+        }
+        for (int64_t k = 0; k < d.num_candidates; ++k) {
+            cand_root[rb + k] = static_cast<uint32_t>(R + d.cand_root[k]);
+            cand_pu[rb + k] = d.cand_pos_u[k];
+            cand_pv[rb + k] = d.cand_pos_v[k];
+            cand_coord[rb + k] = d.cand_coord[k] >= 0 ? static_cast<int32_t>(cb + d.cand_coord[k]) : -1;
+        }
+        for (int64_t i = 0; i < d.num_candidates; ++i) {
+            const uint32_t k = d.root_cand_rank[i];
+            rc_pu[KB + i] = d.cand_pos_u[k];
+            rc_pv[KB + i] = d.cand_pos_v[k];
+            rc_rank[KB + i] = static_cast<uint32_t>(rb + k);
+            rc_coord[KB + i] = d.cand_coord[k] >= 0 ? static_cast<int32_t>(cb + d.cand_coord[k]) : -1;
+        }
+        // this problem's trees, largest first, split into the two work lists
+        roots.resize(d.num_roots);
+        std::iota(roots.begin(), roots.end(), 0u);
+        auto npos = [&](uint32_t r) { return d.tree_offset[r + 1] - d.tree_offset[r]; };
+        std::stable_sort(roots.begin(), roots.end(), [&](uint32_t x, uint32_t y) { return npos(x) > npos(y); });
+        int32_t nb = 0, ns = 0;
+        for (uint32_t r : roots) {
+            const int64_t np = npos(r);
+            if (np > small_max) {
+                big.push_back(static_cast<uint32_t>(R + r));
+                max_big = std::max(max_big, np);
+                const int64_t a0 = d.tree_offset[r];
+                level.assign(static_cast<size_t>(np), 0);
+                uint32_t depth = 0;
+                for (int64_t x = 1; x < np; ++x) {
+                    level[x] = level[d.parent_pos[a0 + x]] + 1;
+                    depth = std::max(depth, level[x]);
+                }
+                narrow.push_back(np < 128ll * (depth + 1) ? 1 : 0);
+                ++nb;
+            } else {
+                small.push_back(static_cast<uint32_t>(R + r));
+                max_small = std::max(max_small, np);
+                ++ns;
+            }
+        }
+        b->big_cum[g + 1] = b->big_cum[g] + nb;
+        b->small_cum[g + 1] = b->small_cum[g] + ns;
+        b->pos_cum[g + 1] = b->pos_cum[g] + d.num_positions;
+        b->cand_cum[g + 1] = b->cand_cum[g] + d.num_candidates;
+    }
+    tree_off[F_tot] = P_tot;
+    rc_off[F_tot] = K_tot;
+    std::vector<uint32_t> pre, end;
+    preorder_intervals(tree_off.data(), parent_pos.data(), static_cast<int32_t>(F_tot), pre, end);
+
+    // LDS of the tree pass: the largest big tree's path words and ready stamps, or one slice per wave
+    const int64_t big_words = (max_big + kWave - 1) / kWave * kWave;
+    const int64_t small_words = (max_small + kWave - 1) / kWave * kWave;
+    const int64_t lds = std::max<int64_t>(8 * big_words + 4 * (big_words / kWave), 8 * kQWaves * small_words);
+    const int64_t lds_max = std::min<int64_t>(g_dev.prop.sharedMemPerBlock, 160 * 1024);
+    if (lds > lds_max) return cleanup(fail(MCB_E_INVALID, "a tree is too large for the blocked form's LDS budget"));
+    b->big_words = static_cast<int32_t>(big_words);
+    b->small_words = static_cast<int32_t>(small_words);
+    b->q_lds = static_cast<int32_t>(lds);
+    int per_cu = 0;
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(k_batch_q), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            b->q_lds) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_q, kQWaves * kWave, b->q_lds) != hipSuccess)
+        return cleanup(fail(MCB_E_DEVICE, "tree pass launch set-up"));
+    b->q_blocks = std::max(1, per_cu) * g_dev.prop.multiProcessorCount;
+
+    // Static launch tables: first-level chunks, merge groups per level, pairs workgroups.
+    std::vector<ChunkInfo> chunks;
+    b->chunk_cum.assign(G + 1, 0);
+    b->pairs_cum.assign(G + 1, 0);
+    for (int32_t g = 0; g < G; ++g) {
+        const CompInfo &c = b->comps[g];
+        const int64_t words = (descs[b->order[g]].num_candidates + 63) / 64;
+        for (int64_t i = 0; i < lvl_n[g][0]; ++i) {
+            ChunkInfo ch{};
+            ch.r0 = c.rb + i * kChunk;
+            ch.slot = lvl_slot[g][0] + i;
+            ch.comp = g;
+            ch.words = static_cast<int32_t>(std::min<int64_t>(kWave, words - i * kWave));
+            chunks.push_back(ch);
+        }
+        b->chunk_cum[g + 1] = b->chunk_cum[g] + static_cast<int32_t>(lvl_n[g][0]);
+        b->pairs_cum[g + 1] = b->pairs_cum[g] + 1 + (c.W + 3) / 4;
+    }
+    std::vector<MergeGroup> groups;
+    b->grp_cum.assign(std::max(0, b->merge_levels), std::vector<int32_t>(G + 1, 0));
+    b->grp_off.assign(std::max(0, b->merge_levels), 0);
+    for (int L = 0; L < b->merge_levels; ++L) {
+        b->grp_off[L] = static_cast<int64_t>(groups.size());
+        for (int32_t g = 0; g < G; ++g) {
+            int32_t n = 0;
+            if (b->comps[g].levels >= L + 2) {
+                n = static_cast<int32_t>(lvl_n[g][L + 1]);
+                for (int32_t gi = 0; gi < n; ++gi) {
+                    MergeGroup mg{};
+                    mg.src = lvl_slot[g][L] + static_cast<int64_t>(gi) * kGroup;
+                    mg.dst = lvl_slot[g][L + 1] + gi;
+                    mg.comp = g;
+                    mg.members = static_cast<int32_t>(std::min<int64_t>(kGroup, lvl_n[g][L] - static_cast<int64_t>(gi) * kGroup));
+                    groups.push_back(mg);
+                }
+            }
+            b->grp_cum[L][g + 1] = b->grp_cum[L][g] + n;
+        }
+    }
+    // Per step: running problems and the prefix counts of their dot and xor workgroups.
+    b->active.resize(b->steps);
+    b->step_off.resize(b->steps);
+    for (int32_t s = 0; s < b->steps; ++s) {
+        int32_t A = 0;
+        while (A < G && b->comps[A].N > s * kBlockK) ++A;
+        b->active[s] = A;
+        b->step_off[s] = static_cast<int64_t>(b->dot_cum.size());
+        const int64_t wt = (s + 1 + kTile - 1) / kTile;
+        int64_t dsum = 0, xsum = 0;
+        b->dot_cum.push_back(0);
+        b->xor_cum.push_back(0);
+        for (int32_t g = 0; g < A; ++g) {
+            const int64_t left = b->comps[g].N - static_cast<int64_t>(s) * kBlockK;
+            const int64_t later = left - std::min<int64_t>(left, kBlockK);
+            dsum += (later + 255) / 256;
+            xsum += (left + 255) / 256 * wt;
+            if (dsum > 0x7FFFFFFF || xsum > 0x7FFFFFFF) return cleanup(fail(MCB_E_INVALID, "batch too large"));
+            b->dot_cum.push_back(static_cast<int32_t>(dsum));
+            b->xor_cum.push_back(static_cast<int32_t>(xsum));
+        }
+    }
+
+    std::vector<CompInfo> comps = b->comps;
+    if ((rc = bat_upload(b, &b->tree_off, tree_off)) || (rc = bat_upload(b, &b->parent_pos, parent_pos)) ||
+        (rc = bat_upload(b, &b->pos_coord, pos_coord)) || (rc = bat_upload(b, &b->pos_pre, pre)) ||
+        (rc = bat_upload(b, &b->pos_end, end)) || (rc = bat_upload(b, &b->big_list, big)) ||
+        (rc = bat_upload(b, &b->small_list, small)) || (rc = bat_upload(b, &b->big_narrow, narrow)) ||
+        (rc = bat_upload(b, &b->rc_off, rc_off)) || (rc = bat_upload(b, &b->rc_pu, rc_pu)) ||
+        (rc = bat_upload(b, &b->rc_pv, rc_pv)) || (rc = bat_upload(b, &b->rc_rank, rc_rank)) ||
+        (rc = bat_upload(b, &b->rc_coord, rc_coord)) || (rc = bat_upload(b, &b->cand_root, cand_root)) ||
+        (rc = bat_upload(b, &b->cand_pu, cand_pu)) || (rc = bat_upload(b, &b->cand_pv, cand_pv)) ||
+        (rc = bat_upload(b, &b->cand_coord, cand_coord)) || (rc = bat_upload(b, &b->root_comp, root_comp)) ||
+        (rc = bat_upload(b, &b->d_comp, comps)) || (rc = bat_upload(b, &b->d_chunks, chunks)) ||
+        (rc = bat_upload(b, &b->d_groups, groups)) || (rc = bat_upload(b, &b->d_pairs_cum, b->pairs_cum)) ||
+        (rc = bat_upload(b, &b->d_dot_cum, b->dot_cum)) || (rc = bat_upload(b, &b->d_xor_cum, b->xor_cum)))
+        return cleanup(rc);
+    if ((rc = bat_alloc(b, &b->Sw, sw)) || (rc = bat_alloc(b, &b->Tt, coords)) || (rc = bat_alloc(b, &b->Ct, coords)) ||
+        (rc = bat_alloc(b, &b->Bc, coords)) || (rc = bat_alloc(b, &b->sel, coords)) ||
+        (rc = bat_alloc(b, &b->Qr, rank_tot)) || (rc = bat_alloc(b, &b->qbits, rank_tot / 64)) ||
+        (rc = bat_alloc(b, &b->sum_cnt, slots)) || (rc = bat_alloc(b, &b->sum_rank, slots * kWave)) ||
+        (rc = bat_alloc(b, &b->sum_q, slots * kWave)) || (rc = bat_alloc(b, &b->blk_q, G * kBlockK)) ||
+        (rc = bat_alloc(b, &b->blk_m, G * kBlockK)) || (rc = bat_alloc(b, &b->blk_g, G * kBlockK)) ||
+        (rc = bat_alloc(b, &b->Dcnt_blk, G * kBlockK)) || (rc = bat_alloc(b, &b->C_blk, blk)) ||
+        (rc = bat_alloc(b, &b->Sf_blk, blk)) || (rc = bat_alloc(b, &b->D_blk, blk)) ||
+        (rc = bat_alloc(b, &b->pair_i, blk)) || (rc = bat_alloc(b, &b->pair_c, blk)) ||
+        (rc = bat_alloc(b, &b->pw_list, pw)) || (rc = bat_alloc(b, &b->pw_off, pwoff)) ||
+        (rc = bat_alloc(b, &b->pw_cnt, G)) || (rc = bat_alloc(b, &b->status, 1)) ||
+        (rc = bat_alloc(b, &b->upd_bytes, 1)))
+        return cleanup(rc);
+    if (b->opt.store_cycles && (rc = bat_alloc(b, &b->C_store, store))) return cleanup(rc);
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(MCB_E_DEVICE, "hipStreamCreate"));
+    if (b->opt.timing) {
+        b->ev_pool.resize(4096);
+        b->ev_kind.resize(2048);
+        for (auto &e : b->ev_pool)
+            if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(MCB_E_DEVICE, "hipEventCreate"));
+    }
+    if ((rc = mcb_batch_reset(b))) return cleanup(rc);
+    *out = b;
+    return MCB_OK;
+}
+
+int32_t mcb_batch_steps(const mcb_batch *b) { return b ? b->steps : -1; }
+
+int32_t mcb_batch_step(const mcb_batch *b) { return b ? b->step : -1; }
+
+int mcb_batch_run(mcb_batch *b, int32_t steps) {
+    if (!b || steps < 0) return fail(MCB_E_INVALID, "invalid arguments");
+    const int32_t end = std::min(b->steps, b->step + steps);
+    while (b->step < end) {
+        int rc = launch_batch_step(b, b->step);
+        if (rc) return rc;
+        ++b->step;
+    }
+    return MCB_OK;
+}
+
+int mcb_batch_sync(mcb_batch *b) {
+    if (!b) return fail(MCB_E_INVALID, "invalid arguments");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    int rc = harvest_events(b);
+    if (rc) return rc;
+    uint32_t st = 0;
+    HIP_TRY(hipMemcpy(&st, b->status, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (st) return fail(MCB_E_NOT_FOUND, "a phase found no odd candidate");
+    return MCB_OK;
+}
+
+int mcb_batch_selected(mcb_batch *b, int32_t problem, int32_t *out, int32_t count) {
+    if (!b || !out || problem < 0 || problem >= b->G) return fail(MCB_E_INVALID, "invalid arguments");
+    const int32_t g = b->slot_of[problem];
+    const CompInfo &c = b->comps[g];
+    if (count < 0 || count > c.N) return fail(MCB_E_INVALID, "invalid arguments");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    std::vector<uint32_t> v(static_cast<size_t>(count));
+    if (count) HIP_TRY(hipMemcpy(v.data(), b->sel + c.cb, sizeof(uint32_t) * count, hipMemcpyDeviceToHost));
+    for (int32_t i = 0; i < count; ++i)
+        out[i] = v[i] == kNoRank ? -1 : static_cast<int32_t>(static_cast<int64_t>(v[i]) - b->rbase[g]);
+    return MCB_OK;
+}
+
+int mcb_batch_cycle_vectors(mcb_batch *b, int32_t problem, uint64_t *out, int32_t count) {
+    if (!b || !out || problem < 0 || problem >= b->G) return fail(MCB_E_INVALID, "invalid arguments");
+    if (!b->C_store) return fail(MCB_E_STATE, "batch was created without store_cycles");
+    const CompInfo &c = b->comps[b->slot_of[problem]];
+    if (count < 0 || count > c.N) return fail(MCB_E_INVALID, "invalid arguments");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (count)
+        HIP_TRY(hipMemcpy(out, b->C_store + c.store, sizeof(uint64_t) * static_cast<size_t>(count) * c.W,
+                          hipMemcpyDeviceToHost));
+    return MCB_OK;
+}
+
+int mcb_batch_kernel_time(mcb_batch *b, int kernel, double *total_ms, int64_t *launches) {
+    if (!b || kernel < 0 || kernel >= MCB_K_COUNT) return fail(MCB_E_INVALID, "invalid arguments");
+    int rc = harvest_events(b);
+    if (rc) return rc;
+    if (total_ms) *total_ms = b->kt_ms[kernel];
+    if (launches) *launches = b->kt_n[kernel];
+    return MCB_OK;
+}
+
+int64_t mcb_batch_device_bytes(const mcb_batch *b) { return b ? b->bytes : -1; }
+
+int mcb_batch_work_counters(mcb_batch *b, int64_t *positions, int64_t *candidates, int64_t *update_bytes) {
+    if (!b) return fail(MCB_E_INVALID, "invalid arguments");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    unsigned long long ub = 0;
+    HIP_TRY(hipMemcpy(&ub, b->upd_bytes, sizeof(ub), hipMemcpyDeviceToHost));
+    if (positions) *positions = b->blk_positions;
+    if (candidates) *candidates = b->blk_candidates;
+    if (update_bytes) *update_bytes = static_cast<int64_t>(ub);
+    return MCB_OK;
+}
+
+int mcb_batch_launch_shape(const mcb_batch *b, int32_t *blocks, int32_t *waves_per_block, int32_t *lds_bytes,
+                           int32_t *steps) {
+    if (!b) return fail(MCB_E_INVALID, "invalid arguments");
+    if (blocks) *blocks = b->q_blocks;
+    if (waves_per_block) *waves_per_block = kQWaves;
+    if (lds_bytes) *lds_bytes = b->q_lds;
+    if (steps) *steps = b->steps;
     return MCB_OK;
 }
 

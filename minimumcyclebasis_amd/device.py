@@ -41,6 +41,21 @@ def copy_bandwidth(nbytes: int = 1 << 30, iters: int = 20) -> float:
     return g.value
 
 
+def _desc_of(problem: Problem):
+    """The solver descriptor of a preprocessed problem and the host arrays it points into (keep them alive)."""
+    info = problem.info
+    keep = [problem.array(n) for n in ("tree_offset", "parent_pos", "pos_coord", "cand_root", "cand_pos_u",
+                                       "cand_pos_v", "cand_coord", "root_cand_offset", "root_cand_rank")]
+    d = SolverDesc()
+    d.num_roots = info.fvs_size
+    d.num_non_tree = info.num_non_tree
+    d.num_candidates = info.num_candidates
+    d.num_positions = info.positions
+    (d.tree_offset, d.parent_pos, d.pos_coord, d.cand_root, d.cand_pos_u, d.cand_pos_v, d.cand_coord,
+     d.root_cand_offset, d.root_cand_rank) = [a.ctypes.data for a in keep]
+    return d, keep
+
+
 class Solver:
     """Device-resident de Pina phase loop for one preprocessed problem."""
 
@@ -57,15 +72,7 @@ class Solver:
         self.F = info.fvs_size
         self.K = info.num_candidates
         self.P = info.positions
-        self._keep = [problem.array(n) for n in ("tree_offset", "parent_pos", "pos_coord", "cand_root", "cand_pos_u",
-                                                 "cand_pos_v", "cand_coord", "root_cand_offset", "root_cand_rank")]
-        d = SolverDesc()
-        d.num_roots = self.F
-        d.num_non_tree = self.N
-        d.num_candidates = self.K
-        d.num_positions = self.P
-        (d.tree_offset, d.parent_pos, d.pos_coord, d.cand_root, d.cand_pos_u, d.cand_pos_v, d.cand_coord,
-         d.root_cand_offset, d.root_cand_rank) = [a.ctypes.data for a in self._keep]
+        d, self._keep = _desc_of(problem)
         o = SolverOptions()
         _lib.hip().mcb_solver_options_init(ctypes.byref(o))
         o.store_cycles = int(bool(store_cycles))
@@ -146,6 +153,104 @@ class Solver:
     def close(self) -> None:
         if getattr(self, "_h", None):
             _lib.hip().mcb_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Blocked phase loops of many independent problems in one device arena (``mcb_batch_*``): step s
+    resolves phases 64s .. 64s+63 of every problem that has them, one launch per kernel for all of
+    them.  Each problem's selections equal its own ``Solver``'s."""
+
+    def __init__(self, problems, store_cycles: bool = False, timing: int | bool = False):
+        problems = list(problems)
+        if not problems:
+            raise ValueError("no problems")
+        for p in problems:
+            if p.info.single_cycle:
+                raise ValueError("a single-cycle graph has no phase loop")
+        self.problems = problems
+        self.N = [p.info.num_non_tree for p in problems]
+        self.W = [p.info.words for p in problems]
+        descs = (SolverDesc * len(problems))()
+        self._keep = []
+        for i, p in enumerate(problems):
+            d, keep = _desc_of(p)
+            descs[i] = d
+            self._keep.append(keep)
+        o = SolverOptions()
+        _lib.hip().mcb_solver_options_init(ctypes.byref(o))
+        o.store_cycles = int(bool(store_cycles))
+        o.timing = int(timing) if isinstance(timing, int) and not isinstance(timing, bool) else int(bool(timing))
+        h = ctypes.c_void_p()
+        _lib.check_hip(_lib.hip().mcb_batch_create(ctypes.cast(descs, ctypes.c_void_p), len(problems),
+                                                   ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+        self.store_cycles = bool(store_cycles)
+
+    @property
+    def steps(self) -> int:
+        return int(_lib.hip().mcb_batch_steps(self._h))
+
+    @property
+    def step(self) -> int:
+        return int(_lib.hip().mcb_batch_step(self._h))
+
+    def reset(self) -> None:
+        _lib.check_hip(_lib.hip().mcb_batch_reset(self._h))
+
+    def run(self, steps: int | None = None) -> None:
+        """Enqueue the next ``steps`` steps (all remaining when None)."""
+        n = self.steps if steps is None else int(steps)
+        _lib.check_hip(_lib.hip().mcb_batch_run(self._h, n))
+
+    def sync(self) -> None:
+        _lib.check_hip(_lib.hip().mcb_batch_sync(self._h))
+
+    def selected(self, problem: int, count: int | None = None) -> np.ndarray:
+        count = self.N[problem] if count is None else int(count)
+        out = np.zeros(count, dtype=np.int32)
+        _lib.check_hip(_lib.hip().mcb_batch_selected(self._h, int(problem), out.ctypes.data, count))
+        return out
+
+    def cycle_vectors(self, problem: int, count: int | None = None) -> np.ndarray:
+        count = self.N[problem] if count is None else int(count)
+        out = np.zeros((count, self.W[problem]), dtype=np.uint64)
+        _lib.check_hip(_lib.hip().mcb_batch_cycle_vectors(self._h, int(problem), out.ctypes.data, count))
+        return out
+
+    def kernel_time(self, kernel: str) -> tuple[float, int]:
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        _lib.check_hip(_lib.hip().mcb_batch_kernel_time(self._h, KERNELS.index(kernel), ctypes.byref(ms),
+                                                        ctypes.byref(n)))
+        return ms.value, n.value
+
+    @property
+    def device_bytes(self) -> int:
+        return int(_lib.hip().mcb_batch_device_bytes(self._h))
+
+    def work_counters(self) -> tuple[int, int, int]:
+        """(tree positions streamed, candidates read, support-update bytes) since reset."""
+        p = ctypes.c_int64()
+        c = ctypes.c_int64()
+        u = ctypes.c_int64()
+        _lib.check_hip(_lib.hip().mcb_batch_work_counters(self._h, ctypes.byref(p), ctypes.byref(c), ctypes.byref(u)))
+        return p.value, c.value, u.value
+
+    def launch_shape(self) -> dict:
+        v = [ctypes.c_int32() for _ in range(4)]
+        _lib.check_hip(_lib.hip().mcb_batch_launch_shape(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("blocks", "waves_per_block", "lds_bytes", "steps"), (x.value for x in v)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.hip().mcb_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -256,6 +361,11 @@ def minimum_cycle_basis(source, reduce_degree_two: bool = True, prune_trees: boo
     dev_ms = sum(s.kernel_time(k)[0] for k in KERNELS)
     mem_mb = s.device_bytes / 2**20
     s.close()
+    return _result_of(prob, sel, t_host, t_loop, dev_ms, mem_mb)
+
+
+def _result_of(prob: Problem, sel: np.ndarray, t_host: float, t_loop: float, dev_ms: float,
+               mem_mb: float) -> MCBResult:
     r, a, b, w = prob.candidate_table()
     ww = w[sel].astype(np.int64)
     total = int(ww.sum())
@@ -273,10 +383,16 @@ class ComponentsResult:
 
 
 def minimum_cycle_basis_components(source, reduce_degree_two: bool = True, prune_trees: bool = True,
-                                   num_threads: int = 0, block_phases: int = 0) -> ComponentsResult:
+                                   num_threads: int = 0, block_phases: int = 0,
+                                   batched: bool = False) -> ComponentsResult:
     """Minimum cycle basis of a graph solved one biconnected component at a time, in process (the reference runs
     main.sh: bicc decomposition, relabelling, one MCB run per component).  ``source`` is a .mtx path or
-    (n, u, v, w).  Each component's result carries its stats row (``MCBResult.stats``)."""
+    (n, u, v, w).  Each component's result carries its stats row (``MCBResult.stats``).
+
+    batched=False runs one ``Solver`` per component, one after the other.  batched=True runs every component's
+    phase loop together in one ``Batch`` (same selections); each row's main-loop time is then the shared batch's."""
+    import time
+
     from . import generators
     from .components import split_components
 
@@ -284,13 +400,36 @@ def minimum_cycle_basis_components(source, reduce_degree_two: bool = True, prune
         n, u, v, w = generators.read_mtx(source)
     else:
         n, u, v, w = source
-    parts = []
-    total = 0
-    cycles = 0
-    for comp in split_components(n, u, v, w):
-        res = minimum_cycle_basis((comp.n, comp.u, comp.v, comp.w), reduce_degree_two=reduce_degree_two,
-                                  prune_trees=prune_trees, num_threads=num_threads, block_phases=block_phases)
-        parts.append((comp, res))
-        total += res.total_weight
-        cycles += res.num_cycles
-    return ComponentsResult(total, cycles, parts)
+    comps = split_components(n, u, v, w)
+    if not batched:
+        parts = []
+        for comp in comps:
+            res = minimum_cycle_basis((comp.n, comp.u, comp.v, comp.w), reduce_degree_two=reduce_degree_two,
+                                      prune_trees=prune_trees, num_threads=num_threads, block_phases=block_phases)
+            parts.append((comp, res))
+        return ComponentsResult(sum(r.total_weight for _, r in parts), sum(r.num_cycles for _, r in parts), parts)
+    probs, t_host = [], []
+    for comp in comps:
+        t0 = time.perf_counter()
+        probs.append(Problem.from_edges(comp.n, comp.u, comp.v, comp.w, reduce_degree_two=reduce_degree_two,
+                                        prune_trees=prune_trees, num_threads=num_threads))
+        t_host.append(time.perf_counter() - t0)
+    loop = [i for i, p in enumerate(probs) if not p.info.single_cycle]
+    results = {}
+    for i, p in enumerate(probs):
+        if p.info.single_cycle:
+            results[i] = minimum_cycle_basis(p)
+            results[i].stats.construction_trees_s = t_host[i]
+    if loop:
+        t1 = time.perf_counter()
+        bt = Batch([probs[i] for i in loop], timing=2)
+        bt.run()
+        bt.sync()
+        t_loop = time.perf_counter() - t1
+        dev_ms = sum(bt.kernel_time(k)[0] for k in KERNELS)
+        mem_mb = bt.device_bytes / 2**20
+        for j, i in enumerate(loop):
+            results[i] = _result_of(probs[i], bt.selected(j), t_host[i], t_loop, dev_ms, mem_mb)
+        bt.close()
+    parts = [(comps[i], results[i]) for i in range(len(comps))]
+    return ComponentsResult(sum(r.total_weight for _, r in parts), sum(r.num_cycles for _, r in parts), parts)
