@@ -37,6 +37,17 @@ WORKLOADS = {
                      lambda: G.chung_lu_lbc(50000, 2.3, 8.0, 10000, seed=1)),
 }
 
+# graphs of many biconnected components: solved component by component, batched (one arena) and plain
+# (one solver per component, one after the other) timed side by side
+COMPONENT_WORKLOADS = {
+    "components2000": ("2000 random 2-connected components, vertex count log-uniform in [50, 5000], average "
+                       "degree U[4, 8], weights U{1..1000}, joined by 1999 bridges, seed 1",
+                       lambda: G.batched_components(k=2000, nmin=50, nmax=5000, seed=1)[:4]),
+    "components200": ("200 random 2-connected components, vertex count log-uniform in [50, 2000], average "
+                      "degree U[4, 8], weights U{1..1000}, joined by 199 bridges, seed 1",
+                      lambda: G.batched_components(k=200, nmin=50, nmax=2000, seed=1)[:4]),
+}
+
 
 def host_cores() -> int:
     """Host threads for the CPU legs: the affinity mask, capped by OMP_NUM_THREADS when set."""
@@ -91,7 +102,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="er20000", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="er20000", choices=sorted(WORKLOADS) + sorted(COMPONENT_WORKLOADS))
+    ap.add_argument("--no-plain", action="store_true",
+                    help="component workloads: skip the plain one-solver-per-component timing")
     ap.add_argument("--no-prune", action="store_true")
     ap.add_argument("--breakdown", action="store_true",
                     help="time every kernel (HIP events between all launches), not only the dominant one")
@@ -118,6 +131,8 @@ def main() -> int:
               flush=True)
         return 0
 
+    if args.workload in COMPONENT_WORKLOADS:
+        return run_components(args)
     desc, gen = WORKLOADS[args.workload]
     n, u, v, w = gen()
     threads = host_cores()
@@ -212,6 +227,7 @@ def main() -> int:
         "host_preprocess_s": round(host_s, 3),
         "upload_s": round(upload_s, 3),
         "block_phases": shape["block_phases"],
+        "tree_pass_waves": shape["waves_per_block"],
         "launches_per_step": k_n // max(args.steps, 1),
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in kt.items() if v[1]},
         "update_bytes_per_step": upd // max(args.steps, 1),
@@ -233,6 +249,180 @@ def main() -> int:
     solver.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, u, v, w, args.cpu_budget, threads)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_baseline_components(comps, work, budget_s: float, threads: int) -> dict:
+    """The CPU oracle (the reference's CPU phase loop restated, mcb/mcb_cycle.cpp:226-305) over whole
+    components taken in a seeded random order until about budget_s is spent; the total is extrapolated by
+    the oracle's work model (work[i]: phases x roots x vertices of component i, 0 for a single cycle)."""
+    from minimumcyclebasis_amd.host import Problem
+    from oracle import pyoracle
+
+    rng = np.random.default_rng(5)
+    order = rng.permutation(len(comps))
+    work = np.asarray(work, dtype=np.float64)
+    spent = done_work = 0.0
+    done = 0
+    median = float(np.median(work[work > 0])) if (work > 0).any() else 0.0
+    for i in order:
+        if work[i] == 0.0:
+            continue
+        # skip a component predicted to overrun the budget (before any rate is known: one above the median)
+        if done_work == 0.0 and work[i] > median:
+            continue
+        if done_work > 0.0 and work[i] * spent / done_work > budget_s - spent:
+            continue
+        c = comps[i]
+        p = Problem.from_edges(c.n, c.u, c.v, c.w, keep_vertex_trees=True, num_threads=threads)
+        t0 = time.perf_counter()
+        pyoracle.run(p, num_threads=threads)
+        spent += time.perf_counter() - t0
+        done_work += work[i]
+        done += 1
+        p.close()
+        if spent >= budget_s:
+            break
+    total = spent * work.sum() / max(done_work, 1.0)
+    return {"value": round(float(total), 3), "unit": "s per phase loop over all components (extrapolated)",
+            "cores": threads, "kind": "port",
+            "sample": f"{done} of {len(comps)} components (random order, whole loops) timed ({spent:.2f} s), "
+                      f"scaled by phases x roots x vertices"}
+
+
+def run_components(args) -> int:
+    """Component workloads: split into biconnected components, preprocess each on the host, then time the
+    batched loop (every component in one arena) and the plain one (one solver per component, in turn)."""
+    from minimumcyclebasis_amd.components import build_problems, split_components
+    from minimumcyclebasis_amd.device import KERNELS, Batch, Solver, device_info
+
+    desc, gen = COMPONENT_WORKLOADS[args.workload]
+    n, u, v, w = gen()
+    threads = host_cores()
+    t0 = time.perf_counter()
+    comps = split_components(n, u, v, w)
+    probs = build_problems(comps, threads=threads, prune_trees=not args.no_prune)
+    host_s = time.perf_counter() - t0
+    loop = [p for p in probs if not p.info.single_cycle]
+    single = len(probs) - len(loop)
+    Ns = [p.info.num_non_tree for p in loop]
+    tot = {k: int(sum(getattr(p.info, k) for p in loop)) for k in ("num_non_tree", "num_candidates", "positions",
+                                                                     "fvs_size")}
+    print(f"[bench] {args.workload}: {len(comps)} components ({single} single cycles) N={tot['num_non_tree']} "
+          f"K={tot['num_candidates']} positions={tot['positions']} F={tot['fvs_size']} host={host_s:.2f}s",
+          file=sys.stderr, flush=True)
+    dev = device_info()
+    t0 = time.perf_counter()
+    bt = Batch(loop, timing=1 if args.breakdown else 2)
+    upload_s = time.perf_counter() - t0
+    print(f"[bench] batch arena {bt.device_bytes / 2**30:.2f} GiB, {bt.steps} steps, upload {upload_s:.2f}s",
+          file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        bt.reset()
+        bt.run()
+        bt.sync()
+        print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
+    kt = {k: [0.0, 0] for k in KERNELS}
+    pos = cand = upd = 0
+    t_total = 0.0
+    for i in range(args.steps):
+        bt.reset()
+        t0 = time.perf_counter()
+        bt.run()
+        bt.sync()
+        dt = time.perf_counter() - t0
+        t_total += dt
+        for k in KERNELS:
+            ms, cnt = bt.kernel_time(k)
+            kt[k][0] += ms
+            kt[k][1] += cnt
+        p_, c_, u_ = bt.work_counters()
+        pos += p_
+        cand += c_
+        upd += u_
+        print(f"[bench] step {i}: {dt:.3f} s", file=sys.stderr, flush=True)
+    batched_s = t_total / args.steps
+    sels = [bt.selected(j) for j in range(len(loop))]
+    total_weight = sum(int(p.array("cand_weight")[sel].astype(np.int64).sum()) for p, sel in zip(loop, sels))
+    total_weight += sum(int(p.info.input_total_weight) for p in probs if p.info.single_cycle)
+    shape = bt.launch_shape()
+    bt.close()
+
+    plain_s = None
+    if not args.no_plain:
+        # the same components, one solver at a time (creation untimed), as the per-component driver runs them
+        plain_s = 0.0
+        for j, p in enumerate(loop):
+            s_ = Solver(p)
+            t0 = time.perf_counter()
+            s_.run()
+            s_.sync()
+            plain_s += time.perf_counter() - t0
+            if not np.array_equal(s_.selected(), sels[j]):
+                print(f"[bench] component {j}: batched and plain selections differ", file=sys.stderr, flush=True)
+                return 1
+            s_.close()
+        print(f"[bench] plain: {plain_s:.3f} s", file=sys.stderr, flush=True)
+
+    k_ms, k_n = kt["parity_search"]
+    avg_ms = k_ms / max(k_n, 1)
+    bytes_per_launch = (8.0 * pos + 16.0 * cand) / max(k_n, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": round(batched_s, 6),
+        "unit": "s per MCB phase loop over all components",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * batched_s, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": args.workload,
+            "graph": desc,
+            "components": len(comps),
+            "single_cycle_components": single,
+            "num_non_tree": tot["num_non_tree"],
+            "largest_num_non_tree": max(Ns),
+            "fvs_size": tot["fvs_size"],
+            "candidates": tot["num_candidates"],
+            "tree_positions": tot["positions"],
+            "pruned_trees": not args.no_prune,
+            "form": "batched",
+            "device": dev["name"],
+        },
+        "total_weight": total_weight,
+        "host_preprocess_s": round(host_s, 3),
+        "upload_s": round(upload_s, 3),
+        "steps_per_loop": shape["steps"],
+        "tree_pass_waves": shape["waves_per_block"],
+        "plain_loop_s": None if plain_s is None else round(plain_s, 6),
+        "speedup_vs_plain": None if plain_s is None else round(plain_s / batched_s, 2),
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in kt.items() if v[1]},
+        "update_bytes_per_step": upd // max(args.steps, 1),
+        "roofline": {
+            "kernel": "k_batch_q",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic("k_batch_q", args.workload) if not args.no_prune else None,
+            "bytes_per_launch": round(bytes_per_launch, 1),
+            "avg_launch_us": round(avg_ms * 1000.0, 3),
+        },
+    }
+    work = [0.0 if p.info.single_cycle else float(p.info.num_non_tree) * p.info.fvs_size * p.info.nodes
+            for p in probs]
+    for p in probs:
+        p.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_components(comps, work, args.cpu_budget, threads)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -119,7 +119,8 @@ typedef struct {
 
 typedef struct {
     int32_t store_cycles;     /* 1: keep every phase's cycle vector (N*W words) for mcb_solver_cycle_vectors */
-    int32_t waves_per_block;  /* parity/search kernel; 0 = automatic */
+    int32_t waves_per_block;  /* blocked form's tree pass: waves per workgroup, 4, 8 or 16; 0 = the fastest
+                                 of those whose LDS fits, timed on the first block at create */
     int32_t max_blocks;       /* 0 = automatic (resident blocks on every compute unit) */
     int32_t timing;           /* HIP events on the solver's stream around kernel launches: 0 none,
                                  1 every kernel, 2 only MCB_K_PARITY_SEARCH (the dominant kernel) */
@@ -204,8 +205,9 @@ void mcb_solver_destroy(mcb_solver *s);
 
 typedef struct mcb_batch mcb_batch;
 
-/* descs[0..count-1] are validated like mcb_solver_create's; of opt, store_cycles and timing are
- * used.  Fails with MCB_E_INVALID when a tree does not fit the blocked form's LDS budget. */
+/* descs[0..count-1] are validated like mcb_solver_create's; of opt, store_cycles, timing and
+ * waves_per_block are used (the width is timed on step 0).  Fails with MCB_E_INVALID when a tree does
+ * not fit the blocked form's LDS budget. */
 int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solver_options *opt, mcb_batch **out);
 
 /* Back to the unit support vectors, step 0. */

@@ -32,6 +32,24 @@ class Component:
         return int(self.u.size) - self.n + 1
 
 
+def build_problems(comps: list[Component], threads: int = 1, **options) -> list:
+    """Host preprocessing (``host.Problem.from_edges``) of every component, ``threads`` components at a
+    time, each on one thread (the library call releases the interpreter lock).  Results in input order."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from .host import Problem
+
+    options.setdefault("num_threads", 1)
+
+    def one(c):
+        return Problem.from_edges(c.n, c.u, c.v, c.w, **options)
+
+    if threads <= 1:
+        return [one(c) for c in comps]
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(one, comps))
+
+
 def split_components(n: int, u, v, w, min_edges: int = 2) -> list[Component]:
     """Biconnected components of a simple undirected graph with at least ``min_edges`` edges (a single edge, a
     bridge, has no cycle), in the order the decomposition completes them."""

@@ -2195,6 +2195,30 @@ void mcb_gpu_destroy(mcb_gpu *g) {
 // Native form
 // ---------------------------------------------------------------------------------------------
 
+namespace {
+
+// Launch plan of the blocked tree pass for nw waves per workgroup.  Trees with more than small_max
+// positions are shared by the whole workgroup, the others take one wave; small_max is the largest
+// multiple of 64 (at most 1024) whose nw one-wave slices fit the LDS the largest tree needs anyway.
+// The roots come in ranges (one per problem of a batch, one for a single problem), each range's trees
+// largest first; big_cum/small_cum count the list entries before each range.
+struct TreePlan {
+    int32_t nw = 0, big_words = 0, small_words = 0, lds = 0, blocks = 0, n_big = 0, n_small = 0;
+    uint32_t *big = nullptr, *small = nullptr;
+    uint8_t *narrow = nullptr;  // per big tree: 1 if its levels average fewer than 128 positions
+    std::vector<int32_t> big_cum, small_cum;
+};
+
+void free_plan(TreePlan &p) {
+    if (p.big) (void)hipFree(p.big);
+    if (p.small) (void)hipFree(p.small);
+    if (p.narrow) (void)hipFree(p.narrow);
+    p.big = p.small = nullptr;
+    p.narrow = nullptr;
+}
+
+}  // namespace
+
 struct mcb_solver {
     mcb_solver_options opt{};
     int32_t F = 0, N = 0, W = 0, max_pos = 0, p_words_big = 0, p_words_small = 0, n_big = 0, n_small = 0;
@@ -2223,9 +2247,11 @@ struct mcb_solver {
     int64_t bytes = 0;
     // blocked form (block_phases > 1)
     int32_t bk = 1;                                   // phases per block; 1 = one kernel per phase
-    int32_t q_big_words = 0, q_small_words = 0, q_blocks = 0, q_lds = 0, n_qbig = 0, n_qsmall = 0;
-    uint32_t *q_big_list = nullptr, *q_small_list = nullptr;
+    int32_t q_waves = 0, q_big_words = 0, q_small_words = 0, q_blocks = 0, q_lds = 0, n_qbig = 0, n_qsmall = 0;
+    uint32_t *q_big_list = nullptr, *q_small_list = nullptr;  // the chosen plan's lists
     uint8_t *q_big_narrow = nullptr;
+    std::vector<TreePlan> plans;
+    int32_t plan = 0;
     uint64_t *Tt = nullptr, *Qr = nullptr, *qbits = nullptr, *sum_q = nullptr, *blk_q = nullptr, *blk_m = nullptr, *blk_g = nullptr;
     uint64_t *C_blk = nullptr, *Sf_blk = nullptr, *Ct = nullptr, *Bc = nullptr, *pair_c = nullptr;
     uint32_t *pw_list = nullptr, *pw_off = nullptr, *pw_cnt = nullptr, *pair_i = nullptr;
@@ -2407,7 +2433,7 @@ int launch_block(mcb_solver *s, int32_t e, int32_t kk) {
     hipLaunchKernelGGL(k_block_transpose, dim3(static_cast<unsigned>((wl + 3) / 4)), dim3(256), 0, s->stream, a);
     HIP_TRY(hipGetLastError());
     if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_PARITY_SEARCH))) return rc;
-    hipLaunchKernelGGL(k_block_q, dim3(s->q_blocks), dim3(kQWaves * kWave), s->q_lds, s->stream, a);
+    hipLaunchKernelGGL(k_block_q, dim3(s->q_blocks), dim3(s->q_waves * kWave), s->q_lds, s->stream, a);
     HIP_TRY(hipGetLastError());
     if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_SELECT))) return rc;
     hipLaunchKernelGGL(k_block_basis, dim3(static_cast<unsigned>((s->lvl_n[0] + 3) / 4)), dim3(256), 0, s->stream, a);
@@ -2522,70 +2548,165 @@ void preorder_intervals(const int64_t *tree_offset, const uint32_t *parent_pos, 
     }
 }
 
-// Buffers and launch shape of the blocked form.  Big trees take a whole workgroup of the tree pass
-// (their path words must fit the LDS of one workgroup); small ones take one wave.  Chooses the per-phase form instead when block_phases asks for it or when the largest
-// tree does not fit.
-template <typename NposFn>
-int setup_blocked(mcb_solver *s, const std::vector<uint32_t> &order, NposFn npos_of, const int64_t *tree_offset,
-                  const uint32_t *parent_pos) {
-    const int want = s->opt.block_phases;
-    if (want < 0 || want > kBlockK) return fail(MCB_E_INVALID, "block_phases must be 0..64");
-    if (want == 1) return MCB_OK;
-    // Small trees take one wave and a slice of 1/kQWaves of the workgroup's LDS; the threshold is the
-    // largest multiple of 64 (at most 1024) whose slices fit the space the largest tree needs anyway.
-    int64_t largest = 0;
-    for (uint32_t r : order) largest = std::max<int64_t>(largest, npos_of(r));
-    const int64_t largest_bytes = 8 * ((largest + kWave - 1) / kWave * kWave);
-    int64_t kSmallMax = std::min<int64_t>(1024, largest_bytes / (8 * kQWaves) / kWave * kWave);
-    kSmallMax = std::max<int64_t>(kSmallMax, kWave);
-    std::vector<uint32_t> big, small;
-    int64_t max_big = 0, max_small = 1;
-    for (uint32_t r : order) {
-        const int64_t np = npos_of(r);
-        if (np > kSmallMax) {
-            big.push_back(r);
-            max_big = std::max(max_big, np);
-        } else {
-            small.push_back(r);
-            max_small = std::max(max_small, np);
+// Levels below the root of every tree.
+std::vector<uint32_t> tree_depths(const int64_t *tree_off, const uint32_t *parent_pos, int64_t F) {
+    std::vector<uint32_t> depth(static_cast<size_t>(F), 0), level;
+    for (int64_t r = 0; r < F; ++r) {
+        const int64_t a0 = tree_off[r], n = tree_off[r + 1] - a0;
+        level.assign(static_cast<size_t>(n), 0);
+        uint32_t d = 0;
+        for (int64_t x = 1; x < n; ++x) {
+            level[x] = level[parent_pos[a0 + x]] + 1;
+            d = std::max(d, level[x]);
         }
+        depth[r] = d;
     }
-    // a big tree whose levels average fewer than 128 positions is walked by one wave
-    std::vector<uint8_t> narrow(big.size(), 0);
-    {
-        std::vector<uint32_t> level;
-        for (size_t i = 0; i < big.size(); ++i) {
-            const uint32_t r = big[i];
-            const int64_t a0 = tree_offset[r], n = tree_offset[r + 1] - a0;
-            level.assign(static_cast<size_t>(n), 0);
-            uint32_t depth = 0;
-            for (int64_t x = 1; x < n; ++x) {
-                level[x] = level[parent_pos[a0 + x]] + 1;
-                depth = std::max(depth, level[x]);
+    return depth;
+}
+
+// Builds the plan for nw waves per workgroup; MCB_E_INVALID when its LDS does not fit one workgroup.
+int plan_tree_pass(int nw, const int64_t *tree_off, const std::vector<uint32_t> &depth,
+                   const std::vector<int64_t> &ranges, const void *kernel, TreePlan &p, int64_t &bytes) {
+    const size_t G = ranges.size() - 1;
+    int64_t largest = 0;
+    for (int64_t r = 0; r < ranges.back(); ++r) largest = std::max<int64_t>(largest, tree_off[r + 1] - tree_off[r]);
+    const int64_t largest_words = (largest + kWave - 1) / kWave * kWave;
+    const int64_t small_max = std::max<int64_t>(kWave, std::min<int64_t>(1024, largest_words / nw / kWave * kWave));
+    std::vector<uint32_t> big, small, order;
+    std::vector<uint8_t> narrow;
+    p.big_cum.assign(G + 1, 0);
+    p.small_cum.assign(G + 1, 0);
+    int64_t max_big = 0, max_small = 1;
+    for (size_t g = 0; g < G; ++g) {
+        order.resize(static_cast<size_t>(ranges[g + 1] - ranges[g]));
+        std::iota(order.begin(), order.end(), static_cast<uint32_t>(ranges[g]));
+        auto npos = [&](uint32_t r) { return tree_off[r + 1] - tree_off[r]; };
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return npos(x) > npos(y); });
+        for (uint32_t r : order) {
+            const int64_t np = npos(r);
+            if (np > small_max) {
+                big.push_back(r);
+                narrow.push_back(np < 128ll * (depth[r] + 1) ? 1 : 0);
+                max_big = std::max(max_big, np);
+            } else {
+                small.push_back(r);
+                max_small = std::max(max_small, np);
             }
-            narrow[i] = n < 128ll * (depth + 1) ? 1 : 0;
         }
+        p.big_cum[g + 1] = static_cast<int32_t>(big.size());
+        p.small_cum[g + 1] = static_cast<int32_t>(small.size());
     }
     const int64_t big_words = (max_big + kWave - 1) / kWave * kWave;
     const int64_t small_words = (max_small + kWave - 1) / kWave * kWave;
-    const int64_t lds = std::max<int64_t>(8 * big_words + 4 * (big_words / kWave), 8 * kQWaves * small_words);
+    const int64_t lds = std::max<int64_t>(8 * big_words + 4 * (big_words / kWave), 8ll * nw * small_words);
     const int64_t lds_max = std::min<int64_t>(g_dev.prop.sharedMemPerBlock, 160 * 1024);
-    if (lds > lds_max) {
-        if (want > 1) return fail(MCB_E_INVALID, "a tree is too large for the blocked form's LDS budget");
+    if (lds > lds_max) return MCB_E_INVALID;
+    p.nw = nw;
+    p.big_words = static_cast<int32_t>(big_words);
+    p.small_words = static_cast<int32_t>(small_words);
+    p.lds = static_cast<int32_t>(lds);
+    p.n_big = static_cast<int32_t>(big.size());
+    p.n_small = static_cast<int32_t>(small.size());
+    HIP_TRY(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_max)));
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, nw * kWave, p.lds));
+    p.blocks = std::max(1, per_cu) * g_dev.prop.multiProcessorCount;
+    int rc;
+    if ((rc = upload(&p.big, big.data(), big.size(), bytes)) || (rc = upload(&p.small, small.data(), small.size(), bytes)) ||
+        (rc = upload(&p.narrow, narrow.data(), narrow.size(), bytes)))
+        return rc;
+    return MCB_OK;
+}
+
+// The plans worth trying: the requested width, or 16, 8 and 4 waves (those whose LDS fits).
+int make_plans(int want, const int64_t *tree_off, const uint32_t *parent_pos, const std::vector<int64_t> &ranges,
+               const void *kernel, std::vector<TreePlan> &plans, int64_t &bytes) {
+    if (want != 0 && want != 4 && want != 8 && want != 16)
+        return fail(MCB_E_INVALID, "waves_per_block must be 0, 4, 8 or 16");
+    const std::vector<uint32_t> depth = tree_depths(tree_off, parent_pos, ranges.back());
+    for (int nw : {16, 8, 4}) {
+        if (want && nw != want) continue;
+        TreePlan p;
+        const int rc = plan_tree_pass(nw, tree_off, depth, ranges, kernel, p, bytes);
+        if (rc == MCB_OK) plans.push_back(std::move(p));
+        else {
+            free_plan(p);
+            if (rc != MCB_E_INVALID) return rc;
+        }
+    }
+    return MCB_OK;
+}
+
+// Makes plan i the tree pass's launch shape.
+void use_plan(mcb_solver *s, size_t i) {
+    const TreePlan &p = s->plans[i];
+    s->plan = static_cast<int32_t>(i);
+    s->q_waves = p.nw;
+    s->q_big_words = p.big_words;
+    s->q_small_words = p.small_words;
+    s->q_lds = p.lds;
+    s->q_blocks = p.blocks;
+    s->n_qbig = p.n_big;
+    s->n_qsmall = p.n_small;
+    s->q_big_list = p.big;
+    s->q_small_list = p.small;
+    s->q_big_narrow = p.narrow;
+}
+
+// Times `launch` (one tree pass under the plan `use(i)` selects) for every plan and leaves the fastest
+// selected: the first of three runs warms up, the best of the other two counts.
+template <typename UseFn, typename LaunchFn>
+int pick_fastest(size_t nplans, hipStream_t stream, UseFn use, LaunchFn launch) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return fail(MCB_E_DEVICE, "hipEventCreate");
+    }
+    float best = 0.0f;
+    size_t bi = 0;
+    int rc = MCB_OK;
+    for (size_t i = 0; i < nplans && rc == MCB_OK; ++i) {
+        use(i);
+        float t = 0.0f;
+        for (int rep = 0; rep < 3; ++rep) {
+            float ms = 0.0f;
+            if (hipEventRecord(e0, stream) != hipSuccess || (rc = launch()) != MCB_OK ||
+                hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                if (rc == MCB_OK) rc = fail(MCB_E_DEVICE, "tree pass timing");
+                break;
+            }
+            if (rep == 1 || (rep == 2 && ms < t)) t = ms;
+        }
+        if (rc == MCB_OK && (i == 0 || t < best)) {
+            best = t;
+            bi = i;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    use(bi);
+    return MCB_OK;
+}
+
+// Buffers and launch shape of the blocked form.  Chooses the per-phase form instead when
+// block_phases asks for it or (block_phases 0) when no tree pass plan fits the LDS.
+int setup_blocked(mcb_solver *s, const int64_t *tree_offset, const uint32_t *parent_pos) {
+    const int want = s->opt.block_phases;
+    if (want < 0 || want > kBlockK) return fail(MCB_E_INVALID, "block_phases must be 0..64");
+    if (want == 1) return MCB_OK;
+    const std::vector<int64_t> ranges{0, static_cast<int64_t>(s->F)};
+    int rc = make_plans(s->opt.waves_per_block, tree_offset, parent_pos, ranges,
+                        reinterpret_cast<const void *>(k_block_q), s->plans, s->bytes);
+    if (rc) return rc;
+    if (s->plans.empty()) {
+        if (want > 1 || s->opt.waves_per_block)
+            return fail(MCB_E_INVALID, "a tree is too large for the blocked form's LDS budget");
         return MCB_OK;  // automatic choice: fall back to one kernel per phase
     }
     s->bk = want > 1 ? want : kBlockK;
-    s->q_big_words = static_cast<int32_t>(big_words);
-    s->q_small_words = static_cast<int32_t>(small_words);
-    s->q_lds = static_cast<int32_t>(lds);
-    s->n_qbig = static_cast<int32_t>(big.size());
-    s->n_qsmall = static_cast<int32_t>(small.size());
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(k_block_q), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                s->q_lds));
-    int per_cu = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_block_q, kQWaves * kWave, s->q_lds));
-    per_cu = std::max(1, per_cu);
-    s->q_blocks = per_cu * g_dev.prop.multiProcessorCount;
+    use_plan(s, 0);
     // greedy-basis levels: kChunk ranks per first-level basis, kGroup bases per merge
     int64_t n = (s->K + kChunk - 1) / kChunk, slots = 0;
     s->levels = 0;
@@ -2599,10 +2720,6 @@ int setup_blocked(mcb_solver *s, const std::vector<uint32_t> &order, NposFn npos
         n = (n + kGroup - 1) / kGroup;
     }
     int64_t &B = s->bytes;
-    int rc;
-    if ((rc = upload(&s->q_big_list, big.data(), big.size(), B))) return rc;
-    if ((rc = upload(&s->q_big_narrow, narrow.data(), narrow.size(), B))) return rc;
-    if ((rc = upload(&s->q_small_list, small.data(), small.size(), B))) return rc;
     if ((rc = dev_alloc(&s->Tt, static_cast<size_t>(s->npad), B))) return rc;
     if ((rc = dev_alloc(&s->Qr, static_cast<size_t>(s->K), B))) return rc;
     if ((rc = dev_alloc(&s->qbits, static_cast<size_t>((s->K + 63) / 64), B))) return rc;
@@ -2642,11 +2759,12 @@ void mcb_solver_destroy(mcb_solver *s) {
                     s->root_min_rank, s->rc_off, s->rc_pu, s->rc_pv, s->rc_rank, s->rc_coord, s->cand_root,
                     s->cand_pu, s->cand_pv, s->cand_coord, s->Sw, s->Scur, s->C, s->C_store, s->D, s->Dcount,
                     s->best, s->done, s->status, s->wave_stats, s->flag, s->E, s->Ecount, s->cdot,
-                    s->q_big_list, s->q_big_narrow, s->q_small_list, s->Tt, s->Qr, s->qbits, s->sum_q, s->blk_q, s->blk_m, s->blk_g, s->C_blk,
+                    s->Tt, s->Qr, s->qbits, s->sum_q, s->blk_q, s->blk_m, s->blk_g, s->C_blk,
                     s->Sf_blk, s->sum_cnt, s->sum_rank, s->D_blk, s->Dcnt_blk, s->upd_bytes,
                     s->Ct, s->Bc, s->pair_c, s->pw_list, s->pw_off, s->pw_cnt, s->pair_i};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (TreePlan &p : s->plans) free_plan(p);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -2762,7 +2880,7 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
     blocks = std::min<int64_t>(blocks, s->n_big + (s->n_small + kBlockWaves - 1) / kBlockWaves);
     s->blocks = static_cast<int>(std::max<int64_t>(1, blocks));
     if ((rc = dev_alloc(&s->wave_stats, 3 * static_cast<size_t>(s->blocks) * s->wpb, B))) return cleanup(rc);
-    if ((rc = setup_blocked(s, order, npos_of, d->tree_offset, d->parent_pos))) return cleanup(rc);
+    if ((rc = setup_blocked(s, d->tree_offset, d->parent_pos))) return cleanup(rc);
     if (s->opt.timing) {
         s->ev_pool.resize(4096);
         s->ev_kind.resize(2048);
@@ -2771,6 +2889,23 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
     }
     rc = mcb_solver_reset(s);
     if (rc) return cleanup(rc);
+    if (s->plans.size() > 1) {
+        // tree pass width: time every plan on the first block (from the unit support vectors)
+        const int32_t kk = std::min(s->bk, s->N);
+        const BlockArgs a = block_args(s, 0, kk);
+        const unsigned words = static_cast<unsigned>((kk + kWave - 1) / kWave);
+        hipLaunchKernelGGL(k_block_transpose, dim3((words + 3) / 4), dim3(256), 0, s->stream, a);
+        if (hipGetLastError() != hipSuccess) return cleanup(fail(MCB_E_DEVICE, "k_block_transpose"));
+        rc = pick_fastest(
+            s->plans.size(), s->stream, [&](size_t i) { use_plan(s, i); },
+            [&]() -> int {
+                const BlockArgs b = block_args(s, 0, kk);
+                hipLaunchKernelGGL(k_block_q, dim3(s->q_blocks), dim3(s->q_waves * kWave), s->q_lds, s->stream, b);
+                HIP_TRY(hipGetLastError());
+                return MCB_OK;
+            });
+        if (rc || (rc = mcb_solver_reset(s))) return cleanup(rc);
+    }
     *out = s;
     return MCB_OK;
 }
@@ -2932,7 +3067,7 @@ int mcb_solver_launch_shape(const mcb_solver *s, int32_t *blocks, int32_t *waves
     if (!s) return fail(MCB_E_INVALID, "invalid arguments");
     const bool blk = s->bk > 1;
     if (blocks) *blocks = blk ? s->q_blocks : s->blocks;
-    if (waves_per_block) *waves_per_block = blk ? kQWaves : s->wpb;
+    if (waves_per_block) *waves_per_block = blk ? s->q_waves : s->wpb;
     if (lds_bytes) *lds_bytes = blk ? s->q_lds : s->lds_bytes;
     if (block_phases) *block_phases = s->bk;
     return MCB_OK;
@@ -2985,7 +3120,7 @@ struct mcb_batch {
     std::vector<CompInfo> comps;    // by arena position
     std::vector<int64_t> rbase;     // rank base by arena position
     std::vector<int32_t> active;    // problems still running at each step
-    std::vector<int32_t> big_cum, small_cum, chunk_cum, pairs_cum;  // [G + 1] prefix counts
+    std::vector<int32_t> chunk_cum, pairs_cum;                      // [G + 1] prefix counts
     std::vector<std::vector<int32_t>> grp_cum;                      // per merge level: [G + 1]
     std::vector<int64_t> grp_off;                                   // per merge level: first group
     std::vector<int64_t> step_off;                                  // per step: first entry of its tables
@@ -2993,11 +3128,11 @@ struct mcb_batch {
     std::vector<int64_t> pos_cum, cand_cum;                         // [G + 1] tree positions, candidates
     // device arrays
     int64_t *tree_off = nullptr, *rc_off = nullptr;
-    uint32_t *parent_pos = nullptr, *pos_pre = nullptr, *pos_end = nullptr, *big_list = nullptr,
-             *small_list = nullptr, *rc_pu = nullptr, *rc_pv = nullptr, *rc_rank = nullptr, *cand_root = nullptr,
-             *cand_pu = nullptr, *cand_pv = nullptr, *root_comp = nullptr;
+    uint32_t *parent_pos = nullptr, *pos_pre = nullptr, *pos_end = nullptr, *rc_pu = nullptr, *rc_pv = nullptr,
+             *rc_rank = nullptr, *cand_root = nullptr, *cand_pu = nullptr, *cand_pv = nullptr, *root_comp = nullptr;
     int32_t *pos_coord = nullptr, *rc_coord = nullptr, *cand_coord = nullptr;
-    uint8_t *big_narrow = nullptr;
+    std::vector<TreePlan> plans;  // tree pass launch plans, ranges = problems
+    int32_t plan = 0;
     CompInfo *d_comp = nullptr;
     ChunkInfo *d_chunks = nullptr;
     MergeGroup *d_groups = nullptr;
@@ -3010,7 +3145,6 @@ struct mcb_batch {
     unsigned long long *upd_bytes = nullptr;
     int64_t coords = 0, sw_words = 0, blk_words = 0, store_words = 0, slots = 0;
     std::vector<void *> allocs;
-    int32_t q_blocks = 0, q_lds = 0, big_words = 0, small_words = 0;
     hipStream_t stream = nullptr;
     int64_t bytes = 0;
     int64_t blk_positions = 0, blk_candidates = 0;
@@ -3046,9 +3180,10 @@ BatchArgs batch_args(const mcb_batch *b, int32_t s) {
     a.pos_coord = b->pos_coord;
     a.pos_pre = b->pos_pre;
     a.pos_end = b->pos_end;
-    a.big_list = b->big_list;
-    a.small_list = b->small_list;
-    a.big_narrow = b->big_narrow;
+    const TreePlan &tp = b->plans[b->plan];
+    a.big_list = tp.big;
+    a.small_list = tp.small;
+    a.big_narrow = tp.narrow;
     a.rc_off = b->rc_off;
     a.rc_pu = b->rc_pu;
     a.rc_pv = b->rc_pv;
@@ -3093,10 +3228,10 @@ BatchArgs batch_args(const mcb_batch *b, int32_t s) {
     a.K = b->K;
     a.step = s;
     a.active = b->active[s];
-    a.n_big = b->big_cum[a.active];
-    a.n_small = b->small_cum[a.active];
-    a.big_words = b->big_words;
-    a.small_words = b->small_words;
+    a.n_big = tp.big_cum[a.active];
+    a.n_small = tp.small_cum[a.active];
+    a.big_words = tp.big_words;
+    a.small_words = tp.small_words;
     a.xor_wt = (s + 1 + kTile - 1) / kTile;
     return a;
 }
@@ -3112,7 +3247,8 @@ int launch_batch_step(mcb_batch *b, int32_t s) {
     hipLaunchKernelGGL(k_batch_transpose, dim3(static_cast<unsigned>((tw + 3) / 4)), dim3(256), 0, b->stream, a);
     HIP_TRY(hipGetLastError());
     if ((rc = timed_end(b)) || (rc = timed_begin(b, MCB_K_PARITY_SEARCH))) return rc;
-    hipLaunchKernelGGL(k_batch_q, dim3(b->q_blocks), dim3(kQWaves * kWave), b->q_lds, b->stream, a);
+    const TreePlan &tp = b->plans[b->plan];
+    hipLaunchKernelGGL(k_batch_q, dim3(tp.blocks), dim3(tp.nw * kWave), tp.lds, b->stream, a);
     HIP_TRY(hipGetLastError());
     if ((rc = timed_end(b)) || (rc = timed_begin(b, MCB_K_SELECT))) return rc;
     a.n_items = b->chunk_cum[A];
@@ -3158,6 +3294,7 @@ void mcb_batch_destroy(mcb_batch *b) {
     if (!b) return;
     for (void *p : b->allocs)
         if (p) (void)hipFree(p);
+    for (TreePlan &p : b->plans) free_plan(p);
     for (hipEvent_t e : b->ev_pool) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -3214,11 +3351,7 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
     b->slot_of.resize(G);
     for (int32_t g = 0; g < G; ++g) b->slot_of[b->order[g]] = g;
 
-    // Bases of every problem's ranges, and the tree classification (the threshold is the largest
-    // multiple of 64, at most 1024, whose one-wave slices fit the LDS the largest tree needs).
-    const int64_t largest_bytes = 8 * ((static_cast<int64_t>(largest) + kWave - 1) / kWave * kWave);
-    const int64_t small_max =
-        std::max<int64_t>(kWave, std::min<int64_t>(1024, largest_bytes / (8 * kQWaves) / kWave * kWave));
+    // Bases of every problem's ranges.
     int64_t F_tot = 0, P_tot = 0, K_tot = 0, rank_tot = 0, coords = 0, sw = 0, blk = 0, pw = 0, pwoff = 0, store = 0,
             slots = 0;
     b->comps.resize(G);
@@ -3284,14 +3417,9 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
     std::vector<uint32_t> parent_pos(P_tot), rc_pu(K_tot), rc_pv(K_tot), rc_rank(K_tot), cand_root(rank_tot, 0),
         cand_pu(rank_tot, 0), cand_pv(rank_tot, 0), root_comp(F_tot);
     std::vector<int32_t> pos_coord(P_tot), rc_coord(K_tot), cand_coord(rank_tot, -1);
-    std::vector<uint32_t> big, small;
-    std::vector<uint8_t> narrow;
-    b->big_cum.assign(G + 1, 0);
-    b->small_cum.assign(G + 1, 0);
     b->pos_cum.assign(G + 1, 0);
     b->cand_cum.assign(G + 1, 0);
-    int64_t max_big = 0, max_small = 1;
-    std::vector<uint32_t> roots, level;
+    std::vector<int64_t> ranges(G + 1, F_tot);
     for (int32_t g = 0; g < G; ++g) {
         const mcb_solver_desc &d = descs[b->order[g]];
         const CompInfo &c = b->comps[g];
@@ -3318,34 +3446,7 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
             rc_rank[KB + i] = static_cast<uint32_t>(rb + k);
             rc_coord[KB + i] = d.cand_coord[k] >= 0 ? static_cast<int32_t>(cb + d.cand_coord[k]) : -1;
         }
-        // this problem's trees, largest first, split into the two work lists
-        roots.resize(d.num_roots);
-        std::iota(roots.begin(), roots.end(), 0u);
-        auto npos = [&](uint32_t r) { return d.tree_offset[r + 1] - d.tree_offset[r]; };
-        std::stable_sort(roots.begin(), roots.end(), [&](uint32_t x, uint32_t y) { return npos(x) > npos(y); });
-        int32_t nb = 0, ns = 0;
-        for (uint32_t r : roots) {
-            const int64_t np = npos(r);
-            if (np > small_max) {
-                big.push_back(static_cast<uint32_t>(R + r));
-                max_big = std::max(max_big, np);
-                const int64_t a0 = d.tree_offset[r];
-                level.assign(static_cast<size_t>(np), 0);
-                uint32_t depth = 0;
-                for (int64_t x = 1; x < np; ++x) {
-                    level[x] = level[d.parent_pos[a0 + x]] + 1;
-                    depth = std::max(depth, level[x]);
-                }
-                narrow.push_back(np < 128ll * (depth + 1) ? 1 : 0);
-                ++nb;
-            } else {
-                small.push_back(static_cast<uint32_t>(R + r));
-                max_small = std::max(max_small, np);
-                ++ns;
-            }
-        }
-        b->big_cum[g + 1] = b->big_cum[g] + nb;
-        b->small_cum[g + 1] = b->small_cum[g] + ns;
+        ranges[g] = R;
         b->pos_cum[g + 1] = b->pos_cum[g] + d.num_positions;
         b->cand_cum[g + 1] = b->cand_cum[g] + d.num_candidates;
     }
@@ -3354,21 +3455,10 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
     std::vector<uint32_t> pre, end;
     preorder_intervals(tree_off.data(), parent_pos.data(), static_cast<int32_t>(F_tot), pre, end);
 
-    // LDS of the tree pass: the largest big tree's path words and ready stamps, or one slice per wave
-    const int64_t big_words = (max_big + kWave - 1) / kWave * kWave;
-    const int64_t small_words = (max_small + kWave - 1) / kWave * kWave;
-    const int64_t lds = std::max<int64_t>(8 * big_words + 4 * (big_words / kWave), 8 * kQWaves * small_words);
-    const int64_t lds_max = std::min<int64_t>(g_dev.prop.sharedMemPerBlock, 160 * 1024);
-    if (lds > lds_max) return cleanup(fail(MCB_E_INVALID, "a tree is too large for the blocked form's LDS budget"));
-    b->big_words = static_cast<int32_t>(big_words);
-    b->small_words = static_cast<int32_t>(small_words);
-    b->q_lds = static_cast<int32_t>(lds);
-    int per_cu = 0;
-    if (hipFuncSetAttribute(reinterpret_cast<const void *>(k_batch_q), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            b->q_lds) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_q, kQWaves * kWave, b->q_lds) != hipSuccess)
-        return cleanup(fail(MCB_E_DEVICE, "tree pass launch set-up"));
-    b->q_blocks = std::max(1, per_cu) * g_dev.prop.multiProcessorCount;
+    if ((rc = make_plans(b->opt.waves_per_block, tree_off.data(), parent_pos.data(), ranges,
+                         reinterpret_cast<const void *>(k_batch_q), b->plans, b->bytes)))
+        return cleanup(rc);
+    if (b->plans.empty()) return cleanup(fail(MCB_E_INVALID, "a tree is too large for the blocked form's LDS budget"));
 
     // Static launch tables: first-level chunks, merge groups per level, pairs workgroups.
     std::vector<ChunkInfo> chunks;
@@ -3435,9 +3525,7 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
     std::vector<CompInfo> comps = b->comps;
     if ((rc = bat_upload(b, &b->tree_off, tree_off)) || (rc = bat_upload(b, &b->parent_pos, parent_pos)) ||
         (rc = bat_upload(b, &b->pos_coord, pos_coord)) || (rc = bat_upload(b, &b->pos_pre, pre)) ||
-        (rc = bat_upload(b, &b->pos_end, end)) || (rc = bat_upload(b, &b->big_list, big)) ||
-        (rc = bat_upload(b, &b->small_list, small)) || (rc = bat_upload(b, &b->big_narrow, narrow)) ||
-        (rc = bat_upload(b, &b->rc_off, rc_off)) || (rc = bat_upload(b, &b->rc_pu, rc_pu)) ||
+        (rc = bat_upload(b, &b->pos_end, end)) || (rc = bat_upload(b, &b->rc_off, rc_off)) || (rc = bat_upload(b, &b->rc_pu, rc_pu)) ||
         (rc = bat_upload(b, &b->rc_pv, rc_pv)) || (rc = bat_upload(b, &b->rc_rank, rc_rank)) ||
         (rc = bat_upload(b, &b->rc_coord, rc_coord)) || (rc = bat_upload(b, &b->cand_root, cand_root)) ||
         (rc = bat_upload(b, &b->cand_pu, cand_pu)) || (rc = bat_upload(b, &b->cand_pv, cand_pv)) ||
@@ -3469,6 +3557,23 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
             if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(MCB_E_DEVICE, "hipEventCreate"));
     }
     if ((rc = mcb_batch_reset(b))) return cleanup(rc);
+    if (b->plans.size() > 1) {
+        // tree pass width: time every plan on step 0 (every problem, from the unit support vectors)
+        const BatchArgs a = batch_args(b, 0);
+        hipLaunchKernelGGL(k_batch_transpose, dim3(static_cast<unsigned>((a.active + 3) / 4)), dim3(256), 0, b->stream,
+                           a);
+        if (hipGetLastError() != hipSuccess) return cleanup(fail(MCB_E_DEVICE, "k_batch_transpose"));
+        rc = pick_fastest(
+            b->plans.size(), b->stream, [&](size_t i) { b->plan = static_cast<int32_t>(i); },
+            [&]() -> int {
+                const BatchArgs q = batch_args(b, 0);
+                const TreePlan &tp = b->plans[b->plan];
+                hipLaunchKernelGGL(k_batch_q, dim3(tp.blocks), dim3(tp.nw * kWave), tp.lds, b->stream, q);
+                HIP_TRY(hipGetLastError());
+                return MCB_OK;
+            });
+        if (rc || (rc = mcb_batch_reset(b))) return cleanup(rc);
+    }
     *out = b;
     return MCB_OK;
 }
@@ -3549,9 +3654,10 @@ int mcb_batch_work_counters(mcb_batch *b, int64_t *positions, int64_t *candidate
 int mcb_batch_launch_shape(const mcb_batch *b, int32_t *blocks, int32_t *waves_per_block, int32_t *lds_bytes,
                            int32_t *steps) {
     if (!b) return fail(MCB_E_INVALID, "invalid arguments");
-    if (blocks) *blocks = b->q_blocks;
-    if (waves_per_block) *waves_per_block = kQWaves;
-    if (lds_bytes) *lds_bytes = b->q_lds;
+    const TreePlan &tp = b->plans[b->plan];
+    if (blocks) *blocks = tp.blocks;
+    if (waves_per_block) *waves_per_block = tp.nw;
+    if (lds_bytes) *lds_bytes = tp.lds;
     if (steps) *steps = b->steps;
     return MCB_OK;
 }

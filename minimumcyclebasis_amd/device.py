@@ -62,7 +62,8 @@ class Solver:
     def __init__(self, problem: Problem, store_cycles: bool = False, timing: int | bool = False,
                  waves_per_block: int = 0, max_blocks: int = 0, block_phases: int = 0):
         """timing: 0/False none, 1/True every kernel, 2 only the dominant kernel (HIP events).
-        block_phases: phases per pass over the trees (0 automatic, 1 one kernel per phase, up to 64)."""
+        block_phases: phases per pass over the trees (0 automatic, 1 one kernel per phase, up to 64).
+        waves_per_block: waves per workgroup of the blocked tree pass (4, 8, 16; 0 timed at create)."""
         info = problem.info
         if info.single_cycle:
             raise ValueError("a single-cycle graph has no phase loop")
@@ -167,7 +168,7 @@ class Batch:
     resolves phases 64s .. 64s+63 of every problem that has them, one launch per kernel for all of
     them.  Each problem's selections equal its own ``Solver``'s."""
 
-    def __init__(self, problems, store_cycles: bool = False, timing: int | bool = False):
+    def __init__(self, problems, store_cycles: bool = False, timing: int | bool = False, waves_per_block: int = 0):
         problems = list(problems)
         if not problems:
             raise ValueError("no problems")
@@ -187,6 +188,7 @@ class Batch:
         _lib.hip().mcb_solver_options_init(ctypes.byref(o))
         o.store_cycles = int(bool(store_cycles))
         o.timing = int(timing) if isinstance(timing, int) and not isinstance(timing, bool) else int(bool(timing))
+        o.waves_per_block = int(waves_per_block)
         h = ctypes.c_void_p()
         _lib.check_hip(_lib.hip().mcb_batch_create(ctypes.cast(descs, ctypes.c_void_p), len(problems),
                                                    ctypes.byref(o), ctypes.byref(h)))

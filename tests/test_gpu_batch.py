@@ -133,3 +133,16 @@ def test_batched_components_driver():
         assert c1.index == c2.index
         assert np.array_equal(r1.selected, r2.selected)
         assert r2.stats.final_cycles == c2.cycle_space_dim
+
+
+@pytest.mark.parametrize("waves", [4, 8, 16])
+def test_batch_tree_pass_widths(waves):
+    probs = _component_problems(12, 20, 600, 3)
+    print(f"case batch widths={waves} problems={len(probs)}", flush=True)
+    bt = Batch(probs, store_cycles=True, waves_per_block=waves)
+    assert bt.launch_shape()["waves_per_block"] == waves
+    bt.run()
+    bt.sync()
+    for j, p in enumerate(probs):
+        _check_against_oracle(bt, j, p)
+    bt.close()
