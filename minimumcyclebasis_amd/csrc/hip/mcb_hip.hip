@@ -977,10 +977,14 @@ __device__ __forceinline__ void run_words(const A &a, const uint64_t *Wd, int64_
 }
 
 // Pass over every tree and every candidate of the block.  Big roots (more than small_words
-// positions) take a whole workgroup each, the others one wave each; both lists are handed out
-// round-robin in list order (largest first).  kk_of(r) is the number of phases in root r's block.
+// positions) take a whole workgroup each, the others one wave each.  This workgroup takes the big
+// trees big[first], big[first + stride], ... (narrow[i]: tree i of the list is narrow) and its waves
+// the small trees small[sfirst + wave], small[sfirst + wave + sstride], ...  kk_of(r) is the number of
+// phases in root r's block.
 template <typename A, typename KkFn>
-__device__ __forceinline__ void tree_pass(const A &a, uint32_t n_big, uint32_t n_small, KkFn kk_of) {
+__device__ __forceinline__ void tree_pass(const A &a, const uint32_t *big, const uint8_t *narrow, uint32_t n_big,
+                                          uint32_t first, uint32_t stride, const uint32_t *small, uint32_t n_small,
+                                          uint32_t sfirst, uint32_t sstride, KkFn kk_of) {
     extern __shared__ uint64_t lds[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = wave_id();
@@ -991,12 +995,12 @@ __device__ __forceinline__ void tree_pass(const A &a, uint32_t n_big, uint32_t n
     for (uint32_t i = threadIdx.x; i < nwin_big; i += blockDim.x) ready[i] = 0;
     __syncthreads();
     uint32_t gen = 0;
-    for (uint32_t it = blockIdx.x; it < n_big; it += gridDim.x) {
-        const uint32_t r = uniform32(a.big_list[it]);
+    for (uint32_t it = first; it < n_big; it += stride) {
+        const uint32_t r = uniform32(big[it]);
         const int64_t off = uniform64(a.tree_off[r]);
         const uint32_t npos = static_cast<uint32_t>(uniform64(a.tree_off[r + 1]) - off);
         ++gen;
-        if (a.big_narrow[it]) {
+        if (narrow[it]) {
             // narrow, deep tree: windows mostly wait on the window before them, so one wave walks it
             // alone (no handoffs between waves); the others wait at the barrier
             if (wid == 0) tree_words(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wbig, lane);
@@ -1013,8 +1017,8 @@ __device__ __forceinline__ void tree_pass(const A &a, uint32_t n_big, uint32_t n
     }
     // the big-tree region is free from here on; each wave takes its own slice for small roots
     uint64_t *Ws = lds + static_cast<size_t>(wid) * a.small_words;
-    for (uint32_t it = blockIdx.x * nw + wid; it < n_small; it += gridDim.x * nw) {
-        const uint32_t r = uniform32(a.small_list[it]);
+    for (uint32_t it = sfirst + wid; it < n_small; it += sstride) {
+        const uint32_t r = uniform32(small[it]);
         const int64_t off = uniform64(a.tree_off[r]);
         const uint32_t npos = static_cast<uint32_t>(uniform64(a.tree_off[r + 1]) - off);
         tree_words(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Ws, lane);
@@ -1023,7 +1027,9 @@ __device__ __forceinline__ void tree_pass(const A &a, uint32_t n_big, uint32_t n
 }
 
 __global__ __launch_bounds__(kQWaves * kWave) void k_block_q(BlockArgs a) {
-    tree_pass(a, static_cast<uint32_t>(a.n_big), static_cast<uint32_t>(a.n_small), [&](uint32_t) { return a.kk; });
+    const uint32_t nw = blockDim.x / kWave;
+    tree_pass(a, a.big_list, a.big_narrow, static_cast<uint32_t>(a.n_big), blockIdx.x, gridDim.x, a.small_list,
+              static_cast<uint32_t>(a.n_small), blockIdx.x * nw, gridDim.x * nw, [&](uint32_t) { return a.kk; });
 }
 
 // Inserts one batch (lane i: word q, ranks ascending with the lane; q = 0 for no entry) into a
@@ -1673,6 +1679,8 @@ __global__ __launch_bounds__(256) void k_block_xor(BlockArgs a) {
 // vectors, so running them side by side changes no problem's result.
 // ---------------------------------------------------------------------------------------------
 
+constexpr int kXcds = 8;  // accelerator complex dies of an MI355X; workgroups are dealt to them round-robin
+
 struct CompInfo {
     int64_t cb;     // first coordinate (Tt, Ct, Bc, sel)
     int64_t rb;     // first rank (Qr, candidate bitmap, candidate arrays)
@@ -1722,7 +1730,9 @@ struct BatchArgs {
     uint32_t *status;
     unsigned long long *upd_bytes;
     int64_t K;  // rank bound
-    int32_t step, active, n_items, n_big, n_small, big_words, small_words, xor_wt;
+    int32_t step, active, n_items, big_words, small_words, xor_wt;
+    int32_t x_big_off[kXcds], x_big_n[kXcds];      // per XCD section of the big-tree list: start, running entries
+    int32_t x_small_off[kXcds], x_small_n[kXcds];  // the same for the small-tree list
 };
 
 __device__ __forceinline__ int32_t batch_kk(int32_t N, int32_t step) {
@@ -1784,11 +1794,18 @@ __global__ __launch_bounds__(256) void k_batch_transpose(BatchArgs a) {
     transpose_body(batch_view(a, g), x - g * wl, lane);
 }
 
+// Workgroups run on the XCD numbered blockIdx.x mod kXcds (round-robin dispatch); problem g's trees
+// are listed in section g mod kXcds, so that each problem's Tt words are gathered through one XCD's L2.
 __global__ __launch_bounds__(kQWaves * kWave) void k_batch_q(BatchArgs a) {
-    tree_pass(a, static_cast<uint32_t>(a.n_big), static_cast<uint32_t>(a.n_small), [&](uint32_t r) {
-        const uint32_t g = uniform32(a.root_comp[r]);
-        return batch_kk(static_cast<int32_t>(uniform32(static_cast<uint32_t>(a.comp[g].N))), a.step);
-    });
+    const uint32_t nw = blockDim.x / kWave;
+    // blocks x, x + kXcds, ... serve section x (the grid is a multiple of kXcds)
+    const uint32_t x = blockIdx.x % kXcds, local = blockIdx.x / kXcds, nloc = gridDim.x / kXcds;
+    const uint32_t bo = static_cast<uint32_t>(a.x_big_off[x]), so = static_cast<uint32_t>(a.x_small_off[x]);
+    tree_pass(a, a.big_list + bo, a.big_narrow + bo, static_cast<uint32_t>(a.x_big_n[x]), local, nloc,
+              a.small_list + so, static_cast<uint32_t>(a.x_small_n[x]), local * nw, nloc * nw, [&](uint32_t r) {
+                  const uint32_t g = uniform32(a.root_comp[r]);
+                  return batch_kk(static_cast<int32_t>(uniform32(static_cast<uint32_t>(a.comp[g].N))), a.step);
+              });
 }
 
 __device__ __forceinline__ void batch_finalize(const BatchArgs &a, int g, int lane) {
@@ -2201,12 +2218,14 @@ namespace {
 // positions are shared by the whole workgroup, the others take one wave; small_max is the largest
 // multiple of 64 (at most 1024) whose nw one-wave slices fit the LDS the largest tree needs anyway.
 // The roots come in ranges (one per problem of a batch, one for a single problem), each range's trees
-// largest first; big_cum/small_cum count the list entries before each range.
+// largest first.  Range g is listed in section g mod nsplit (sections one after the other);
+// big_cum[x][g] / small_cum[x][g] count section x's entries from ranges before g.
 struct TreePlan {
     int32_t nw = 0, big_words = 0, small_words = 0, lds = 0, blocks = 0, n_big = 0, n_small = 0;
     uint32_t *big = nullptr, *small = nullptr;
     uint8_t *narrow = nullptr;  // per big tree: 1 if its levels average fewer than 128 positions
-    std::vector<int32_t> big_cum, small_cum;
+    std::vector<int32_t> big_off, small_off;  // first entry of each section
+    std::vector<std::vector<int32_t>> big_cum, small_cum;
 };
 
 void free_plan(TreePlan &p) {
@@ -2565,36 +2584,51 @@ std::vector<uint32_t> tree_depths(const int64_t *tree_off, const uint32_t *paren
 }
 
 // Builds the plan for nw waves per workgroup; MCB_E_INVALID when its LDS does not fit one workgroup.
-int plan_tree_pass(int nw, const int64_t *tree_off, const std::vector<uint32_t> &depth,
+int plan_tree_pass(int nw, int nsplit, const int64_t *tree_off, const std::vector<uint32_t> &depth,
                    const std::vector<int64_t> &ranges, const void *kernel, TreePlan &p, int64_t &bytes) {
     const size_t G = ranges.size() - 1;
     int64_t largest = 0;
     for (int64_t r = 0; r < ranges.back(); ++r) largest = std::max<int64_t>(largest, tree_off[r + 1] - tree_off[r]);
     const int64_t largest_words = (largest + kWave - 1) / kWave * kWave;
     const int64_t small_max = std::max<int64_t>(kWave, std::min<int64_t>(1024, largest_words / nw / kWave * kWave));
-    std::vector<uint32_t> big, small, order;
-    std::vector<uint8_t> narrow;
-    p.big_cum.assign(G + 1, 0);
-    p.small_cum.assign(G + 1, 0);
+    std::vector<std::vector<uint32_t>> big(nsplit), small(nsplit);
+    std::vector<std::vector<uint8_t>> narrow(nsplit);
+    std::vector<uint32_t> order;
+    p.big_cum.assign(nsplit, std::vector<int32_t>(G + 1, 0));
+    p.small_cum.assign(nsplit, std::vector<int32_t>(G + 1, 0));
     int64_t max_big = 0, max_small = 1;
     for (size_t g = 0; g < G; ++g) {
+        const int x = static_cast<int>(g % nsplit);
         order.resize(static_cast<size_t>(ranges[g + 1] - ranges[g]));
         std::iota(order.begin(), order.end(), static_cast<uint32_t>(ranges[g]));
         auto npos = [&](uint32_t r) { return tree_off[r + 1] - tree_off[r]; };
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return npos(x) > npos(y); });
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return npos(a) > npos(b); });
         for (uint32_t r : order) {
             const int64_t np = npos(r);
             if (np > small_max) {
-                big.push_back(r);
-                narrow.push_back(np < 128ll * (depth[r] + 1) ? 1 : 0);
+                big[x].push_back(r);
+                narrow[x].push_back(np < 128ll * (depth[r] + 1) ? 1 : 0);
                 max_big = std::max(max_big, np);
             } else {
-                small.push_back(r);
+                small[x].push_back(r);
                 max_small = std::max(max_small, np);
             }
         }
-        p.big_cum[g + 1] = static_cast<int32_t>(big.size());
-        p.small_cum[g + 1] = static_cast<int32_t>(small.size());
+        for (int y = 0; y < nsplit; ++y) {
+            p.big_cum[y][g + 1] = static_cast<int32_t>(big[y].size());
+            p.small_cum[y][g + 1] = static_cast<int32_t>(small[y].size());
+        }
+    }
+    std::vector<uint32_t> big_all, small_all;
+    std::vector<uint8_t> narrow_all;
+    p.big_off.assign(nsplit, 0);
+    p.small_off.assign(nsplit, 0);
+    for (int x = 0; x < nsplit; ++x) {
+        p.big_off[x] = static_cast<int32_t>(big_all.size());
+        p.small_off[x] = static_cast<int32_t>(small_all.size());
+        big_all.insert(big_all.end(), big[x].begin(), big[x].end());
+        narrow_all.insert(narrow_all.end(), narrow[x].begin(), narrow[x].end());
+        small_all.insert(small_all.end(), small[x].begin(), small[x].end());
     }
     const int64_t big_words = (max_big + kWave - 1) / kWave * kWave;
     const int64_t small_words = (max_small + kWave - 1) / kWave * kWave;
@@ -2605,29 +2639,30 @@ int plan_tree_pass(int nw, const int64_t *tree_off, const std::vector<uint32_t> 
     p.big_words = static_cast<int32_t>(big_words);
     p.small_words = static_cast<int32_t>(small_words);
     p.lds = static_cast<int32_t>(lds);
-    p.n_big = static_cast<int32_t>(big.size());
-    p.n_small = static_cast<int32_t>(small.size());
+    p.n_big = static_cast<int32_t>(big_all.size());
+    p.n_small = static_cast<int32_t>(small_all.size());
     HIP_TRY(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_max)));
     int per_cu = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, nw * kWave, p.lds));
     p.blocks = std::max(1, per_cu) * g_dev.prop.multiProcessorCount;
     int rc;
-    if ((rc = upload(&p.big, big.data(), big.size(), bytes)) || (rc = upload(&p.small, small.data(), small.size(), bytes)) ||
-        (rc = upload(&p.narrow, narrow.data(), narrow.size(), bytes)))
+    if ((rc = upload(&p.big, big_all.data(), big_all.size(), bytes)) ||
+        (rc = upload(&p.small, small_all.data(), small_all.size(), bytes)) ||
+        (rc = upload(&p.narrow, narrow_all.data(), narrow_all.size(), bytes)))
         return rc;
     return MCB_OK;
 }
 
 // The plans worth trying: the requested width, or 16, 8 and 4 waves (those whose LDS fits).
-int make_plans(int want, const int64_t *tree_off, const uint32_t *parent_pos, const std::vector<int64_t> &ranges,
-               const void *kernel, std::vector<TreePlan> &plans, int64_t &bytes) {
+int make_plans(int want, int nsplit, const int64_t *tree_off, const uint32_t *parent_pos,
+               const std::vector<int64_t> &ranges, const void *kernel, std::vector<TreePlan> &plans, int64_t &bytes) {
     if (want != 0 && want != 4 && want != 8 && want != 16)
         return fail(MCB_E_INVALID, "waves_per_block must be 0, 4, 8 or 16");
     const std::vector<uint32_t> depth = tree_depths(tree_off, parent_pos, ranges.back());
     for (int nw : {16, 8, 4}) {
         if (want && nw != want) continue;
         TreePlan p;
-        const int rc = plan_tree_pass(nw, tree_off, depth, ranges, kernel, p, bytes);
+        const int rc = plan_tree_pass(nw, nsplit, tree_off, depth, ranges, kernel, p, bytes);
         if (rc == MCB_OK) plans.push_back(std::move(p));
         else {
             free_plan(p);
@@ -2697,7 +2732,7 @@ int setup_blocked(mcb_solver *s, const int64_t *tree_offset, const uint32_t *par
     if (want < 0 || want > kBlockK) return fail(MCB_E_INVALID, "block_phases must be 0..64");
     if (want == 1) return MCB_OK;
     const std::vector<int64_t> ranges{0, static_cast<int64_t>(s->F)};
-    int rc = make_plans(s->opt.waves_per_block, tree_offset, parent_pos, ranges,
+    int rc = make_plans(s->opt.waves_per_block, 1, tree_offset, parent_pos, ranges,
                         reinterpret_cast<const void *>(k_block_q), s->plans, s->bytes);
     if (rc) return rc;
     if (s->plans.empty()) {
@@ -3173,6 +3208,12 @@ int bat_upload(mcb_batch *b, T **p, const std::vector<T> &v) {
     return rc;
 }
 
+// Grid of k_batch_q: the plan's resident blocks rounded up to a multiple of kXcds, so that every
+// XCD section has the same number of blocks and at least one.
+unsigned batch_q_grid(const TreePlan &tp) {
+    return static_cast<unsigned>((tp.blocks + kXcds - 1) / kXcds * kXcds);
+}
+
 BatchArgs batch_args(const mcb_batch *b, int32_t s) {
     BatchArgs a{};
     a.tree_off = b->tree_off;
@@ -3228,8 +3269,12 @@ BatchArgs batch_args(const mcb_batch *b, int32_t s) {
     a.K = b->K;
     a.step = s;
     a.active = b->active[s];
-    a.n_big = tp.big_cum[a.active];
-    a.n_small = tp.small_cum[a.active];
+    for (int x = 0; x < kXcds; ++x) {
+        a.x_big_off[x] = tp.big_off[x];
+        a.x_big_n[x] = tp.big_cum[x][a.active];
+        a.x_small_off[x] = tp.small_off[x];
+        a.x_small_n[x] = tp.small_cum[x][a.active];
+    }
     a.big_words = tp.big_words;
     a.small_words = tp.small_words;
     a.xor_wt = (s + 1 + kTile - 1) / kTile;
@@ -3248,7 +3293,7 @@ int launch_batch_step(mcb_batch *b, int32_t s) {
     HIP_TRY(hipGetLastError());
     if ((rc = timed_end(b)) || (rc = timed_begin(b, MCB_K_PARITY_SEARCH))) return rc;
     const TreePlan &tp = b->plans[b->plan];
-    hipLaunchKernelGGL(k_batch_q, dim3(tp.blocks), dim3(tp.nw * kWave), tp.lds, b->stream, a);
+    hipLaunchKernelGGL(k_batch_q, dim3(batch_q_grid(tp)), dim3(tp.nw * kWave), tp.lds, b->stream, a);
     HIP_TRY(hipGetLastError());
     if ((rc = timed_end(b)) || (rc = timed_begin(b, MCB_K_SELECT))) return rc;
     a.n_items = b->chunk_cum[A];
@@ -3455,7 +3500,7 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
     std::vector<uint32_t> pre, end;
     preorder_intervals(tree_off.data(), parent_pos.data(), static_cast<int32_t>(F_tot), pre, end);
 
-    if ((rc = make_plans(b->opt.waves_per_block, tree_off.data(), parent_pos.data(), ranges,
+    if ((rc = make_plans(b->opt.waves_per_block, kXcds, tree_off.data(), parent_pos.data(), ranges,
                          reinterpret_cast<const void *>(k_batch_q), b->plans, b->bytes)))
         return cleanup(rc);
     if (b->plans.empty()) return cleanup(fail(MCB_E_INVALID, "a tree is too large for the blocked form's LDS budget"));
@@ -3568,7 +3613,7 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
             [&]() -> int {
                 const BatchArgs q = batch_args(b, 0);
                 const TreePlan &tp = b->plans[b->plan];
-                hipLaunchKernelGGL(k_batch_q, dim3(tp.blocks), dim3(tp.nw * kWave), tp.lds, b->stream, q);
+                hipLaunchKernelGGL(k_batch_q, dim3(batch_q_grid(tp)), dim3(tp.nw * kWave), tp.lds, b->stream, q);
                 HIP_TRY(hipGetLastError());
                 return MCB_OK;
             });
@@ -3655,7 +3700,7 @@ int mcb_batch_launch_shape(const mcb_batch *b, int32_t *blocks, int32_t *waves_p
                            int32_t *steps) {
     if (!b) return fail(MCB_E_INVALID, "invalid arguments");
     const TreePlan &tp = b->plans[b->plan];
-    if (blocks) *blocks = tp.blocks;
+    if (blocks) *blocks = static_cast<int32_t>(batch_q_grid(tp));
     if (waves_per_block) *waves_per_block = tp.nw;
     if (lds_bytes) *lds_bytes = tp.lds;
     if (steps) *steps = b->steps;
