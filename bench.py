@@ -105,6 +105,9 @@ def main() -> int:
     ap.add_argument("--workload", default="er20000", choices=sorted(WORKLOADS) + sorted(COMPONENT_WORKLOADS))
     ap.add_argument("--no-plain", action="store_true",
                     help="component workloads: skip the plain one-solver-per-component timing")
+    ap.add_argument("--fvs", choices=["plain", "learned"], default="plain",
+                    help="single graphs: FVS construction (learned: the MLP-ordered greedy; the plain-FVS loop "
+                         "is then timed beside it and the weights compared)")
     ap.add_argument("--no-prune", action="store_true")
     ap.add_argument("--breakdown", action="store_true",
                     help="time every kernel (HIP events between all launches), not only the dominant one")
@@ -137,7 +140,16 @@ def main() -> int:
     n, u, v, w = gen()
     threads = host_cores()
     t0 = time.perf_counter()
-    prob = Problem.from_edges(n, u, v, w, prune_trees=not args.no_prune, num_threads=threads)
+    fvs_info = None
+    order = None
+    if args.fvs == "learned":
+        from minimumcyclebasis_amd.learned_fvs import learned_order
+
+        ch = learned_order(n, u, v, w)
+        order = ch.order
+        fvs_info = {"fvs_reference": ch.fvs_reference, "fvs_learned": ch.fvs_learned, "learned_used": ch.used,
+                    "learned_order_s": round(time.perf_counter() - t0, 3)}
+    prob = Problem.from_edges(n, u, v, w, prune_trees=not args.no_prune, num_threads=threads, fvs_order=order)
     host_s = time.perf_counter() - t0
     info = prob.info
     print(f"[bench] {args.workload}: n'={info.nodes} m'={info.directed_edges // 2} N={info.num_non_tree} "
@@ -247,6 +259,31 @@ def main() -> int:
         },
     }
     solver.close()
+    if fvs_info is not None:
+        # the same loop with the reference FVS order, for the comparison and the weight check
+        t0 = time.perf_counter()
+        plain = Problem.from_edges(n, u, v, w, prune_trees=not args.no_prune, num_threads=threads)
+        fvs_info["plain_host_preprocess_s"] = round(time.perf_counter() - t0, 3)
+        ps = Solver(plain, block_phases=args.block_phases)
+        ps.run()
+        ps.sync()
+        ps.reset()
+        t0 = time.perf_counter()
+        ps.run()
+        ps.sync()
+        fvs_info["plain_loop_s"] = round(time.perf_counter() - t0, 6)
+        plain_weight = int(plain.array("cand_weight")[ps.selected()].astype(np.int64).sum())
+        ps.close()
+        fvs_info["plain_candidates"] = plain.info.num_candidates
+        fvs_info["plain_tree_positions"] = plain.info.positions
+        fvs_info["weights_equal"] = plain_weight == total_weight
+        fvs_info["speedup_vs_plain"] = round(fvs_info["plain_loop_s"] / (ms_per_step / 1000.0), 3)
+        plain.close()
+        out["config"]["form"] = "learned-fvs"
+        out["learned_fvs"] = fvs_info
+        if not fvs_info["weights_equal"]:
+            print("[bench] learned and plain FVS give different MCB weights", file=sys.stderr, flush=True)
+            return 1
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, u, v, w, args.cpu_budget, threads)
     print(json.dumps(out), flush=True)

@@ -50,6 +50,8 @@ typedef struct {
     const int32_t *fvs_order; /* optional: vertex ids of the reduced graph in the order the reverse-delete
                                pass tries them (NULL: the reference order, degree asc then id desc) */
     int fvs_order_len;
+    int stop_after_fvs;     /* 1: build only the reduced multigraph and its FVS (the arrays up to
+                               MCB_A_ROOTS; info through fvs_size and num_non_tree) */
 } mcb_build_options;
 
 void mcb_build_options_init(mcb_build_options *opt);
@@ -129,6 +131,26 @@ int mcb_fvs_reverse_delete(int32_t nodes, int32_t directed_edges, const uint32_t
  * as FVS::test_fvs (FVS.cpp:145-164); 0 if it is a forest. */
 int mcb_induced_has_cycle(int32_t nodes, const uint32_t *row_offsets, const uint32_t *cols,
                           const uint8_t *in_set);
+
+/* Learned-FVS inputs (no counterpart in the reference; they feed the vertex scorer whose picks are
+ * passed as mcb_build_options.fvs_order).  Per vertex of the multigraph induced by the vertices with
+ * alive[v] != 0 (alive NULL: all), 4 floats: log(1 + degree), log(1 + sum of the neighbours' degrees),
+ * core number / largest core number, and mean incident weight / mean weight over all edge ends; zeros
+ * for the other vertices.  Parallel edges count with multiplicity. */
+int mcb_vertex_features(int32_t nodes, const uint32_t *row_offsets, const uint32_t *cols, const int32_t *weights,
+                        const uint8_t *alive, float *out);
+
+/* Clears alive[v] for every vertex left with at most one edge to alive vertices, transitively (such
+ * vertices lie on no cycle of the alive subgraph).  Returns the number still alive or <0. */
+int mcb_fvs_peel(int32_t nodes, const uint32_t *row_offsets, const uint32_t *cols, uint8_t *alive);
+
+/* A smaller FVS than the reference order's, used as the training target of the vertex scorer: vertices
+ * left with at most one edge are peeled away, then the vertex of largest remaining degree (lowest id
+ * on ties) is taken and peeled, until nothing is left; the picks are then made minimal by the
+ * reverse-delete pass (the other vertices first, then the picks last to first).  Writes rank per vertex
+ * or -1 like mcb_fvs_reverse_delete; returns F or <0. */
+int mcb_fvs_greedy(int32_t nodes, int32_t directed_edges, const uint32_t *row_offsets, const uint32_t *cols,
+                   int32_t *fvs_rank);
 
 /* Biconnected components of a simple undirected graph (edge list, 0-based).  Writes, per edge, its
  * component id (components numbered in order of completion); returns the count or <0. */

@@ -81,6 +81,15 @@ SIGNATURES = {
     "mcb_batch_launch_shape": (ctypes.c_int, [P, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32),
                                               ctypes.POINTER(i32)]),
     "mcb_batch_destroy": (None, [P]),
+    "mcb_mlp_create": (ctypes.c_int, [ctypes.POINTER(P)]),
+    "mcb_mlp_set_params": (ctypes.c_int, [P, P]),
+    "mcb_mlp_get_params": (ctypes.c_int, [P, P]),
+    "mcb_mlp_forward": (ctypes.c_int, [P, P, i64, P]),
+    "mcb_mlp_gradient": (ctypes.c_int, [P, P, P, i64, P, ctypes.POINTER(ctypes.c_float)]),
+    "mcb_mlp_set_data": (ctypes.c_int, [P, P, P, i64]),
+    "mcb_mlp_train_epoch": (ctypes.c_int, [P, P, i64, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
+    "mcb_mlp_destroy": (None, [P]),
+    "mcb_mlp_last_error": (ctypes.c_char_p, []),
     "mcb_hip_device_count": (ctypes.c_int, []),
     "mcb_hip_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                            ctypes.POINTER(i64)]),

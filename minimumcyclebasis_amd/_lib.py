@@ -51,6 +51,7 @@ class BuildOptions(ctypes.Structure):
         ("num_threads", ctypes.c_int),
         ("fvs_order", ctypes.POINTER(ctypes.c_int32)),
         ("fvs_order_len", ctypes.c_int),
+        ("stop_after_fvs", ctypes.c_int),
     ]
 
 
@@ -103,6 +104,9 @@ def host() -> ctypes.CDLL:
         lib.mcb_fvs_reverse_delete.argtypes = [ctypes.c_int32, ctypes.c_int32, P, P, P, P]
         lib.mcb_induced_has_cycle.argtypes = [ctypes.c_int32, P, P, P]
         lib.mcb_biconnected_components.argtypes = [ctypes.c_int32, ctypes.c_int32, P, P, P]
+        lib.mcb_vertex_features.argtypes = [ctypes.c_int32, P, P, P, P, P]
+        lib.mcb_fvs_peel.argtypes = [ctypes.c_int32, P, P, P]
+        lib.mcb_fvs_greedy.argtypes = [ctypes.c_int32, ctypes.c_int32, P, P, P]
         _host = lib
     return _host
 

@@ -682,8 +682,12 @@ struct BlockArgs {
     const uint32_t *parent_pos;
     const int32_t *pos_coord;
     const uint32_t *pos_pre, *pos_end;
-    const uint32_t *big_list, *small_list;
-    const uint8_t *big_narrow;  // per big tree: 1 if its levels average fewer than 128 positions
+    const uint32_t *big_list, *small_list;  // item ids
+    const uint8_t *big_narrow;  // per big item: 1 if its tree's levels average fewer than 128 positions
+    const uint32_t *item_root;  // per item: root, and its candidate run [item_cb, item_ce) of the rc_* arrays
+    const int64_t *item_cb, *item_ce;
+    uint64_t *scratch;          // [blocks][huge_words] path words of trees too big for LDS
+    int64_t huge_words;
     const int64_t *rc_off;
     const uint32_t *rc_pu, *rc_pv, *rc_rank;
     const int32_t *rc_coord;
@@ -976,11 +980,14 @@ __device__ __forceinline__ void run_words(const A &a, const uint64_t *Wd, int64_
     }
 }
 
-// Pass over every tree and every candidate of the block.  Big roots (more than small_words
-// positions) take a whole workgroup each, the others one wave each.  This workgroup takes the big
-// trees big[first], big[first + stride], ... (narrow[i]: tree i of the list is narrow) and its waves
-// the small trees small[sfirst + wave], small[sfirst + wave + sstride], ...  kk_of(r) is the number of
-// phases in root r's block.
+// Pass over every tree and every candidate of the block.  The work lists hold items (a root and a
+// rank-ascending run of its candidates, item_root / item_cb / item_ce): big trees (more than
+// small_words positions) take a whole workgroup per item, the others one wave per item.  This
+// workgroup takes the big items big[first], big[first + stride], ... (narrow[i]: item i's tree is
+// narrow) and its waves the small items small[sfirst + wave], small[sfirst + wave + sstride], ...
+// A big tree of more than big_words positions keeps its path words in this workgroup's region of
+// scratch (huge_words words) instead of LDS; its ready stamps stay in LDS and are cleared first, so
+// that no stale word can pass for a stamp.  kk_of(r) is the number of phases in root r's block.
 template <typename A, typename KkFn>
 __device__ __forceinline__ void tree_pass(const A &a, const uint32_t *big, const uint8_t *narrow, uint32_t n_big,
                                           uint32_t first, uint32_t stride, const uint32_t *small, uint32_t n_small,
@@ -996,33 +1003,47 @@ __device__ __forceinline__ void tree_pass(const A &a, const uint32_t *big, const
     __syncthreads();
     uint32_t gen = 0;
     for (uint32_t it = first; it < n_big; it += stride) {
-        const uint32_t r = uniform32(big[it]);
+        const uint32_t item = uniform32(big[it]);
+        const uint32_t r = uniform32(a.item_root[item]);
         const int64_t off = uniform64(a.tree_off[r]);
         const uint32_t npos = static_cast<uint32_t>(uniform64(a.tree_off[r + 1]) - off);
         ++gen;
-        if (narrow[it]) {
-            // narrow, deep tree: windows mostly wait on the window before them, so one wave walks it
-            // alone (no handoffs between waves); the others wait at the barrier
-            if (wid == 0) tree_words(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wbig, lane);
+        const bool nar = narrow[it] != 0;
+        const int64_t cb = uniform64(a.item_cb[item]), ce = uniform64(a.item_ce[item]);
+        // path words, then each wave takes one contiguous run of the item's candidates (the two calls
+        // keep the LDS and the scratch accesses in their own address spaces)
+        auto one_tree = [&](uint64_t *Wd, uint32_t *rd) {
+            if (nar) {
+                // narrow, deep tree: windows mostly wait on the window before them, so one wave walks
+                // it alone (no handoffs between waves); the others wait at the barrier
+                if (wid == 0) tree_words(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wd, lane);
+            } else {
+                tree_words_coop(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wd, rd, gen, wid, nw, lane);
+            }
+            __syncthreads();
+            const int64_t seg = (ce - cb + nw - 1) / nw;
+            const int64_t s0 = cb + seg * wid, s1 = s0 + seg < ce ? s0 + seg : ce;
+            if (s0 < s1) run_words(a, Wd, s0, s1, kk_of(r), lane);
+            __syncthreads();
+        };
+        if (npos <= static_cast<uint32_t>(a.big_words)) {
+            one_tree(Wbig, ready);
         } else {
-            tree_words_coop(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wbig, ready, gen, wid, nw, lane);
+            uint32_t *rd = reinterpret_cast<uint32_t *>(lds);
+            for (uint32_t i = threadIdx.x; i < (npos + kWave - 1) / kWave; i += blockDim.x) rd[i] = 0;
+            __syncthreads();
+            one_tree(a.scratch + static_cast<int64_t>(blockIdx.x) * a.huge_words, rd);
         }
-        __syncthreads();
-        // each wave takes one contiguous run of the root's candidates
-        const int64_t cb = uniform64(a.rc_off[r]), ce = uniform64(a.rc_off[r + 1]);
-        const int64_t seg = (ce - cb + nw - 1) / nw;
-        const int64_t s0 = cb + seg * wid, s1 = s0 + seg < ce ? s0 + seg : ce;
-        if (s0 < s1) run_words(a, Wbig, s0, s1, kk_of(r), lane);
-        __syncthreads();
     }
     // the big-tree region is free from here on; each wave takes its own slice for small roots
     uint64_t *Ws = lds + static_cast<size_t>(wid) * a.small_words;
     for (uint32_t it = sfirst + wid; it < n_small; it += sstride) {
-        const uint32_t r = uniform32(small[it]);
+        const uint32_t item = uniform32(small[it]);
+        const uint32_t r = uniform32(a.item_root[item]);
         const int64_t off = uniform64(a.tree_off[r]);
         const uint32_t npos = static_cast<uint32_t>(uniform64(a.tree_off[r + 1]) - off);
         tree_words(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Ws, lane);
-        run_words(a, Ws, uniform64(a.rc_off[r]), uniform64(a.rc_off[r + 1]), kk_of(r), lane);
+        run_words(a, Ws, uniform64(a.item_cb[item]), uniform64(a.item_ce[item]), kk_of(r), lane);
     }
 }
 
@@ -1709,8 +1730,12 @@ struct BatchArgs {
     const uint32_t *parent_pos;
     const int32_t *pos_coord;  // shifted by the problem's cb
     const uint32_t *pos_pre, *pos_end;
-    const uint32_t *big_list, *small_list;  // problem-major, each problem's trees largest first
+    const uint32_t *big_list, *small_list;  // item ids, problem-major, each problem's trees largest first
     const uint8_t *big_narrow;
+    const uint32_t *item_root;
+    const int64_t *item_cb, *item_ce;
+    uint64_t *scratch;
+    int64_t huge_words;
     const int64_t *rc_off;
     const uint32_t *rc_pu, *rc_pv, *rc_rank;  // ranks shifted by rb
     const int32_t *rc_coord;
@@ -2220,20 +2245,34 @@ namespace {
 // The roots come in ranges (one per problem of a batch, one for a single problem), each range's trees
 // largest first.  Range g is listed in section g mod nsplit (sections one after the other);
 // big_cum[x][g] / small_cum[x][g] count section x's entries from ranges before g.
+// Entries are items: a root and a run of its candidates.  A big tree whose candidates exceed both
+// kItemCands and the plan's mean candidates per workgroup is listed once per run of that size, so that
+// a root holding a large share of all candidates spreads over several workgroups (each recomputes the
+// path words).  Big trees whose path words do not fit the LDS next to their ready stamps keep them in
+// scratch, huge_words words per workgroup.
+constexpr int64_t kItemCands = 8192;
+
 struct TreePlan {
     int32_t nw = 0, big_words = 0, small_words = 0, lds = 0, blocks = 0, n_big = 0, n_small = 0;
+    int64_t huge_words = 0;
     uint32_t *big = nullptr, *small = nullptr;
-    uint8_t *narrow = nullptr;  // per big tree: 1 if its levels average fewer than 128 positions
+    uint8_t *narrow = nullptr;  // per big item: 1 if its tree's levels average fewer than 128 positions
+    uint32_t *item_root = nullptr;
+    int64_t *item_cb = nullptr, *item_ce = nullptr;
+    uint64_t *scratch = nullptr;
     std::vector<int32_t> big_off, small_off;  // first entry of each section
     std::vector<std::vector<int32_t>> big_cum, small_cum;
+    std::vector<int64_t> pos_cum, cand_cum;   // tree positions and candidates streamed, ranges before g
 };
 
 void free_plan(TreePlan &p) {
-    if (p.big) (void)hipFree(p.big);
-    if (p.small) (void)hipFree(p.small);
-    if (p.narrow) (void)hipFree(p.narrow);
-    p.big = p.small = nullptr;
+    void *ptrs[] = {p.big, p.small, p.narrow, p.item_root, p.item_cb, p.item_ce, p.scratch};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    p.big = p.small = p.item_root = nullptr;
     p.narrow = nullptr;
+    p.item_cb = p.item_ce = nullptr;
+    p.scratch = nullptr;
 }
 
 }  // namespace
@@ -2388,6 +2427,14 @@ BlockArgs block_args(const mcb_solver *s, int32_t e, int32_t kk) {
     a.pos_coord = s->pos_coord;
     a.pos_pre = s->pos_pre;
     a.pos_end = s->pos_end;
+    if (!s->plans.empty()) {
+        const TreePlan &tp = s->plans[s->plan];
+        a.item_root = tp.item_root;
+        a.item_cb = tp.item_cb;
+        a.item_ce = tp.item_ce;
+        a.scratch = tp.scratch;
+        a.huge_words = tp.huge_words;
+    }
     a.big_list = s->q_big_list;
     a.big_narrow = s->q_big_narrow;
     a.small_list = s->q_small_list;
@@ -2480,8 +2527,8 @@ int launch_block(mcb_solver *s, int32_t e, int32_t kk) {
     hipLaunchKernelGGL(k_block_xor, dim3(static_cast<unsigned>((cols + 255) / 256), static_cast<unsigned>((wl + kTile - 1) / kTile)),
                        dim3(256), sizeof(uint64_t) * kBlockK * kTile, s->stream, a);
     HIP_TRY(hipGetLastError());
-    s->blk_positions += s->Ptot;
-    s->blk_candidates += s->K;
+    s->blk_positions += s->plans[s->plan].pos_cum.back();
+    s->blk_candidates += s->plans[s->plan].cand_cum.back();
     return timed_end(s);
 }
 
@@ -2584,19 +2631,64 @@ std::vector<uint32_t> tree_depths(const int64_t *tree_off, const uint32_t *paren
 }
 
 // Builds the plan for nw waves per workgroup; MCB_E_INVALID when its LDS does not fit one workgroup.
-int plan_tree_pass(int nw, int nsplit, const int64_t *tree_off, const std::vector<uint32_t> &depth,
-                   const std::vector<int64_t> &ranges, const void *kernel, TreePlan &p, int64_t &bytes) {
+// rc_off: each root's candidate run in the rc_* arrays.
+int plan_tree_pass(int nw, int nsplit, const int64_t *tree_off, const int64_t *rc_off,
+                   const std::vector<uint32_t> &depth, const std::vector<int64_t> &ranges, const void *kernel,
+                   TreePlan &p, int64_t &bytes) {
     const size_t G = ranges.size() - 1;
+    const int64_t lds_max = std::min<int64_t>(g_dev.prop.sharedMemPerBlock, 160 * 1024);
+    // largest multiple of 64 positions whose words and stamps fit the LDS
+    const int64_t big_cap = lds_max * 64 / (8 * 64 + 4) / kWave * kWave;
     int64_t largest = 0;
     for (int64_t r = 0; r < ranges.back(); ++r) largest = std::max<int64_t>(largest, tree_off[r + 1] - tree_off[r]);
-    const int64_t largest_words = (largest + kWave - 1) / kWave * kWave;
+    const int64_t largest_words = std::min<int64_t>((largest + kWave - 1) / kWave * kWave, big_cap);
     const int64_t small_max = std::max<int64_t>(kWave, std::min<int64_t>(1024, largest_words / nw / kWave * kWave));
+    // sizes first: the LDS, hence the resident workgroups, hence the run length of split candidates
+    int64_t max_big = 0, max_small = 1, max_huge = 0, total_cands = rc_off[ranges.back()] - rc_off[0];
+    for (int64_t r = 0; r < ranges.back(); ++r) {
+        const int64_t np = tree_off[r + 1] - tree_off[r];
+        if (rc_off[r + 1] == rc_off[r]) continue;
+        if (np > small_max) {
+            if (np > big_cap) max_huge = std::max(max_huge, np);
+            else max_big = std::max(max_big, np);
+        } else {
+            max_small = std::max(max_small, np);
+        }
+    }
+    const int64_t big_words = (max_big + kWave - 1) / kWave * kWave;
+    const int64_t small_words = (max_small + kWave - 1) / kWave * kWave;
+    const int64_t huge_words = (max_huge + kWave - 1) / kWave * kWave;
+    const int64_t lds = std::max<int64_t>({8 * big_words + 4 * (big_words / kWave), 8ll * nw * small_words,
+                                           4 * (huge_words / kWave)});
+    if (lds > lds_max) return MCB_E_INVALID;
+    p.nw = nw;
+    p.big_words = static_cast<int32_t>(big_words);
+    p.small_words = static_cast<int32_t>(small_words);
+    p.huge_words = huge_words;
+    p.lds = static_cast<int32_t>(std::max<int64_t>(lds, 8));
+    HIP_TRY(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_max)));
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, nw * kWave, p.lds));
+    // a multiple of the XCD count (the batched tree pass deals workgroups to XCD sections)
+    p.blocks = std::max(1, per_cu) * g_dev.prop.multiProcessorCount;
+    p.blocks = (p.blocks + kXcds - 1) / kXcds * kXcds;
+    const int64_t run = std::max<int64_t>(kItemCands, (total_cands + p.blocks - 1) / p.blocks);
+
     std::vector<std::vector<uint32_t>> big(nsplit), small(nsplit);
     std::vector<std::vector<uint8_t>> narrow(nsplit);
-    std::vector<uint32_t> order;
+    std::vector<uint32_t> order, item_root;
+    std::vector<int64_t> item_cb, item_ce;
     p.big_cum.assign(nsplit, std::vector<int32_t>(G + 1, 0));
     p.small_cum.assign(nsplit, std::vector<int32_t>(G + 1, 0));
-    int64_t max_big = 0, max_small = 1;
+    p.pos_cum.assign(G + 1, 0);
+    p.cand_cum.assign(G + 1, 0);
+    int64_t pos = 0, cand = 0;
+    auto add_item = [&](uint32_t r, int64_t cb, int64_t ce) {
+        item_root.push_back(r);
+        item_cb.push_back(cb);
+        item_ce.push_back(ce);
+        return static_cast<uint32_t>(item_root.size() - 1);
+    };
     for (size_t g = 0; g < G; ++g) {
         const int x = static_cast<int>(g % nsplit);
         order.resize(static_cast<size_t>(ranges[g + 1] - ranges[g]));
@@ -2604,16 +2696,22 @@ int plan_tree_pass(int nw, int nsplit, const int64_t *tree_off, const std::vecto
         auto npos = [&](uint32_t r) { return tree_off[r + 1] - tree_off[r]; };
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return npos(a) > npos(b); });
         for (uint32_t r : order) {
-            const int64_t np = npos(r);
+            const int64_t np = npos(r), cb = rc_off[r], ce = rc_off[r + 1];
+            if (ce == cb) continue;  // no candidate: nothing to emit
+            cand += ce - cb;
             if (np > small_max) {
-                big[x].push_back(r);
-                narrow[x].push_back(np < 128ll * (depth[r] + 1) ? 1 : 0);
-                max_big = std::max(max_big, np);
+                for (int64_t c0 = cb; c0 < ce; c0 += run) {
+                    big[x].push_back(add_item(r, c0, std::min(ce, c0 + run)));
+                    narrow[x].push_back(np < 128ll * (depth[r] + 1) ? 1 : 0);
+                    pos += np;
+                }
             } else {
-                small[x].push_back(r);
-                max_small = std::max(max_small, np);
+                small[x].push_back(add_item(r, cb, ce));
+                pos += np;
             }
         }
+        p.pos_cum[g + 1] = pos;
+        p.cand_cum[g + 1] = cand;
         for (int y = 0; y < nsplit; ++y) {
             p.big_cum[y][g + 1] = static_cast<int32_t>(big[y].size());
             p.small_cum[y][g + 1] = static_cast<int32_t>(small[y].size());
@@ -2630,31 +2728,22 @@ int plan_tree_pass(int nw, int nsplit, const int64_t *tree_off, const std::vecto
         narrow_all.insert(narrow_all.end(), narrow[x].begin(), narrow[x].end());
         small_all.insert(small_all.end(), small[x].begin(), small[x].end());
     }
-    const int64_t big_words = (max_big + kWave - 1) / kWave * kWave;
-    const int64_t small_words = (max_small + kWave - 1) / kWave * kWave;
-    const int64_t lds = std::max<int64_t>(8 * big_words + 4 * (big_words / kWave), 8ll * nw * small_words);
-    const int64_t lds_max = std::min<int64_t>(g_dev.prop.sharedMemPerBlock, 160 * 1024);
-    if (lds > lds_max) return MCB_E_INVALID;
-    p.nw = nw;
-    p.big_words = static_cast<int32_t>(big_words);
-    p.small_words = static_cast<int32_t>(small_words);
-    p.lds = static_cast<int32_t>(lds);
     p.n_big = static_cast<int32_t>(big_all.size());
     p.n_small = static_cast<int32_t>(small_all.size());
-    HIP_TRY(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_max)));
-    int per_cu = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, nw * kWave, p.lds));
-    p.blocks = std::max(1, per_cu) * g_dev.prop.multiProcessorCount;
     int rc;
     if ((rc = upload(&p.big, big_all.data(), big_all.size(), bytes)) ||
         (rc = upload(&p.small, small_all.data(), small_all.size(), bytes)) ||
-        (rc = upload(&p.narrow, narrow_all.data(), narrow_all.size(), bytes)))
+        (rc = upload(&p.narrow, narrow_all.data(), narrow_all.size(), bytes)) ||
+        (rc = upload(&p.item_root, item_root.data(), item_root.size(), bytes)) ||
+        (rc = upload(&p.item_cb, item_cb.data(), item_cb.size(), bytes)) ||
+        (rc = upload(&p.item_ce, item_ce.data(), item_ce.size(), bytes)))
         return rc;
+    if (huge_words && (rc = dev_alloc(&p.scratch, static_cast<size_t>(p.blocks) * huge_words, bytes))) return rc;
     return MCB_OK;
 }
 
 // The plans worth trying: the requested width, or 16, 8 and 4 waves (those whose LDS fits).
-int make_plans(int want, int nsplit, const int64_t *tree_off, const uint32_t *parent_pos,
+int make_plans(int want, int nsplit, const int64_t *tree_off, const uint32_t *parent_pos, const int64_t *rc_off,
                const std::vector<int64_t> &ranges, const void *kernel, std::vector<TreePlan> &plans, int64_t &bytes) {
     if (want != 0 && want != 4 && want != 8 && want != 16)
         return fail(MCB_E_INVALID, "waves_per_block must be 0, 4, 8 or 16");
@@ -2662,7 +2751,7 @@ int make_plans(int want, int nsplit, const int64_t *tree_off, const uint32_t *pa
     for (int nw : {16, 8, 4}) {
         if (want && nw != want) continue;
         TreePlan p;
-        const int rc = plan_tree_pass(nw, nsplit, tree_off, depth, ranges, kernel, p, bytes);
+        const int rc = plan_tree_pass(nw, nsplit, tree_off, rc_off, depth, ranges, kernel, p, bytes);
         if (rc == MCB_OK) plans.push_back(std::move(p));
         else {
             free_plan(p);
@@ -2687,6 +2776,7 @@ void use_plan(mcb_solver *s, size_t i) {
     s->q_small_list = p.small;
     s->q_big_narrow = p.narrow;
 }
+
 
 // Times `launch` (one tree pass under the plan `use(i)` selects) for every plan and leaves the fastest
 // selected: the first of three runs warms up, the best of the other two counts.
@@ -2727,12 +2817,12 @@ int pick_fastest(size_t nplans, hipStream_t stream, UseFn use, LaunchFn launch) 
 
 // Buffers and launch shape of the blocked form.  Chooses the per-phase form instead when
 // block_phases asks for it or (block_phases 0) when no tree pass plan fits the LDS.
-int setup_blocked(mcb_solver *s, const int64_t *tree_offset, const uint32_t *parent_pos) {
+int setup_blocked(mcb_solver *s, const int64_t *tree_offset, const uint32_t *parent_pos, const int64_t *rc_off) {
     const int want = s->opt.block_phases;
     if (want < 0 || want > kBlockK) return fail(MCB_E_INVALID, "block_phases must be 0..64");
     if (want == 1) return MCB_OK;
     const std::vector<int64_t> ranges{0, static_cast<int64_t>(s->F)};
-    int rc = make_plans(s->opt.waves_per_block, 1, tree_offset, parent_pos, ranges,
+    int rc = make_plans(s->opt.waves_per_block, 1, tree_offset, parent_pos, rc_off, ranges,
                         reinterpret_cast<const void *>(k_block_q), s->plans, s->bytes);
     if (rc) return rc;
     if (s->plans.empty()) {
@@ -2915,7 +3005,7 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
     blocks = std::min<int64_t>(blocks, s->n_big + (s->n_small + kBlockWaves - 1) / kBlockWaves);
     s->blocks = static_cast<int>(std::max<int64_t>(1, blocks));
     if ((rc = dev_alloc(&s->wave_stats, 3 * static_cast<size_t>(s->blocks) * s->wpb, B))) return cleanup(rc);
-    if ((rc = setup_blocked(s, d->tree_offset, d->parent_pos))) return cleanup(rc);
+    if ((rc = setup_blocked(s, d->tree_offset, d->parent_pos, d->root_cand_offset))) return cleanup(rc);
     if (s->opt.timing) {
         s->ev_pool.resize(4096);
         s->ev_kind.resize(2048);
@@ -3160,7 +3250,6 @@ struct mcb_batch {
     std::vector<int64_t> grp_off;                                   // per merge level: first group
     std::vector<int64_t> step_off;                                  // per step: first entry of its tables
     std::vector<int32_t> dot_cum, xor_cum;
-    std::vector<int64_t> pos_cum, cand_cum;                         // [G + 1] tree positions, candidates
     // device arrays
     int64_t *tree_off = nullptr, *rc_off = nullptr;
     uint32_t *parent_pos = nullptr, *pos_pre = nullptr, *pos_end = nullptr, *rc_pu = nullptr, *rc_pv = nullptr,
@@ -3225,6 +3314,11 @@ BatchArgs batch_args(const mcb_batch *b, int32_t s) {
     a.big_list = tp.big;
     a.small_list = tp.small;
     a.big_narrow = tp.narrow;
+    a.item_root = tp.item_root;
+    a.item_cb = tp.item_cb;
+    a.item_ce = tp.item_ce;
+    a.scratch = tp.scratch;
+    a.huge_words = tp.huge_words;
     a.rc_off = b->rc_off;
     a.rc_pu = b->rc_pu;
     a.rc_pv = b->rc_pv;
@@ -3326,8 +3420,8 @@ int launch_batch_step(mcb_batch *b, int32_t s) {
     hipLaunchKernelGGL(k_batch_xor, dim3(static_cast<unsigned>(b->xor_cum[off + A])), dim3(256),
                        sizeof(uint64_t) * kBlockK * kTile, b->stream, a);
     HIP_TRY(hipGetLastError());
-    b->blk_positions += b->pos_cum[A];
-    b->blk_candidates += b->cand_cum[A];
+    b->blk_positions += tp.pos_cum[A];
+    b->blk_candidates += tp.cand_cum[A];
     return timed_end(b);
 }
 
@@ -3462,8 +3556,6 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
     std::vector<uint32_t> parent_pos(P_tot), rc_pu(K_tot), rc_pv(K_tot), rc_rank(K_tot), cand_root(rank_tot, 0),
         cand_pu(rank_tot, 0), cand_pv(rank_tot, 0), root_comp(F_tot);
     std::vector<int32_t> pos_coord(P_tot), rc_coord(K_tot), cand_coord(rank_tot, -1);
-    b->pos_cum.assign(G + 1, 0);
-    b->cand_cum.assign(G + 1, 0);
     std::vector<int64_t> ranges(G + 1, F_tot);
     for (int32_t g = 0; g < G; ++g) {
         const mcb_solver_desc &d = descs[b->order[g]];
@@ -3492,15 +3584,13 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
             rc_coord[KB + i] = d.cand_coord[k] >= 0 ? static_cast<int32_t>(cb + d.cand_coord[k]) : -1;
         }
         ranges[g] = R;
-        b->pos_cum[g + 1] = b->pos_cum[g] + d.num_positions;
-        b->cand_cum[g + 1] = b->cand_cum[g] + d.num_candidates;
     }
     tree_off[F_tot] = P_tot;
     rc_off[F_tot] = K_tot;
     std::vector<uint32_t> pre, end;
     preorder_intervals(tree_off.data(), parent_pos.data(), static_cast<int32_t>(F_tot), pre, end);
 
-    if ((rc = make_plans(b->opt.waves_per_block, kXcds, tree_off.data(), parent_pos.data(), ranges,
+    if ((rc = make_plans(b->opt.waves_per_block, kXcds, tree_off.data(), parent_pos.data(), rc_off.data(), ranges,
                          reinterpret_cast<const void *>(k_batch_q), b->plans, b->bytes)))
         return cleanup(rc);
     if (b->plans.empty()) return cleanup(fail(MCB_E_INVALID, "a tree is too large for the blocked form's LDS budget"));

@@ -22,6 +22,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -667,6 +668,12 @@ int build_from_input(InputGraph &in, int64_t input_weight_sum, const mcb_build_o
     P->roots.resize(F);
     for (int32_t v = 0; v < g.n; ++v)
         if (P->fvs_rank[v] >= 0) P->roots[P->fvs_rank[v]] = static_cast<uint32_t>(v);
+    if (opt.stop_after_fvs) {  // the reduced graph and its FVS only
+        info.num_non_tree = static_cast<int32_t>(g.rows.size() / 2) - g.n + 1;
+        info.words = (info.num_non_tree + 63) / 64;
+        *out = P.release();
+        return MCB_OK;
+    }
 
     // Coordinates.
     int rc = spanning_tree_coordinates(g, src, P->nt_edges);
@@ -808,6 +815,179 @@ void mcb_build_options_init(mcb_build_options *opt) {
 }
 
 const char *mcb_host_last_error(void) { return g_err.c_str(); }
+
+int mcb_vertex_features(int32_t nodes, const uint32_t *ro, const uint32_t *cols, const int32_t *weights,
+                        const uint8_t *alive, float *out) {
+    if (nodes <= 0 || !ro || !cols || !weights || !out) return fail(MCB_E_INVALID, "invalid arguments");
+    try {
+        const int32_t n = nodes;
+        auto live = [&](uint32_t x) { return !alive || alive[x]; };
+        std::vector<int64_t> deg(n, 0), two(n, 0);
+        std::vector<double> wsum(n, 0.0);
+        double wall = 0.0;
+        int64_t ends = 0;
+        for (int32_t v = 0; v < n; ++v) {
+            if (!live(v)) continue;
+            for (uint32_t e = ro[v]; e < ro[v + 1]; ++e)
+                if (live(cols[e])) {
+                    ++deg[v];
+                    wsum[v] += weights[e];
+                }
+            wall += wsum[v];
+            ends += deg[v];
+        }
+        for (int32_t v = 0; v < n; ++v) {
+            if (!live(v)) continue;
+            for (uint32_t e = ro[v]; e < ro[v + 1]; ++e)
+                if (live(cols[e])) two[v] += deg[cols[e]];
+        }
+        const double wmean = ends ? wall / static_cast<double>(ends) : 1.0;
+        // core numbers: peel vertices in order of current degree (bucket queue; parallel edges count)
+        int64_t maxd = 0;
+        for (int32_t v = 0; v < n; ++v) maxd = std::max(maxd, deg[v]);
+        std::vector<int64_t> d(deg), core(n, 0);
+        std::vector<int32_t> bin(static_cast<size_t>(maxd) + 2, 0), pos(n), vert(n);
+        for (int32_t v = 0; v < n; ++v) ++bin[d[v]];
+        int32_t start = 0;
+        for (int64_t k = 0; k <= maxd; ++k) {
+            const int32_t c = bin[k];
+            bin[k] = start;
+            start += c;
+        }
+        for (int32_t v = 0; v < n; ++v) {
+            pos[v] = bin[d[v]]++;
+            vert[pos[v]] = v;
+        }
+        for (int64_t k = maxd; k > 0; --k) bin[k] = bin[k - 1];
+        bin[0] = 0;
+        int64_t maxcore = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            const int32_t v = vert[i];
+            core[v] = d[v];
+            maxcore = std::max(maxcore, core[v]);
+            if (!live(v)) continue;
+            for (uint32_t e = ro[v]; e < ro[v + 1]; ++e) {
+                const int32_t u = static_cast<int32_t>(cols[e]);
+                if (live(u) && d[u] > d[v]) {  // move u one bucket down
+                    const int64_t du = d[u];
+                    const int32_t pu = pos[u], pw = bin[du], w = vert[pw];
+                    if (u != w) {
+                        pos[u] = pw;
+                        vert[pu] = w;
+                        pos[w] = pu;
+                        vert[pw] = u;
+                    }
+                    ++bin[du];
+                    --d[u];
+                }
+            }
+        }
+        for (int32_t v = 0; v < n; ++v) {
+            float *f = out + static_cast<size_t>(v) * 4;
+            if (!live(v)) {
+                f[0] = f[1] = f[2] = f[3] = 0.0f;
+                continue;
+            }
+            f[0] = static_cast<float>(std::log1p(static_cast<double>(deg[v])));
+            f[1] = static_cast<float>(std::log1p(static_cast<double>(two[v])));
+            f[2] = static_cast<float>(static_cast<double>(core[v]) / static_cast<double>(std::max<int64_t>(1, maxcore)));
+            f[3] = static_cast<float>(deg[v] ? wsum[v] / static_cast<double>(deg[v]) / std::max(wmean, 1e-9) : 0.0);
+        }
+    } catch (const std::bad_alloc &) {
+        return fail(MCB_E_NOMEM, "out of memory");
+    }
+    return MCB_OK;
+}
+
+int mcb_fvs_peel(int32_t nodes, const uint32_t *ro, const uint32_t *cols, uint8_t *alive) {
+    if (nodes <= 0 || !ro || !cols || !alive) return fail(MCB_E_INVALID, "invalid arguments");
+    try {
+        const int32_t n = nodes;
+        std::vector<int64_t> d(n, 0);
+        std::vector<int32_t> stack;
+        for (int32_t v = 0; v < n; ++v) {
+            if (!alive[v]) continue;
+            for (uint32_t e = ro[v]; e < ro[v + 1]; ++e) d[v] += alive[cols[e]] ? 1 : 0;
+            if (d[v] <= 1) stack.push_back(v);
+        }
+        for (int32_t v : stack) alive[v] = 0;
+        while (!stack.empty()) {
+            const int32_t v = stack.back();
+            stack.pop_back();
+            for (uint32_t e = ro[v]; e < ro[v + 1]; ++e) {
+                const int32_t u = static_cast<int32_t>(cols[e]);
+                if (alive[u] && --d[u] <= 1) {
+                    alive[u] = 0;
+                    stack.push_back(u);
+                }
+            }
+        }
+        int32_t left = 0;
+        for (int32_t v = 0; v < n; ++v) left += alive[v] ? 1 : 0;
+        return left;
+    } catch (const std::bad_alloc &) {
+        return fail(MCB_E_NOMEM, "out of memory");
+    }
+}
+
+int mcb_fvs_greedy(int32_t nodes, int32_t directed_edges, const uint32_t *ro, const uint32_t *cols, int32_t *fvs_rank) {
+    if (nodes <= 0 || directed_edges < 0 || !ro || !cols || !fvs_rank) return fail(MCB_E_INVALID, "invalid arguments");
+    if (ro[nodes] != static_cast<uint32_t>(directed_edges))
+        return fail(MCB_E_INVALID, "row offsets do not match the edge count");
+    try {
+        const int32_t n = nodes;
+        std::vector<int64_t> d(n);
+        std::vector<uint8_t> alive(n, 1);
+        for (int32_t v = 0; v < n; ++v) d[v] = ro[v + 1] - ro[v];
+        std::vector<int32_t> stack, picked;
+        // removes v and every vertex left with at most one incident edge, transitively
+        auto remove = [&](int32_t v0) {
+            stack.push_back(v0);
+            alive[v0] = 0;
+            while (!stack.empty()) {
+                const int32_t v = stack.back();
+                stack.pop_back();
+                for (uint32_t e = ro[v]; e < ro[v + 1]; ++e) {
+                    const int32_t u = static_cast<int32_t>(cols[e]);
+                    if (!alive[u]) continue;
+                    if (--d[u] <= 1) {
+                        alive[u] = 0;
+                        stack.push_back(u);
+                    }
+                }
+            }
+        };
+        for (int32_t v = 0; v < n; ++v)
+            if (alive[v] && d[v] <= 1) remove(v);
+        // max current degree first (ties: lowest id); stale heap entries are skipped
+        std::priority_queue<std::pair<int64_t, int32_t>> heap;
+        for (int32_t v = 0; v < n; ++v)
+            if (alive[v]) heap.push({d[v], -v});
+        while (!heap.empty()) {
+            const auto [dv, nv] = heap.top();
+            heap.pop();
+            const int32_t v = -nv;
+            if (!alive[v] || dv != d[v]) {
+                if (alive[v]) heap.push({d[v], -v});
+                continue;
+            }
+            picked.push_back(v);
+            remove(v);
+        }
+        // minimal subset: reverse-delete trying the vertices outside the pick first, then the picks
+        // from the last to the first
+        std::vector<uint8_t> in_pick(n, 0);
+        for (int32_t v : picked) in_pick[v] = 1;
+        std::vector<int32_t> order;
+        order.reserve(n);
+        for (int32_t v = 0; v < n; ++v)
+            if (!in_pick[v]) order.push_back(v);
+        for (auto it = picked.rbegin(); it != picked.rend(); ++it) order.push_back(*it);
+        return fvs_reverse_delete_impl(n, ro, cols, order.data(), fvs_rank);
+    } catch (const std::bad_alloc &) {
+        return fail(MCB_E_NOMEM, "out of memory");
+    }
+}
 
 int mcb_problem_build_edges(int32_t nodes, int32_t edges, const int32_t *u, const int32_t *v, const int32_t *w,
                             const mcb_build_options *opt, mcb_problem **out) {

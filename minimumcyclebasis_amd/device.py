@@ -333,12 +333,24 @@ class MCBResult:
 
 
 def minimum_cycle_basis(source, reduce_degree_two: bool = True, prune_trees: bool = True, num_threads: int = 0,
-                        fvs_order=None, block_phases: int = 0) -> MCBResult:
+                        fvs_order=None, block_phases: int = 0, fvs: str = "plain") -> MCBResult:
     """Minimum cycle basis weight and cycles of a graph given as a .mtx path or (n, u, v, w).  The result's
-    ``stats`` is the run's row for the reference's stats CSV (``stats.append_stats_csv``)."""
+    ``stats`` is the run's row for the reference's stats CSV (``stats.append_stats_csv``).
+
+    fvs="learned" orders the FVS pass by the learned vertex scorer (``learned_fvs``), falling back to the
+    reference order unless that gives a smaller verified FVS; the weight is the same either way."""
     import time
 
     t0 = time.perf_counter()
+    if fvs not in ("plain", "learned"):
+        raise ValueError("fvs must be 'plain' or 'learned'")
+    if fvs == "learned" and not isinstance(source, Problem) and fvs_order is None:
+        from . import generators
+        from .learned_fvs import learned_order
+
+        if isinstance(source, (str, bytes)) or hasattr(source, "__fspath__"):
+            source = generators.read_mtx(source)
+        fvs_order = learned_order(*source, reduce_degree_two=reduce_degree_two).order
     if isinstance(source, Problem):
         prob = source
     elif isinstance(source, (str, bytes)) or hasattr(source, "__fspath__"):

@@ -61,9 +61,10 @@ class Info:
 
 
 def _options(reduce_degree_two: bool, prune_trees: bool, keep_vertex_trees: bool, num_threads: int,
-             fvs_order=None):
+             fvs_order=None, stop_after_fvs: bool = False):
     opt = _lib.BuildOptions()
     _lib.host().mcb_build_options_init(ctypes.byref(opt))
+    opt.stop_after_fvs = int(bool(stop_after_fvs))
     opt.reduce_degree_two = int(bool(reduce_degree_two))
     opt.prune_trees = int(bool(prune_trees))
     opt.keep_vertex_trees = int(bool(keep_vertex_trees))
@@ -108,6 +109,19 @@ class Problem:
         if not (u.shape == v.shape == w.shape):
             raise ValueError("u, v, w must have the same length")
         opt, _keep = _options(reduce_degree_two, prune_trees, keep_vertex_trees, num_threads, fvs_order)
+        h = ctypes.c_void_p()
+        _lib.check_host(_lib.host().mcb_problem_build_edges(
+            int(n), int(u.size), u.ctypes.data, v.ctypes.data, w.ctypes.data, ctypes.byref(opt), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def reduced_graph(cls, n: int, u, v, w, reduce_degree_two: bool = True) -> "Problem":
+        """Only the reduced multigraph and its reference-order FVS (rows, cols, weights, row offsets,
+        fvs rank, roots; info through fvs_size and num_non_tree): cheap, no shortest-path trees."""
+        u = np.ascontiguousarray(u, dtype=np.int32)
+        v = np.ascontiguousarray(v, dtype=np.int32)
+        w = np.ascontiguousarray(w, dtype=np.int32)
+        opt, _keep = _options(reduce_degree_two, False, False, 1, None, stop_after_fvs=True)
         h = ctypes.c_void_p()
         _lib.check_host(_lib.host().mcb_problem_build_edges(
             int(n), int(u.size), u.ctypes.data, v.ctypes.data, w.ctypes.data, ctypes.byref(opt), ctypes.byref(h)))
@@ -194,4 +208,39 @@ def biconnected_components(n: int, u, v) -> np.ndarray:
     out = np.zeros(u.size, dtype=np.int32)
     _lib.check_host(_lib.host().mcb_biconnected_components(int(n), int(u.size), u.ctypes.data, v.ctypes.data,
                                                            out.ctypes.data))
+    return out
+
+
+def vertex_features(row_offsets, cols, weights, alive=None) -> np.ndarray:
+    """Per vertex of a CSR multigraph (of the subgraph induced by ``alive`` when given): log(1 + degree),
+    log(1 + neighbours' degree sum), core number / max core, mean incident weight / mean weight (float32,
+    n x 4; zeros for vertices not alive)."""
+    ro = np.ascontiguousarray(row_offsets, dtype=np.uint32)
+    c = np.ascontiguousarray(cols, dtype=np.uint32)
+    wt = np.ascontiguousarray(weights, dtype=np.int32)
+    al = None if alive is None else np.ascontiguousarray(alive, dtype=np.uint8)
+    n = ro.size - 1
+    out = np.zeros((n, 4), dtype=np.float32)
+    _lib.check_host(_lib.host().mcb_vertex_features(n, ro.ctypes.data, c.ctypes.data, wt.ctypes.data,
+                                                    None if al is None else al.ctypes.data, out.ctypes.data))
+    return out
+
+
+def fvs_peel(row_offsets, cols, alive: np.ndarray) -> int:
+    """Clears (in place) every alive vertex left with at most one edge to alive vertices, transitively;
+    returns the number still alive.  ``alive`` must be a contiguous uint8 array."""
+    ro = np.ascontiguousarray(row_offsets, dtype=np.uint32)
+    c = np.ascontiguousarray(cols, dtype=np.uint32)
+    if alive.dtype != np.uint8 or not alive.flags.c_contiguous:
+        raise ValueError("alive must be a contiguous uint8 array")
+    return _lib.check_host(_lib.host().mcb_fvs_peel(ro.size - 1, ro.ctypes.data, c.ctypes.data, alive.ctypes.data))
+
+
+def fvs_greedy(row_offsets, cols) -> np.ndarray:
+    """The peeling max-degree greedy FVS made minimal by reverse-delete; root rank per vertex (-1 = not in F)."""
+    ro = np.ascontiguousarray(row_offsets, dtype=np.uint32)
+    c = np.ascontiguousarray(cols, dtype=np.uint32)
+    n = ro.size - 1
+    out = np.zeros(n, dtype=np.int32)
+    _lib.check_host(_lib.host().mcb_fvs_greedy(n, int(c.size), ro.ctypes.data, c.ctypes.data, out.ctypes.data))
     return out

@@ -156,3 +156,42 @@ def test_tree_pass_widths(name, waves):
     assert np.array_equal(s.selected(), ref["selected"])
     assert np.array_equal(s.cycle_vectors(), ref["C"])
     s.close()
+
+
+def _wheel(rim: int, seed: int):
+    """A hub joined to every vertex of a rim cycle: every root's tree spans all vertices."""
+    rng = np.random.default_rng(seed)
+    r = np.arange(rim)
+    u = np.concatenate([r, np.full(rim, rim)])
+    v = np.concatenate([(r + 1) % rim, r])
+    return rim + 1, u, v, rng.integers(1, 101, size=2 * rim)
+
+
+@pytest.mark.parametrize("waves", [0, 4])
+def test_tree_larger_than_lds_and_split_candidates(waves):
+    """Trees of 25 001 positions (path words beyond the LDS, kept in global scratch) and roots with more
+    than 8192 candidates (listed as several items) against the oracle."""
+    from minimumcyclebasis_amd.host import Problem
+
+    n, u, v, w = _wheel(25000, 3)
+    p = Problem.from_edges(n, u, v, w, keep_vertex_trees=True, prune_trees=True)
+    print(f"case wheel25000 waves={waves} F={p.info.fvs_size} max_pos={p.info.max_positions} "
+          f"K={p.info.num_candidates}", flush=True)
+    assert p.info.max_positions > 21000 and p.info.num_candidates > 8192
+    ref = pyoracle.run(p, phases=700, want_c=True)
+    s = Solver(p, store_cycles=True, waves_per_block=waves)
+    assert s.launch_shape()["block_phases"] == 64
+    s.run(700)
+    s.sync()
+    assert np.array_equal(s.selected(700), ref["selected"])
+    assert np.array_equal(s.cycle_vectors(700), ref["C"])
+    s.run()
+    s.sync()
+    r, a, b, wt = p.candidate_table()
+    plain = Solver(p, block_phases=1)
+    plain.run()
+    plain.sync()
+    assert int(wt[s.selected()].astype(np.int64).sum()) == int(wt[plain.selected()].astype(np.int64).sum())
+    assert np.array_equal(s.selected(), plain.selected())
+    s.close()
+    plain.close()
