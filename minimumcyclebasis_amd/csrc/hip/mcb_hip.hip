@@ -711,7 +711,7 @@ struct BlockArgs {
     uint64_t *C_blk;     // [64][W] cycle vectors of the block
     uint32_t *D_blk;     // [64][W] nonzero words of each below e
     uint32_t *Dcnt_blk;  // [64]
-    uint64_t *Sf_blk;    // [64][W] final support vectors of the block
+    uint64_t *Sf_blk;    // [W][64] final support vectors of the block (word-major)
     uint64_t *C_store;   // optional [N][W]
     uint64_t *Ct;        // [npad] bit i of Ct[c] = bit c of C_{e+i}
     uint64_t *Bc;        // [npad] update coefficients of each later column
@@ -1403,8 +1403,8 @@ __device__ void cycle_into_lds(const int64_t *tree_off, const uint32_t *pos_pre,
     }
 }
 
-// C_{e+i} of phase e+i's winner (and its nonzero words below e), and the final support vector
-// S_{e+i} = sum over l of m_i[l] T_l.  One workgroup; Cl: W words of LDS, cnt: one LDS counter.
+// C_{e+i} of phase e+i's winner and its nonzero words below e.  One workgroup; Cl: W words of LDS,
+// cnt: one LDS counter.
 template <typename A>
 __device__ __forceinline__ void cycles_body(const A &a, const BlkView &v, int i, uint64_t *Cl, uint32_t *cnt) {
     const int tid = threadIdx.x;
@@ -1425,21 +1425,27 @@ __device__ __forceinline__ void cycles_body(const A &a, const BlkView &v, int i,
         if (v.C_store) v.C_store[(static_cast<int64_t>(v.e) + i) * v.W + w] = x;
         if (w < dlim && x) Dout[atomicAdd(cnt, 1u)] = static_cast<uint32_t>(w);
     }
-    const uint64_t m = v.blk_m[i];
-    const int32_t wl = (v.e + v.kk + kWave - 1) / kWave;
-    for (int w = tid; w < wl; w += blockDim.x) {
-        uint64_t x = 0;
-        for (int l = 0; l < v.kk; ++l) {
-            if (!((m >> l) & 1ull)) continue;
-            const int64_t j = static_cast<int64_t>(v.e) + l;
-            uint64_t t = v.Sw[static_cast<int64_t>(w) * v.npad + j];
-            if ((j >> 6) == w) t |= 1ull << (j & 63);
-            x ^= t;
-        }
-        v.Sf_blk[static_cast<int64_t>(i) * v.W + w] = x;
-    }
     __syncthreads();
     if (tid == 0) v.Dcnt_blk[i] = *cnt;
+}
+
+// The block's final support vectors S_{e+i} = sum over l of m_i[l] T_l on word w (a 64 x 64 product
+// over GF(2)): lane l loads T_l's word w (with its own bit), lane i accumulates the rows m_i selects.
+// Sf_blk[w][i] holds the result.  One wave per word.
+__device__ __forceinline__ void final_body(const BlkView &v, int64_t w, int lane) {
+    const int64_t j = static_cast<int64_t>(v.e) + lane;
+    uint64_t x = 0;
+    if (lane < v.kk) {
+        x = v.Sw[w * v.npad + j];
+        if ((j >> 6) == w) x |= 1ull << (j & 63);
+    }
+    const uint64_t m = lane < v.kk ? v.blk_m[lane] : 0ull;
+    uint64_t acc = 0;
+    for (int l = 0; l < v.kk; ++l) {
+        const uint64_t t = rl64(x, l);
+        if ((m >> l) & 1ull) acc ^= t;
+    }
+    v.Sf_blk[w * kBlockK + lane] = acc;
 }
 
 // Workgroup i: phase e+i of the block.
@@ -1447,6 +1453,14 @@ __global__ __launch_bounds__(256) void k_block_cycles(BlockArgs a) {
     extern __shared__ uint64_t lds[];
     __shared__ uint32_t cnt;
     cycles_body(a, view_of(a), blockIdx.x, lds, &cnt);
+}
+
+// Wave w: word w of the block's final support vectors.
+__global__ __launch_bounds__(256) void k_block_final(BlockArgs a) {
+    const int64_t w = static_cast<int64_t>(blockIdx.x) * 4 + wave_id();
+    const int64_t wl = (static_cast<int64_t>(a.e) + a.kk + kWave - 1) / kWave;
+    if (w >= wl) return;
+    final_body(view_of(a), w, threadIdx.x & (kWave - 1));
 }
 
 // Ct[64w + b] bit i = bit b of word w of C_{e+i}: one wave per word.
@@ -1627,7 +1641,7 @@ __global__ __launch_bounds__(256) void k_block_dot(BlockArgs a) {
 
 // Tile (x, y): columns e + 256x .. +255, words kTile*y .. +kTile-1.  Column e+i < e+kk receives its
 // final vector S_{e+i}; a later column j with Bc[j] != 0 takes S_j ^= sum_i b_i S_{e+i}.
-// LDS: the block's final vectors on the tile's words [64][kTile].
+// LDS: the block's final vectors on the tile's words, T[t][i] = word w0 + t of S_{e+i}.
 __device__ __forceinline__ void xor_body(const BlkView &v, int64_t xt, int32_t yt, uint64_t *lds,
                                          unsigned long long *upd_bytes) {
     constexpr int kW = 8;
@@ -1638,8 +1652,8 @@ __device__ __forceinline__ void xor_body(const BlkView &v, int64_t xt, int32_t y
     const int tn = wl - w0 < kTile ? wl - w0 : kTile;
     uint64_t *T = lds;
     for (int x = tid; x < kBlockK * kTile; x += blockDim.x) {
-        const int i = x / kTile, t = x % kTile;
-        T[x] = (i < kk && t < tn) ? v.Sf_blk[static_cast<int64_t>(i) * v.W + w0 + t] : 0ull;
+        const int t = x / kBlockK, i = x % kBlockK;
+        T[x] = (i < kk && t < tn) ? v.Sf_blk[static_cast<int64_t>(w0 + t) * kBlockK + i] : 0ull;
     }
     __syncthreads();
     const int64_t j = static_cast<int64_t>(v.e) + xt * blockDim.x + tid;
@@ -1647,7 +1661,7 @@ __device__ __forceinline__ void xor_body(const BlkView &v, int64_t xt, int32_t y
     if (j < static_cast<int64_t>(v.e) + kk) {
         const int i = static_cast<int>(j - v.e);
         for (int t = 0; t < tn; ++t) {
-            uint64_t x = T[i * kTile + t];
+            uint64_t x = T[t * kBlockK + i];
             if ((j >> 6) == w0 + t) x &= ~(1ull << (j & 63));
             v.Sw[static_cast<int64_t>(w0 + t) * v.npad + j] = x;
         }
@@ -1661,9 +1675,8 @@ __device__ __forceinline__ void xor_body(const BlkView &v, int64_t xt, int32_t y
 #pragma unroll 4
                 for (int i = 0; i < kBlockK; ++i) {
                     const uint64_t m = 0ull - ((b >> i) & 1ull);
-                    const uint64_t *row = T + i * kTile + t0;
 #pragma unroll
-                    for (int u = 0; u < kW; ++u) xs[u] ^= row[u] & m;
+                    for (int u = 0; u < kW; ++u) xs[u] ^= T[(t0 + u) * kBlockK + i] & m;
                 }
                 uint64_t sv[kW];
 #pragma unroll
@@ -1891,6 +1904,15 @@ __global__ __launch_bounds__(256) void k_batch_cycles(BatchArgs a) {
     const BlkView v = batch_view(a, g);
     if (i >= v.kk) return;
     cycles_body(a, v, i, lds, &cnt);
+}
+
+// Wave x: word x % (step + 1) of problem x / (step + 1)'s final support vectors.
+__global__ __launch_bounds__(256) void k_batch_final(BatchArgs a) {
+    const int64_t x = static_cast<int64_t>(blockIdx.x) * 4 + wave_id();
+    const int64_t wl = a.step + 1;
+    if (x >= static_cast<int64_t>(a.active) * wl) return;
+    const int g = static_cast<int>(x / wl);
+    final_body(batch_view(a, g), x - g * wl, threadIdx.x & (kWave - 1));
 }
 
 // Problem g owns workgroups pairs_cum[g] .. pairs_cum[g+1]-1: the first lists its words, the others
@@ -2511,6 +2533,8 @@ int launch_block(mcb_solver *s, int32_t e, int32_t kk) {
     }
     if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_CYCLE))) return rc;
     hipLaunchKernelGGL(k_block_cycles, dim3(kk), dim3(256), sizeof(uint64_t) * s->W, s->stream, a);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_block_final, dim3(static_cast<unsigned>((wl + 3) / 4)), dim3(256), 0, s->stream, a);
     HIP_TRY(hipGetLastError());
     const int32_t dlim = (e + kWave - 1) / kWave;
     hipLaunchKernelGGL(k_block_pairs, dim3(static_cast<unsigned>(1 + (s->W + 3) / 4)), dim3(256),
@@ -3406,6 +3430,8 @@ int launch_batch_step(mcb_batch *b, int32_t s) {
     // problems are in descending phase count, so problem 0 has the most words
     hipLaunchKernelGGL(k_batch_cycles, dim3(static_cast<unsigned>(A) * kBlockK), dim3(256),
                        sizeof(uint64_t) * b->comps[0].W, b->stream, a);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_batch_final, dim3(static_cast<unsigned>((tw + 3) / 4)), dim3(256), 0, b->stream, a);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_batch_pairs, dim3(static_cast<unsigned>(b->pairs_cum[A])), dim3(256),
                        sizeof(uint64_t) * s + 2 * 256 * sizeof(uint32_t), b->stream, a);
