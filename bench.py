@@ -240,6 +240,7 @@ def main() -> int:
         "upload_s": round(upload_s, 3),
         "block_phases": shape["block_phases"],
         "tree_pass_waves": shape["waves_per_block"],
+        "tree_pass_lds": shape["lds_bytes"],
         "launches_per_step": k_n // max(args.steps, 1),
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in kt.items() if v[1]},
         "update_bytes_per_step": upd // max(args.steps, 1),
@@ -438,6 +439,7 @@ def run_components(args) -> int:
         "upload_s": round(upload_s, 3),
         "steps_per_loop": shape["steps"],
         "tree_pass_waves": shape["waves_per_block"],
+        "tree_pass_lds": shape["lds_bytes"],
         "plain_loop_s": None if plain_s is None else round(plain_s, 6),
         "speedup_vs_plain": None if plain_s is None else round(plain_s / batched_s, 2),
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in kt.items() if v[1]},

@@ -124,9 +124,11 @@ typedef struct {
     int32_t max_blocks;       /* 0 = automatic (resident blocks on every compute unit) */
     int32_t timing;           /* HIP events on the solver's stream around kernel launches: 0 none,
                                  1 every kernel, 2 only MCB_K_PARITY_SEARCH (the dominant kernel) */
-    int32_t block_phases;     /* phases resolved per pass over the trees: 0 = automatic (64 when the
-                                 largest tree's 64-bit path words fit one workgroup's LDS, else 1),
+    int32_t block_phases;     /* phases resolved per pass over the trees: 0 = automatic (64),
                                  1 = one kernel per phase, 2..64 = blocked with that many */
+    int32_t tree_cap;         /* blocked form's tree pass: largest tree whose path words stay in LDS, in
+                                 positions (larger ones use global scratch); 0 = timed at create among
+                                 as-large-as-fits, 8192 and 2048; -1 = as large as fits */
 } mcb_solver_options;
 
 void mcb_solver_options_init(mcb_solver_options *opt);
@@ -205,8 +207,8 @@ void mcb_solver_destroy(mcb_solver *s);
 
 typedef struct mcb_batch mcb_batch;
 
-/* descs[0..count-1] are validated like mcb_solver_create's; of opt, store_cycles, timing and
- * waves_per_block are used (the width is timed on step 0).  Fails with MCB_E_INVALID when a tree does
+/* descs[0..count-1] are validated like mcb_solver_create's; of opt, store_cycles, timing,
+ * waves_per_block and tree_cap are used (the plan is timed on step 0).  Fails with MCB_E_INVALID when a tree does
  * not fit the blocked form's LDS budget. */
 int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solver_options *opt, mcb_batch **out);
 

@@ -32,6 +32,7 @@ class SolverOptions(ctypes.Structure):
         ("max_blocks", i32),
         ("timing", i32),
         ("block_phases", i32),
+        ("tree_cap", i32),
     ]
 
 

@@ -2287,8 +2287,9 @@ struct TreePlan {
     std::vector<int64_t> pos_cum, cand_cum;   // tree positions and candidates streamed, ranges before g
 };
 
+// The scratch is shared by a solver's plans (one runs at a time) and freed by its owner.
 void free_plan(TreePlan &p) {
-    void *ptrs[] = {p.big, p.small, p.narrow, p.item_root, p.item_cb, p.item_ce, p.scratch};
+    void *ptrs[] = {p.big, p.small, p.narrow, p.item_root, p.item_cb, p.item_ce};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     p.big = p.small = p.item_root = nullptr;
@@ -2332,6 +2333,7 @@ struct mcb_solver {
     uint8_t *q_big_narrow = nullptr;
     std::vector<TreePlan> plans;
     int32_t plan = 0;
+    uint64_t *q_scratch = nullptr;  // path words of trees beyond a plan's LDS cap
     uint64_t *Tt = nullptr, *Qr = nullptr, *qbits = nullptr, *sum_q = nullptr, *blk_q = nullptr, *blk_m = nullptr, *blk_g = nullptr;
     uint64_t *C_blk = nullptr, *Sf_blk = nullptr, *Ct = nullptr, *Bc = nullptr, *pair_c = nullptr;
     uint32_t *pw_list = nullptr, *pw_off = nullptr, *pw_cnt = nullptr, *pair_i = nullptr;
@@ -2656,13 +2658,15 @@ std::vector<uint32_t> tree_depths(const int64_t *tree_off, const uint32_t *paren
 
 // Builds the plan for nw waves per workgroup; MCB_E_INVALID when its LDS does not fit one workgroup.
 // rc_off: each root's candidate run in the rc_* arrays.
-int plan_tree_pass(int nw, int nsplit, const int64_t *tree_off, const int64_t *rc_off,
+// cap_words: the largest tree kept in LDS (0: as large as fits).
+int plan_tree_pass(int nw, int64_t cap_words, int nsplit, const int64_t *tree_off, const int64_t *rc_off,
                    const std::vector<uint32_t> &depth, const std::vector<int64_t> &ranges, const void *kernel,
                    TreePlan &p, int64_t &bytes) {
     const size_t G = ranges.size() - 1;
     const int64_t lds_max = std::min<int64_t>(g_dev.prop.sharedMemPerBlock, 160 * 1024);
-    // largest multiple of 64 positions whose words and stamps fit the LDS
-    const int64_t big_cap = lds_max * 64 / (8 * 64 + 4) / kWave * kWave;
+    // largest multiple of 64 positions whose words and stamps fit the LDS, or the requested cap
+    int64_t big_cap = lds_max * 64 / (8 * 64 + 4) / kWave * kWave;
+    if (cap_words > 0) big_cap = std::min(big_cap, cap_words);
     int64_t largest = 0;
     for (int64_t r = 0; r < ranges.back(); ++r) largest = std::max<int64_t>(largest, tree_off[r + 1] - tree_off[r]);
     const int64_t largest_words = std::min<int64_t>((largest + kWave - 1) / kWave * kWave, big_cap);
@@ -2762,26 +2766,50 @@ int plan_tree_pass(int nw, int nsplit, const int64_t *tree_off, const int64_t *r
         (rc = upload(&p.item_cb, item_cb.data(), item_cb.size(), bytes)) ||
         (rc = upload(&p.item_ce, item_ce.data(), item_ce.size(), bytes)))
         return rc;
-    if (huge_words && (rc = dev_alloc(&p.scratch, static_cast<size_t>(p.blocks) * huge_words, bytes))) return rc;
     return MCB_OK;
 }
 
-// The plans worth trying: the requested width, or 16, 8 and 4 waves (those whose LDS fits).
-int make_plans(int want, int nsplit, const int64_t *tree_off, const uint32_t *parent_pos, const int64_t *rc_off,
-               const std::vector<int64_t> &ranges, const void *kernel, std::vector<TreePlan> &plans, int64_t &bytes) {
+// The plans worth trying: the requested width, or 16, 8 and 4 waves, each with the largest LDS-resident
+// tree as large as fits, 8192 or 2048 positions (trees above the cap keep their words in scratch: less LDS
+// per workgroup, more workgroups per CU); caps that change nothing are left out.  *scratch receives one
+// buffer large enough for every plan (one runs at a time), owned by the caller.
+int make_plans(int want, int want_cap, int nsplit, const int64_t *tree_off, const uint32_t *parent_pos,
+               const int64_t *rc_off, const std::vector<int64_t> &ranges, const void *kernel,
+               std::vector<TreePlan> &plans, int64_t &bytes, uint64_t **scratch) {
     if (want != 0 && want != 4 && want != 8 && want != 16)
         return fail(MCB_E_INVALID, "waves_per_block must be 0, 4, 8 or 16");
+    if (want_cap < -1 || (want_cap > 0 && want_cap % kWave))
+        return fail(MCB_E_INVALID, "tree_cap must be -1, 0 or a positive multiple of 64");
     const std::vector<uint32_t> depth = tree_depths(tree_off, parent_pos, ranges.back());
+    int64_t largest = 0;
+    for (int64_t r = 0; r < ranges.back(); ++r) largest = std::max<int64_t>(largest, tree_off[r + 1] - tree_off[r]);
     for (int nw : {16, 8, 4}) {
         if (want && nw != want) continue;
-        TreePlan p;
-        const int rc = plan_tree_pass(nw, nsplit, tree_off, rc_off, depth, ranges, kernel, p, bytes);
-        if (rc == MCB_OK) plans.push_back(std::move(p));
-        else {
-            free_plan(p);
-            if (rc != MCB_E_INVALID) return rc;
+        for (int64_t cap : {int64_t{0}, int64_t{8192}, int64_t{2048}}) {
+            if (want_cap == -1 && cap) continue;
+            if (want_cap > 0) {
+                if (cap) continue;
+                cap = want_cap;
+            } else if (cap && cap >= largest) {
+                continue;  // the same plan as without a cap
+            }
+            TreePlan p;
+            const int rc = plan_tree_pass(nw, cap, nsplit, tree_off, rc_off, depth, ranges, kernel, p, bytes);
+            if (rc == MCB_OK) plans.push_back(std::move(p));
+            else {
+                free_plan(p);
+                if (rc != MCB_E_INVALID) return rc;
+            }
         }
     }
+    int64_t words = 0;
+    for (const TreePlan &p : plans) words = std::max<int64_t>(words, static_cast<int64_t>(p.blocks) * p.huge_words);
+    *scratch = nullptr;
+    if (words) {
+        int rc = dev_alloc(scratch, static_cast<size_t>(words), bytes);
+        if (rc) return rc;
+    }
+    for (TreePlan &p : plans) p.scratch = p.huge_words ? *scratch : nullptr;
     return MCB_OK;
 }
 
@@ -2846,8 +2874,8 @@ int setup_blocked(mcb_solver *s, const int64_t *tree_offset, const uint32_t *par
     if (want < 0 || want > kBlockK) return fail(MCB_E_INVALID, "block_phases must be 0..64");
     if (want == 1) return MCB_OK;
     const std::vector<int64_t> ranges{0, static_cast<int64_t>(s->F)};
-    int rc = make_plans(s->opt.waves_per_block, 1, tree_offset, parent_pos, rc_off, ranges,
-                        reinterpret_cast<const void *>(k_block_q), s->plans, s->bytes);
+    int rc = make_plans(s->opt.waves_per_block, s->opt.tree_cap, 1, tree_offset, parent_pos, rc_off, ranges,
+                        reinterpret_cast<const void *>(k_block_q), s->plans, s->bytes, &s->q_scratch);
     if (rc) return rc;
     if (s->plans.empty()) {
         if (want > 1 || s->opt.waves_per_block)
@@ -2914,6 +2942,7 @@ void mcb_solver_destroy(mcb_solver *s) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (TreePlan &p : s->plans) free_plan(p);
+    if (s->q_scratch) (void)hipFree(s->q_scratch);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -3281,6 +3310,7 @@ struct mcb_batch {
     int32_t *pos_coord = nullptr, *rc_coord = nullptr, *cand_coord = nullptr;
     std::vector<TreePlan> plans;  // tree pass launch plans, ranges = problems
     int32_t plan = 0;
+    uint64_t *q_scratch = nullptr;
     CompInfo *d_comp = nullptr;
     ChunkInfo *d_chunks = nullptr;
     MergeGroup *d_groups = nullptr;
@@ -3460,6 +3490,7 @@ void mcb_batch_destroy(mcb_batch *b) {
     for (void *p : b->allocs)
         if (p) (void)hipFree(p);
     for (TreePlan &p : b->plans) free_plan(p);
+    if (b->q_scratch) (void)hipFree(b->q_scratch);
     for (hipEvent_t e : b->ev_pool) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -3616,8 +3647,9 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
     std::vector<uint32_t> pre, end;
     preorder_intervals(tree_off.data(), parent_pos.data(), static_cast<int32_t>(F_tot), pre, end);
 
-    if ((rc = make_plans(b->opt.waves_per_block, kXcds, tree_off.data(), parent_pos.data(), rc_off.data(), ranges,
-                         reinterpret_cast<const void *>(k_batch_q), b->plans, b->bytes)))
+    if ((rc = make_plans(b->opt.waves_per_block, b->opt.tree_cap, kXcds, tree_off.data(), parent_pos.data(),
+                         rc_off.data(), ranges,
+                         reinterpret_cast<const void *>(k_batch_q), b->plans, b->bytes, &b->q_scratch)))
         return cleanup(rc);
     if (b->plans.empty()) return cleanup(fail(MCB_E_INVALID, "a tree is too large for the blocked form's LDS budget"));
 

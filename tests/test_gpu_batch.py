@@ -138,11 +138,12 @@ def test_batched_components_driver():
         assert r2.stats.final_cycles == c2.cycle_space_dim
 
 
+@pytest.mark.parametrize("cap", [-1, 128])
 @pytest.mark.parametrize("waves", [4, 8, 16])
-def test_batch_tree_pass_widths(waves):
+def test_batch_tree_pass_widths(waves, cap):
     probs = _component_problems(12, 20, 600, 3)
-    print(f"case batch widths={waves} problems={len(probs)}", flush=True)
-    bt = Batch(probs, store_cycles=True, waves_per_block=waves)
+    print(f"case batch widths={waves} cap={cap} problems={len(probs)}", flush=True)
+    bt = Batch(probs, store_cycles=True, waves_per_block=waves, tree_cap=cap)
     assert bt.launch_shape()["waves_per_block"] == waves
     bt.run()
     bt.sync()

@@ -140,16 +140,18 @@ def test_rerun_after_reset(block):
     assert np.array_equal(s.selected(), first)
 
 
+@pytest.mark.parametrize("cap", [-1, 64, 128])
 @pytest.mark.parametrize("waves", [4, 8, 16])
 @pytest.mark.parametrize("name", ["grid16x16", "er150", "delaunay150", "chunglu300", "biconn200"])
-def test_tree_pass_widths(name, waves):
+def test_tree_pass_widths(name, waves, cap):
     """Every width of the blocked tree pass (which trees go whole-workgroup and which one-wave changes
-    with it) gives the oracle's selections and cycle vectors."""
+    with it), with the largest LDS-resident tree as large as fits or capped (larger trees in global
+    scratch), gives the oracle's selections and cycle vectors."""
     fx = load_trace(name)
     p = problem_of(fx, "reduced", keep_vertex_trees=True, prune_trees=True)
-    print(f"case {name} waves={waves}", flush=True)
+    print(f"case {name} waves={waves} cap={cap}", flush=True)
     ref = pyoracle.run(p, want_c=True)
-    s = Solver(p, store_cycles=True, waves_per_block=waves)
+    s = Solver(p, store_cycles=True, waves_per_block=waves, tree_cap=cap)
     assert s.launch_shape()["waves_per_block"] == waves
     s.run()
     s.sync()
