@@ -673,9 +673,21 @@ __global__ void k_copy_u64(const uint64_t *__restrict__ src, uint64_t *__restric
 constexpr int kBlockK = 64;      // phases per block (word width)
 constexpr int kQWaves = 16;      // waves per workgroup of the tree pass
 constexpr int kChunk = 4096;     // ranks per first-level greedy basis (one candidate-bitmap word per lane)
-constexpr int kGroup = 64;       // bases merged per group
-constexpr int kMaxLevels = 8;    // kChunk * kGroup^(kMaxLevels-1) covers 2^32 ranks
-constexpr int kTile = 32;        // support words staged in LDS per update step
+constexpr int kMaxLevels = 8;    // kChunk * 16^(kMaxLevels-1) covers 2^32 ranks
+
+// Bases merged per group: one wave inserts a group's bases one after the other, so a group's time
+// grows with its size, and each level is one launch.  Of 16 and 64, the size that gives fewer levels
+// (16 on a tie).
+inline int merge_group_size(int64_t chunks) {
+    auto levels = [&](int64_t gsz) {
+        int l = 0;
+        for (int64_t n = chunks; n > 1; n = (n + gsz - 1) / gsz) ++l;
+        return l;
+    };
+    return levels(64) < levels(16) ? 64 : 16;
+}
+constexpr int kTile = 16;        // support words per workgroup of the XOR update (table form)
+constexpr int kTileDirect = 32;  // the same, direct form (batched: many small components per step)
 
 struct BlockArgs {
     const int64_t *tree_off;
@@ -723,7 +735,7 @@ struct BlockArgs {
     uint32_t *status;
     unsigned long long *upd_bytes;  // accumulated algorithmic bytes of the update
     int64_t K;
-    int32_t e, kk, N, W, n_big, n_small, big_words, small_words;
+    int32_t e, kk, N, W, n_big, n_small, big_words, small_words, group;
 };
 
 // One block of kk phases of one problem in that problem's own numbering: its support matrix
@@ -1344,7 +1356,7 @@ __device__ __forceinline__ void merge_group(int64_t s0, int members, int64_t slo
     if (lane == 0) sum_cnt[slot] = cnt;
 }
 
-// One merge level: wave g merges bases g*kGroup .. g*kGroup+kGroup-1 of level L into slot g of level
+// One merge level: wave g merges bases g*group .. g*group+group-1 of level L into slot g of level
 // L+1.  At the top level the single wave also finalises the block.  The bases of level L were
 // written by the previous launch.
 constexpr int kMergeWaves = 2;  // waves per workgroup of k_block_merge (16 KB of LDS table each)
@@ -1355,9 +1367,9 @@ __global__ __launch_bounds__(kMergeWaves * kWave) void k_block_merge(BlockArgs a
     const int wv = wave_id();
     const int64_t g = static_cast<int64_t>(blockIdx.x) * kMergeWaves + wv;
     if (g >= a.lvl_n[L + 1]) return;
-    const int64_t rest = a.lvl_n[L] - g * kGroup;
-    const int members = static_cast<int>(rest < kGroup ? rest : kGroup);
-    merge_group(a.lvl_slot[L] + g * kGroup, members, a.lvl_slot[L + 1] + g, a.kk, a.sum_cnt, a.sum_rank, a.sum_q,
+    const int64_t rest = a.lvl_n[L] - g * a.group;
+    const int members = static_cast<int>(rest < a.group ? rest : a.group);
+    merge_group(a.lvl_slot[L] + g * a.group, members, a.lvl_slot[L + 1] + g, a.kk, a.sum_cnt, a.sum_rank, a.sum_q,
                 s_offs[wv], s_tab[wv], lane);
     if (L + 2 == a.levels) {
         __threadfence();
@@ -1639,19 +1651,24 @@ __global__ __launch_bounds__(256) void k_block_dot(BlockArgs a) {
     dot_body(view_of(a), blockIdx.x, lds, &seg_end, a.upd_bytes);
 }
 
-// Tile (x, y): columns e + 256x .. +255, words kTile*y .. +kTile-1.  Column e+i < e+kk receives its
-// final vector S_{e+i}; a later column j with Bc[j] != 0 takes S_j ^= sum_i b_i S_{e+i}.
-// LDS: the block's final vectors on the tile's words, T[t][i] = word w0 + t of S_{e+i}.
-__device__ __forceinline__ void xor_body(const BlkView &v, int64_t xt, int32_t yt, uint64_t *lds,
-                                         unsigned long long *upd_bytes) {
+// The XOR update of tile (x, y) of kTileDirect words, combining the block's final vectors staged in LDS
+// (T[t][i]) directly: 64 masked XORs per word.  Cheaper than the tables of xor_body when a step's tiles
+// hold few columns each (the batched form's small components).
+__device__ __forceinline__ void xor_direct(const BlkView &v, int64_t xt, int32_t yt, uint64_t *lds,
+                                           unsigned long long *upd_bytes) {
     constexpr int kW = 8;
     const int tid = threadIdx.x;
     const int kk = v.kk;
     const int32_t wl = (v.e + kk + kWave - 1) / kWave;
-    const int32_t w0 = yt * kTile;
-    const int tn = wl - w0 < kTile ? wl - w0 : kTile;
-    uint64_t *T = lds;
-    for (int x = tid; x < kBlockK * kTile; x += blockDim.x) {
+    const int32_t w0 = yt * kTileDirect;
+    const int tn = wl - w0 < kTileDirect ? wl - w0 : kTileDirect;
+    {  // nothing to do unless some column of the tile is the block's own or takes an update
+        const int64_t jc = static_cast<int64_t>(v.e) + xt * blockDim.x + tid;
+        const bool need = jc < static_cast<int64_t>(v.e) + kk || (jc < v.N && v.Bc[jc] != 0);
+        if (!__syncthreads_or(need)) return;
+    }
+    uint64_t *T = lds;  // [kTileDirect][64]
+    for (int x = tid; x < kBlockK * kTileDirect; x += blockDim.x) {
         const int t = x / kBlockK, i = x % kBlockK;
         T[x] = (i < kk && t < tn) ? v.Sf_blk[static_cast<int64_t>(w0 + t) * kBlockK + i] : 0ull;
     }
@@ -1677,6 +1694,84 @@ __device__ __forceinline__ void xor_body(const BlkView &v, int64_t xt, int32_t y
                     const uint64_t m = 0ull - ((b >> i) & 1ull);
 #pragma unroll
                     for (int u = 0; u < kW; ++u) xs[u] ^= T[(t0 + u) * kBlockK + i] & m;
+                }
+                uint64_t sv[kW];
+#pragma unroll
+                for (int u = 0; u < kW; ++u) {
+                    if (t0 + u >= tn) xs[u] = 0;
+                    sv[u] = xs[u] ? v.Sw[static_cast<int64_t>(w0 + t0 + u) * v.npad + j] : 0ull;
+                }
+#pragma unroll
+                for (int u = 0; u < kW; ++u)
+                    if (xs[u]) {
+                        v.Sw[static_cast<int64_t>(w0 + t0 + u) * v.npad + j] = sv[u] ^ xs[u];
+                        nbytes += 16;
+                    }
+            }
+        }
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) nbytes += __shfl_down(nbytes, o);
+    if ((tid & (kWave - 1)) == 0 && nbytes) atomicAdd(upd_bytes, static_cast<unsigned long long>(nbytes));
+}
+
+// Tile (x, y): columns e + 256x .. +255, words kTile*y .. +kTile-1.  Column e+i < e+kk receives its
+// final vector S_{e+i}; a later column j with Bc[j] != 0 takes S_j ^= sum_i b_i S_{e+i}.  The sums go
+// through tables in LDS ("four Russians"): Tab[t][g][x] is the XOR of word t of S_{e+4g+k} over the set
+// bits k of the nibble x, so a word of the update is 16 lookups, one per nibble of b.  Thread (t, g)
+// builds table (t, g); a nibble's 16 entries are 128 contiguous bytes, so lookups do not conflict.
+__device__ __forceinline__ void xor_body(const BlkView &v, int64_t xt, int32_t yt, uint64_t *lds,
+                                         unsigned long long *upd_bytes) {
+    static_assert(kTile * 16 == 256, "one table per thread of the workgroup");
+    constexpr int kW = 8;
+    const int tid = threadIdx.x;
+    const int kk = v.kk;
+    const int32_t wl = (v.e + kk + kWave - 1) / kWave;
+    const int32_t w0 = yt * kTile;
+    const int tn = wl - w0 < kTile ? wl - w0 : kTile;
+    {  // nothing to do unless some column of the tile is the block's own or takes an update
+        const int64_t jc = static_cast<int64_t>(v.e) + xt * blockDim.x + tid;
+        const bool need = jc < static_cast<int64_t>(v.e) + kk || (jc < v.N && v.Bc[jc] != 0);
+        if (!__syncthreads_or(need)) return;
+    }
+    uint64_t *Tab = lds;  // [kTile][16][16]
+    {
+        const int t = tid >> 4, g = tid & 15;
+        uint64_t r[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = 4 * g + k;
+            r[k] = (t < tn && i < kk) ? v.Sf_blk[static_cast<int64_t>(w0 + t) * kBlockK + i] : 0ull;
+        }
+        uint64_t e[16];
+        e[0] = 0;
+#pragma unroll
+        for (int x = 1; x < 16; ++x) e[x] = e[x & (x - 1)] ^ r[__builtin_ctz(x)];
+        uint64_t *row = Tab + (t * 16 + g) * 16;
+#pragma unroll
+        for (int x = 0; x < 16; ++x) row[x] = e[x];
+    }
+    __syncthreads();
+    const int64_t j = static_cast<int64_t>(v.e) + xt * blockDim.x + tid;
+    uint64_t nbytes = 0;
+    if (j < static_cast<int64_t>(v.e) + kk) {
+        const int i = static_cast<int>(j - v.e);
+        for (int t = 0; t < tn; ++t) {
+            uint64_t x = Tab[(t * 16 + (i >> 2)) * 16 + (1 << (i & 3))];
+            if ((j >> 6) == w0 + t) x &= ~(1ull << (j & 63));
+            v.Sw[static_cast<int64_t>(w0 + t) * v.npad + j] = x;
+        }
+    } else if (j < v.N) {
+        const uint64_t b = v.Bc[j];
+        if (b) {
+            for (int t0 = 0; t0 < tn; t0 += kW) {
+                uint64_t xs[kW];
+#pragma unroll
+                for (int u = 0; u < kW; ++u) xs[u] = 0;
+#pragma unroll 2
+                for (int g = 0; g < 16; ++g) {
+                    const int nib = static_cast<int>((b >> (4 * g)) & 15ull);
+#pragma unroll
+                    for (int u = 0; u < kW; ++u) xs[u] ^= Tab[((t0 + u) * 16 + g) * 16 + nib];
                 }
                 uint64_t sv[kW];
 #pragma unroll
@@ -1949,7 +2044,7 @@ __global__ __launch_bounds__(256) void k_batch_xor(BatchArgs a) {
     const int32_t t = x - a.xor_cum[g];
     const int32_t ct = (v.N - v.e + 255) / 256;
     const int32_t yt = static_cast<int32_t>(uniform32(static_cast<uint32_t>(t / ct)));
-    xor_body(v, t - yt * ct, yt, lds, a.upd_bytes);
+    xor_direct(v, t - yt * ct, yt, lds, a.upd_bytes);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2339,7 +2434,7 @@ struct mcb_solver {
     uint32_t *pw_list = nullptr, *pw_off = nullptr, *pw_cnt = nullptr, *pair_i = nullptr;
     uint32_t *sum_cnt = nullptr, *sum_rank = nullptr, *D_blk = nullptr, *Dcnt_blk = nullptr;
     unsigned long long *upd_bytes = nullptr;
-    int32_t levels = 0;
+    int32_t levels = 0, group = 16;
     int64_t lvl_slot[kMaxLevels] = {}, lvl_n[kMaxLevels] = {};
     int64_t blk_positions = 0, blk_candidates = 0;  // work of the tree passes since reset
     // timing
@@ -2484,6 +2579,7 @@ BlockArgs block_args(const mcb_solver *s, int32_t e, int32_t kk) {
         a.lvl_n[L] = s->lvl_n[L];
     }
     a.levels = s->levels;
+    a.group = s->group;
     a.sel = s->best;
     a.blk_q = s->blk_q;
     a.blk_m = s->blk_m;
@@ -2551,7 +2647,7 @@ int launch_block(mcb_solver *s, int32_t e, int32_t kk) {
     }
     const int64_t cols = s->N - e;
     hipLaunchKernelGGL(k_block_xor, dim3(static_cast<unsigned>((cols + 255) / 256), static_cast<unsigned>((wl + kTile - 1) / kTile)),
-                       dim3(256), sizeof(uint64_t) * kBlockK * kTile, s->stream, a);
+                       dim3(256), sizeof(uint64_t) * 16 * 16 * kTile, s->stream, a);
     HIP_TRY(hipGetLastError());
     s->blk_positions += s->plans[s->plan].pos_cum.back();
     s->blk_candidates += s->plans[s->plan].cand_cum.back();
@@ -2884,8 +2980,9 @@ int setup_blocked(mcb_solver *s, const int64_t *tree_offset, const uint32_t *par
     }
     s->bk = want > 1 ? want : kBlockK;
     use_plan(s, 0);
-    // greedy-basis levels: kChunk ranks per first-level basis, kGroup bases per merge
+    // greedy-basis levels: kChunk ranks per first-level basis, group bases per merge
     int64_t n = (s->K + kChunk - 1) / kChunk, slots = 0;
+    s->group = merge_group_size(n);
     s->levels = 0;
     for (;;) {
         if (s->levels >= kMaxLevels) return fail(MCB_E_INVALID, "too many candidates for the basis levels");
@@ -2894,7 +2991,7 @@ int setup_blocked(mcb_solver *s, const int64_t *tree_offset, const uint32_t *par
         slots += n;
         ++s->levels;
         if (n == 1) break;
-        n = (n + kGroup - 1) / kGroup;
+        n = (n + s->group - 1) / s->group;
     }
     int64_t &B = s->bytes;
     if ((rc = dev_alloc(&s->Tt, static_cast<size_t>(s->npad), B))) return rc;
@@ -3425,7 +3522,7 @@ BatchArgs batch_args(const mcb_batch *b, int32_t s) {
     }
     a.big_words = tp.big_words;
     a.small_words = tp.small_words;
-    a.xor_wt = (s + 1 + kTile - 1) / kTile;
+    a.xor_wt = (s + 1 + kTileDirect - 1) / kTileDirect;
     return a;
 }
 
@@ -3474,7 +3571,7 @@ int launch_batch_step(mcb_batch *b, int32_t s) {
         HIP_TRY(hipGetLastError());
     }
     hipLaunchKernelGGL(k_batch_xor, dim3(static_cast<unsigned>(b->xor_cum[off + A])), dim3(256),
-                       sizeof(uint64_t) * kBlockK * kTile, b->stream, a);
+                       sizeof(uint64_t) * kBlockK * kTileDirect, b->stream, a);
     HIP_TRY(hipGetLastError());
     b->blk_positions += tp.pos_cum[A];
     b->blk_candidates += tp.cand_cum[A];
@@ -3554,6 +3651,7 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
     b->rbase.resize(G);
     std::vector<int64_t> root_base(G), pos_base(G), cand_base(G), slot_base(G);
     std::vector<std::vector<int64_t>> lvl_n(G), lvl_slot(G);
+    std::vector<int64_t> gsize(G, 16);
     int32_t max_levels = 0;
     for (int32_t g = 0; g < G; ++g) {
         const mcb_solver_desc &d = descs[b->order[g]];
@@ -3574,12 +3672,13 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
         cand_base[g] = K_tot;
         slot_base[g] = slots;
         int64_t n = (d.num_candidates + kChunk - 1) / kChunk;
+        gsize[g] = merge_group_size(n);
         for (;;) {
             lvl_slot[g].push_back(slots);
             lvl_n[g].push_back(n);
             slots += n;
             if (n == 1) break;
-            n = (n + kGroup - 1) / kGroup;
+            n = (n + gsize[g] - 1) / gsize[g];
         }
         c.levels = static_cast<int32_t>(lvl_n[g].size());
         if (c.levels > kMaxLevels) return cleanup(fail(MCB_E_INVALID, "too many candidates for the basis levels"));
@@ -3682,10 +3781,11 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
                 n = static_cast<int32_t>(lvl_n[g][L + 1]);
                 for (int32_t gi = 0; gi < n; ++gi) {
                     MergeGroup mg{};
-                    mg.src = lvl_slot[g][L] + static_cast<int64_t>(gi) * kGroup;
+                    mg.src = lvl_slot[g][L] + static_cast<int64_t>(gi) * gsize[g];
                     mg.dst = lvl_slot[g][L + 1] + gi;
                     mg.comp = g;
-                    mg.members = static_cast<int32_t>(std::min<int64_t>(kGroup, lvl_n[g][L] - static_cast<int64_t>(gi) * kGroup));
+                    mg.members =
+                        static_cast<int32_t>(std::min<int64_t>(gsize[g], lvl_n[g][L] - static_cast<int64_t>(gi) * gsize[g]));
                     groups.push_back(mg);
                 }
             }
@@ -3700,7 +3800,7 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
         while (A < G && b->comps[A].N > s * kBlockK) ++A;
         b->active[s] = A;
         b->step_off[s] = static_cast<int64_t>(b->dot_cum.size());
-        const int64_t wt = (s + 1 + kTile - 1) / kTile;
+        const int64_t wt = (s + 1 + kTileDirect - 1) / kTileDirect;
         int64_t dsum = 0, xsum = 0;
         b->dot_cum.push_back(0);
         b->xor_cum.push_back(0);
