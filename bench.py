@@ -105,6 +105,10 @@ def main() -> int:
     ap.add_argument("--workload", default="er20000", choices=sorted(WORKLOADS) + sorted(COMPONENT_WORKLOADS))
     ap.add_argument("--no-plain", action="store_true",
                     help="component workloads: skip the plain one-solver-per-component timing")
+    ap.add_argument("--tree-waves", type=int, default=0,
+                    help="tree pass waves per workgroup (4, 8, 16; 0 timed at create)")
+    ap.add_argument("--tree-cap", type=int, default=0,
+                    help="largest tree kept in LDS by the tree pass (0 timed at create, -1 as large as fits)")
     ap.add_argument("--fvs", choices=["plain", "learned"], default="plain",
                     help="single graphs: FVS construction (learned: the MLP-ordered greedy; the plain-FVS loop "
                          "is then timed beside it and the weights compared)")
@@ -156,7 +160,8 @@ def main() -> int:
           f"F={info.fvs_size} K={info.num_candidates} positions={info.positions} host={host_s:.2f}s",
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    solver = Solver(prob, timing=1 if args.breakdown else 2, block_phases=args.block_phases)
+    solver = Solver(prob, timing=1 if args.breakdown else 2, block_phases=args.block_phases,
+                    waves_per_block=args.tree_waves, tree_cap=args.tree_cap)
     upload_s = time.perf_counter() - t0
     dev = device_info()
     phases = args.phases if 0 < args.phases < info.num_non_tree else None
@@ -352,7 +357,7 @@ def run_components(args) -> int:
           file=sys.stderr, flush=True)
     dev = device_info()
     t0 = time.perf_counter()
-    bt = Batch(loop, timing=1 if args.breakdown else 2)
+    bt = Batch(loop, timing=1 if args.breakdown else 2, waves_per_block=args.tree_waves, tree_cap=args.tree_cap)
     upload_s = time.perf_counter() - t0
     print(f"[bench] batch arena {bt.device_bytes / 2**30:.2f} GiB, {bt.steps} steps, upload {upload_s:.2f}s",
           file=sys.stderr, flush=True)
