@@ -112,6 +112,8 @@ def main() -> int:
     ap.add_argument("--fvs", choices=["plain", "learned"], default="plain",
                     help="single graphs: FVS construction (learned: the MLP-ordered greedy; the plain-FVS loop "
                          "is then timed beside it and the weights compared)")
+    ap.add_argument("--graph", action="store_true",
+                    help="also time the loop with no HIP events: stream-launched, and replayed as one HIP graph")
     ap.add_argument("--no-prune", action="store_true")
     ap.add_argument("--breakdown", action="store_true",
                     help="time every kernel (HIP events between all launches), not only the dominant one")
@@ -265,6 +267,11 @@ def main() -> int:
         },
     }
     solver.close()
+    if args.graph:
+        out["untimed"] = untimed_legs(
+            lambda: Solver(prob, block_phases=args.block_phases, waves_per_block=args.tree_waves,
+                           tree_cap=args.tree_cap),
+            args.steps, lambda o: np.array_equal(o.selected(), sel))
     if fvs_info is not None:
         # the same loop with the reference FVS order, for the comparison and the weight check
         t0 = time.perf_counter()
@@ -294,6 +301,32 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(n, u, v, w, args.cpu_budget, threads)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def untimed_legs(make, steps: int, want) -> dict:
+    """Loop times with no HIP events around any kernel: stream-launched (run) and as one HIP graph replay
+    (run_graph).  `make()` builds a fresh solver or batch without timing; `want(obj)` checks its results
+    against the timed run's.  Each leg runs once untimed first (the graph leg captures there)."""
+    out = {}
+    for leg in ("stream", "graph"):
+        obj = make()
+        go = obj.run_graph if leg == "graph" else obj.run
+        go()
+        obj.sync()
+        t = 0.0
+        for _ in range(steps):
+            obj.reset()
+            t0 = time.perf_counter()
+            go()
+            obj.sync()
+            t += time.perf_counter() - t0
+        if not want(obj):
+            obj.close()
+            raise RuntimeError(f"{leg} leg: results differ from the timed run")
+        obj.close()
+        out[f"{leg}_untimed_loop_s"] = round(t / steps, 6)
+        print(f"[bench] {leg} untimed: {t / steps:.4f} s", file=sys.stderr, flush=True)
+    return out
 
 
 def cpu_baseline_components(comps, work, budget_s: float, threads: int) -> dict:
@@ -391,6 +424,11 @@ def run_components(args) -> int:
     total_weight += sum(int(p.info.input_total_weight) for p in probs if p.info.single_cycle)
     shape = bt.launch_shape()
     bt.close()
+    untimed = None
+    if args.graph:
+        untimed = untimed_legs(
+            lambda: Batch(loop, waves_per_block=args.tree_waves, tree_cap=args.tree_cap), args.steps,
+            lambda o: all(np.array_equal(o.selected(j), sels[j]) for j in range(len(loop))))
 
     plain_s = None
     if not args.no_plain:
@@ -461,6 +499,8 @@ def run_components(args) -> int:
             "avg_launch_us": round(avg_ms * 1000.0, 3),
         },
     }
+    if untimed is not None:
+        out["untimed"] = untimed
     work = [0.0 if p.info.single_cycle else float(p.info.num_non_tree) * p.info.fvs_size * p.info.nodes
             for p in probs]
     for p in probs:

@@ -141,6 +141,10 @@ int mcb_solver_reset(mcb_solver *s);
 /* Enqueues the next `phases` phases on the solver's stream (asynchronous). */
 int mcb_solver_run(mcb_solver *s, int32_t phases);
 
+/* The whole loop (phase 0 to N) as one HIP graph: captured at the first call, replayed after each
+ * reset.  Needs phase 0 and timing off.  Same results as mcb_solver_run(s, N). */
+int mcb_solver_run_graph(mcb_solver *s);
+
 /* Waits for the stream; returns MCB_E_NOT_FOUND if a phase found no odd candidate. */
 int mcb_solver_sync(mcb_solver *s);
 
@@ -221,6 +225,10 @@ int32_t mcb_batch_step(const mcb_batch *b);
 
 /* Enqueues the next `steps` steps (fewer at the end). */
 int mcb_batch_run(mcb_batch *b, int32_t steps);
+
+/* Every step as one HIP graph (captured at the first call, replayed after each reset); needs step 0
+ * and timing off. */
+int mcb_batch_run_graph(mcb_batch *b);
 
 /* Waits; MCB_E_NOT_FOUND if some phase of some problem found no odd candidate. */
 int mcb_batch_sync(mcb_batch *b);

@@ -57,6 +57,7 @@ SIGNATURES = {
     "mcb_solver_reset": (ctypes.c_int, [P]),
     "mcb_solver_run": (ctypes.c_int, [P, i32]),
     "mcb_solver_sync": (ctypes.c_int, [P]),
+    "mcb_solver_run_graph": (ctypes.c_int, [P]),
     "mcb_solver_phase": (i32, [P]),
     "mcb_solver_selected": (ctypes.c_int, [P, P, i32]),
     "mcb_solver_support_vectors": (ctypes.c_int, [P, P]),
@@ -74,6 +75,7 @@ SIGNATURES = {
     "mcb_batch_step": (i32, [P]),
     "mcb_batch_run": (ctypes.c_int, [P, i32]),
     "mcb_batch_sync": (ctypes.c_int, [P]),
+    "mcb_batch_run_graph": (ctypes.c_int, [P]),
     "mcb_batch_selected": (ctypes.c_int, [P, i32, P, i32]),
     "mcb_batch_cycle_vectors": (ctypes.c_int, [P, i32, P, i32]),
     "mcb_batch_kernel_time": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]),

@@ -2429,6 +2429,9 @@ struct mcb_solver {
     std::vector<TreePlan> plans;
     int32_t plan = 0;
     uint64_t *q_scratch = nullptr;  // path words of trees beyond a plan's LDS cap
+    hipGraph_t graph = nullptr;     // the whole loop, captured by mcb_solver_run_graph
+    hipGraphExec_t graph_exec = nullptr;
+    int64_t graph_positions = 0, graph_candidates = 0;  // tree pass work of one replay
     uint64_t *Tt = nullptr, *Qr = nullptr, *qbits = nullptr, *sum_q = nullptr, *blk_q = nullptr, *blk_m = nullptr, *blk_g = nullptr;
     uint64_t *C_blk = nullptr, *Sf_blk = nullptr, *Ct = nullptr, *Bc = nullptr, *pair_c = nullptr;
     uint32_t *pw_list = nullptr, *pw_off = nullptr, *pw_cnt = nullptr, *pair_i = nullptr;
@@ -3040,6 +3043,8 @@ void mcb_solver_destroy(mcb_solver *s) {
         if (p) (void)hipFree(p);
     for (TreePlan &p : s->plans) free_plan(p);
     if (s->q_scratch) (void)hipFree(s->q_scratch);
+    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -3236,6 +3241,37 @@ int mcb_solver_run(mcb_solver *s, int32_t phases) {
     return MCB_OK;
 }
 
+int mcb_solver_run_graph(mcb_solver *s) {
+    if (!s) return fail(MCB_E_INVALID, "invalid arguments");
+    if (s->phase != 0) return fail(MCB_E_STATE, "the graph runs the whole loop from phase 0 (reset first)");
+    if (s->opt.timing) return fail(MCB_E_STATE, "graph replay needs timing off");
+    if (!s->graph_exec) {
+        HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        int rc = MCB_OK;
+        for (int32_t e = 0; e < s->N && rc == MCB_OK;) {
+            const int32_t kk = std::min(s->bk, s->N - e);
+            rc = s->bk > 1 ? launch_block(s, e, kk) : launch_phase(s, e);
+            e += kk;
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(s->stream, &g);
+        if (rc || ec != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : fail(MCB_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        }
+        s->graph = g;
+        HIP_TRY(hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0));
+        // the counters were zero at phase 0; capture ran the host side of every launch once
+        s->graph_positions = s->blk_positions;
+        s->graph_candidates = s->blk_candidates;
+    }
+    HIP_TRY(hipGraphLaunch(s->graph_exec, s->stream));
+    s->blk_positions = s->graph_positions;
+    s->blk_candidates = s->graph_candidates;
+    s->phase = s->N;
+    return MCB_OK;
+}
+
 int mcb_solver_sync(mcb_solver *s) {
     if (!s) return fail(MCB_E_INVALID, "invalid arguments");
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -3408,6 +3444,9 @@ struct mcb_batch {
     std::vector<TreePlan> plans;  // tree pass launch plans, ranges = problems
     int32_t plan = 0;
     uint64_t *q_scratch = nullptr;
+    hipGraph_t graph = nullptr;   // every step, captured by mcb_batch_run_graph
+    hipGraphExec_t graph_exec = nullptr;
+    int64_t graph_positions = 0, graph_candidates = 0;
     CompInfo *d_comp = nullptr;
     ChunkInfo *d_chunks = nullptr;
     MergeGroup *d_groups = nullptr;
@@ -3588,6 +3627,8 @@ void mcb_batch_destroy(mcb_batch *b) {
         if (p) (void)hipFree(p);
     for (TreePlan &p : b->plans) free_plan(p);
     if (b->q_scratch) (void)hipFree(b->q_scratch);
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
     for (hipEvent_t e : b->ev_pool) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -3883,6 +3924,33 @@ int mcb_batch_run(mcb_batch *b, int32_t steps) {
         if (rc) return rc;
         ++b->step;
     }
+    return MCB_OK;
+}
+
+int mcb_batch_run_graph(mcb_batch *b) {
+    if (!b) return fail(MCB_E_INVALID, "invalid arguments");
+    if (b->step != 0) return fail(MCB_E_STATE, "the graph runs every step from step 0 (reset first)");
+    if (b->opt.timing) return fail(MCB_E_STATE, "graph replay needs timing off");
+    if (!b->graph_exec) {
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int rc = MCB_OK;
+        for (int32_t st = 0; st < b->steps && rc == MCB_OK; ++st) rc = launch_batch_step(b, st);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+        if (rc || ec != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : fail(MCB_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        }
+        b->graph = g;
+        HIP_TRY(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+        // the counters were zero at phase 0; capture ran the host side of every launch once
+        b->graph_positions = b->blk_positions;
+        b->graph_candidates = b->blk_candidates;
+    }
+    HIP_TRY(hipGraphLaunch(b->graph_exec, b->stream));
+    b->blk_positions = b->graph_positions;
+    b->blk_candidates = b->graph_candidates;
+    b->step = b->steps;
     return MCB_OK;
 }
 

@@ -96,6 +96,10 @@ class Solver:
         n = self.N if phases is None else int(phases)
         _lib.check_hip(_lib.hip().mcb_solver_run(self._h, n))
 
+    def run_graph(self) -> None:
+        """The whole loop from phase 0 as one HIP graph (captured once, replayed after each reset; timing off)."""
+        _lib.check_hip(_lib.hip().mcb_solver_run_graph(self._h))
+
     def sync(self) -> None:
         _lib.check_hip(_lib.hip().mcb_solver_sync(self._h))
 
@@ -214,6 +218,10 @@ class Batch:
         """Enqueue the next ``steps`` steps (all remaining when None)."""
         n = self.steps if steps is None else int(steps)
         _lib.check_hip(_lib.hip().mcb_batch_run(self._h, n))
+
+    def run_graph(self) -> None:
+        """Every step from step 0 as one HIP graph (captured once, replayed after each reset; timing off)."""
+        _lib.check_hip(_lib.hip().mcb_batch_run_graph(self._h))
 
     def sync(self) -> None:
         _lib.check_hip(_lib.hip().mcb_batch_sync(self._h))
