@@ -408,42 +408,116 @@ struct HeapLess {  // include/Dijkstra.h:37-43: the heap's top is the least dist
     }
 };
 
+// One vertex of a shortest-path search, kept together because the relax loop reads them together.
+struct VertexState {
+    int32_t dist;    // -1: not reached
+    int32_t pedge;   // offset of the edge parent -> vertex (-1 for the root and unreached vertices)
+    int32_t level;   // hop depth; set when the vertex is finalised
+    uint32_t branch; // the child of the root whose subtree holds the vertex (~0 for the root)
+    uint8_t in_tree, blocked, tent_blocked, is_root;  // is_root: in the FVS (fixed per problem)
+};
+
 struct SpTree {
-    std::vector<int32_t> dist, level, pedge;
-    std::vector<uint8_t> in_tree;
+    std::vector<VertexState> v;
     std::vector<uint32_t> order;  // vertices in finalisation order, root excluded
 };
 
-// dijkstra::dijkstra_sp (include/Dijkstra.h:78-127): lazy-deletion binary heap, strict '<' relax.
-void dijkstra(const Multigraph &g, uint32_t src, SpTree &t) {
-    const int32_t n = g.n;
-    t.dist.assign(n, -1);
-    t.level.assign(n, -1);
-    t.pedge.assign(n, -1);
-    t.in_tree.assign(n, 0);
-    t.order.clear();
-    t.dist[src] = 0;
-    t.level[src] = 0;
-    std::priority_queue<std::pair<int, int>, std::vector<std::pair<int, int>>, HeapLess> pq;
-    pq.push(std::make_pair(static_cast<int>(src), 0));
-    while (!pq.empty()) {
-        std::pair<int, int> val = pq.top();
-        pq.pop();
-        const uint32_t u = static_cast<uint32_t>(val.first);
-        if (t.in_tree[u]) continue;
-        if (u != src) t.order.push_back(u);
-        t.in_tree[u] = 1;
-        for (uint32_t off = g.ro[u]; off < g.ro[u + 1]; ++off) {
-            const uint32_t c = g.cols[off];
-            if (t.in_tree[c]) continue;
-            const int32_t cand = t.dist[u] + g.w[off];
-            if (t.dist[c] == -1 || cand < t.dist[c]) {
-                t.dist[c] = cand;
-                pq.push(std::make_pair(static_cast<int>(c), cand));
-                t.pedge[c] = static_cast<int32_t>(off);
-                t.level[c] = t.level[u] + 1;
+// Per-thread scratch of the root loop.  Every per-vertex entry holds its idle value (dist, pedge,
+// level -1; flags 0; branch, pos_of ~0) except at the vertices listed in `touched`, which the next
+// root resets, so a root costs time in the part of the graph its search reached.
+struct Workspace {
+    SpTree t;
+    std::vector<uint8_t> keep;
+    std::vector<uint32_t> pos_of, touched, open_rows, level_count, level_start;
+    std::vector<std::pair<int, int>> heap;
+    int32_t n = -1;
+
+    void prepare(int32_t nodes, const int32_t *fvs_rank) {
+        if (n != nodes) {
+            n = nodes;
+            t.v.assign(n, VertexState{-1, -1, -1, 0xFFFFFFFFu, 0, 0, 0, 0});
+            for (int32_t i = 0; i < n; ++i) t.v[i].is_root = fvs_rank[i] >= 0 ? 1 : 0;
+            keep.assign(n, 0);
+            pos_of.assign(n, 0xFFFFFFFFu);
+        } else {
+            for (uint32_t x : touched) {
+                VertexState &s = t.v[x];
+                s.dist = s.pedge = s.level = -1;
+                s.branch = 0xFFFFFFFFu;
+                s.in_tree = s.blocked = s.tent_blocked = 0;
+                keep[x] = 0;
+                pos_of[x] = 0xFFFFFFFFu;
             }
         }
+        touched.clear();
+        t.order.clear();
+    }
+};
+
+// dijkstra::dijkstra_sp (include/Dijkstra.h:78-127): lazy-deletion binary heap, strict '<' relax.
+// The heap is std::push_heap / std::pop_heap over a vector with the reference's comparator, the
+// operations std::priority_queue is specified to perform, on a vector reused across roots.  At each
+// pop the vertex's parent is final, so blocked (the tree path from src holds a root of smaller id
+// than src: include/Dijkstra.h:205-254 rejects every cycle through such a vertex), branch and level
+// are set there.
+//
+// stop_when_closed: stop as soon as every reached, unfinalised vertex is blocked through its current
+// parent.  From then on every vertex finalised is blocked (its final parent is finalised before it,
+// and by induction blocked), so no later vertex can be a candidate endpoint or lie on the path of
+// one.  The pops up to that point are the full search's pops (same heap, same sequence), so the
+// finalised part of the tree, its order and its distances are the full search's.
+void dijkstra(const Multigraph &g, uint32_t src, bool stop_when_closed, Workspace &ws) {
+    VertexState *V = ws.t.v.data();
+    const uint32_t *ro = g.ro.data(), *cols = g.cols.data(), *rows = g.rows.data();
+    const int32_t *wt = g.w.data();
+    V[src].dist = 0;
+    V[src].level = 0;
+    ws.touched.push_back(src);
+    int64_t open = 1;  // reached, unfinalised vertices whose current path is not blocked
+    std::vector<std::pair<int, int>> &heap = ws.heap;
+    heap.clear();
+    const HeapLess less;
+    heap.push_back(std::make_pair(static_cast<int>(src), 0));
+    std::push_heap(heap.begin(), heap.end(), less);
+    while (!heap.empty()) {
+        const uint32_t u = static_cast<uint32_t>(heap.front().first);
+        std::pop_heap(heap.begin(), heap.end(), less);
+        heap.pop_back();
+        VertexState &su = V[u];
+        if (su.in_tree) continue;
+        su.in_tree = 1;
+        if (u != src) {
+            ws.t.order.push_back(u);
+            const uint32_t p = rows[su.pedge];
+            const VertexState &sp = V[p];
+            su.blocked = (sp.blocked || (su.is_root && u < src)) ? 1 : 0;
+            su.branch = (p == src) ? u : sp.branch;
+            su.level = sp.level + 1;
+        }
+        const bool bu = su.blocked != 0;
+        if (!bu) --open;
+        const int32_t du = su.dist;
+        for (uint32_t off = ro[u]; off < ro[u + 1]; ++off) {
+            const uint32_t c = cols[off];
+            VertexState &sc = V[c];
+            if (sc.in_tree) continue;
+            const int32_t cand = du + wt[off];
+            if (sc.dist == -1 || cand < sc.dist) {
+                const uint8_t nb = (bu || (sc.is_root && c < src)) ? 1 : 0;
+                if (sc.dist == -1) {
+                    ws.touched.push_back(c);
+                    open += !nb;
+                } else {
+                    open += static_cast<int64_t>(!nb) - static_cast<int64_t>(!sc.tent_blocked);
+                }
+                sc.tent_blocked = nb;
+                sc.dist = cand;
+                sc.pedge = static_cast<int32_t>(off);
+                heap.push_back(std::make_pair(static_cast<int>(c), cand));
+                std::push_heap(heap.begin(), heap.end(), less);
+            }
+        }
+        if (stop_when_closed && open == 0) break;
     }
 }
 
@@ -459,13 +533,6 @@ struct RootOutput {
     int32_t depth = 0;
 };
 
-struct Workspace {
-    SpTree t;
-    std::vector<uint8_t> blocked, keep, is_tree_edge;
-    std::vector<uint32_t> branch, pos_of, nt_list, level_count, level_start;
-    std::vector<uint32_t> cand_list;  // edge offsets accepted, in SP non-tree order
-    std::vector<int32_t> cand_w;
-};
 
 }  // namespace
 
@@ -486,59 +553,49 @@ struct mcb_problem {
 namespace {
 
 // One root: Dijkstra, SP non-tree edges (include/Dijkstra.h:129-149), the candidate filter of
-// include/Dijkstra.h:205-254 and the grouping of include/cycle_searcher.h:45-53.
+// include/Dijkstra.h:205-254 and the grouping of include/cycle_searcher.h:45-53.  The pruned layout
+// without per-vertex parent rows stops the search once the rest of the graph is blocked (see
+// dijkstra); the unpruned layout and the parent rows need the whole tree.
 void process_root(const mcb_problem &P, uint32_t src, bool prune, Workspace &ws, RootOutput &out,
                   int32_t *vertex_parent_row) {
     const Multigraph &g = P.g;
     const int32_t n = g.n;
     SpTree &t = ws.t;
-    dijkstra(g, src, t);
+    ws.prepare(n, P.fvs_rank.data());
+    dijkstra(g, src, prune && !vertex_parent_row, ws);
+    const VertexState *V = t.v.data();
     if (vertex_parent_row)
-        for (int32_t v = 0; v < n; ++v) vertex_parent_row[v] = t.pedge[v];
+        for (int32_t v = 0; v < n; ++v) vertex_parent_row[v] = V[v].pedge;
 
-    // SP non-tree edges: first direction met in offset order.
-    const size_t m2 = g.rows.size();
-    ws.is_tree_edge.assign(m2, 0);
-    for (uint32_t v : t.order) ws.is_tree_edge[t.pedge[v]] = 1;
-    ws.nt_list.clear();
-    for (size_t i = 0; i < m2; ++i) {
-        if (ws.is_tree_edge[i] == 1) continue;
-        const uint8_t r = ws.is_tree_edge[g.rev[i]];
-        if (r == 1) {
-            ws.is_tree_edge[i] = 1;
-        } else if (r == 2) {
-            ws.is_tree_edge[i] = 2;
-        } else {
-            ws.is_tree_edge[i] = 2;
-            ws.nt_list.push_back(static_cast<uint32_t>(i));
-        }
-    }
-
-    // Top-down: blocked[x] = the path root..x holds a root vertex of smaller id than src;
-    // branch[x] = the child of src whose subtree holds x.
-    ws.blocked.assign(n, 0);
-    ws.branch.assign(n, 0xFFFFFFFFu);
-    for (uint32_t v : t.order) {
-        const uint32_t p = g.rows[t.pedge[v]];
-        ws.blocked[v] = ws.blocked[p] || (P.fvs_rank[v] >= 0 && v < src);
-        ws.branch[v] = (p == src) ? v : ws.branch[p];
-    }
-
-    // Accept (u,v): the tree paths from u and v meet first at src and avoid smaller roots.
+    // Accept (u,v): the tree paths from u and v meet first at src and avoid smaller roots.  The
+    // reference lists the SP non-tree edges in offset order, each at the first of its two directions
+    // (include/Dijkstra.h:135-148), and filters that list.  An accepted edge has both ends finalised
+    // and unblocked, and its first direction lies in the row of its smaller end, so scanning the rows
+    // of the finalised unblocked vertices in ascending id, and in each row the edges to a larger (or,
+    // for a loop, the same) vertex, meets the accepted edges in the same order.  An edge is a tree
+    // edge when it is the parent edge of either end.
+    ws.open_rows.clear();
+    ws.open_rows.push_back(src);
+    for (uint32_t v : t.order)
+        if (!V[v].blocked) ws.open_rows.push_back(v);
+    std::sort(ws.open_rows.begin(), ws.open_rows.end());
     std::unordered_map<unsigned long long, uint32_t> groups_of;
     std::vector<std::vector<uint32_t>> groups;
-    for (uint32_t e : ws.nt_list) {
-        const uint32_t u = g.rows[e], v = g.cols[e];
-        if (ws.blocked[u] || ws.blocked[v]) continue;
-        if (!(u == src || v == src || ws.branch[u] != ws.branch[v])) continue;
-        const unsigned long long key =
-            (static_cast<unsigned long long>(std::min(u, v)) << 32) | std::max(u, v);
-        auto it = groups_of.find(key);
-        if (it == groups_of.end()) {
-            groups_of.insert(std::make_pair(key, static_cast<uint32_t>(groups.size())));
-            groups.emplace_back(1, e);
-        } else {
-            groups[it->second].push_back(e);
+    for (uint32_t u : ws.open_rows) {
+        for (uint32_t e = g.ro[u]; e < g.ro[u + 1]; ++e) {
+            const uint32_t v = g.cols[e];
+            if (v < u || (v == u && e > g.rev[e])) continue;  // met from the other direction
+            if (!V[v].in_tree || V[v].blocked) continue;
+            if (V[v].pedge == static_cast<int32_t>(e) || V[u].pedge == static_cast<int32_t>(g.rev[e])) continue;
+            if (!(u == src || v == src || V[u].branch != V[v].branch)) continue;
+            const unsigned long long key = (static_cast<unsigned long long>(u) << 32) | v;
+            auto it = groups_of.find(key);
+            if (it == groups_of.end()) {
+                groups_of.insert(std::make_pair(key, static_cast<uint32_t>(groups.size())));
+                groups.emplace_back(1, e);
+            } else {
+                groups[it->second].push_back(e);
+            }
         }
     }
     out.cand_edge.clear();
@@ -546,19 +603,20 @@ void process_root(const mcb_problem &P, uint32_t src, bool prune, Workspace &ws,
     for (auto &kv : groups_of)
         for (uint32_t e : groups[kv.second]) {
             out.cand_edge.push_back(e);
-            out.cand_weight.push_back(t.dist[g.rows[e]] + t.dist[g.cols[e]] + g.w[e]);
+            out.cand_weight.push_back(V[g.rows[e]].dist + V[g.cols[e]].dist + g.w[e]);
         }
 
     // Which vertices get a position: all of them, or the closure of the candidate endpoints
     // under the parent relation (every ancestor of a kept vertex is kept).
-    ws.keep.assign(n, prune ? 0 : 1);
+    if (!prune)
+        for (int32_t v = 0; v < n; ++v) ws.keep[v] = 1;
     ws.keep[src] = 1;
     if (prune) {
         for (uint32_t e : out.cand_edge) {
             for (uint32_t x : {g.rows[e], g.cols[e]}) {
                 while (!ws.keep[x]) {
                     ws.keep[x] = 1;
-                    x = g.rows[t.pedge[x]];
+                    x = g.rows[V[x].pedge];
                 }
             }
         }
@@ -566,25 +624,24 @@ void process_root(const mcb_problem &P, uint32_t src, bool prune, Workspace &ws,
     // BFS-level layout: level buckets, finalisation order inside a level (parents come first).
     int32_t maxlev = 0;
     for (uint32_t v : t.order)
-        if (ws.keep[v]) maxlev = std::max(maxlev, t.level[v]);
+        if (ws.keep[v]) maxlev = std::max(maxlev, V[v].level);
     ws.level_count.assign(maxlev + 2, 0);
     ws.level_count[0] = 1;
     for (uint32_t v : t.order)
-        if (ws.keep[v]) ws.level_count[t.level[v]]++;
+        if (ws.keep[v]) ws.level_count[V[v].level]++;
     ws.level_start.assign(maxlev + 2, 0);
     for (int32_t l = 1; l <= maxlev + 1; ++l) ws.level_start[l] = ws.level_start[l - 1] + ws.level_count[l - 1];
     const uint32_t npos = ws.level_start[maxlev + 1];
-    ws.pos_of.assign(n, 0xFFFFFFFFu);
     ws.pos_of[src] = 0;
     ws.level_start[0] = 1;
     for (uint32_t v : t.order)
-        if (ws.keep[v]) ws.pos_of[v] = ws.level_start[t.level[v]]++;
+        if (ws.keep[v]) ws.pos_of[v] = ws.level_start[V[v].level]++;
     out.parent_pos.assign(npos, 0xFFFFFFFFu);
     out.coord.assign(npos, -1);
     for (uint32_t v : t.order) {
         if (!ws.keep[v]) continue;
         const uint32_t pos = ws.pos_of[v];
-        const uint32_t e = static_cast<uint32_t>(t.pedge[v]);
+        const uint32_t e = static_cast<uint32_t>(V[v].pedge);
         out.parent_pos[pos] = ws.pos_of[g.rows[e]];
         out.coord[pos] = P.nt_map[e];
     }
@@ -1134,12 +1191,14 @@ int mcb_problem_reference_layout(const mcb_problem *p, int32_t r, uint32_t *rows
     const Multigraph &g = p->g;
     const int32_t n = g.n;
     const uint32_t src = p->roots[r];
-    SpTree t;
-    dijkstra(g, src, t);
+    Workspace ws;
+    ws.prepare(n, p->fvs_rank.data());
+    dijkstra(g, src, false, ws);
+    const SpTree &t = ws.t;
     std::vector<LevelSorter> edges;
     edges.reserve(n);
     edges.push_back({-1, 0});
-    for (uint32_t v : t.order) edges.push_back({t.pedge[v], t.level[v]});
+    for (uint32_t v : t.order) edges.push_back({t.v[v].pedge, t.v[v].level});
     std::sort(edges.begin(), edges.end(),
               [](const LevelSorter &a, const LevelSorter &b) { return a.level < b.level; });
     std::memset(rows, 0, sizeof(uint32_t) * (n + 1));

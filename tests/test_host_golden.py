@@ -34,12 +34,15 @@ def test_reduced_graph_and_coordinates(name, mode):
     assert np.count_nonzero(ntm >= 0) == 2 * info.num_non_tree
 
 
+@pytest.mark.parametrize("prune", [False, True])
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("name", trace_names())
-def test_candidate_ranking(name, mode):
+def test_candidate_ranking(name, mode, prune):
+    """The ranked candidates, from the whole shortest-path trees and from the searches that stop once
+    the rest of the graph is blocked (the pruned layout)."""
     fx = load_trace(name)
     ref = fx[mode]
-    p = problem_of(fx, mode)
+    p = problem_of(fx, mode, prune_trees=prune)
     # mcb/Test.cpp prints no count line; the listed candidates are the count there
     assert p.info.num_candidates == ref.get("num_candidates", len(ref["candidates"])) == len(ref["candidates"])
     r, a, b, w = p.candidate_table()
@@ -85,3 +88,31 @@ def test_pruned_layout_keeps_candidates(name):
         seg = par[to[rt]:to[rt + 1]]
         assert seg[0] == 0xFFFFFFFF
         assert np.all(seg[1:] < np.arange(1, seg.size))
+
+
+def _stop_cases():
+    from minimumcyclebasis_amd import generators as G
+    return {
+        "delaunay3000": lambda: G.delaunay(3000, seed=5),          # small integer weights: many distance ties
+        "grid40x30": lambda: G.grid(40, 30, 5, seed=6),            # weights 1..5: ties everywhere
+        "er3000": lambda: G.erdos_renyi_lbc(3000, 9000, 1000, seed=7),
+        "chunglu4000": lambda: G.chung_lu_lbc(4000, 2.3, 8.0, 10000, seed=8),
+        "grid_unit": lambda: G.grid(25, 25, 1, seed=9),             # all weights equal
+    }
+
+
+@pytest.mark.parametrize("case", ["delaunay3000", "grid40x30", "er3000", "chunglu4000", "grid_unit"])
+@pytest.mark.parametrize("reduce", [True, False])
+def test_stopped_searches_give_the_same_problem(case, reduce):
+    """The pruned layout is built from searches that stop once every unfinished vertex is blocked;
+    with per-vertex parent rows requested the searches run to the end.  Every array of the two
+    problems agrees: candidates, their ranks and weights, and the position layouts."""
+    gr = _stop_cases()[case]()
+    full = Problem.from_edges(*gr, reduce_degree_two=reduce, prune_trees=True, keep_vertex_trees=True, num_threads=4)
+    stop = Problem.from_edges(*gr, reduce_degree_two=reduce, prune_trees=True, num_threads=4)
+    assert stop.info.num_candidates == full.info.num_candidates > 0
+    for key in ("cand_root", "cand_edge", "cand_weight", "cand_pos_u", "cand_pos_v", "cand_coord", "tree_offset",
+                "parent_pos", "pos_coord", "root_cand_offset", "root_cand_rank"):
+        assert np.array_equal(stop.array(key), full.array(key)), key
+    full.close()
+    stop.close()
