@@ -408,6 +408,73 @@ struct HeapLess {  // include/Dijkstra.h:37-43: the heap's top is the least dist
     }
 };
 
+// The binary heap of the search, as std::priority_queue keeps it (a max-heap under the comparator
+// "larger distance is less", so the top holds the least distance), with its moves written out so
+// that the child choice is branch-free.  The element moves are those of std::push_heap /
+// std::pop_heap in libstdc++, which fix the pop order among equal distances: push sifts the new
+// element up while its parent is strictly less (larger distance); pop moves the last element out,
+// walks the hole from the top to a leaf always into the greater child (the right one unless it is
+// strictly less than the left; a lone last child at an even length), and sifts the moved element up
+// from there.  The full-tree searches keep std::priority_queue itself, and the tests require both to
+// give the same trees.
+struct SearchHeap {
+    std::vector<std::pair<int, int>> a;  // (vertex, distance)
+
+    void clear() { a.clear(); }
+    bool empty() const { return a.empty(); }
+    int top_vertex() const { return a[0].first; }
+    static bool less(const std::pair<int, int> &x, const std::pair<int, int> &y) { return x.second > y.second; }
+
+    void sift_up(size_t hole, size_t top, std::pair<int, int> v) {
+        while (hole > top) {
+            const size_t parent = (hole - 1) / 2;
+            if (!less(a[parent], v)) break;
+            a[hole] = a[parent];
+            hole = parent;
+        }
+        a[hole] = v;
+    }
+    void push(int vertex, int dist) {
+        a.emplace_back(vertex, dist);
+        sift_up(a.size() - 1, 0, a.back());
+    }
+    void pop() {
+        const size_t len = a.size() - 1;  // the heap after removing the top
+        if (len == 0) {
+            a.pop_back();
+            return;
+        }
+        const std::pair<int, int> v = a[len];
+        a[len] = a[0];  // the old top goes to the end, then off
+        size_t hole = 0, child = 0;
+        if (len > 2) {
+            const size_t lim = (len - 1) / 2;
+            while (child < lim) {
+                child = 2 * (child + 1);
+                child -= static_cast<size_t>(a[child].second > a[child - 1].second);
+                a[hole] = a[child];
+                hole = child;
+            }
+        }
+        if ((len & 1) == 0 && child == (len - 2) / 2) {
+            child = 2 * (child + 1);
+            a[hole] = a[child - 1];
+            hole = child - 1;
+        }
+        sift_up(hole, 0, v);
+        a.pop_back();
+    }
+};
+
+struct StdHeap {
+    std::priority_queue<std::pair<int, int>, std::vector<std::pair<int, int>>, HeapLess> q;
+    void clear() { q = decltype(q)(); }
+    bool empty() const { return q.empty(); }
+    int top_vertex() const { return q.top().first; }
+    void push(int vertex, int dist) { q.push(std::make_pair(vertex, dist)); }
+    void pop() { q.pop(); }
+};
+
 // One vertex of a shortest-path search, kept together because the relax loop reads them together.
 struct VertexState {
     int32_t dist;    // -1: not reached
@@ -429,7 +496,8 @@ struct Workspace {
     SpTree t;
     std::vector<uint8_t> keep;
     std::vector<uint32_t> pos_of, touched, open_rows, level_count, level_start;
-    std::vector<std::pair<int, int>> heap;
+    SearchHeap heap;
+    StdHeap std_heap;
     int32_t n = -1;
 
     void prepare(int32_t nodes, const int32_t *fvs_rank) {
@@ -455,8 +523,7 @@ struct Workspace {
 };
 
 // dijkstra::dijkstra_sp (include/Dijkstra.h:78-127): lazy-deletion binary heap, strict '<' relax.
-// The heap is std::push_heap / std::pop_heap over a vector with the reference's comparator, the
-// operations std::priority_queue is specified to perform, on a vector reused across roots.  At each
+// Heap: std::priority_queue's (StdHeap) or SearchHeap, which pops in the same order.  At each
 // pop the vertex's parent is final, so blocked (the tree path from src holds a root of smaller id
 // than src: include/Dijkstra.h:205-254 rejects every cycle through such a vertex), branch and level
 // are set there.
@@ -466,7 +533,8 @@ struct Workspace {
 // and by induction blocked), so no later vertex can be a candidate endpoint or lie on the path of
 // one.  The pops up to that point are the full search's pops (same heap, same sequence), so the
 // finalised part of the tree, its order and its distances are the full search's.
-void dijkstra(const Multigraph &g, uint32_t src, bool stop_when_closed, Workspace &ws) {
+template <typename Heap>
+void dijkstra(const Multigraph &g, uint32_t src, bool stop_when_closed, Workspace &ws, Heap &heap) {
     VertexState *V = ws.t.v.data();
     const uint32_t *ro = g.ro.data(), *cols = g.cols.data(), *rows = g.rows.data();
     const int32_t *wt = g.w.data();
@@ -474,15 +542,11 @@ void dijkstra(const Multigraph &g, uint32_t src, bool stop_when_closed, Workspac
     V[src].level = 0;
     ws.touched.push_back(src);
     int64_t open = 1;  // reached, unfinalised vertices whose current path is not blocked
-    std::vector<std::pair<int, int>> &heap = ws.heap;
     heap.clear();
-    const HeapLess less;
-    heap.push_back(std::make_pair(static_cast<int>(src), 0));
-    std::push_heap(heap.begin(), heap.end(), less);
+    heap.push(static_cast<int>(src), 0);
     while (!heap.empty()) {
-        const uint32_t u = static_cast<uint32_t>(heap.front().first);
-        std::pop_heap(heap.begin(), heap.end(), less);
-        heap.pop_back();
+        const uint32_t u = static_cast<uint32_t>(heap.top_vertex());
+        heap.pop();
         VertexState &su = V[u];
         if (su.in_tree) continue;
         su.in_tree = 1;
@@ -513,8 +577,7 @@ void dijkstra(const Multigraph &g, uint32_t src, bool stop_when_closed, Workspac
                 sc.tent_blocked = nb;
                 sc.dist = cand;
                 sc.pedge = static_cast<int32_t>(off);
-                heap.push_back(std::make_pair(static_cast<int>(c), cand));
-                std::push_heap(heap.begin(), heap.end(), less);
+                heap.push(static_cast<int>(c), cand);
             }
         }
         if (stop_when_closed && open == 0) break;
@@ -562,7 +625,10 @@ void process_root(const mcb_problem &P, uint32_t src, bool prune, Workspace &ws,
     const int32_t n = g.n;
     SpTree &t = ws.t;
     ws.prepare(n, P.fvs_rank.data());
-    dijkstra(g, src, prune && !vertex_parent_row, ws);
+    if (prune && !vertex_parent_row)
+        dijkstra(g, src, true, ws, ws.heap);
+    else
+        dijkstra(g, src, false, ws, ws.std_heap);
     const VertexState *V = t.v.data();
     if (vertex_parent_row)
         for (int32_t v = 0; v < n; ++v) vertex_parent_row[v] = V[v].pedge;
@@ -578,7 +644,13 @@ void process_root(const mcb_problem &P, uint32_t src, bool prune, Workspace &ws,
     ws.open_rows.push_back(src);
     for (uint32_t v : t.order)
         if (!V[v].blocked) ws.open_rows.push_back(v);
-    std::sort(ws.open_rows.begin(), ws.open_rows.end());
+    if (ws.open_rows.size() * 16 < static_cast<size_t>(n)) {
+        std::sort(ws.open_rows.begin(), ws.open_rows.end());
+    } else {  // most of the graph: list the finalised unblocked vertices by a pass over the ids
+        ws.open_rows.clear();
+        for (int32_t v = 0; v < n; ++v)
+            if (V[v].in_tree && !V[v].blocked) ws.open_rows.push_back(static_cast<uint32_t>(v));
+    }
     std::unordered_map<unsigned long long, uint32_t> groups_of;
     std::vector<std::vector<uint32_t>> groups;
     for (uint32_t u : ws.open_rows) {
@@ -795,10 +867,20 @@ int build_from_input(InputGraph &in, int64_t input_weight_sum, const mcb_build_o
             }
         }
     }
+    // The sort runs on (weight, collection index) pairs compared by weight alone: std::sort's
+    // control flow depends only on the outcomes of its comparisons, which are those of sorting the
+    // indices by c_w[index], so the permutation (ties included) is the same, without a gather per
+    // comparison.
+    struct Keyed {
+        int32_t w;
+        uint32_t k;
+    };
+    std::vector<Keyed> keyed(K);
+    for (int64_t k = 0; k < K; ++k) keyed[k] = Keyed{c_w[k], static_cast<uint32_t>(k)};
+    std::sort(keyed.begin(), keyed.end(), [](const Keyed &a, const Keyed &b) { return a.w < b.w; });
     std::vector<uint32_t> rank_order(K);
-    std::iota(rank_order.begin(), rank_order.end(), 0u);
-    std::sort(rank_order.begin(), rank_order.end(),
-              [&](uint32_t a, uint32_t b) { return c_w[a] < c_w[b]; });
+    for (int64_t k = 0; k < K; ++k) rank_order[k] = keyed[k].k;
+    std::vector<Keyed>().swap(keyed);
     P->cand_root.resize(K);
     P->cand_edge.resize(K);
     P->cand_weight.resize(K);
@@ -1193,7 +1275,7 @@ int mcb_problem_reference_layout(const mcb_problem *p, int32_t r, uint32_t *rows
     const uint32_t src = p->roots[r];
     Workspace ws;
     ws.prepare(n, p->fvs_rank.data());
-    dijkstra(g, src, false, ws);
+    dijkstra(g, src, false, ws, ws.std_heap);
     const SpTree &t = ws.t;
     std::vector<LevelSorter> edges;
     edges.reserve(n);
