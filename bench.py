@@ -83,14 +83,15 @@ def cpu_baseline(n, u, v, w, budget_s: float, threads: int) -> dict:
             "sample": f"first {X} of {N} phases timed ({tx:.2f} s), scaled by N/{X}"}
 
 
-def measured_traffic(kernel: str, workload: str):
-    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this kernel and
-    workload (counter values scaled by the copy-kernel calibration), or None."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}.md")
+def measured_traffic(kernel: str, workload: str | None, key: str = "traffic_bytes_per_launch:"):
+    """HBM bytes per launch (or per step, by key) from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this kernel and workload (counter values scaled by the copy-kernel calibration), or None."""
+    name = kernel if workload is None else f"{kernel}_{workload}"
+    path = os.path.join(ROOT, "profiles", f"{name}.md")
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key):
                     return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -109,6 +110,8 @@ def main() -> int:
                     help="tree pass waves per workgroup (4, 8, 16; 0 timed at create)")
     ap.add_argument("--tree-cap", type=int, default=0,
                     help="largest tree kept in LDS by the tree pass (0 timed at create, -1 as large as fits)")
+    ap.add_argument("--merge-group", type=int, default=0,
+                    help="first-level bases merged per wave of the greedy basis (0 automatic, 2..64)")
     ap.add_argument("--fvs", choices=["plain", "learned"], default="plain",
                     help="single graphs: FVS construction (learned: the MLP-ordered greedy; the plain-FVS loop "
                          "is then timed beside it and the weights compared)")
@@ -163,7 +166,7 @@ def main() -> int:
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     solver = Solver(prob, timing=1 if args.breakdown else 2, block_phases=args.block_phases,
-                    waves_per_block=args.tree_waves, tree_cap=args.tree_cap)
+                    waves_per_block=args.tree_waves, tree_cap=args.tree_cap, merge_group=args.merge_group)
     upload_s = time.perf_counter() - t0
     dev = device_info()
     phases = args.phases if 0 < args.phases < info.num_non_tree else None
@@ -265,6 +268,8 @@ def main() -> int:
                             "support_update": 0.0 if blocked else round(upd / max(k_n, 1), 1)},
             "avg_launch_us": round(avg_ms * 1000.0, 3),
         },
+        "roofline_update": update_roofline(kt, upd, args.steps, "k_block_dot+k_block_xor",
+                                           f"k_block_update_{args.workload}") if blocked and not args.no_prune else None,
     }
     solver.close()
     if args.graph:
@@ -301,6 +306,20 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(n, u, v, w, args.cpu_budget, threads)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def update_roofline(kt, upd: int, steps: int, kernels: str, profile: str):
+    """The support vector update (dot products and XORs of later vectors): the bytes the update
+    kernels count as they run (each later vector's live prefix read once per block, and read and
+    written once more where the block's combination is nonzero) over their event-timed time."""
+    ms, n = kt["update"]
+    if n == 0 or ms <= 0:
+        return None
+    gbs = upd / (ms * 1e-3) / 1e9
+    return {"kernels": kernels, "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_step": upd // max(steps, 1),
+            "ms_per_step": round(ms / steps, 3), "launch_pairs_per_step": n // max(steps, 1),
+            "traffic": measured_traffic(profile, None, "traffic_bytes_per_step:")}
 
 
 def untimed_legs(make, steps: int, want) -> dict:
@@ -390,7 +409,8 @@ def run_components(args) -> int:
           file=sys.stderr, flush=True)
     dev = device_info()
     t0 = time.perf_counter()
-    bt = Batch(loop, timing=1 if args.breakdown else 2, waves_per_block=args.tree_waves, tree_cap=args.tree_cap)
+    bt = Batch(loop, timing=1 if args.breakdown else 2, waves_per_block=args.tree_waves, tree_cap=args.tree_cap,
+               merge_group=args.merge_group)
     upload_s = time.perf_counter() - t0
     print(f"[bench] batch arena {bt.device_bytes / 2**30:.2f} GiB, {bt.steps} steps, upload {upload_s:.2f}s",
           file=sys.stderr, flush=True)
@@ -498,6 +518,8 @@ def run_components(args) -> int:
             "bytes_per_launch": round(bytes_per_launch, 1),
             "avg_launch_us": round(avg_ms * 1000.0, 3),
         },
+        "roofline_update": update_roofline(kt, upd, args.steps, "k_batch_dot+k_batch_xor",
+                                           f"k_batch_update_{args.workload}"),
     }
     if untimed is not None:
         out["untimed"] = untimed

@@ -123,12 +123,15 @@ typedef struct {
                                  of those whose LDS fits, timed on the first block at create */
     int32_t max_blocks;       /* 0 = automatic (resident blocks on every compute unit) */
     int32_t timing;           /* HIP events on the solver's stream around kernel launches: 0 none,
-                                 1 every kernel, 2 only MCB_K_PARITY_SEARCH (the dominant kernel) */
+                                 1 every kernel, 2 only MCB_K_PARITY_SEARCH (the dominant kernel) and
+                                 MCB_K_UPDATE (the support vector update) */
     int32_t block_phases;     /* phases resolved per pass over the trees: 0 = automatic (64),
                                  1 = one kernel per phase, 2..64 = blocked with that many */
     int32_t tree_cap;         /* blocked form's tree pass: largest tree whose path words stay in LDS, in
                                  positions (larger ones use global scratch); 0 = timed at create among
                                  as-large-as-fits, 8192 and 2048; -1 = as large as fits */
+    int32_t merge_group;      /* blocked form's greedy basis: first-level bases merged per merge wave,
+                                 2..64; 0 = automatic */
 } mcb_solver_options;
 
 void mcb_solver_options_init(mcb_solver_options *opt);
@@ -179,7 +182,8 @@ enum {
     MCB_K_COUNT = 5
 };
 
-/* Accumulated event time (ms) and launch count of one kernel since the last reset (timing=1). */
+/* Accumulated event time (ms) and launch count of one kernel since the last reset (the kernels
+ * that the timing option times). */
 int mcb_solver_kernel_time(mcb_solver *s, int kernel, double *total_ms, int64_t *launches);
 
 /* Device bytes the solver holds. */

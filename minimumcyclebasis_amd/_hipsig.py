@@ -33,6 +33,7 @@ class SolverOptions(ctypes.Structure):
         ("timing", i32),
         ("block_phases", i32),
         ("tree_cap", i32),
+        ("merge_group", i32),
     ]
 
 

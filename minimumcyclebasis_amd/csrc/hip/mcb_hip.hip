@@ -678,7 +678,8 @@ constexpr int kMaxLevels = 8;    // kChunk * 16^(kMaxLevels-1) covers 2^32 ranks
 // Bases merged per group: one wave inserts a group's bases one after the other, so a group's time
 // grows with its size, and each level is one launch.  Of 16 and 64, the size that gives fewer levels
 // (16 on a tie).
-inline int merge_group_size(int64_t chunks) {
+inline int merge_group_size(int64_t chunks, int want) {
+    if (want > 0) return want;
     auto levels = [&](int64_t gsz) {
         int l = 0;
         for (int64_t n = chunks; n > 1; n = (n + gsz - 1) / gsz) ++l;
@@ -1119,35 +1120,41 @@ __device__ __forceinline__ void basis_insert(uint64_t &piv, uint64_t &P, uint32_
     }
 }
 
-// basis_insert with the reduction by existing pivots done through a table in LDS: tab[g][x] is
-// the XOR of the pivots of the columns 8g+t for the set bits t of x (zero for columns without a
-// pivot), so a word reduces with eight lookups.  The table is rebuilt, when pivots were added since
-// it was built, before a batch that has bits in pivot columns.  tab: 8 x 256 words for this wave.
+// basis_insert with the reduction by existing pivots done through tables in LDS: tab[g][x] is the
+// XOR of the pivots of the columns 4g+t for the set bits t of x (zero for columns without a pivot),
+// so a word reduces with sixteen independent lookups.  The tables are rebuilt, when pivots changed
+// since they were built, before a batch that has bits in pivot columns: lane l builds entries
+// 4(l%4) .. 4(l%4)+3 of group l/4 from that group's four pivots.  tab: kMergeTab words for this wave.
+constexpr int kMergeTab = 16 * 16;
 __device__ __forceinline__ void basis_insert_tab(uint64_t &piv, uint64_t &P, uint32_t &cnt, bool &stale,
                                                  uint64_t *tab, uint64_t q, uint32_t rank, uint32_t *out_rank,
                                                  uint64_t *out_q, int lane) {
     uint64_t v = q;
     if (__ballot((v & P) != 0)) {
         if (stale) {
-            for (int g = 0; g < 8; ++g) {
-                uint64_t pt[8];
+            const int g = lane >> 2;
+            uint64_t pt[4];
 #pragma unroll
-                for (int t = 0; t < 8; ++t) pt[t] = rl64(piv, 8 * g + t);
+            for (int t = 0; t < 4; ++t) {
+                const int src = 4 * g + t;
+                const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(piv)), src));
+                const uint32_t hi = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(piv >> 32)), src));
+                pt[t] = (static_cast<uint64_t>(hi) << 32) | lo;
+            }
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int x = lane + kWave * k;
-                    uint64_t e = 0;
+            for (int k = 0; k < 4; ++k) {
+                const int x = 4 * (lane & 3) + k;
+                uint64_t e = 0;
 #pragma unroll
-                    for (int t = 0; t < 8; ++t) e ^= pt[t] & (0ull - static_cast<uint64_t>((x >> t) & 1));
-                    tab[g * 256 + x] = e;
-                }
+                for (int t = 0; t < 4; ++t) e ^= pt[t] & (0ull - static_cast<uint64_t>((x >> t) & 1));
+                tab[g * 16 + x] = e;
             }
             __builtin_amdgcn_wave_barrier();
             stale = false;
         }
         uint64_t r = 0;
 #pragma unroll
-        for (int g = 0; g < 8; ++g) r ^= tab[g * 256 + static_cast<int>((v >> (8 * g)) & 255ull)];
+        for (int g = 0; g < 16; ++g) r ^= tab[g * 16 + static_cast<int>((v >> (4 * g)) & 15ull)];
         v ^= r;
     }
     uint64_t mine = 0;
@@ -1303,13 +1310,16 @@ __global__ __launch_bounds__(256) void k_block_basis(BlockArgs a) {
     }
 }
 
-// Merges the `members` bases in slots s0 .. s0+members-1 into slot `slot`.  Their entries,
-// concatenated in order, are in rank order; a prefix sum of the bases' sizes locates entry t, and the
-// entries are inserted 64 at a time with the next batch's loads in flight.  One wave; s_offs: 64
-// words, tab: the wave's 8 x 256 reduction table.
+// Merges the `members` bases (at most 64) in slots s0 .. s0+members-1 into slot `slot`.  Their
+// entries, concatenated in order, are in rank order.  All of them are first copied into LDS (one
+// coalesced load per base, sixteen bases in flight: the bases were written by the previous launch,
+// often through another XCD's L2, so a load per batch would put one memory latency on every batch of
+// the serial insertion), then inserted 64 at a time.  One wave; tab: the wave's kMergeTab-word
+// reduction tables, st_q / st_r: kMergeEntries entries.
+constexpr int kMergeEntries = 64 * kWave;
 __device__ __forceinline__ void merge_group(int64_t s0, int members, int64_t slot, int kk, uint32_t *sum_cnt,
-                                            uint32_t *sum_rank, uint64_t *sum_q, uint32_t *s_offs, uint64_t *tab,
-                                            int lane) {
+                                            uint32_t *sum_rank, uint64_t *sum_q, uint64_t *tab, uint64_t *st_q,
+                                            uint32_t *st_r, int lane) {
     const uint32_t own = lane < members ? sum_cnt[s0 + lane] : 0u;
     uint32_t incl = own;
 #pragma unroll
@@ -1318,39 +1328,43 @@ __device__ __forceinline__ void merge_group(int64_t s0, int members, int64_t slo
         if (lane >= o) incl += t;
     }
     const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), kWave - 1));
-    s_offs[lane] = incl - own;
-    __builtin_amdgcn_wave_barrier();
-    // entry t of the concatenation: base "lo" (the last base starting at or before t), position t - start
-    auto locate = [&](uint32_t t) -> int64_t {
-        int lo = 0, hi = kWave - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_offs[mid] <= t) lo = mid;
-            else hi = mid - 1;
+    const uint32_t excl = incl - own;
+    constexpr int kInFlight = 16;
+    for (int m0 = 0; m0 < members; m0 += kInFlight) {
+        uint64_t q[kInFlight];
+        uint32_t r[kInFlight];
+#pragma unroll
+        for (int u = 0; u < kInFlight; ++u) {
+            const int m = m0 + u;
+            const uint32_t c = m < members ? static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(own), m)) : 0u;
+            q[u] = 0;
+            r[u] = 0;
+            if (static_cast<uint32_t>(lane) < c) {
+                const int64_t at = (s0 + m) * kWave + lane;
+                q[u] = sum_q[at];
+                r[u] = sum_rank[at];
+            }
         }
-        return (s0 + lo) * kWave + (t - s_offs[lo]);
-    };
+#pragma unroll
+        for (int u = 0; u < kInFlight; ++u) {
+            const int m = m0 + u;
+            const uint32_t c = m < members ? static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(own), m)) : 0u;
+            if (static_cast<uint32_t>(lane) < c) {
+                const uint32_t o = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(excl), m)) + lane;
+                st_q[o] = q[u];
+                st_r[o] = r[u];
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     uint64_t piv = 0, P = 0;
     uint32_t cnt = 0;
     bool stale = true;
-    uint64_t qn = 0;
-    uint32_t rn = 0;
-    if (lane < static_cast<int>(total)) {
-        const int64_t at = locate(lane);
-        qn = sum_q[at];
-        rn = sum_rank[at];
-    }
     for (uint32_t b0 = 0; b0 < total && cnt < static_cast<uint32_t>(kk); b0 += kWave) {
-        const uint64_t q = qn;
-        const uint32_t r = rn;
-        const uint32_t tn = b0 + kWave + lane;
-        qn = 0;
-        rn = 0;
-        if (tn < total) {
-            const int64_t at = locate(tn);
-            qn = sum_q[at];
-            rn = sum_rank[at];
-        }
+        const uint32_t t = b0 + lane;
+        const uint64_t q = t < total ? st_q[t] : 0ull;
+        const uint32_t r = t < total ? st_r[t] : 0u;
         basis_insert_tab(piv, P, cnt, stale, tab, q, r, sum_rank + slot * kWave, sum_q + slot * kWave, lane);
     }
     if (lane == 0) sum_cnt[slot] = cnt;
@@ -1359,10 +1373,11 @@ __device__ __forceinline__ void merge_group(int64_t s0, int members, int64_t slo
 // One merge level: wave g merges bases g*group .. g*group+group-1 of level L into slot g of level
 // L+1.  At the top level the single wave also finalises the block.  The bases of level L were
 // written by the previous launch.
-constexpr int kMergeWaves = 2;  // waves per workgroup of k_block_merge (16 KB of LDS table each)
+constexpr int kMergeWaves = 1;  // waves per workgroup of k_block_merge (50 KB of LDS each)
 __global__ __launch_bounds__(kMergeWaves * kWave) void k_block_merge(BlockArgs a, int L) {
-    __shared__ uint32_t s_offs[kMergeWaves][kWave];
-    __shared__ uint64_t s_tab[kMergeWaves][8 * 256];
+    __shared__ uint64_t s_tab[kMergeWaves][kMergeTab];
+    __shared__ uint64_t s_stq[kMergeWaves][kMergeEntries];
+    __shared__ uint32_t s_str[kMergeWaves][kMergeEntries];
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = wave_id();
     const int64_t g = static_cast<int64_t>(blockIdx.x) * kMergeWaves + wv;
@@ -1370,7 +1385,7 @@ __global__ __launch_bounds__(kMergeWaves * kWave) void k_block_merge(BlockArgs a
     const int64_t rest = a.lvl_n[L] - g * a.group;
     const int members = static_cast<int>(rest < a.group ? rest : a.group);
     merge_group(a.lvl_slot[L] + g * a.group, members, a.lvl_slot[L + 1] + g, a.kk, a.sum_cnt, a.sum_rank, a.sum_q,
-                s_offs[wv], s_tab[wv], lane);
+                s_tab[wv], s_stq[wv], s_str[wv], lane);
     if (L + 2 == a.levels) {
         __threadfence();
         block_finalize(a, lane);
@@ -1973,8 +1988,9 @@ __global__ __launch_bounds__(256) void k_batch_basis(BatchArgs a) {
 // Wave x: merge group x of level L (of the running problems that have a level L+1); a problem's top
 // group also finalises its block.
 __global__ __launch_bounds__(kMergeWaves * kWave) void k_batch_merge(BatchArgs a, int L) {
-    __shared__ uint32_t s_offs[kMergeWaves][kWave];
-    __shared__ uint64_t s_tab[kMergeWaves][8 * 256];
+    __shared__ uint64_t s_tab[kMergeWaves][kMergeTab];
+    __shared__ uint64_t s_stq[kMergeWaves][kMergeEntries];
+    __shared__ uint32_t s_str[kMergeWaves][kMergeEntries];
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = wave_id();
     const int64_t x = static_cast<int64_t>(blockIdx.x) * kMergeWaves + wv;
@@ -1984,7 +2000,7 @@ __global__ __launch_bounds__(kMergeWaves * kWave) void k_batch_merge(BatchArgs a
     const CompInfo &c = a.comp[g];
     const int32_t kk = batch_kk(static_cast<int32_t>(uniform32(static_cast<uint32_t>(c.N))), a.step);
     merge_group(uniform64(mg.src), static_cast<int>(uniform32(static_cast<uint32_t>(mg.members))), uniform64(mg.dst),
-                kk, a.sum_cnt, a.sum_rank, a.sum_q, s_offs[wv], s_tab[wv], lane);
+                kk, a.sum_cnt, a.sum_rank, a.sum_q, s_tab[wv], s_stq[wv], s_str[wv], lane);
     if (static_cast<int>(uniform32(static_cast<uint32_t>(c.levels))) == L + 2) {
         __threadfence();
         batch_finalize(a, g, lane);
@@ -2467,7 +2483,8 @@ int harvest_events(S *s) {
 
 template <typename S>
 int timed_begin(S *s, int kind) {
-    s->timing_open = s->opt.timing == 1 || (s->opt.timing == 2 && kind == MCB_K_PARITY_SEARCH);
+    s->timing_open =
+        s->opt.timing == 1 || (s->opt.timing == 2 && (kind == MCB_K_PARITY_SEARCH || kind == MCB_K_UPDATE));
     if (!s->timing_open) return MCB_OK;
     if (s->ev_used + 2 > s->ev_pool.size()) {
         int rc = harvest_events(s);
@@ -2985,7 +3002,7 @@ int setup_blocked(mcb_solver *s, const int64_t *tree_offset, const uint32_t *par
     use_plan(s, 0);
     // greedy-basis levels: kChunk ranks per first-level basis, group bases per merge
     int64_t n = (s->K + kChunk - 1) / kChunk, slots = 0;
-    s->group = merge_group_size(n);
+    s->group = merge_group_size(n, s->opt.merge_group);
     s->levels = 0;
     for (;;) {
         if (s->levels >= kMaxLevels) return fail(MCB_E_INVALID, "too many candidates for the basis levels");
@@ -3056,6 +3073,8 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
     int32_t max_pos = 0;
     int rc = validate_desc(d, &max_pos);
     if (rc) return rc;
+    if (opt && (opt->merge_group < 0 || opt->merge_group == 1 || opt->merge_group > 64))
+        return fail(MCB_E_INVALID, "merge_group must be 0 or 2..64");
     rc = ensure_device();
     if (rc) return rc;
     mcb_solver *s = new (std::nothrow) mcb_solver();
@@ -3666,6 +3685,8 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
         if (validate_desc(&descs[i], &mp)) return fail(MCB_E_INVALID, "problem " + std::to_string(i) + ": " + g_err);
         largest = std::max(largest, mp);
     }
+    if (opt && (opt->merge_group < 0 || opt->merge_group == 1 || opt->merge_group > 64))
+        return fail(MCB_E_INVALID, "merge_group must be 0 or 2..64");
     int rc = ensure_device();
     if (rc) return rc;
     mcb_batch *b = new (std::nothrow) mcb_batch();
@@ -3713,7 +3734,7 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
         cand_base[g] = K_tot;
         slot_base[g] = slots;
         int64_t n = (d.num_candidates + kChunk - 1) / kChunk;
-        gsize[g] = merge_group_size(n);
+        gsize[g] = merge_group_size(n, b->opt.merge_group);
         for (;;) {
             lvl_slot[g].push_back(slots);
             lvl_n[g].push_back(n);
