@@ -816,18 +816,20 @@ __device__ __forceinline__ uint64_t rl64(uint64_t x, int l) {
     return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
-// Settles the lanes of one window whose parent lies in the same window: a lane takes its parent's
-// word once the parent is final; each round finalises at least the lowest pending lane.
+// Settles the lanes of one window whose parent lies in the same window, by pointer jumping: a
+// pending lane holds the XOR of the terms on its path up to (not including) its in-window ancestor
+// anc; each round it takes anc's value and anc's ancestor (both as they were before the round), so
+// the distance to a settled ancestor halves and a window of 64 settles in at most six rounds.  A
+// parent precedes its child, so every chain ends at a lane whose parent lies before the window.
 __device__ __forceinline__ uint64_t settle_words(uint64_t val, bool pending, uint32_t p, uint32_t wb, int lane) {
-    const uint64_t own = val;
-    bool done = !pending;
-    while (__ballot(!done)) {
-        const int src = pending ? static_cast<int>(p - wb) : lane;
-        const uint64_t pv = __shfl(val, src);
-        const int pd = __shfl(static_cast<int>(done), src);
-        if (!done && pd) {
-            val = own ^ pv;
-            done = true;
+    int anc = pending ? static_cast<int>(p - wb) : -1;
+    while (__ballot(anc >= 0)) {
+        const int src = anc >= 0 ? anc : lane;
+        const uint64_t av = __shfl(val, src);
+        const int aa = __shfl(anc, src);
+        if (anc >= 0) {
+            val ^= av;
+            anc = aa;
         }
     }
     return val;
