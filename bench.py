@@ -126,6 +126,8 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--phases", type=int, default=0,
                     help="profiling aid: run only the first PHASES phases per step (0 = the full loop)")
+    ap.add_argument("--progress-phases", type=int, default=0, metavar="K",
+                    help="profiling aid: run the loop K phases at a time, waiting and printing a line after each")
     ap.add_argument("--copy-kernel", type=int, default=0, metavar="BYTES",
                     help="calibration aid: run only the device copy kernel over BYTES bytes and exit")
     args = ap.parse_args()
@@ -171,10 +173,20 @@ def main() -> int:
     dev = device_info()
     phases = args.phases if 0 < args.phases < info.num_non_tree else None
 
+    def run_loop():
+        if args.progress_phases <= 0:
+            solver.run(phases)
+            solver.sync()
+            return
+        end = phases if phases is not None else info.num_non_tree
+        while solver.phase < end:
+            solver.run(min(args.progress_phases, end - solver.phase))
+            solver.sync()
+            print(f"[bench] phase {solver.phase} of {end}", file=sys.stderr, flush=True)
+
     for i in range(args.warmup):
         solver.reset()
-        solver.run(phases)
-        solver.sync()
+        run_loop()
         print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
 
     from minimumcyclebasis_amd.device import KERNELS
@@ -186,8 +198,7 @@ def main() -> int:
     for i in range(args.steps):
         solver.reset()
         t0 = time.perf_counter()
-        solver.run(phases)
-        solver.sync()
+        run_loop()
         dt = time.perf_counter() - t0
         t_total += dt
         for k in KERNELS:

@@ -34,7 +34,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <functional>
 #include <numeric>
+#include <queue>
 #include <string>
 #include <vector>
 
@@ -696,6 +698,7 @@ struct BlockArgs {
     const int32_t *pos_coord;
     const uint32_t *pos_pre, *pos_end;
     const uint32_t *big_list, *small_list;  // item ids
+    const int32_t *wg_big, *wave_small;     // each workgroup's big items, each wave's small items
     const uint8_t *big_narrow;  // per big item: 1 if its tree's levels average fewer than 128 positions
     const uint32_t *item_root;  // per item: root, and its candidate run [item_cb, item_ce) of the rc_* arrays
     const int64_t *item_cb, *item_ce;
@@ -998,15 +1001,16 @@ __device__ __forceinline__ void run_words(const A &a, const uint64_t *Wd, int64_
 // Pass over every tree and every candidate of the block.  The work lists hold items (a root and a
 // rank-ascending run of its candidates, item_root / item_cb / item_ce): big trees (more than
 // small_words positions) take a whole workgroup per item, the others one wave per item.  This
-// workgroup takes the big items big[first], big[first + stride], ... (narrow[i]: item i's tree is
-// narrow) and its waves the small items small[sfirst + wave], small[sfirst + wave + sstride], ...
+// workgroup takes the big items big[first], big[first + stride], ... below big[end] (narrow[i]: item
+// i's tree is narrow) and this wave the small items small[sfirst], small[sfirst + sstride], ... below
+// small[send].
 // A big tree of more than big_words positions keeps its path words in this workgroup's region of
 // scratch (huge_words words) instead of LDS; its ready stamps stay in LDS and are cleared first, so
 // that no stale word can pass for a stamp.  kk_of(r) is the number of phases in root r's block.
 template <typename A, typename KkFn>
-__device__ __forceinline__ void tree_pass(const A &a, const uint32_t *big, const uint8_t *narrow, uint32_t n_big,
-                                          uint32_t first, uint32_t stride, const uint32_t *small, uint32_t n_small,
-                                          uint32_t sfirst, uint32_t sstride, KkFn kk_of) {
+__device__ __forceinline__ void tree_pass(const A &a, const uint32_t *big, const uint8_t *narrow, uint32_t first,
+                                          uint32_t end, uint32_t stride, const uint32_t *small, uint32_t sfirst,
+                                          uint32_t send, uint32_t sstride, KkFn kk_of) {
     extern __shared__ uint64_t lds[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = wave_id();
@@ -1017,7 +1021,7 @@ __device__ __forceinline__ void tree_pass(const A &a, const uint32_t *big, const
     for (uint32_t i = threadIdx.x; i < nwin_big; i += blockDim.x) ready[i] = 0;
     __syncthreads();
     uint32_t gen = 0;
-    for (uint32_t it = first; it < n_big; it += stride) {
+    for (uint32_t it = first; it < end; it += stride) {
         const uint32_t item = uniform32(big[it]);
         const uint32_t r = uniform32(a.item_root[item]);
         const int64_t off = uniform64(a.tree_off[r]);
@@ -1052,7 +1056,7 @@ __device__ __forceinline__ void tree_pass(const A &a, const uint32_t *big, const
     }
     // the big-tree region is free from here on; each wave takes its own slice for small roots
     uint64_t *Ws = lds + static_cast<size_t>(wid) * a.small_words;
-    for (uint32_t it = sfirst + wid; it < n_small; it += sstride) {
+    for (uint32_t it = sfirst; it < send; it += sstride) {
         const uint32_t item = uniform32(small[it]);
         const uint32_t r = uniform32(a.item_root[item]);
         const int64_t off = uniform64(a.tree_off[r]);
@@ -1064,8 +1068,11 @@ __device__ __forceinline__ void tree_pass(const A &a, const uint32_t *big, const
 
 __global__ __launch_bounds__(kQWaves * kWave) void k_block_q(BlockArgs a) {
     const uint32_t nw = blockDim.x / kWave;
-    tree_pass(a, a.big_list, a.big_narrow, static_cast<uint32_t>(a.n_big), blockIdx.x, gridDim.x, a.small_list,
-              static_cast<uint32_t>(a.n_small), blockIdx.x * nw, gridDim.x * nw, [&](uint32_t) { return a.kk; });
+    const uint32_t c = blockIdx.x * nw + static_cast<uint32_t>(wave_id());
+    tree_pass(a, a.big_list, a.big_narrow, static_cast<uint32_t>(a.wg_big[blockIdx.x]),
+              static_cast<uint32_t>(a.wg_big[blockIdx.x + 1]), 1u, a.small_list,
+              static_cast<uint32_t>(a.wave_small[c]), static_cast<uint32_t>(a.wave_small[c + 1]), 1u,
+              [&](uint32_t) { return a.kk; });
 }
 
 // Inserts one batch (lane i: word q, ranks ascending with the lane; q = 0 for no entry) into a
@@ -1951,8 +1958,9 @@ __global__ __launch_bounds__(kQWaves * kWave) void k_batch_q(BatchArgs a) {
     // blocks x, x + kXcds, ... serve section x (the grid is a multiple of kXcds)
     const uint32_t x = blockIdx.x % kXcds, local = blockIdx.x / kXcds, nloc = gridDim.x / kXcds;
     const uint32_t bo = static_cast<uint32_t>(a.x_big_off[x]), so = static_cast<uint32_t>(a.x_small_off[x]);
-    tree_pass(a, a.big_list + bo, a.big_narrow + bo, static_cast<uint32_t>(a.x_big_n[x]), local, nloc,
-              a.small_list + so, static_cast<uint32_t>(a.x_small_n[x]), local * nw, nloc * nw, [&](uint32_t r) {
+    tree_pass(a, a.big_list + bo, a.big_narrow + bo, local, static_cast<uint32_t>(a.x_big_n[x]), nloc,
+              a.small_list + so, local * nw + static_cast<uint32_t>(wave_id()), static_cast<uint32_t>(a.x_small_n[x]),
+              nloc * nw, [&](uint32_t r) {
                   const uint32_t g = uniform32(a.root_comp[r]);
                   return batch_kk(static_cast<int32_t>(uniform32(static_cast<uint32_t>(a.comp[g].N))), a.step);
               });
@@ -2392,6 +2400,9 @@ struct TreePlan {
     int64_t huge_words = 0;
     uint32_t *big = nullptr, *small = nullptr;
     uint8_t *narrow = nullptr;  // per big item: 1 if its tree's levels average fewer than 128 positions
+    // one problem: big items of workgroup b are big[wg_big[b] .. wg_big[b+1]), small items of wave c
+    // (workgroup c / nw) small[wave_small[c] .. wave_small[c+1]) (null for a batch: round-robin)
+    int32_t *wg_big = nullptr, *wave_small = nullptr;
     uint32_t *item_root = nullptr;
     int64_t *item_cb = nullptr, *item_ce = nullptr;
     uint64_t *scratch = nullptr;
@@ -2402,10 +2413,11 @@ struct TreePlan {
 
 // The scratch is shared by a solver's plans (one runs at a time) and freed by its owner.
 void free_plan(TreePlan &p) {
-    void *ptrs[] = {p.big, p.small, p.narrow, p.item_root, p.item_cb, p.item_ce};
+    void *ptrs[] = {p.big, p.small, p.narrow, p.item_root, p.item_cb, p.item_ce, p.wg_big, p.wave_small};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     p.big = p.small = p.item_root = nullptr;
+    p.wg_big = p.wave_small = nullptr;
     p.narrow = nullptr;
     p.item_cb = p.item_ce = nullptr;
     p.scratch = nullptr;
@@ -2444,6 +2456,7 @@ struct mcb_solver {
     int32_t q_waves = 0, q_big_words = 0, q_small_words = 0, q_blocks = 0, q_lds = 0, n_qbig = 0, n_qsmall = 0;
     uint32_t *q_big_list = nullptr, *q_small_list = nullptr;  // the chosen plan's lists
     uint8_t *q_big_narrow = nullptr;
+    int32_t *q_wg_big = nullptr, *q_wave_small = nullptr;
     std::vector<TreePlan> plans;
     int32_t plan = 0;
     uint64_t *q_scratch = nullptr;  // path words of trees beyond a plan's LDS cap
@@ -2579,6 +2592,8 @@ BlockArgs block_args(const mcb_solver *s, int32_t e, int32_t kk) {
     a.big_list = s->q_big_list;
     a.big_narrow = s->q_big_narrow;
     a.small_list = s->q_small_list;
+    a.wg_big = s->q_wg_big;
+    a.wave_small = s->q_wave_small;
     a.rc_off = s->rc_off;
     a.rc_pu = s->rc_pu;
     a.rc_pv = s->rc_pv;
@@ -2874,6 +2889,78 @@ int plan_tree_pass(int nw, int64_t cap_words, int nsplit, const int64_t *tree_of
         narrow_all.insert(narrow_all.end(), narrow[x].begin(), narrow[x].end());
         small_all.insert(small_all.end(), small[x].begin(), small[x].end());
     }
+    std::vector<int32_t> wg_big, wave_small;
+    if (nsplit == 1) {
+        // One problem: longest-processing-time assignment under a cost model (ns of one wave, from
+        // timings of this pass: a tree position 16, a candidate 35, an item 300).  A coop tree's work
+        // is shared by the nw waves; a narrow tree is walked by one wave while the others wait, so
+        // it occupies the whole workgroup.  Big items go, costliest first, to the least loaded
+        // workgroup; then small items to the least loaded wave, each wave starting from its
+        // workgroup's big-item time.
+        const int nwv = nw;
+        auto ncand = [&](uint32_t it) { return static_cast<double>(item_ce[it] - item_cb[it]); };
+        auto npos_of = [&](uint32_t it) {
+            return static_cast<double>(tree_off[item_root[it] + 1] - tree_off[item_root[it]]);
+        };
+        std::vector<double> bcost(big_all.size());
+        for (size_t i = 0; i < big_all.size(); ++i) {
+            const uint32_t it = big_all[i];
+            bcost[i] = narrow_all[i] ? 9.0 * npos_of(it) + 35.0 * ncand(it) / nwv + 300.0
+                                     : (16.0 * npos_of(it) + 35.0 * ncand(it)) / nwv + 300.0;
+        }
+        std::vector<size_t> bo(big_all.size());
+        std::iota(bo.begin(), bo.end(), size_t{0});
+        std::stable_sort(bo.begin(), bo.end(), [&](size_t x, size_t y) { return bcost[x] > bcost[y]; });
+        using Load = std::pair<double, int32_t>;
+        std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+        for (int32_t w = 0; w < p.blocks; ++w) heap.push({0.0, w});
+        std::vector<std::vector<size_t>> per_wg(static_cast<size_t>(p.blocks));
+        std::vector<double> wg_load(static_cast<size_t>(p.blocks), 0.0);
+        for (size_t i : bo) {
+            Load l = heap.top();
+            heap.pop();
+            per_wg[static_cast<size_t>(l.second)].push_back(i);
+            l.first += bcost[i];
+            wg_load[static_cast<size_t>(l.second)] = l.first;
+            heap.push(l);
+        }
+        std::vector<uint32_t> nb;
+        std::vector<uint8_t> nn;
+        wg_big.push_back(0);
+        for (const auto &v : per_wg) {
+            for (size_t i : v) {
+                nb.push_back(big_all[i]);
+                nn.push_back(narrow_all[i]);
+            }
+            wg_big.push_back(static_cast<int32_t>(nb.size()));
+        }
+        big_all.swap(nb);
+        narrow_all.swap(nn);
+        std::vector<double> scost(small_all.size());
+        for (size_t i = 0; i < small_all.size(); ++i)
+            scost[i] = 16.0 * npos_of(small_all[i]) + 35.0 * ncand(small_all[i]) + 300.0;
+        std::vector<size_t> so(small_all.size());
+        std::iota(so.begin(), so.end(), size_t{0});
+        std::stable_sort(so.begin(), so.end(), [&](size_t x, size_t y) { return scost[x] > scost[y]; });
+        const size_t waves = static_cast<size_t>(p.blocks) * static_cast<size_t>(nwv);
+        std::priority_queue<Load, std::vector<Load>, std::greater<Load>> wheap;
+        for (size_t c = 0; c < waves; ++c) wheap.push({wg_load[c / nwv], static_cast<int32_t>(c)});
+        std::vector<std::vector<size_t>> per_wave(waves);
+        for (size_t i : so) {
+            Load l = wheap.top();
+            wheap.pop();
+            per_wave[static_cast<size_t>(l.second)].push_back(i);
+            l.first += scost[i];
+            wheap.push(l);
+        }
+        std::vector<uint32_t> ns;
+        wave_small.push_back(0);
+        for (const auto &v : per_wave) {
+            for (size_t i : v) ns.push_back(small_all[i]);
+            wave_small.push_back(static_cast<int32_t>(ns.size()));
+        }
+        small_all.swap(ns);
+    }
     p.n_big = static_cast<int32_t>(big_all.size());
     p.n_small = static_cast<int32_t>(small_all.size());
     int rc;
@@ -2883,6 +2970,9 @@ int plan_tree_pass(int nw, int64_t cap_words, int nsplit, const int64_t *tree_of
         (rc = upload(&p.item_root, item_root.data(), item_root.size(), bytes)) ||
         (rc = upload(&p.item_cb, item_cb.data(), item_cb.size(), bytes)) ||
         (rc = upload(&p.item_ce, item_ce.data(), item_ce.size(), bytes)))
+        return rc;
+    if (nsplit == 1 && ((rc = upload(&p.wg_big, wg_big.data(), wg_big.size(), bytes)) ||
+                        (rc = upload(&p.wave_small, wave_small.data(), wave_small.size(), bytes))))
         return rc;
     return MCB_OK;
 }
@@ -2945,6 +3035,8 @@ void use_plan(mcb_solver *s, size_t i) {
     s->q_big_list = p.big;
     s->q_small_list = p.small;
     s->q_big_narrow = p.narrow;
+    s->q_wg_big = p.wg_big;
+    s->q_wave_small = p.wave_small;
 }
 
 
