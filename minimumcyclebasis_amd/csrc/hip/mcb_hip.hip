@@ -839,8 +839,11 @@ __device__ __forceinline__ uint64_t settle_words(uint64_t val, bool pending, uin
 }
 
 // Path words of kUnroll windows of one wave's tree: word[pos] = word[parent] ^ Tt[coordinate].
+// kRing: Wd is global scratch and ring (rmask + 1 words of LDS, a power of two of at least two
+// windows) keeps the most recent windows' words, so a parent in one of them is read from LDS.
+template <bool kRing>
 __device__ __forceinline__ void word_group(const WindowGroup &g, uint32_t base, uint32_t npos, const uint64_t *Tt,
-                                           uint64_t *Wd, int lane) {
+                                           uint64_t *Wd, uint64_t *ring, uint32_t rmask, int lane) {
     uint64_t t[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) t[u] = Tt[g.co[u] >= 0 ? g.co[u] : 0];
@@ -852,24 +855,34 @@ __device__ __forceinline__ void word_group(const WindowGroup &g, uint32_t base, 
         const uint32_t p = v ? g.par[u] : kNoParent;
         const uint64_t tv = (v && g.co[u] >= 0) ? t[u] : 0ull;
         const bool before = p < wb;
-        uint64_t val = tv ^ (before ? Wd[p] : 0ull);
+        uint64_t pw = 0;
+        if (before) {
+            // the ring holds positions [wb + 64 - ring size, wb) while window wb is walked
+            if (kRing && p + rmask + 1 >= wb + kWave) pw = ring[p & rmask];
+            else pw = Wd[p];
+        }
+        uint64_t val = tv ^ pw;
         const bool inwin = (p != kNoParent) && (p >= wb);
         if (__ballot(inwin)) val = settle_words(val, inwin, p, wb, lane);
-        if (v) Wd[wb + lane] = val;
+        if (v) {
+            Wd[wb + lane] = val;
+            if (kRing) ring[(wb + lane) & rmask] = val;
+        }
     }
 }
 
+template <bool kRing = false>
 __device__ __forceinline__ void tree_words(const uint32_t *pp, const int32_t *pc, uint32_t npos, const uint64_t *Tt,
-                                           uint64_t *Wd, int lane) {
+                                           uint64_t *Wd, int lane, uint64_t *ring = nullptr, uint32_t rmask = 0) {
     constexpr uint32_t G = kWave * kUnroll;
     WindowGroup A, B;
     load_group(A, pp, pc, npos, lane, kWave);
     for (uint32_t base = 0; base < npos; base += 2 * G) {
         load_group(B, pp, pc, npos, base + G + lane, kWave);
-        word_group(A, base, npos, Tt, Wd, lane);
+        word_group<kRing>(A, base, npos, Tt, Wd, ring, rmask, lane);
         if (base + G >= npos) break;
         load_group(A, pp, pc, npos, base + 2 * G + lane, kWave);
-        word_group(B, base + G, npos, Tt, Wd, lane);
+        word_group<kRing>(B, base + G, npos, Tt, Wd, ring, rmask, lane);
     }
 }
 
@@ -1018,6 +1031,10 @@ __device__ __forceinline__ void tree_pass(const A &a, const uint32_t *big, const
     uint64_t *Wbig = lds;
     uint32_t *ready = reinterpret_cast<uint32_t *>(lds + a.big_words);
     const uint32_t nwin_big = static_cast<uint32_t>(a.big_words) / kWave;
+    // ring of a scratch tree's latest windows: the largest power of two of words in the LDS tree
+    // region, if it holds at least two windows (the ready stamps of a narrow tree are not used)
+    uint32_t ring_mask = 0;
+    if (a.big_words >= 2 * kWave) ring_mask = (1u << (31 - __builtin_clz(static_cast<uint32_t>(a.big_words)))) - 1;
     for (uint32_t i = threadIdx.x; i < nwin_big; i += blockDim.x) ready[i] = 0;
     __syncthreads();
     uint32_t gen = 0;
@@ -1031,11 +1048,19 @@ __device__ __forceinline__ void tree_pass(const A &a, const uint32_t *big, const
         const int64_t cb = uniform64(a.item_cb[item]), ce = uniform64(a.item_ce[item]);
         // path words, then each wave takes one contiguous run of the item's candidates (the two calls
         // keep the LDS and the scratch accesses in their own address spaces)
-        auto one_tree = [&](uint64_t *Wd, uint32_t *rd) {
+        auto one_tree = [&](uint64_t *Wd, uint32_t *rd, bool in_scratch) {
             if (nar) {
                 // narrow, deep tree: windows mostly wait on the window before them, so one wave walks
-                // it alone (no handoffs between waves); the others wait at the barrier
-                if (wid == 0) tree_words(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wd, lane);
+                // it alone (no handoffs between waves); the others wait at the barrier.  A tree in
+                // scratch keeps its latest windows in the idle LDS tree region as well (ring), so
+                // that the walk does not wait on global memory for every window.
+                if (wid == 0) {
+                    if (in_scratch && ring_mask) {
+                        tree_words<true>(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wd, lane, Wbig, ring_mask);
+                    } else {
+                        tree_words(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wd, lane);
+                    }
+                }
             } else {
                 tree_words_coop(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wd, rd, gen, wid, nw, lane);
             }
@@ -1046,12 +1071,12 @@ __device__ __forceinline__ void tree_pass(const A &a, const uint32_t *big, const
             __syncthreads();
         };
         if (npos <= static_cast<uint32_t>(a.big_words)) {
-            one_tree(Wbig, ready);
+            one_tree(Wbig, ready, false);
         } else {
             uint32_t *rd = reinterpret_cast<uint32_t *>(lds);
             for (uint32_t i = threadIdx.x; i < (npos + kWave - 1) / kWave; i += blockDim.x) rd[i] = 0;
             __syncthreads();
-            one_tree(a.scratch + static_cast<int64_t>(blockIdx.x) * a.huge_words, rd);
+            one_tree(a.scratch + static_cast<int64_t>(blockIdx.x) * a.huge_words, rd, true);
         }
     }
     // the big-tree region is free from here on; each wave takes its own slice for small roots
