@@ -2821,7 +2821,11 @@ int plan_tree_pass(int nw, int64_t cap_words, int nsplit, const int64_t *tree_of
                    const std::vector<uint32_t> &depth, const std::vector<int64_t> &ranges, const void *kernel,
                    TreePlan &p, int64_t &bytes) {
     const size_t G = ranges.size() - 1;
-    const int64_t lds_max = std::min<int64_t>(g_dev.prop.sharedMemPerBlock, 160 * 1024);
+    // dynamic LDS: what the kernel's own static LDS leaves of a workgroup's budget
+    hipFuncAttributes fattr{};
+    HIP_TRY(hipFuncGetAttributes(&fattr, kernel));
+    const int64_t lds_max =
+        std::min<int64_t>(g_dev.prop.sharedMemPerBlock, 160 * 1024) - static_cast<int64_t>(fattr.sharedSizeBytes);
     // largest multiple of 64 positions whose words and stamps fit the LDS, or the requested cap
     int64_t big_cap = lds_max * 64 / (8 * 64 + 4) / kWave * kWave;
     if (cap_words > 0) big_cap = std::min(big_cap, cap_words);
