@@ -1100,40 +1100,6 @@ __global__ __launch_bounds__(kQWaves * kWave) void k_block_q(BlockArgs a) {
               [&](uint32_t) { return a.kk; });
 }
 
-// Inserts one batch (lane i: word q, ranks ascending with the lane; q = 0 for no entry) into a
-// wave's basis held as piv (lane b: the pivot whose lowest set bit is b, or 0), column by column:
-// a column that has a pivot is cleared from every lane; otherwise the first lane with that bit
-// becomes its pivot and the bit is cleared from the lanes after it.  The lanes that become pivots
-// are exactly the batch vectors independent of the basis and of the batch vectors before them, and
-// the column at which a vector becomes a pivot is the lowest set bit it has after elimination by
-// everything before it in rank order.  Returns the mask of new pivot lanes; *col (if given) receives,
-// in each such lane, its column.
-__device__ __forceinline__ uint64_t basis_columns(uint64_t &piv, uint64_t q, int kk, int lane, int *col) {
-    uint64_t v = q;
-    uint64_t newp = 0;
-    for (int b = 0; b < kk; ++b) {
-        const uint64_t pb = rl64(piv, b);
-        const bool has = (v >> b) & 1ull;
-        if (pb) {
-            if (has) v ^= pb;
-        } else {
-            const uint64_t m = __ballot(has);
-            if (m) {
-                const int p = __ffsll(static_cast<unsigned long long>(m)) - 1;
-                const uint64_t pv = rl64(v, p);
-                if (lane == b) piv = pv;
-                if (lane > p && has) v ^= pv;
-                if (lane == p) {
-                    v = 0;
-                    if (col) *col = b;
-                }
-                newp |= 1ull << p;
-            }
-        }
-    }
-    return newp;
-}
-
 // Greedy insertion of one batch into a wave basis; the new independent vectors (original word and
 // rank) are appended in rank order to (out_rank, out_q); cnt (wave-uniform) stops at kk.
 __device__ __forceinline__ void basis_insert(uint64_t &piv, uint64_t &P, uint32_t &cnt, uint64_t q, uint32_t rank,
@@ -1221,6 +1187,16 @@ __device__ __forceinline__ void basis_insert_tab(uint64_t &piv, uint64_t &P, uin
 
 // The block's greedy basis (slot top) in rank order -> phase of each winner, masks m_i, the
 // products <C_{e+i}, S_{e+l}> and the chosen ranks (sel[i], i < kk).  One wave.
+//
+// Lane l holds column l of the rank-ordered word matrix (bit r: bit l of winner r's word, i.e. the
+// product of winner r with T_l) and the mask m_l of phase l (initially T_l alone).  Phase b takes the
+// first winner r not yet taken whose bit in column b is set; every later phase l whose column has a
+// set bit at r (an odd product with the winner under its current mask) takes m_l ^= m_b and, the
+// products being linear in the mask, column l ^= column b.  This is the sequential update of phases
+// e..e+kk-1 on the block's words.  The winner chosen for each column is the one the row-wise
+// elimination in rank order picks, since both read the same Schur complement entry.  At the end
+// column l holds the products with the final S_{e+l}, so bit l of the products of phase i's winner
+// is column l's bit at that winner: one ballot per phase.
 __device__ void finalize_core(int kk, const uint32_t *sum_cnt, const uint32_t *sum_rank, const uint64_t *sum_q,
                               int64_t top, uint32_t *sel, uint64_t *blk_q, uint64_t *blk_m, uint64_t *blk_g,
                               uint32_t *status, int lane) {
@@ -1229,43 +1205,48 @@ __device__ void finalize_core(int kk, const uint32_t *sum_cnt, const uint32_t *s
     if (ok) {
         const uint64_t q = lane < kk ? sum_q[top * kWave + lane] : 0ull;
         const uint32_t rk = lane < kk ? sum_rank[top * kWave + lane] : 0u;
-        // phase of each winner: the column at which it becomes a pivot in rank order
-        uint64_t piv = 0;
-        int phase_of = -1;
-        const uint64_t all = basis_columns(piv, q, kk, lane, &phase_of);
-        if (all != (kk == kWave ? ~0ull : ((1ull << kk) - 1))) ok = false;
+        uint64_t col = 0;
+        for (int j = 0; j < kk; ++j) {
+            const uint64_t bj = __ballot((q >> j) & 1ull);
+            if (lane == j) col = bj;
+        }
+        uint64_t m = lane < kk ? (1ull << lane) : 0ull;
+        uint64_t taken = 0;  // wave-uniform
+        int win = 0;         // lane i: rank position of phase i's winner
+        for (int b = 0; b < kk; ++b) {
+            const uint64_t cb = rl64(col, b);
+            const uint64_t avail = cb & ~taken;
+            if (!avail) {
+                ok = false;
+                break;
+            }
+            const int r = __ffsll(static_cast<unsigned long long>(avail)) - 1;
+            taken |= 1ull << r;
+            if (lane == b) win = r;
+            const uint64_t mb = rl64(m, b);
+            if (lane > b && lane < kk && ((col >> r) & 1ull)) {
+                m ^= mb;
+                col ^= cb;
+            }
+        }
         if (ok) {
-            uint64_t qi = 0;
-            uint32_t ri = 0;
+            const uint32_t qlo = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(q)), win));
+            const uint32_t qhi = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(q >> 32)), win));
+            const uint64_t qi = (static_cast<uint64_t>(qhi) << 32) | qlo;
+            const uint32_t ri = static_cast<uint32_t>(__shfl(static_cast<int>(rk), win));
+            uint64_t prod = 0;  // lane i: bit l = <C_{e+i}, final S_{e+l}>
             for (int i = 0; i < kk; ++i) {
-                const uint64_t b = __ballot(phase_of == i);
-                const int f = b ? __ffsll(static_cast<unsigned long long>(b)) - 1 : 0;
-                const uint64_t qf = rl64(q, f);
-                const uint32_t rf = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rk), f));
-                if (lane == i) {
-                    qi = qf;
-                    ri = rf;
-                }
+                const int wi = __builtin_amdgcn_readlane(win, i);
+                const uint64_t bi = __ballot((col >> wi) & 1ull);
+                if (lane == i) prod = bi;
             }
-            // masks: after phase i, every later mask with an odd product with the winner takes m_i
-            uint64_t m = lane < kk ? (1ull << lane) : 0ull;
-            for (int i = 0; i < kk; ++i) {
-                const uint64_t qw = rl64(qi, i);
-                const uint64_t mi = rl64(m, i);
-                if (lane > i && lane < kk && (__popcll(m & qw) & 1)) m ^= mi;
-            }
-            uint64_t g = 0;
-            for (int l = 0; l < kk; ++l) {
-                const uint64_t ml = rl64(m, l);
-                if (l < lane) g |= static_cast<uint64_t>(__popcll(ml & qi) & 1) << l;
-            }
-            const bool odd = lane >= kk || (__popcll(m & qi) & 1);
+            const bool odd = lane >= kk || ((prod >> lane) & 1ull);
             if (__ballot(!odd)) ok = false;
             if (lane < kk) {
                 sel[lane] = ri;
                 blk_q[lane] = qi;
                 blk_m[lane] = m;
-                blk_g[lane] = g;
+                blk_g[lane] = prod & ((1ull << lane) - 1);
             }
         }
     }
