@@ -1764,6 +1764,7 @@ __device__ __forceinline__ void xor_body(const BlkView &v, int64_t xt, int32_t y
         if (!__syncthreads_or(need)) return;
     }
     uint64_t *Tab = lds;  // [kTile][16][16]
+    bool tile_nz;
     {
         const int t = tid >> 4, g = tid & 15;
         uint64_t r[4];
@@ -1772,6 +1773,7 @@ __device__ __forceinline__ void xor_body(const BlkView &v, int64_t xt, int32_t y
             const int i = 4 * g + k;
             r[k] = (t < tn && i < kk) ? v.Sf_blk[static_cast<int64_t>(w0 + t) * kBlockK + i] : 0ull;
         }
+        tile_nz = (r[0] | r[1] | r[2] | r[3]) != 0;
         uint64_t e[16];
         e[0] = 0;
 #pragma unroll
@@ -1781,6 +1783,8 @@ __device__ __forceinline__ void xor_body(const BlkView &v, int64_t xt, int32_t y
         for (int x = 0; x < 16; ++x) row[x] = e[x];
     }
     __syncthreads();
+    // a tile where every final vector of the block is zero changes no later column
+    tile_nz = __syncthreads_or(tile_nz) != 0;
     const int64_t j = static_cast<int64_t>(v.e) + xt * blockDim.x + tid;
     uint64_t nbytes = 0;
     if (j < static_cast<int64_t>(v.e) + kk) {
@@ -1790,7 +1794,7 @@ __device__ __forceinline__ void xor_body(const BlkView &v, int64_t xt, int32_t y
             if ((j >> 6) == w0 + t) x &= ~(1ull << (j & 63));
             v.Sw[static_cast<int64_t>(w0 + t) * v.npad + j] = x;
         }
-    } else if (j < v.N) {
+    } else if (j < v.N && tile_nz) {
         const uint64_t b = v.Bc[j];
         if (b) {
             for (int t0 = 0; t0 < tn; t0 += kW) {
