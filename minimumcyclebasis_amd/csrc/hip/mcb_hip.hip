@@ -675,6 +675,7 @@ __global__ void k_copy_u64(const uint64_t *__restrict__ src, uint64_t *__restric
 constexpr int kBlockK = 64;      // phases per block (word width)
 constexpr int kQWaves = 16;      // waves per workgroup of the tree pass
 constexpr int kChunk = 4096;     // ranks per first-level greedy basis (one candidate-bitmap word per lane)
+constexpr int64_t kXorWorkgroups = 2048;  // grid of k_block_xor (eight workgroups per CU)
 constexpr int kMaxLevels = 8;    // kChunk * 16^(kMaxLevels-1) covers 2^32 ranks
 
 // Bases merged per group: one wave inserts a group's bases one after the other, so a group's time
@@ -731,6 +732,8 @@ struct BlockArgs {
     uint64_t *C_store;   // optional [N][W]
     uint64_t *Ct;        // [npad] bit i of Ct[c] = bit c of C_{e+i}
     uint64_t *Bc;        // [npad] update coefficients of each later column
+    uint32_t *ulist;     // [npad] the later columns whose coefficients are nonzero (any order)
+    uint32_t *ucount;    // [1] their number (cleared by k_block_transpose)
     uint32_t *pw_list;   // [W] words below e where some C_{e+i} is nonzero
     uint32_t *pw_off;    // [W+1] first pair of each listed word (prefix sums)
     uint32_t *pw_cnt;    // [1] listed words
@@ -758,6 +761,7 @@ struct BlkView {
     uint64_t *Tt, *Ct, *Bc;
     uint32_t *pw_list, *pw_off, *pw_cnt, *pair_i;
     uint64_t *pair_c;
+    uint32_t *ulist, *ucount;  // later columns that take an update (null: not listed)
 };
 
 __device__ __forceinline__ BlkView view_of(const BlockArgs &a) {
@@ -786,6 +790,8 @@ __device__ __forceinline__ BlkView view_of(const BlockArgs &a) {
     v.pw_cnt = a.pw_cnt;
     v.pair_i = a.pair_i;
     v.pair_c = a.pair_c;
+    v.ulist = a.ulist;
+    v.ucount = a.ucount;
     return v;
 }
 
@@ -804,6 +810,7 @@ __device__ __forceinline__ void transpose_body(const BlkView &v, int64_t w, int 
 }
 
 __global__ __launch_bounds__(256) void k_block_transpose(BlockArgs a) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.ucount = 0;  // k_block_dot lists this block's updated columns
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t w = static_cast<int64_t>(blockIdx.x) * 4 + wave_id();
     const int64_t wl = (static_cast<int64_t>(a.e) + a.kk + kWave - 1) / kWave;
@@ -1661,7 +1668,7 @@ __device__ __forceinline__ void dot_body(const BlkView &v, int64_t tile, uint64_
         s0 = s1;
     }
     __syncthreads();
-    uint64_t nbytes = 0;
+    uint64_t nbytes = 0, bcol = 0;
     if (col) {
         uint64_t b = 0;
         for (int i = 0; i < kk; ++i) {
@@ -1669,7 +1676,19 @@ __device__ __forceinline__ void dot_body(const BlkView &v, int64_t tile, uint64_
             b |= bi << i;
         }
         v.Bc[j] = b;
+        bcol = b;
         nbytes = 2 + 8ull * nl;
+    }
+    if (v.ulist) {  // list the columns that take an update, one atomic per wave
+        const bool upd = col && bcol != 0;
+        const uint64_t m = __ballot(upd);
+        if (m) {
+            const int lane = tid & (kWave - 1);
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(v.ucount, static_cast<uint32_t>(__popcll(m)));
+            base = static_cast<uint32_t>(__shfl(static_cast<int>(base), 0));
+            if (upd) v.ulist[base + __popcll(m & ((1ull << lane) - 1))] = static_cast<uint32_t>(j);
+        }
     }
     for (int o = kWave / 2; o > 0; o >>= 1) nbytes += __shfl_down(nbytes, o);
     if ((tid & (kWave - 1)) == 0 && nbytes) atomicAdd(upd_bytes, static_cast<unsigned long long>(nbytes));
@@ -1744,59 +1763,69 @@ __device__ __forceinline__ void xor_direct(const BlkView &v, int64_t xt, int32_t
     if ((tid & (kWave - 1)) == 0 && nbytes) atomicAdd(upd_bytes, static_cast<unsigned long long>(nbytes));
 }
 
-// Tile (x, y): columns e + 256x .. +255, words kTile*y .. +kTile-1.  Column e+i < e+kk receives its
-// final vector S_{e+i}; a later column j with Bc[j] != 0 takes S_j ^= sum_i b_i S_{e+i}.  The sums go
-// through tables in LDS ("four Russians"): Tab[t][g][x] is the XOR of word t of S_{e+4g+k} over the set
-// bits k of the nibble x, so a word of the update is 16 lookups, one per nibble of b.  Thread (t, g)
-// builds table (t, g); a nibble's 16 entries are 128 contiguous bytes, so lookups do not conflict.
-__device__ __forceinline__ void xor_body(const BlkView &v, int64_t xt, int32_t yt, uint64_t *lds,
-                                         unsigned long long *upd_bytes) {
+// The XOR update of one block over (column tile, word tile) pairs, dealt to a fixed grid: column tile
+// 0 is the block's own 64 columns, which take their final vectors; tiles 1.. are the listed later
+// columns (ulist, 256 each), which take S_j ^= sum_i b_ji S_{e+i}.  Per pair the workgroup builds
+// four-Russians tables of kTile words in LDS (Tab[t][g][x]: the XOR of the final vectors 4g+k, k in
+// x, at word w0+t), so a word costs 16 lookups.  A tile whose final vectors are all zero changes no
+// later column.  Columns whose coefficients are zero are never visited.
+__device__ __forceinline__ void xor_body(const BlkView &v, uint64_t *lds, unsigned long long *upd_bytes) {
     static_assert(kTile * 16 == 256, "one table per thread of the workgroup");
     constexpr int kW = 8;
     const int tid = threadIdx.x;
     const int kk = v.kk;
     const int32_t wl = (v.e + kk + kWave - 1) / kWave;
-    const int32_t w0 = yt * kTile;
-    const int tn = wl - w0 < kTile ? wl - w0 : kTile;
-    {  // nothing to do unless some column of the tile is the block's own or takes an update
-        const int64_t jc = static_cast<int64_t>(v.e) + xt * blockDim.x + tid;
-        const bool need = jc < static_cast<int64_t>(v.e) + kk || (jc < v.N && v.Bc[jc] != 0);
-        if (!__syncthreads_or(need)) return;
-    }
-    uint64_t *Tab = lds;  // [kTile][16][16]
-    bool tile_nz;
-    {
-        const int t = tid >> 4, g = tid & 15;
-        uint64_t r[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int i = 4 * g + k;
-            r[k] = (t < tn && i < kk) ? v.Sf_blk[static_cast<int64_t>(w0 + t) * kBlockK + i] : 0ull;
-        }
-        tile_nz = (r[0] | r[1] | r[2] | r[3]) != 0;
-        uint64_t e[16];
-        e[0] = 0;
-#pragma unroll
-        for (int x = 1; x < 16; ++x) e[x] = e[x & (x - 1)] ^ r[__builtin_ctz(x)];
-        uint64_t *row = Tab + (t * 16 + g) * 16;
-#pragma unroll
-        for (int x = 0; x < 16; ++x) row[x] = e[x];
-    }
-    __syncthreads();
-    // a tile where every final vector of the block is zero changes no later column
-    tile_nz = __syncthreads_or(tile_nz) != 0;
-    const int64_t j = static_cast<int64_t>(v.e) + xt * blockDim.x + tid;
+    const int64_t ytiles = (wl + kTile - 1) / kTile;
+    const uint32_t J = *v.ucount;
+    const int64_t tiles = (1 + (static_cast<int64_t>(J) + blockDim.x - 1) / blockDim.x) * ytiles;
     uint64_t nbytes = 0;
-    if (j < static_cast<int64_t>(v.e) + kk) {
-        const int i = static_cast<int>(j - v.e);
-        for (int t = 0; t < tn; ++t) {
-            uint64_t x = Tab[(t * 16 + (i >> 2)) * 16 + (1 << (i & 3))];
-            if ((j >> 6) == w0 + t) x &= ~(1ull << (j & 63));
-            v.Sw[static_cast<int64_t>(w0 + t) * v.npad + j] = x;
+    uint64_t *Tab = lds;  // [kTile][16][16]
+    for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const int64_t ct = tile / ytiles;
+        const int32_t w0 = static_cast<int32_t>(tile % ytiles) * kTile;
+        const int tn = wl - w0 < kTile ? wl - w0 : kTile;
+        int64_t j = -1;
+        uint64_t b = 0;
+        if (ct == 0) {
+            if (tid < kk) j = static_cast<int64_t>(v.e) + tid;
+        } else {
+            const int64_t idx = (ct - 1) * blockDim.x + tid;
+            if (idx < static_cast<int64_t>(J)) {
+                j = v.ulist[idx];
+                b = v.Bc[j];
+            }
         }
-    } else if (j < v.N && tile_nz) {
-        const uint64_t b = v.Bc[j];
-        if (b) {
+        __syncthreads();  // the previous pair's tables are no longer read
+        bool tile_nz;
+        {
+            const int t = tid >> 4, g = tid & 15;
+            uint64_t r[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int i = 4 * g + k;
+                r[k] = (t < tn && i < kk) ? v.Sf_blk[static_cast<int64_t>(w0 + t) * kBlockK + i] : 0ull;
+            }
+            tile_nz = (r[0] | r[1] | r[2] | r[3]) != 0;
+            uint64_t e[16];
+            e[0] = 0;
+#pragma unroll
+            for (int x = 1; x < 16; ++x) e[x] = e[x & (x - 1)] ^ r[__builtin_ctz(x)];
+            uint64_t *row = Tab + (t * 16 + g) * 16;
+#pragma unroll
+            for (int x = 0; x < 16; ++x) row[x] = e[x];
+        }
+        __syncthreads();
+        tile_nz = __syncthreads_or(tile_nz) != 0;
+        if (ct == 0) {
+            if (j >= 0) {
+                const int i = static_cast<int>(j - v.e);
+                for (int t = 0; t < tn; ++t) {
+                    uint64_t x = Tab[(t * 16 + (i >> 2)) * 16 + (1 << (i & 3))];
+                    if ((j >> 6) == w0 + t) x &= ~(1ull << (j & 63));
+                    v.Sw[static_cast<int64_t>(w0 + t) * v.npad + j] = x;
+                }
+            }
+        } else if (j >= 0 && tile_nz && b) {
             for (int t0 = 0; t0 < tn; t0 += kW) {
                 uint64_t xs[kW];
 #pragma unroll
@@ -1828,7 +1857,7 @@ __device__ __forceinline__ void xor_body(const BlkView &v, int64_t xt, int32_t y
 
 __global__ __launch_bounds__(256) void k_block_xor(BlockArgs a) {
     extern __shared__ uint64_t lds[];
-    xor_body(view_of(a), blockIdx.x, static_cast<int32_t>(blockIdx.y), lds, a.upd_bytes);
+    xor_body(view_of(a), lds, a.upd_bytes);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1927,6 +1956,8 @@ __device__ __forceinline__ BlkView batch_view(const BatchArgs &a, int g) {
     v.D_blk = a.D_blk + cblk;
     v.pair_i = a.pair_i + cblk;
     v.pair_c = a.pair_c + cblk;
+    v.ulist = nullptr;
+    v.ucount = nullptr;
     v.Dcnt_blk = a.Dcnt_blk + static_cast<int64_t>(g) * kBlockK;
     v.C_store = a.C_store ? a.C_store + uniform64(c.store) : nullptr;
     v.Tt = a.Tt + v.cb;
@@ -2477,6 +2508,7 @@ struct mcb_solver {
     uint64_t *C_blk = nullptr, *Sf_blk = nullptr, *Ct = nullptr, *Bc = nullptr, *pair_c = nullptr;
     uint32_t *pw_list = nullptr, *pw_off = nullptr, *pw_cnt = nullptr, *pair_i = nullptr;
     uint32_t *sum_cnt = nullptr, *sum_rank = nullptr, *D_blk = nullptr, *Dcnt_blk = nullptr;
+    uint32_t *ulist = nullptr, *ucount = nullptr;  // the block's updated later columns
     unsigned long long *upd_bytes = nullptr;
     int32_t levels = 0, group = 16;
     int64_t lvl_slot[kMaxLevels] = {}, lvl_n[kMaxLevels] = {};
@@ -2634,6 +2666,8 @@ BlockArgs block_args(const mcb_solver *s, int32_t e, int32_t kk) {
     a.C_blk = s->C_blk;
     a.D_blk = s->D_blk;
     a.Dcnt_blk = s->Dcnt_blk;
+    a.ulist = s->ulist;
+    a.ucount = s->ucount;
     a.Sf_blk = s->Sf_blk;
     a.C_store = s->C_store;
     a.Ct = s->Ct;
@@ -2692,9 +2726,11 @@ int launch_block(mcb_solver *s, int32_t e, int32_t kk) {
         hipLaunchKernelGGL(k_block_dot, dim3(static_cast<unsigned>((later + 255) / 256)), dim3(256), dlds, s->stream, a);
         HIP_TRY(hipGetLastError());
     }
-    const int64_t cols = s->N - e;
-    hipLaunchKernelGGL(k_block_xor, dim3(static_cast<unsigned>((cols + 255) / 256), static_cast<unsigned>((wl + kTile - 1) / kTile)),
-                       dim3(256), sizeof(uint64_t) * 16 * 16 * kTile, s->stream, a);
+    // (column tile, word tile) pairs over the block's own columns and the listed later ones, dealt to at
+    // most kXorWorkgroups workgroups (the list's length is known only on the device)
+    const int64_t most = (1 + (std::max<int64_t>(later, 0) + 255) / 256) * ((wl + kTile - 1) / kTile);
+    hipLaunchKernelGGL(k_block_xor, dim3(static_cast<unsigned>(std::min<int64_t>(most, kXorWorkgroups))), dim3(256),
+                       sizeof(uint64_t) * 16 * 16 * kTile, s->stream, a);
     HIP_TRY(hipGetLastError());
     s->blk_positions += s->plans[s->plan].pos_cum.back();
     s->blk_candidates += s->plans[s->plan].cand_cum.back();
@@ -3135,6 +3171,9 @@ int setup_blocked(mcb_solver *s, const int64_t *tree_offset, const uint32_t *par
     if ((rc = dev_alloc(&s->Sf_blk, static_cast<size_t>(kBlockK) * s->W, B))) return rc;
     if ((rc = dev_alloc(&s->D_blk, static_cast<size_t>(kBlockK) * s->W, B))) return rc;
     if ((rc = dev_alloc(&s->Dcnt_blk, kBlockK, B))) return rc;
+    if ((rc = dev_alloc(&s->ulist, static_cast<size_t>(s->npad), B))) return rc;
+    if ((rc = dev_alloc(&s->ucount, 1, B))) return rc;
+    HIP_TRY(hipMemset(s->ucount, 0, sizeof(uint32_t)));
     if ((rc = dev_alloc(&s->upd_bytes, 1, B))) return rc;
     if ((rc = dev_alloc(&s->Ct, static_cast<size_t>(s->npad), B))) return rc;
     if ((rc = dev_alloc(&s->Bc, static_cast<size_t>(s->npad), B))) return rc;
@@ -3163,7 +3202,7 @@ void mcb_solver_destroy(mcb_solver *s) {
                     s->best, s->done, s->status, s->wave_stats, s->flag, s->E, s->Ecount, s->cdot,
                     s->Tt, s->Qr, s->qbits, s->sum_q, s->blk_q, s->blk_m, s->blk_g, s->C_blk,
                     s->Sf_blk, s->sum_cnt, s->sum_rank, s->D_blk, s->Dcnt_blk, s->upd_bytes,
-                    s->Ct, s->Bc, s->pair_c, s->pw_list, s->pw_off, s->pw_cnt, s->pair_i};
+                    s->Ct, s->Bc, s->pair_c, s->pw_list, s->pw_off, s->pw_cnt, s->pair_i, s->ulist, s->ucount};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (TreePlan &p : s->plans) free_plan(p);
