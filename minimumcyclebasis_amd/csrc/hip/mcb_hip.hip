@@ -1336,12 +1336,14 @@ __global__ __launch_bounds__(256) void k_block_basis(BlockArgs a) {
 // entries, concatenated in order, are in rank order.  All of them are first copied into LDS (one
 // coalesced load per base, sixteen bases in flight: the bases were written by the previous launch,
 // often through another XCD's L2, so a load per batch would put one memory latency on every batch of
-// the serial insertion), then inserted 64 at a time.  One wave; tab: the wave's kMergeTab-word
-// reduction tables, st_q / st_r: kMergeEntries entries.
+// the serial insertion), then inserted 64 at a time.  The kMergeWaves waves of the workgroup share the
+// loads; wave 0 inserts (the others return false).  tab: kMergeTab words of reduction tables, st_q /
+// st_r: kMergeEntries entries.
+constexpr int kMergeWaves = 4;  // waves per merge workgroup (one merge group, 50 KB of LDS)
 constexpr int kMergeEntries = 64 * kWave;
-__device__ __forceinline__ void merge_group(int64_t s0, int members, int64_t slot, int kk, uint32_t *sum_cnt,
+__device__ __forceinline__ bool merge_group(int64_t s0, int members, int64_t slot, int kk, uint32_t *sum_cnt,
                                             uint32_t *sum_rank, uint64_t *sum_q, uint64_t *tab, uint64_t *st_q,
-                                            uint32_t *st_r, int lane) {
+                                            uint32_t *st_r, int lane, int wv) {
     const uint32_t own = lane < members ? sum_cnt[s0 + lane] : 0u;
     uint32_t incl = own;
 #pragma unroll
@@ -1352,7 +1354,7 @@ __device__ __forceinline__ void merge_group(int64_t s0, int members, int64_t slo
     const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), kWave - 1));
     const uint32_t excl = incl - own;
     constexpr int kInFlight = 16;
-    for (int m0 = 0; m0 < members; m0 += kInFlight) {
+    for (int m0 = wv * kInFlight; m0 < members; m0 += kMergeWaves * kInFlight) {
         uint64_t q[kInFlight];
         uint32_t r[kInFlight];
 #pragma unroll
@@ -1378,8 +1380,8 @@ __device__ __forceinline__ void merge_group(int64_t s0, int members, int64_t slo
             }
         }
     }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __syncthreads();  // every wave's share is in LDS; wave 0 inserts
+    if (wv != 0) return false;
     uint64_t piv = 0, P = 0;
     uint32_t cnt = 0;
     bool stale = true;
@@ -1390,24 +1392,25 @@ __device__ __forceinline__ void merge_group(int64_t s0, int members, int64_t slo
         basis_insert_tab(piv, P, cnt, stale, tab, q, r, sum_rank + slot * kWave, sum_q + slot * kWave, lane);
     }
     if (lane == 0) sum_cnt[slot] = cnt;
+    return true;
 }
 
 // One merge level: wave g merges bases g*group .. g*group+group-1 of level L into slot g of level
 // L+1.  At the top level the single wave also finalises the block.  The bases of level L were
 // written by the previous launch.
-constexpr int kMergeWaves = 1;  // waves per workgroup of k_block_merge (50 KB of LDS each)
 __global__ __launch_bounds__(kMergeWaves * kWave) void k_block_merge(BlockArgs a, int L) {
-    __shared__ uint64_t s_tab[kMergeWaves][kMergeTab];
-    __shared__ uint64_t s_stq[kMergeWaves][kMergeEntries];
-    __shared__ uint32_t s_str[kMergeWaves][kMergeEntries];
+    __shared__ uint64_t s_tab[kMergeTab];
+    __shared__ uint64_t s_stq[kMergeEntries];
+    __shared__ uint32_t s_str[kMergeEntries];
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = wave_id();
-    const int64_t g = static_cast<int64_t>(blockIdx.x) * kMergeWaves + wv;
+    const int64_t g = blockIdx.x;
     if (g >= a.lvl_n[L + 1]) return;
     const int64_t rest = a.lvl_n[L] - g * a.group;
     const int members = static_cast<int>(rest < a.group ? rest : a.group);
-    merge_group(a.lvl_slot[L] + g * a.group, members, a.lvl_slot[L + 1] + g, a.kk, a.sum_cnt, a.sum_rank, a.sum_q,
-                s_tab[wv], s_stq[wv], s_str[wv], lane);
+    if (!merge_group(a.lvl_slot[L] + g * a.group, members, a.lvl_slot[L + 1] + g, a.kk, a.sum_cnt, a.sum_rank,
+                     a.sum_q, s_tab, s_stq, s_str, lane, wv))
+        return;
     if (L + 2 == a.levels) {
         __threadfence();
         block_finalize(a, lane);
@@ -2039,19 +2042,20 @@ __global__ __launch_bounds__(256) void k_batch_basis(BatchArgs a) {
 // Wave x: merge group x of level L (of the running problems that have a level L+1); a problem's top
 // group also finalises its block.
 __global__ __launch_bounds__(kMergeWaves * kWave) void k_batch_merge(BatchArgs a, int L) {
-    __shared__ uint64_t s_tab[kMergeWaves][kMergeTab];
-    __shared__ uint64_t s_stq[kMergeWaves][kMergeEntries];
-    __shared__ uint32_t s_str[kMergeWaves][kMergeEntries];
+    __shared__ uint64_t s_tab[kMergeTab];
+    __shared__ uint64_t s_stq[kMergeEntries];
+    __shared__ uint32_t s_str[kMergeEntries];
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = wave_id();
-    const int64_t x = static_cast<int64_t>(blockIdx.x) * kMergeWaves + wv;
+    const int64_t x = blockIdx.x;
     if (x >= a.n_items) return;
     const MergeGroup &mg = a.groups[x];
     const int g = static_cast<int>(uniform32(static_cast<uint32_t>(mg.comp)));
     const CompInfo &c = a.comp[g];
     const int32_t kk = batch_kk(static_cast<int32_t>(uniform32(static_cast<uint32_t>(c.N))), a.step);
-    merge_group(uniform64(mg.src), static_cast<int>(uniform32(static_cast<uint32_t>(mg.members))), uniform64(mg.dst),
-                kk, a.sum_cnt, a.sum_rank, a.sum_q, s_tab[wv], s_stq[wv], s_str[wv], lane);
+    if (!merge_group(uniform64(mg.src), static_cast<int>(uniform32(static_cast<uint32_t>(mg.members))),
+                     uniform64(mg.dst), kk, a.sum_cnt, a.sum_rank, a.sum_q, s_tab, s_stq, s_str, lane, wv))
+        return;
     if (static_cast<int>(uniform32(static_cast<uint32_t>(c.levels))) == L + 2) {
         __threadfence();
         batch_finalize(a, g, lane);
@@ -2706,8 +2710,8 @@ int launch_block(mcb_solver *s, int32_t e, int32_t kk) {
     hipLaunchKernelGGL(k_block_basis, dim3(static_cast<unsigned>((s->lvl_n[0] + 3) / 4)), dim3(256), 0, s->stream, a);
     HIP_TRY(hipGetLastError());
     for (int L = 0; L + 1 < s->levels; ++L) {
-        hipLaunchKernelGGL(k_block_merge, dim3(static_cast<unsigned>((s->lvl_n[L + 1] + kMergeWaves - 1) / kMergeWaves)),
-                           dim3(kMergeWaves * kWave), 0, s->stream, a, L);
+        hipLaunchKernelGGL(k_block_merge, dim3(static_cast<unsigned>(s->lvl_n[L + 1])), dim3(kMergeWaves * kWave), 0,
+                           s->stream, a, L);
         HIP_TRY(hipGetLastError());
     }
     if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_CYCLE))) return rc;
@@ -3754,8 +3758,7 @@ int launch_batch_step(mcb_batch *b, int32_t s) {
         if (n == 0) break;  // no running problem has a level L+1, nor any higher one
         a.groups = b->d_groups + b->grp_off[L];
         a.n_items = n;
-        hipLaunchKernelGGL(k_batch_merge, dim3(static_cast<unsigned>((n + kMergeWaves - 1) / kMergeWaves)),
-                           dim3(kMergeWaves * kWave), 0, b->stream, a, L);
+        hipLaunchKernelGGL(k_batch_merge, dim3(static_cast<unsigned>(n)), dim3(kMergeWaves * kWave), 0, b->stream, a, L);
         HIP_TRY(hipGetLastError());
     }
     if ((rc = timed_end(b)) || (rc = timed_begin(b, MCB_K_CYCLE))) return rc;
