@@ -1530,18 +1530,38 @@ __device__ __forceinline__ void pairs_transpose(const BlkView &v, int64_t w, int
 // C_{e+i}) pairs and the prefix sums of the pair counts, so that the pairs of consecutive listed
 // words are contiguous.  One workgroup; lds: ceil(e/64) words + 2 * blockDim.x u32.
 __device__ __forceinline__ void pairs_list(const BlkView &v, uint64_t *lds, uint32_t *nlist, uint32_t *npairs) {
+    __shared__ uint32_t pre[kBlockK + 1];  // prefix sums of the nonzero-word counts of C_{e+i}
+    __shared__ uint32_t wsum[2][8];        // per-wave totals of the scan below
     const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1), wid = wave_id();
     // U[w]: bit i set when C_{e+i} is nonzero at word w (w below e)
     const int32_t dlim = (v.e + kWave - 1) / kWave;
     uint64_t *U = lds;
     uint32_t *part = reinterpret_cast<uint32_t *>(U + dlim);  // [2 * blockDim.x]: words, pairs per thread
     for (int x = tid; x < dlim; x += blockDim.x) U[x] = 0;
+    if (wid == 0) {
+        const uint32_t c = lane < v.kk ? v.Dcnt_blk[lane] : 0u;
+        uint32_t incl = c;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        pre[lane + 1] = incl;
+        if (lane == 0) pre[0] = 0;
+    }
     __syncthreads();
-    for (int i = 0; i < v.kk; ++i) {
-        const uint32_t nd = v.Dcnt_blk[i];
-        const uint32_t *Di = v.D_blk + static_cast<int64_t>(i) * v.W;
-        for (uint32_t q = tid; q < nd; q += blockDim.x)
-            atomicOr(reinterpret_cast<unsigned long long *>(&U[Di[q]]), 1ull << i);
+    // every (i, listed word of C_{e+i}) entry at once: the loads are independent
+    const uint32_t total = pre[v.kk];
+    for (uint32_t idx = tid; idx < total; idx += blockDim.x) {
+        int lo = 0, hi = v.kk - 1;  // the last i whose entries start at or before idx
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= idx) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t w = v.D_blk[static_cast<int64_t>(lo) * v.W + (idx - pre[lo])];
+        atomicOr(reinterpret_cast<unsigned long long *>(&U[w]), 1ull << lo);
     }
     __syncthreads();
     const int per = (dlim + static_cast<int>(blockDim.x) - 1) / static_cast<int>(blockDim.x);
@@ -1552,20 +1572,36 @@ __device__ __forceinline__ void pairs_list(const BlkView &v, uint64_t *lds, uint
             ++nw;
             np += static_cast<uint32_t>(__popcll(U[w]));
         }
-    part[tid] = nw;
-    part[blockDim.x + tid] = np;
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t sw = 0, sp = 0;
-        for (unsigned t = 0; t < blockDim.x; ++t) {
-            const uint32_t a1 = part[t], a2 = part[blockDim.x + t];
-            part[t] = sw;
-            part[blockDim.x + t] = sp;
-            sw += a1;
-            sp += a2;
+    // exclusive scans of (words, pairs) over the threads: within each wave, then across the waves
+    uint32_t iw = nw, ip = np;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t tw = __shfl_up(iw, o), tp = __shfl_up(ip, o);
+        if (lane >= o) {
+            iw += tw;
+            ip += tp;
         }
-        *nlist = sw;
-        *npairs = sp;
+    }
+    if (lane == kWave - 1) {
+        wsum[0][wid] = iw;
+        wsum[1][wid] = ip;
+    }
+    __syncthreads();
+    uint32_t bw = 0, bp = 0, tw = 0, tp = 0;
+    const int nwav = static_cast<int>(blockDim.x) / kWave;
+    for (int k = 0; k < nwav; ++k) {
+        if (k < wid) {
+            bw += wsum[0][k];
+            bp += wsum[1][k];
+        }
+        tw += wsum[0][k];
+        tp += wsum[1][k];
+    }
+    part[tid] = bw + iw - nw;
+    part[blockDim.x + tid] = bp + ip - np;
+    if (tid == 0) {
+        *nlist = tw;
+        *npairs = tp;
     }
     __syncthreads();
     uint32_t x = part[tid], off = part[blockDim.x + tid];
