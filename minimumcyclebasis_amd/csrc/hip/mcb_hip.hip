@@ -1650,7 +1650,7 @@ constexpr int kSegWords = 1024;
 constexpr int kSegPairs = 2048;
 __device__ __forceinline__ void dot_body(const BlkView &v, int64_t tile, uint64_t *lds, uint32_t *seg_end,
                                          unsigned long long *upd_bytes) {
-    constexpr int kB = 8;
+    constexpr int kB = 16;
     const int tid = threadIdx.x;
     const int kk = v.kk;
     uint64_t *G = lds;
