@@ -1500,16 +1500,16 @@ __device__ __forceinline__ void final_body(const BlkView &v, int64_t w, int lane
     v.Sf_blk[w * kBlockK + lane] = acc;
 }
 
-// Workgroup i: phase e+i of the block.
-__global__ __launch_bounds__(256) void k_block_cycles(BlockArgs a) {
+// After the block's finalisation, in one launch: workgroup i < kk builds phase e+i's cycle vector
+// (cycles_body), the others the final support vectors, one word per wave (final_body).
+__global__ __launch_bounds__(256) void k_block_cycles_final(BlockArgs a) {
     extern __shared__ uint64_t lds[];
     __shared__ uint32_t cnt;
-    cycles_body(a, view_of(a), blockIdx.x, lds, &cnt);
-}
-
-// Wave w: word w of the block's final support vectors.
-__global__ __launch_bounds__(256) void k_block_final(BlockArgs a) {
-    const int64_t w = static_cast<int64_t>(blockIdx.x) * 4 + wave_id();
+    if (static_cast<int>(blockIdx.x) < a.kk) {
+        cycles_body(a, view_of(a), blockIdx.x, lds, &cnt);
+        return;
+    }
+    const int64_t w = static_cast<int64_t>(blockIdx.x - a.kk) * 4 + wave_id();
     const int64_t wl = (static_cast<int64_t>(a.e) + a.kk + kWave - 1) / kWave;
     if (w >= wl) return;
     final_body(view_of(a), w, threadIdx.x & (kWave - 1));
@@ -2751,9 +2751,8 @@ int launch_block(mcb_solver *s, int32_t e, int32_t kk) {
         HIP_TRY(hipGetLastError());
     }
     if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_CYCLE))) return rc;
-    hipLaunchKernelGGL(k_block_cycles, dim3(kk), dim3(256), sizeof(uint64_t) * s->W, s->stream, a);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_block_final, dim3(static_cast<unsigned>((wl + 3) / 4)), dim3(256), 0, s->stream, a);
+    hipLaunchKernelGGL(k_block_cycles_final, dim3(static_cast<unsigned>(kk + (wl + 3) / 4)), dim3(256),
+                       sizeof(uint64_t) * s->W, s->stream, a);
     HIP_TRY(hipGetLastError());
     const int32_t dlim = (e + kWave - 1) / kWave;
     hipLaunchKernelGGL(k_block_pairs, dim3(static_cast<unsigned>(1 + (s->W + 3) / 4)), dim3(256),

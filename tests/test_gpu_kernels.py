@@ -197,3 +197,41 @@ def test_tree_larger_than_lds_and_split_candidates(waves):
     assert np.array_equal(s.selected(), plain.selected())
     s.close()
     plain.close()
+
+
+_MERGE_CASES = {}
+
+
+def _merge_case(case):
+    """The problem and its oracle result, built once per session."""
+    if case not in _MERGE_CASES:
+        from minimumcyclebasis_amd import generators as G
+        from minimumcyclebasis_amd.host import Problem
+
+        gr = {"er1500": lambda: G.erdos_renyi_lbc(1500, 5000, 1000, seed=11),
+              "delaunay2000": lambda: G.delaunay(2000, seed=4)}[case]()
+        p = Problem.from_edges(*gr, keep_vertex_trees=True, prune_trees=True)
+        _MERGE_CASES[case] = (p, pyoracle.run(p, want_c=True))
+    return _MERGE_CASES[case]
+
+
+@pytest.mark.parametrize("group", [0, 2, 3, 8, 64])
+@pytest.mark.parametrize("case", ["er1500", "delaunay2000"])
+def test_merge_group_sizes(case, group):
+    """The greedy basis of a block merged in groups of 2, 3, 8 or 64 first-level bases (36 and 9 chunks of
+    4096 ranks here, so up to six merge levels) gives the oracle's selections and cycle vectors."""
+    p, ref = _merge_case(case)
+    print(f"case {case} merge_group={group} K={p.info.num_candidates}", flush=True)
+    s = Solver(p, store_cycles=True, merge_group=group)
+    s.run()
+    s.sync()
+    assert np.array_equal(s.selected(), ref["selected"])
+    assert np.array_equal(s.cycle_vectors(), ref["C"])
+    s.close()
+
+
+def test_merge_group_rejected():
+    p = problem_of(load_trace("grid16x16"), "reduced", keep_vertex_trees=True, prune_trees=True)
+    for bad in (1, 65, -2):
+        with pytest.raises(RuntimeError):
+            Solver(p, merge_group=bad)
