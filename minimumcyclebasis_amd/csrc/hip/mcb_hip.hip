@@ -1665,22 +1665,28 @@ __device__ __forceinline__ void dot_body(const BlkView &v, int64_t tile, uint64_
     const uint32_t nl = *v.pw_cnt;
     for (uint32_t s0 = 0; s0 < nl;) {
         __syncthreads();
-        if (tid == 0) {  // the longest run of listed words from s0 whose pairs fit
-            const uint32_t p0 = v.pw_off[s0];
-            uint32_t lo = s0 + 1, hi = s0 + kSegWords < nl ? s0 + kSegWords : nl;
+        // the pair offsets of the next kSegWords listed words, staged once, then the longest run of
+        // them from s0 whose pairs fit (searched in LDS)
+        const uint32_t hw = s0 + kSegWords < nl ? s0 + kSegWords : nl;
+        for (uint32_t x = tid; x <= hw - s0; x += blockDim.x) Lo[x] = v.pw_off[s0 + x];
+        __syncthreads();
+        if (tid == 0) {
+            const uint32_t q0 = Lo[0];
+            uint32_t lo = s0 + 1, hi = hw;
             while (lo < hi) {
                 const uint32_t mid = (lo + hi + 1) / 2;
-                if (v.pw_off[mid] - p0 <= static_cast<uint32_t>(kSegPairs)) lo = mid;
+                if (Lo[mid - s0] - q0 <= static_cast<uint32_t>(kSegPairs)) lo = mid;
                 else hi = mid - 1;
             }
             *seg_end = lo;
         }
         __syncthreads();
         const uint32_t s1 = *seg_end;
-        const uint32_t p0 = v.pw_off[s0], p1 = v.pw_off[s1];
+        const uint32_t p0 = Lo[0], p1 = Lo[s1 - s0];
+        __syncthreads();  // every thread holds p0 and p1 before the offsets are rebased
         for (uint32_t x = tid; x < s1 - s0; x += blockDim.x) {
             Lw[x] = v.pw_list[s0 + x];
-            Lo[x] = v.pw_off[s0 + x] - p0;
+            Lo[x] -= p0;
         }
         if (tid == 0) Lo[s1 - s0] = p1 - p0;
         for (uint32_t q = tid; q < p1 - p0; q += blockDim.x) {
