@@ -1747,7 +1747,8 @@ __global__ __launch_bounds__(256) void k_block_dot(BlockArgs a) {
 
 // The XOR update of tile (x, y) of kTileDirect words, combining the block's final vectors staged in LDS
 // (T[t][i]) directly: 64 masked XORs per word.  Cheaper than the tables of xor_body when a step's tiles
-// hold few columns each (the batched form's small components).
+// hold few columns each (the batched form's small components).  Column tile x covers the listed later
+// columns ulist[256 x ..] (those with nonzero coefficients); tile 0 also writes the block's own columns.
 __device__ __forceinline__ void xor_direct(const BlkView &v, int64_t xt, int32_t yt, uint64_t *lds,
                                            unsigned long long *upd_bytes) {
     constexpr int kW = 8;
@@ -1756,52 +1757,49 @@ __device__ __forceinline__ void xor_direct(const BlkView &v, int64_t xt, int32_t
     const int32_t wl = (v.e + kk + kWave - 1) / kWave;
     const int32_t w0 = yt * kTileDirect;
     const int tn = wl - w0 < kTileDirect ? wl - w0 : kTileDirect;
-    {  // nothing to do unless some column of the tile is the block's own or takes an update
-        const int64_t jc = static_cast<int64_t>(v.e) + xt * blockDim.x + tid;
-        const bool need = jc < static_cast<int64_t>(v.e) + kk || (jc < v.N && v.Bc[jc] != 0);
-        if (!__syncthreads_or(need)) return;
-    }
+    const int64_t J = uniform32(*v.ucount);
+    if (xt > 0 && xt * blockDim.x >= J) return;  // no listed column in this tile
     uint64_t *T = lds;  // [kTileDirect][64]
     for (int x = tid; x < kBlockK * kTileDirect; x += blockDim.x) {
         const int t = x / kBlockK, i = x % kBlockK;
         T[x] = (i < kk && t < tn) ? v.Sf_blk[static_cast<int64_t>(w0 + t) * kBlockK + i] : 0ull;
     }
     __syncthreads();
-    const int64_t j = static_cast<int64_t>(v.e) + xt * blockDim.x + tid;
     uint64_t nbytes = 0;
-    if (j < static_cast<int64_t>(v.e) + kk) {
-        const int i = static_cast<int>(j - v.e);
+    if (xt == 0 && tid < kk) {
+        const int64_t j = static_cast<int64_t>(v.e) + tid;
         for (int t = 0; t < tn; ++t) {
-            uint64_t x = T[t * kBlockK + i];
+            uint64_t x = T[t * kBlockK + tid];
             if ((j >> 6) == w0 + t) x &= ~(1ull << (j & 63));
             v.Sw[static_cast<int64_t>(w0 + t) * v.npad + j] = x;
         }
-    } else if (j < v.N) {
+    }
+    const int64_t idx = xt * blockDim.x + tid;
+    if (idx < J) {
+        const int64_t j = v.ulist[idx];
         const uint64_t b = v.Bc[j];
-        if (b) {
-            for (int t0 = 0; t0 < tn; t0 += kW) {
-                uint64_t xs[kW];
+        for (int t0 = 0; t0 < tn; t0 += kW) {
+            uint64_t xs[kW];
 #pragma unroll
-                for (int u = 0; u < kW; ++u) xs[u] = 0;
+            for (int u = 0; u < kW; ++u) xs[u] = 0;
 #pragma unroll 4
-                for (int i = 0; i < kBlockK; ++i) {
-                    const uint64_t m = 0ull - ((b >> i) & 1ull);
+            for (int i = 0; i < kBlockK; ++i) {
+                const uint64_t m = 0ull - ((b >> i) & 1ull);
 #pragma unroll
-                    for (int u = 0; u < kW; ++u) xs[u] ^= T[(t0 + u) * kBlockK + i] & m;
-                }
-                uint64_t sv[kW];
-#pragma unroll
-                for (int u = 0; u < kW; ++u) {
-                    if (t0 + u >= tn) xs[u] = 0;
-                    sv[u] = xs[u] ? v.Sw[static_cast<int64_t>(w0 + t0 + u) * v.npad + j] : 0ull;
-                }
-#pragma unroll
-                for (int u = 0; u < kW; ++u)
-                    if (xs[u]) {
-                        v.Sw[static_cast<int64_t>(w0 + t0 + u) * v.npad + j] = sv[u] ^ xs[u];
-                        nbytes += 16;
-                    }
+                for (int u = 0; u < kW; ++u) xs[u] ^= T[(t0 + u) * kBlockK + i] & m;
             }
+            uint64_t sv[kW];
+#pragma unroll
+            for (int u = 0; u < kW; ++u) {
+                if (t0 + u >= tn) xs[u] = 0;
+                sv[u] = xs[u] ? v.Sw[static_cast<int64_t>(w0 + t0 + u) * v.npad + j] : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < kW; ++u)
+                if (xs[u]) {
+                    v.Sw[static_cast<int64_t>(w0 + t0 + u) * v.npad + j] = sv[u] ^ xs[u];
+                    nbytes += 16;
+                }
         }
     }
     for (int o = kWave / 2; o > 0; o >>= 1) nbytes += __shfl_down(nbytes, o);
@@ -1968,6 +1966,8 @@ struct BatchArgs {
     uint32_t *sum_cnt, *sum_rank, *sel;
     uint64_t *blk_q, *blk_m, *blk_g, *C_blk, *Sf_blk, *C_store, *Ct, *Bc, *pair_c;
     uint32_t *D_blk, *Dcnt_blk, *pw_list, *pw_off, *pw_cnt, *pair_i;
+    uint32_t *ulist;   // per problem at its coordinate base: the later columns that take this step's update
+    uint32_t *ucount;  // [problems] their number (cleared by k_batch_transpose)
     uint32_t *status;
     unsigned long long *upd_bytes;
     int64_t K;  // rank bound
@@ -2001,8 +2001,8 @@ __device__ __forceinline__ BlkView batch_view(const BatchArgs &a, int g) {
     v.D_blk = a.D_blk + cblk;
     v.pair_i = a.pair_i + cblk;
     v.pair_c = a.pair_c + cblk;
-    v.ulist = nullptr;
-    v.ucount = nullptr;
+    v.ulist = a.ulist + v.cb;
+    v.ucount = a.ucount + g;
     v.Dcnt_blk = a.Dcnt_blk + static_cast<int64_t>(g) * kBlockK;
     v.C_store = a.C_store ? a.C_store + uniform64(c.store) : nullptr;
     v.Tt = a.Tt + v.cb;
@@ -2034,6 +2034,7 @@ __global__ __launch_bounds__(256) void k_batch_transpose(BatchArgs a) {
     const int64_t wl = a.step + 1;
     if (x >= static_cast<int64_t>(a.active) * wl) return;
     const int g = static_cast<int>(x / wl);
+    if (x == static_cast<int64_t>(g) * wl && lane == 0) a.ucount[g] = 0;  // k_batch_dot lists problem g's columns
     transpose_body(batch_view(a, g), x - g * wl, lane);
 }
 
@@ -3666,7 +3667,8 @@ struct mcb_batch {
              *blk_m = nullptr, *blk_g = nullptr, *C_blk = nullptr, *Sf_blk = nullptr, *C_store = nullptr,
              *Ct = nullptr, *Bc = nullptr, *pair_c = nullptr;
     uint32_t *sum_cnt = nullptr, *sum_rank = nullptr, *sel = nullptr, *D_blk = nullptr, *Dcnt_blk = nullptr,
-             *pw_list = nullptr, *pw_off = nullptr, *pw_cnt = nullptr, *pair_i = nullptr, *status = nullptr;
+             *pw_list = nullptr, *pw_off = nullptr, *pw_cnt = nullptr, *pair_i = nullptr, *status = nullptr,
+             *ulist = nullptr, *ucount = nullptr;
     unsigned long long *upd_bytes = nullptr;
     int64_t coords = 0, sw_words = 0, blk_words = 0, store_words = 0, slots = 0;
     std::vector<void *> allocs;
@@ -3759,6 +3761,8 @@ BatchArgs batch_args(const mcb_batch *b, int32_t s) {
     a.pw_off = b->pw_off;
     a.pw_cnt = b->pw_cnt;
     a.pair_i = b->pair_i;
+    a.ulist = b->ulist;
+    a.ucount = b->ucount;
     a.status = b->status;
     a.upd_bytes = b->upd_bytes;
     a.K = b->K;
@@ -4091,6 +4095,7 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
         (rc = bat_alloc(b, &b->pair_i, blk)) || (rc = bat_alloc(b, &b->pair_c, blk)) ||
         (rc = bat_alloc(b, &b->pw_list, pw)) || (rc = bat_alloc(b, &b->pw_off, pwoff)) ||
         (rc = bat_alloc(b, &b->pw_cnt, G)) || (rc = bat_alloc(b, &b->status, 1)) ||
+        (rc = bat_alloc(b, &b->ulist, coords)) || (rc = bat_alloc(b, &b->ucount, G)) ||
         (rc = bat_alloc(b, &b->upd_bytes, 1)))
         return cleanup(rc);
     if (b->opt.store_cycles && (rc = bat_alloc(b, &b->C_store, store))) return cleanup(rc);
