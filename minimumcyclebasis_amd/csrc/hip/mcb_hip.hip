@@ -1642,16 +1642,23 @@ __global__ __launch_bounds__(256) void k_block_pairs(BlockArgs a) {
 // The block's update of a later column j >= e+kk is S_j ^= sum_i b_i S_{e+i} with
 // b_i = <C_{e+i}, S_j> ^ sum_{l<i} b_l <C_{e+i}, S_{e+l}>.  S_j has support below e besides its own
 // bit, so the products read only the listed words (where some C_{e+i} is nonzero below e) plus bit j
-// of each C (Ct[j]).  Column tile t: thread j = e + kk + 256 t + tid computes b into Bc[j].  The list
-// and its pairs are staged in LDS in segments of at most kSegWords words and kSegPairs pairs.
+// of each C (Ct[j]).  Column tile t: lane l of every wave takes column j = e + kk + 64 t + l, the four
+// waves take alternate runs of kB listed words, and wave 0 combines their partial products and computes
+// b into Bc[j].  The list and its pairs are staged in LDS in segments of at most kSegWords words and
+// kSegPairs pairs.
 // LDS: G rows [64] | pair words [kSegPairs] | pair phases [kSegPairs] | listed words [kSegWords] |
 // their pair offsets [kSegWords + 1].
 constexpr int kSegWords = 1024;
 constexpr int kSegPairs = 2048;
+constexpr int kDotWaves = 4;      // waves per product workgroup
+constexpr int kDotCols = kWave;   // later columns per product workgroup
 __device__ __forceinline__ void dot_body(const BlkView &v, int64_t tile, uint64_t *lds, uint32_t *seg_end,
                                          unsigned long long *upd_bytes) {
     constexpr int kB = 16;
+    static_assert(kDotWaves * kWave <= kSegPairs, "partial products fit the pair words");
     const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wv = tid / kWave;
     const int kk = v.kk;
     uint64_t *G = lds;
     uint64_t *Pc = G + kBlockK;
@@ -1659,9 +1666,9 @@ __device__ __forceinline__ void dot_body(const BlkView &v, int64_t tile, uint64_
     uint32_t *Lw = Pi + kSegPairs;
     uint32_t *Lo = Lw + kSegWords;
     for (int x = tid; x < kBlockK; x += blockDim.x) G[x] = x < kk ? v.blk_g[x] : 0ull;
-    const int64_t j = static_cast<int64_t>(v.e) + kk + tile * blockDim.x + tid;
+    const int64_t j = static_cast<int64_t>(v.e) + kk + tile * kDotCols + lane;
     const bool col = j < v.N;
-    uint64_t d = col ? v.Ct[j] : 0ull;
+    uint64_t d = col && wv == 0 ? v.Ct[j] : 0ull;
     const uint32_t nl = *v.pw_cnt;
     for (uint32_t s0 = 0; s0 < nl;) {
         __syncthreads();
@@ -1695,7 +1702,7 @@ __device__ __forceinline__ void dot_body(const BlkView &v, int64_t tile, uint64_
         }
         __syncthreads();
         if (col) {
-            for (uint32_t x0 = 0; x0 < s1 - s0; x0 += kB) {
+            for (uint32_t x0 = wv * kB; x0 < s1 - s0; x0 += kDotWaves * kB) {
                 uint64_t sv[kB];
 #pragma unroll
                 for (int k = 0; k < kB; ++k) {
@@ -1712,7 +1719,11 @@ __device__ __forceinline__ void dot_body(const BlkView &v, int64_t tile, uint64_
         }
         s0 = s1;
     }
+    __syncthreads();  // the pair words are no longer read: they take the waves' partial products
+    Pc[tid] = d;
     __syncthreads();
+    if (wv != 0) return;
+    for (int w = 1; w < kDotWaves; ++w) d ^= Pc[w * kWave + lane];
     uint64_t nbytes = 0, bcol = 0;
     if (col) {
         uint64_t b = 0;
@@ -1728,7 +1739,6 @@ __device__ __forceinline__ void dot_body(const BlkView &v, int64_t tile, uint64_
         const bool upd = col && bcol != 0;
         const uint64_t m = __ballot(upd);
         if (m) {
-            const int lane = tid & (kWave - 1);
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(v.ucount, static_cast<uint32_t>(__popcll(m)));
             base = static_cast<uint32_t>(__shfl(static_cast<int>(base), 0));
@@ -1739,7 +1749,7 @@ __device__ __forceinline__ void dot_body(const BlkView &v, int64_t tile, uint64_
     if ((tid & (kWave - 1)) == 0 && nbytes) atomicAdd(upd_bytes, static_cast<unsigned long long>(nbytes));
 }
 
-__global__ __launch_bounds__(256) void k_block_dot(BlockArgs a) {
+__global__ __launch_bounds__(kDotWaves * kWave) void k_block_dot(BlockArgs a) {
     extern __shared__ uint64_t lds[];
     __shared__ uint32_t seg_end;
     dot_body(view_of(a), blockIdx.x, lds, &seg_end, a.upd_bytes);
@@ -2141,7 +2151,7 @@ __global__ __launch_bounds__(256) void k_batch_pairs(BatchArgs a) {
     pairs_list(v, lds, &nlist, &npairs);
 }
 
-__global__ __launch_bounds__(256) void k_batch_dot(BatchArgs a) {
+__global__ __launch_bounds__(kDotWaves * kWave) void k_batch_dot(BatchArgs a) {
     extern __shared__ uint64_t lds[];
     __shared__ uint32_t seg_end;
     const int32_t x = static_cast<int32_t>(blockIdx.x);
@@ -2769,7 +2779,8 @@ int launch_block(mcb_solver *s, int32_t e, int32_t kk) {
     const int64_t later = s->N - e - kk;
     if (later > 0) {
         const size_t dlds = sizeof(uint64_t) * (kBlockK + kSegPairs) + sizeof(uint32_t) * (kSegPairs + 2 * kSegWords + 1);
-        hipLaunchKernelGGL(k_block_dot, dim3(static_cast<unsigned>((later + 255) / 256)), dim3(256), dlds, s->stream, a);
+        hipLaunchKernelGGL(k_block_dot, dim3(static_cast<unsigned>((later + kDotCols - 1) / kDotCols)),
+                           dim3(kDotWaves * kWave), dlds, s->stream, a);
         HIP_TRY(hipGetLastError());
     }
     // (column tile, word tile) pairs over the block's own columns and the listed later ones, dealt to at
@@ -3820,7 +3831,7 @@ int launch_batch_step(mcb_batch *b, int32_t s) {
     const int32_t nd = b->dot_cum[off + A];
     if (nd > 0) {
         const size_t dlds = sizeof(uint64_t) * (kBlockK + kSegPairs) + sizeof(uint32_t) * (kSegPairs + 2 * kSegWords + 1);
-        hipLaunchKernelGGL(k_batch_dot, dim3(static_cast<unsigned>(nd)), dim3(256), dlds, b->stream, a);
+        hipLaunchKernelGGL(k_batch_dot, dim3(static_cast<unsigned>(nd)), dim3(kDotWaves * kWave), dlds, b->stream, a);
         HIP_TRY(hipGetLastError());
     }
     hipLaunchKernelGGL(k_batch_xor, dim3(static_cast<unsigned>(b->xor_cum[off + A])), dim3(256),
@@ -4064,7 +4075,7 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
         for (int32_t g = 0; g < A; ++g) {
             const int64_t left = b->comps[g].N - static_cast<int64_t>(s) * kBlockK;
             const int64_t later = left - std::min<int64_t>(left, kBlockK);
-            dsum += (later + 255) / 256;
+            dsum += (later + kDotCols - 1) / kDotCols;
             xsum += (left + 255) / 256 * wt;
             if (dsum > 0x7FFFFFFF || xsum > 0x7FFFFFFF) return cleanup(fail(MCB_E_INVALID, "batch too large"));
             b->dot_cum.push_back(static_cast<int32_t>(dsum));
