@@ -1642,15 +1642,15 @@ __global__ __launch_bounds__(256) void k_block_pairs(BlockArgs a) {
 // The block's update of a later column j >= e+kk is S_j ^= sum_i b_i S_{e+i} with
 // b_i = <C_{e+i}, S_j> ^ sum_{l<i} b_l <C_{e+i}, S_{e+l}>.  S_j has support below e besides its own
 // bit, so the products read only the listed words (where some C_{e+i} is nonzero below e) plus bit j
-// of each C (Ct[j]).  Column tile t: lane l of every wave takes column j = e + kk + 64 t + l, the four
-// waves take alternate runs of kB listed words, and wave 0 combines their partial products and computes
-// b into Bc[j].  The list and its pairs are staged in LDS in segments of at most kSegWords words and
-// kSegPairs pairs.
+// of each C (Ct[j]).  Column tile t: lane l of every wave takes column j = e + kk + 64 t + l, the
+// kDotWaves waves take runs of kB listed words in turn (a shorter chain of dependent loads per lane),
+// and wave 0 combines their partial products and computes b into Bc[j].  The list and its pairs are
+// staged in LDS in segments of at most kSegWords words and kSegPairs pairs.
 // LDS: G rows [64] | pair words [kSegPairs] | pair phases [kSegPairs] | listed words [kSegWords] |
 // their pair offsets [kSegWords + 1].
 constexpr int kSegWords = 1024;
 constexpr int kSegPairs = 2048;
-constexpr int kDotWaves = 4;      // waves per product workgroup
+constexpr int kDotWaves = 8;      // waves per product workgroup
 constexpr int kDotCols = kWave;   // later columns per product workgroup
 __device__ __forceinline__ void dot_body(const BlkView &v, int64_t tile, uint64_t *lds, uint32_t *seg_end,
                                          unsigned long long *upd_bytes) {
