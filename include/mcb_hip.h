@@ -10,7 +10,7 @@
  *    include/compressed_trees.h:16-26 (per chunk row: tree_rows u32[chunk*(n+1)],
  *    tree_cols u32[chunk*n], edge_offset i32[chunk*n], precompute_value u32[chunk*n]).
  *    A host driver written against the reference (mcb/mcb_cycle_deg_two_remove.cpp) can call it
- *    through a thin shim.
+ *    through a thin shim: a class gpu_struct whose methods forward to these functions.
  *
  * 2. The native form (mcb_solver_*).  Trees, candidates and every support vector stay resident in
  *    HBM; each phase runs (a) tree-path parity fused with (b) the minimum-rank odd-candidate
