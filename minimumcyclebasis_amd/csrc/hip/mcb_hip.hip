@@ -1204,6 +1204,24 @@ __device__ __forceinline__ void basis_insert_tab(uint64_t &piv, uint64_t &P, uin
 // elimination in rank order picks, since both read the same Schur complement entry.  At the end
 // column l holds the products with the final S_{e+l}, so bit l of the products of phase i's winner
 // is column l's bit at that winner: one ballot per phase.
+// 64 x 64 bit transpose across a wave: lane r holds row r (bit c: entry (r, c)); lane c receives
+// column c (bit r: entry (r, c)).  Six rounds of swapping the off-diagonal j x j blocks of every 2j x 2j
+// block with the partner lane l ^ j, j = 32, 16, ..., 1.
+__device__ __forceinline__ uint64_t transpose64(uint64_t x, int lane) {
+    constexpr uint64_t kLow[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                                  0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int j = 32 >> k;
+        const uint64_t m = kLow[k];
+        const uint32_t lo = static_cast<uint32_t>(__shfl_xor(static_cast<int>(static_cast<uint32_t>(x)), j));
+        const uint32_t hi = static_cast<uint32_t>(__shfl_xor(static_cast<int>(static_cast<uint32_t>(x >> 32)), j));
+        const uint64_t y = (static_cast<uint64_t>(hi) << 32) | lo;
+        x = (lane & j) ? ((x & ~m) | ((y & ~m) >> j)) : ((x & m) | ((y & m) << j));
+    }
+    return x;
+}
+
 __device__ void finalize_core(int kk, const uint32_t *sum_cnt, const uint32_t *sum_rank, const uint64_t *sum_q,
                               int64_t top, uint32_t *sel, uint64_t *blk_q, uint64_t *blk_m, uint64_t *blk_g,
                               uint32_t *status, int lane) {
@@ -1212,11 +1230,9 @@ __device__ void finalize_core(int kk, const uint32_t *sum_cnt, const uint32_t *s
     if (ok) {
         const uint64_t q = lane < kk ? sum_q[top * kWave + lane] : 0ull;
         const uint32_t rk = lane < kk ? sum_rank[top * kWave + lane] : 0u;
-        uint64_t col = 0;
-        for (int j = 0; j < kk; ++j) {
-            const uint64_t bj = __ballot((q >> j) & 1ull);
-            if (lane == j) col = bj;
-        }
+        // column j of the rank-ordered word matrix (words have no bit at or above kk, rows at or above
+        // kk are zero)
+        uint64_t col = transpose64(q, lane);
         uint64_t m = lane < kk ? (1ull << lane) : 0ull;
         uint64_t taken = 0;  // wave-uniform
         int win = 0;         // lane i: rank position of phase i's winner
@@ -1241,12 +1257,12 @@ __device__ void finalize_core(int kk, const uint32_t *sum_cnt, const uint32_t *s
             const uint32_t qhi = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(q >> 32)), win));
             const uint64_t qi = (static_cast<uint64_t>(qhi) << 32) | qlo;
             const uint32_t ri = static_cast<uint32_t>(__shfl(static_cast<int>(rk), win));
-            uint64_t prod = 0;  // lane i: bit l = <C_{e+i}, final S_{e+l}>
-            for (int i = 0; i < kk; ++i) {
-                const int wi = __builtin_amdgcn_readlane(win, i);
-                const uint64_t bi = __ballot((col >> wi) & 1ull);
-                if (lane == i) prod = bi;
-            }
+            // lane i: bit l = <C_{e+i}, final S_{e+l}>, bit win_i of column l: row win_i of the
+            // transposed columns (columns at or above kk are zero)
+            const uint64_t rows = transpose64(col, lane);
+            const uint32_t plo = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(rows)), win));
+            const uint32_t phi = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(rows >> 32)), win));
+            const uint64_t prod = lane < kk ? (static_cast<uint64_t>(phi) << 32) | plo : 0ull;
             const bool odd = lane >= kk || ((prod >> lane) & 1ull);
             if (__ballot(!odd)) ok = false;
             if (lane < kk) {
