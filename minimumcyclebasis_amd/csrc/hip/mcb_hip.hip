@@ -1113,18 +1113,14 @@ __device__ __forceinline__ void basis_insert(uint64_t &piv, uint64_t &P, uint32_
                                              uint32_t *out_rank, uint64_t *out_q, int lane) {
     uint64_t mine = 0;
     basis_reduce_insert(piv, P, q, lane, [&]() { mine = 1; });
-    uint64_t newp = __ballot(mine != 0);
-    while (newp) {
-        const int p = __ffsll(static_cast<unsigned long long>(newp)) - 1;
-        newp &= newp - 1;
-        const uint64_t qp = rl64(q, p);
-        const uint32_t rp = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rank), p));
-        if (lane == 0) {
-            out_rank[cnt] = rp;
-            out_q[cnt] = qp;
-        }
-        ++cnt;
+    // each new pivot lane writes its own entry, after the new pivots of the lanes before it
+    const uint64_t newp = __ballot(mine != 0);
+    if (mine) {
+        const uint32_t at = cnt + static_cast<uint32_t>(__popcll(newp & ((1ull << lane) - 1)));
+        out_rank[at] = rank;
+        out_q[at] = q;
     }
+    cnt += static_cast<uint32_t>(__popcll(newp));
 }
 
 // basis_insert with the reduction by existing pivots done through tables in LDS: tab[g][x] is the
@@ -1178,18 +1174,14 @@ __device__ __forceinline__ void basis_insert_tab(uint64_t &piv, uint64_t &P, uin
         if ((v >> col) & 1ull) v ^= pvv;
         nz = __ballot(v != 0);
     }
-    uint64_t newp = __ballot(mine != 0);
-    while (newp) {
-        const int p = __ffsll(static_cast<unsigned long long>(newp)) - 1;
-        newp &= newp - 1;
-        const uint64_t qp = rl64(q, p);
-        const uint32_t rp = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rank), p));
-        if (lane == 0) {
-            out_rank[cnt] = rp;
-            out_q[cnt] = qp;
-        }
-        ++cnt;
+    // each new pivot lane writes its own entry, after the new pivots of the lanes before it
+    const uint64_t newp = __ballot(mine != 0);
+    if (mine) {
+        const uint32_t at = cnt + static_cast<uint32_t>(__popcll(newp & ((1ull << lane) - 1)));
+        out_rank[at] = rank;
+        out_q[at] = q;
     }
+    cnt += static_cast<uint32_t>(__popcll(newp));
 }
 
 // The block's greedy basis (slot top) in rank order -> phase of each winner, masks m_i, the
