@@ -795,15 +795,29 @@ __device__ __forceinline__ BlkView view_of(const BlockArgs &a) {
     return v;
 }
 
+// 64 x 64 bit transpose across a wave: lane r holds row r (bit c: entry (r, c)); lane c receives
+// column c (bit r: entry (r, c)).  Six rounds of swapping the off-diagonal j x j blocks of every 2j x 2j
+// block with the partner lane l ^ j, j = 32, 16, ..., 1.
+__device__ __forceinline__ uint64_t transpose64(uint64_t x, int lane) {
+    constexpr uint64_t kLow[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                                  0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int j = 32 >> k;
+        const uint64_t m = kLow[k];
+        const uint32_t lo = static_cast<uint32_t>(__shfl_xor(static_cast<int>(static_cast<uint32_t>(x)), j));
+        const uint32_t hi = static_cast<uint32_t>(__shfl_xor(static_cast<int>(static_cast<uint32_t>(x >> 32)), j));
+        const uint64_t y = (static_cast<uint64_t>(hi) << 32) | lo;
+        x = (lane & j) ? ((x & ~m) | ((y & ~m) >> j)) : ((x & m) | ((y & m) << j));
+    }
+    return x;
+}
+
 // Transposes the block's vectors: wave w builds Tt[64w .. 64w+63] from word w of columns
 // e .. e+kk-1 (bit b of that word in lane l is bit l of Tt[64w+b]); own bits are added.
 __device__ __forceinline__ void transpose_body(const BlkView &v, int64_t w, int lane) {
     const uint64_t x = lane < v.kk ? v.Sw[w * v.npad + v.e + lane] : 0ull;
-    uint64_t mine = 0;
-    for (int b = 0; b < kWave; ++b) {
-        const uint64_t m = __ballot(static_cast<int>((x >> b) & 1ull));
-        if (lane == b) mine = m;
-    }
+    uint64_t mine = transpose64(x, lane);
     const int64_t c = w * kWave + lane;
     if (c >= v.e && c < static_cast<int64_t>(v.e) + v.kk) mine |= 1ull << (c - v.e);
     v.Tt[c] = mine;
@@ -1196,24 +1210,6 @@ __device__ __forceinline__ void basis_insert_tab(uint64_t &piv, uint64_t &P, uin
 // elimination in rank order picks, since both read the same Schur complement entry.  At the end
 // column l holds the products with the final S_{e+l}, so bit l of the products of phase i's winner
 // is column l's bit at that winner: one ballot per phase.
-// 64 x 64 bit transpose across a wave: lane r holds row r (bit c: entry (r, c)); lane c receives
-// column c (bit r: entry (r, c)).  Six rounds of swapping the off-diagonal j x j blocks of every 2j x 2j
-// block with the partner lane l ^ j, j = 32, 16, ..., 1.
-__device__ __forceinline__ uint64_t transpose64(uint64_t x, int lane) {
-    constexpr uint64_t kLow[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
-                                  0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        const int j = 32 >> k;
-        const uint64_t m = kLow[k];
-        const uint32_t lo = static_cast<uint32_t>(__shfl_xor(static_cast<int>(static_cast<uint32_t>(x)), j));
-        const uint32_t hi = static_cast<uint32_t>(__shfl_xor(static_cast<int>(static_cast<uint32_t>(x >> 32)), j));
-        const uint64_t y = (static_cast<uint64_t>(hi) << 32) | lo;
-        x = (lane & j) ? ((x & ~m) | ((y & ~m) >> j)) : ((x & m) | ((y & m) << j));
-    }
-    return x;
-}
-
 __device__ void finalize_core(int kk, const uint32_t *sum_cnt, const uint32_t *sum_rank, const uint64_t *sum_q,
                               int64_t top, uint32_t *sel, uint64_t *blk_q, uint64_t *blk_m, uint64_t *blk_g,
                               uint32_t *status, int lane) {
@@ -1526,12 +1522,7 @@ __global__ __launch_bounds__(256) void k_block_cycles_final(BlockArgs a) {
 // Ct[64w + b] bit i = bit b of word w of C_{e+i}: one wave per word.
 __device__ __forceinline__ void pairs_transpose(const BlkView &v, int64_t w, int lane) {
     const uint64_t x = lane < v.kk ? v.C_blk[static_cast<int64_t>(lane) * v.W + w] : 0ull;
-    uint64_t mine = 0;
-    for (int b = 0; b < kWave; ++b) {
-        const uint64_t m = __ballot(static_cast<int>((x >> b) & 1ull));
-        if (lane == b) mine = m;
-    }
-    v.Ct[w * kWave + lane] = mine;
+    v.Ct[w * kWave + lane] = transpose64(x, lane);
 }
 
 // The words below e where some C_{e+i} is nonzero, in ascending order, each with its (i, word of
