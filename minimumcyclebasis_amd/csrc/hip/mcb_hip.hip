@@ -1272,6 +1272,21 @@ __device__ __forceinline__ void block_finalize(const BlockArgs &a, int lane) {
                   a.blk_g, a.status, lane);
 }
 
+// Position of the k-th (from 0) set bit of w, which has more than k set bits: halving by popcounts.
+__device__ __forceinline__ int select_bit(uint64_t w, uint32_t k) {
+    int pos = 0;
+#pragma unroll
+    for (int s = 32; s >= 2; s >>= 1) {
+        const uint32_t c = static_cast<uint32_t>(__popcll(w & ((1ull << s) - 1)));
+        if (k >= c) {
+            k -= c;
+            w >>= s;
+            pos += s;
+        }
+    }
+    return pos + (k >= static_cast<uint32_t>(w & 1ull) ? 1 : 0);
+}
+
 // First level of the block's greedy basis for the ranks [r0, r0 + 64 * nwords) (r0 a multiple of
 // 64, nwords <= 64) into one slot; only the words the tree pass emitted take part (the others cannot
 // be in the basis), and the bitmap words read are cleared for the next block.  Lane l reads bitmap
@@ -1308,9 +1323,7 @@ __device__ __forceinline__ void basis_chunk(uint64_t *qbits, const uint64_t *Qr,
                 if (s_offs[mid] <= t) lo = mid;
                 else hi = mid - 1;
             }
-            uint64_t w = s_words[lo];
-            for (uint32_t k = t - s_offs[lo]; k > 0; --k) w &= w - 1;
-            rank = static_cast<uint32_t>(r0 + lo * kWave + (__ffsll(static_cast<unsigned long long>(w)) - 1));
+            rank = static_cast<uint32_t>(r0 + lo * kWave + select_bit(s_words[lo], t - s_offs[lo]));
             q = Qr[rank];
         }
         basis_insert(piv, P, cnt, q, rank, sum_rank + slot * kWave, sum_q + slot * kWave, lane);
