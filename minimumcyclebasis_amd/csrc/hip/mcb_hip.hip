@@ -1435,8 +1435,9 @@ __global__ __launch_bounds__(kMergeWaves * kWave) void k_block_merge(BlockArgs a
 }
 
 // C of one chosen candidate into LDS (W words, cleared by the caller): the coordinates of the edges
-// on root -> u and root -> v, found by testing every position of the root's tree against the DFS
-// intervals of u and v, and the candidate's own edge.  Coordinates are counted from cb.
+// on root -> u and root -> v, found by testing the positions of the root's tree against the DFS
+// intervals of u and v, and the candidate's own edge.  A parent precedes its child (checked at
+// create), so only the positions up to max(u, v) are tested.  Coordinates are counted from cb.
 __device__ void cycle_into_lds(const int64_t *tree_off, const uint32_t *pos_pre, const uint32_t *pos_end,
                                const int32_t *pos_coord, uint32_t root, uint32_t cu, uint32_t cv, int32_t ccoord,
                                int64_t cb, uint64_t *Cl) {
@@ -1446,20 +1447,22 @@ __device__ void cycle_into_lds(const int64_t *tree_off, const uint32_t *pos_pre,
     const uint32_t npos = static_cast<uint32_t>(tree_off[root + 1] - off);
     const uint32_t pu = pos_pre[off + cu];
     const uint32_t pv = pos_pre[off + cv];
+    const uint32_t last = cu > cv ? cu : cv;
+    const uint32_t lim = last + 1 < npos ? last + 1 : npos;
     if (tid == 0 && ccoord >= 0) {
         const int64_t c = ccoord - cb;
         atomicOr(reinterpret_cast<unsigned long long *>(&Cl[c >> 6]), 1ull << (c & 63));
     }
-    for (uint32_t x0 = 1 + tid; x0 < npos; x0 += kB * blockDim.x) {
+    for (uint32_t x0 = 1 + tid; x0 < lim; x0 += kB * blockDim.x) {
         uint32_t lo[kB], hi[kB];
         int32_t c[kB];
 #pragma unroll
         for (int k = 0; k < kB; ++k) {
             const uint32_t x = x0 + k * blockDim.x;
-            const uint32_t at = x < npos ? x : 0u;
+            const uint32_t at = x < lim ? x : 0u;
             lo[k] = pos_pre[off + at];
             hi[k] = pos_end[off + at];
-            c[k] = x < npos ? pos_coord[off + at] : -1;
+            c[k] = x < lim ? pos_coord[off + at] : -1;
         }
 #pragma unroll
         for (int k = 0; k < kB; ++k) {
