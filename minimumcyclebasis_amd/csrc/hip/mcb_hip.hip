@@ -1630,13 +1630,34 @@ __device__ __forceinline__ void pairs_list(const BlkView &v, uint64_t *lds, uint
             const int i = __ffsll(static_cast<unsigned long long>(m)) - 1;
             m &= m - 1;
             v.pair_i[off] = static_cast<uint32_t>(i);
-            v.pair_c[off] = v.C_blk[static_cast<int64_t>(i) * v.W + w];
+            v.pair_c[off] = static_cast<uint64_t>(w);  // the word; its value of C_{e+i} follows below
             ++off;
         }
     }
     if (tid == 0) {
         v.pw_off[*nlist] = *npairs;
         *v.pw_cnt = *nlist;
+    }
+    __syncthreads();  // the pairs above are visible to the whole workgroup
+    // every pair's word of C at once, four independent loads in flight per thread
+    constexpr int kP = 4;
+    for (uint32_t q0 = tid; q0 < tp; q0 += kP * blockDim.x) {
+        uint32_t pi[kP];
+        uint64_t pw[kP], pc[kP];
+#pragma unroll
+        for (int k = 0; k < kP; ++k) {
+            const uint32_t q = q0 + k * blockDim.x;
+            pi[k] = q < tp ? v.pair_i[q] : 0u;
+            pw[k] = q < tp ? v.pair_c[q] : 0ull;
+        }
+#pragma unroll
+        for (int k = 0; k < kP; ++k) {
+            const int64_t at = static_cast<int64_t>(pi[k]) * v.W + static_cast<int64_t>(pw[k]);
+            pc[k] = q0 + k * blockDim.x < tp ? v.C_blk[at] : 0ull;
+        }
+#pragma unroll
+        for (int k = 0; k < kP; ++k)
+            if (q0 + k * blockDim.x < tp) v.pair_c[q0 + k * blockDim.x] = pc[k];
     }
 }
 
