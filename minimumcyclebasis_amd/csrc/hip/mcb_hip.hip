@@ -700,7 +700,7 @@ struct BlockArgs {
     const uint32_t *pos_pre, *pos_end;
     const uint32_t *big_list, *small_list;  // item ids
     const int32_t *wg_big, *wave_small;     // each workgroup's big items, each wave's small items
-    const uint8_t *big_narrow;  // per big item: 1 if its tree's levels average fewer than 128 positions
+    const uint8_t *big_narrow;  // per big item: 1 if its tree's levels average fewer than 64 positions
     const uint32_t *item_root;  // per item: root, and its candidate run [item_cb, item_ce) of the rc_* arrays
     const int64_t *item_cb, *item_ce;
     uint64_t *scratch;          // [blocks][huge_words] path words of trees too big for LDS
@@ -2533,7 +2533,7 @@ struct TreePlan {
     int32_t nw = 0, big_words = 0, small_words = 0, lds = 0, blocks = 0, n_big = 0, n_small = 0;
     int64_t huge_words = 0;
     uint32_t *big = nullptr, *small = nullptr;
-    uint8_t *narrow = nullptr;  // per big item: 1 if its tree's levels average fewer than 128 positions
+    uint8_t *narrow = nullptr;  // per big item: 1 if its tree's levels average fewer than 64 positions
     // one problem: big items of workgroup b are big[wg_big[b] .. wg_big[b+1]), small items of wave c
     // (workgroup c / nw) small[wave_small[c] .. wave_small[c+1]) (null for a batch: round-robin)
     int32_t *wg_big = nullptr, *wave_small = nullptr;
@@ -3006,7 +3006,7 @@ int plan_tree_pass(int nw, int64_t cap_words, int nsplit, const int64_t *tree_of
             if (np > small_max) {
                 for (int64_t c0 = cb; c0 < ce; c0 += run) {
                     big[x].push_back(add_item(r, c0, std::min(ce, c0 + run)));
-                    narrow[x].push_back(np < 128ll * (depth[r] + 1) ? 1 : 0);
+                    narrow[x].push_back(np < 64ll * (depth[r] + 1) ? 1 : 0);
                     pos += np;
                 }
             } else {
