@@ -2527,7 +2527,7 @@ namespace {
 // a root holding a large share of all candidates spreads over several workgroups (each recomputes the
 // path words).  Big trees whose path words do not fit the LDS next to their ready stamps keep them in
 // scratch, huge_words words per workgroup.
-constexpr int64_t kItemCands = 8192;
+constexpr int64_t kItemCands = 2048;
 
 struct TreePlan {
     int32_t nw = 0, big_words = 0, small_words = 0, lds = 0, blocks = 0, n_big = 0, n_small = 0;
