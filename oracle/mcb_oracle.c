@@ -205,3 +205,175 @@ done:
     free(taken);
     return rc;
 }
+
+/* ---------------------------------------------------------------------------------------------
+ * Per-phase checks on the position layout
+ * ------------------------------------------------------------------------------------------- */
+
+int oracle_position_parity(int32_t F, const int64_t *tree_offset, const uint32_t *parent_pos,
+                           const int32_t *pos_coord, int32_t W, const uint64_t *S, uint8_t *par,
+                           int32_t num_threads) {
+    if (F <= 0 || !tree_offset || !parent_pos || !pos_coord || W <= 0 || !S || !par) return -3;
+    const int threads = num_threads > 0 ? num_threads : omp_get_max_threads();
+    int rc = 0;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 64)
+    for (int32_t r = 0; r < F; ++r) {
+        const int64_t a = tree_offset[r], b = tree_offset[r + 1];
+        if (b <= a || parent_pos[a] != 0xFFFFFFFFu) {
+#pragma omp atomic write
+            rc = -3;
+            continue;
+        }
+        par[a] = 0; /* the root's own value (include/work_per_thread.h:207) */
+        for (int64_t p = a + 1; p < b; ++p) {
+            const uint32_t q = parent_pos[p];
+            const int32_t c = pos_coord[p];
+            if ((int64_t)q >= p - a || c >= 64 * W) {
+#pragma omp atomic write
+                rc = -3;
+                break;
+            }
+            /* a child's value is its parent's plus the S bit of the tree edge's coordinate, mod 2
+             * (include/work_per_thread.h:189-200); parents precede children in the list */
+            par[p] = (uint8_t)(par[a + q] ^ (c >= 0 ? get_bit(S, c) : 0u));
+        }
+    }
+    return rc;
+}
+
+int64_t oracle_phase_select(int32_t F, const int64_t *tree_offset, const uint32_t *parent_pos,
+                            const int32_t *pos_coord, int64_t K, const uint32_t *cand_root,
+                            const uint32_t *cand_pos_u, const uint32_t *cand_pos_v, const int32_t *cand_coord,
+                            int32_t W, const uint64_t *S, const uint8_t *taken, int32_t num_threads) {
+    if (F <= 0 || !tree_offset || K < 0 || !cand_root || !cand_pos_u || !cand_pos_v || !cand_coord || W <= 0 || !S)
+        return -3;
+    const int64_t P = tree_offset[F];
+    uint8_t *par = (uint8_t *)malloc((size_t)(P > 0 ? P : 1));
+    if (!par) return -5;
+    int rc = oracle_position_parity(F, tree_offset, parent_pos, pos_coord, W, S, par, num_threads);
+    if (rc) {
+        free(par);
+        return rc;
+    }
+    int64_t chosen = -1;
+    /* the list in rank order; a chosen candidate was erased from it (mcb/mcb_cycle.cpp:279-283) */
+    for (int64_t k = 0; k < K; ++k) {
+        if (taken && taken[k]) continue;
+        const uint32_t root = cand_root[k];
+        if (root >= (uint32_t)F) { chosen = -3; break; }
+        const int64_t a = tree_offset[root], np = tree_offset[root + 1] - a;
+        if ((int64_t)cand_pos_u[k] >= np || (int64_t)cand_pos_v[k] >= np || cand_coord[k] >= 64 * W) {
+            chosen = -3;
+            break;
+        }
+        unsigned bit = cand_coord[k] >= 0 ? get_bit(S, cand_coord[k]) : 0u;
+        bit ^= par[a + cand_pos_u[k]];
+        bit ^= par[a + cand_pos_v[k]];
+        if (bit == 1) { chosen = k; break; }
+    }
+    free(par);
+    return chosen;
+}
+
+/* The coordinates of rank k's cycle, visited once each: the closing edge, then the edges on the
+ * way up from u and from v to the root (the two paths meet only at the root, include/cycles.h:
+ * 144-195, so setting and toggling bits agree). */
+static int cycle_walk(const int64_t *tree_offset, const uint32_t *parent_pos, const int32_t *pos_coord, int64_t K,
+                      const uint32_t *cand_root, const uint32_t *cand_pos_u, const uint32_t *cand_pos_v,
+                      const int32_t *cand_coord, int64_t k, void (*visit)(void *, int32_t), void *ctx) {
+    if (k < 0 || k >= K) return -3;
+    const int64_t a = tree_offset[cand_root[k]], np = tree_offset[cand_root[k] + 1] - a;
+    if (cand_coord[k] >= 0) visit(ctx, cand_coord[k]);
+    const uint32_t ends[2] = {cand_pos_u[k], cand_pos_v[k]};
+    for (int t = 0; t < 2; ++t) {
+        uint32_t p = ends[t];
+        if ((int64_t)p >= np) return -3;
+        int64_t steps = 0;
+        while (p != 0) {
+            if (pos_coord[a + p] >= 0) visit(ctx, pos_coord[a + p]);
+            p = parent_pos[a + p];
+            if ((int64_t)p >= np || ++steps > np) return -3;
+        }
+    }
+    return 0;
+}
+
+typedef struct {
+    uint64_t *C;
+    int32_t W;
+    int bad;
+} cv_ctx;
+
+static void cv_visit(void *ctx, int32_t c) {
+    cv_ctx *x = (cv_ctx *)ctx;
+    if (c >= 64 * x->W) { x->bad = 1; return; }
+    x->C[c >> 6] ^= 1ull << (c & 63);
+}
+
+int oracle_cycle_vector(const int64_t *tree_offset, const uint32_t *parent_pos, const int32_t *pos_coord,
+                        int64_t K, const uint32_t *cand_root, const uint32_t *cand_pos_u,
+                        const uint32_t *cand_pos_v, const int32_t *cand_coord, int64_t k, int32_t W,
+                        uint64_t *C) {
+    if (!tree_offset || !parent_pos || !pos_coord || !cand_root || !cand_pos_u || !cand_pos_v || !cand_coord ||
+        W <= 0 || !C)
+        return -3;
+    memset(C, 0, sizeof(uint64_t) * (size_t)W);
+    cv_ctx x = {C, W, 0};
+    int rc = cycle_walk(tree_offset, parent_pos, pos_coord, K, cand_root, cand_pos_u, cand_pos_v, cand_coord, k,
+                        cv_visit, &x);
+    return rc ? rc : (x.bad ? -3 : 0);
+}
+
+typedef struct {
+    uint64_t *Y;
+    uint64_t r;
+    int32_t N;
+    int bad;
+} orth_ctx;
+
+static void orth_visit(void *ctx, int32_t c) {
+    orth_ctx *x = (orth_ctx *)ctx;
+    if (c >= x->N) { x->bad = 1; return; }
+    x->Y[c] ^= x->r;
+}
+
+int oracle_orthogonality(const int64_t *tree_offset, const uint32_t *parent_pos, const int32_t *pos_coord,
+                         int64_t K, const uint32_t *cand_root, const uint32_t *cand_pos_u,
+                         const uint32_t *cand_pos_v, const int32_t *cand_coord, int32_t N,
+                         const int32_t *selected, const uint64_t *r, int32_t nsamp, const int32_t *samp,
+                         int32_t W, const uint64_t *S_samp, uint64_t *out) {
+    if (!tree_offset || !parent_pos || !pos_coord || !cand_root || !cand_pos_u || !cand_pos_v || !cand_coord ||
+        N <= 0 || !selected || !r || nsamp < 0 || (nsamp > 0 && (!samp || !S_samp || !out)) || W * 64 < N)
+        return -3;
+    /* Y[c] = XOR of r[i] over the phases i done so far whose cycle vector holds coordinate c; then
+     * bit t of XOR_{c in S_j} Y[c] is XOR over those i with bit t of r[i] of <C_i, S_j>. */
+    uint64_t *Y = (uint64_t *)calloc((size_t)N, sizeof(uint64_t));
+    if (!Y) return -5;
+    int rc = 0;
+    int32_t i = 0;
+    for (int32_t s = 0; s < nsamp && !rc; ++s) {
+        const int32_t j = samp[s];
+        if (j < 0 || j >= N || (s > 0 && j < samp[s - 1])) { rc = -3; break; }
+        for (; i < j && !rc; ++i) {
+            orth_ctx x = {Y, r[i], N, 0};
+            rc = cycle_walk(tree_offset, parent_pos, pos_coord, K, cand_root, cand_pos_u, cand_pos_v, cand_coord,
+                            selected[i], orth_visit, &x);
+            if (!rc && x.bad) rc = -3;
+        }
+        if (rc) break;
+        const uint64_t *Sj = S_samp + (size_t)s * W;
+        uint64_t acc = 0;
+        for (int32_t w = 0; w < W; ++w) {
+            uint64_t bits = Sj[w];
+            while (bits) {
+                const int32_t c = w * 64 + __builtin_ctzll(bits);
+                bits &= bits - 1;
+                if (c >= N) { rc = -3; break; }
+                acc ^= Y[c];
+            }
+        }
+        out[s] = acc;
+    }
+    free(Y);
+    return rc;
+}

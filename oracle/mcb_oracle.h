@@ -40,6 +40,50 @@ int oracle_run(int32_t n, int32_t F, const uint32_t *roots, const uint32_t *rows
                const uint32_t *cand_root, const uint32_t *cand_edge, int32_t phases, int32_t num_threads,
                int32_t *selected, uint64_t *S_out, uint64_t *C_out);
 
+/* ---------------------------------------------------------------------------------------------
+ * Per-phase checks on the position layout (no F x n vertex rows, so they reach the largest graphs).
+ *
+ * A root r's tree is listed as positions tree_offset[r] .. tree_offset[r+1]-1, position 0 the root,
+ * parent_pos[p] (relative to the root's first position) < p, pos_coord[p] the coordinate of the
+ * edge to the parent or -1.  Candidates are in rank order: cand_root, the positions of the two
+ * endpoints in that root's list, and cand_coord, the coordinate of the closing edge or -1.
+ * ------------------------------------------------------------------------------------------- */
+
+/* Parity of every position for S: par[p] = par[parent] ^ S[pos_coord[p]], par[root] = 0
+ * (worker_thread::precompute_supportVec, include/work_per_thread.h:154-210, per position). */
+int oracle_position_parity(int32_t F, const int64_t *tree_offset, const uint32_t *parent_pos,
+                           const int32_t *pos_coord, int32_t W, const uint64_t *S, uint8_t *par,
+                           int32_t num_threads);
+
+/* One phase's list scan (mcb/mcb_cycle.cpp:254-285; mcb/mcb_cycle_deg_two_remove.cpp:329-358):
+ * the first rank k with taken[k] == 0 (taken may be NULL) whose cycle is odd for S, i.e.
+ * S[cand_coord[k]] ^ par[u] ^ par[v] = 1.  Returns that rank, -1 when there is none, -3 on bad
+ * arguments, -5 when out of memory. */
+int64_t oracle_phase_select(int32_t F, const int64_t *tree_offset, const uint32_t *parent_pos,
+                            const int32_t *pos_coord, int64_t K, const uint32_t *cand_root,
+                            const uint32_t *cand_pos_u, const uint32_t *cand_pos_v, const int32_t *cand_coord,
+                            int32_t W, const uint64_t *S, const uint8_t *taken, int32_t num_threads);
+
+/* cycle::get_cycle_vector (include/cycles.h:144-195) of rank k: the coordinates on the paths
+ * root -> u and root -> v and of the closing edge, as W words in C. */
+int oracle_cycle_vector(const int64_t *tree_offset, const uint32_t *parent_pos, const int32_t *pos_coord,
+                        int64_t K, const uint32_t *cand_root, const uint32_t *cand_pos_u,
+                        const uint32_t *cand_pos_v, const int32_t *cand_coord, int64_t k, int32_t W,
+                        uint64_t *C);
+
+/* Orthogonality of later support vectors to earlier cycle vectors (the de Pina invariant
+ * <C_i, S_j> = 0 for i < j, mcb/mcb_cycle.cpp:296-301), tested for every i < j at once through 64
+ * random combinations.  r[i] (i < N) is a random 64-bit word per phase; bit t of out[s] is
+ * XOR over i < samp[s] with bit t of r[i] set of <C_i, S_samp[s]>, where C_i is the cycle vector
+ * of rank selected[i].  samp must be ascending (each < N), S_samp holds their support vectors
+ * (nsamp x W).  All-zero outputs mean every tested pair is orthogonal (a single nonzero product
+ * leaves a given bit zero with probability 1/2, independently per bit). */
+int oracle_orthogonality(const int64_t *tree_offset, const uint32_t *parent_pos, const int32_t *pos_coord,
+                         int64_t K, const uint32_t *cand_root, const uint32_t *cand_pos_u,
+                         const uint32_t *cand_pos_v, const int32_t *cand_coord, int32_t N,
+                         const int32_t *selected, const uint64_t *r, int32_t nsamp, const int32_t *samp,
+                         int32_t W, const uint64_t *S_samp, uint64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

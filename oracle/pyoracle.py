@@ -22,6 +22,11 @@ def lib() -> ctypes.CDLL:
         i32, i64 = ctypes.c_int32, ctypes.c_int64
         L.oracle_parity.argtypes = [i32, i32, P, P, P, P, P, i32, P, P, i32]
         L.oracle_run.argtypes = [i32, i32, P, P, P, P, P, i32, i64, P, P, i32, i32, P, P, P]
+        L.oracle_position_parity.argtypes = [i32, P, P, P, i32, P, P, i32]
+        L.oracle_phase_select.argtypes = [i32, P, P, P, i64, P, P, P, P, i32, P, P, i32]
+        L.oracle_phase_select.restype = i64
+        L.oracle_cycle_vector.argtypes = [P, P, P, i64, P, P, P, P, i64, i32, P]
+        L.oracle_orthogonality.argtypes = [P, P, P, i64, P, P, P, P, i32, P, P, i32, P, i32, P, P]
         _lib = L
     return _lib
 
@@ -67,3 +72,85 @@ def run(problem, phases: int | None = None, num_threads: int = 0, want_s: bool =
     if rc:
         raise RuntimeError(f"oracle_run failed: {rc}")
     return {"selected": sel, "S": S, "C": C}
+
+
+# ---------------------------------------------------------------------------------------------
+# Per-phase checks on the position layout (tree_offset / parent_pos / pos_coord and the rank-ordered
+# candidate arrays of a Problem), for graphs too large for the per-vertex rows above.
+# ---------------------------------------------------------------------------------------------
+
+def _layout(problem):
+    return [problem.array(n) for n in ("tree_offset", "parent_pos", "pos_coord")]
+
+
+def _cands(problem):
+    return [problem.array(n) for n in ("cand_root", "cand_pos_u", "cand_pos_v", "cand_coord")]
+
+
+def position_parity(problem, S: np.ndarray, num_threads: int = 0) -> np.ndarray:
+    """Parity of every position of every root's layout for S (uint8, one per position)."""
+    to, pp, pc = _layout(problem)
+    S = _c(S, np.uint64)
+    out = np.zeros(problem.info.positions, dtype=np.uint8)
+    rc = lib().oracle_position_parity(problem.info.fvs_size, to.ctypes.data, pp.ctypes.data, pc.ctypes.data,
+                                      int(S.size), S.ctypes.data, out.ctypes.data, int(num_threads))
+    if rc:
+        raise RuntimeError(f"oracle_position_parity failed: {rc}")
+    return out
+
+
+def phase_select(problem, S: np.ndarray, taken=None, num_threads: int = 0) -> int:
+    """Rank of the first candidate (in rank order, skipping ``taken`` ranks) whose cycle is odd for S;
+    -1 when none is."""
+    to, pp, pc = _layout(problem)
+    cr, cu, cv, cc = _cands(problem)
+    S = _c(S, np.uint64)
+    K = problem.info.num_candidates
+    tk = None
+    if taken is not None:
+        tk = np.zeros(K, dtype=np.uint8)
+        tk[np.asarray(taken, dtype=np.int64)] = 1
+    k = lib().oracle_phase_select(problem.info.fvs_size, to.ctypes.data, pp.ctypes.data, pc.ctypes.data, K,
+                                  cr.ctypes.data, cu.ctypes.data, cv.ctypes.data, cc.ctypes.data, int(S.size),
+                                  S.ctypes.data, None if tk is None else tk.ctypes.data, int(num_threads))
+    if k < -1:
+        raise RuntimeError(f"oracle_phase_select failed: {k}")
+    return int(k)
+
+
+def cycle_vector(problem, rank: int) -> np.ndarray:
+    """Cycle vector (W words) of the candidate of this rank."""
+    to, pp, pc = _layout(problem)
+    cr, cu, cv, cc = _cands(problem)
+    W = problem.info.words
+    out = np.zeros(W, dtype=np.uint64)
+    rc = lib().oracle_cycle_vector(to.ctypes.data, pp.ctypes.data, pc.ctypes.data, problem.info.num_candidates,
+                                   cr.ctypes.data, cu.ctypes.data, cv.ctypes.data, cc.ctypes.data, int(rank), W,
+                                   out.ctypes.data)
+    if rc:
+        raise RuntimeError(f"oracle_cycle_vector failed: {rc}")
+    return out
+
+
+def orthogonality(problem, selected: np.ndarray, samples, S_samples: np.ndarray, seed: int = 0) -> np.ndarray:
+    """For each sample phase j (ascending), 64 random GF(2) combinations of <C_i, S_j> over every i < j
+    (C_i the cycle vector of rank selected[i]); all zero when S_j is orthogonal to every earlier cycle."""
+    to, pp, pc = _layout(problem)
+    cr, cu, cv, cc = _cands(problem)
+    N, W = problem.info.num_non_tree, problem.info.words
+    sel = _c(selected, np.int32)
+    if sel.size < N:
+        sel = np.concatenate([sel, np.zeros(N - sel.size, dtype=np.int32)])
+    samp = _c(samples, np.int32)
+    if samp.size and int(samp.max()) > 0 and int(sel[: int(samp.max())].min(initial=0)) < 0:
+        raise ValueError("a phase before a sample has no selection")
+    Ss = _c(S_samples, np.uint64).reshape(samp.size, W)
+    r = np.random.default_rng(seed).integers(0, 2**64, size=N, dtype=np.uint64)
+    out = np.zeros(samp.size, dtype=np.uint64)
+    rc = lib().oracle_orthogonality(to.ctypes.data, pp.ctypes.data, pc.ctypes.data, problem.info.num_candidates,
+                                    cr.ctypes.data, cu.ctypes.data, cv.ctypes.data, cc.ctypes.data, N,
+                                    sel.ctypes.data, r.ctypes.data, int(samp.size), samp.ctypes.data, W,
+                                    Ss.ctypes.data, out.ctypes.data)
+    if rc:
+        raise RuntimeError(f"oracle_orthogonality failed: {rc}")
+    return out
