@@ -160,6 +160,11 @@ int mcb_solver_selected(mcb_solver *s, int32_t *out, int32_t count);
 /* Support vectors as used in their phases, row-major N x W uint64 (row e = S_e). */
 int mcb_solver_support_vectors(mcb_solver *s, uint64_t *out);
 
+/* Support vectors S_j of the `count` phases listed in phases[] (each in 0..N-1), as used in their
+ * phases, row-major count x W: the rows of mcb_solver_support_vectors without copying all N x W words
+ * to the host.  Rows of phases not run yet hold their current state. */
+int mcb_solver_support_vector_rows(mcb_solver *s, const int32_t *phases, int32_t count, uint64_t *out);
+
 /* Cycle vectors of the first `count` phases, row-major count x W (needs store_cycles). */
 int mcb_solver_cycle_vectors(mcb_solver *s, uint64_t *out, int32_t count);
 

@@ -62,6 +62,7 @@ SIGNATURES = {
     "mcb_solver_phase": (i32, [P]),
     "mcb_solver_selected": (ctypes.c_int, [P, P, i32]),
     "mcb_solver_support_vectors": (ctypes.c_int, [P, P]),
+    "mcb_solver_support_vector_rows": (ctypes.c_int, [P, P, i32, P]),
     "mcb_solver_cycle_vectors": (ctypes.c_int, [P, P, i32]),
     "mcb_solver_parity": (ctypes.c_int, [P, P, P]),
     "mcb_solver_kernel_time": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]),

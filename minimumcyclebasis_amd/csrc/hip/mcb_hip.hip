@@ -2912,6 +2912,19 @@ void preorder_intervals(const int64_t *tree_offset, const uint32_t *parent_pos, 
     }
 }
 
+// Lets `kernel` take `bytes` of dynamic LDS, beyond the default 64 KB when a workgroup's LDS allows
+// it; MCB_E_INVALID with `what` when it cannot hold them.
+int reserve_dynamic_lds(const void *kernel, int64_t bytes, const char *what) {
+    hipFuncAttributes fattr{};
+    HIP_TRY(hipFuncGetAttributes(&fattr, kernel));
+    const int64_t lds_max =
+        std::min<int64_t>(g_dev.prop.sharedMemPerBlock, 160 * 1024) - static_cast<int64_t>(fattr.sharedSizeBytes);
+    if (bytes > lds_max) return fail(MCB_E_INVALID, what);
+    if (bytes > kLdsLimit)
+        HIP_TRY(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_max)));
+    return MCB_OK;
+}
+
 // Levels below the root of every tree.
 std::vector<uint32_t> tree_depths(const int64_t *tree_off, const uint32_t *parent_pos, int64_t F) {
     std::vector<uint32_t> depth(static_cast<size_t>(F), 0), level;
@@ -3235,6 +3248,13 @@ int setup_blocked(mcb_solver *s, const int64_t *tree_offset, const uint32_t *par
             return fail(MCB_E_INVALID, "a tree is too large for the blocked form's LDS budget");
         return MCB_OK;  // automatic choice: fall back to one kernel per phase
     }
+    // the block's cycle vectors (8 B per word of C) and its pair list (8 B per word below e plus two
+    // 256-entry scans) stage in LDS: up to about 1.29 M coordinates in a 160 KB workgroup
+    if ((rc = reserve_dynamic_lds(reinterpret_cast<const void *>(k_block_cycles_final), 8ll * s->W,
+                                  "support vector too long for the cycle kernel's LDS")) ||
+        (rc = reserve_dynamic_lds(reinterpret_cast<const void *>(k_block_pairs), 8ll * s->W + 2048,
+                                  "support vector too long for the pair kernel's LDS")))
+        return rc;
     s->bk = want > 1 ? want : kBlockK;
     use_plan(s, 0);
     // greedy-basis levels: kChunk ranks per first-level basis, group bases per merge
@@ -3406,20 +3426,29 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
     // its word list (12 bytes per word of C_e).
     const int64_t tree_bytes = 8ll * s->p_words_big + 8ll * ((s->p_words_big + 1) / 2) +
                                8ll * kBlockWaves * s->p_words_small;
+    // Only the per-phase form (and mcb_solver_parity) needs it: 12 W bytes of LDS cap that form at
+    // about 349 000 coordinates, while the blocked form takes up to 8 W + 2 KB of a 160 KB workgroup.
     const int64_t need = std::max<int64_t>(8ll * s->W + tree_bytes, 12ll * s->W);
-    if (need > kLdsLimit)
-        return cleanup(fail(MCB_E_INVALID, "support vector and tree layout exceed the LDS budget"));
-    s->lds_bytes = static_cast<int>(need);
-    int per_cu = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_phase<true, true>, kBlockWaves * kWave,
-                                                         s->lds_bytes));
-    per_cu = std::max(1, per_cu);
-    int64_t blocks = static_cast<int64_t>(per_cu) * g_dev.prop.multiProcessorCount;
-    if (s->opt.max_blocks > 0) blocks = std::min<int64_t>(blocks, s->opt.max_blocks);
-    blocks = std::min<int64_t>(blocks, s->n_big + (s->n_small + kBlockWaves - 1) / kBlockWaves);
+    const bool phase_fits = need <= kLdsLimit;
+    if (!phase_fits && s->opt.block_phases == 1)
+        return cleanup(fail(MCB_E_INVALID, "support vector and tree layout exceed the per-phase kernel's LDS budget"));
+    s->lds_bytes = phase_fits ? static_cast<int>(need) : 0;
+    int64_t blocks = 1;
+    if (phase_fits) {
+        int per_cu = 0;
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_phase<true, true>, kBlockWaves * kWave,
+                                                             s->lds_bytes));
+        per_cu = std::max(1, per_cu);
+        blocks = static_cast<int64_t>(per_cu) * g_dev.prop.multiProcessorCount;
+        if (s->opt.max_blocks > 0) blocks = std::min<int64_t>(blocks, s->opt.max_blocks);
+        blocks = std::min<int64_t>(blocks, s->n_big + (s->n_small + kBlockWaves - 1) / kBlockWaves);
+    }
     s->blocks = static_cast<int>(std::max<int64_t>(1, blocks));
     if ((rc = dev_alloc(&s->wave_stats, 3 * static_cast<size_t>(s->blocks) * s->wpb, B))) return cleanup(rc);
     if ((rc = setup_blocked(s, d->tree_offset, d->parent_pos, d->root_cand_offset))) return cleanup(rc);
+    if (s->bk <= 1 && !phase_fits)
+        return cleanup(fail(MCB_E_INVALID, "a tree exceeds the blocked form's LDS budget and the support vector "
+                                           "the per-phase kernel's"));
     if (s->opt.timing) {
         s->ev_pool.resize(4096);
         s->ev_kind.resize(2048);
@@ -3564,6 +3593,23 @@ int mcb_solver_support_vectors(mcb_solver *s, uint64_t *out) {
     return MCB_OK;
 }
 
+int mcb_solver_support_vector_rows(mcb_solver *s, const int32_t *phases, int32_t count, uint64_t *out) {
+    if (!s || count < 0 || (count > 0 && (!phases || !out))) return fail(MCB_E_INVALID, "invalid arguments");
+    for (int32_t i = 0; i < count; ++i)
+        if (phases[i] < 0 || phases[i] >= s->N) return fail(MCB_E_INVALID, "phase out of range");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const size_t pitch = sizeof(uint64_t) * static_cast<size_t>(s->npad);
+    for (int32_t i = 0; i < count; ++i) {
+        const int32_t j = phases[i];
+        uint64_t *row = out + static_cast<size_t>(i) * s->W;
+        // column j of the word-major matrix: one word per row of W rows
+        HIP_TRY(hipMemcpy2D(row, sizeof(uint64_t), s->Sw + j, pitch, sizeof(uint64_t), static_cast<size_t>(s->W),
+                            hipMemcpyDeviceToHost));
+        row[j >> 6] |= 1ull << (j & 63);
+    }
+    return MCB_OK;
+}
+
 int mcb_solver_cycle_vectors(mcb_solver *s, uint64_t *out, int32_t count) {
     if (!s || !out || count < 0 || count > s->N) return fail(MCB_E_INVALID, "invalid arguments");
     if (!s->C_store) return fail(MCB_E_STATE, "solver was created without store_cycles");
@@ -3574,6 +3620,7 @@ int mcb_solver_cycle_vectors(mcb_solver *s, uint64_t *out, int32_t count) {
 
 int mcb_solver_parity(mcb_solver *s, const uint64_t *S, uint8_t *out) {
     if (!s || !S || !out) return fail(MCB_E_INVALID, "invalid arguments");
+    if (s->lds_bytes == 0) return fail(MCB_E_STATE, "support vector too long for the per-phase parity kernel's LDS");
     uint64_t *dS = nullptr;
     uint8_t *dP = nullptr;
     int64_t dummy = 0;
@@ -4009,6 +4056,12 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
     b->slots = slots;
     b->merge_levels = max_levels - 1;
     b->steps = (b->comps[0].N + kBlockK - 1) / kBlockK;
+    // the largest problem's cycle vectors and pair list stage in LDS (as in the single-problem form)
+    if ((rc = reserve_dynamic_lds(reinterpret_cast<const void *>(k_batch_cycles), 8ll * b->comps[0].W,
+                                  "support vector too long for the cycle kernel's LDS")) ||
+        (rc = reserve_dynamic_lds(reinterpret_cast<const void *>(k_batch_pairs), 8ll * b->steps + 2048,
+                                  "support vector too long for the pair kernel's LDS")))
+        return cleanup(rc);
 
     // Arena arrays: trees and candidates, problem after problem in arena order, coordinates and ranks
     // shifted into each problem's range.

@@ -121,6 +121,14 @@ class Solver:
         _lib.check_hip(_lib.hip().mcb_solver_support_vectors(self._h, out.ctypes.data))
         return out
 
+    def support_vector_rows(self, phases) -> np.ndarray:
+        """Rows S_j (W words each) of the listed phases, without copying the whole N x W matrix."""
+        ph = np.ascontiguousarray(phases, dtype=np.int32)
+        out = np.zeros((ph.size, self.W), dtype=np.uint64)
+        _lib.check_hip(_lib.hip().mcb_solver_support_vector_rows(self._h, ph.ctypes.data, int(ph.size),
+                                                                  out.ctypes.data))
+        return out
+
     def cycle_vectors(self, count: int | None = None) -> np.ndarray:
         count = self.phase if count is None else int(count)
         out = np.zeros((count, self.W), dtype=np.uint64)

@@ -1,10 +1,13 @@
 """Full-size GPU runs: the 64x64 grid against the reference's own stats CSV (Total_Weight,
 Final_cycles) and the oracle's phase sequence; the Erdos-Renyi 20000/60000 LBC workload through
 size-independent properties and a phase-prefix comparison with the oracle."""
+import os
+
 import numpy as np
 import pytest
 
-from helpers import graph_of, load_csv
+from fullsize import check_loop, sample_phases
+from helpers import GOLDEN, edge_digest, graph_of, load_csv
 from minimumcyclebasis_amd import generators as G
 from minimumcyclebasis_amd.device import Solver, minimum_cycle_basis
 from minimumcyclebasis_amd.host import Problem
@@ -76,17 +79,20 @@ def test_er20000_properties_and_prefix():
     assert sel1.size == N and np.unique(sel1).size == N
     assert np.array_equal(sel1, sel2)
     assert np.array_equal(sel1, sel3)
-    # cycle weights are non-decreasing in rank; every chosen rank is a valid candidate
-    assert sel1.max() < p_full.info.num_candidates
-    # S_e is a unit vector plus lower coordinates only (triangular support)
-    for e in (0, 1, N // 3, N - 1):
-        row = S[e]
-        bits = np.flatnonzero(np.unpackbits(row.view(np.uint8), bitorder="little"))
-        assert bits.max() == e
-    # the first phases against the oracle
+    # sampled phases given the GPU's own S_e (selection, <C_e, S_e> = 1, triangular support) and every
+    # later S_j orthogonal to all earlier cycles
+    check_loop(p_pr, sel2, lambda ph: S[ph], sample_phases(N, spread=60), seed=7, label="er20000")
+    # the first phases against the whole-loop oracle
     p_o = Problem.from_edges(n, u, v, w, keep_vertex_trees=True)
     ref = pyoracle.run(p_o, phases=12)
     assert np.array_equal(sel1[:12], ref["selected"])
+    # the reference's own mcb_cpu on the same graph (its stats CSV, when the fixture has been written)
+    if os.path.exists(os.path.join(GOLDEN, "csv_er20000.json")):
+        fx = load_csv("er20000")
+        assert edge_digest(n, u, v, w) == fx["edge_digest"]
+        assert int(p_pr.array("cand_weight")[sel2].astype(np.int64).sum()) == fx["mcb_cpu"]["total_weight"]
+        assert N == fx["mcb_cpu"]["final_cycles"]
+        assert p_pr.info.fvs_size == fx["mcb_cpu"]["fvs_size"]
 
 
 def test_components_driver_matches_whole_graph():

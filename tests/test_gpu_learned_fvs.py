@@ -1,15 +1,16 @@
 """GPU tests of the learned FVS scorer: the device MLP's forward pass and gradient against the CPU oracle
 (oracle/mlp.py with the device's BF16 rounding points), training on the device lowering the loss, and the
 shipped parameters giving smaller feedback vertex sets than the reference order on held-out graphs of the
-training generator, with the same MCB weight."""
+training generator, and smaller ones than the same round construction scored by degree alone (what the
+network adds), with the same MCB weight."""
 import numpy as np
 import pytest
 
 from minimumcyclebasis_amd import generators as G
 from minimumcyclebasis_amd.device import minimum_cycle_basis
 from minimumcyclebasis_amd.host import Problem
-from minimumcyclebasis_amd.learned_fvs import (N_PARAMS, Scorer, choose_order, default_scorer, training_rows,
-                                               training_set)
+from minimumcyclebasis_amd.learned_fvs import (N_PARAMS, Scorer, choose_order, default_scorer, degree_score,
+                                               graph_of, scored_greedy, training_rows, training_set)
 from oracle import mlp
 
 pytestmark = pytest.mark.gpu
@@ -94,6 +95,25 @@ def test_shipped_scorer_shrinks_the_fvs():
         assert ch.used == (ch.fvs_learned < ch.fvs_reference)
         p.close()
     assert used >= 1 and learned < ref, (learned, ref)
+
+
+def test_trained_network_beats_degree_scoring():
+    """What the network adds: on held-out seeds of the training generator (n log-uniform in [2000, 20000]),
+    the round construction scored by the trained network takes fewer roots in total than the same
+    construction scored by the remaining degree alone (learned_fvs.degree_score)."""
+    sc = default_scorer()
+    net = deg = 0
+    for seed in range(900, 912):
+        n = int(np.exp(np.random.default_rng(seed).uniform(np.log(2000), np.log(20000))))
+        p = Problem.reduced_graph(*G.chung_lu_lbc(n, 2.3, 8.0, 10000, seed=seed))
+        ro, cols, w = graph_of(p)
+        a = int((scored_greedy(ro, cols, w, sc.logits)[0] >= 0).sum())
+        b = int((scored_greedy(ro, cols, w, degree_score)[0] >= 0).sum())
+        print(f"case held-out seed={seed} n={n} F network={a} degree={b} reference={p.info.fvs_size}", flush=True)
+        net += a
+        deg += b
+        p.close()
+    assert net < deg, (net, deg)
 
 
 @pytest.mark.parametrize("seed", [11, 12])
