@@ -3,10 +3,13 @@
 
 A step is one complete de Pina phase loop (all N phases: tree-path parity + minimum-rank odd
 candidate search, cycle vector, GF(2) support-vector update) over one synthetic graph whose
-trees and candidates are already resident in HBM; the host preprocessing runs once, untimed,
-and is reported separately.  Prints one JSON line.
+trees and candidates are already resident in HBM.  The default workload is the 100 000-point
+planar Delaunay graph, timed in two forms side by side: plain (the reference's FVS order) and
+learned (the FVS order of the learned vertex scorer); each form's host preprocessing and upload
+are timed once and reported with its end-to-end time.  Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--workload er20000] [--no-prune] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--workload delaunay100k] [--fvs both|plain|learned]
+                    [--no-prune] [--no-cpu-baseline]
 """
 from __future__ import annotations
 
@@ -27,12 +30,12 @@ METRIC = "MCB wall-time & support-phase HBM GB/s; batched+learned-FVS vs plain, 
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
-    # the configuration the metric is quoted on (fits one GPU)
+    # the planar target configuration (fits one GPU): the default
+    "delaunay100k": ("Delaunay triangulation of 100000 uniform points, weight round(1000*length), seed 1",
+                     lambda: G.delaunay(100000, seed=1)),
     "er20000": ("Erdos-Renyi G(n=20000, m=60000), largest biconnected component, weights U{1..1000}, seed 1",
                 lambda: G.erdos_renyi_lbc(20000, 60000, 1000, seed=1)),
     "grid64": ("64x64 grid, weights U{1..100}, seed 1", lambda: G.grid(64, 64, 100, seed=1)),
-    "delaunay100k": ("Delaunay triangulation of 100000 uniform points, weight round(1000*length), seed 1",
-                     lambda: G.delaunay(100000, seed=1)),
     "chunglu50000": ("Chung-Lu n=50000 gamma 2.3 avg degree 8, LBC, weights U{1..10000}, seed 1",
                      lambda: G.chung_lu_lbc(50000, 2.3, 8.0, 10000, seed=1)),
 }
@@ -61,26 +64,59 @@ def host_cores() -> int:
     return max(1, n)
 
 
-def cpu_baseline(n, u, v, w, budget_s: float, threads: int) -> dict:
-    """The CPU oracle (a restatement of the reference's CPU phase loop, mcb/mcb_cycle.cpp:226-305,
-    same preprocessing) on the first X phases of the same workload, X chosen for ~budget_s of work;
-    the full-loop time is extrapolated linearly in phases."""
+def cpu_baseline(n, u, v, w, sel: np.ndarray, S: np.ndarray, threads: int, prune: bool, phases: int = 12,
+                 roots: int = 512) -> dict:
+    """The CPU oracle (the reference's CPU phase loop, mcb/mcb_cycle.cpp:237-301, restated in C) timed on
+    ``phases`` phases spread evenly over the whole loop, each started from the GPU's own S_e (``S``, every
+    support vector as used in its phase; ``sel`` the GPU's selections, erased from the list as the reference
+    erases them).  Per sampled phase: the per-vertex parity BFS of ``roots`` roots spread over the FVS (scaled by
+    F / roots: every root's BFS covers its whole tree every phase), the list scan up to the first odd candidate,
+    and the cycle vector with the products of every later vector over all W words.  The full-loop time is the
+    mean scaled parity time times N plus the scan and product times integrated over the phases (trapezoid
+    rule).  The XORs of the update are inside the timed call but run on the GPU's final later vectors, whose
+    products with C_e are all even, so they cost nothing here (the reference XORs where a product is odd)."""
     from minimumcyclebasis_amd.host import Problem
     from oracle import pyoracle
 
-    p = Problem.from_edges(n, u, v, w, keep_vertex_trees=True, num_threads=threads)
-    N = p.info.num_non_tree
-    X = 1
-    while True:  # each run restarts at phase 0; grow X until one run takes about budget_s
-        t0 = time.perf_counter()
-        pyoracle.run(p, phases=X, num_threads=threads)
-        tx = time.perf_counter() - t0
-        if tx >= 0.5 * budget_s or X >= N:
-            break
-        X = int(min(N, max(2 * X, X * budget_s / max(tx, 1e-6))))
+    t_build = time.perf_counter()
+    p = Problem.from_edges(n, u, v, w, prune_trees=prune, keep_vertex_trees=True, num_threads=threads)
+    t_build = time.perf_counter() - t_build
+    info = p.info
+    N, F, nv, K = info.num_non_tree, info.fvs_size, info.nodes, info.num_candidates
+    if S.shape != (N, info.words) or sel.size != N:
+        raise ValueError("the GPU's support vectors and selections do not match the problem")
+    to = p.array("tree_offset")
+    flat = (np.repeat(np.arange(F, dtype=np.int64), np.diff(to)) * nv + p.array("pos_vertex").astype(np.int64))
+    P = np.zeros(F * nv, dtype=np.uint8)
+    sample = np.unique(np.linspace(0, N - 1, min(phases, N)).astype(np.int64))
+    rs = np.unique(np.linspace(0, F - 1, min(roots, F)).astype(np.int32))
+    t_par, t_scan, t_upd = [], [], []
+    for e in sample:
+        # the per-vertex parity of every root at the candidates' endpoints, untimed (the sampled roots' BFS,
+        # timed, then covers their whole trees)
+        P[flat] = pyoracle.position_parity(p, S[e], num_threads=threads)
+        later = S[e + 1:]
+        k, a, b, c = pyoracle.phase_timed(p, int(e), S[e], sel[:e], P, later, rs, num_threads=threads)
+        if k != sel[e]:
+            raise RuntimeError(f"CPU oracle chose rank {k} in phase {e}, the GPU {sel[e]}")
+        t_par.append(a * F / rs.size)
+        t_scan.append(b)
+        t_upd.append(c)
+    rest = np.asarray(t_scan) + np.asarray(t_upd)
+    total = float(np.mean(t_par)) * N
+    if sample.size > 1:  # sum over e = 0..N-1 of the piecewise-linear interpolation of the samples
+        total += float(np.sum((rest[1:] + rest[:-1]) * np.diff(sample)) / 2.0 + (rest[0] + rest[-1]) / 2.0)
+    else:
+        total += float(rest[0]) * N
+    spent = float(np.sum(t_par) * rs.size / F + np.sum(rest))
     p.close()
-    return {"value": tx * N / X, "unit": "s per full phase loop (extrapolated)", "cores": threads, "kind": "port",
-            "sample": f"first {X} of {N} phases timed ({tx:.2f} s), scaled by N/{X}"}
+    return {"value": round(total, 1), "unit": "s per full phase loop (extrapolated)", "cores": threads,
+            "kind": "port",
+            "sample": f"{sample.size} phases evenly spread over all {N} (each started from the GPU's S_e, chosen "
+                      f"rank checked equal), {spent:.2f} s timed: parity BFS of {rs.size} of {F} roots scaled by "
+                      f"{F / rs.size:.1f}, list scan and products with all later vectors timed whole; per phase "
+                      f"parity {np.mean(t_par):.3f} s, scan+products {rest.min():.3f}..{rest.max():.3f} s; "
+                      f"vertex trees built in {t_build:.1f} s, untimed"}
 
 
 def measured_traffic(kernel: str, workload: str | None, key: str = "traffic_bytes_per_launch:"):
@@ -101,9 +137,9 @@ def measured_traffic(kernel: str, workload: str | None, key: str = "traffic_byte
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="er20000", choices=sorted(WORKLOADS) + sorted(COMPONENT_WORKLOADS))
+    ap.add_argument("--workload", default="delaunay100k", choices=sorted(WORKLOADS) + sorted(COMPONENT_WORKLOADS))
     ap.add_argument("--no-plain", action="store_true",
                     help="component workloads: skip the plain one-solver-per-component timing")
     ap.add_argument("--tree-waves", type=int, default=0,
@@ -112,9 +148,9 @@ def main() -> int:
                     help="largest tree kept in LDS by the tree pass (0 timed at create, -1 as large as fits)")
     ap.add_argument("--merge-group", type=int, default=0,
                     help="first-level bases merged per wave of the greedy basis (0 automatic, 2..64)")
-    ap.add_argument("--fvs", choices=["plain", "learned"], default="plain",
-                    help="single graphs: FVS construction (learned: the MLP-ordered greedy; the plain-FVS loop "
-                         "is then timed beside it and the weights compared)")
+    ap.add_argument("--fvs", choices=["both", "plain", "learned"], default="both",
+                    help="single graphs: FVS order(s) timed.  both: the plain (reference order) and the learned "
+                         "form side by side, the learned one the headline; their weights must agree")
     ap.add_argument("--graph", action="store_true",
                     help="also time the loop with no HIP events: stream-launched, and replayed as one HIP graph")
     ap.add_argument("--no-prune", action="store_true")
@@ -123,7 +159,8 @@ def main() -> int:
     ap.add_argument("--block-phases", type=int, default=0,
                     help="phases per pass over the trees (0 automatic, 1 one kernel per phase)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="component workloads: CPU oracle seconds")
+    ap.add_argument("--cpu-phases", type=int, default=12, help="single graphs: phases the CPU oracle times")
     ap.add_argument("--phases", type=int, default=0,
                     help="profiling aid: run only the first PHASES phases per step (0 = the full loop)")
     ap.add_argument("--progress-phases", type=int, default=0, metavar="K",
@@ -136,8 +173,7 @@ def main() -> int:
               "(multi-GPU sharding by root is a design note, not part of this build)", file=sys.stderr)
         return 2
 
-    from minimumcyclebasis_amd.device import Solver, copy_bandwidth, device_info
-    from minimumcyclebasis_amd.host import Problem
+    from minimumcyclebasis_amd.device import copy_bandwidth
 
     if args.copy_kernel:
         gbs = copy_bandwidth(args.copy_kernel, iters=max(1, args.steps))
@@ -147,13 +183,20 @@ def main() -> int:
 
     if args.workload in COMPONENT_WORKLOADS:
         return run_components(args)
-    desc, gen = WORKLOADS[args.workload]
-    n, u, v, w = gen()
-    threads = host_cores()
+    return run_single(args)
+
+
+def time_leg(form: str, graph, args, threads: int) -> dict:
+    """One form of the single-graph workload: host preprocessing (with the learned FVS order for
+    form "learned"), upload (solver creation), then `warmup` untimed and `steps` timed phase loops."""
+    from minimumcyclebasis_amd.device import KERNELS, Solver
+    from minimumcyclebasis_amd.host import Problem
+
+    n, u, v, w = graph
     t0 = time.perf_counter()
     fvs_info = None
     order = None
-    if args.fvs == "learned":
+    if form == "learned":
         from minimumcyclebasis_amd.learned_fvs import learned_order
 
         ch = learned_order(n, u, v, w)
@@ -163,14 +206,13 @@ def main() -> int:
     prob = Problem.from_edges(n, u, v, w, prune_trees=not args.no_prune, num_threads=threads, fvs_order=order)
     host_s = time.perf_counter() - t0
     info = prob.info
-    print(f"[bench] {args.workload}: n'={info.nodes} m'={info.directed_edges // 2} N={info.num_non_tree} "
+    print(f"[bench] {args.workload} {form}: n'={info.nodes} m'={info.directed_edges // 2} N={info.num_non_tree} "
           f"F={info.fvs_size} K={info.num_candidates} positions={info.positions} host={host_s:.2f}s",
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     solver = Solver(prob, timing=1 if args.breakdown else 2, block_phases=args.block_phases,
                     waves_per_block=args.tree_waves, tree_cap=args.tree_cap, merge_group=args.merge_group)
     upload_s = time.perf_counter() - t0
-    dev = device_info()
     phases = args.phases if 0 < args.phases < info.num_non_tree else None
 
     def run_loop():
@@ -187,10 +229,7 @@ def main() -> int:
     for i in range(args.warmup):
         solver.reset()
         run_loop()
-        print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
-
-    from minimumcyclebasis_amd.device import KERNELS
-
+        print(f"[bench] {form} warmup {i} done", file=sys.stderr, flush=True)
     kt = {k: [0.0, 0] for k in KERNELS}
     pos = cand = upd = 0
     t_total = 0.0
@@ -210,9 +249,37 @@ def main() -> int:
         cand += c_
         upd += u_
         sel = solver.selected()
-        print(f"[bench] step {i}: {dt:.3f} s", file=sys.stderr, flush=True)
-    ms_per_step = 1000.0 * t_total / args.steps
-    total_weight = int(prob.array("cand_weight")[sel].astype(np.int64).sum())
+        print(f"[bench] {form} step {i}: {dt:.3f} s", file=sys.stderr, flush=True)
+    loop_s = t_total / args.steps
+    return {"form": form, "prob": prob, "solver": solver, "host_s": host_s, "upload_s": upload_s, "loop_s": loop_s,
+            "kt": kt, "pos": pos, "cand": cand, "upd": upd, "sel": sel, "shape": solver.launch_shape(),
+            "fvs": fvs_info, "total_weight": int(prob.array("cand_weight")[sel].astype(np.int64).sum())}
+
+
+def leg_summary(leg: dict) -> dict:
+    info = leg["prob"].info
+    out = {"fvs_size": info.fvs_size, "candidates": info.num_candidates, "tree_positions": info.positions,
+           "host_preprocess_s": round(leg["host_s"], 3), "upload_s": round(leg["upload_s"], 3),
+           "loop_s": round(leg["loop_s"], 6),
+           "end_to_end_s": round(leg["host_s"] + leg["upload_s"] + leg["loop_s"], 3),
+           "total_weight": leg["total_weight"]}
+    if leg["fvs"]:
+        out.update(leg["fvs"])
+    return out
+
+
+def run_single(args) -> int:
+    from minimumcyclebasis_amd.device import Solver, device_info
+
+    desc, gen = WORKLOADS[args.workload]
+    graph = gen()
+    threads = host_cores()
+    forms = ["plain", "learned"] if args.fvs == "both" else [args.fvs]
+    legs = {f: time_leg(f, graph, args, threads) for f in forms}
+    head = legs[forms[-1]]
+    dev = device_info()
+    prob, info, steps = head["prob"], head["prob"].info, args.steps
+    kt, pos, cand, upd = head["kt"], head["pos"], head["cand"], head["upd"]
 
     # Algorithmic bytes of one launch of the dominant kernel.  One kernel per phase: 8 per tree
     # position streamed (u32 parent position + i32 coordinate), 16 per candidate read (u_pos, v_pos,
@@ -220,7 +287,7 @@ def main() -> int:
     # counts them.  Blocked: the tree pass streams every position (8 B) and every candidate (16 B);
     # its few independent candidate words are written sparsely and the update is a kernel of its
     # own, reported apart.
-    shape = solver.launch_shape()
+    shape = head["shape"]
     dominant = "k_phase" if shape["block_phases"] == 1 else "k_block_q"
     k_ms, k_n = kt["parity_search"]
     avg_ms = k_ms / max(k_n, 1)
@@ -229,15 +296,15 @@ def main() -> int:
     dom_bytes = 8.0 * pos + cand_bytes * cand + (0 if blocked else upd)
     bytes_per_launch = dom_bytes / max(k_n, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-
+    loop_s = head["loop_s"]
     out = {
         "metric": METRIC,
-        "value": round(ms_per_step / 1000.0, 6),
+        "value": round(loop_s, 6),
         "unit": "s per MCB phase loop",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(1000.0 * loop_s, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -253,18 +320,19 @@ def main() -> int:
             "candidates": info.num_candidates,
             "tree_positions": info.positions,
             "pruned_trees": not args.no_prune,
-            "form": "plain",
+            "form": "learned-fvs" if head["form"] == "learned" else "plain",
             "device": dev["name"],
         },
-        "total_weight": total_weight,
-        "host_preprocess_s": round(host_s, 3),
-        "upload_s": round(upload_s, 3),
+        "total_weight": head["total_weight"],
+        "host_preprocess_s": round(head["host_s"], 3),
+        "upload_s": round(head["upload_s"], 3),
+        "end_to_end_s": round(head["host_s"] + head["upload_s"] + loop_s, 3),
         "block_phases": shape["block_phases"],
         "tree_pass_waves": shape["waves_per_block"],
         "tree_pass_lds": shape["lds_bytes"],
-        "launches_per_step": k_n // max(args.steps, 1),
-        "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in kt.items() if v[1]},
-        "update_bytes_per_step": upd // max(args.steps, 1),
+        "launches_per_step": k_n // max(steps, 1),
+        "kernel_ms_per_step": {k: round(v[0] / steps, 3) for k, v in kt.items() if v[1]},
+        "update_bytes_per_step": upd // max(steps, 1),
         "roofline": {
             "kernel": dominant,
             "bound": "hbm",
@@ -279,42 +347,41 @@ def main() -> int:
                             "support_update": 0.0 if blocked else round(upd / max(k_n, 1), 1)},
             "avg_launch_us": round(avg_ms * 1000.0, 3),
         },
-        "roofline_update": update_roofline(kt, upd, args.steps, "k_block_dot+k_block_xor",
+        "roofline_update": update_roofline(kt, upd, steps, "k_block_dot+k_block_xor",
                                            f"k_block_update_{args.workload}") if blocked and not args.no_prune else None,
     }
-    solver.close()
+    if len(legs) == 2:
+        pl, le = legs["plain"], legs["learned"]
+        out["forms"] = {"plain": leg_summary(pl), "learned": leg_summary(le)}
+        e2e = {f: out["forms"][f]["end_to_end_s"] for f in ("plain", "learned")}
+        out["speedup_vs_plain"] = {"loop": round(pl["loop_s"] / le["loop_s"], 3),
+                                   "end_to_end": round(e2e["plain"] / e2e["learned"], 3)}
+        out["weights_equal"] = pl["total_weight"] == le["total_weight"]
+        if not out["weights_equal"]:
+            print("[bench] learned and plain FVS give different MCB weights", file=sys.stderr, flush=True)
+            return 1
+    elif head["fvs"]:
+        out["learned_fvs"] = head["fvs"]
     if args.graph:
+        sel = head["sel"]
         out["untimed"] = untimed_legs(
             lambda: Solver(prob, block_phases=args.block_phases, waves_per_block=args.tree_waves,
                            tree_cap=args.tree_cap),
             args.steps, lambda o: np.array_equal(o.selected(), sel))
-    if fvs_info is not None:
-        # the same loop with the reference FVS order, for the comparison and the weight check
-        t0 = time.perf_counter()
-        plain = Problem.from_edges(n, u, v, w, prune_trees=not args.no_prune, num_threads=threads)
-        fvs_info["plain_host_preprocess_s"] = round(time.perf_counter() - t0, 3)
-        ps = Solver(plain, block_phases=args.block_phases)
-        ps.run()
-        ps.sync()
-        ps.reset()
-        t0 = time.perf_counter()
-        ps.run()
-        ps.sync()
-        fvs_info["plain_loop_s"] = round(time.perf_counter() - t0, 6)
-        plain_weight = int(plain.array("cand_weight")[ps.selected()].astype(np.int64).sum())
-        ps.close()
-        fvs_info["plain_candidates"] = plain.info.num_candidates
-        fvs_info["plain_tree_positions"] = plain.info.positions
-        fvs_info["weights_equal"] = plain_weight == total_weight
-        fvs_info["speedup_vs_plain"] = round(fvs_info["plain_loop_s"] / (ms_per_step / 1000.0), 3)
-        plain.close()
-        out["config"]["form"] = "learned-fvs"
-        out["learned_fvs"] = fvs_info
-        if not fvs_info["weights_equal"]:
-            print("[bench] learned and plain FVS give different MCB weights", file=sys.stderr, flush=True)
-            return 1
+    base = legs.get("plain", head)
+    S = base["solver"].support_vectors() if not args.no_cpu_baseline else None
+    for leg in legs.values():
+        leg["solver"].close()
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(n, u, v, w, args.cpu_budget, threads)
+        # the reference's FVS order: the CPU oracle restates the reference's loop on the reference's root set
+        if base["form"] != "plain":
+            raise RuntimeError("the CPU baseline needs the plain form's loop")
+        print("[bench] CPU baseline: sampled phases of the CPU oracle", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(*graph, base["sel"], S, threads, prune=not args.no_prune,
+                                           phases=args.cpu_phases)
+        del S
+    for leg in legs.values():
+        leg["prob"].close()
     print(json.dumps(out), flush=True)
     return 0
 

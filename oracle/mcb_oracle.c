@@ -377,3 +377,93 @@ int oracle_orthogonality(const int64_t *tree_offset, const uint32_t *parent_pos,
     free(Y);
     return rc;
 }
+
+/* ---------------------------------------------------------------------------------------------
+ * One timed phase (the CPU baseline)
+ * ------------------------------------------------------------------------------------------- */
+
+int64_t oracle_phase_timed(int32_t n, int32_t F, const uint32_t *roots, const uint32_t *rows, const uint32_t *cols,
+                           const int32_t *vertex_parent, const int32_t *nt_map, int32_t N, int64_t K,
+                           const uint32_t *cand_root, const uint32_t *cand_edge, const int32_t *root_sample,
+                           int32_t nsample, int32_t W, const uint64_t *S_e, const uint8_t *taken, uint8_t *P,
+                           uint64_t *later, int64_t nlater, int32_t num_threads, double *t) {
+    if (n <= 0 || F <= 0 || N <= 0 || K <= 0 || !roots || !rows || !cols || !vertex_parent || !nt_map || !cand_root ||
+        !cand_edge || nsample < 0 || (nsample > 0 && !root_sample) || W != (N + 63) / 64 || !S_e || !P || nlater < 0 ||
+        (nlater > 0 && !later) || !t)
+        return -3;
+    const int threads = num_threads > 0 ? num_threads : omp_get_max_threads();
+    for (int32_t i = 0; i < nsample; ++i)
+        if (root_sample[i] < 0 || root_sample[i] >= F) return -3;
+    /* the sampled roots' children lists, built untimed (the reference keeps its trees across phases) */
+    children_t *ch = (children_t *)calloc((size_t)(nsample > 0 ? nsample : 1), sizeof(children_t));
+    if (!ch) return -5;
+    int rc = 0;
+    for (int32_t i = 0; i < nsample && !rc; ++i)
+        if (build_children(n, rows, cols, vertex_parent + (size_t)root_sample[i] * n, &ch[i])) rc = -5;
+    int64_t chosen = -1;
+    if (rc) goto done;
+
+    double t0 = omp_get_wtime();
+#pragma omp parallel num_threads(threads)
+    {
+        int32_t *queue = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+#pragma omp for schedule(dynamic, 8)
+        for (int32_t i = 0; i < nsample; ++i) {
+            const int32_t r = root_sample[i];
+            if (queue) parity_one_root(n, roots[r], &ch[i], nt_map, S_e, P + (size_t)r * n, queue);
+        }
+        if (!queue) {
+#pragma omp atomic write
+            rc = -5;
+        }
+        free(queue);
+    }
+    t[0] = omp_get_wtime() - t0;
+    if (rc) goto done;
+
+    t0 = omp_get_wtime();
+    for (int64_t k = 0; k < K; ++k) {
+        if (taken && taken[k]) continue;
+        const uint32_t edge = cand_edge[k];
+        const uint8_t *Pr = P + (size_t)cand_root[k] * n;
+        unsigned bit = 0;
+        if (nt_map[edge] >= 0) bit = get_bit(S_e, nt_map[edge]);
+        bit = (bit + Pr[rows[edge]]) % 2;
+        bit = (bit + Pr[cols[edge]]) % 2;
+        if (bit == 1) { chosen = k; break; }
+    }
+    t[1] = omp_get_wtime() - t0;
+    if (chosen < 0) { rc = -8; goto done; }
+
+    t0 = omp_get_wtime();
+    {
+        uint64_t *C = (uint64_t *)calloc((size_t)W, sizeof(uint64_t));
+        if (!C) { rc = -5; goto done; }
+        const uint32_t edge = cand_edge[chosen];
+        const int32_t *vp = vertex_parent + (size_t)cand_root[chosen] * n;
+        uint32_t row = rows[edge], col = cols[edge];
+        if (nt_map[edge] >= 0) set_bit(C, nt_map[edge]);
+        while (vp[row] != -1) {
+            uint32_t eo = (uint32_t)vp[row];
+            if (nt_map[eo] >= 0) set_bit(C, nt_map[eo]);
+            row = (rows[eo] != row) ? rows[eo] : cols[eo];
+        }
+        while (vp[col] != -1) {
+            uint32_t eo = (uint32_t)vp[col];
+            if (nt_map[eo] >= 0) set_bit(C, nt_map[eo]);
+            col = (rows[eo] != col) ? rows[eo] : cols[eo];
+        }
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (int64_t j = 0; j < nlater; ++j) {
+            uint64_t *Sj = later + (size_t)j * W;
+            if (dot_product(C, Sj, W) == 1)
+                for (int32_t i = 0; i < W; ++i) Sj[i] ^= S_e[i];
+        }
+        free(C);
+    }
+    t[2] = omp_get_wtime() - t0;
+done:
+    for (int32_t i = 0; i < nsample; ++i) free_children(&ch[i]);
+    free(ch);
+    return rc ? rc : chosen;
+}

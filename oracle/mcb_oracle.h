@@ -40,6 +40,23 @@ int oracle_run(int32_t n, int32_t F, const uint32_t *roots, const uint32_t *rows
                const uint32_t *cand_root, const uint32_t *cand_edge, int32_t phases, int32_t num_threads,
                int32_t *selected, uint64_t *S_out, uint64_t *C_out);
 
+/* One phase e of the loop of mcb/mcb_cycle.cpp:237-301 started from a given S_e, timed in parts for
+ * the CPU baseline (seconds in t[0..2]):
+ *   t[0] precompute: per-vertex parity BFS (include/work_per_thread.h:154-210) of the roots listed in
+ *        root_sample (nsample of them), on num_threads threads;
+ *   t[1] inspection: the list scan from rank 0 skipping taken ranks (mcb/mcb_cycle.cpp:254-285) over
+ *        P (F x n bytes, per-vertex parity of every root, filled by the caller for this S_e);
+ *   t[2] independence: the cycle vector of the chosen rank (include/cycles.h:83-134) and, for each of
+ *        the nlater vectors in later (nlater x W words), the product with it over all W words and the
+ *        XOR of S_e where the product is odd (mcb/mcb_cycle.cpp:296-301).
+ * The parity of the sampled roots is written into P as well.  Returns the chosen rank, -8 when no
+ * candidate is odd, -3 on bad arguments, -5 when out of memory. */
+int64_t oracle_phase_timed(int32_t n, int32_t F, const uint32_t *roots, const uint32_t *rows, const uint32_t *cols,
+                           const int32_t *vertex_parent, const int32_t *nt_map, int32_t N, int64_t K,
+                           const uint32_t *cand_root, const uint32_t *cand_edge, const int32_t *root_sample,
+                           int32_t nsample, int32_t W, const uint64_t *S_e, const uint8_t *taken, uint8_t *P,
+                           uint64_t *later, int64_t nlater, int32_t num_threads, double *t);
+
 /* ---------------------------------------------------------------------------------------------
  * Per-phase checks on the position layout (no F x n vertex rows, so they reach the largest graphs).
  *

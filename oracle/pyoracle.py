@@ -27,6 +27,9 @@ def lib() -> ctypes.CDLL:
         L.oracle_phase_select.restype = i64
         L.oracle_cycle_vector.argtypes = [P, P, P, i64, P, P, P, P, i64, i32, P]
         L.oracle_orthogonality.argtypes = [P, P, P, i64, P, P, P, P, i32, P, P, i32, P, i32, P, P]
+        L.oracle_phase_timed.argtypes = [i32, i32, P, P, P, P, P, i32, i64, P, P, P, i32, i32, P, P, P, P, i64, i32,
+                                         P]
+        L.oracle_phase_timed.restype = i64
         _lib = L
     return _lib
 
@@ -72,6 +75,39 @@ def run(problem, phases: int | None = None, num_threads: int = 0, want_s: bool =
     if rc:
         raise RuntimeError(f"oracle_run failed: {rc}")
     return {"selected": sel, "S": S, "C": C}
+
+
+def phase_timed(problem, e: int, S_e: np.ndarray, taken, P: np.ndarray, later: np.ndarray, root_sample,
+                num_threads: int = 0):
+    """One phase of the reference's CPU loop from S_e, timed in parts: returns (chosen rank, seconds of
+    the parity BFS of the ``root_sample`` roots, of the list scan over ``P`` (F x n uint8, per-vertex
+    parity of every root, filled by the caller), and of the cycle vector plus the products with (and
+    XORs into) the rows of ``later`` (modified in place))."""
+    info = problem.info
+    n, F, N, K, W = info.nodes, info.fvs_size, info.num_non_tree, info.num_candidates, info.words
+    vp = problem.array("vertex_parent")
+    if vp.size != n * F:
+        raise ValueError("problem was built without keep_vertex_trees")
+    if P.dtype != np.uint8 or P.size != n * F or not P.flags.c_contiguous:
+        raise ValueError("P must be a contiguous uint8 array of F x n")
+    if later.dtype != np.uint64 or not later.flags.c_contiguous or (later.size and later.shape[-1] != W):
+        raise ValueError("later must be contiguous uint64 rows of W words")
+    S_e = _c(S_e, np.uint64)
+    rs = _c(root_sample, np.int32)
+    tk = np.zeros(K, dtype=np.uint8)
+    if taken is not None and len(taken):
+        tk[np.asarray(taken, dtype=np.int64)] = 1
+    t = np.zeros(3, dtype=np.float64)
+    k = lib().oracle_phase_timed(n, F, problem.array("roots").ctypes.data, problem.array("rows").ctypes.data,
+                                 problem.array("cols").ctypes.data, vp.ctypes.data,
+                                 problem.array("nt_map").ctypes.data, N, K, problem.array("cand_root").ctypes.data,
+                                 problem.array("cand_edge").ctypes.data, rs.ctypes.data, int(rs.size), W,
+                                 S_e.ctypes.data, tk.ctypes.data, P.ctypes.data,
+                                 later.ctypes.data if later.size else None, int(later.size // W if W else 0),
+                                 int(num_threads), t.ctypes.data)
+    if k < 0:
+        raise RuntimeError(f"oracle_phase_timed failed: {k}")
+    return int(k), float(t[0]), float(t[1]), float(t[2])
 
 
 # ---------------------------------------------------------------------------------------------

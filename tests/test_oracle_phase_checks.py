@@ -111,3 +111,22 @@ def test_planted_errors_are_caught(name):
     wrong = S[e].copy()
     wrong[e // 64] ^= np.uint64(1) << np.uint64(e % 64)
     assert pyoracle.phase_select(p, wrong, taken=sel[:e]) != sel[e]
+
+
+@pytest.mark.parametrize("name", ["grid6x5", "er120_ties", "delaunay150"])
+def test_timed_phase_is_one_phase_of_the_loop(name):
+    """The CPU baseline's timed phase, started from the loop's state after e phases, chooses the loop's
+    rank and leaves the later vectors in the loop's state after e + 1 phases."""
+    fx = load_trace(name)
+    p = problem_of(fx, "reduced", keep_vertex_trees=True, prune_trees=True)
+    N, F, n = p.info.num_non_tree, p.info.fvs_size, p.info.nodes
+    sel = pyoracle.run(p, num_threads=2)["selected"]
+    for e in sorted({0, 1, N // 3, N - 2}):
+        before = pyoracle.run(p, phases=e, want_s=True, num_threads=2)["S"]
+        after = pyoracle.run(p, phases=e + 1, want_s=True, num_threads=2)["S"]
+        P = np.ascontiguousarray(pyoracle.parity(p, before[e]).reshape(-1))
+        later = np.ascontiguousarray(before[e + 1:])
+        k, *_ = pyoracle.phase_timed(p, e, before[e], sel[:e], P, later, np.arange(F, dtype=np.int32),
+                                     num_threads=2)
+        assert k == sel[e]
+        assert np.array_equal(later, after[e + 1:]), f"phase {e}: update of the later vectors"
