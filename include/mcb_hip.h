@@ -172,6 +172,15 @@ int mcb_solver_cycle_vectors(mcb_solver *s, uint64_t *out, int32_t count);
  * out[p] = XOR of S bits along the path root -> position p (kernel (a) alone). */
 int mcb_solver_parity(mcb_solver *s, const uint64_t *S, uint8_t *out);
 
+/* mcb_solver_parity without device allocation: no counterpart in the reference.  For callers that
+ * cannot allocate device memory per call (HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  S and out are host arrays as in mcb_solver_parity; S and the parity bytes are staged in a
+ * 64 MiB device array of the library (one per device, held until the process ends), so 8 * W rounded up
+ * to 256 bytes plus the number of tree positions must fit in 64 MiB (MCB_E_INVALID otherwise).  Calls on
+ * one device share that array: calls that overlap on one device race.  Runs on the current device, the
+ * one the solver was created on. */
+int mcb_solver_parity_noalloc(mcb_solver *s, const uint64_t *S, uint8_t *out);
+
 /* Kernel ids for timing.  One kernel per phase: every launch of the phase kernel (update of the
  * previous phase, parity + search, cycle vector) counts under MCB_K_PARITY_SEARCH.  Blocked: per
  * block of phases, the pass over the trees (path words and candidate words) counts under

@@ -65,6 +65,7 @@ SIGNATURES = {
     "mcb_solver_support_vector_rows": (ctypes.c_int, [P, P, i32, P]),
     "mcb_solver_cycle_vectors": (ctypes.c_int, [P, P, i32]),
     "mcb_solver_parity": (ctypes.c_int, [P, P, P]),
+    "mcb_solver_parity_noalloc": (ctypes.c_int, [P, P, P]),
     "mcb_solver_kernel_time": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]),
     "mcb_solver_device_bytes": (i64, [P]),
     "mcb_solver_work_counters": (ctypes.c_int, [P, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i64)]),

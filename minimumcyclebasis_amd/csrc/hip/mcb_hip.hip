@@ -35,6 +35,7 @@
 #include <cstring>
 #include <new>
 #include <functional>
+#include <mutex>
 #include <numeric>
 #include <queue>
 #include <string>
@@ -2925,6 +2926,33 @@ int reserve_dynamic_lds(const void *kernel, int64_t bytes, const char *what) {
     return MCB_OK;
 }
 
+// Scratch of mcb_solver_parity_noalloc: S (W words, padded to 256 bytes), then one parity byte per tree
+// position.  Each device holds its own copy until the process ends.  Every call on a device shares it and
+// it keeps the last call's contents; calls that overlap on one device race.
+constexpr size_t kParityScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) uint8_t scratch[kParityScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on the current device, looked up at the first call on that device.
+// Host code uses only this address: the array's name is a host-side stand-in, not device memory.
+int scratch_address(void **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) return fail(MCB_E_DEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(MCB_E_DEVICE, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return MCB_OK;
+}
+
 // Levels below the root of every tree.
 std::vector<uint32_t> tree_depths(const int64_t *tree_off, const uint32_t *parent_pos, int64_t F) {
     std::vector<uint32_t> depth(static_cast<size_t>(F), 0), level;
@@ -3643,6 +3671,32 @@ int mcb_solver_parity(mcb_solver *s, const uint64_t *S, uint8_t *out) {
     (void)hipFree(dS);
     (void)hipFree(dP);
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return fail(MCB_E_DEVICE, "parity kernel failed");
+    return MCB_OK;
+}
+
+int mcb_solver_parity_noalloc(mcb_solver *s, const uint64_t *S, uint8_t *out) {
+    if (!s || !S || !out) return fail(MCB_E_INVALID, "invalid arguments");
+    if (s->lds_bytes == 0) return fail(MCB_E_STATE, "support vector too long for the per-phase parity kernel's LDS");
+    const size_t s_bytes = (sizeof(uint64_t) * static_cast<size_t>(s->W) + 255) & ~static_cast<size_t>(255);
+    if (s_bytes + static_cast<size_t>(s->Ptot) > kParityScratchBytes)
+        return fail(MCB_E_INVALID, "support vector and tree positions exceed the " +
+                                       std::to_string(kParityScratchBytes >> 20) +
+                                       " MiB scratch of the no-allocation parity entry point");
+    void *base = nullptr;
+    int rc = scratch_address(&base);
+    if (rc) return rc;
+    uint64_t *dS = static_cast<uint64_t *>(base);
+    uint8_t *dP = static_cast<uint8_t *>(base) + s_bytes;
+    HIP_TRY(hipMemcpyAsync(dS, S, sizeof(uint64_t) * s->W, hipMemcpyHostToDevice, s->stream));
+    PhaseArgs a = phase_args(s, 0);
+    a.S = dS;
+    a.parity_out = dP;  // the kernel writes every position of every root
+    a.wave_stats = nullptr;
+    a.upd_tiles = 0;
+    hipLaunchKernelGGL((k_phase<true, false>), dim3(s->blocks), dim3(s->wpb * kWave), s->lds_bytes, s->stream, a);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, dP, static_cast<size_t>(s->Ptot), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
     return MCB_OK;
 }
 

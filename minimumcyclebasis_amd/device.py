@@ -144,6 +144,16 @@ class Solver:
         _lib.check_hip(_lib.hip().mcb_solver_parity(self._h, S.ctypes.data, out.ctypes.data))
         return out
 
+    def parity_noalloc(self, S: np.ndarray) -> np.ndarray:
+        """As ``parity``, staged in the library's fixed device scratch instead of allocating (at most 64 MiB:
+        8 W bytes rounded up to 256 plus one byte per tree position)."""
+        S = np.ascontiguousarray(S, dtype=np.uint64)
+        if S.size != self.W:
+            raise ValueError("S must have W words")
+        out = np.zeros(self.P, dtype=np.uint8)
+        _lib.check_hip(_lib.hip().mcb_solver_parity_noalloc(self._h, S.ctypes.data, out.ctypes.data))
+        return out
+
     def kernel_time(self, kernel: str) -> tuple[float, int]:
         ms = ctypes.c_double()
         n = ctypes.c_int64()
