@@ -67,6 +67,14 @@ typedef struct mcb_gpu mcb_gpu;
 int mcb_gpu_create(int num_edges, int num_non_tree_edges, int size_vector, int original_nodes, int fvs_size,
                    int chunk_size, int nstreams, mcb_gpu **out);
 
+/* mcb_gpu_create with at most max_slots chunk slots on the device (0: as mcb_gpu_create).  With fewer
+ * slots than chunks the trees stay on the host and mcb_gpu_process_shortest_path must be called with
+ * multiple_transfer, which sends each chunk's trees to slot (chunk mod slots) before its parity pass
+ * (gpu/gpu_struct.cu:132-156,175-186 stream the same way past 32 chunks).  No counterpart in the
+ * reference's signatures: it lets a caller bound the device memory of the trees. */
+int mcb_gpu_create_slots(int num_edges, int num_non_tree_edges, int size_vector, int original_nodes, int fvs_size,
+                         int chunk_size, int nstreams, int max_slots, mcb_gpu **out);
+
 /* gpu_struct::initialize_memory (gpu_struct.cu:77-102): uploads the coordinate map (num_edges
  * int32, -1 for global tree edges) and, when every chunk has a slot, every chunk's tree_cols and
  * edge_offset.  tree_rows is accepted for layout compatibility; the kernels only need the parent

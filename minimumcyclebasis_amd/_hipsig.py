@@ -45,6 +45,7 @@ SIGNATURES = {
     "free_pinned_memory": (None, [P]),
     "calculate_chunk_size": (ctypes.c_size_t, [ctypes.c_size_t] * 4),
     "mcb_gpu_create": (ctypes.c_int, [ctypes.c_int] * 7 + [ctypes.POINTER(P)]),
+    "mcb_gpu_create_slots": (ctypes.c_int, [ctypes.c_int] * 8 + [ctypes.POINTER(P)]),
     "mcb_gpu_initialize_memory": (ctypes.c_int, [P, P, P, P, P, ctypes.c_int]),
     "mcb_gpu_copy_support_vector": (ctypes.c_int, [P, P, ctypes.POINTER(ctypes.c_float)]),
     "mcb_gpu_process_shortest_path": (ctypes.c_int, [P, P, P, P, P, ctypes.c_int, ctypes.c_int,

@@ -2358,8 +2358,14 @@ size_t calculate_chunk_size(size_t num_nodes, size_t num_edges, size_t size_vect
 
 int mcb_gpu_create(int num_edges, int num_non_tree_edges, int size_vector, int original_nodes, int fvs_size,
                    int chunk_size, int nstreams, mcb_gpu **out) {
+    return mcb_gpu_create_slots(num_edges, num_non_tree_edges, size_vector, original_nodes, fvs_size, chunk_size,
+                                nstreams, 0, out);
+}
+
+int mcb_gpu_create_slots(int num_edges, int num_non_tree_edges, int size_vector, int original_nodes, int fvs_size,
+                         int chunk_size, int nstreams, int max_slots, mcb_gpu **out) {
     if (!out || num_edges <= 0 || num_non_tree_edges <= 0 || size_vector <= 0 || original_nodes <= 0 ||
-        fvs_size <= 0 || chunk_size <= 0 || nstreams <= 0)
+        fvs_size <= 0 || chunk_size <= 0 || nstreams <= 0 || max_slots < 0)
         return fail(MCB_E_INVALID, "invalid arguments");
     *out = nullptr;
     int rc = ensure_device();
@@ -2377,8 +2383,9 @@ int mcb_gpu_create(int num_edges, int num_non_tree_edges, int size_vector, int o
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
     const size_t reserve = static_cast<size_t>(1) << 30;
-    g->all_resident = static_cast<size_t>(g->num_chunks) * per_slot + reserve < free_b;
-    g->nslots = g->all_resident ? g->num_chunks : std::min(32, g->num_chunks);
+    g->all_resident = static_cast<size_t>(g->num_chunks) * per_slot + reserve < free_b &&
+                      (max_slots == 0 || g->num_chunks <= max_slots);
+    g->nslots = g->all_resident ? g->num_chunks : std::min(max_slots > 0 ? max_slots : 32, g->num_chunks);
     g->nstreams = std::min(32, std::max(1, std::min(nstreams, g->num_chunks)));
     auto cleanup = [&](int code) {
         mcb_gpu_destroy(g);

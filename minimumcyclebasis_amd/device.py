@@ -305,10 +305,17 @@ class GpuStruct:
     """
 
     def __init__(self, num_edges: int, num_non_tree_edges: int, size_vector: int, original_nodes: int,
-                 fvs_size: int, chunk_size: int, nstreams: int):
+                 fvs_size: int, chunk_size: int, nstreams: int, max_slots: int = 0):
+        """max_slots > 0 keeps at most that many chunk slots on the device; with fewer slots than chunks the
+        trees are streamed every phase (process_shortest_path with multiple_transfer)."""
         h = ctypes.c_void_p()
-        _lib.check_hip(_lib.hip().mcb_gpu_create(num_edges, num_non_tree_edges, size_vector, original_nodes,
-                                                 fvs_size, chunk_size, nstreams, ctypes.byref(h)))
+        if max_slots:
+            _lib.check_hip(_lib.hip().mcb_gpu_create_slots(num_edges, num_non_tree_edges, size_vector,
+                                                           original_nodes, fvs_size, chunk_size, nstreams,
+                                                           int(max_slots), ctypes.byref(h)))
+        else:
+            _lib.check_hip(_lib.hip().mcb_gpu_create(num_edges, num_non_tree_edges, size_vector, original_nodes,
+                                                     fvs_size, chunk_size, nstreams, ctypes.byref(h)))
         self._h = h
         self.num_chunks = (fvs_size + chunk_size - 1) // chunk_size
 

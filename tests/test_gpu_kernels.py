@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from helpers import load_trace, problem_of, trace_names
+from minimumcyclebasis_amd import _lib
 from minimumcyclebasis_amd.device import GpuStruct, Solver
 from oracle import pyoracle
 
@@ -45,11 +46,14 @@ def test_parity_native_layout(name, mode, prune):
     s.close()
 
 
+@pytest.mark.parametrize("slots", [0, 1, 4])
 @pytest.mark.parametrize("multiple_transfer", [False, True])
 @pytest.mark.parametrize("chunk", [3, 720])
 @pytest.mark.parametrize("name", ["grid4x4", "delaunay40", "chunglu60", "er150", "grid16x16"])
-def test_parity_reference_layout(name, chunk, multiple_transfer):
-    """gpu_struct drop-in: compressed_trees chunk arrays in, per-position parity out."""
+def test_parity_reference_layout(name, chunk, multiple_transfer, slots):
+    """gpu_struct drop-in: compressed_trees chunk arrays in, per-position parity out.  slots > 0 caps the device
+    chunk slots: with more chunks than slots the trees are streamed every phase (gpu/gpu_struct.cu:132-156,
+    175-186), and a call without multiple_transfer is refused."""
     fx = load_trace(name)
     p = problem_of(fx, "reduced", keep_vertex_trees=True)
     n, F, N, W = p.info.nodes, p.info.fvs_size, p.info.num_non_tree, p.info.words
@@ -75,11 +79,19 @@ def test_parity_reference_layout(name, chunk, multiple_transfer):
         eo_c.append(E)
         ni_c.append(NI)
         out_c.append(np.full(chunk * n, 9, dtype=np.uint32))
-    g = GpuStruct(p.info.directed_edges, N, W, n, F, chunk, nchunks)
+    g = GpuStruct(p.info.directed_edges, N, W, n, F, chunk, nchunks, max_slots=slots)
     g.initialize_memory(p.array("nt_map"), rows_c, cols_c, eo_c)
+    streamed = 0 < slots < nchunks
+    if streamed and not multiple_transfer:
+        with pytest.raises(_lib.MCBError):
+            g.process_shortest_path(out_c, multiple_transfer=False)
+        g.close()
+        return
     rng = np.random.default_rng(3)
-    for density in (0.1, 0.6):
+    for density in (0.1, 0.6, 0.3):
         S = random_support(rng, N, W, density)
+        for o in out_c:
+            o.fill(9)
         g.copy_support_vector(S)
         g.process_shortest_path(out_c, multiple_transfer=multiple_transfer)
         want = pyoracle.parity(p, S)
