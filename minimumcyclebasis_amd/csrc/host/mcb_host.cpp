@@ -732,7 +732,20 @@ void process_root(const mcb_problem &P, uint32_t src, bool prune, Workspace &ws,
     }
 }
 
+// Stage times of build_from_input on stderr when MCB_HOST_TIMING is set (seconds since the previous mark).
+struct StageClock {
+    bool on = std::getenv("MCB_HOST_TIMING") != nullptr;
+    double t = omp_get_wtime();
+    void mark(const char *stage) {
+        if (!on) return;
+        const double now = omp_get_wtime();
+        std::fprintf(stderr, "[mcb_host] %-28s %8.3f s\n", stage, now - t);
+        t = now;
+    }
+};
+
 int build_from_input(InputGraph &in, int64_t input_weight_sum, const mcb_build_options &opt, mcb_problem **out) {
+    StageClock clock;
     std::unique_ptr<mcb_problem> P(new (std::nothrow) mcb_problem());
     if (!P) return fail(MCB_E_NOMEM, "out of memory");
     finish_input_csr(in);
@@ -778,6 +791,7 @@ int build_from_input(InputGraph &in, int64_t input_weight_sum, const mcb_build_o
         build_reduced(in, nullptr, nullptr, P->g);
     }
     in = InputGraph();  // release the input graph early
+    clock.mark("csr + degree-two contraction");
 
     Multigraph &g = P->g;
     info.source_vertex = src;
@@ -794,6 +808,7 @@ int build_from_input(InputGraph &in, int64_t input_weight_sum, const mcb_build_o
     int F = fvs_reverse_delete_impl(g.n, g.ro.data(), g.cols.data(), order, P->fvs_rank.data());
     if (F < 0) return F;
     info.fvs_size = F;
+    clock.mark("fvs");
     P->roots.resize(F);
     for (int32_t v = 0; v < g.n; ++v)
         if (P->fvs_rank[v] >= 0) P->roots[P->fvs_rank[v]] = static_cast<uint32_t>(v);
@@ -815,6 +830,7 @@ int build_from_input(InputGraph &in, int64_t input_weight_sum, const mcb_build_o
     for (size_t i = 0; i < g.rows.size(); ++i)
         if (P->nt_map[i] < 0 && P->nt_map[g.rev[i]] >= 0) P->nt_map[i] = P->nt_map[g.rev[i]];
 
+    clock.mark("spanning tree, coordinates");
     // Per-root trees and candidates.
     std::vector<RootOutput> per_root(F);
     if (opt.keep_vertex_trees) P->vertex_parent.assign(static_cast<size_t>(F) * g.n, -1);
@@ -835,6 +851,7 @@ int build_from_input(InputGraph &in, int64_t input_weight_sum, const mcb_build_o
         }
     }
     if (err) return fail(MCB_E_NOMEM, "out of memory while building shortest-path trees");
+    clock.mark("shortest-path trees");
 
     // Collect candidates root by root (roots ascend by vertex id) and rank them by weight with
     // the same std::sort over the same sequence (mcb_cycle_deg_two_remove.cpp:247-258).
@@ -878,6 +895,7 @@ int build_from_input(InputGraph &in, int64_t input_weight_sum, const mcb_build_o
     std::vector<Keyed> keyed(K);
     for (int64_t k = 0; k < K; ++k) keyed[k] = Keyed{c_w[k], static_cast<uint32_t>(k)};
     std::sort(keyed.begin(), keyed.end(), [](const Keyed &a, const Keyed &b) { return a.w < b.w; });
+    clock.mark("candidate collect + sort");
     std::vector<uint32_t> rank_order(K);
     for (int64_t k = 0; k < K; ++k) rank_order[k] = keyed[k].k;
     std::vector<Keyed>().swap(keyed);
@@ -905,6 +923,7 @@ int build_from_input(InputGraph &in, int64_t input_weight_sum, const mcb_build_o
         std::vector<int64_t> fill(P->root_cand_offset.begin(), P->root_cand_offset.end() - 1);
         for (int64_t k = 0; k < K; ++k) P->root_cand_rank[fill[P->cand_root[k]]++] = static_cast<uint32_t>(k);
     }
+    clock.mark("rank arrays");
     // Concatenated position layouts.
     P->tree_offset.assign(F + 1, 0);
     for (int32_t r = 0; r < F; ++r)
@@ -921,6 +940,7 @@ int build_from_input(InputGraph &in, int64_t input_weight_sum, const mcb_build_o
         RootOutput().parent_pos.swap(per_root[r].parent_pos);
         per_root[r] = RootOutput();
     }
+    clock.mark("position layout");
     *out = P.release();
     return MCB_OK;
 }
