@@ -677,6 +677,13 @@ constexpr int kBlockK = 64;      // phases per block (word width)
 constexpr int kQWaves = 16;      // waves per workgroup of the tree pass
 constexpr int kChunk = 4096;     // ranks per first-level greedy basis (one candidate-bitmap word per lane)
 constexpr int64_t kXorWorkgroups = 2048;  // grid of k_block_xor (eight workgroups per CU)
+constexpr int64_t kDotWorkgroups = 512;   // grid of k_block_dot (two eight-wave workgroups per CU)
+// The update's byte counter is spread over this many slots (slot = workgroup index mod kByteSlots), so
+// that the waves of a launch do not queue on one address; the host sums them.
+constexpr int kByteSlots = 64;
+__device__ __forceinline__ void add_bytes(unsigned long long *slots, uint64_t n) {
+    atomicAdd(slots + (blockIdx.x & (kByteSlots - 1)), static_cast<unsigned long long>(n));
+}
 constexpr int kMaxLevels = 8;    // kChunk * 16^(kMaxLevels-1) covers 2^32 ranks
 
 // Bases merged per group: one wave inserts a group's bases one after the other, so a group's time
@@ -1688,7 +1695,98 @@ __global__ __launch_bounds__(256) void k_block_pairs(BlockArgs a) {
 constexpr int kSegWords = 1024;
 constexpr int kSegPairs = 2048;
 constexpr int kDotWaves = 8;      // waves per product workgroup
-constexpr int kDotCols = kWave;   // later columns per product workgroup
+constexpr int kDotCols = kWave;   // later columns per product tile
+
+// Column j's products d (bit i: <C_{e+i}, S_j>) into its coefficients b (the recurrence over the block's
+// phases with the rows G), written to Bc[j]; the columns with b != 0 go to the update list (one atomic
+// per wave).  Returns the bytes the wave's columns read (in lane 0).  One wave, lane = column.
+__device__ __forceinline__ uint64_t finish_column(const BlkView &v, const uint64_t *G, int64_t j, bool col,
+                                                  uint64_t d, uint32_t nl, int lane) {
+    uint64_t nbytes = 0, bcol = 0;
+    if (col) {
+        uint64_t b = 0;
+        for (int i = 0; i < v.kk; ++i) {
+            const uint64_t bi = ((d >> i) ^ static_cast<uint64_t>(__popcll(b & G[i]))) & 1ull;
+            b |= bi << i;
+        }
+        v.Bc[j] = b;
+        bcol = b;
+        nbytes = 2 + 8ull * nl;
+    }
+    if (v.ulist) {  // list the columns that take an update, one atomic per wave
+        const bool upd = col && bcol != 0;
+        const uint64_t m = __ballot(upd);
+        if (m) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(v.ucount, static_cast<uint32_t>(__popcll(m)));
+            base = static_cast<uint32_t>(__shfl(static_cast<int>(base), 0));
+            if (upd) v.ulist[base + __popcll(m & ((1ull << lane) - 1))] = static_cast<uint32_t>(j);
+        }
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) nbytes += __shfl_down(nbytes, o);
+    return nbytes;  // the wave's total in lane 0
+}
+
+// The later columns when the block's word list is short (at most kShortList words, their pairs in one
+// LDS segment), as on sparse graphs: the list, its pairs and the rows G are staged once per workgroup,
+// then each wave takes whole 64-column tiles in turn (lane = column: one load per listed word, all in
+// flight together), so no workgroup repeats the staging per tile.  Returns false, having done nothing,
+// for a longer list.  LDS as dot_body's.
+constexpr uint32_t kShortList = 48;
+__device__ __forceinline__ bool dot_short(const BlkView &v, uint64_t *lds, unsigned long long *upd_bytes) {
+    constexpr int kB = 16;
+    const uint32_t nl = uniform32(*v.pw_cnt);
+    if (nl > kShortList) return false;
+    const uint32_t np = uniform32(v.pw_off[nl]) - uniform32(v.pw_off[0]);
+    if (np > static_cast<uint32_t>(kSegPairs)) return false;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wv = tid / kWave;
+    const int nwv = static_cast<int>(blockDim.x) / kWave;
+    uint64_t *G = lds;
+    uint64_t *Pc = G + kBlockK;
+    uint32_t *Pi = reinterpret_cast<uint32_t *>(Pc + kSegPairs);
+    uint32_t *Lw = Pi + kSegPairs;
+    uint32_t *Lo = Lw + kSegWords;
+    const uint32_t p0 = nl ? v.pw_off[0] : 0u;
+    for (int x = tid; x < kBlockK; x += blockDim.x) G[x] = x < v.kk ? v.blk_g[x] : 0ull;
+    for (uint32_t x = tid; x < nl; x += blockDim.x) {
+        Lw[x] = v.pw_list[x];
+        Lo[x] = v.pw_off[x] - p0;
+    }
+    if (tid == 0) Lo[nl] = np;
+    for (uint32_t q = tid; q < np; q += blockDim.x) {
+        Pc[q] = v.pair_c[p0 + q];
+        Pi[q] = v.pair_i[p0 + q];
+    }
+    __syncthreads();
+    const int64_t later = static_cast<int64_t>(v.N) - v.e - v.kk;
+    const int64_t tiles = (later + kDotCols - 1) / kDotCols;
+    uint64_t nbytes = 0;
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * nwv + wv; t < tiles;
+         t += static_cast<int64_t>(gridDim.x) * nwv) {
+        const int64_t j = static_cast<int64_t>(v.e) + v.kk + t * kDotCols + lane;
+        const bool col = j < v.N;
+        uint64_t d = col ? v.Ct[j] : 0ull;
+        for (uint32_t x0 = 0; x0 < nl; x0 += kB) {
+            uint64_t sv[kB];
+#pragma unroll
+            for (int k = 0; k < kB; ++k) {
+                const uint32_t x = x0 + k < nl ? x0 + k : x0;
+                sv[k] = col ? v.Sw[static_cast<int64_t>(Lw[x]) * v.npad + j] : 0ull;
+            }
+#pragma unroll
+            for (int k = 0; k < kB; ++k) {
+                if (x0 + k >= nl) break;
+                for (uint32_t q = Lo[x0 + k]; q < Lo[x0 + k + 1]; ++q)
+                    d ^= static_cast<uint64_t>(__popcll(Pc[q] & sv[k]) & 1) << Pi[q];
+            }
+        }
+        nbytes += finish_column(v, G, j, col, d, nl, lane);
+    }
+    if (lane == 0 && nbytes) add_bytes(upd_bytes, nbytes);
+    return true;
+}
 __device__ __forceinline__ void dot_body(const BlkView &v, int64_t tile, uint64_t *lds, uint32_t *seg_end,
                                          unsigned long long *upd_bytes) {
     constexpr int kB = 16;
@@ -1759,37 +1857,21 @@ __device__ __forceinline__ void dot_body(const BlkView &v, int64_t tile, uint64_
     __syncthreads();  // the pair words are no longer read: they take the waves' partial products
     Pc[tid] = d;
     __syncthreads();
-    if (wv != 0) return;
-    for (int w = 1; w < kDotWaves; ++w) d ^= Pc[w * kWave + lane];
-    uint64_t nbytes = 0, bcol = 0;
-    if (col) {
-        uint64_t b = 0;
-        for (int i = 0; i < kk; ++i) {
-            const uint64_t bi = ((d >> i) ^ static_cast<uint64_t>(__popcll(b & G[i]))) & 1ull;
-            b |= bi << i;
-        }
-        v.Bc[j] = b;
-        bcol = b;
-        nbytes = 2 + 8ull * nl;
+    if (wv == 0) {
+        for (int w = 1; w < kDotWaves; ++w) d ^= Pc[w * kWave + lane];
+        const uint64_t nbytes = finish_column(v, G, j, col, d, nl, lane);
+        if (lane == 0 && nbytes) add_bytes(upd_bytes, nbytes);
     }
-    if (v.ulist) {  // list the columns that take an update, one atomic per wave
-        const bool upd = col && bcol != 0;
-        const uint64_t m = __ballot(upd);
-        if (m) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(v.ucount, static_cast<uint32_t>(__popcll(m)));
-            base = static_cast<uint32_t>(__shfl(static_cast<int>(base), 0));
-            if (upd) v.ulist[base + __popcll(m & ((1ull << lane) - 1))] = static_cast<uint32_t>(j);
-        }
-    }
-    for (int o = kWave / 2; o > 0; o >>= 1) nbytes += __shfl_down(nbytes, o);
-    if ((tid & (kWave - 1)) == 0 && nbytes) atomicAdd(upd_bytes, static_cast<unsigned long long>(nbytes));
+    __syncthreads();  // the LDS is free for the next tile
 }
 
 __global__ __launch_bounds__(kDotWaves * kWave) void k_block_dot(BlockArgs a) {
     extern __shared__ uint64_t lds[];
     __shared__ uint32_t seg_end;
-    dot_body(view_of(a), blockIdx.x, lds, &seg_end, a.upd_bytes);
+    const BlkView v = view_of(a);
+    if (dot_short(v, lds, a.upd_bytes)) return;
+    const int64_t tiles = (static_cast<int64_t>(v.N) - v.e - v.kk + kDotCols - 1) / kDotCols;
+    for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) dot_body(v, t, lds, &seg_end, a.upd_bytes);
 }
 
 // The XOR update of tile (x, y) of kTileDirect words, combining the block's final vectors staged in LDS
@@ -1850,7 +1932,7 @@ __device__ __forceinline__ void xor_direct(const BlkView &v, int64_t xt, int32_t
         }
     }
     for (int o = kWave / 2; o > 0; o >>= 1) nbytes += __shfl_down(nbytes, o);
-    if ((tid & (kWave - 1)) == 0 && nbytes) atomicAdd(upd_bytes, static_cast<unsigned long long>(nbytes));
+    if ((tid & (kWave - 1)) == 0 && nbytes) add_bytes(upd_bytes, nbytes);
 }
 
 // The XOR update of one block over (column tile, word tile) pairs, dealt to a fixed grid: column tile
@@ -1942,7 +2024,7 @@ __device__ __forceinline__ void xor_body(const BlkView &v, uint64_t *lds, unsign
         }
     }
     for (int o = kWave / 2; o > 0; o >>= 1) nbytes += __shfl_down(nbytes, o);
-    if ((tid & (kWave - 1)) == 0 && nbytes) atomicAdd(upd_bytes, static_cast<unsigned long long>(nbytes));
+    if ((tid & (kWave - 1)) == 0 && nbytes) add_bytes(upd_bytes, nbytes);
 }
 
 __global__ __launch_bounds__(256) void k_block_xor(BlockArgs a) {
@@ -2823,7 +2905,11 @@ int launch_block(mcb_solver *s, int32_t e, int32_t kk) {
     const int64_t later = s->N - e - kk;
     if (later > 0) {
         const size_t dlds = sizeof(uint64_t) * (kBlockK + kSegPairs) + sizeof(uint32_t) * (kSegPairs + 2 * kSegWords + 1);
-        hipLaunchKernelGGL(k_block_dot, dim3(static_cast<unsigned>((later + kDotCols - 1) / kDotCols)),
+        // a fixed grid: a short word list is staged once per workgroup and its waves take tiles in turn;
+        // a long one is taken tile by tile by whole workgroups
+        hipLaunchKernelGGL(k_block_dot,
+                           dim3(static_cast<unsigned>(std::min<int64_t>((later + kDotCols - 1) / kDotCols,
+                                                                        kDotWorkgroups))),
                            dim3(kDotWaves * kWave), dlds, s->stream, a);
         HIP_TRY(hipGetLastError());
     }
@@ -3322,7 +3408,7 @@ int setup_blocked(mcb_solver *s, const int64_t *tree_offset, const uint32_t *par
     if ((rc = dev_alloc(&s->ulist, static_cast<size_t>(s->npad), B))) return rc;
     if ((rc = dev_alloc(&s->ucount, 1, B))) return rc;
     HIP_TRY(hipMemset(s->ucount, 0, sizeof(uint32_t)));
-    if ((rc = dev_alloc(&s->upd_bytes, 1, B))) return rc;
+    if ((rc = dev_alloc(&s->upd_bytes, kByteSlots, B))) return rc;
     if ((rc = dev_alloc(&s->Ct, static_cast<size_t>(s->npad), B))) return rc;
     if ((rc = dev_alloc(&s->Bc, static_cast<size_t>(s->npad), B))) return rc;
     if ((rc = dev_alloc(&s->pw_list, static_cast<size_t>(s->W), B))) return rc;
@@ -3537,7 +3623,7 @@ int mcb_solver_reset(mcb_solver *s) {
     if (s->bk > 1) {
         HIP_TRY(hipMemsetAsync(s->Tt, 0, sizeof(uint64_t) * static_cast<size_t>(s->npad), s->stream));
         HIP_TRY(hipMemsetAsync(s->qbits, 0, sizeof(uint64_t) * static_cast<size_t>((s->K + 63) / 64), s->stream));
-        HIP_TRY(hipMemsetAsync(s->upd_bytes, 0, sizeof(unsigned long long), s->stream));
+        HIP_TRY(hipMemsetAsync(s->upd_bytes, 0, sizeof(unsigned long long) * kByteSlots, s->stream));
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->phase = 0;
@@ -3731,7 +3817,9 @@ int mcb_solver_work_counters(mcb_solver *s, int64_t *positions, int64_t *candida
     }
     if (s->bk > 1) {
         unsigned long long ub = 0;
-        HIP_TRY(hipMemcpy(&ub, s->upd_bytes, sizeof(ub), hipMemcpyDeviceToHost));
+        unsigned long long slots[kByteSlots];
+        HIP_TRY(hipMemcpy(slots, s->upd_bytes, sizeof(slots), hipMemcpyDeviceToHost));
+        for (unsigned long long x : slots) ub += x;
         u += ub;
     }
     if (positions) *positions = static_cast<int64_t>(p);
@@ -4009,7 +4097,7 @@ int mcb_batch_reset(mcb_batch *b) {
     HIP_TRY(hipMemsetAsync(b->Sw, 0, sizeof(uint64_t) * static_cast<size_t>(b->sw_words), b->stream));
     HIP_TRY(hipMemsetAsync(b->Tt, 0, sizeof(uint64_t) * static_cast<size_t>(b->coords), b->stream));
     HIP_TRY(hipMemsetAsync(b->qbits, 0, sizeof(uint64_t) * static_cast<size_t>(b->K / 64), b->stream));
-    HIP_TRY(hipMemsetAsync(b->upd_bytes, 0, sizeof(unsigned long long), b->stream));
+    HIP_TRY(hipMemsetAsync(b->upd_bytes, 0, sizeof(unsigned long long) * kByteSlots, b->stream));
     HIP_TRY(hipMemsetAsync(b->status, 0, sizeof(uint32_t), b->stream));
     hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<unsigned>((b->coords + 255) / 256)), dim3(256), 0, b->stream,
                        b->sel, kNoRank, b->coords);
@@ -4257,7 +4345,7 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
         (rc = bat_alloc(b, &b->pw_list, pw)) || (rc = bat_alloc(b, &b->pw_off, pwoff)) ||
         (rc = bat_alloc(b, &b->pw_cnt, G)) || (rc = bat_alloc(b, &b->status, 1)) ||
         (rc = bat_alloc(b, &b->ulist, coords)) || (rc = bat_alloc(b, &b->ucount, G)) ||
-        (rc = bat_alloc(b, &b->upd_bytes, 1)))
+        (rc = bat_alloc(b, &b->upd_bytes, kByteSlots)))
         return cleanup(rc);
     if (b->opt.store_cycles && (rc = bat_alloc(b, &b->C_store, store))) return cleanup(rc);
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
@@ -4383,7 +4471,9 @@ int mcb_batch_work_counters(mcb_batch *b, int64_t *positions, int64_t *candidate
     if (!b) return fail(MCB_E_INVALID, "invalid arguments");
     HIP_TRY(hipStreamSynchronize(b->stream));
     unsigned long long ub = 0;
-    HIP_TRY(hipMemcpy(&ub, b->upd_bytes, sizeof(ub), hipMemcpyDeviceToHost));
+    unsigned long long slots[kByteSlots];
+    HIP_TRY(hipMemcpy(slots, b->upd_bytes, sizeof(slots), hipMemcpyDeviceToHost));
+    for (unsigned long long x : slots) ub += x;
     if (positions) *positions = b->blk_positions;
     if (candidates) *candidates = b->blk_candidates;
     if (update_bytes) *update_bytes = static_cast<int64_t>(ub);

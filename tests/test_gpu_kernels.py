@@ -152,6 +152,27 @@ def test_rerun_after_reset(block):
     assert np.array_equal(s.selected(), first)
 
 
+@pytest.mark.parametrize("block,pieces", [(0, (64, 10, 3, 100)), (7, (5, 7, 20, 1, 13)), (33, (40, 33, 2))])
+@pytest.mark.parametrize("name", ["grid16x16", "er150", "chunglu300"])
+def test_runs_in_pieces(name, block, pieces):
+    """The loop enqueued in uneven pieces: a block cut short by the end of a run leaves the previous
+    block's update of its next columns pending, and the next run applies it first (instead of on the
+    fly); every selection, support vector and cycle vector still equals the oracle's."""
+    fx = load_trace(name)
+    p = problem_of(fx, "reduced", keep_vertex_trees=True, prune_trees=True)
+    print(f"case pieces {name} block={block} pieces={pieces}", flush=True)
+    ref = pyoracle.run(p, want_s=True, want_c=True)
+    s = Solver(p, store_cycles=True, block_phases=block)
+    for k in pieces:
+        s.run(k)
+    s.run()
+    s.sync()
+    assert np.array_equal(s.selected(), ref["selected"])
+    assert np.array_equal(s.support_vectors(), ref["S"])
+    assert np.array_equal(s.cycle_vectors(), ref["C"])
+    s.close()
+
+
 @pytest.mark.parametrize("cap", [-1, 64, 128])
 @pytest.mark.parametrize("waves", [4, 8, 16])
 @pytest.mark.parametrize("name", ["grid16x16", "er150", "delaunay150", "chunglu300", "biconn200"])
