@@ -701,16 +701,23 @@ inline int merge_group_size(int64_t chunks, int want) {
 constexpr int kTile = 16;        // support words per workgroup of the XOR update (table form)
 constexpr int kTileDirect = 32;  // the same, direct form (batched: many small components per step)
 
+// One entry of the tree pass's work lists: a root's tree (its first position, its position count with
+// bit 31 set for a narrow tree) and a rank-ascending run [cb, ce) of its candidates in the rc_* arrays.
+// The lists hold these records in the order the workgroups and waves take them, so that an item costs
+// one load, issued while the item before it runs.
+struct ItemMeta {
+    int64_t off, cb, ce;
+    uint32_t npos, root;
+};
+constexpr uint32_t kNarrowBit = 0x80000000u;
+
 struct BlockArgs {
     const int64_t *tree_off;
     const uint32_t *parent_pos;
     const int32_t *pos_coord;
     const uint32_t *pos_pre, *pos_end;
-    const uint32_t *big_list, *small_list;  // item ids
+    const ItemMeta *big_list, *small_list;  // the work lists (big items narrow-flagged)
     const int32_t *wg_big, *wave_small;     // each workgroup's big items, each wave's small items
-    const uint8_t *big_narrow;  // per big item: 1 if its tree's levels average fewer than 64 positions
-    const uint32_t *item_root;  // per item: root, and its candidate run [item_cb, item_ce) of the rc_* arrays
-    const int64_t *item_cb, *item_ce;
     uint64_t *scratch;          // [blocks][huge_words] path words of trees too big for LDS
     int64_t huge_words;
     const int64_t *rc_off;
@@ -1041,7 +1048,7 @@ __device__ __forceinline__ void run_words(const A &a, const uint64_t *Wd, int64_
 }
 
 // Pass over every tree and every candidate of the block.  The work lists hold items (a root and a
-// rank-ascending run of its candidates, item_root / item_cb / item_ce): big trees (more than
+// rank-ascending run of its candidates, as ItemMeta records): big trees (more than
 // small_words positions) take a whole workgroup per item, the others one wave per item.  This
 // workgroup takes the big items big[first], big[first + stride], ... below big[end] (narrow[i]: item
 // i's tree is narrow) and this wave the small items small[sfirst], small[sfirst + sstride], ... below
@@ -1049,10 +1056,24 @@ __device__ __forceinline__ void run_words(const A &a, const uint64_t *Wd, int64_
 // A big tree of more than big_words positions keeps its path words in this workgroup's region of
 // scratch (huge_words words) instead of LDS; its ready stamps stay in LDS and are cleared first, so
 // that no stale word can pass for a stamp.  kk_of(r) is the number of phases in root r's block.
+// Work-list entry i (clamped below end, so that the load is unconditional; zeros for an empty list),
+// wave-uniform.
+__device__ __forceinline__ ItemMeta load_item(const ItemMeta *list, uint32_t i, uint32_t end) {
+    ItemMeta m{};
+    if (end == 0) return m;
+    const ItemMeta *p = list + (i < end ? i : end - 1);
+    m.off = uniform64(p->off);
+    m.cb = uniform64(p->cb);
+    m.ce = uniform64(p->ce);
+    m.npos = uniform32(p->npos);
+    m.root = uniform32(p->root);
+    return m;
+}
+
 template <typename A, typename KkFn>
-__device__ __forceinline__ void tree_pass(const A &a, const uint32_t *big, const uint8_t *narrow, uint32_t first,
-                                          uint32_t end, uint32_t stride, const uint32_t *small, uint32_t sfirst,
-                                          uint32_t send, uint32_t sstride, KkFn kk_of) {
+__device__ __forceinline__ void tree_pass(const A &a, const ItemMeta *big, uint32_t first, uint32_t end,
+                                          uint32_t stride, const ItemMeta *small, uint32_t sfirst, uint32_t send,
+                                          uint32_t sstride, KkFn kk_of) {
     extern __shared__ uint64_t lds[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = wave_id();
@@ -1067,14 +1088,16 @@ __device__ __forceinline__ void tree_pass(const A &a, const uint32_t *big, const
     for (uint32_t i = threadIdx.x; i < nwin_big; i += blockDim.x) ready[i] = 0;
     __syncthreads();
     uint32_t gen = 0;
+    ItemMeta next = load_item(big, first, end);
     for (uint32_t it = first; it < end; it += stride) {
-        const uint32_t item = uniform32(big[it]);
-        const uint32_t r = uniform32(a.item_root[item]);
-        const int64_t off = uniform64(a.tree_off[r]);
-        const uint32_t npos = static_cast<uint32_t>(uniform64(a.tree_off[r + 1]) - off);
+        const ItemMeta cur = next;
+        next = load_item(big, it + stride, end);  // in flight while this item runs
+        const uint32_t r = cur.root;
+        const int64_t off = cur.off;
+        const uint32_t npos = cur.npos & ~kNarrowBit;
         ++gen;
-        const bool nar = narrow[it] != 0;
-        const int64_t cb = uniform64(a.item_cb[item]), ce = uniform64(a.item_ce[item]);
+        const bool nar = (cur.npos & kNarrowBit) != 0;
+        const int64_t cb = cur.cb, ce = cur.ce;
         // path words, then each wave takes one contiguous run of the item's candidates (the two calls
         // keep the LDS and the scratch accesses in their own address spaces)
         auto one_tree = [&](uint64_t *Wd, uint32_t *rd, bool in_scratch) {
@@ -1110,20 +1133,20 @@ __device__ __forceinline__ void tree_pass(const A &a, const uint32_t *big, const
     }
     // the big-tree region is free from here on; each wave takes its own slice for small roots
     uint64_t *Ws = lds + static_cast<size_t>(wid) * a.small_words;
+    ItemMeta snext = load_item(small, sfirst, send);
     for (uint32_t it = sfirst; it < send; it += sstride) {
-        const uint32_t item = uniform32(small[it]);
-        const uint32_t r = uniform32(a.item_root[item]);
-        const int64_t off = uniform64(a.tree_off[r]);
-        const uint32_t npos = static_cast<uint32_t>(uniform64(a.tree_off[r + 1]) - off);
-        tree_words(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Ws, lane);
-        run_words(a, Ws, uniform64(a.item_cb[item]), uniform64(a.item_ce[item]), kk_of(r), lane);
+        const ItemMeta cur = snext;
+        snext = load_item(small, it + sstride, send);  // in flight while this item runs
+        const uint32_t npos = cur.npos & ~kNarrowBit;
+        tree_words(a.parent_pos + cur.off, a.pos_coord + cur.off, npos, a.Tt, Ws, lane);
+        run_words(a, Ws, cur.cb, cur.ce, kk_of(cur.root), lane);
     }
 }
 
 __global__ __launch_bounds__(kQWaves * kWave) void k_block_q(BlockArgs a) {
     const uint32_t nw = blockDim.x / kWave;
     const uint32_t c = blockIdx.x * nw + static_cast<uint32_t>(wave_id());
-    tree_pass(a, a.big_list, a.big_narrow, static_cast<uint32_t>(a.wg_big[blockIdx.x]),
+    tree_pass(a, a.big_list, static_cast<uint32_t>(a.wg_big[blockIdx.x]),
               static_cast<uint32_t>(a.wg_big[blockIdx.x + 1]), 1u, a.small_list,
               static_cast<uint32_t>(a.wave_small[c]), static_cast<uint32_t>(a.wave_small[c + 1]), 1u,
               [&](uint32_t) { return a.kk; });
@@ -2073,10 +2096,7 @@ struct BatchArgs {
     const uint32_t *parent_pos;
     const int32_t *pos_coord;  // shifted by the problem's cb
     const uint32_t *pos_pre, *pos_end;
-    const uint32_t *big_list, *small_list;  // item ids, problem-major, each problem's trees largest first
-    const uint8_t *big_narrow;
-    const uint32_t *item_root;
-    const int64_t *item_cb, *item_ce;
+    const ItemMeta *big_list, *small_list;  // work lists, problem-major, each problem's trees largest first
     uint64_t *scratch;
     int64_t huge_words;
     const int64_t *rc_off;
@@ -2174,7 +2194,7 @@ __global__ __launch_bounds__(kQWaves * kWave) void k_batch_q(BatchArgs a) {
     // blocks x, x + kXcds, ... serve section x (the grid is a multiple of kXcds)
     const uint32_t x = blockIdx.x % kXcds, local = blockIdx.x / kXcds, nloc = gridDim.x / kXcds;
     const uint32_t bo = static_cast<uint32_t>(a.x_big_off[x]), so = static_cast<uint32_t>(a.x_small_off[x]);
-    tree_pass(a, a.big_list + bo, a.big_narrow + bo, local, static_cast<uint32_t>(a.x_big_n[x]), nloc,
+    tree_pass(a, a.big_list + bo, local, static_cast<uint32_t>(a.x_big_n[x]), nloc,
               a.small_list + so, local * nw + static_cast<uint32_t>(wave_id()), static_cast<uint32_t>(a.x_small_n[x]),
               nloc * nw, [&](uint32_t r) {
                   const uint32_t g = uniform32(a.root_comp[r]);
@@ -2622,13 +2642,10 @@ constexpr int64_t kItemCands = 2048;
 struct TreePlan {
     int32_t nw = 0, big_words = 0, small_words = 0, lds = 0, blocks = 0, n_big = 0, n_small = 0;
     int64_t huge_words = 0;
-    uint32_t *big = nullptr, *small = nullptr;
-    uint8_t *narrow = nullptr;  // per big item: 1 if its tree's levels average fewer than 64 positions
+    ItemMeta *big = nullptr, *small = nullptr;  // the work lists (big items flagged narrow in npos)
     // one problem: big items of workgroup b are big[wg_big[b] .. wg_big[b+1]), small items of wave c
     // (workgroup c / nw) small[wave_small[c] .. wave_small[c+1]) (null for a batch: round-robin)
     int32_t *wg_big = nullptr, *wave_small = nullptr;
-    uint32_t *item_root = nullptr;
-    int64_t *item_cb = nullptr, *item_ce = nullptr;
     uint64_t *scratch = nullptr;
     std::vector<int32_t> big_off, small_off;  // first entry of each section
     std::vector<std::vector<int32_t>> big_cum, small_cum;
@@ -2637,13 +2654,11 @@ struct TreePlan {
 
 // The scratch is shared by a solver's plans (one runs at a time) and freed by its owner.
 void free_plan(TreePlan &p) {
-    void *ptrs[] = {p.big, p.small, p.narrow, p.item_root, p.item_cb, p.item_ce, p.wg_big, p.wave_small};
+    void *ptrs[] = {p.big, p.small, p.wg_big, p.wave_small};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
-    p.big = p.small = p.item_root = nullptr;
+    p.big = p.small = nullptr;
     p.wg_big = p.wave_small = nullptr;
-    p.narrow = nullptr;
-    p.item_cb = p.item_ce = nullptr;
     p.scratch = nullptr;
 }
 
@@ -2678,8 +2693,7 @@ struct mcb_solver {
     // blocked form (block_phases > 1)
     int32_t bk = 1;                                   // phases per block; 1 = one kernel per phase
     int32_t q_waves = 0, q_big_words = 0, q_small_words = 0, q_blocks = 0, q_lds = 0, n_qbig = 0, n_qsmall = 0;
-    uint32_t *q_big_list = nullptr, *q_small_list = nullptr;  // the chosen plan's lists
-    uint8_t *q_big_narrow = nullptr;
+    ItemMeta *q_big_list = nullptr, *q_small_list = nullptr;  // the chosen plan's lists
     int32_t *q_wg_big = nullptr, *q_wave_small = nullptr;
     std::vector<TreePlan> plans;
     int32_t plan = 0;
@@ -2808,14 +2822,10 @@ BlockArgs block_args(const mcb_solver *s, int32_t e, int32_t kk) {
     a.pos_end = s->pos_end;
     if (!s->plans.empty()) {
         const TreePlan &tp = s->plans[s->plan];
-        a.item_root = tp.item_root;
-        a.item_cb = tp.item_cb;
-        a.item_ce = tp.item_ce;
         a.scratch = tp.scratch;
         a.huge_words = tp.huge_words;
     }
     a.big_list = s->q_big_list;
-    a.big_narrow = s->q_big_narrow;
     a.small_list = s->q_small_list;
     a.wg_big = s->q_wg_big;
     a.wave_small = s->q_wave_small;
@@ -3240,13 +3250,22 @@ int plan_tree_pass(int nw, int64_t cap_words, int nsplit, const int64_t *tree_of
     }
     p.n_big = static_cast<int32_t>(big_all.size());
     p.n_small = static_cast<int32_t>(small_all.size());
+    auto meta_of = [&](uint32_t it, bool nar) {
+        const uint32_t r = item_root[it];
+        ItemMeta m{};
+        m.off = tree_off[r];
+        m.cb = item_cb[it];
+        m.ce = item_ce[it];
+        m.npos = static_cast<uint32_t>(tree_off[r + 1] - tree_off[r]) | (nar ? kNarrowBit : 0u);
+        m.root = r;
+        return m;
+    };
+    std::vector<ItemMeta> big_meta(big_all.size()), small_meta(small_all.size());
+    for (size_t i = 0; i < big_all.size(); ++i) big_meta[i] = meta_of(big_all[i], narrow_all[i] != 0);
+    for (size_t i = 0; i < small_all.size(); ++i) small_meta[i] = meta_of(small_all[i], false);
     int rc;
-    if ((rc = upload(&p.big, big_all.data(), big_all.size(), bytes)) ||
-        (rc = upload(&p.small, small_all.data(), small_all.size(), bytes)) ||
-        (rc = upload(&p.narrow, narrow_all.data(), narrow_all.size(), bytes)) ||
-        (rc = upload(&p.item_root, item_root.data(), item_root.size(), bytes)) ||
-        (rc = upload(&p.item_cb, item_cb.data(), item_cb.size(), bytes)) ||
-        (rc = upload(&p.item_ce, item_ce.data(), item_ce.size(), bytes)))
+    if ((rc = upload(&p.big, big_meta.data(), big_meta.size(), bytes)) ||
+        (rc = upload(&p.small, small_meta.data(), small_meta.size(), bytes)))
         return rc;
     if (nsplit == 1 && ((rc = upload(&p.wg_big, wg_big.data(), wg_big.size(), bytes)) ||
                         (rc = upload(&p.wave_small, wave_small.data(), wave_small.size(), bytes))))
@@ -3311,7 +3330,6 @@ void use_plan(mcb_solver *s, size_t i) {
     s->n_qsmall = p.n_small;
     s->q_big_list = p.big;
     s->q_small_list = p.small;
-    s->q_big_narrow = p.narrow;
     s->q_wg_big = p.wg_big;
     s->q_wave_small = p.wave_small;
 }
@@ -3959,10 +3977,6 @@ BatchArgs batch_args(const mcb_batch *b, int32_t s) {
     const TreePlan &tp = b->plans[b->plan];
     a.big_list = tp.big;
     a.small_list = tp.small;
-    a.big_narrow = tp.narrow;
-    a.item_root = tp.item_root;
-    a.item_cb = tp.item_cb;
-    a.item_ce = tp.item_ce;
     a.scratch = tp.scratch;
     a.huge_words = tp.huge_words;
     a.rc_off = b->rc_off;
