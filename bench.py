@@ -156,6 +156,9 @@ def main() -> int:
     ap.add_argument("--no-prune", action="store_true")
     ap.add_argument("--breakdown", action="store_true",
                     help="time every kernel (HIP events between all launches), not only the dominant one")
+    ap.add_argument("--timing-every", type=int, default=16, metavar="K",
+                    help="single graphs: HIP events around the timed kernels of every K-th block only (an event "
+                         "pair holds the stream a few microseconds; 1 = every block)")
     ap.add_argument("--block-phases", type=int, default=0,
                     help="phases per pass over the trees (0 automatic, 1 one kernel per phase)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -211,7 +214,8 @@ def time_leg(form: str, graph, args, threads: int) -> dict:
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     solver = Solver(prob, timing=1 if args.breakdown else 2, block_phases=args.block_phases,
-                    waves_per_block=args.tree_waves, tree_cap=args.tree_cap, merge_group=args.merge_group)
+                    waves_per_block=args.tree_waves, tree_cap=args.tree_cap, merge_group=args.merge_group,
+                    timing_every=args.timing_every)
     upload_s = time.perf_counter() - t0
     phases = args.phases if 0 < args.phases < info.num_non_tree else None
 
@@ -287,14 +291,19 @@ def run_single(args) -> int:
     # counts them.  Blocked: the tree pass streams every position (8 B) and every candidate (16 B);
     # its few independent candidate words are written sparsely and the update is a kernel of its
     # own, reported apart.
+    # The work counters cover every launch; the events only the sampled blocks (every
+    # --timing-every-th), whose average duration stands for all of them.
     shape = head["shape"]
     dominant = "k_phase" if shape["block_phases"] == 1 else "k_block_q"
     k_ms, k_n = kt["parity_search"]
     avg_ms = k_ms / max(k_n, 1)
     blocked = shape["block_phases"] > 1
+    run_phases = args.phases if 0 < args.phases < info.num_non_tree else info.num_non_tree
+    per_step = -(-run_phases // shape["block_phases"])  # launches of the dominant kernel per loop
+    launches = per_step * steps
     cand_bytes = 16.0
     dom_bytes = 8.0 * pos + cand_bytes * cand + (0 if blocked else upd)
-    bytes_per_launch = dom_bytes / max(k_n, 1)
+    bytes_per_launch = dom_bytes / launches
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     loop_s = head["loop_s"]
     out = {
@@ -330,8 +339,9 @@ def run_single(args) -> int:
         "block_phases": shape["block_phases"],
         "tree_pass_waves": shape["waves_per_block"],
         "tree_pass_lds": shape["lds_bytes"],
-        "launches_per_step": k_n // max(steps, 1),
-        "kernel_ms_per_step": {k: round(v[0] / steps, 3) for k, v in kt.items() if v[1]},
+        "launches_per_step": per_step,
+        "timed_launches": k_n,
+        "kernel_ms_per_step": {k: round(v[0] / v[1] * per_step, 3) for k, v in kt.items() if v[1]},
         "update_bytes_per_step": upd // max(steps, 1),
         "roofline": {
             "kernel": dominant,
@@ -342,12 +352,12 @@ def run_single(args) -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(dominant, args.workload) if not args.no_prune else None,
             "bytes_per_launch": round(bytes_per_launch, 1),
-            "bytes_split": {"tree_positions": round(8.0 * pos / max(k_n, 1), 1),
-                            "candidates": round(cand_bytes * cand / max(k_n, 1), 1),
-                            "support_update": 0.0 if blocked else round(upd / max(k_n, 1), 1)},
+            "bytes_split": {"tree_positions": round(8.0 * pos / launches, 1),
+                            "candidates": round(cand_bytes * cand / launches, 1),
+                            "support_update": 0.0 if blocked else round(upd / launches, 1)},
             "avg_launch_us": round(avg_ms * 1000.0, 3),
         },
-        "roofline_update": update_roofline(kt, upd, steps, "k_block_dot+k_block_xor",
+        "roofline_update": update_roofline(kt, upd, steps, per_step, "k_block_dot+k_block_xor",
                                            f"k_block_update_{args.workload}") if blocked and not args.no_prune else None,
     }
     if len(legs) == 2:
@@ -386,17 +396,19 @@ def run_single(args) -> int:
     return 0
 
 
-def update_roofline(kt, upd: int, steps: int, kernels: str, profile: str):
+def update_roofline(kt, upd: int, steps: int, per_step: int, kernels: str, profile: str):
     """The support vector update (dot products and XORs of later vectors): the bytes the update
     kernels count as they run (each later vector's live prefix read once per block, and read and
-    written once more where the block's combination is nonzero) over their event-timed time."""
+    written once more where the block's combination is nonzero) over their event-timed time.  The
+    events may cover a sample of the per_step blocks of a loop; their mean stands for every block."""
     ms, n = kt["update"]
     if n == 0 or ms <= 0:
         return None
-    gbs = upd / (ms * 1e-3) / 1e9
+    ms_step = ms / n * per_step
+    gbs = upd / steps / (ms_step * 1e-3) / 1e9
     return {"kernels": kernels, "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_step": upd // max(steps, 1),
-            "ms_per_step": round(ms / steps, 3), "launch_pairs_per_step": n // max(steps, 1),
+            "ms_per_step": round(ms_step, 3), "launch_pairs_per_step": per_step, "timed_pairs": n,
             "traffic": measured_traffic(profile, None, "traffic_bytes_per_step:")}
 
 
@@ -596,7 +608,7 @@ def run_components(args) -> int:
             "bytes_per_launch": round(bytes_per_launch, 1),
             "avg_launch_us": round(avg_ms * 1000.0, 3),
         },
-        "roofline_update": update_roofline(kt, upd, args.steps, "k_batch_dot+k_batch_xor",
+        "roofline_update": update_roofline(kt, upd, args.steps, k_n // max(args.steps, 1), "k_batch_dot+k_batch_xor",
                                            f"k_batch_update_{args.workload}"),
     }
     if untimed is not None:

@@ -140,6 +140,10 @@ typedef struct {
                                  as-large-as-fits, 8192 and 2048; -1 = as large as fits */
     int32_t merge_group;      /* blocked form's greedy basis: first-level bases merged per merge wave,
                                  2..64; 0 = automatic */
+    int32_t timing_every;     /* with timing on, only every timing_every-th block of phases (one kernel per
+                                 phase: every timing_every-th phase; batch: step) records events, counted
+                                 from the last reset; 0 or 1 = every one.  An event pair costs the stream a
+                                 few microseconds, so a sample keeps a timed loop close to an untimed one. */
 } mcb_solver_options;
 
 void mcb_solver_options_init(mcb_solver_options *opt);
@@ -238,7 +242,7 @@ void mcb_solver_destroy(mcb_solver *s);
 typedef struct mcb_batch mcb_batch;
 
 /* descs[0..count-1] are validated like mcb_solver_create's; of opt, store_cycles, timing,
- * waves_per_block and tree_cap are used (the plan is timed on step 0).  Fails with MCB_E_INVALID when a tree does
+ * timing_every, waves_per_block and tree_cap are used (the plan is timed on step 0).  Fails with MCB_E_INVALID when a tree does
  * not fit the blocked form's LDS budget. */
 int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solver_options *opt, mcb_batch **out);
 

@@ -34,6 +34,7 @@ class SolverOptions(ctypes.Structure):
         ("block_phases", i32),
         ("tree_cap", i32),
         ("merge_group", i32),
+        ("timing_every", i32),
     ]
 
 

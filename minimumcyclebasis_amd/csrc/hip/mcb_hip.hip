@@ -27,6 +27,7 @@
 // S_e rotates through four contiguous buffers and C_e through two, so a phase reads earlier
 // phases' outputs while writing its own.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cstdint>
@@ -2717,6 +2718,9 @@ struct mcb_solver {
     std::vector<int> ev_kind;
     size_t ev_used = 0;
     bool timing_open = false;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;  // the open group's events, until launches take them
+    bool timing_sample = true;  // this block (phase, step) records events
+    int64_t timing_seq = 0;     // blocks (phases, steps) launched since reset
 };
 
 namespace {
@@ -2735,25 +2739,55 @@ int harvest_events(S *s) {
     return MCB_OK;
 }
 
+// Called once per block of phases (phase, step) before its launches: whether its kernels record events.
+template <typename S>
+void timing_sample(S *s) {
+    const int64_t every = s->opt.timing_every > 1 ? s->opt.timing_every : 1;
+    s->timing_sample = s->timing_seq++ % every == 0;
+}
+
 template <typename S>
 int timed_begin(S *s, int kind) {
-    s->timing_open =
-        s->opt.timing == 1 || (s->opt.timing == 2 && (kind == MCB_K_PARITY_SEARCH || kind == MCB_K_UPDATE));
+    s->timing_open = s->timing_sample && (s->opt.timing == 1 || (s->opt.timing == 2 && (kind == MCB_K_PARITY_SEARCH ||
+                                                                                           kind == MCB_K_UPDATE)));
     if (!s->timing_open) return MCB_OK;
     if (s->ev_used + 2 > s->ev_pool.size()) {
         int rc = harvest_events(s);
         if (rc) return rc;
     }
     s->ev_kind[s->ev_used / 2] = kind;
-    HIP_TRY(hipEventRecord(s->ev_pool[s->ev_used++], s->stream));
+    s->ev_start = s->ev_pool[s->ev_used++];
+    s->ev_stop = s->ev_pool[s->ev_used++];
     return MCB_OK;
 }
 
+// Closes the group timed_begin opened.  Its launches carried its events; a group with no launch
+// records them as markers.
 template <typename S>
 int timed_end(S *s) {
     if (!s->timing_open) return MCB_OK;
     s->timing_open = false;
-    HIP_TRY(hipEventRecord(s->ev_pool[s->ev_used++], s->stream));
+    if (s->ev_start) HIP_TRY(hipEventRecord(s->ev_start, s->stream));
+    if (s->ev_stop) HIP_TRY(hipEventRecord(s->ev_stop, s->stream));
+    s->ev_start = s->ev_stop = nullptr;
+    return MCB_OK;
+}
+
+// Launches a kernel on the solver's (batch's) stream.  In an open timing group the group's first
+// launch carries the start event and the launch flagged `last` the stop event; the runtime stamps
+// them with those kernels' own start and end, so the group's time holds no marker packets and no
+// gap before its first kernel or after its last.  Launches without events go through the plain path
+// (graph capture uses it).
+template <typename S, typename... KArgs, typename... Args>
+int launch_on(S *s, bool last, void (*kernel)(KArgs...), dim3 grid, dim3 block, size_t lds, Args... args) {
+    hipEvent_t start = s->ev_start, stop = last ? s->ev_stop : nullptr;
+    s->ev_start = nullptr;
+    if (last) s->ev_stop = nullptr;
+    if (start || stop)
+        hipExtLaunchKernelGGL(kernel, grid, block, static_cast<uint32_t>(lds), s->stream, start, stop, 0u, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, lds, s->stream, args...);
+    HIP_TRY(hipGetLastError());
     return MCB_OK;
 }
 
@@ -2888,58 +2922,59 @@ BlockArgs block_args(const mcb_solver *s, int32_t e, int32_t kk) {
 int launch_block(mcb_solver *s, int32_t e, int32_t kk) {
     const BlockArgs a = block_args(s, e, kk);
     const int64_t wl = (static_cast<int64_t>(e) + kk + kWave - 1) / kWave;
+    timing_sample(s);
     int rc = timed_begin(s, MCB_K_TRANSPOSE);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_block_transpose, dim3(static_cast<unsigned>((wl + 3) / 4)), dim3(256), 0, s->stream, a);
-    HIP_TRY(hipGetLastError());
+    if ((rc = launch_on(s, true, k_block_transpose, dim3(static_cast<unsigned>((wl + 3) / 4)), dim3(256), 0, a)))
+        return rc;
     if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_PARITY_SEARCH))) return rc;
-    hipLaunchKernelGGL(k_block_q, dim3(s->q_blocks), dim3(s->q_waves * kWave), s->q_lds, s->stream, a);
-    HIP_TRY(hipGetLastError());
+    if ((rc = launch_on(s, true, k_block_q, dim3(s->q_blocks), dim3(s->q_waves * kWave), s->q_lds, a))) return rc;
     if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_SELECT))) return rc;
-    hipLaunchKernelGGL(k_block_basis, dim3(static_cast<unsigned>((s->lvl_n[0] + 3) / 4)), dim3(256), 0, s->stream, a);
-    HIP_TRY(hipGetLastError());
-    for (int L = 0; L + 1 < s->levels; ++L) {
-        hipLaunchKernelGGL(k_block_merge, dim3(static_cast<unsigned>(s->lvl_n[L + 1])), dim3(kMergeWaves * kWave), 0,
-                           s->stream, a, L);
-        HIP_TRY(hipGetLastError());
-    }
+    if ((rc = launch_on(s, s->levels == 1, k_block_basis, dim3(static_cast<unsigned>((s->lvl_n[0] + 3) / 4)),
+                        dim3(256), 0, a)))
+        return rc;
+    for (int L = 0; L + 1 < s->levels; ++L)
+        if ((rc = launch_on(s, L + 2 == s->levels, k_block_merge, dim3(static_cast<unsigned>(s->lvl_n[L + 1])),
+                            dim3(kMergeWaves * kWave), 0, a, L)))
+            return rc;
     if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_CYCLE))) return rc;
-    hipLaunchKernelGGL(k_block_cycles_final, dim3(static_cast<unsigned>(kk + (wl + 3) / 4)), dim3(256),
-                       sizeof(uint64_t) * s->W, s->stream, a);
-    HIP_TRY(hipGetLastError());
+    if ((rc = launch_on(s, false, k_block_cycles_final, dim3(static_cast<unsigned>(kk + (wl + 3) / 4)), dim3(256),
+                        sizeof(uint64_t) * s->W, a)))
+        return rc;
     const int32_t dlim = (e + kWave - 1) / kWave;
-    hipLaunchKernelGGL(k_block_pairs, dim3(static_cast<unsigned>(1 + (s->W + 3) / 4)), dim3(256),
-                       sizeof(uint64_t) * dlim + 2 * 256 * sizeof(uint32_t), s->stream, a);
-    HIP_TRY(hipGetLastError());
+    if ((rc = launch_on(s, true, k_block_pairs, dim3(static_cast<unsigned>(1 + (s->W + 3) / 4)), dim3(256),
+                        sizeof(uint64_t) * dlim + 2 * 256 * sizeof(uint32_t), a)))
+        return rc;
     if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_UPDATE))) return rc;
     const int64_t later = s->N - e - kk;
     if (later > 0) {
         const size_t dlds = sizeof(uint64_t) * (kBlockK + kSegPairs) + sizeof(uint32_t) * (kSegPairs + 2 * kSegWords + 1);
         // a fixed grid: a short word list is staged once per workgroup and its waves take tiles in turn;
         // a long one is taken tile by tile by whole workgroups
-        hipLaunchKernelGGL(k_block_dot,
-                           dim3(static_cast<unsigned>(std::min<int64_t>((later + kDotCols - 1) / kDotCols,
-                                                                        kDotWorkgroups))),
-                           dim3(kDotWaves * kWave), dlds, s->stream, a);
-        HIP_TRY(hipGetLastError());
+        if ((rc = launch_on(s, false, k_block_dot,
+                            dim3(static_cast<unsigned>(std::min<int64_t>((later + kDotCols - 1) / kDotCols,
+                                                                         kDotWorkgroups))),
+                            dim3(kDotWaves * kWave), dlds, a)))
+            return rc;
     }
     // (column tile, word tile) pairs over the block's own columns and the listed later ones, dealt to at
     // most kXorWorkgroups workgroups (the list's length is known only on the device)
     const int64_t most = (1 + (std::max<int64_t>(later, 0) + 255) / 256) * ((wl + kTile - 1) / kTile);
-    hipLaunchKernelGGL(k_block_xor, dim3(static_cast<unsigned>(std::min<int64_t>(most, kXorWorkgroups))), dim3(256),
-                       sizeof(uint64_t) * 16 * 16 * kTile, s->stream, a);
-    HIP_TRY(hipGetLastError());
+    if ((rc = launch_on(s, true, k_block_xor, dim3(static_cast<unsigned>(std::min<int64_t>(most, kXorWorkgroups))),
+                        dim3(256), sizeof(uint64_t) * 16 * 16 * kTile, a)))
+        return rc;
     s->blk_positions += s->plans[s->plan].pos_cum.back();
     s->blk_candidates += s->plans[s->plan].cand_cum.back();
     return timed_end(s);
 }
 
 int launch_phase(mcb_solver *s, int32_t e) {
+    timing_sample(s);
     int rc = timed_begin(s, MCB_K_PARITY_SEARCH);
     if (rc) return rc;
     const PhaseArgs a = phase_args(s, e);
-    hipLaunchKernelGGL((k_phase<true, true>), dim3(s->blocks), dim3(s->wpb * kWave), s->lds_bytes, s->stream, a);
-    HIP_TRY(hipGetLastError());
+    if ((rc = launch_on(s, true, k_phase<true, true>, dim3(s->blocks), dim3(s->wpb * kWave), s->lds_bytes, a)))
+        return rc;
     return timed_end(s);
 }
 
@@ -3474,6 +3509,7 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
     if (rc) return rc;
     if (opt && (opt->merge_group < 0 || opt->merge_group == 1 || opt->merge_group > 64))
         return fail(MCB_E_INVALID, "merge_group must be 0 or 2..64");
+    if (opt && opt->timing_every < 0) return fail(MCB_E_INVALID, "timing_every must be 0 or more");
     rc = ensure_device();
     if (rc) return rc;
     mcb_solver *s = new (std::nothrow) mcb_solver();
@@ -3648,6 +3684,7 @@ int mcb_solver_reset(mcb_solver *s) {
     s->blk_positions = 0;
     s->blk_candidates = 0;
     s->ev_used = 0;
+    s->timing_seq = 0;
     for (int i = 0; i < MCB_K_COUNT; ++i) {
         s->kt_ms[i] = 0;
         s->kt_n[i] = 0;
@@ -3943,6 +3980,9 @@ struct mcb_batch {
     std::vector<int> ev_kind;
     size_t ev_used = 0;
     bool timing_open = false;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;  // the open group's events, until launches take them
+    bool timing_sample = true;  // this block (phase, step) records events
+    int64_t timing_seq = 0;     // blocks (phases, steps) launched since reset
 };
 
 namespace {
@@ -4042,47 +4082,50 @@ int launch_batch_step(mcb_batch *b, int32_t s) {
     BatchArgs a = batch_args(b, s);
     const int32_t A = a.active;
     const int64_t off = b->step_off[s];
+    timing_sample(b);
     int rc = timed_begin(b, MCB_K_TRANSPOSE);
     if (rc) return rc;
     const int64_t tw = static_cast<int64_t>(A) * (s + 1);
-    hipLaunchKernelGGL(k_batch_transpose, dim3(static_cast<unsigned>((tw + 3) / 4)), dim3(256), 0, b->stream, a);
-    HIP_TRY(hipGetLastError());
+    if ((rc = launch_on(b, true, k_batch_transpose, dim3(static_cast<unsigned>((tw + 3) / 4)), dim3(256), 0, a)))
+        return rc;
     if ((rc = timed_end(b)) || (rc = timed_begin(b, MCB_K_PARITY_SEARCH))) return rc;
     const TreePlan &tp = b->plans[b->plan];
-    hipLaunchKernelGGL(k_batch_q, dim3(batch_q_grid(tp)), dim3(tp.nw * kWave), tp.lds, b->stream, a);
-    HIP_TRY(hipGetLastError());
+    if ((rc = launch_on(b, true, k_batch_q, dim3(batch_q_grid(tp)), dim3(tp.nw * kWave), tp.lds, a))) return rc;
     if ((rc = timed_end(b)) || (rc = timed_begin(b, MCB_K_SELECT))) return rc;
+    // merge levels that some running problem has (no running problem has a level L+1 past the first empty one)
+    int nm = 0;
+    while (nm < b->merge_levels && b->grp_cum[nm][A] != 0) ++nm;
     a.n_items = b->chunk_cum[A];
-    hipLaunchKernelGGL(k_batch_basis, dim3(static_cast<unsigned>((a.n_items + 3) / 4)), dim3(256), 0, b->stream, a);
-    HIP_TRY(hipGetLastError());
-    for (int L = 0; L < b->merge_levels; ++L) {
+    if ((rc = launch_on(b, nm == 0, k_batch_basis, dim3(static_cast<unsigned>((a.n_items + 3) / 4)), dim3(256), 0, a)))
+        return rc;
+    for (int L = 0; L < nm; ++L) {
         const int32_t n = b->grp_cum[L][A];
-        if (n == 0) break;  // no running problem has a level L+1, nor any higher one
         a.groups = b->d_groups + b->grp_off[L];
         a.n_items = n;
-        hipLaunchKernelGGL(k_batch_merge, dim3(static_cast<unsigned>(n)), dim3(kMergeWaves * kWave), 0, b->stream, a, L);
-        HIP_TRY(hipGetLastError());
+        if ((rc = launch_on(b, L + 1 == nm, k_batch_merge, dim3(static_cast<unsigned>(n)), dim3(kMergeWaves * kWave),
+                            0, a, L)))
+            return rc;
     }
     if ((rc = timed_end(b)) || (rc = timed_begin(b, MCB_K_CYCLE))) return rc;
     // problems are in descending phase count, so problem 0 has the most words
-    hipLaunchKernelGGL(k_batch_cycles, dim3(static_cast<unsigned>(A) * kBlockK), dim3(256),
-                       sizeof(uint64_t) * b->comps[0].W, b->stream, a);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_batch_final, dim3(static_cast<unsigned>((tw + 3) / 4)), dim3(256), 0, b->stream, a);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_batch_pairs, dim3(static_cast<unsigned>(b->pairs_cum[A])), dim3(256),
-                       sizeof(uint64_t) * s + 2 * 256 * sizeof(uint32_t), b->stream, a);
-    HIP_TRY(hipGetLastError());
+    if ((rc = launch_on(b, false, k_batch_cycles, dim3(static_cast<unsigned>(A) * kBlockK), dim3(256),
+                        sizeof(uint64_t) * b->comps[0].W, a)))
+        return rc;
+    if ((rc = launch_on(b, false, k_batch_final, dim3(static_cast<unsigned>((tw + 3) / 4)), dim3(256), 0, a)))
+        return rc;
+    if ((rc = launch_on(b, true, k_batch_pairs, dim3(static_cast<unsigned>(b->pairs_cum[A])), dim3(256),
+                        sizeof(uint64_t) * s + 2 * 256 * sizeof(uint32_t), a)))
+        return rc;
     if ((rc = timed_end(b)) || (rc = timed_begin(b, MCB_K_UPDATE))) return rc;
     const int32_t nd = b->dot_cum[off + A];
     if (nd > 0) {
         const size_t dlds = sizeof(uint64_t) * (kBlockK + kSegPairs) + sizeof(uint32_t) * (kSegPairs + 2 * kSegWords + 1);
-        hipLaunchKernelGGL(k_batch_dot, dim3(static_cast<unsigned>(nd)), dim3(kDotWaves * kWave), dlds, b->stream, a);
-        HIP_TRY(hipGetLastError());
+        if ((rc = launch_on(b, false, k_batch_dot, dim3(static_cast<unsigned>(nd)), dim3(kDotWaves * kWave), dlds, a)))
+            return rc;
     }
-    hipLaunchKernelGGL(k_batch_xor, dim3(static_cast<unsigned>(b->xor_cum[off + A])), dim3(256),
-                       sizeof(uint64_t) * kBlockK * kTileDirect, b->stream, a);
-    HIP_TRY(hipGetLastError());
+    if ((rc = launch_on(b, true, k_batch_xor, dim3(static_cast<unsigned>(b->xor_cum[off + A])), dim3(256),
+                        sizeof(uint64_t) * kBlockK * kTileDirect, a)))
+        return rc;
     b->blk_positions += tp.pos_cum[A];
     b->blk_candidates += tp.cand_cum[A];
     return timed_end(b);
@@ -4121,6 +4164,7 @@ int mcb_batch_reset(mcb_batch *b) {
     b->blk_positions = 0;
     b->blk_candidates = 0;
     b->ev_used = 0;
+    b->timing_seq = 0;
     for (int i = 0; i < MCB_K_COUNT; ++i) {
         b->kt_ms[i] = 0;
         b->kt_n[i] = 0;
@@ -4139,6 +4183,7 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
     }
     if (opt && (opt->merge_group < 0 || opt->merge_group == 1 || opt->merge_group > 64))
         return fail(MCB_E_INVALID, "merge_group must be 0 or 2..64");
+    if (opt && opt->timing_every < 0) return fail(MCB_E_INVALID, "timing_every must be 0 or more");
     int rc = ensure_device();
     if (rc) return rc;
     mcb_batch *b = new (std::nothrow) mcb_batch();

@@ -61,8 +61,9 @@ class Solver:
 
     def __init__(self, problem: Problem, store_cycles: bool = False, timing: int | bool = False,
                  waves_per_block: int = 0, max_blocks: int = 0, block_phases: int = 0, tree_cap: int = 0,
-                 merge_group: int = 0):
+                 merge_group: int = 0, timing_every: int = 0):
         """timing: 0/False none, 1/True every kernel, 2 the tree pass and the update only (HIP events).
+        timing_every: with timing on, only every timing_every-th block of phases records events (0, 1 all).
         block_phases: phases per pass over the trees (0 automatic, 1 one kernel per phase, up to 64).
         waves_per_block: waves per workgroup of the blocked tree pass (4, 8, 16; 0 timed at create).
         tree_cap: largest tree kept in LDS by the tree pass (0 timed at create, -1 as large as fits).
@@ -86,6 +87,7 @@ class Solver:
         o.block_phases = int(block_phases)
         o.tree_cap = int(tree_cap)
         o.merge_group = int(merge_group)
+        o.timing_every = int(timing_every)
         h = ctypes.c_void_p()
         _lib.check_hip(_lib.hip().mcb_solver_create(ctypes.byref(d), ctypes.byref(o), ctypes.byref(h)))
         self._h = h
@@ -196,7 +198,7 @@ class Batch:
     them.  Each problem's selections equal its own ``Solver``'s."""
 
     def __init__(self, problems, store_cycles: bool = False, timing: int | bool = False, waves_per_block: int = 0,
-                 tree_cap: int = 0, merge_group: int = 0):
+                 tree_cap: int = 0, merge_group: int = 0, timing_every: int = 0):
         problems = list(problems)
         if not problems:
             raise ValueError("no problems")
@@ -219,6 +221,7 @@ class Batch:
         o.waves_per_block = int(waves_per_block)
         o.tree_cap = int(tree_cap)
         o.merge_group = int(merge_group)
+        o.timing_every = int(timing_every)
         h = ctypes.c_void_p()
         _lib.check_hip(_lib.hip().mcb_batch_create(ctypes.cast(descs, ctypes.c_void_p), len(problems),
                                                    ctypes.byref(o), ctypes.byref(h)))

@@ -268,3 +268,26 @@ def test_merge_group_rejected():
     for bad in (1, 65, -2):
         with pytest.raises(RuntimeError):
             Solver(p, merge_group=bad)
+
+
+@pytest.mark.parametrize("timing,every", [(2, 3), (1, 5), (2, 1)])
+def test_sampled_timing(timing, every):
+    """Events on every `every`-th block only: that many timed launches per kind, counted again from 0
+    after a reset, and the selections of an untimed run."""
+    p, ref = _merge_case("er1500")
+    blocks = -(-p.info.num_non_tree // 64)
+    print(f"case er1500 timing={timing} every={every} blocks={blocks}", flush=True)
+    s = Solver(p, timing=timing, timing_every=every)
+    kinds = ("parity_search", "update") if timing == 2 else ("parity_search", "select", "cycle", "update", "transpose")
+    for _ in range(2):
+        s.reset()
+        s.run()
+        s.sync()
+        assert np.array_equal(s.selected(), ref["selected"])
+        for k in kinds:
+            ms, n = s.kernel_time(k)
+            assert n == -(-blocks // every), k
+            assert ms > 0, k
+    s.close()
+    with pytest.raises(RuntimeError):
+        Solver(p, timing=2, timing_every=-1)
