@@ -737,6 +737,7 @@ struct BlockArgs {
     int64_t lvl_slot[kMaxLevels];  // first slot of each level
     int64_t lvl_n[kMaxLevels];     // bases on each level
     int32_t levels;
+    uint32_t *arrive;    // [slots] members of each merge group done so far, by the group's slot (zero between launches)
     uint32_t *sel;       // [N] rank chosen in each phase
     uint64_t *blk_q;     // [64] word of phase i's winner
     uint64_t *blk_m;     // [64] mask m_i
@@ -847,6 +848,9 @@ __global__ __launch_bounds__(256) void k_block_transpose(BlockArgs a) {
     if (w >= wl) return;
     transpose_body(view_of(a), w, lane);
 }
+
+// The lowest set bit of x alone (0 for x = 0).
+__device__ __forceinline__ uint64_t lowbit(uint64_t x) { return x & (0ull - x); }
 
 // Value of a 64-bit register in lane l (l wave-uniform).
 __device__ __forceinline__ uint64_t rl64(uint64_t x, int l) {
@@ -975,8 +979,8 @@ __device__ __forceinline__ void tree_words_coop(const uint32_t *pp, const int32_
 // pivots (one XOR per pivot column the word has); then, while a lane is nonzero, the first such
 // lane becomes the pivot of its lowest set bit, that bit is cleared from the other pivots and from
 // the later lanes.  The lanes that become pivots are exactly the batch vectors independent of the
-// basis and of the batch vectors before them; emit() is called in each, in rank order.  Returns
-// the number of new pivots (wave-uniform).
+// basis and of the batch vectors before them; emit() is called in each of them (together, once the
+// batch is inserted).  Returns the number of new pivots (wave-uniform).
 template <typename EmitFn>
 __device__ __forceinline__ int basis_reduce_insert(uint64_t &piv, uint64_t &P, uint64_t q, int lane, EmitFn emit) {
     uint64_t v = q;
@@ -984,26 +988,33 @@ __device__ __forceinline__ int basis_reduce_insert(uint64_t &piv, uint64_t &P, u
     if (__ballot((v & P) != 0)) {
         uint64_t r = 0;
         for (uint64_t pm = P; pm; pm &= pm - 1) {
-            const int col = __ffsll(static_cast<unsigned long long>(pm)) - 1;
-            const uint64_t pc = rl64(piv, col);
-            if ((v >> col) & 1ull) r ^= pc;
+            const uint64_t cb = lowbit(pm);
+            const uint64_t pc = rl64(piv, __ffsll(static_cast<unsigned long long>(cb)) - 1);
+            r ^= (v & cb) ? pc : 0ull;
         }
         v ^= r;
     }
-    int added = 0;
+    // One new pivot per step, without branches inside the step: the chain from one ballot to the
+    // next is a lane read, the pivot row's lowest bit and one masked XOR.  A lane becomes a pivot at
+    // most once (its own word is cleared then), so its emit() can wait until the loop ends.
+    const uint64_t me = 1ull << lane;
+    uint64_t mine = 0;
     uint64_t nz = __ballot(v != 0);
+    int added = 0;
     while (nz) {
-        const int p = __ffsll(static_cast<unsigned long long>(nz)) - 1;
+        const uint64_t pbit = lowbit(nz);
+        const int p = __ffsll(static_cast<unsigned long long>(pbit)) - 1;
         const uint64_t pvv = rl64(v, p);
-        const int col = __builtin_amdgcn_readfirstlane(__ffsll(static_cast<unsigned long long>(pvv)) - 1);
-        if (lane == col) piv = pvv;
-        else if ((piv >> col) & 1ull) piv ^= pvv;
-        P = static_cast<uint64_t>(uniform64(static_cast<int64_t>(P | (1ull << col))));
+        const uint64_t cbit = lowbit(pvv);
+        mine |= me & pbit;
+        piv = (me & cbit) ? pvv : ((piv & cbit) ? piv ^ pvv : piv);
+        P |= cbit;
         ++added;
-        if (lane == p) emit();
-        if ((v >> col) & 1ull) v ^= pvv;
+        v ^= (v & cbit) ? pvv : 0ull;
         nz = __ballot(v != 0);
     }
+    P = static_cast<uint64_t>(uniform64(static_cast<int64_t>(P)));
+    if (mine) emit();
     return added;
 }
 
@@ -1206,18 +1217,19 @@ __device__ __forceinline__ void basis_insert_tab(uint64_t &piv, uint64_t &P, uin
         for (int g = 0; g < 16; ++g) r ^= tab[g * 16 + static_cast<int>((v >> (4 * g)) & 15ull)];
         v ^= r;
     }
+    const uint64_t me = 1ull << lane;
     uint64_t mine = 0;
     uint64_t nz = __ballot(v != 0);
-    while (nz) {
-        const int p = __ffsll(static_cast<unsigned long long>(nz)) - 1;
+    if (nz) stale = true;
+    while (nz) {  // as in basis_reduce_insert
+        const uint64_t pbit = lowbit(nz);
+        const int p = __ffsll(static_cast<unsigned long long>(pbit)) - 1;
         const uint64_t pvv = rl64(v, p);
-        const int col = __ffsll(static_cast<unsigned long long>(pvv)) - 1;
-        if (lane == col) piv = pvv;
-        else if ((piv >> col) & 1ull) piv ^= pvv;
-        P |= 1ull << col;
-        stale = true;
-        if (lane == p) mine = 1;
-        if ((v >> col) & 1ull) v ^= pvv;
+        const uint64_t cbit = lowbit(pvv);
+        mine |= me & pbit;
+        piv = (me & cbit) ? pvv : ((piv & cbit) ? piv ^ pvv : piv);
+        P |= cbit;
+        v ^= (v & cbit) ? pvv : 0ull;
         nz = __ballot(v != 0);
     }
     // each new pivot lane writes its own entry, after the new pivots of the lanes before it
@@ -1254,24 +1266,25 @@ __device__ void finalize_core(int kk, const uint32_t *sum_cnt, const uint32_t *s
         // kk are zero)
         uint64_t col = transpose64(q, lane);
         uint64_t m = lane < kk ? (1ull << lane) : 0ull;
-        uint64_t taken = 0;  // wave-uniform
-        int win = 0;         // lane i: rank position of phase i's winner
+        uint64_t taken = 0;   // wave-uniform
+        uint64_t winbit = 0;  // lane i: phase i's winner as a bit of the rank positions
+        bool missing = false; // a column without an untaken bit (then the block fails)
+        // Branch-free steps (columns at or above kk are zero, so they never take an update): the
+        // chain from one step to the next is a lane read, the lowest untaken bit and a masked XOR.
+#pragma unroll 4
         for (int b = 0; b < kk; ++b) {
             const uint64_t cb = rl64(col, b);
-            const uint64_t avail = cb & ~taken;
-            if (!avail) {
-                ok = false;
-                break;
-            }
-            const int r = __ffsll(static_cast<unsigned long long>(avail)) - 1;
-            taken |= 1ull << r;
-            if (lane == b) win = r;
+            const uint64_t rbit = lowbit(cb & ~taken);
+            missing |= rbit == 0;
+            taken |= rbit;
+            winbit = lane == b ? rbit : winbit;
             const uint64_t mb = rl64(m, b);
-            if (lane > b && lane < kk && ((col >> r) & 1ull)) {
-                m ^= mb;
-                col ^= cb;
-            }
+            const bool hit = lane > b && (col & rbit) != 0;
+            m ^= hit ? mb : 0ull;
+            col ^= hit ? cb : 0ull;
         }
+        ok = !missing;
+        const int win = winbit ? __ffsll(static_cast<unsigned long long>(winbit)) - 1 : 0;
         if (ok) {
             const uint32_t qlo = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(q)), win));
             const uint32_t qhi = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(q >> 32)), win));
@@ -1363,24 +1376,6 @@ __device__ __forceinline__ void basis_chunk(uint64_t *qbits, const uint64_t *Qr,
     if (lane == 0) sum_cnt[slot] = cnt;
 }
 
-// Wave c: ranks [c*kChunk, (c+1)*kChunk) -> slot c of the first level.
-__global__ __launch_bounds__(256) void k_block_basis(BlockArgs a) {
-    __shared__ uint64_t s_words[4][kWave];
-    __shared__ uint32_t s_offs[4][kWave];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wv = wave_id();
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * 4 + wv;
-    if (c >= a.lvl_n[0]) return;
-    const int64_t r0 = c * kChunk;
-    const int64_t nwords = (a.K + kWave - 1) / kWave - (r0 >> 6);
-    basis_chunk(a.qbits, a.Qr, r0, static_cast<int>(nwords < kWave ? nwords : kWave), a.lvl_slot[0] + c, a.kk,
-                a.sum_cnt, a.sum_rank, a.sum_q, s_words[wv], s_offs[wv], lane);
-    if (a.levels == 1) {
-        __threadfence();
-        block_finalize(a, lane);
-    }
-}
-
 // Merges the `members` bases (at most 64) in slots s0 .. s0+members-1 into slot `slot`.  Their
 // entries, concatenated in order, are in rank order.  All of them are first copied into LDS (one
 // coalesced load per base, sixteen bases in flight: the bases were written by the previous launch,
@@ -1444,26 +1439,108 @@ __device__ __forceinline__ bool merge_group(int64_t s0, int members, int64_t slo
     return true;
 }
 
-// One merge level: wave g merges bases g*group .. g*group+group-1 of level L into slot g of level
-// L+1.  At the top level the single wave also finalises the block.  The bases of level L were
-// written by the previous launch.
-__global__ __launch_bounds__(kMergeWaves * kWave) void k_block_merge(BlockArgs a, int L) {
-    __shared__ uint64_t s_tab[kMergeTab];
-    __shared__ uint64_t s_stq[kMergeEntries];
-    __shared__ uint32_t s_str[kMergeEntries];
+// Merges the `members` bases (at most 64) in slots s0 .. s0+members-1 into slot `slot` with one
+// wave.  Their entries, concatenated in order, are in rank order; they are inserted 64 at a time,
+// each batch loaded while the one before it is inserted.  A merge reaches kk pivots after a few
+// batches, so only those are read.  offs: 64 words of LDS (the members' first entries), tab:
+// kMergeTab words of reduction tables.
+__device__ __forceinline__ void merge_stream(int64_t s0, int members, int64_t slot, int kk, uint32_t *sum_cnt,
+                                             uint32_t *sum_rank, uint64_t *sum_q, uint64_t *tab, uint32_t *offs,
+                                             int lane) {
+    const uint32_t own = lane < members ? sum_cnt[s0 + lane] : 0u;
+    uint32_t incl = own;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), kWave - 1));
+    offs[lane] = incl - own;  // the lanes past the members hold total
+    __builtin_amdgcn_wave_barrier();
+    // entry t: the last member whose first entry is at or before t owns it
+    auto fetch = [&](uint32_t t, uint64_t &q, uint32_t &r) {
+        q = 0;
+        r = 0;
+        if (t < total) {
+            int lo = 0, hi = kWave - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (offs[mid] <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            const int64_t at = (s0 + lo) * kWave + (t - offs[lo]);
+            q = sum_q[at];
+            r = sum_rank[at];
+        }
+    };
+    uint64_t qn;
+    uint32_t rn;
+    fetch(static_cast<uint32_t>(lane), qn, rn);
+    uint64_t piv = 0, P = 0;
+    uint32_t cnt = 0;
+    bool stale = true;
+    for (uint32_t b0 = 0; b0 < total && cnt < static_cast<uint32_t>(kk); b0 += kWave) {
+        const uint64_t q = qn;
+        const uint32_t r = rn;
+        fetch(b0 + kWave + lane, qn, rn);
+        basis_insert_tab(piv, P, cnt, stale, tab, q, r, sum_rank + slot * kWave, sum_q + slot * kWave, lane);
+    }
+    if (lane == 0) sum_cnt[slot] = cnt;
+}
+
+// Wave c: ranks [c*kChunk, (c+1)*kChunk) -> slot c of the first level (and the block's finalisation
+// when that is the only level).
+__global__ __launch_bounds__(256) void k_block_basis(BlockArgs a) {
+    __shared__ uint64_t s_words[4][kWave];
+    __shared__ uint32_t s_offs[4][kWave];
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = wave_id();
-    const int64_t g = blockIdx.x;
-    if (g >= a.lvl_n[L + 1]) return;
-    const int64_t rest = a.lvl_n[L] - g * a.group;
-    const int members = static_cast<int>(rest < a.group ? rest : a.group);
-    if (!merge_group(a.lvl_slot[L] + g * a.group, members, a.lvl_slot[L + 1] + g, a.kk, a.sum_cnt, a.sum_rank,
-                     a.sum_q, s_tab, s_stq, s_str, lane, wv))
-        return;
-    if (L + 2 == a.levels) {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * 4 + wv;
+    if (c >= a.lvl_n[0]) return;
+    const int64_t r0 = c * kChunk;
+    const int64_t nwords = (a.K + kWave - 1) / kWave - (r0 >> 6);
+    basis_chunk(a.qbits, a.Qr, r0, static_cast<int>(nwords < kWave ? nwords : kWave), a.lvl_slot[0] + c, a.kk,
+                a.sum_cnt, a.sum_rank, a.sum_q, s_words[wv], s_offs[wv], lane);
+    if (a.levels == 1) {
         __threadfence();
         block_finalize(a, lane);
     }
+}
+
+// Every merge level of the block's greedy basis in one launch of one-wave workgroups.  Workgroup g
+// merges first-level group g (written by the previous launch); from there on, the last member of
+// each merge group to finish merges that group: each wave publishes its merged base with an
+// agent-scope release fence before it counts itself in the group's arrival counter, and the last
+// arrival clears the counter for the next block and takes an acquire fence before it reads the
+// others' bases.  The wave that completes the top level finalises the block.  (The first level
+// stays a launch of its own: fences in its hundreds of waves, each writing back and invalidating
+// its L2, cost more than the launch.)
+__global__ __launch_bounds__(64) void k_block_cascade(BlockArgs a) {
+    __shared__ uint32_t s_offs[kWave];
+    __shared__ uint64_t s_tab[kMergeTab];
+    const int lane = threadIdx.x & (kWave - 1);
+    int64_t c = blockIdx.x;
+    if (c >= a.lvl_n[1]) return;
+    for (int L = 0; L + 1 < a.levels; ++L) {
+        const int64_t g = L == 0 ? c : c / a.group;
+        const int64_t rest = a.lvl_n[L] - g * a.group;
+        const int members = static_cast<int>(rest < a.group ? rest : a.group);
+        const int64_t dst = a.lvl_slot[L + 1] + g;
+        if (L > 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            uint32_t before = 0;
+            if (lane == 0) before = atomicAdd(a.arrive + dst, 1u);
+            before = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(before), 0));
+            if (before + 1 != static_cast<uint32_t>(members)) return;
+            if (lane == 0) atomicExch(a.arrive + dst, 0u);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        merge_stream(a.lvl_slot[L] + g * a.group, members, dst, a.kk, a.sum_cnt, a.sum_rank, a.sum_q, s_tab,
+                     s_offs, lane);
+        c = g;
+    }
+    __threadfence();
+    block_finalize(a, lane);
 }
 
 // C of one chosen candidate into LDS (W words, cleared by the caller): the coordinates of the edges
@@ -2706,6 +2783,7 @@ struct mcb_solver {
     uint64_t *C_blk = nullptr, *Sf_blk = nullptr, *Ct = nullptr, *Bc = nullptr, *pair_c = nullptr;
     uint32_t *pw_list = nullptr, *pw_off = nullptr, *pw_cnt = nullptr, *pair_i = nullptr;
     uint32_t *sum_cnt = nullptr, *sum_rank = nullptr, *D_blk = nullptr, *Dcnt_blk = nullptr;
+    uint32_t *arrive = nullptr;  // merge group arrivals of k_block_cascade
     uint32_t *ulist = nullptr, *ucount = nullptr;  // the block's updated later columns
     unsigned long long *upd_bytes = nullptr;
     int32_t levels = 0, group = 16;
@@ -2886,6 +2964,7 @@ BlockArgs block_args(const mcb_solver *s, int32_t e, int32_t kk) {
     }
     a.levels = s->levels;
     a.group = s->group;
+    a.arrive = s->arrive;
     a.sel = s->best;
     a.blk_q = s->blk_q;
     a.blk_m = s->blk_m;
@@ -2933,10 +3012,9 @@ int launch_block(mcb_solver *s, int32_t e, int32_t kk) {
     if ((rc = launch_on(s, s->levels == 1, k_block_basis, dim3(static_cast<unsigned>((s->lvl_n[0] + 3) / 4)),
                         dim3(256), 0, a)))
         return rc;
-    for (int L = 0; L + 1 < s->levels; ++L)
-        if ((rc = launch_on(s, L + 2 == s->levels, k_block_merge, dim3(static_cast<unsigned>(s->lvl_n[L + 1])),
-                            dim3(kMergeWaves * kWave), 0, a, L)))
-            return rc;
+    if (s->levels > 1 &&
+        (rc = launch_on(s, true, k_block_cascade, dim3(static_cast<unsigned>(s->lvl_n[1])), dim3(kWave), 0, a)))
+        return rc;
     if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_CYCLE))) return rc;
     if ((rc = launch_on(s, false, k_block_cycles_final, dim3(static_cast<unsigned>(kk + (wl + 3) / 4)), dim3(256),
                         sizeof(uint64_t) * s->W, a)))
@@ -3449,6 +3527,8 @@ int setup_blocked(mcb_solver *s, const int64_t *tree_offset, const uint32_t *par
     if ((rc = dev_alloc(&s->Qr, static_cast<size_t>(s->K), B))) return rc;
     if ((rc = dev_alloc(&s->qbits, static_cast<size_t>((s->K + 63) / 64), B))) return rc;
     if ((rc = dev_alloc(&s->sum_cnt, static_cast<size_t>(slots), B))) return rc;
+    if ((rc = dev_alloc(&s->arrive, static_cast<size_t>(slots), B))) return rc;
+    HIP_TRY(hipMemset(s->arrive, 0, static_cast<size_t>(slots) * sizeof(uint32_t)));
     if ((rc = dev_alloc(&s->sum_rank, static_cast<size_t>(slots) * kWave, B))) return rc;
     if ((rc = dev_alloc(&s->sum_q, static_cast<size_t>(slots) * kWave, B))) return rc;
     if ((rc = dev_alloc(&s->blk_q, kBlockK, B))) return rc;
@@ -3488,7 +3568,7 @@ void mcb_solver_destroy(mcb_solver *s) {
                     s->cand_pu, s->cand_pv, s->cand_coord, s->Sw, s->Scur, s->C, s->C_store, s->D, s->Dcount,
                     s->best, s->done, s->status, s->wave_stats, s->flag, s->E, s->Ecount, s->cdot,
                     s->Tt, s->Qr, s->qbits, s->sum_q, s->blk_q, s->blk_m, s->blk_g, s->C_blk,
-                    s->Sf_blk, s->sum_cnt, s->sum_rank, s->D_blk, s->Dcnt_blk, s->upd_bytes,
+                    s->Sf_blk, s->sum_cnt, s->arrive, s->sum_rank, s->D_blk, s->Dcnt_blk, s->upd_bytes,
                     s->Ct, s->Bc, s->pair_c, s->pw_list, s->pw_off, s->pw_cnt, s->pair_i, s->ulist, s->ucount};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
