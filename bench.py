@@ -64,8 +64,8 @@ def host_cores() -> int:
     return max(1, n)
 
 
-def cpu_baseline(n, u, v, w, sel: np.ndarray, S: np.ndarray, threads: int, prune: bool, phases: int = 12,
-                 roots: int = 512) -> dict:
+def cpu_baseline(n, u, v, w, sel: np.ndarray, S: np.ndarray, threads: int, prune: bool, phases: int = 24,
+                 roots: int = 4096) -> dict:
     """The CPU oracle (the reference's CPU phase loop, mcb/mcb_cycle.cpp:237-301, restated in C) timed on
     ``phases`` phases spread evenly over the whole loop, each started from the GPU's own S_e (``S``, every
     support vector as used in its phase; ``sel`` the GPU's selections, erased from the list as the reference
@@ -156,6 +156,8 @@ def main() -> int:
     ap.add_argument("--no-prune", action="store_true")
     ap.add_argument("--breakdown", action="store_true",
                     help="time every kernel (HIP events between all launches), not only the dominant one")
+    ap.add_argument("--no-timing", action="store_true",
+                    help="no HIP events at all (counter passes: the kernel times and rooflines come out empty)")
     ap.add_argument("--timing-every", type=int, default=16, metavar="K",
                     help="single graphs: HIP events around the timed kernels of every K-th block only (an event "
                          "pair holds the stream a few microseconds; 1 = every block)")
@@ -163,7 +165,7 @@ def main() -> int:
                     help="phases per pass over the trees (0 automatic, 1 one kernel per phase)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="component workloads: CPU oracle seconds")
-    ap.add_argument("--cpu-phases", type=int, default=12, help="single graphs: phases the CPU oracle times")
+    ap.add_argument("--cpu-phases", type=int, default=24, help="single graphs: phases the CPU oracle times")
     ap.add_argument("--phases", type=int, default=0,
                     help="profiling aid: run only the first PHASES phases per step (0 = the full loop)")
     ap.add_argument("--progress-phases", type=int, default=0, metavar="K",
@@ -213,9 +215,9 @@ def time_leg(form: str, graph, args, threads: int) -> dict:
           f"F={info.fvs_size} K={info.num_candidates} positions={info.positions} host={host_s:.2f}s",
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    solver = Solver(prob, timing=1 if args.breakdown else 2, block_phases=args.block_phases,
-                    waves_per_block=args.tree_waves, tree_cap=args.tree_cap, merge_group=args.merge_group,
-                    timing_every=args.timing_every)
+    timing = 0 if args.no_timing else 1 if args.breakdown else 2
+    solver = Solver(prob, timing=timing, block_phases=args.block_phases, waves_per_block=args.tree_waves,
+                    tree_cap=args.tree_cap, merge_group=args.merge_group, timing_every=args.timing_every)
     upload_s = time.perf_counter() - t0
     phases = args.phases if 0 < args.phases < info.num_non_tree else None
 
@@ -499,7 +501,8 @@ def run_components(args) -> int:
           file=sys.stderr, flush=True)
     dev = device_info()
     t0 = time.perf_counter()
-    bt = Batch(loop, timing=1 if args.breakdown else 2, waves_per_block=args.tree_waves, tree_cap=args.tree_cap,
+    timing = 0 if args.no_timing else 1 if args.breakdown else 2
+    bt = Batch(loop, timing=timing, waves_per_block=args.tree_waves, tree_cap=args.tree_cap,
                merge_group=args.merge_group)
     upload_s = time.perf_counter() - t0
     print(f"[bench] batch arena {bt.device_bytes / 2**30:.2f} GiB, {bt.steps} steps, upload {upload_s:.2f}s",
