@@ -12,9 +12,11 @@ for tiny graphs, the edge lists) and the reference's outputs:
   printed MSB-first per 64-bit word, words ascending (include/bit_vector.h:97-116,143-149).
 * ``csv_*.json``       - the stats CSV lines of mcb_cpu (mcb/mcb_non_cpu_rd.cpp) and
   mcb_cpu_baseline (mcb/mcb_cycle.cpp) on larger graphs: Fvs_size, Initial_Cycles,
-  Final_cycles, Total_Weight (include/stats.h:256-338).
+  Final_cycles, Total_Weight (include/stats.h:256-338).  The extra-large tier (``--xlarge``,
+  ER G(20000, 60000)) runs only mcb_cpu, the partner of the degree-two-contracted path; it takes
+  hours on 8 threads.
 
-    python tests/golden/make_golden.py [--only NAME] [--skip-large]
+    python tests/golden/make_golden.py [--only NAME] [--skip-large] [--xlarge [--threads T]]
 """
 from __future__ import annotations
 
@@ -75,6 +77,11 @@ LARGE = {
     "delaunay2000": ("delaunay", {"n": 2000, "seed": 1}),
     "er2000": ("erdos_renyi_lbc", {"n": 2000, "m": 6000, "wmax": 1000, "seed": 1}),
     "chunglu2000": ("chung_lu_lbc", {"n": 2000, "gamma": 2.3, "avg_degree": 8.0, "wmax": 10000, "seed": 1}),
+}
+
+
+XLARGE = {
+    "er20000": ("erdos_renyi_lbc", {"n": 20000, "m": 60000, "wmax": 1000, "seed": 1}),
 }
 
 
@@ -228,8 +235,26 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", default=None)
     ap.add_argument("--skip-large", action="store_true")
+    ap.add_argument("--xlarge", action="store_true", help="only the extra-large tier (mcb_cpu alone)")
+    ap.add_argument("--threads", type=int, default=8)
     args = ap.parse_args()
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True, stdout=subprocess.DEVNULL)
+    if args.xlarge:
+        with tempfile.TemporaryDirectory() as td:
+            for name, (gen, kw) in XLARGE.items():
+                if args.only and name != args.only:
+                    continue
+                n, u, v, w = make_graph(gen, kw)
+                mtx = os.path.join(td, name + ".mtx")
+                G.write_mtx(mtx, n, u, v, w)
+                print(f"{name}: n={n} m={len(u)}, mcb_cpu on {args.threads} threads", flush=True)
+                fx = {"name": name, "tier": "xlarge", "generator": gen, "kwargs": kw,
+                      "edge_digest": edge_digest(n, u, v, w), "threads": args.threads,
+                      "mcb_cpu": run_csv(os.path.join(REF_DIR, "mcb_cpu"), mtx, args.threads)}
+                with open(os.path.join(HERE, f"csv_{name}.json"), "w") as f:
+                    json.dump(fx, f, indent=1)
+                print(f"{name}: {fx['mcb_cpu']}", flush=True)
+        return
     test_rd = os.path.join(REF_DIR, "Test_deg_two_remove")
     test_plain = os.path.join(REF_DIR, "Test")
     with tempfile.TemporaryDirectory() as td:
