@@ -282,6 +282,48 @@ int mcb_batch_launch_shape(const mcb_batch *b, int32_t *blocks, int32_t *waves_p
 void mcb_batch_destroy(mcb_batch *b);
 
 /* ---------------------------------------------------------------------------------------------
+ * Shortest-path trees of the FVS roots on the device.  Replaces the host's per-root searches
+ * (include/Dijkstra.h:78-127 with the candidate filter of include/Dijkstra.h:205-254, run one root per
+ * OpenMP thread at mcb/mcb_cycle_deg_two_remove.cpp:232-242 and include/work_per_thread.h:92-147) for
+ * the pruned layout: every root's stopped search, with the reference heap's pop order, on one wave per
+ * root.  Its results feed mcb_problem_finish_trees (include/mcb_host.h) of a problem built with
+ * external_trees=1, which ranks the candidates and lays the trees out exactly as the host search would.
+ * Errors: mcb_sptree_last_error().
+ * ------------------------------------------------------------------------------------------- */
+
+typedef struct mcb_sptree mcb_sptree;
+
+typedef struct {
+    int64_t slabs;          /* per-wave workspaces (waves of the widest stage) */
+    int64_t launches;
+    int64_t stage_roots[3]; /* roots finished in each stage: LDS heap of 1/8, 1/2, all of a CU's LDS */
+    int64_t host_roots;     /* roots whose heap outgrew the last stage run: left to the host library */
+    int64_t candidates;     /* candidate and layout entries handed over */
+    int64_t kept;
+    double device_ms;       /* all stages on the device, launches to the last status read */
+} mcb_sptree_stats;
+
+/* Uploads the reduced multigraph (the MCB_A_* arrays of the same names, include/mcb_host.h) and sizes
+ * the per-wave workspaces (at most a quarter of the free device memory). */
+int mcb_sptree_create(int32_t nodes, int32_t directed_edges, const uint32_t *row_offsets, const uint32_t *rows,
+                      const uint32_t *cols, const int32_t *weights, const uint32_t *reverse, const int32_t *fvs_rank,
+                      const uint32_t *roots, int32_t num_roots, mcb_sptree **out);
+
+/* Searches every root, in up to `stages` (1..3) launches of growing LDS heap; a root whose heap outgrows
+ * a stage is searched again in the next.  Blocks until the results are on the host. */
+int mcb_sptree_run(mcb_sptree *t, int stages);
+
+/* The results of the last run in the form mcb_problem_finish_trees takes (valid until the next run or
+ * mcb_sptree_destroy); roots left to the host have status != 0. */
+int mcb_sptree_results(const mcb_sptree *t, mcb_root_trees *view);
+
+int mcb_sptree_get_stats(const mcb_sptree *t, mcb_sptree_stats *stats);
+
+void mcb_sptree_destroy(mcb_sptree *t);
+
+const char *mcb_sptree_last_error(void);
+
+/* ---------------------------------------------------------------------------------------------
  * Utilities
  * ------------------------------------------------------------------------------------------- */
 

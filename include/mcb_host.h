@@ -52,6 +52,9 @@ typedef struct {
     int fvs_order_len;
     int stop_after_fvs;     /* 1: build only the reduced multigraph and its FVS (the arrays up to
                                MCB_A_ROOTS; info through fvs_size and num_non_tree) */
+    int external_trees;     /* 1: build up to the coordinates (the arrays up to MCB_A_NT_MAP) and leave the
+                               shortest-path trees to mcb_problem_finish_trees, fed by a search elsewhere
+                               (the device search of mcb_hip.h); needs prune_trees=1, keep_vertex_trees=0 */
 } mcb_build_options;
 
 void mcb_build_options_init(mcb_build_options *opt);
@@ -121,6 +124,39 @@ int mcb_problem_reference_layout(const mcb_problem *p, int32_t r, uint32_t *rows
                                  int32_t *edge_offset, uint32_t *nodes_index);
 
 void mcb_problem_free(mcb_problem *p);
+
+/* Per-root results of the stopped shortest-path searches (include/Dijkstra.h:78-127 with the candidate
+ * filter of include/Dijkstra.h:205-254), in the form a device search hands them over.  For root index r
+ * with status[r] == 0:
+ *   cand_edge/cand_weight[cand_start[r] .. + cand_count[r]]  the accepted candidate edges (offset of the
+ *       direction listed first, include/Dijkstra.h:135-148) and their cycle weights d(u)+d(v)+w(e), in any
+ *       order;
+ *   kept[4*(kept_start[r] .. + kept_count[r])]  the vertices of the pruned layout other than the root (the
+ *       candidate endpoints and all their tree ancestors), each as four 32-bit words (vertex, parent edge
+ *       offset, hop level, finalisation index with the root at 0), in any order.
+ * A root with status[r] != 0 is searched by the host library itself. */
+typedef struct {
+    const int32_t *status;
+    const int64_t *cand_start;
+    const int32_t *cand_count;
+    const uint32_t *cand_edge;
+    const int32_t *cand_weight;
+    const int64_t *kept_start;
+    const int32_t *kept_count;
+    const uint32_t *kept;
+} mcb_root_trees;
+
+/* Completes a problem built with external_trees=1 from per-root search results: candidate lists in the
+ * reference's order and rank (mcb/mcb_cycle_deg_two_remove.cpp:244-258), pruned BFS-level layouts.  The
+ * result is identical to building the problem with the trees searched on the host. */
+int mcb_problem_finish_trees(mcb_problem *p, const mcb_root_trees *trees, int num_threads);
+
+/* The host's own stopped search of root index r in the mcb_root_trees form (a problem built with
+ * external_trees=1 or prune_trees=1).  Writes at most cand_cap candidates and kept_cap kept vertices
+ * (4 words each) and their counts; returns MCB_E_INVALID, with the counts set, when a capacity is short. */
+int mcb_problem_root_search(const mcb_problem *p, int32_t r, int64_t cand_cap, uint32_t *cand_edge,
+                            int32_t *cand_weight, int64_t kept_cap, uint32_t *kept, int32_t *num_cand,
+                            int32_t *num_kept);
 
 /* Reverse-delete feedback vertex set of the reduced multigraph (FVS.cpp:166-227), given the order in
  * which vertices are tried (NULL: degree asc, id desc).  Writes rank per vertex or -1; returns F or <0. */

@@ -38,6 +38,18 @@ class SolverOptions(ctypes.Structure):
     ]
 
 
+class SpTreeStats(ctypes.Structure):
+    _fields_ = [
+        ("slabs", i64),
+        ("launches", i64),
+        ("stage_roots", i64 * 3),
+        ("host_roots", i64),
+        ("candidates", i64),
+        ("kept", i64),
+        ("device_ms", ctypes.c_double),
+    ]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "init_cuda": (None, []),
@@ -98,6 +110,12 @@ SIGNATURES = {
     "mcb_mlp_train_epoch": (ctypes.c_int, [P, P, i64, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
     "mcb_mlp_destroy": (None, [P]),
     "mcb_mlp_last_error": (ctypes.c_char_p, []),
+    "mcb_sptree_create": (ctypes.c_int, [i32, i32, P, P, P, P, P, P, P, i32, ctypes.POINTER(P)]),
+    "mcb_sptree_run": (ctypes.c_int, [P, ctypes.c_int]),
+    "mcb_sptree_results": (ctypes.c_int, [P, P]),
+    "mcb_sptree_get_stats": (ctypes.c_int, [P, ctypes.POINTER(SpTreeStats)]),
+    "mcb_sptree_destroy": (None, [P]),
+    "mcb_sptree_last_error": (ctypes.c_char_p, []),
     "mcb_hip_device_count": (ctypes.c_int, []),
     "mcb_hip_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                            ctypes.POINTER(i64)]),

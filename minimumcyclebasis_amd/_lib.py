@@ -52,7 +52,15 @@ class BuildOptions(ctypes.Structure):
         ("fvs_order", ctypes.POINTER(ctypes.c_int32)),
         ("fvs_order_len", ctypes.c_int),
         ("stop_after_fvs", ctypes.c_int),
+        ("external_trees", ctypes.c_int),
     ]
+
+
+class RootTrees(ctypes.Structure):
+    """mcb_root_trees (include/mcb_host.h): per-root results of the stopped shortest-path searches."""
+    _fields_ = [(name, ctypes.c_void_p) for name in
+                ("status", "cand_start", "cand_count", "cand_edge", "cand_weight", "kept_start", "kept_count",
+                 "kept")]
 
 
 class ProblemInfo(ctypes.Structure):
@@ -99,6 +107,9 @@ def host() -> ctypes.CDLL:
         lib.mcb_problem_get_info.argtypes = [P, ctypes.POINTER(ProblemInfo)]
         lib.mcb_problem_array.argtypes = [P, ctypes.c_int, ctypes.POINTER(P), ctypes.POINTER(ctypes.c_int64)]
         lib.mcb_problem_reference_layout.argtypes = [P, ctypes.c_int32, P, P, P, P]
+        lib.mcb_problem_finish_trees.argtypes = [P, ctypes.POINTER(RootTrees), ctypes.c_int]
+        lib.mcb_problem_root_search.argtypes = [P, ctypes.c_int32, ctypes.c_int64, P, P, ctypes.c_int64, P,
+                                                ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
         lib.mcb_problem_free.argtypes = [P]
         lib.mcb_problem_free.restype = None
         lib.mcb_fvs_reverse_delete.argtypes = [ctypes.c_int32, ctypes.c_int32, P, P, P, P]

@@ -19,7 +19,8 @@ ROOT = os.path.dirname(PKG)
 INC = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "lib")
 HOST_SRC = [os.path.join(PKG, "csrc", "host", "mcb_host.cpp")]
-HIP_SRC = [os.path.join(PKG, "csrc", "hip", "mcb_hip.hip"), os.path.join(PKG, "csrc", "hip", "mcb_mlp.hip")]
+HIP_SRC = [os.path.join(PKG, "csrc", "hip", "mcb_hip.hip"), os.path.join(PKG, "csrc", "hip", "mcb_mlp.hip"),
+           os.path.join(PKG, "csrc", "hip", "mcb_sptree.hip")]
 ARCH = os.environ.get("MCB_OFFLOAD_ARCH", "gfx950")
 
 HOST_FLAGS = ["-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-Wall", "-Wextra", f"-I{INC}"]

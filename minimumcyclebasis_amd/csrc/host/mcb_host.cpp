@@ -611,6 +611,7 @@ struct mcb_problem {
     std::vector<uint32_t> parent_pos, root_cand_rank;
     std::vector<int32_t> pos_coord, vertex_parent;
     std::vector<uint32_t> pos_vertex;
+    bool trees_pending = false;  // built with external_trees: mcb_problem_finish_trees completes it
 };
 
 namespace {
@@ -732,6 +733,106 @@ void process_root(const mcb_problem &P, uint32_t src, bool prune, Workspace &ws,
     }
 }
 
+// One root's outputs from the results of its stopped search, as a device search hands them over: the
+// accepted candidate edges with their weights and the vertices of the pruned layout other than the root
+// (vertex, parent edge, level, finalisation index), both in any order.  The accepted edges are listed in
+// ascending offset, which is the order process_root meets them (rows of ascending vertex, each row in
+// offset order), so the groups and their iteration order are process_root's; the layout puts the kept
+// vertices in level buckets, finalisation order inside a level, as process_root does.
+void root_from_search(const mcb_problem &P, uint32_t src, int32_t nc, const uint32_t *ce, const int32_t *cw,
+                      int32_t nk, const uint32_t *kept, Workspace &ws, RootOutput &out) {
+    const Multigraph &g = P.g;
+    ws.prepare(g.n, P.fvs_rank.data());
+    std::vector<std::pair<uint32_t, int32_t>> cands(nc);
+    for (int32_t i = 0; i < nc; ++i) cands[i] = std::make_pair(ce[i], cw[i]);
+    std::sort(cands.begin(), cands.end());
+    std::unordered_map<unsigned long long, uint32_t> groups_of;
+    std::vector<std::vector<uint32_t>> groups;  // indices into cands
+    for (uint32_t i = 0; i < cands.size(); ++i) {
+        const uint32_t e = cands[i].first;
+        const unsigned long long key = (static_cast<unsigned long long>(g.rows[e]) << 32) | g.cols[e];
+        auto it = groups_of.find(key);
+        if (it == groups_of.end()) {
+            groups_of.insert(std::make_pair(key, static_cast<uint32_t>(groups.size())));
+            groups.emplace_back(1, i);
+        } else {
+            groups[it->second].push_back(i);
+        }
+    }
+    out.cand_edge.clear();
+    out.cand_weight.clear();
+    for (auto &kv : groups_of)
+        for (uint32_t i : groups[kv.second]) {
+            out.cand_edge.push_back(cands[i].first);
+            out.cand_weight.push_back(cands[i].second);
+        }
+    // (level, finalisation index, vertex, parent edge), sorted: level buckets, finalisation order inside
+    struct Kept {
+        uint32_t level, ord, v, pedge;
+        bool operator<(const Kept &o) const { return level != o.level ? level < o.level : ord < o.ord; }
+    };
+    std::vector<Kept> ks(nk);
+    for (int32_t i = 0; i < nk; ++i) ks[i] = Kept{kept[4 * i + 2], kept[4 * i + 3], kept[4 * i], kept[4 * i + 1]};
+    std::sort(ks.begin(), ks.end());
+    const uint32_t npos = static_cast<uint32_t>(nk) + 1;
+    ws.pos_of[src] = 0;
+    ws.touched.push_back(src);
+    for (uint32_t i = 0; i < ks.size(); ++i) {
+        ws.pos_of[ks[i].v] = i + 1;
+        ws.touched.push_back(ks[i].v);
+    }
+    out.parent_pos.assign(npos, 0xFFFFFFFFu);
+    out.coord.assign(npos, -1);
+    for (uint32_t i = 0; i < ks.size(); ++i) {
+        out.parent_pos[i + 1] = ws.pos_of[g.rows[ks[i].pedge]];
+        out.coord[i + 1] = P.nt_map[ks[i].pedge];
+    }
+    out.depth = ks.empty() ? 0 : static_cast<int32_t>(ks.back().level);
+    out.cand_pu.resize(out.cand_edge.size());
+    out.cand_pv.resize(out.cand_edge.size());
+    for (size_t k = 0; k < out.cand_edge.size(); ++k) {
+        out.cand_pu[k] = ws.pos_of[g.rows[out.cand_edge[k]]];
+        out.cand_pv[k] = ws.pos_of[g.cols[out.cand_edge[k]]];
+    }
+}
+
+// The host's stopped search of one root in the form root_from_search takes (the finalisation index
+// counts the root as 0).
+void search_results(const mcb_problem &P, uint32_t src, Workspace &ws, std::vector<uint32_t> &ce,
+                    std::vector<int32_t> &cw, std::vector<uint32_t> &kept) {
+    const Multigraph &g = P.g;
+    ws.prepare(g.n, P.fvs_rank.data());
+    dijkstra(g, src, true, ws, ws.heap);
+    const VertexState *V = ws.t.v.data();
+    ce.clear();
+    cw.clear();
+    kept.clear();
+    for (uint32_t i = 0; i < ws.t.order.size(); ++i) ws.pos_of[ws.t.order[i]] = i + 1;  // finalisation index
+    std::vector<uint32_t> open_rows(1, src);
+    for (uint32_t v : ws.t.order)
+        if (!V[v].blocked) open_rows.push_back(v);
+    std::sort(open_rows.begin(), open_rows.end());
+    for (uint32_t u : open_rows)
+        for (uint32_t e = g.ro[u]; e < g.ro[u + 1]; ++e) {
+            const uint32_t v = g.cols[e];
+            if (v < u || (v == u && e > g.rev[e])) continue;
+            if (!V[v].in_tree || V[v].blocked) continue;
+            if (V[v].pedge == static_cast<int32_t>(e) || V[u].pedge == static_cast<int32_t>(g.rev[e])) continue;
+            if (!(u == src || v == src || V[u].branch != V[v].branch)) continue;
+            ce.push_back(e);
+            cw.push_back(V[u].dist + V[v].dist + g.w[e]);
+        }
+    ws.keep[src] = 1;
+    for (uint32_t e : ce)
+        for (uint32_t x : {g.rows[e], g.cols[e]})
+            while (!ws.keep[x]) {
+                ws.keep[x] = 1;
+                kept.insert(kept.end(), {x, static_cast<uint32_t>(V[x].pedge), static_cast<uint32_t>(V[x].level),
+                                         ws.pos_of[x]});
+                x = g.rows[V[x].pedge];
+            }
+}
+
 // Stage times of build_from_input on stderr when MCB_HOST_TIMING is set (seconds since the previous mark).
 struct StageClock {
     bool on = std::getenv("MCB_HOST_TIMING") != nullptr;
@@ -743,6 +844,8 @@ struct StageClock {
         t = now;
     }
 };
+
+int assemble(mcb_problem &P, std::vector<RootOutput> &per_root, bool keep_vertex_trees, StageClock &clock);
 
 int build_from_input(InputGraph &in, int64_t input_weight_sum, const mcb_build_options &opt, mcb_problem **out) {
     StageClock clock;
@@ -831,6 +934,13 @@ int build_from_input(InputGraph &in, int64_t input_weight_sum, const mcb_build_o
         if (P->nt_map[i] < 0 && P->nt_map[g.rev[i]] >= 0) P->nt_map[i] = P->nt_map[g.rev[i]];
 
     clock.mark("spanning tree, coordinates");
+    if (opt.external_trees) {
+        if (!opt.prune_trees || opt.keep_vertex_trees)
+            return fail(MCB_E_INVALID, "external trees need prune_trees=1 and keep_vertex_trees=0");
+        P->trees_pending = true;
+        *out = P.release();
+        return MCB_OK;
+    }
     // Per-root trees and candidates.
     std::vector<RootOutput> per_root(F);
     if (opt.keep_vertex_trees) P->vertex_parent.assign(static_cast<size_t>(F) * g.n, -1);
@@ -852,7 +962,18 @@ int build_from_input(InputGraph &in, int64_t input_weight_sum, const mcb_build_o
     }
     if (err) return fail(MCB_E_NOMEM, "out of memory while building shortest-path trees");
     clock.mark("shortest-path trees");
+    int rc2 = assemble(*P, per_root, opt.keep_vertex_trees != 0, clock);
+    if (rc2) return rc2;
+    *out = P.release();
+    return MCB_OK;
+}
 
+// Candidates of every root in the reference's collection order, ranked by weight, and the
+// concatenated position layouts (mcb/mcb_cycle_deg_two_remove.cpp:244-267).
+int assemble(mcb_problem &Pr, std::vector<RootOutput> &per_root, bool keep_vertex_trees, StageClock &clock) {
+    mcb_problem *P = &Pr;
+    mcb_problem_info &info = P->info;
+    const int32_t F = info.fvs_size;
     // Collect candidates root by root (roots ascend by vertex id) and rank them by weight with
     // the same std::sort over the same sequence (mcb_cycle_deg_two_remove.cpp:247-258).
     int64_t K = 0, positions = 0;
@@ -930,18 +1051,17 @@ int build_from_input(InputGraph &in, int64_t input_weight_sum, const mcb_build_o
         P->tree_offset[r + 1] = P->tree_offset[r] + static_cast<int64_t>(per_root[r].parent_pos.size());
     P->parent_pos.resize(positions);
     P->pos_coord.resize(positions);
-    if (opt.keep_vertex_trees) P->pos_vertex.resize(positions);
+    if (keep_vertex_trees) P->pos_vertex.resize(positions);
     for (int32_t r = 0; r < F; ++r) {
         std::copy(per_root[r].parent_pos.begin(), per_root[r].parent_pos.end(),
                   P->parent_pos.begin() + P->tree_offset[r]);
         std::copy(per_root[r].coord.begin(), per_root[r].coord.end(), P->pos_coord.begin() + P->tree_offset[r]);
-        if (opt.keep_vertex_trees)
+        if (keep_vertex_trees)
             std::copy(per_root[r].vertex.begin(), per_root[r].vertex.end(), P->pos_vertex.begin() + P->tree_offset[r]);
         RootOutput().parent_pos.swap(per_root[r].parent_pos);
         per_root[r] = RootOutput();
     }
     clock.mark("position layout");
-    *out = P.release();
     return MCB_OK;
 }
 
@@ -1327,6 +1447,66 @@ int mcb_problem_reference_layout(const mcb_problem *p, int32_t r, uint32_t *rows
 }
 
 void mcb_problem_free(mcb_problem *p) { delete p; }
+
+int mcb_problem_finish_trees(mcb_problem *p, const mcb_root_trees *t, int num_threads) {
+    if (!p || !t || !t->status) return fail(MCB_E_INVALID, "invalid arguments");
+    if (!p->trees_pending) return fail(MCB_E_STATE, "the problem was not built with external_trees=1");
+    const int32_t F = p->info.fvs_size;
+    StageClock clock;
+    std::vector<RootOutput> per_root(F);
+    const int threads = num_threads > 0 ? num_threads : omp_get_max_threads();
+    int err = 0;
+#pragma omp parallel num_threads(threads)
+    {
+        Workspace ws;
+#pragma omp for schedule(dynamic, 16)
+        for (int32_t r = 0; r < F; ++r) {
+            try {
+                if (t->status[r] == 0) {
+                    const int64_t c0 = t->cand_start[r], k0 = t->kept_start[r];
+                    root_from_search(*p, p->roots[r], t->cand_count[r], t->cand_edge + c0, t->cand_weight + c0,
+                                     t->kept_count[r], t->kept + 4 * k0, ws, per_root[r]);
+                } else {
+                    process_root(*p, p->roots[r], true, ws, per_root[r], nullptr);
+                }
+            } catch (const std::bad_alloc &) {
+#pragma omp atomic write
+                err = 1;
+            }
+        }
+    }
+    if (err) return fail(MCB_E_NOMEM, "out of memory while laying out shortest-path trees");
+    clock.mark("trees from search results");
+    const int rc = assemble(*p, per_root, false, clock);
+    if (rc) return rc;
+    p->trees_pending = false;
+    return MCB_OK;
+}
+
+int mcb_problem_root_search(const mcb_problem *p, int32_t r, int64_t cand_cap, uint32_t *cand_edge,
+                            int32_t *cand_weight, int64_t kept_cap, uint32_t *kept, int32_t *num_cand,
+                            int32_t *num_kept) {
+    if (!p || !num_cand || !num_kept || r < 0 || r >= p->info.fvs_size || p->roots.empty())
+        return fail(MCB_E_INVALID, "invalid arguments");
+    Workspace ws;
+    std::vector<uint32_t> ce, kv;
+    std::vector<int32_t> cw;
+    try {
+        search_results(*p, p->roots[r], ws, ce, cw, kv);
+    } catch (const std::bad_alloc &) {
+        return fail(MCB_E_NOMEM, "out of memory");
+    }
+    *num_cand = static_cast<int32_t>(ce.size());
+    *num_kept = static_cast<int32_t>(kv.size() / 4);
+    if (static_cast<int64_t>(ce.size()) > cand_cap || static_cast<int64_t>(kv.size() / 4) > kept_cap)
+        return fail(MCB_E_INVALID, "capacity too small for the root's results");
+    if (!ce.empty() && (!cand_edge || !cand_weight)) return fail(MCB_E_INVALID, "invalid arguments");
+    if (!kv.empty() && !kept) return fail(MCB_E_INVALID, "invalid arguments");
+    std::copy(ce.begin(), ce.end(), cand_edge);
+    std::copy(cw.begin(), cw.end(), cand_weight);
+    std::copy(kv.begin(), kv.end(), kept);
+    return MCB_OK;
+}
 
 int mcb_fvs_reverse_delete(int32_t nodes, int32_t directed_edges, const uint32_t *row_offsets, const uint32_t *cols,
                            const int32_t *order, int32_t *fvs_rank) {

@@ -3,7 +3,9 @@
 * ``Solver``    - the native device-resident phase loop over a ``host.Problem``.
 * ``GpuStruct`` - the reference-compatible chunked parity interface (gpu_struct,
   gpu/gpu_struct.cuh:15-84) over numpy chunk arrays in the compressed_trees layout.
-* ``minimum_cycle_basis`` - end to end: preprocess on the host, run every phase on the GPU.
+* ``SpTreeSearch`` - the roots' shortest-path trees searched on the device; ``problem_from_edges`` builds a
+  problem with them (``trees="device"``) or on the host (``trees="host"``), with identical arrays.
+* ``minimum_cycle_basis`` - end to end: preprocess, run every phase on the GPU.
 
 There is no CPU fallback: without the HIP library or a GPU every call raises.
 """
@@ -15,7 +17,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._hipsig import SolverDesc, SolverOptions
+from ._hipsig import SolverDesc, SolverOptions, SpTreeStats
 from .host import Problem
 from .stats import RunStats, stats_of
 
@@ -364,6 +366,114 @@ class GpuStruct:
             self.close()
         except Exception:
             pass
+
+
+class SpTreeSearch:
+    """The stopped shortest-path searches of a problem's FVS roots on the device (mcb_sptree_*,
+    include/mcb_hip.h); ``finish(problem)`` completes a problem built with ``Problem.for_external_trees``."""
+
+    def __init__(self, problem: Problem):
+        self._h = ctypes.c_void_p()
+        info = problem.info
+        keep = [problem.array(a) for a in ("row_offsets", "rows", "cols", "weights", "reverse", "fvs_rank", "roots")]
+        rc = _lib.hip().mcb_sptree_create(info.nodes, info.directed_edges, *[a.ctypes.data for a in keep],
+                                          info.fvs_size, ctypes.byref(self._h))
+        self._check(rc)
+        self.num_roots = info.fvs_size
+
+    @staticmethod
+    def _check(rc: int) -> int:
+        if rc < 0:
+            raise _lib.MCBError(rc, _lib.hip().mcb_sptree_last_error().decode(errors="replace"))
+        return rc
+
+    def run(self, stages: int = 3) -> None:
+        self._check(_lib.hip().mcb_sptree_run(self._h, int(stages)))
+
+    def view(self) -> _lib.RootTrees:
+        v = _lib.RootTrees()
+        self._check(_lib.hip().mcb_sptree_results(self._h, ctypes.byref(v)))
+        return v
+
+    def results(self) -> dict:
+        """Copies of the per-root results (status, starts, counts, pools) as numpy arrays."""
+        v = self.view()
+        F = self.num_roots
+
+        def arr(ptr, dt, count):
+            if count == 0 or not ptr:
+                return np.zeros(count, dtype=dt)
+            return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(np.ctypeslib.as_ctypes_type(dt))),
+                                         shape=(count,)).copy()
+
+        out = {"status": arr(v.status, np.int32, F), "cand_start": arr(v.cand_start, np.int64, F),
+               "cand_count": arr(v.cand_count, np.int32, F), "kept_start": arr(v.kept_start, np.int64, F),
+               "kept_count": arr(v.kept_count, np.int32, F)}
+        done = out["status"] == 0
+        nc = int((out["cand_start"] + out["cand_count"])[done].max(initial=0))
+        nk = int((out["kept_start"] + out["kept_count"])[done].max(initial=0))
+        out["cand_edge"] = arr(v.cand_edge, np.uint32, nc)
+        out["cand_weight"] = arr(v.cand_weight, np.int32, nc)
+        out["kept"] = arr(v.kept, np.uint32, 4 * nk).reshape(nk, 4)
+        return out
+
+    def finish(self, problem: Problem, num_threads: int = 0) -> None:
+        v = self.view()
+        _lib.check_host(_lib.host().mcb_problem_finish_trees(problem.handle, ctypes.byref(v), int(num_threads)))
+        problem._refresh()
+
+    def stats(self) -> dict:
+        st = SpTreeStats()
+        self._check(_lib.hip().mcb_sptree_get_stats(self._h, ctypes.byref(st)))
+        return {"slabs": st.slabs, "launches": st.launches, "stage_roots": list(st.stage_roots),
+                "host_roots": st.host_roots, "candidates": st.candidates, "kept": st.kept,
+                "device_ms": st.device_ms}
+
+    def close(self) -> None:
+        if self._h:
+            _lib.hip().mcb_sptree_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def problem_from_edges(n: int, u, v, w, trees: str = "device", reduce_degree_two: bool = True,
+                       num_threads: int = 0, fvs_order=None, stages: int = 3, timings: dict | None = None) -> Problem:
+    """A pruned-layout problem whose roots' shortest-path trees are searched on the device (trees="device")
+    or by the host library (trees="host"); both give identical arrays.  ``timings`` (optional dict) receives
+    the seconds of each part and the device search's statistics."""
+    import time
+
+    if trees == "host":
+        t0 = time.perf_counter()
+        p = Problem.from_edges(n, u, v, w, reduce_degree_two=reduce_degree_two, prune_trees=True,
+                               num_threads=num_threads, fvs_order=fvs_order)
+        if timings is not None:
+            timings["host_s"] = time.perf_counter() - t0
+        return p
+    if trees != "device":
+        raise ValueError("trees must be 'device' or 'host'")
+    t0 = time.perf_counter()
+    p = Problem.for_external_trees(n, u, v, w, reduce_degree_two=reduce_degree_two, num_threads=num_threads,
+                                   fvs_order=fvs_order)
+    t1 = time.perf_counter()
+    if p.info.single_cycle:
+        return p
+    sp = SpTreeSearch(p)
+    t2 = time.perf_counter()
+    sp.run(stages)
+    t3 = time.perf_counter()
+    sp.finish(p, num_threads)
+    t4 = time.perf_counter()
+    if timings is not None:
+        timings.update({"graph_fvs_coords_s": t1 - t0, "upload_s": t2 - t1, "device_search_s": t3 - t2,
+                        "finish_s": t4 - t3, "search": sp.stats()})
+    sp.close()
+    return p
 
 
 @dataclass

@@ -61,10 +61,11 @@ class Info:
 
 
 def _options(reduce_degree_two: bool, prune_trees: bool, keep_vertex_trees: bool, num_threads: int,
-             fvs_order=None, stop_after_fvs: bool = False):
+             fvs_order=None, stop_after_fvs: bool = False, external_trees: bool = False):
     opt = _lib.BuildOptions()
     _lib.host().mcb_build_options_init(ctypes.byref(opt))
     opt.stop_after_fvs = int(bool(stop_after_fvs))
+    opt.external_trees = int(bool(external_trees))
     opt.reduce_degree_two = int(bool(reduce_degree_two))
     opt.prune_trees = int(bool(prune_trees))
     opt.keep_vertex_trees = int(bool(keep_vertex_trees))
@@ -113,6 +114,62 @@ class Problem:
         _lib.check_host(_lib.host().mcb_problem_build_edges(
             int(n), int(u.size), u.ctypes.data, v.ctypes.data, w.ctypes.data, ctypes.byref(opt), ctypes.byref(h)))
         return cls(h)
+
+    @classmethod
+    def for_external_trees(cls, n: int, u, v, w, reduce_degree_two: bool = True, num_threads: int = 0,
+                           fvs_order=None) -> "Problem":
+        """The reduced graph, its FVS and coordinates, with the pruned shortest-path trees left to
+        ``finish_trees`` (fed by the device search, ``device.SpTreeSearch``, or by ``root_search``)."""
+        u = np.ascontiguousarray(u, dtype=np.int32)
+        v = np.ascontiguousarray(v, dtype=np.int32)
+        w = np.ascontiguousarray(w, dtype=np.int32)
+        if not (u.shape == v.shape == w.shape):
+            raise ValueError("u, v, w must have the same length")
+        opt, _keep = _options(reduce_degree_two, True, False, num_threads, fvs_order, external_trees=True)
+        h = ctypes.c_void_p()
+        _lib.check_host(_lib.host().mcb_problem_build_edges(
+            int(n), int(u.size), u.ctypes.data, v.ctypes.data, w.ctypes.data, ctypes.byref(opt), ctypes.byref(h)))
+        return cls(h)
+
+    def root_search(self, r: int):
+        """The host's stopped search of root index ``r`` in the form the device search returns:
+        (accepted candidate edges, their weights, kept vertices as rows of (vertex, parent edge, level,
+        finalisation index))."""
+        nc = ctypes.c_int32()
+        nk = ctypes.c_int32()
+        host = _lib.host()
+        rc = host.mcb_problem_root_search(self._h, int(r), 0, None, None, 0, None, ctypes.byref(nc), ctypes.byref(nk))
+        if rc not in (0, -3):
+            _lib.check_host(rc)
+        ce = np.zeros(nc.value, dtype=np.uint32)
+        cw = np.zeros(nc.value, dtype=np.int32)
+        kept = np.zeros((nk.value, 4), dtype=np.uint32)
+        _lib.check_host(host.mcb_problem_root_search(self._h, int(r), ce.size, ce.ctypes.data, cw.ctypes.data,
+                                                     kept.shape[0], kept.ctypes.data, ctypes.byref(nc),
+                                                     ctypes.byref(nk)))
+        return ce, cw, kept
+
+    def finish_trees(self, status, cand_start, cand_count, cand_edge, cand_weight, kept_start, kept_count, kept,
+                     num_threads: int = 0) -> None:
+        """Completes a problem built by ``for_external_trees`` from per-root search results (roots whose
+        status is nonzero are searched by the host library)."""
+        arrs = [np.ascontiguousarray(status, dtype=np.int32), np.ascontiguousarray(cand_start, dtype=np.int64),
+                np.ascontiguousarray(cand_count, dtype=np.int32), np.ascontiguousarray(cand_edge, dtype=np.uint32),
+                np.ascontiguousarray(cand_weight, dtype=np.int32), np.ascontiguousarray(kept_start, dtype=np.int64),
+                np.ascontiguousarray(kept_count, dtype=np.int32), np.ascontiguousarray(kept, dtype=np.uint32)]
+        F = self.info.fvs_size
+        if any(a.size != F for a in (arrs[0], arrs[1], arrs[2], arrs[5], arrs[6])):
+            raise ValueError("per-root arrays must have one entry per root")
+        t = _lib.RootTrees(*[a.ctypes.data for a in arrs])
+        _lib.check_host(_lib.host().mcb_problem_finish_trees(self._h, ctypes.byref(t), int(num_threads)))
+        self._refresh()
+
+    def _refresh(self) -> None:
+        info = _lib.ProblemInfo()
+        _lib.check_host(_lib.host().mcb_problem_get_info(self._h, ctypes.byref(info)))
+        self.info = Info(**{f: getattr(info, f) for f, _ in _lib.ProblemInfo._fields_})
+        self.info.single_cycle = bool(self.info.single_cycle)
+        self._cache.clear()
 
     @classmethod
     def reduced_graph(cls, n: int, u, v, w, reduce_degree_two: bool = True) -> "Problem":
