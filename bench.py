@@ -6,10 +6,12 @@ candidate search, cycle vector, GF(2) support-vector update) over one synthetic 
 trees and candidates are already resident in HBM.  The default workload is the 100 000-point
 planar Delaunay graph, timed in two forms side by side: plain (the reference's FVS order) and
 learned (the FVS order of the learned vertex scorer); each form's host preprocessing and upload
-are timed once and reported with its end-to-end time.  Prints one JSON line.
+are timed once and reported with its end-to-end time.  The roots' shortest-path trees are searched on
+the GPU by default (``--trees host``: by the host library; the plain form also times that build and
+checks the arrays identical).  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--workload delaunay100k] [--fvs both|plain|learned]
-                    [--no-prune] [--no-cpu-baseline]
+                    [--trees device|host] [--no-prune] [--no-cpu-baseline]
 """
 from __future__ import annotations
 
@@ -154,6 +156,10 @@ def main() -> int:
     ap.add_argument("--graph", action="store_true",
                     help="also time the loop with no HIP events: stream-launched, and replayed as one HIP graph")
     ap.add_argument("--no-prune", action="store_true")
+    ap.add_argument("--trees", choices=["device", "host"], default="device",
+                    help="where the roots' shortest-path trees are searched (identical results)")
+    ap.add_argument("--no-host-trees", action="store_true",
+                    help="skip the comparison build with host-searched trees (plain form)")
     ap.add_argument("--breakdown", action="store_true",
                     help="time every kernel (HIP events between all launches), not only the dominant one")
     ap.add_argument("--no-timing", action="store_true",
@@ -208,12 +214,32 @@ def time_leg(form: str, graph, args, threads: int) -> dict:
         order = ch.order
         fvs_info = {"fvs_reference": ch.fvs_reference, "fvs_learned": ch.fvs_learned, "learned_used": ch.used,
                     "learned_order_s": round(time.perf_counter() - t0, 3)}
-    prob = Problem.from_edges(n, u, v, w, prune_trees=not args.no_prune, num_threads=threads, fvs_order=order)
+    trees = "host" if args.no_prune else args.trees
+    split = {}
+    if trees == "device":
+        from minimumcyclebasis_amd.device import problem_from_edges
+
+        prob = problem_from_edges(n, u, v, w, trees="device", num_threads=threads, fvs_order=order, timings=split)
+    else:
+        prob = Problem.from_edges(n, u, v, w, prune_trees=not args.no_prune, num_threads=threads, fvs_order=order)
     host_s = time.perf_counter() - t0
     info = prob.info
     print(f"[bench] {args.workload} {form}: n'={info.nodes} m'={info.directed_edges // 2} N={info.num_non_tree} "
-          f"F={info.fvs_size} K={info.num_candidates} positions={info.positions} host={host_s:.2f}s",
-          file=sys.stderr, flush=True)
+          f"F={info.fvs_size} K={info.num_candidates} positions={info.positions} preprocessing={host_s:.2f}s "
+          f"(trees on the {trees}) {split}", file=sys.stderr, flush=True)
+    host_trees_s = None
+    if trees == "device" and form == "plain" and not args.no_host_trees:
+        # the same problem with the trees searched by the host library, for comparison (arrays identical)
+        t1 = time.perf_counter()
+        ref = Problem.from_edges(n, u, v, w, prune_trees=True, num_threads=threads, fvs_order=order)
+        host_trees_s = time.perf_counter() - t1
+        same = all(np.array_equal(ref.array(a), prob.array(a)) for a in
+                   ("cand_edge", "cand_root", "cand_weight", "tree_offset", "parent_pos", "pos_coord"))
+        ref.close()
+        if not same:
+            raise RuntimeError("device and host shortest-path trees differ")
+        print(f"[bench] {form}: host-searched trees {host_trees_s:.2f}s, identical arrays", file=sys.stderr,
+              flush=True)
     t0 = time.perf_counter()
     timing = 0 if args.no_timing else 1 if args.breakdown else 2
     solver = Solver(prob, timing=timing, block_phases=args.block_phases, waves_per_block=args.tree_waves,
@@ -257,9 +283,18 @@ def time_leg(form: str, graph, args, threads: int) -> dict:
         sel = solver.selected()
         print(f"[bench] {form} step {i}: {dt:.3f} s", file=sys.stderr, flush=True)
     loop_s = t_total / args.steps
+    pre = {"trees": trees}
+    if split:
+        pre.update({k: round(v, 3) for k, v in split.items() if k != "search"})
+        pre["search_device_ms"] = round(split["search"]["device_ms"], 1)
+        pre["search_roots_per_stage"] = split["search"]["stage_roots"]
+        pre["search_roots_left_to_host"] = split["search"]["host_roots"]
+    if host_trees_s is not None:
+        pre["host_trees_preprocess_s"] = round(host_trees_s, 3)
     return {"form": form, "prob": prob, "solver": solver, "host_s": host_s, "upload_s": upload_s, "loop_s": loop_s,
             "kt": kt, "pos": pos, "cand": cand, "upd": upd, "sel": sel, "shape": solver.launch_shape(),
-            "fvs": fvs_info, "total_weight": int(prob.array("cand_weight")[sel].astype(np.int64).sum())}
+            "fvs": fvs_info, "total_weight": int(prob.array("cand_weight")[sel].astype(np.int64).sum()),
+            "pre": pre}
 
 
 def leg_summary(leg: dict) -> dict:
@@ -268,7 +303,7 @@ def leg_summary(leg: dict) -> dict:
            "host_preprocess_s": round(leg["host_s"], 3), "upload_s": round(leg["upload_s"], 3),
            "loop_s": round(leg["loop_s"], 6),
            "end_to_end_s": round(leg["host_s"] + leg["upload_s"] + leg["loop_s"], 3),
-           "total_weight": leg["total_weight"]}
+           "total_weight": leg["total_weight"], "preprocessing": leg["pre"]}
     if leg["fvs"]:
         out.update(leg["fvs"])
     return out
@@ -336,6 +371,7 @@ def run_single(args) -> int:
         },
         "total_weight": head["total_weight"],
         "host_preprocess_s": round(head["host_s"], 3),
+        "preprocessing": head["pre"],
         "upload_s": round(head["upload_s"], 3),
         "end_to_end_s": round(head["host_s"] + head["upload_s"] + loop_s, 3),
         "block_phases": shape["block_phases"],

@@ -310,8 +310,13 @@ int mcb_sptree_create(int32_t nodes, int32_t directed_edges, const uint32_t *row
                       const uint32_t *roots, int32_t num_roots, mcb_sptree **out);
 
 /* Searches every root, in up to `stages` (1..3) launches of growing LDS heap; a root whose heap outgrows
- * a stage is searched again in the next.  Blocks until the results are on the host. */
+ * a stage is searched again in the next, unless that stage finished under an eighth of its roots (then
+ * the rest are left to the host).  Blocks until the results are on the host. */
 int mcb_sptree_run(mcb_sptree *t, int stages);
+
+/* Waves per CU of each stage (count 1..3, each 1..16, not increasing; default 8, 2, 1): a stage's LDS heap
+ * holds the CU's LDS share of one wave at 12 bytes per entry.  A count below 3 repeats the last value. */
+int mcb_sptree_set_stages(mcb_sptree *t, const int32_t *waves_per_cu, int count);
 
 /* The results of the last run in the form mcb_problem_finish_trees takes (valid until the next run or
  * mcb_sptree_destroy); roots left to the host have status != 0. */

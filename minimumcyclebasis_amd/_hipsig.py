@@ -112,6 +112,7 @@ SIGNATURES = {
     "mcb_mlp_last_error": (ctypes.c_char_p, []),
     "mcb_sptree_create": (ctypes.c_int, [i32, i32, P, P, P, P, P, P, P, i32, ctypes.POINTER(P)]),
     "mcb_sptree_run": (ctypes.c_int, [P, ctypes.c_int]),
+    "mcb_sptree_set_stages": (ctypes.c_int, [P, P, ctypes.c_int]),
     "mcb_sptree_results": (ctypes.c_int, [P, P]),
     "mcb_sptree_get_stats": (ctypes.c_int, [P, ctypes.POINTER(SpTreeStats)]),
     "mcb_sptree_destroy": (None, [P]),

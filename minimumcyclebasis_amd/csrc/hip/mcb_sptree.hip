@@ -61,6 +61,7 @@ constexpr uint32_t kBlk = 2u;         // state.w bit 1: blocked (tentative until
 constexpr uint32_t kRootBit = 0x80000000u;  // colr: the far end is an FVS root
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr int kStages = 3;
+constexpr int kMaxWavesPerCu = 16;  // workspaces are made for this many waves per CU
 
 struct SpArgs {
     const uint32_t *ro;     // [n+1] row offsets
@@ -101,6 +102,9 @@ __device__ __forceinline__ int32_t unii(int32_t x) { return __builtin_amdgcn_rea
 __device__ __forceinline__ uint64_t uni64(uint64_t x) {
     return (static_cast<uint64_t>(uni(static_cast<uint32_t>(x >> 32))) << 32) | uni(static_cast<uint32_t>(x));
 }
+__device__ __forceinline__ uint32_t lane_u32(uint32_t x, int l) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), l));
+}
 // LDS written by one lane and read by another: the wave's LDS operations complete in order, so only
 // the compiler must not move them across this point.
 __device__ __forceinline__ void lds_order() {
@@ -114,10 +118,15 @@ __device__ __forceinline__ void mem_order() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// Heap entries: (distance, vertex) in hp, the vertex's row start in hr; comparisons use the distance
+// only.  The caller keeps the top entry in scalar registers (td, tv, tr): every step reads it, and both
+// operations know what they leave at index 0.
+
 // std::push_heap of (vd, vv, vr): the new element at index len sifts up while its parent's distance
 // is strictly larger.  Returns false (heap unchanged) when the heap is full.
-__device__ __forceinline__ bool heap_push(int32_t *hd, uint32_t *hv, uint32_t *hr, uint32_t &len, uint32_t cap,
-                                          int32_t vd, uint32_t vv, uint32_t vr, int lane) {
+__device__ __forceinline__ bool heap_push(uint2 *hp, uint32_t *hr, uint32_t &len, uint32_t cap, int32_t vd,
+                                          uint32_t vv, uint32_t vr, int32_t &td, uint32_t &tv, uint32_t &tr,
+                                          int lane) {
     if (len >= cap) return false;
     const uint32_t hole = len;
     len = hole + 1;
@@ -125,94 +134,118 @@ __device__ __forceinline__ bool heap_push(int32_t *hd, uint32_t *hv, uint32_t *h
     const uint32_t k = static_cast<uint32_t>(lane) + 1;                 // lane k-1 looks at ancestor k
     const bool act = k <= depth;
     const uint32_t ak = act ? ((hole + 1) >> k) - 1 : 0u;
-    const int32_t dk = hd[ak];
-    const uint32_t vk = hv[ak], rk = hr[ak];
-    const uint32_t K = static_cast<uint32_t>(__popcll(__ballot(act && dk > vd)));
+    const uint2 ek = hp[ak];
+    const uint32_t rk = hr[ak];
+    const uint32_t K = static_cast<uint32_t>(__popcll(__ballot(act && static_cast<int32_t>(ek.x) > vd)));
     lds_order();
     if (k <= K) {
         const uint32_t dst = ((hole + 1) >> (k - 1)) - 1;
-        hd[dst] = dk;
-        hv[dst] = vk;
+        hp[dst] = ek;
         hr[dst] = rk;
     }
+    const uint32_t fin = ((hole + 1) >> K) - 1;
     if (lane == 0) {
-        const uint32_t fin = ((hole + 1) >> K) - 1;
-        hd[fin] = vd;
-        hv[fin] = vv;
+        hp[fin] = make_uint2(static_cast<uint32_t>(vd), vv);
         hr[fin] = vr;
     }
     lds_order();
+    if (fin == 0) {
+        td = vd;
+        tv = vv;
+        tr = vr;
+    }
     return true;
 }
 
-// std::pop_heap without reading the top (the caller has it): with lp = len - 1 elements left, the hole
-// walks from the top while its index is below (lp - 1) / 2, each step into the right child unless that
-// one's distance is strictly larger than the left one's, then into a lone last child when lp is even;
-// the old last element then sifts up from the hole.  Path nodes p_0 = 0 .. p_L sit in lanes 0..L.
-__device__ __forceinline__ void heap_pop(int32_t *hd, uint32_t *hv, uint32_t *hr, uint32_t &len, int lane) {
+// std::pop_heap of the top (td, tv, tr): with lp = len - 1 elements left, the hole walks from the top
+// while its index is below (lp - 1) / 2, each step into the right child unless that one's distance is
+// strictly larger than the left one's, then into a lone last child when lp is even; the old last element
+// then sifts up from the hole.  The walk takes six levels per round: lane j loads both children of node j
+// of the six-level subtree under the round's first hole (the top subtree's addresses are fixed, so the
+// first round's loads go out with the last element's) and keeps the child it would step into; a ballot
+// of those choices gives the path, and one lane permute per round hands lane i the entry the hole takes
+// from below at path node p_i.  Leaves (td, tv, tr) the new top when elements remain.
+__device__ __forceinline__ void heap_pop(uint2 *hp, uint32_t *hr, uint32_t &len, int32_t &td, uint32_t &tv,
+                                         uint32_t &tr, int lane) {
     const uint32_t lp = len - 1;
     len = lp;
     if (lp == 0) return;
-    const int32_t vd = unii(hd[lp]);
-    const uint32_t vv = uni(hv[lp]), vr = uni(hr[lp]);
-    const uint32_t lim = (lp - 1) >> 1;
-    uint32_t hole = 0, L = 0, my_p = 0;
-    bool walking = lim > 0;
     const uint32_t j = static_cast<uint32_t>(lane);               // node j of a six-level subtree
     const uint32_t t = 31u - static_cast<uint32_t>(__clz(j + 1));  // its depth in the subtree
+    const uint32_t lim = (lp - 1) >> 1;
+    const uint2 last = hp[lp];
+    const uint32_t last_r = hr[lp];
+    uint32_t hole = 0, L = 0, my_p = 0;
+    uint2 nxt = make_uint2(0u, 0u);  // lane i < L: the entry below path node p_i that moves up into it
+    uint32_t nxt_r = 0;
+    bool walking = lim > 0;
     while (walking) {
         const uint32_t g = ((hole + 1) << t) - 1 + (j + 1 - (1u << t));
         const bool valid = j < 63 && g < lim;
         const uint32_t c2 = valid ? 2 * g + 2 : 0u;
-        const int32_t dr = hd[c2];
-        const int32_t dl = hd[valid ? c2 - 1 : 0u];
-        const uint64_t goes_left = __ballot(valid && dr > dl);
+        const uint2 er = hp[c2];
+        const uint2 el = hp[valid ? c2 - 1 : 0u];
+        const uint32_t rr = hr[c2];
+        const uint32_t rl = hr[valid ? c2 - 1 : 0u];
+        const bool left = valid && static_cast<int32_t>(er.x) > static_cast<int32_t>(el.x);
+        const uint64_t goes_left = __ballot(left);
         const uint64_t vmask = __ballot(valid);
-        uint32_t loc = 0;
+        const uint32_t L0 = L;
+        uint32_t loc = 0, src = 0;
         for (int s = 0; s < 6; ++s) {
             if (!((vmask >> loc) & 1ull)) {
                 walking = false;
                 break;
             }
             const uint32_t lb = static_cast<uint32_t>((goes_left >> loc) & 1ull);
+            src = (j == L) ? loc : src;
             hole = 2 * hole + 2 - lb;
             ++L;
             my_p = (j == L) ? hole : my_p;
             loc = 2 * loc + 2 - lb;
         }
+        const uint32_t cx = left ? el.x : er.x, cy = left ? el.y : er.y, cr = left ? rl : rr;
+        const uint32_t fx = __shfl(cx, static_cast<int>(src)), fy = __shfl(cy, static_cast<int>(src));
+        const uint32_t fr = __shfl(cr, static_cast<int>(src));
+        const bool mine = j >= L0 && j < L;
+        nxt = mine ? make_uint2(fx, fy) : nxt;
+        nxt_r = mine ? fr : nxt_r;
     }
     if ((lp & 1u) == 0 && hole == ((lp - 2) >> 1)) {
+        const uint2 lc = hp[2 * hole + 1];
+        const uint32_t lcr = hr[2 * hole + 1];
+        nxt = (j == L) ? lc : nxt;
+        nxt_r = (j == L) ? lcr : nxt_r;
         hole = 2 * hole + 1;
         ++L;
         my_p = (j == L) ? hole : my_p;
     }
-    const bool onp = j >= 1 && j <= L;
-    const uint32_t pi = onp ? my_p : 0u;
-    const int32_t od = hd[pi];
-    const uint32_t ov = hv[pi], orr = hr[pi];
-    const uint32_t K = static_cast<uint32_t>(__popcll(__ballot(onp && od > vd)));
-    const int32_t nd = __shfl_down(od, 1);
-    const uint32_t nv = __shfl_down(ov, 1), nr = __shfl_down(orr, 1);
-    lds_order();
+    const int32_t vd = static_cast<int32_t>(uni(last.x));
+    const uint32_t K = static_cast<uint32_t>(__popcll(__ballot(j < L && static_cast<int32_t>(nxt.x) > vd)));
     const uint32_t cut = L - K;
+    lds_order();
     if (j < cut) {
-        hd[my_p] = nd;
-        hv[my_p] = nv;
-        hr[my_p] = nr;
+        hp[my_p] = nxt;
+        hr[my_p] = nxt_r;
     } else if (j == cut) {
-        hd[my_p] = vd;
-        hv[my_p] = vv;
-        hr[my_p] = vr;
+        hp[my_p] = last;
+        hr[my_p] = last_r;
     }
     lds_order();
+    if (cut > 0) {
+        td = static_cast<int32_t>(lane_u32(nxt.x, 0));
+        tv = lane_u32(nxt.y, 0);
+        tr = lane_u32(nxt_r, 0);
+    } else {
+        td = vd;
+        tv = uni(last.y);
+        tr = uni(last_r);
+    }
 }
 
-__device__ __forceinline__ uint32_t lane_u32(uint32_t x, int l) {
-    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), l));
-}
 
 // One root: the stopped search, then its candidates and layout vertices.  Returns the status.
-__device__ int search_root(const SpArgs &a, uint32_t r, int32_t *hd, uint32_t *hv, uint32_t *hr, uint32_t *cnt,
+__device__ int search_root(const SpArgs &a, uint32_t r, uint2 *hp, uint32_t *hr, uint32_t *cnt,
                            uint4 *st, uint2 *ax, uint32_t *kp, uint32_t *ol, uint2 *cb, uint4 *kb, int lane) {
     const uint32_t src = uni(a.roots[r]);
     const uint32_t epoch = a.epoch_base + r + 1;
@@ -220,52 +253,78 @@ __device__ int search_root(const SpArgs &a, uint32_t r, int32_t *hd, uint32_t *h
     if (lane == 0) {
         st[src] = make_uint4(epoch, 0u, kNone, 0u);
         ax[src] = make_uint2(kNone, 0u);
-        hd[0] = 0;
-        hv[0] = src;
+        hp[0] = make_uint2(0u, src);
         hr[0] = a.ro[src];
     }
+    int32_t td = 0;
+    uint32_t tv = src, tr = uni(a.ro[src]);
     mem_order();
     lds_order();
     uint32_t len = 1, nfin = 0, nopen = 0;
     int32_t open = 1;
     bool full = false;
+    // The state and row of the next heap top are loaded with each step's neighbour loads (pf_*).  They
+    // stand for that vertex when it is popped next, with the writes of the step in between forwarded
+    // from the registers of the lanes that made them (fw_*): the prefetch was issued before them.
+    bool pf_ok = false;
+    uint32_t pf_v = kNone, pf_bits = 0, pf_br = 0, pf_rend = 0, pf_cr = 0, pf_rs = 0;
+    int32_t pf_w = 0;
+    uint64_t fw_mask = 0;
+    uint32_t fw_c = kNone, fw_bits = 0, fw_br = 0, fw_fin = kNone;
     while (len > 0) {
-        const uint32_t u = uni(hv[0]);
-        const int32_t du = unii(hd[0]);
-        const uint32_t urow = uni(hr[0]);
-        heap_pop(hd, hv, hr, len, lane);
-        mem_order();
-        // the vertex's state and its row in one round trip (the row loads are speculative)
-        const uint4 su4 = st[u];
-        const uint32_t bru = uni(ax[u].x);
-        const uint32_t rend = uni(a.ro[u + 1]);
-        const uint32_t off0 = min(urow + static_cast<uint32_t>(lane), a.m2 - 1);
-        const uint32_t cr0 = a.colr[off0];
-        const int32_t w0 = a.w[off0];
-        const uint32_t rs0 = a.rs[off0];
-        const uint32_t sbits = uni(su4.w);
+        const uint32_t u = tv;
+        const int32_t du = td;
+        const uint32_t urow = tr;
+        heap_pop(hp, hr, len, td, tv, tr, lane);
+        const uint32_t nu = len > 0 ? tv : kNone;  // the next top unless this step pushes above it
+        const uint32_t nrow = len > 0 ? tr : 0u;
+        uint32_t sbits, bru, rend, cr0, rs0;
+        int32_t w0;
+        if (pf_ok && pf_v == u) {
+            const uint64_t m = __ballot(((fw_mask >> lane) & 1ull) && fw_c == u);
+            if (u == fw_fin) {
+                sbits = kIn;
+                bru = 0;
+            } else if (m) {
+                const int l = __ffsll(static_cast<unsigned long long>(m)) - 1;
+                sbits = lane_u32(fw_bits, l);
+                bru = lane_u32(fw_br, l);
+            } else {
+                sbits = pf_bits;
+                bru = pf_br;
+            }
+            rend = pf_rend;
+            cr0 = pf_cr;
+            w0 = pf_w;
+            rs0 = pf_rs;
+        } else {
+            mem_order();
+            // the vertex's state and its row in one round trip (the row loads are speculative)
+            const uint4 su4 = st[u];
+            bru = uni(ax[u].x);
+            rend = uni(a.ro[u + 1]);
+            const uint32_t off0 = min(urow + static_cast<uint32_t>(lane), a.m2 - 1);
+            cr0 = a.colr[off0];
+            w0 = a.w[off0];
+            rs0 = a.rs[off0];
+            sbits = uni(su4.w);
+        }
+        pf_ok = false;
         if (sbits & kIn) continue;  // a stale entry (lazy deletion)
         const bool bu = (sbits & kBlk) != 0;
         const uint32_t lev = sbits >> 2;
-        if (lane == 0) {
-            reinterpret_cast<uint32_t *>(&st[u])[3] = sbits | kIn;
-            ax[u].y = nfin;
-            if (!bu) ol[nopen] = u;
-        }
-        ++nfin;
-        if (!bu) {
-            --open;
-            ++nopen;
-        }
         const uint32_t deg = rend - urow;
         const uint32_t child_bits = ((lev + 1) << 2);
+        const bool one_chunk = deg <= 64;
+        fw_mask = 0;
+        fw_fin = u;
         for (uint32_t base = 0; base < deg && !full; base += 64) {
             const bool act = base + static_cast<uint32_t>(lane) < deg;
             const uint32_t off = urow + base + static_cast<uint32_t>(lane);
             uint32_t cr = cr0, rsc = rs0;
             int32_t wt = w0;
+            mem_order();  // every earlier step's state writes are visible to the loads below
             if (base > 0) {
-                mem_order();
                 const uint32_t o = act ? off : urow;
                 cr = a.colr[o];
                 wt = a.w[o];
@@ -273,6 +332,25 @@ __device__ int search_root(const SpArgs &a, uint32_t r, int32_t *hd, uint32_t *h
             }
             const uint32_t c = act ? (cr & ~kRootBit) : u;
             const uint4 sc = st[c];
+            if (base == 0 && one_chunk && nu != kNone) {
+                const uint4 p4 = st[nu];
+                const uint32_t pbr = ax[nu].x;
+                const uint32_t pre = a.ro[nu + 1];
+                const uint32_t poff = min(nrow + static_cast<uint32_t>(lane), a.m2 - 1);
+                pf_cr = a.colr[poff];
+                pf_w = a.w[poff];
+                pf_rs = a.rs[poff];
+                pf_bits = uni(p4.w);
+                pf_br = uni(pbr);
+                pf_rend = uni(pre);
+                pf_v = nu;
+                pf_ok = true;
+            }
+            if (base == 0 && lane == 0) {  // finalise u (after the loads above: none of them reads it)
+                reinterpret_cast<uint32_t *>(&st[u])[3] = sbits | kIn;
+                ax[u].y = nfin;
+                if (!bu) ol[nopen] = u;
+            }
             const bool reached = act && sc.x == epoch;
             const bool live = act && c != u && !(reached && (sc.w & kIn));
             const int32_t cand = du + wt;
@@ -322,19 +400,30 @@ __device__ int search_root(const SpArgs &a, uint32_t r, int32_t *hd, uint32_t *h
                     }
                 }
             }
+            const uint32_t wbits = child_bits | (nb ? kBlk : 0u);
+            const uint32_t wbr = (u == src) ? c : bru;
             if ((writer >> lane) & 1ull) {
-                st[c] = make_uint4(epoch, static_cast<uint32_t>(cand), off, child_bits | (nb ? kBlk : 0u));
-                ax[c].x = (u == src) ? c : bru;
+                st[c] = make_uint4(epoch, static_cast<uint32_t>(cand), off, wbits);
+                ax[c].x = wbr;
             }
+            fw_mask = writer;
+            fw_c = c;
+            fw_bits = wbits;
+            fw_br = wbr;
             while (improve) {
                 const int l = __ffsll(static_cast<unsigned long long>(improve)) - 1;
                 improve &= improve - 1;
-                if (!heap_push(hd, hv, hr, len, H, static_cast<int32_t>(lane_u32(static_cast<uint32_t>(cand), l)),
-                               lane_u32(c, l), lane_u32(rsc, l), lane)) {
+                if (!heap_push(hp, hr, len, H, static_cast<int32_t>(lane_u32(static_cast<uint32_t>(cand), l)),
+                               lane_u32(c, l), lane_u32(rsc, l), td, tv, tr, lane)) {
                     full = true;
                     break;
                 }
             }
+        }
+        ++nfin;
+        if (!bu) {
+            --open;
+            ++nopen;
         }
         if (full) return 1;
         if (open == 0) break;
@@ -417,11 +506,10 @@ __device__ int search_root(const SpArgs &a, uint32_t r, int32_t *hd, uint32_t *h
 }
 
 __global__ void __launch_bounds__(64) k_sp_trees(SpArgs a) {
-    extern __shared__ uint32_t sp_lds[];
+    extern __shared__ __align__(16) uint32_t sp_lds[];
     __shared__ uint32_t cnt[2];
     const uint32_t H = a.heap_cap;
-    int32_t *hd = reinterpret_cast<int32_t *>(sp_lds);
-    uint32_t *hv = sp_lds + H;
+    uint2 *hp = reinterpret_cast<uint2 *>(sp_lds);
     uint32_t *hr = sp_lds + 2 * H;
     const int lane = static_cast<int>(threadIdx.x);
     const size_t slab = blockIdx.x;
@@ -437,7 +525,7 @@ __global__ void __launch_bounds__(64) k_sp_trees(SpArgs a) {
         wi = uni(__shfl(wi, 0));
         if (wi >= a.num_work) break;
         const uint32_t r = uni(a.work[wi]);
-        const int s = search_root(a, r, hd, hv, hr, cnt, st, ax, kp, ol, cb, kb, lane);
+        const int s = search_root(a, r, hp, hr, cnt, st, ax, kp, ol, cb, kb, lane);
         if (lane == 0) a.status[r] = s;
         mem_order();
     }
@@ -608,7 +696,7 @@ int mcb_sptree_create(int32_t nodes, int32_t directed_edges, const uint32_t *row
                               t->cand_stride * sizeof(uint2);
     size_t free_b = 0, total_b = 0;
     SP_TRY(hipMemGetInfo(&free_b, &total_b));
-    uint64_t slabs = std::min<uint64_t>(t->F, static_cast<uint64_t>(t->waves_per_cu[0]) * t->cus);
+    uint64_t slabs = std::min<uint64_t>(t->F, static_cast<uint64_t>(kMaxWavesPerCu) * t->cus);
     slabs = std::min<uint64_t>(slabs, std::max<uint64_t>(1, (free_b / 4) / per_slab));
     t->slabs = slabs;
     SP_TRY(dalloc(&t->state, slabs * n));
@@ -737,8 +825,12 @@ int mcb_sptree_run(mcb_sptree *t, int stages) {
             work = again;
             continue;
         }
+        // a stage that finished under an eighth of its roots says the rest need heaps far beyond LDS
+        // (an expander, where every root reaches the whole graph): leave them to the host at once
+        const size_t finished = work.size() - next_stage.size();
         work = next_stage;
         ++stage;
+        if (finished * 8 < finished + work.size()) break;
     }
     st.host_roots = static_cast<int64_t>(work.size());
     SP_TRY(hipEventRecord(e1, t->stream));
@@ -780,6 +872,21 @@ int mcb_sptree_run(mcb_sptree *t, int stages) {
     st.kept = static_cast<int64_t>(nk);
     t->stats = st;
     t->have_results = true;
+    return MCB_OK;
+}
+
+int mcb_sptree_set_stages(mcb_sptree *t, const int32_t *waves_per_cu, int count) {
+    if (!t || !waves_per_cu || count < 1 || count > kStages) return sfail(MCB_E_INVALID, "invalid arguments");
+    for (int i = 0; i < count; ++i) {
+        const int wpc = waves_per_cu[i];
+        if (wpc < 1 || wpc > kMaxWavesPerCu || (i > 0 && wpc > waves_per_cu[i - 1]))
+            return sfail(MCB_E_INVALID, "waves per CU must be 1..16 and not increase from stage to stage");
+        if (static_cast<uint64_t>(wpc) * t->cus > t->slabs && static_cast<uint64_t>(wpc) * t->cus < t->F)
+            return sfail(MCB_E_INVALID, "more waves than the workspaces made at create");
+        if (t->lds_per_cu / static_cast<uint32_t>(wpc) <= t->lds_static + 12u * 64u)
+            return sfail(MCB_E_INVALID, "too many waves per CU for a heap of 64 entries");
+    }
+    for (int i = 0; i < kStages; ++i) t->waves_per_cu[i] = waves_per_cu[std::min(i, count - 1)];
     return MCB_OK;
 }
 

@@ -387,6 +387,10 @@ class SpTreeSearch:
             raise _lib.MCBError(rc, _lib.hip().mcb_sptree_last_error().decode(errors="replace"))
         return rc
 
+    def set_stages(self, waves_per_cu) -> None:
+        a = np.ascontiguousarray(waves_per_cu, dtype=np.int32)
+        self._check(_lib.hip().mcb_sptree_set_stages(self._h, a.ctypes.data, int(a.size)))
+
     def run(self, stages: int = 3) -> None:
         self._check(_lib.hip().mcb_sptree_run(self._h, int(stages)))
 
@@ -487,12 +491,14 @@ class MCBResult:
 
 
 def minimum_cycle_basis(source, reduce_degree_two: bool = True, prune_trees: bool = True, num_threads: int = 0,
-                        fvs_order=None, block_phases: int = 0, fvs: str = "plain") -> MCBResult:
+                        fvs_order=None, block_phases: int = 0, fvs: str = "plain", trees: str = "device") -> MCBResult:
     """Minimum cycle basis weight and cycles of a graph given as a .mtx path or (n, u, v, w).  The result's
     ``stats`` is the run's row for the reference's stats CSV (``stats.append_stats_csv``).
 
     fvs="learned" orders the FVS pass by the learned vertex scorer (``learned_fvs``), falling back to the
-    reference order unless that gives a smaller verified FVS; the weight is the same either way."""
+    reference order unless that gives a smaller verified FVS; the weight is the same either way.
+    trees="device" searches the roots' shortest-path trees on the GPU (pruned layout, edge-list input),
+    "host" in the host library; the problem is identical either way."""
     import time
 
     t0 = time.perf_counter()
@@ -510,6 +516,10 @@ def minimum_cycle_basis(source, reduce_degree_two: bool = True, prune_trees: boo
     elif isinstance(source, (str, bytes)) or hasattr(source, "__fspath__"):
         prob = Problem.from_mtx(source, reduce_degree_two=reduce_degree_two, prune_trees=prune_trees,
                                 num_threads=num_threads, fvs_order=fvs_order)
+    elif prune_trees and trees == "device":
+        n, u, v, w = source
+        prob = problem_from_edges(n, u, v, w, trees="device", reduce_degree_two=reduce_degree_two,
+                                  num_threads=num_threads, fvs_order=fvs_order)
     else:
         n, u, v, w = source
         prob = Problem.from_edges(n, u, v, w, reduce_degree_two=reduce_degree_two, prune_trees=prune_trees,

@@ -96,15 +96,40 @@ def test_wheel_hub_rows_longer_than_a_wave():
     check_graph((n, u, v, w), "wheel700")
 
 
+def check_staged(gr, label, waves_per_cu, stages):
+    print(f"case {label} waves per CU {waves_per_cu} stages={stages}", flush=True)
+    ref = Problem.from_edges(*gr, prune_trees=True)
+    p = Problem.for_external_trees(*gr)
+    sp = SpTreeSearch(p)
+    sp.set_stages(waves_per_cu)
+    sp.run(stages)
+    st = sp.stats()
+    print(st, flush=True)
+    sp.finish(p)
+    sp.close()
+    arrays_equal(p, ref)
+    ref.close()
+    p.close()
+    return st
+
+
 def test_stages_and_host_fallback():
-    """Heaps beyond the first stage's capacity: with one stage they are left to the host, with three they
-    run on the device; both finish to the host-built arrays."""
+    """Delaunay 60 000 with a first stage of 16 waves per CU (heaps of about 850 entries): about one root in a
+    hundred outgrows it.  With one stage those go to the host library (the mixed hand-over), with two they run
+    in the second stage; both finish to the host-built arrays."""
+    gr = G.delaunay(60000, seed=3)
+    st1 = check_staged(gr, "delaunay60000 one stage", [16, 8, 2], 1)
+    assert 0 < st1["host_roots"] < st1["stage_roots"][0]
+    st2 = check_staged(gr, "delaunay60000 two stages", [16, 8, 2], 2)
+    assert st2["stage_roots"][1] > 0 and st2["host_roots"] == 0
+
+
+def test_expander_goes_to_the_host_after_one_stage():
+    """ER 6000 / 18 000: every root's heap outgrows LDS; the first stage finishes under an eighth of the roots,
+    so the rest go straight to the host library (no further launches)."""
     gr = G.erdos_renyi_lbc(6000, 18000, 1000, seed=2)
-    st1 = check_graph(gr, "er6000 one stage", stages=1, per_root=False)
-    st3 = check_graph(gr, "er6000 three stages", stages=3, per_root=False)
-    print(st1, st3, flush=True)
-    assert st1["host_roots"] > 0
-    assert st3["stage_roots"][1] + st3["stage_roots"][2] > 0
+    st = check_staged(gr, "er6000", [8, 2, 1], 3)
+    assert st["launches"] == 1 and st["host_roots"] * 8 > 7 * (st["host_roots"] + st["stage_roots"][0])
 
 
 def test_pools_grow(monkeypatch):
