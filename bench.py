@@ -579,16 +579,18 @@ def run_components(args) -> int:
             lambda: Batch(loop, waves_per_block=args.tree_waves, tree_cap=args.tree_cap), args.steps,
             lambda o: all(np.array_equal(o.selected(j), sels[j]) for j in range(len(loop))))
 
-    plain_s = None
+    plain_s = plain_upload_s = None
     if not args.no_plain:
-        # the same components, one solver at a time (creation untimed), as the per-component driver runs them
-        plain_s = 0.0
+        # the same components, one solver at a time, as the per-component driver runs them (creation timed apart)
+        plain_s = plain_upload_s = 0.0
         for j, p in enumerate(loop):
-            s_ = Solver(p)
             t0 = time.perf_counter()
+            s_ = Solver(p)
+            t1 = time.perf_counter()
             s_.run()
             s_.sync()
-            plain_s += time.perf_counter() - t0
+            plain_s += time.perf_counter() - t1
+            plain_upload_s += t1 - t0
             if not np.array_equal(s_.selected(), sels[j]):
                 print(f"[bench] component {j}: batched and plain selections differ", file=sys.stderr, flush=True)
                 return 1
@@ -634,6 +636,11 @@ def run_components(args) -> int:
         "tree_pass_lds": shape["lds_bytes"],
         "plain_loop_s": None if plain_s is None else round(plain_s, 6),
         "speedup_vs_plain": None if plain_s is None else round(plain_s / batched_s, 2),
+        "end_to_end_s": round(host_s + upload_s + batched_s, 3),
+        "plain_upload_s": None if plain_upload_s is None else round(plain_upload_s, 3),
+        "plain_end_to_end_s": None if plain_s is None else round(host_s + plain_upload_s + plain_s, 3),
+        "speedup_vs_plain_end_to_end": None if plain_s is None else
+        round((host_s + plain_upload_s + plain_s) / (host_s + upload_s + batched_s), 3),
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in kt.items() if v[1]},
         "update_bytes_per_step": upd // max(args.steps, 1),
         "roofline": {

@@ -30,6 +30,8 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -40,6 +42,7 @@
 #include <numeric>
 #include <queue>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mcb_hip.h"
@@ -3103,30 +3106,100 @@ int validate_desc(const mcb_solver_desc *d, int32_t *max_pos) {
 // DFS pre-order intervals of every tree layout: pre[x] is x's index in a pre-order walk of its
 // root's tree (children in position order) and end[x] = pre[x] + subtree size, so that x is an
 // ancestor-or-self of y exactly when pre[x] <= pre[y] < end[x].
-void preorder_intervals(const int64_t *tree_offset, const uint32_t *parent_pos, int32_t F, std::vector<uint32_t> &pre,
-                        std::vector<uint32_t> &end) {
-    const int64_t P = tree_offset[F];
-    pre.assign(P, 0);
-    end.assign(P, 0);
-    std::vector<uint32_t> size, next;
-    for (int32_t r = 0; r < F; ++r) {
-        const int64_t a = tree_offset[r];
-        const uint32_t n = static_cast<uint32_t>(tree_offset[r + 1] - a);
-        const uint32_t *pp = parent_pos + a;
-        size.assign(n, 1);
-        for (uint32_t x = n; x-- > 1;) size[pp[x]] += size[x];
-        next.assign(n, 0);
-        uint32_t *pr = pre.data() + a, *en = end.data() + a;
-        pr[0] = 0;
-        next[0] = 1;
-        for (uint32_t x = 1; x < n; ++x) {  // parents precede children
-            const uint32_t p = pp[x];
-            pr[x] = next[p];
-            next[p] += size[x];
-            next[x] = pr[x] + 1;
-        }
-        for (uint32_t x = 0; x < n; ++x) en[x] = pr[x] + size[x];
+// Host threads for the set-up loops: OMP_NUM_THREADS when set, else the hardware's, at most 16.
+int host_threads() {
+    int t = static_cast<int>(std::thread::hardware_concurrency());
+    if (const char *env = std::getenv("OMP_NUM_THREADS")) t = std::atoi(env);
+    return std::max(1, std::min(16, t));
+}
+
+// f(i) for every i in [0, n), items dealt in chunks of `grain` to the host threads.
+template <typename Fn>
+void parallel_for(int64_t n, int64_t grain, Fn f) {
+    const int T = host_threads();
+    if (T <= 1 || n <= grain) {
+        for (int64_t i = 0; i < n; ++i) f(i);
+        return;
     }
+    std::atomic<int64_t> next{0};
+    auto work = [&]() {
+        for (;;) {
+            const int64_t i0 = next.fetch_add(grain);
+            if (i0 >= n) return;
+            const int64_t i1 = std::min(n, i0 + grain);
+            for (int64_t i = i0; i < i1; ++i) f(i);
+        }
+    };
+    std::vector<std::thread> pool;
+    const int64_t want = std::min<int64_t>(T, (n + grain - 1) / grain);
+    for (int64_t t = 1; t < want; ++t) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+}
+
+// An allocator that leaves trivially constructible elements uninitialised on resize: the arena arrays are
+// written in full by the assembly, so zero-filling gigabytes on one thread first is wasted.
+template <typename T>
+struct NoInit : std::allocator<T> {
+    template <typename U>
+    struct rebind {
+        using other = NoInit<U>;
+    };
+    NoInit() = default;
+    template <typename U>
+    NoInit(const NoInit<U> &) noexcept {}
+    template <typename U>
+    void construct(U *p) noexcept {
+        ::new (static_cast<void *>(p)) U;
+    }
+    template <typename U, typename... A>
+    void construct(U *p, A &&...args) {
+        ::new (static_cast<void *>(p)) U(std::forward<A>(args)...);
+    }
+};
+template <typename T>
+using RawVec = std::vector<T, NoInit<T>>;
+
+// Set-up stage times on stderr when MCB_HIP_TIMING is set (seconds since the previous mark).
+struct SetupClock {
+    bool on = std::getenv("MCB_HIP_TIMING") != nullptr;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    void mark(const char *stage) {
+        if (!on) return;
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[mcb_hip] %-28s %8.3f s\n", stage, std::chrono::duration<double>(now - t).count());
+        t = now;
+    }
+};
+
+template <typename Vec>
+void preorder_intervals(const int64_t *tree_offset, const uint32_t *parent_pos, int32_t F, Vec &pre, Vec &end) {
+    const int64_t P = tree_offset[F];
+    pre.resize(P);
+    end.resize(P);
+    // roots in chunks of 64, each thread with its own scratch
+    parallel_for((F + 63) / 64, 1, [&](int64_t chunk) {
+        thread_local std::vector<uint32_t> size, next;
+        const int32_t r1 = static_cast<int32_t>(std::min<int64_t>(F, 64 * (chunk + 1)));
+        for (int32_t r = static_cast<int32_t>(64 * chunk); r < r1; ++r) {
+            const int64_t a = tree_offset[r];
+            const uint32_t n = static_cast<uint32_t>(tree_offset[r + 1] - a);
+            const uint32_t *pp = parent_pos + a;
+            size.assign(n, 1);
+            for (uint32_t x = n; x-- > 1;) size[pp[x]] += size[x];
+            next.assign(n, 0);
+            uint32_t *pr = pre.data() + a, *en = end.data() + a;
+            pr[0] = 0;
+            next[0] = 1;
+            for (uint32_t x = 1; x < n; ++x) {  // parents precede children
+                const uint32_t p = pp[x];
+                pr[x] = next[p];
+                next[p] += size[x];
+                next[x] = pr[x] + 1;
+            }
+            for (uint32_t x = 0; x < n; ++x) en[x] = pr[x] + size[x];
+        }
+    });
 }
 
 // Lets `kernel` take `bytes` of dynamic LDS, beyond the default 64 KB when a workgroup's LDS allows
@@ -4074,8 +4147,8 @@ int bat_alloc(mcb_batch *b, T **p, size_t n) {
     return rc;
 }
 
-template <typename T>
-int bat_upload(mcb_batch *b, T **p, const std::vector<T> &v) {
+template <typename T, typename A>
+int bat_upload(mcb_batch *b, T **p, const std::vector<T, A> &v) {
     int rc = upload(p, v.data(), v.size(), b->bytes);
     if (!rc) b->allocs.push_back(*p);
     return rc;
@@ -4254,12 +4327,20 @@ int mcb_batch_reset(mcb_batch *b) {
 
 int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solver_options *opt, mcb_batch **out) {
     if (!descs || count <= 0 || !out) return fail(MCB_E_INVALID, "invalid arguments");
+    SetupClock clock;
     *out = nullptr;
     int32_t largest = 0;
-    for (int32_t i = 0; i < count; ++i) {
-        int32_t mp = 0;
-        if (validate_desc(&descs[i], &mp)) return fail(MCB_E_INVALID, "problem " + std::to_string(i) + ": " + g_err);
-        largest = std::max(largest, mp);
+    {
+        // every problem checked on the host threads; the first failure (lowest index) is reported
+        std::vector<int32_t> mp(count, 0);
+        std::vector<std::string> why(count);
+        parallel_for(count, 1, [&](int64_t i) {
+            if (validate_desc(&descs[i], &mp[i])) why[i] = g_err.empty() ? std::string("invalid") : g_err;
+        });
+        for (int32_t i = 0; i < count; ++i) {
+            if (!why[i].empty()) return fail(MCB_E_INVALID, "problem " + std::to_string(i) + ": " + why[i]);
+            largest = std::max(largest, mp[i]);
+        }
     }
     if (opt && (opt->merge_group < 0 || opt->merge_group == 1 || opt->merge_group > 64))
         return fail(MCB_E_INVALID, "merge_group must be 0 or 2..64");
@@ -4353,12 +4434,14 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
 
     // Arena arrays: trees and candidates, problem after problem in arena order, coordinates and ranks
     // shifted into each problem's range.
-    std::vector<int64_t> tree_off(F_tot + 1), rc_off(F_tot + 1);
-    std::vector<uint32_t> parent_pos(P_tot), rc_pu(K_tot), rc_pv(K_tot), rc_rank(K_tot), cand_root(rank_tot, 0),
-        cand_pu(rank_tot, 0), cand_pv(rank_tot, 0), root_comp(F_tot);
-    std::vector<int32_t> pos_coord(P_tot), rc_coord(K_tot), cand_coord(rank_tot, -1);
+    RawVec<int64_t> tree_off(F_tot + 1), rc_off(F_tot + 1);
+    RawVec<uint32_t> parent_pos(P_tot), rc_pu(K_tot), rc_pv(K_tot), rc_rank(K_tot), cand_root(rank_tot),
+        cand_pu(rank_tot), cand_pv(rank_tot), root_comp(F_tot);
+    RawVec<int32_t> pos_coord(P_tot), rc_coord(K_tot), cand_coord(rank_tot);
     std::vector<int64_t> ranges(G + 1, F_tot);
-    for (int32_t g = 0; g < G; ++g) {
+    clock.mark("bases and tables");
+    parallel_for(G, 1, [&](int64_t gi) {
+        const int32_t g = static_cast<int32_t>(gi);
         const mcb_solver_desc &d = descs[b->order[g]];
         const CompInfo &c = b->comps[g];
         const int64_t R = root_base[g], PB = pos_base[g], KB = cand_base[g], rb = c.rb, cb = c.cb;
@@ -4377,6 +4460,12 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
             cand_pv[rb + k] = d.cand_pos_v[k];
             cand_coord[rb + k] = d.cand_coord[k] >= 0 ? static_cast<int32_t>(cb + d.cand_coord[k]) : -1;
         }
+        // the padding of the problem's rank range (up to the next range, a multiple of 64) holds no candidate
+        const int64_t rank_end = g + 1 < G ? b->comps[g + 1].rb : rank_tot;
+        for (int64_t k = rb + d.num_candidates; k < rank_end; ++k) {
+            cand_root[k] = cand_pu[k] = cand_pv[k] = 0;
+            cand_coord[k] = -1;
+        }
         for (int64_t i = 0; i < d.num_candidates; ++i) {
             const uint32_t k = d.root_cand_rank[i];
             rc_pu[KB + i] = d.cand_pos_u[k];
@@ -4385,17 +4474,20 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
             rc_coord[KB + i] = d.cand_coord[k] >= 0 ? static_cast<int32_t>(cb + d.cand_coord[k]) : -1;
         }
         ranges[g] = R;
-    }
+    });
     tree_off[F_tot] = P_tot;
     rc_off[F_tot] = K_tot;
-    std::vector<uint32_t> pre, end;
+    clock.mark("arena assembly");
+    RawVec<uint32_t> pre, end;
     preorder_intervals(tree_off.data(), parent_pos.data(), static_cast<int32_t>(F_tot), pre, end);
+    clock.mark("preorder intervals");
 
     if ((rc = make_plans(b->opt.waves_per_block, b->opt.tree_cap, kXcds, tree_off.data(), parent_pos.data(),
                          rc_off.data(), ranges,
                          reinterpret_cast<const void *>(k_batch_q), b->plans, b->bytes, &b->q_scratch)))
         return cleanup(rc);
     if (b->plans.empty()) return cleanup(fail(MCB_E_INVALID, "a tree is too large for the blocked form's LDS budget"));
+    clock.mark("tree pass plans");
 
     // Static launch tables: first-level chunks, merge groups per level, pairs workgroups.
     std::vector<ChunkInfo> chunks;
@@ -4461,6 +4553,7 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
     }
 
     std::vector<CompInfo> comps = b->comps;
+    clock.mark("launch tables");
     if ((rc = bat_upload(b, &b->tree_off, tree_off)) || (rc = bat_upload(b, &b->parent_pos, parent_pos)) ||
         (rc = bat_upload(b, &b->pos_coord, pos_coord)) || (rc = bat_upload(b, &b->pos_pre, pre)) ||
         (rc = bat_upload(b, &b->pos_end, end)) || (rc = bat_upload(b, &b->rc_off, rc_off)) || (rc = bat_upload(b, &b->rc_pu, rc_pu)) ||
@@ -4487,6 +4580,7 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
         (rc = bat_alloc(b, &b->upd_bytes, kByteSlots)))
         return cleanup(rc);
     if (b->opt.store_cycles && (rc = bat_alloc(b, &b->C_store, store))) return cleanup(rc);
+    clock.mark("uploads and allocations");
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(MCB_E_DEVICE, "hipStreamCreate"));
     if (b->opt.timing) {
@@ -4513,6 +4607,7 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
             });
         if (rc || (rc = mcb_batch_reset(b))) return cleanup(rc);
     }
+    clock.mark("reset and plan timing");
     *out = b;
     return MCB_OK;
 }
