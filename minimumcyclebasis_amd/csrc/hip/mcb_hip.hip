@@ -120,15 +120,18 @@ __device__ __forceinline__ uint32_t bit_of(const uint64_t *s, int32_t c) {
 // Parents and coordinates of kUnroll windows, loaded raw (clamped address, no select): the
 // values stay in flight until the group is processed.  Buffers alternate between two groups so
 // that no register holding an in-flight load is copied (a copy would wait for the load).
-struct WindowGroup {
-    uint32_t par[kUnroll];
-    int32_t co[kUnroll];
+template <int U>
+struct WindowGroupT {
+    uint32_t par[U];
+    int32_t co[U];
 };
+using WindowGroup = WindowGroupT<kUnroll>;
 
-__device__ __forceinline__ void load_group(WindowGroup &g, const uint32_t *pp, const int32_t *pc, uint32_t npos,
+template <int U>
+__device__ __forceinline__ void load_group(WindowGroupT<U> &g, const uint32_t *pp, const int32_t *pc, uint32_t npos,
                                            uint32_t first_pos, uint32_t stride) {
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
+    for (int u = 0; u < U; ++u) {
         const uint32_t pos = first_pos + u * stride;
         const uint32_t at = pos < npos ? pos : 0u;
         g.par[u] = pp[at];
@@ -930,6 +933,131 @@ __device__ __forceinline__ void tree_words(const uint32_t *pp, const int32_t *pc
     }
 }
 
+// Pointer jumping as in settle_words, carrying each lane's anchor: a lane whose jump lands on a settled
+// ancestor takes that ancestor's anchor along with its value (both final, being from before the round).
+__device__ __forceinline__ uint64_t settle_anchored(uint64_t val, uint32_t &anchor, bool pending, uint32_t p,
+                                                    uint32_t wb, int lane) {
+    int anc = pending ? static_cast<int>(p - wb) : -1;
+    while (__ballot(anc >= 0)) {
+        const int src = anc >= 0 ? anc : lane;
+        const uint64_t av = __shfl(val, src);
+        const int aa = __shfl(anc, src);
+        const uint32_t an = static_cast<uint32_t>(__shfl(static_cast<int>(anchor), src));
+        if (anc >= 0) {
+            val ^= av;
+            anc = aa;
+            if (aa < 0) anchor = an;
+        }
+    }
+    return val;
+}
+
+// One group of a segment walk (positions [s0, s1) of a narrow tree, one wave): Wd[q] becomes the XOR of
+// the Tt words on the path from q up to its first ancestor before s0, excluded, and anch[q] that
+// ancestor (kNoParent when the path reaches the root inside the segment).  rv/ra: the wave's ring of its
+// latest rmask + 1 positions (words and anchors) in LDS.
+constexpr int kSegUnroll = 4;  // windows per prefetched group of a segment walk
+using SegWindows = WindowGroupT<kSegUnroll>;
+
+__device__ __forceinline__ void segment_group(const SegWindows &g, uint32_t base, uint32_t s0, uint32_t s1,
+                                              const uint64_t *Tt, uint64_t *Wd, uint32_t *anch, uint64_t *rv,
+                                              uint32_t *ra, uint32_t rmask, int lane) {
+    uint64_t t[kSegUnroll];
+#pragma unroll
+    for (int u = 0; u < kSegUnroll; ++u) t[u] = Tt[g.co[u] >= 0 ? g.co[u] : 0];
+#pragma unroll
+    for (int u = 0; u < kSegUnroll; ++u) {
+        const uint32_t wb = base + u * kWave;
+        if (wb >= s1) break;
+        const bool v = wb + lane < s1;
+        const uint32_t p = v ? g.par[u] : kNoParent;
+        const uint64_t tv = (v && g.co[u] >= 0) ? t[u] : 0ull;
+        const bool outside = p == kNoParent || p < s0;  // the path leaves the segment at p (or ends at the root)
+        const bool before = !outside && p < wb;
+        const bool near = before && p + rmask + 1 >= wb + kWave;
+        const bool far = before && !near;
+        // a parent older than the ring: this wave's own earlier stores must be complete before it is read
+        if (__ballot(far)) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        }
+        uint64_t pw = 0;
+        uint32_t an = p;
+        if (near) {
+            pw = rv[p & rmask];
+            an = ra[p & rmask];
+        } else if (far) {
+            pw = Wd[p];
+            an = anch[p];
+        }
+        uint64_t val = tv ^ pw;
+        const bool inwin = v && !outside && p >= wb;
+        if (__ballot(inwin)) val = settle_anchored(val, an, inwin, p, wb, lane);
+        if (v) {
+            const uint32_t q = wb + lane;
+            Wd[q] = val;
+            anch[q] = an;
+            rv[q & rmask] = val;
+            ra[q & rmask] = an;
+        }
+    }
+}
+
+// A narrow tree in scratch walked by all nw waves of the workgroup: wave s walks the positions
+// [s·seg, (s+1)·seg) relative to the segment (segment_group); then the segments after the first are
+// resolved in order, every thread taking positions of one segment at a time: the word of q is its
+// relative word XOR the final word of its anchor, which lies in an earlier segment.  The serial chain
+// is one segment's windows plus nw − 1 parallel passes, instead of all the tree's windows.
+__device__ void tree_words_segmented(const uint32_t *pp, const int32_t *pc, uint32_t npos, const uint64_t *Tt,
+                                     uint64_t *Wd, uint32_t *anch, uint64_t *ring_base, uint32_t rmask, int wid,
+                                     int nw, int lane) {
+    const uint32_t seg = ((npos + nw - 1) / nw + kWave - 1) / kWave * kWave;
+    const uint32_t s0 = static_cast<uint32_t>(wid) * seg, s1 = min(npos, s0 + seg);
+    uint64_t *rv = ring_base + static_cast<size_t>(wid) * (rmask + 1);
+    uint32_t *ra = reinterpret_cast<uint32_t *>(ring_base + static_cast<size_t>(nw) * (rmask + 1)) +
+                   static_cast<size_t>(wid) * (rmask + 1);
+    if (s0 < s1) {
+        constexpr uint32_t G = kWave * kSegUnroll;
+        SegWindows A, B;
+        load_group(A, pp, pc, s1, s0 + lane, kWave);
+        for (uint32_t base = s0; base < s1; base += 2 * G) {
+            load_group(B, pp, pc, s1, base + G + lane, kWave);
+            segment_group(A, base, s0, s1, Tt, Wd, anch, rv, ra, rmask, lane);
+            if (base + G >= s1) break;
+            load_group(A, pp, pc, s1, base + 2 * G + lane, kWave);
+            segment_group(B, base + G, s0, s1, Tt, Wd, anch, rv, ra, rmask, lane);
+        }
+    }
+    __syncthreads();
+    for (int sg = 1; sg < nw; ++sg) {
+        const uint32_t a0 = static_cast<uint32_t>(sg) * seg;
+        if (a0 >= npos) break;
+        const uint32_t a1 = min(npos, a0 + seg);
+        constexpr int kR = 4;
+        for (uint32_t q0 = a0 + threadIdx.x; q0 < a1; q0 += kR * blockDim.x) {
+            uint32_t an[kR];
+            uint64_t w[kR], aw[kR];
+#pragma unroll
+            for (int k = 0; k < kR; ++k) {
+                const uint32_t q = q0 + k * blockDim.x;
+                an[k] = anch[q < a1 ? q : a0];
+            }
+#pragma unroll
+            for (int k = 0; k < kR; ++k) {
+                const uint32_t q = q0 + k * blockDim.x;
+                w[k] = Wd[q < a1 ? q : a0];
+                aw[k] = Wd[an[k] != kNoParent ? an[k] : 0u];
+            }
+#pragma unroll
+            for (int k = 0; k < kR; ++k) {
+                const uint32_t q = q0 + k * blockDim.x;
+                if (q < a1 && an[k] != kNoParent) Wd[q] = w[k] ^ aw[k];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // The same for a big tree shared by all waves of the workgroup: window w belongs to wave w mod nw
 // and is published with a ready stamp (this root's generation) once its words are in LDS.  A wave
 // waits only for lower windows, and the lowest unfinished window never waits, so every wait ends.
@@ -1100,6 +1228,13 @@ __device__ __forceinline__ void tree_pass(const A &a, const ItemMeta *big, uint3
     // region, if it holds at least two windows (the ready stamps of a narrow tree are not used)
     uint32_t ring_mask = 0;
     if (a.big_words >= 2 * kWave) ring_mask = (1u << (31 - __builtin_clz(static_cast<uint32_t>(a.big_words)))) - 1;
+    // per-wave rings of a segmented walk (12 bytes per position, words then anchors) in the same LDS tree
+    // region: the largest power of two of positions that fits nw times, if it holds two windows
+    uint32_t seg_mask = 0;
+    if (nw >= 2) {
+        const uint32_t per = static_cast<uint32_t>(a.big_words) * 8u / static_cast<uint32_t>(nw) / 12u;
+        if (per >= 2 * kWave) seg_mask = (1u << (31 - __builtin_clz(per))) - 1;
+    }
     for (uint32_t i = threadIdx.x; i < nwin_big; i += blockDim.x) ready[i] = 0;
     __syncthreads();
     uint32_t gen = 0;
@@ -1121,7 +1256,13 @@ __device__ __forceinline__ void tree_pass(const A &a, const ItemMeta *big, uint3
                 // it alone (no handoffs between waves); the others wait at the barrier.  A tree in
                 // scratch keeps its latest windows in the idle LDS tree region as well (ring), so
                 // that the walk does not wait on global memory for every window.
-                if (wid == 0) {
+                if (in_scratch && seg_mask) {
+                    // all waves, a segment each, then the segments resolved in order
+                    uint32_t *anch = reinterpret_cast<uint32_t *>(a.scratch + static_cast<int64_t>(gridDim.x) * a.huge_words) +
+                                     static_cast<int64_t>(blockIdx.x) * a.huge_words;
+                    tree_words_segmented(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wd, anch, Wbig, seg_mask,
+                                         wid, nw, lane);
+                } else if (wid == 0) {
                     if (in_scratch && ring_mask) {
                         tree_words<true>(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wd, lane, Wbig, ring_mask);
                     } else {
@@ -1158,7 +1299,7 @@ __device__ __forceinline__ void tree_pass(const A &a, const ItemMeta *big, uint3
     }
 }
 
-__global__ __launch_bounds__(kQWaves * kWave) void k_block_q(BlockArgs a) {
+__global__ __launch_bounds__(kQWaves * kWave, 5) void k_block_q(BlockArgs a) {
     const uint32_t nw = blockDim.x / kWave;
     const uint32_t c = blockIdx.x * nw + static_cast<uint32_t>(wave_id());
     tree_pass(a, a.big_list, static_cast<uint32_t>(a.wg_big[blockIdx.x]),
@@ -2270,7 +2411,7 @@ __global__ __launch_bounds__(256) void k_batch_transpose(BatchArgs a) {
 
 // Workgroups run on the XCD numbered blockIdx.x mod kXcds (round-robin dispatch); problem g's trees
 // are listed in section g mod kXcds, so that each problem's Tt words are gathered through one XCD's L2.
-__global__ __launch_bounds__(kQWaves * kWave) void k_batch_q(BatchArgs a) {
+__global__ __launch_bounds__(kQWaves * kWave, 5) void k_batch_q(BatchArgs a) {
     const uint32_t nw = blockDim.x / kWave;
     // blocks x, x + kXcds, ... serve section x (the grid is a multiple of kXcds)
     const uint32_t x = blockIdx.x % kXcds, local = blockIdx.x / kXcds, nloc = gridDim.x / kXcds;
@@ -3493,7 +3634,9 @@ int make_plans(int want, int want_cap, int nsplit, const int64_t *tree_off, cons
         }
     }
     int64_t words = 0;
-    for (const TreePlan &p : plans) words = std::max<int64_t>(words, static_cast<int64_t>(p.blocks) * p.huge_words);
+    // every workgroup's path words, then every workgroup's anchors (u32) for the segmented walk
+    for (const TreePlan &p : plans)
+        words = std::max<int64_t>(words, static_cast<int64_t>(p.blocks) * p.huge_words * 3 / 2 + 1);
     *scratch = nullptr;
     if (words) {
         int rc = dev_alloc(scratch, static_cast<size_t>(words), bytes);
