@@ -180,6 +180,25 @@ int mcb_vertex_features(int32_t nodes, const uint32_t *row_offsets, const uint32
  * vertices lie on no cycle of the alive subgraph).  Returns the number still alive or <0. */
 int mcb_fvs_peel(int32_t nodes, const uint32_t *row_offsets, const uint32_t *cols, uint8_t *alive);
 
+/* The learned construction's rounds in one native state (no counterpart in the reference): the vertices
+ * left are those of the 2-core (mcb_fvs_peel) of the graph minus the picks so far.  Each round reads the
+ * feature rows of the vertices left (mcb_vertex_features of that subgraph, rows in ascending vertex id,
+ * with the ids), scores them, and takes the k best (higher score first, NaN last, ties to the lower id);
+ * the picks are removed and the rest peeled again.  A round costs time in the size of what is left
+ * (the state compacts itself), not in the size of the graph. */
+typedef struct mcb_fvs_rounds mcb_fvs_rounds;
+int mcb_fvs_rounds_create(int32_t nodes, const uint32_t *row_offsets, const uint32_t *cols, const int32_t *weights,
+                          mcb_fvs_rounds **out);
+/* Number of vertices left. */
+int32_t mcb_fvs_rounds_left(const mcb_fvs_rounds *h);
+/* Writes the ids (ascending) and 4-float feature rows of the vertices left; cap is the row capacity.
+ * Returns the number of rows or <0. */
+int32_t mcb_fvs_rounds_features(mcb_fvs_rounds *h, int32_t cap, int32_t *ids, float *rows);
+/* Takes the k best of the last feature rows by score (count = that call's row count), writes their ids
+ * best first to picks[0..min(k, count)), removes them and peels.  Returns the number left or <0. */
+int32_t mcb_fvs_rounds_take_top(mcb_fvs_rounds *h, const float *scores, int32_t count, int32_t k, int32_t *picks);
+void mcb_fvs_rounds_free(mcb_fvs_rounds *h);
+
 /* A smaller FVS than the reference order's, used as the training target of the vertex scorer: vertices
  * left with at most one edge are peeled away, then the vertex of largest remaining degree (lowest id
  * on ties) is taken and peeled, until nothing is left; the picks are then made minimal by the

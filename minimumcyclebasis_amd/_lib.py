@@ -118,6 +118,15 @@ def host() -> ctypes.CDLL:
         lib.mcb_vertex_features.argtypes = [ctypes.c_int32, P, P, P, P, P]
         lib.mcb_fvs_peel.argtypes = [ctypes.c_int32, P, P, P]
         lib.mcb_fvs_greedy.argtypes = [ctypes.c_int32, ctypes.c_int32, P, P, P]
+        lib.mcb_fvs_rounds_create.argtypes = [ctypes.c_int32, P, P, P, ctypes.POINTER(P)]
+        lib.mcb_fvs_rounds_left.argtypes = [P]
+        lib.mcb_fvs_rounds_left.restype = ctypes.c_int32
+        lib.mcb_fvs_rounds_features.argtypes = [P, ctypes.c_int32, P, P]
+        lib.mcb_fvs_rounds_features.restype = ctypes.c_int32
+        lib.mcb_fvs_rounds_take_top.argtypes = [P, P, ctypes.c_int32, ctypes.c_int32, P]
+        lib.mcb_fvs_rounds_take_top.restype = ctypes.c_int32
+        lib.mcb_fvs_rounds_free.argtypes = [P]
+        lib.mcb_fvs_rounds_free.restype = None
         _host = lib
     return _host
 

@@ -3505,12 +3505,15 @@ int plan_tree_pass(int nw, int64_t cap_words, int nsplit, const int64_t *tree_of
     }
     std::vector<int32_t> wg_big, wave_small;
     if (nsplit == 1) {
-        // One problem: longest-processing-time assignment under a cost model (ns of one wave, from
-        // timings of this pass: a tree position 16, a candidate 35, an item 300).  A coop tree's work
-        // is shared by the nw waves; a narrow tree is walked by one wave while the others wait, so
-        // it occupies the whole workgroup.  Big items go, costliest first, to the least loaded
-        // workgroup; then small items to the least loaded wave, each wave starting from its
-        // workgroup's big-item time.
+        // One problem: longest-processing-time assignment under a cost model (ns of one wave, fitted
+        // to per-item timestamps of this pass on Delaunay graphs): a small item 2700 + 8.8 per tree
+        // position + 17 per candidate; a narrow tree walked by one wave while the others wait
+        // (occupying the whole workgroup) 10.7 per position in LDS, 6.8 per position in scratch,
+        // where every wave walks a segment, then its candidate run 2500 + 18.4 per candidate over
+        // the nw waves; a wide (coop) tree's walk and run shared by the nw waves.  Most items are
+        // small, so the per-item term dominates their cost.  Big items go, costliest first, to the
+        // least loaded workgroup; then small items to the least loaded wave, each wave starting from
+        // its workgroup's big-item time.
         const int nwv = nw;
         auto ncand = [&](uint32_t it) { return static_cast<double>(item_ce[it] - item_cb[it]); };
         auto npos_of = [&](uint32_t it) {
@@ -3519,8 +3522,9 @@ int plan_tree_pass(int nw, int64_t cap_words, int nsplit, const int64_t *tree_of
         std::vector<double> bcost(big_all.size());
         for (size_t i = 0; i < big_all.size(); ++i) {
             const uint32_t it = big_all[i];
-            bcost[i] = narrow_all[i] ? 9.0 * npos_of(it) + 35.0 * ncand(it) / nwv + 300.0
-                                     : (16.0 * npos_of(it) + 35.0 * ncand(it)) / nwv + 300.0;
+            const double np = npos_of(it);
+            bcost[i] = narrow_all[i] ? (np > big_words ? 6.8 : 10.7) * np + 2500.0 + 18.4 * ncand(it) / nwv
+                                     : 2700.0 + (8.8 * np + 17.0 * ncand(it)) / nwv;
         }
         std::vector<size_t> bo(big_all.size());
         std::iota(bo.begin(), bo.end(), size_t{0});
@@ -3552,7 +3556,7 @@ int plan_tree_pass(int nw, int64_t cap_words, int nsplit, const int64_t *tree_of
         narrow_all.swap(nn);
         std::vector<double> scost(small_all.size());
         for (size_t i = 0; i < small_all.size(); ++i)
-            scost[i] = 16.0 * npos_of(small_all[i]) + 35.0 * ncand(small_all[i]) + 300.0;
+            scost[i] = 2700.0 + 8.8 * npos_of(small_all[i]) + 17.0 * ncand(small_all[i]);
         std::vector<size_t> so(small_all.size());
         std::iota(so.begin(), so.end(), size_t{0});
         std::stable_sort(so.begin(), so.end(), [&](size_t x, size_t y) { return scost[x] > scost[y]; });

@@ -98,7 +98,6 @@ struct SpArgs {
 __device__ __forceinline__ uint32_t uni(uint32_t x) {
     return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x)));
 }
-__device__ __forceinline__ int32_t unii(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ uint64_t uni64(uint64_t x) {
     return (static_cast<uint64_t>(uni(static_cast<uint32_t>(x >> 32))) << 32) | uni(static_cast<uint32_t>(x));
 }

@@ -845,7 +845,8 @@ struct StageClock {
     }
 };
 
-int assemble(mcb_problem &P, std::vector<RootOutput> &per_root, bool keep_vertex_trees, StageClock &clock);
+int assemble(mcb_problem &P, std::vector<RootOutput> &per_root, bool keep_vertex_trees, int threads,
+             StageClock &clock);
 
 int build_from_input(InputGraph &in, int64_t input_weight_sum, const mcb_build_options &opt, mcb_problem **out) {
     StageClock clock;
@@ -962,7 +963,7 @@ int build_from_input(InputGraph &in, int64_t input_weight_sum, const mcb_build_o
     }
     if (err) return fail(MCB_E_NOMEM, "out of memory while building shortest-path trees");
     clock.mark("shortest-path trees");
-    int rc2 = assemble(*P, per_root, opt.keep_vertex_trees != 0, clock);
+    int rc2 = assemble(*P, per_root, opt.keep_vertex_trees != 0, threads, clock);
     if (rc2) return rc2;
     *out = P.release();
     return MCB_OK;
@@ -970,96 +971,117 @@ int build_from_input(InputGraph &in, int64_t input_weight_sum, const mcb_build_o
 
 // Candidates of every root in the reference's collection order, ranked by weight, and the
 // concatenated position layouts (mcb/mcb_cycle_deg_two_remove.cpp:244-267).
-int assemble(mcb_problem &Pr, std::vector<RootOutput> &per_root, bool keep_vertex_trees, StageClock &clock) {
+int assemble(mcb_problem &Pr, std::vector<RootOutput> &per_root, bool keep_vertex_trees, int threads,
+             StageClock &clock) {
     mcb_problem *P = &Pr;
     mcb_problem_info &info = P->info;
     const int32_t F = info.fvs_size;
     // Collect candidates root by root (roots ascend by vertex id) and rank them by weight with
-    // the same std::sort over the same sequence (mcb_cycle_deg_two_remove.cpp:247-258).
-    int64_t K = 0, positions = 0;
+    // the same std::sort over the same sequence (mcb_cycle_deg_two_remove.cpp:247-258).  Everything
+    // but the sort runs on the library's threads: each root's slice of the collection and of the
+    // layout starts at a prefix sum, and the per-root rank lists are a stable counting sort.
+    std::vector<int64_t> cand_at(static_cast<size_t>(F) + 1, 0);
+    P->tree_offset.assign(static_cast<size_t>(F) + 1, 0);
     int32_t max_pos = 0, max_depth = 0;
-    for (const RootOutput &o : per_root) {
-        K += static_cast<int64_t>(o.cand_edge.size());
-        positions += static_cast<int64_t>(o.parent_pos.size());
+    for (int32_t r = 0; r < F; ++r) {
+        const RootOutput &o = per_root[r];
+        cand_at[r + 1] = cand_at[r] + static_cast<int64_t>(o.cand_edge.size());
+        P->tree_offset[r + 1] = P->tree_offset[r] + static_cast<int64_t>(o.parent_pos.size());
         max_pos = std::max<int32_t>(max_pos, static_cast<int32_t>(o.parent_pos.size()));
         max_depth = std::max(max_depth, o.depth);
     }
+    const int64_t K = cand_at[F], positions = P->tree_offset[F];
     if (K > 0xFFFFFFFFll - 1) return fail(MCB_E_GRAPH, "more than 2^32-1 candidate cycles");
     info.num_candidates = K;
     info.positions = positions;
     info.max_positions = max_pos;
     info.max_depth = max_depth;
 
-    std::vector<uint32_t> c_root(K), c_edge(K), c_pu(K), c_pv(K);
-    std::vector<int32_t> c_w(K);
-    {
-        int64_t k = 0;
-        for (int32_t r = 0; r < F; ++r) {
-            const RootOutput &o = per_root[r];
-            for (size_t i = 0; i < o.cand_edge.size(); ++i, ++k) {
-                c_root[k] = static_cast<uint32_t>(r);
-                c_edge[k] = o.cand_edge[i];
-                c_w[k] = o.cand_weight[i];
-                c_pu[k] = o.cand_pu[i];
-                c_pv[k] = o.cand_pv[i];
-            }
-        }
-    }
     // The sort runs on (weight, collection index) pairs compared by weight alone: std::sort's
     // control flow depends only on the outcomes of its comparisons, which are those of sorting the
-    // indices by c_w[index], so the permutation (ties included) is the same, without a gather per
+    // indices by weight, so the permutation (ties included) is the same, without a gather per
     // comparison.
     struct Keyed {
         int32_t w;
         uint32_t k;
     };
     std::vector<Keyed> keyed(K);
-    for (int64_t k = 0; k < K; ++k) keyed[k] = Keyed{c_w[k], static_cast<uint32_t>(k)};
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
+    for (int32_t r = 0; r < F; ++r) {
+        const RootOutput &o = per_root[r];
+        for (size_t i = 0; i < o.cand_edge.size(); ++i) {
+            const int64_t k = cand_at[r] + static_cast<int64_t>(i);
+            keyed[k] = Keyed{o.cand_weight[i], static_cast<uint32_t>(k)};
+        }
+    }
     std::sort(keyed.begin(), keyed.end(), [](const Keyed &a, const Keyed &b) { return a.w < b.w; });
     clock.mark("candidate collect + sort");
-    std::vector<uint32_t> rank_order(K);
-    for (int64_t k = 0; k < K; ++k) rank_order[k] = keyed[k].k;
-    std::vector<Keyed>().swap(keyed);
+    // collection index -> root, then the ranked arrays gathered from the roots' own lists
+    std::vector<uint32_t> root_of(K);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
+    for (int32_t r = 0; r < F; ++r)
+        std::fill(root_of.begin() + cand_at[r], root_of.begin() + cand_at[r + 1], static_cast<uint32_t>(r));
     P->cand_root.resize(K);
     P->cand_edge.resize(K);
     P->cand_weight.resize(K);
     P->cand_pu.resize(K);
     P->cand_pv.resize(K);
     P->cand_coord.resize(K);
+#pragma omp parallel for schedule(static) num_threads(threads)
     for (int64_t k = 0; k < K; ++k) {
-        const uint32_t s = rank_order[k];
-        P->cand_root[k] = c_root[s];
-        P->cand_edge[k] = c_edge[s];
-        P->cand_weight[k] = c_w[s];
-        P->cand_pu[k] = c_pu[s];
-        P->cand_pv[k] = c_pv[s];
-        P->cand_coord[k] = P->nt_map[c_edge[s]];
+        const uint32_t s = keyed[k].k, r = root_of[s];
+        const RootOutput &o = per_root[r];
+        const size_t i = static_cast<size_t>(s - cand_at[r]);
+        P->cand_root[k] = r;
+        P->cand_edge[k] = o.cand_edge[i];
+        P->cand_weight[k] = o.cand_weight[i];
+        P->cand_pu[k] = o.cand_pu[i];
+        P->cand_pv[k] = o.cand_pv[i];
+        P->cand_coord[k] = P->nt_map[o.cand_edge[i]];
     }
-    // Per-root candidate lists, ranks ascending.
-    P->root_cand_offset.assign(F + 1, 0);
-    for (int64_t k = 0; k < K; ++k) P->root_cand_offset[P->cand_root[k] + 1]++;
-    for (int32_t r = 0; r < F; ++r) P->root_cand_offset[r + 1] += P->root_cand_offset[r];
+    std::vector<Keyed>().swap(keyed);
+    std::vector<uint32_t>().swap(root_of);
+    // Per-root candidate lists, ranks ascending: each root's list has exactly its own candidates, so
+    // root r's slice starts at cand_at[r]; thread t's chunk of ranks starts inside each slice after the
+    // earlier chunks' counts for that root.
+    P->root_cand_offset.assign(cand_at.begin(), cand_at.end());
     P->root_cand_rank.resize(K);
     {
-        std::vector<int64_t> fill(P->root_cand_offset.begin(), P->root_cand_offset.end() - 1);
-        for (int64_t k = 0; k < K; ++k) P->root_cand_rank[fill[P->cand_root[k]]++] = static_cast<uint32_t>(k);
+        const int T = std::max(1, std::min<int>(threads, static_cast<int>(K / 65536) + 1));
+        std::vector<int64_t> cnt(static_cast<size_t>(T) * F, 0);
+#pragma omp parallel num_threads(T)
+        {
+            const int t = omp_get_thread_num(), nt = omp_get_num_threads();  // nt <= T
+            const int64_t chunk = (K + nt - 1) / nt;
+            const int64_t k0 = std::min(K, chunk * t), k1 = std::min(K, k0 + chunk);
+            int64_t *c = cnt.data() + static_cast<size_t>(t) * F;
+            for (int64_t k = k0; k < k1; ++k) ++c[P->cand_root[k]];
+#pragma omp barrier
+#pragma omp for schedule(static)
+            for (int32_t r = 0; r < F; ++r) {
+                int64_t at = cand_at[r];
+                for (int u = 0; u < omp_get_num_threads(); ++u) {
+                    const int64_t x = cnt[static_cast<size_t>(u) * F + r];
+                    cnt[static_cast<size_t>(u) * F + r] = at;
+                    at += x;
+                }
+            }
+            for (int64_t k = k0; k < k1; ++k) P->root_cand_rank[c[P->cand_root[k]]++] = static_cast<uint32_t>(k);
+        }
     }
     clock.mark("rank arrays");
     // Concatenated position layouts.
-    P->tree_offset.assign(F + 1, 0);
-    for (int32_t r = 0; r < F; ++r)
-        P->tree_offset[r + 1] = P->tree_offset[r] + static_cast<int64_t>(per_root[r].parent_pos.size());
     P->parent_pos.resize(positions);
     P->pos_coord.resize(positions);
     if (keep_vertex_trees) P->pos_vertex.resize(positions);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
     for (int32_t r = 0; r < F; ++r) {
-        std::copy(per_root[r].parent_pos.begin(), per_root[r].parent_pos.end(),
-                  P->parent_pos.begin() + P->tree_offset[r]);
-        std::copy(per_root[r].coord.begin(), per_root[r].coord.end(), P->pos_coord.begin() + P->tree_offset[r]);
+        RootOutput &o = per_root[r];
+        std::copy(o.parent_pos.begin(), o.parent_pos.end(), P->parent_pos.begin() + P->tree_offset[r]);
+        std::copy(o.coord.begin(), o.coord.end(), P->pos_coord.begin() + P->tree_offset[r]);
         if (keep_vertex_trees)
-            std::copy(per_root[r].vertex.begin(), per_root[r].vertex.end(), P->pos_vertex.begin() + P->tree_offset[r]);
-        RootOutput().parent_pos.swap(per_root[r].parent_pos);
-        per_root[r] = RootOutput();
+            std::copy(o.vertex.begin(), o.vertex.end(), P->pos_vertex.begin() + P->tree_offset[r]);
+        o = RootOutput();
     }
     clock.mark("position layout");
     return MCB_OK;
@@ -1095,88 +1117,281 @@ void mcb_build_options_init(mcb_build_options *opt) {
 
 const char *mcb_host_last_error(void) { return g_err.c_str(); }
 
+namespace {
+// Features of the vertices of a CSR multigraph restricted to alive[v] != 0 (alive NULL: all).  With ids
+// NULL, rows for every vertex (zeros for the others); else the rows of the alive vertices only, in
+// ascending vertex order, with their ids.  Weight sums are sums of integers, exact in double, so the
+// rows do not depend on which vertices the CSR leaves out (a compacted graph gives the same rows).
+struct FeatureScratch {
+    std::vector<int64_t> deg, two, d, core;
+    std::vector<double> wsum;
+    std::vector<int32_t> bin, pos, vert;
+};
+
+void vertex_features_csr(int32_t n, const uint32_t *ro, const uint32_t *cols, const int32_t *weights,
+                         const uint8_t *alive, float *out, int32_t *ids, const int32_t *id_of, FeatureScratch &fs) {
+    auto live = [&](uint32_t x) { return !alive || alive[x]; };
+    std::vector<int64_t> &deg = fs.deg, &two = fs.two, &d = fs.d, &core = fs.core;
+    std::vector<double> &wsum = fs.wsum;
+    deg.assign(n, 0);
+    two.assign(n, 0);
+    wsum.assign(n, 0.0);
+    // Per-vertex passes on the library's threads above a size where that pays; the weight total is
+    // summed in vertex order afterwards (sums of integers, exact either way).
+    const bool par = n >= (1 << 15);
+#pragma omp parallel for schedule(static) if (par)
+    for (int32_t v = 0; v < n; ++v) {
+        if (!live(v)) continue;
+        int64_t dv = 0;
+        double wv = 0.0;
+        for (uint32_t e = ro[v]; e < ro[v + 1]; ++e)
+            if (live(cols[e])) {
+                ++dv;
+                wv += weights[e];
+            }
+        deg[v] = dv;
+        wsum[v] = wv;
+    }
+    int64_t maxd = 0;
+#pragma omp parallel for schedule(static) reduction(max : maxd) if (par)
+    for (int32_t v = 0; v < n; ++v) {
+        maxd = std::max(maxd, deg[v]);
+        if (!live(v)) continue;
+        int64_t t = 0;
+        for (uint32_t e = ro[v]; e < ro[v + 1]; ++e)
+            if (live(cols[e])) t += deg[cols[e]];
+        two[v] = t;
+    }
+    double wall = 0.0;
+    int64_t ends = 0;
+    for (int32_t v = 0; v < n; ++v) {
+        wall += wsum[v];
+        ends += deg[v];
+    }
+    const double wmean = ends ? wall / static_cast<double>(ends) : 1.0;
+    // core numbers: peel vertices in order of current degree (bucket queue; parallel edges count)
+    d.assign(deg.begin(), deg.end());
+    core.assign(n, 0);
+    std::vector<int32_t> &bin = fs.bin, &pos = fs.pos, &vert = fs.vert;
+    bin.assign(static_cast<size_t>(maxd) + 2, 0);
+    pos.resize(n);
+    vert.resize(n);
+    for (int32_t v = 0; v < n; ++v) ++bin[d[v]];
+    int32_t start = 0;
+    for (int64_t k = 0; k <= maxd; ++k) {
+        const int32_t c = bin[k];
+        bin[k] = start;
+        start += c;
+    }
+    for (int32_t v = 0; v < n; ++v) {
+        pos[v] = bin[d[v]]++;
+        vert[pos[v]] = v;
+    }
+    for (int64_t k = maxd; k > 0; --k) bin[k] = bin[k - 1];
+    bin[0] = 0;
+    int64_t maxcore = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const int32_t v = vert[i];
+        core[v] = d[v];
+        maxcore = std::max(maxcore, core[v]);
+        if (!live(v)) continue;
+        for (uint32_t e = ro[v]; e < ro[v + 1]; ++e) {
+            const int32_t u = static_cast<int32_t>(cols[e]);
+            if (live(u) && d[u] > d[v]) {  // move u one bucket down
+                const int64_t du = d[u];
+                const int32_t pu = pos[u], pw = bin[du], w = vert[pw];
+                if (u != w) {
+                    pos[u] = pw;
+                    vert[pu] = w;
+                    pos[w] = pu;
+                    vert[pw] = u;
+                }
+                ++bin[du];
+                --d[u];
+            }
+        }
+    }
+    // output row of each vertex (its rank among the alive ones when only those are written)
+    std::vector<int32_t> &row = fs.pos;
+    int32_t rows = 0;
+    for (int32_t v = 0; v < n; ++v) {
+        row[v] = ids ? rows : v;
+        rows += live(v) ? 1 : 0;
+    }
+#pragma omp parallel for schedule(static) if (par)
+    for (int32_t v = 0; v < n; ++v) {
+        if (!live(v) && ids) continue;
+        float *f = out + static_cast<size_t>(row[v]) * 4;
+        if (ids) ids[row[v]] = id_of ? id_of[v] : v;
+        if (!live(v)) {
+            f[0] = f[1] = f[2] = f[3] = 0.0f;
+            continue;
+        }
+        f[0] = static_cast<float>(std::log1p(static_cast<double>(deg[v])));
+        f[1] = static_cast<float>(std::log1p(static_cast<double>(two[v])));
+        f[2] = static_cast<float>(static_cast<double>(core[v]) / static_cast<double>(std::max<int64_t>(1, maxcore)));
+        f[3] = static_cast<float>(deg[v] ? wsum[v] / static_cast<double>(deg[v]) / std::max(wmean, 1e-9) : 0.0);
+    }
+}
+}  // namespace
+
 int mcb_vertex_features(int32_t nodes, const uint32_t *ro, const uint32_t *cols, const int32_t *weights,
                         const uint8_t *alive, float *out) {
     if (nodes <= 0 || !ro || !cols || !weights || !out) return fail(MCB_E_INVALID, "invalid arguments");
     try {
-        const int32_t n = nodes;
-        auto live = [&](uint32_t x) { return !alive || alive[x]; };
-        std::vector<int64_t> deg(n, 0), two(n, 0);
-        std::vector<double> wsum(n, 0.0);
-        double wall = 0.0;
-        int64_t ends = 0;
-        for (int32_t v = 0; v < n; ++v) {
-            if (!live(v)) continue;
-            for (uint32_t e = ro[v]; e < ro[v + 1]; ++e)
-                if (live(cols[e])) {
-                    ++deg[v];
-                    wsum[v] += weights[e];
-                }
-            wall += wsum[v];
-            ends += deg[v];
-        }
-        for (int32_t v = 0; v < n; ++v) {
-            if (!live(v)) continue;
-            for (uint32_t e = ro[v]; e < ro[v + 1]; ++e)
-                if (live(cols[e])) two[v] += deg[cols[e]];
-        }
-        const double wmean = ends ? wall / static_cast<double>(ends) : 1.0;
-        // core numbers: peel vertices in order of current degree (bucket queue; parallel edges count)
-        int64_t maxd = 0;
-        for (int32_t v = 0; v < n; ++v) maxd = std::max(maxd, deg[v]);
-        std::vector<int64_t> d(deg), core(n, 0);
-        std::vector<int32_t> bin(static_cast<size_t>(maxd) + 2, 0), pos(n), vert(n);
-        for (int32_t v = 0; v < n; ++v) ++bin[d[v]];
-        int32_t start = 0;
-        for (int64_t k = 0; k <= maxd; ++k) {
-            const int32_t c = bin[k];
-            bin[k] = start;
-            start += c;
-        }
-        for (int32_t v = 0; v < n; ++v) {
-            pos[v] = bin[d[v]]++;
-            vert[pos[v]] = v;
-        }
-        for (int64_t k = maxd; k > 0; --k) bin[k] = bin[k - 1];
-        bin[0] = 0;
-        int64_t maxcore = 0;
-        for (int32_t i = 0; i < n; ++i) {
-            const int32_t v = vert[i];
-            core[v] = d[v];
-            maxcore = std::max(maxcore, core[v]);
-            if (!live(v)) continue;
-            for (uint32_t e = ro[v]; e < ro[v + 1]; ++e) {
-                const int32_t u = static_cast<int32_t>(cols[e]);
-                if (live(u) && d[u] > d[v]) {  // move u one bucket down
-                    const int64_t du = d[u];
-                    const int32_t pu = pos[u], pw = bin[du], w = vert[pw];
-                    if (u != w) {
-                        pos[u] = pw;
-                        vert[pu] = w;
-                        pos[w] = pu;
-                        vert[pw] = u;
-                    }
-                    ++bin[du];
-                    --d[u];
-                }
-            }
-        }
-        for (int32_t v = 0; v < n; ++v) {
-            float *f = out + static_cast<size_t>(v) * 4;
-            if (!live(v)) {
-                f[0] = f[1] = f[2] = f[3] = 0.0f;
-                continue;
-            }
-            f[0] = static_cast<float>(std::log1p(static_cast<double>(deg[v])));
-            f[1] = static_cast<float>(std::log1p(static_cast<double>(two[v])));
-            f[2] = static_cast<float>(static_cast<double>(core[v]) / static_cast<double>(std::max<int64_t>(1, maxcore)));
-            f[3] = static_cast<float>(deg[v] ? wsum[v] / static_cast<double>(deg[v]) / std::max(wmean, 1e-9) : 0.0);
-        }
+        FeatureScratch fs;
+        vertex_features_csr(nodes, ro, cols, weights, alive, out, nullptr, nullptr, fs);
     } catch (const std::bad_alloc &) {
         return fail(MCB_E_NOMEM, "out of memory");
     }
     return MCB_OK;
 }
+
+// The round construction's state: the 2-core (parallel edges counted) of the graph minus the picks so
+// far, kept as a CSR over local ids that is compacted to the surviving vertices whenever fewer than three
+// quarters of its vertices are still alive, so that a round costs time in the size of what is left.
+struct mcb_fvs_rounds {
+    int32_t n = 0;                   // vertices of the input graph
+    std::vector<uint32_t> ro, cols;  // local CSR (vertices ascending in input id, row order kept)
+    std::vector<int32_t> w;
+    std::vector<int32_t> gid;        // local -> input id
+    std::vector<int32_t> lid;        // input id -> local id (-1: gone)
+    std::vector<uint8_t> alive;      // local
+    std::vector<int32_t> d;          // edge ends to alive vertices, per alive local vertex
+    std::vector<int32_t> last_ids;   // ids of the last feature rows, in row order
+    FeatureScratch scratch;
+    int32_t left = 0;
+
+    void kill(int32_t v, std::vector<int32_t> &stack) {
+        alive[v] = 0;
+        --left;
+        stack.push_back(v);
+        while (!stack.empty()) {
+            const int32_t x = stack.back();
+            stack.pop_back();
+            for (uint32_t e = ro[x]; e < ro[x + 1]; ++e) {
+                const int32_t u = static_cast<int32_t>(cols[e]);
+                if (alive[u] && --d[u] <= 1) {
+                    alive[u] = 0;
+                    --left;
+                    stack.push_back(u);
+                }
+            }
+        }
+    }
+
+    void compact() {
+        const int32_t ln = static_cast<int32_t>(gid.size());
+        std::vector<int32_t> map(ln, -1);
+        int32_t m = 0;
+        for (int32_t v = 0; v < ln; ++v)
+            if (alive[v]) map[v] = m++;
+        std::vector<uint32_t> nro(static_cast<size_t>(m) + 1, 0), ncols;
+        std::vector<int32_t> nw, ngid(m), nd(m);
+        for (int32_t v = 0; v < ln; ++v) {
+            if (map[v] < 0) {
+                lid[gid[v]] = -1;
+                continue;
+            }
+            const int32_t a = map[v];
+            for (uint32_t e = ro[v]; e < ro[v + 1]; ++e)
+                if (map[cols[e]] >= 0) {
+                    ncols.push_back(static_cast<uint32_t>(map[cols[e]]));
+                    nw.push_back(w[e]);
+                }
+            nro[a + 1] = static_cast<uint32_t>(ncols.size());
+            ngid[a] = gid[v];
+            lid[gid[v]] = a;
+            nd[a] = d[v];
+        }
+        ro.swap(nro);
+        cols.swap(ncols);
+        w.swap(nw);
+        gid.swap(ngid);
+        d.swap(nd);
+        alive.assign(m, 1);
+    }
+};
+
+int mcb_fvs_rounds_create(int32_t nodes, const uint32_t *ro, const uint32_t *cols, const int32_t *weights,
+                          mcb_fvs_rounds **out) {
+    if (nodes <= 0 || !ro || !cols || !weights || !out) return fail(MCB_E_INVALID, "invalid arguments");
+    *out = nullptr;
+    try {
+        std::unique_ptr<mcb_fvs_rounds> h(new mcb_fvs_rounds());
+        const int32_t n = nodes;
+        h->n = n;
+        h->ro.assign(ro, ro + n + 1);
+        h->cols.assign(cols, cols + ro[n]);
+        h->w.assign(weights, weights + ro[n]);
+        for (uint32_t e = 0; e < ro[n]; ++e)
+            if (cols[e] >= static_cast<uint32_t>(n)) return fail(MCB_E_INVALID, "column index out of range");
+        h->gid.resize(n);
+        std::iota(h->gid.begin(), h->gid.end(), 0);
+        h->lid = h->gid;
+        h->alive.assign(n, 1);
+        h->d.resize(n);
+        h->left = n;
+        std::vector<int32_t> stack;
+        for (int32_t v = 0; v < n; ++v) h->d[v] = static_cast<int32_t>(ro[v + 1] - ro[v]);
+        for (int32_t v = 0; v < n; ++v)
+            if (h->alive[v] && h->d[v] <= 1) h->kill(v, stack);
+        if (static_cast<int64_t>(h->left) * 4 < static_cast<int64_t>(n) * 3) h->compact();
+        *out = h.release();
+    } catch (const std::bad_alloc &) {
+        return fail(MCB_E_NOMEM, "out of memory");
+    }
+    return MCB_OK;
+}
+
+int32_t mcb_fvs_rounds_left(const mcb_fvs_rounds *h) { return h ? h->left : fail(MCB_E_INVALID, "invalid arguments"); }
+
+int32_t mcb_fvs_rounds_features(mcb_fvs_rounds *h, int32_t cap, int32_t *ids, float *out) {
+    if (!h || !ids || !out) return fail(MCB_E_INVALID, "invalid arguments");
+    if (cap < h->left) return fail(MCB_E_INVALID, "output capacity below the number of vertices left");
+    try {
+        vertex_features_csr(static_cast<int32_t>(h->gid.size()), h->ro.data(), h->cols.data(), h->w.data(),
+                            h->alive.data(), out, ids, h->gid.data(), h->scratch);
+        h->last_ids.assign(ids, ids + h->left);
+    } catch (const std::bad_alloc &) {
+        return fail(MCB_E_NOMEM, "out of memory");
+    }
+    return h->left;
+}
+
+int32_t mcb_fvs_rounds_take_top(mcb_fvs_rounds *h, const float *scores, int32_t count, int32_t k, int32_t *picks) {
+    if (!h || !scores || !picks || k < 0) return fail(MCB_E_INVALID, "invalid arguments");
+    if (h->last_ids.empty() || count != static_cast<int32_t>(h->last_ids.size()) || count != h->left)
+        return fail(MCB_E_STATE, "scores do not belong to the last feature rows");
+    k = std::min(k, count);
+    try {
+        // best first: higher score, NaN last, ties by row (= ascending id)
+        std::vector<int32_t> rows(count);
+        std::iota(rows.begin(), rows.end(), 0);
+        auto before = [&](int32_t a, int32_t b) {
+            const float sa = scores[a], sb = scores[b];
+            const bool na = std::isnan(sa), nb = std::isnan(sb);
+            if (na != nb) return nb;
+            if (!na && sa != sb) return sa > sb;
+            return a < b;
+        };
+        std::partial_sort(rows.begin(), rows.begin() + k, rows.end(), before);
+        std::vector<int32_t> stack;
+        for (int32_t i = 0; i < k; ++i) {
+            const int32_t g = h->last_ids[rows[i]];
+            picks[i] = g;
+            const int32_t v = h->lid[g];
+            if (h->alive[v]) h->kill(v, stack);
+        }
+        h->last_ids.clear();
+        if (static_cast<int64_t>(h->left) * 4 < static_cast<int64_t>(h->gid.size()) * 3) h->compact();
+    } catch (const std::bad_alloc &) {
+        return fail(MCB_E_NOMEM, "out of memory");
+    }
+    return h->left;
+}
+
+void mcb_fvs_rounds_free(mcb_fvs_rounds *h) { delete h; }
 
 int mcb_fvs_peel(int32_t nodes, const uint32_t *ro, const uint32_t *cols, uint8_t *alive) {
     if (nodes <= 0 || !ro || !cols || !alive) return fail(MCB_E_INVALID, "invalid arguments");
@@ -1453,7 +1668,12 @@ int mcb_problem_finish_trees(mcb_problem *p, const mcb_root_trees *t, int num_th
     if (!p->trees_pending) return fail(MCB_E_STATE, "the problem was not built with external_trees=1");
     const int32_t F = p->info.fvs_size;
     StageClock clock;
-    std::vector<RootOutput> per_root(F);
+    std::vector<RootOutput> per_root;
+    try {
+        per_root.resize(F);
+    } catch (const std::bad_alloc &) {
+        return fail(MCB_E_NOMEM, "out of memory");
+    }
     const int threads = num_threads > 0 ? num_threads : omp_get_max_threads();
     int err = 0;
 #pragma omp parallel num_threads(threads)
@@ -1477,7 +1697,12 @@ int mcb_problem_finish_trees(mcb_problem *p, const mcb_root_trees *t, int num_th
     }
     if (err) return fail(MCB_E_NOMEM, "out of memory while laying out shortest-path trees");
     clock.mark("trees from search results");
-    const int rc = assemble(*p, per_root, false, clock);
+    int rc;
+    try {
+        rc = assemble(*p, per_root, false, threads, clock);
+    } catch (const std::bad_alloc &) {
+        return fail(MCB_E_NOMEM, "out of memory while assembling the trees");
+    }
     if (rc) return rc;
     p->trees_pending = false;
     return MCB_OK;

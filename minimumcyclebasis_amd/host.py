@@ -293,6 +293,52 @@ def fvs_peel(row_offsets, cols, alive: np.ndarray) -> int:
     return _lib.check_host(_lib.host().mcb_fvs_peel(ro.size - 1, ro.ctypes.data, c.ctypes.data, alive.ctypes.data))
 
 
+class FvsRounds:
+    """The learned construction's round state (``mcb_fvs_rounds_*``): the 2-core of the graph minus the
+    picks so far, compacted as it shrinks, so a round costs time in what is left."""
+
+    def __init__(self, row_offsets, cols, weights):
+        ro = np.ascontiguousarray(row_offsets, dtype=np.uint32)
+        c = np.ascontiguousarray(cols, dtype=np.uint32)
+        wt = np.ascontiguousarray(weights, dtype=np.int32)
+        if c.size != wt.size or c.size != int(ro[-1]):
+            raise ValueError("cols and weights must hold row_offsets[-1] entries")
+        h = ctypes.c_void_p()
+        _lib.check_host(_lib.host().mcb_fvs_rounds_create(ro.size - 1, ro.ctypes.data, c.ctypes.data,
+                                                          wt.ctypes.data, ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def left(self) -> int:
+        return _lib.check_host(_lib.host().mcb_fvs_rounds_left(self._h))
+
+    def features(self) -> tuple[np.ndarray, np.ndarray]:
+        """(ids ascending, their feature rows) of the vertices left."""
+        k = self.left
+        ids = np.zeros(k, dtype=np.int32)
+        X = np.zeros((k, 4), dtype=np.float32)
+        _lib.check_host(_lib.host().mcb_fvs_rounds_features(self._h, k, ids.ctypes.data, X.ctypes.data))
+        return ids, X
+
+    def take_top(self, scores, k: int) -> np.ndarray:
+        """Takes the k best rows of the last ``features`` call by score (ties: lower id), best first."""
+        s = np.ascontiguousarray(scores, dtype=np.float32)
+        out = np.zeros(min(int(k), s.size), dtype=np.int32)
+        _lib.check_host(_lib.host().mcb_fvs_rounds_take_top(self._h, s.ctypes.data, s.size, int(k), out.ctypes.data))
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.host().mcb_fvs_rounds_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def fvs_greedy(row_offsets, cols) -> np.ndarray:
     """The peeling max-degree greedy FVS made minimal by reverse-delete; root rank per vertex (-1 = not in F)."""
     ro = np.ascontiguousarray(row_offsets, dtype=np.uint32)

@@ -10,6 +10,8 @@ degree, neighbours' degree sum, core number, mean incident weight).  The constru
 vertices that lie on no remaining cycle are peeled away (``host.fvs_peel``), the remaining ones are
 scored, and the best-scored 2 % are taken; then the picks are made a minimal FVS by the reverse-delete
 pass (the other vertices first in the reference order, then the picks from the last to the first).
+The rounds run in one native state (``host.FvsRounds``) that compacts the graph as it shrinks, so the
+whole construction costs time in the sum of the round sizes rather than rounds x graph size.
 The MLP is trained to predict membership in the set that the one-at-a-time peeling max-degree greedy
 finds (``host.fvs_greedy``), on rows of that process's own rounds, over graphs of the same generator.
 
@@ -27,7 +29,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from .host import Problem, fvs_greedy, fvs_peel, fvs_reverse_delete, induced_has_cycle, vertex_features
+from .host import FvsRounds, Problem, fvs_greedy, fvs_reverse_delete, induced_has_cycle
 
 N_PARAMS = 4545
 PARAMS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "fvs_mlp.json")
@@ -142,20 +144,17 @@ def scored_greedy(row_offsets, cols, weights, score_fn, frac: float = ROUND_FRAC
     sees every round's remaining vertices and their features."""
     ro = np.ascontiguousarray(row_offsets, dtype=np.uint32)
     n = ro.size - 1
-    alive = np.ones(n, dtype=np.uint8)
-    left = fvs_peel(ro, cols, alive)
+    rounds = FvsRounds(ro, cols, weights)
     picks = []
-    while left > 0:
-        ids = np.flatnonzero(alive)
-        X = vertex_features(ro, cols, weights, alive)[ids]
-        if on_round:
-            on_round(ids, X)
-        s = np.asarray(score_fn(X))
-        k = max(1, int(np.ceil(frac * ids.size)))
-        top = ids[np.argsort(-s, kind="stable")[:k]]
-        picks.append(top)
-        alive[top] = 0
-        left = fvs_peel(ro, cols, alive)
+    try:
+        while rounds.left > 0:
+            ids, X = rounds.features()
+            if on_round:
+                on_round(ids, X)
+            s = np.asarray(score_fn(X), dtype=np.float32)
+            picks.append(rounds.take_top(s, max(1, int(np.ceil(frac * ids.size)))))
+    finally:
+        rounds.close()
     picks = np.concatenate(picks) if picks else np.zeros(0, dtype=np.int64)
     in_pick = np.zeros(n, dtype=bool)
     in_pick[picks] = True
