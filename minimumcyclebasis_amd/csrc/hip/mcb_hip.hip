@@ -1190,6 +1190,60 @@ __device__ __forceinline__ void run_words(const A &a, const uint64_t *Wd, int64_
     }
 }
 
+// A small item of at most one window of positions and one batch of candidates.
+__device__ __forceinline__ bool is_tiny(const ItemMeta &m) {
+    return (m.npos & ~kNarrowBit) <= static_cast<uint32_t>(kWave) && m.ce - m.cb <= kWave;
+}
+
+// What a tiny item reads from its tree layout and candidate run: lane x its position x (parent,
+// coordinate) and its candidate cb + x (ends, rank, own coordinate); loads of a lane past the item's
+// end are clamped to its first entry, and nothing is loaded for a non-tiny or absent item.
+struct TinyLoads {
+    uint32_t par, pu, pv, rk;
+    int32_t co, c;
+};
+
+template <typename A>
+__device__ __forceinline__ TinyLoads load_tiny(const A &a, const ItemMeta &m, bool present, int lane) {
+    TinyLoads t{kNoParent, 0u, 0u, 0u, -1, -1};
+    if (present && is_tiny(m)) {
+        const uint32_t npos = m.npos & ~kNarrowBit;
+        const int64_t at = m.off + (static_cast<uint32_t>(lane) < npos ? lane : 0);
+        t.par = a.parent_pos[at];
+        t.co = a.pos_coord[at];
+        const int64_t k = m.cb + lane < m.ce ? m.cb + lane : m.cb;
+        t.pu = a.rc_pu[k];
+        t.pv = a.rc_pv[k];
+        t.rk = a.rc_rank[k];
+        t.c = a.rc_coord[k];
+    }
+    return t;
+}
+
+// A tiny item from its loads: the path words of its one window settle by pointer jumping (every
+// parent lies in the window), each candidate's word is its ends' words (lane permutes) ^ Tt[own
+// coordinate], and the batch goes through the run's greedy basis as run_words would insert it.
+template <typename A>
+__device__ __forceinline__ void tiny_item(const A &a, const TinyLoads &t, uint32_t npos, uint32_t ncand, int lane) {
+    const bool v = static_cast<uint32_t>(lane) < npos;
+    const bool kv = static_cast<uint32_t>(lane) < ncand;
+    const uint64_t tv = (v && t.co >= 0) ? a.Tt[t.co] : 0ull;
+    const uint64_t tc = (kv && t.c >= 0) ? a.Tt[t.c] : 0ull;
+    const uint32_t p = v ? t.par : kNoParent;
+    const bool inwin = p != kNoParent;
+    uint64_t val = tv;
+    if (__ballot(inwin)) val = settle_words(val, inwin, p, 0u, lane);
+    const uint64_t wu = __shfl(val, static_cast<int>(t.pu & (kWave - 1)));
+    const uint64_t wv = __shfl(val, static_cast<int>(t.pv & (kWave - 1)));
+    const uint64_t q = kv ? (wu ^ wv ^ tc) : 0ull;
+    uint64_t piv = 0, P = 0;
+    const uint32_t rk = t.rk;
+    basis_reduce_insert(piv, P, q, lane, [&]() {
+        a.Qr[rk] = q;
+        atomicOr(reinterpret_cast<unsigned long long *>(a.qbits + (rk >> 6)), 1ull << (rk & 63));
+    });
+}
+
 // Pass over every tree and every candidate of the block.  The work lists hold items (a root and a
 // rank-ascending run of its candidates, as ItemMeta records): big trees (more than
 // small_words positions) take a whole workgroup per item, the others one wave per item.  This
@@ -1287,15 +1341,27 @@ __device__ __forceinline__ void tree_pass(const A &a, const ItemMeta *big, uint3
             one_tree(a.scratch + static_cast<int64_t>(blockIdx.x) * a.huge_words, rd, true);
         }
     }
-    // the big-tree region is free from here on; each wave takes its own slice for small roots
+    // the big-tree region is free from here on; each wave takes its own slice for small roots.
+    // A tiny item (one window of positions, one batch of candidates) is software-pipelined: its
+    // positions and candidates are loaded while the wave works on the item before it, and its path
+    // words stay in registers (a candidate reads its ends' words by lane permutes).
     uint64_t *Ws = lds + static_cast<size_t>(wid) * a.small_words;
-    ItemMeta snext = load_item(small, sfirst, send);
+    ItemMeta cur = load_item(small, sfirst, send);
+    ItemMeta nxt = load_item(small, sfirst + sstride, send);
+    TinyLoads tl = load_tiny(a, cur, sfirst < send, lane);
     for (uint32_t it = sfirst; it < send; it += sstride) {
-        const ItemMeta cur = snext;
-        snext = load_item(small, it + sstride, send);  // in flight while this item runs
+        const ItemMeta later = load_item(small, it + 2 * sstride, send);  // in flight two items ahead
+        const TinyLoads tn = load_tiny(a, nxt, it + sstride < send, lane);  // in flight while this item runs
         const uint32_t npos = cur.npos & ~kNarrowBit;
-        tree_words(a.parent_pos + cur.off, a.pos_coord + cur.off, npos, a.Tt, Ws, lane);
-        run_words(a, Ws, cur.cb, cur.ce, kk_of(cur.root), lane);
+        if (is_tiny(cur)) {
+            if (kk_of(cur.root) > 0) tiny_item(a, tl, npos, static_cast<uint32_t>(cur.ce - cur.cb), lane);
+        } else {
+            tree_words(a.parent_pos + cur.off, a.pos_coord + cur.off, npos, a.Tt, Ws, lane);
+            run_words(a, Ws, cur.cb, cur.ce, kk_of(cur.root), lane);
+        }
+        cur = nxt;
+        nxt = later;
+        tl = tn;
     }
 }
 
