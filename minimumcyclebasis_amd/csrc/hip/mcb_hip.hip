@@ -683,6 +683,7 @@ __global__ void k_copy_u64(const uint64_t *__restrict__ src, uint64_t *__restric
 constexpr int kBlockK = 64;      // phases per block (word width)
 constexpr int kQWaves = 16;      // waves per workgroup of the tree pass
 constexpr int kChunk = 4096;     // ranks per first-level greedy basis (one candidate-bitmap word per lane)
+constexpr int kMaxRounds = 16;   // bitmap rounds of 64 words per first-level basis of the blocked form
 constexpr int64_t kXorWorkgroups = 2048;  // grid of k_block_xor (eight workgroups per CU)
 constexpr int64_t kDotWorkgroups = 512;   // grid of k_block_dot (two eight-wave workgroups per CU)
 // The update's byte counter is spread over this many slots (slot = workgroup index mod kByteSlots), so
@@ -766,6 +767,7 @@ struct BlockArgs {
     unsigned long long *upd_bytes;  // accumulated algorithmic bytes of the update
     int64_t K;
     int32_t e, kk, N, W, n_big, n_small, big_words, small_words, group;
+    int32_t chunk_words;  // bitmap words per first-level basis
 };
 
 // One block of kk phases of one problem in that problem's own numbering: its support matrix
@@ -1543,44 +1545,65 @@ __device__ __forceinline__ int select_bit(uint64_t w, uint32_t k) {
 }
 
 // First level of the block's greedy basis for the ranks [r0, r0 + 64 * nwords) (r0 a multiple of
-// 64, nwords <= 64) into one slot; only the words the tree pass emitted take part (the others cannot
-// be in the basis), and the bitmap words read are cleared for the next block.  Lane l reads bitmap
-// word l; a prefix sum of the popcounts lists the emitted ranks in rank order, which are inserted 64
-// at a time.
+// 64, nwords <= 64 * kR) into one slot; only the words the tree pass emitted take part (the others
+// cannot be in the basis), and the bitmap words read are cleared for the next block.  The bitmap is
+// read in rounds of 64 words, all rounds' loads in flight together; prefix sums of the popcounts
+// list the emitted ranks in rank order (s_words / s_offs: 64 * kR entries of LDS), and they are
+// inserted 64 at a time, each batch's words loaded while the one before it is inserted.
+template <int kR>
 __device__ __forceinline__ void basis_chunk(uint64_t *qbits, const uint64_t *Qr, int64_t r0, int nwords, int64_t slot,
                                             int kk, uint32_t *sum_cnt, uint32_t *sum_rank, uint64_t *sum_q,
                                             uint64_t *s_words, uint32_t *s_offs, int lane) {
     const int64_t wbase = r0 >> 6;
-    const bool valid = lane < nwords;
-    const uint64_t word = valid ? qbits[wbase + lane] : 0ull;
-    if (word) qbits[wbase + lane] = 0;  // ready for the next block
-    const uint32_t own = static_cast<uint32_t>(__popcll(word));
-    uint32_t incl = own;
+    const int rounds = (nwords + kWave - 1) / kWave;
+    uint64_t word[kR];
 #pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const uint32_t t = __shfl_up(incl, o);
-        if (lane >= o) incl += t;
+    for (int j = 0; j < kR; ++j) {
+        const int w = j * kWave + lane;
+        word[j] = (j < rounds && w < nwords) ? qbits[wbase + w] : 0ull;
     }
-    const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), kWave - 1));
-    s_words[lane] = word;
-    s_offs[lane] = incl - own;
+    uint32_t total = 0;
+#pragma unroll
+    for (int j = 0; j < kR; ++j) {
+        if (j >= rounds) break;
+        if (word[j]) qbits[wbase + j * kWave + lane] = 0;  // ready for the next block
+        const uint32_t own = static_cast<uint32_t>(__popcll(word[j]));
+        uint32_t incl = own;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        s_words[j * kWave + lane] = word[j];
+        s_offs[j * kWave + lane] = total + incl - own;
+        total += static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), kWave - 1));
+    }
     __builtin_amdgcn_wave_barrier();
-    uint64_t piv = 0, P = 0;
-    uint32_t cnt = 0;
-    for (uint32_t b0 = 0; b0 < total && cnt < static_cast<uint32_t>(kk); b0 += kWave) {
-        const uint32_t t = b0 + lane;
-        uint64_t q = 0;
-        uint32_t rank = 0;
+    const int nx = rounds * kWave;
+    // entry t: the last word whose first entry is at or before t owns it
+    auto fetch = [&](uint32_t t, uint64_t &q, uint32_t &rank) {
+        q = 0;
+        rank = 0;
         if (t < total) {
-            int lo = 0, hi = kWave - 1;  // the last lane whose first entry is at or before t owns entry t
+            int lo = 0, hi = nx - 1;
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
                 if (s_offs[mid] <= t) lo = mid;
                 else hi = mid - 1;
             }
-            rank = static_cast<uint32_t>(r0 + lo * kWave + select_bit(s_words[lo], t - s_offs[lo]));
+            rank = static_cast<uint32_t>(r0 + static_cast<int64_t>(lo) * kWave + select_bit(s_words[lo], t - s_offs[lo]));
             q = Qr[rank];
         }
+    };
+    uint64_t qn;
+    uint32_t rn;
+    fetch(static_cast<uint32_t>(lane), qn, rn);
+    uint64_t piv = 0, P = 0;
+    uint32_t cnt = 0;
+    for (uint32_t b0 = 0; b0 < total && cnt < static_cast<uint32_t>(kk); b0 += kWave) {
+        const uint64_t q = qn;
+        const uint32_t rank = rn;
+        fetch(b0 + kWave + static_cast<uint32_t>(lane), qn, rn);
         basis_insert(piv, P, cnt, q, rank, sum_rank + slot * kWave, sum_q + slot * kWave, lane);
     }
     if (lane == 0) sum_cnt[slot] = cnt;
@@ -1698,19 +1721,20 @@ __device__ __forceinline__ void merge_stream(int64_t s0, int members, int64_t sl
     if (lane == 0) sum_cnt[slot] = cnt;
 }
 
-// Wave c: ranks [c*kChunk, (c+1)*kChunk) -> slot c of the first level (and the block's finalisation
-// when that is the only level).
+// Wave c: the bitmap words [c * chunk_words, (c + 1) * chunk_words) (chunk_words a multiple of 64, at
+// most 64 * kMaxRounds) -> slot c of the first level (and the block's finalisation when that is the only
+// level).
 __global__ __launch_bounds__(256) void k_block_basis(BlockArgs a) {
-    __shared__ uint64_t s_words[4][kWave];
-    __shared__ uint32_t s_offs[4][kWave];
+    __shared__ uint64_t s_words[4][kMaxRounds * kWave];
+    __shared__ uint32_t s_offs[4][kMaxRounds * kWave];
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = wave_id();
     const int64_t c = static_cast<int64_t>(blockIdx.x) * 4 + wv;
     if (c >= a.lvl_n[0]) return;
-    const int64_t r0 = c * kChunk;
-    const int64_t nwords = (a.K + kWave - 1) / kWave - (r0 >> 6);
-    basis_chunk(a.qbits, a.Qr, r0, static_cast<int>(nwords < kWave ? nwords : kWave), a.lvl_slot[0] + c, a.kk,
-                a.sum_cnt, a.sum_rank, a.sum_q, s_words[wv], s_offs[wv], lane);
+    const int64_t w0 = c * a.chunk_words;
+    const int64_t nwords = (a.K + kWave - 1) / kWave - w0;
+    basis_chunk<kMaxRounds>(a.qbits, a.Qr, w0 * kWave, static_cast<int>(nwords < a.chunk_words ? nwords : a.chunk_words),
+                            a.lvl_slot[0] + c, a.kk, a.sum_cnt, a.sum_rank, a.sum_q, s_words[wv], s_offs[wv], lane);
     if (a.levels == 1) {
         __threadfence();
         block_finalize(a, lane);
@@ -2511,7 +2535,7 @@ __global__ __launch_bounds__(256) void k_batch_basis(BatchArgs a) {
     const int g = static_cast<int>(uniform32(static_cast<uint32_t>(ch.comp)));
     const CompInfo &c = a.comp[g];
     const int32_t kk = batch_kk(static_cast<int32_t>(uniform32(static_cast<uint32_t>(c.N))), a.step);
-    basis_chunk(a.qbits, a.Qr, uniform64(ch.r0), static_cast<int>(uniform32(static_cast<uint32_t>(ch.words))),
+    basis_chunk<1>(a.qbits, a.Qr, uniform64(ch.r0), static_cast<int>(uniform32(static_cast<uint32_t>(ch.words))),
                 uniform64(ch.slot), kk, a.sum_cnt, a.sum_rank, a.sum_q, s_words[wv], s_offs[wv], lane);
     if (uniform32(static_cast<uint32_t>(c.levels)) == 1u) {
         __threadfence();
@@ -2996,7 +3020,7 @@ struct mcb_solver {
     uint32_t *arrive = nullptr;  // merge group arrivals of k_block_cascade
     uint32_t *ulist = nullptr, *ucount = nullptr;  // the block's updated later columns
     unsigned long long *upd_bytes = nullptr;
-    int32_t levels = 0, group = 16;
+    int32_t levels = 0, group = 16, chunk_words = kWave;
     int64_t lvl_slot[kMaxLevels] = {}, lvl_n[kMaxLevels] = {};
     int64_t blk_positions = 0, blk_candidates = 0;  // work of the tree passes since reset
     // timing
@@ -3174,6 +3198,7 @@ BlockArgs block_args(const mcb_solver *s, int32_t e, int32_t kk) {
     }
     a.levels = s->levels;
     a.group = s->group;
+    a.chunk_words = s->chunk_words;
     a.arrive = s->arrive;
     a.sel = s->best;
     a.blk_q = s->blk_q;
@@ -3795,8 +3820,18 @@ int setup_blocked(mcb_solver *s, const int64_t *tree_offset, const uint32_t *par
         return rc;
     s->bk = want > 1 ? want : kBlockK;
     use_plan(s, 0);
-    // greedy-basis levels: kChunk ranks per first-level basis, group bases per merge
+    // greedy-basis levels: chunk_words bitmap words per first-level basis, group bases per merge.  The
+    // bitmap is sparse (a block's tree pass emits a few thousand of millions of ranks), so by default
+    // a first-level basis reads as many rounds of 64 words as keep the first level within one merge
+    // group of 64: one merge then finishes the block, with no arrival between workgroups.  A chosen
+    // merge group keeps one round (4096 ranks) per first-level basis.
     int64_t n = (s->K + kChunk - 1) / kChunk, slots = 0;
+    s->chunk_words = kWave;
+    if (s->opt.merge_group == 0 && n > 64) {
+        const int64_t rounds = std::min<int64_t>(kMaxRounds, (n + 63) / 64);
+        s->chunk_words = static_cast<int32_t>(rounds * kWave);
+        n = (s->K + static_cast<int64_t>(kChunk) * rounds - 1) / (static_cast<int64_t>(kChunk) * rounds);
+    }
     s->group = merge_group_size(n, s->opt.merge_group);
     s->levels = 0;
     for (;;) {

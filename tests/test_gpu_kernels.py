@@ -263,6 +263,35 @@ def test_merge_group_sizes(case, group):
     s.close()
 
 
+@pytest.mark.parametrize("n", [20000, 60000])
+def test_first_level_rounds_agree_with_one_round(n):
+    """By default a first-level basis of a large problem reads several 64-word rounds of the candidate
+    bitmap (Delaunay 20k: about 500k candidates, three rounds; 60k: nine), so that one merge finishes the
+    block; with a chosen merge group it reads one round and merges over two levels.  Both give the same
+    selections and cycle vectors, and the sampled phases pass the oracle's per-phase checks."""
+    from fullsize import check_loop, sample_phases
+    from minimumcyclebasis_amd import generators as G
+    from minimumcyclebasis_amd.host import Problem
+
+    p = Problem.from_edges(*G.delaunay(n, seed=7), prune_trees=True)
+    print(f"case delaunay{n} K={p.info.num_candidates}", flush=True)
+    store = n <= 20000  # the cycle vectors of the larger case would take 2 GB per solver
+    a = Solver(p, store_cycles=store)
+    a.run()
+    a.sync()
+    b = Solver(p, store_cycles=store, merge_group=64)
+    b.run()
+    b.sync()
+    assert np.array_equal(a.selected(), b.selected())
+    if store:
+        assert np.array_equal(a.cycle_vectors(), b.cycle_vectors())
+    check_loop(p, a.selected(), a.support_vector_rows, sample_phases(p.info.num_non_tree, spread=10), seed=5,
+               label=f"delaunay{n}")
+    a.close()
+    b.close()
+    p.close()
+
+
 def test_merge_group_rejected():
     p = problem_of(load_trace("grid16x16"), "reduced", keep_vertex_trees=True, prune_trees=True)
     for bad in (1, 65, -2):
