@@ -887,6 +887,35 @@ __device__ __forceinline__ uint64_t settle_words(uint64_t val, bool pending, uin
     return val;
 }
 
+// Relative words of kU windows at once: in window u, a lane whose parent lies in the window starts with
+// jmp[u] = that parent's lane, the others with jmp[u] = -1 and anc[u] = their parent (before the
+// window, or kNoParent).  Pointer jumping, all windows in lockstep so that their lane permutes are in
+// flight together: a pending lane takes its in-window ancestor's value and ancestor (both as they were
+// before the round), and on landing on a settled lane, that lane's anc.  Afterwards rel[u] is the XOR
+// of the terms on the lane's path up to its first ancestor before the window (excluded) and anc[u] that
+// ancestor.  A parent precedes its child, so every chain ends within six rounds.
+template <int kU>
+__device__ __forceinline__ void settle_windows(uint64_t (&rel)[kU], uint32_t (&anc)[kU], int (&jmp)[kU], int lane) {
+    for (;;) {
+        bool pend = false;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) pend |= jmp[u] >= 0;
+        if (!__ballot(pend)) break;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int src = jmp[u] >= 0 ? jmp[u] : lane;
+            const uint64_t av = __shfl(rel[u], src);
+            const int aj = __shfl(jmp[u], src);
+            const uint32_t aa = static_cast<uint32_t>(__shfl(static_cast<int>(anc[u]), src));
+            if (jmp[u] >= 0) {
+                rel[u] ^= av;
+                jmp[u] = aj;
+                if (aj < 0) anc[u] = aa;
+            }
+        }
+    }
+}
+
 // Path words of kUnroll windows of one wave's tree: word[pos] = word[parent] ^ Tt[coordinate].
 // kRing: Wd is global scratch and ring (rmask + 1 words of LDS, a power of two of at least two
 // windows) keeps the most recent windows' words, so a parent in one of them is read from LDS.
@@ -935,66 +964,61 @@ __device__ __forceinline__ void tree_words(const uint32_t *pp, const int32_t *pc
     }
 }
 
-// Pointer jumping as in settle_words, carrying each lane's anchor: a lane whose jump lands on a settled
-// ancestor takes that ancestor's anchor along with its value (both final, being from before the round).
-__device__ __forceinline__ uint64_t settle_anchored(uint64_t val, uint32_t &anchor, bool pending, uint32_t p,
-                                                    uint32_t wb, int lane) {
-    int anc = pending ? static_cast<int>(p - wb) : -1;
-    while (__ballot(anc >= 0)) {
-        const int src = anc >= 0 ? anc : lane;
-        const uint64_t av = __shfl(val, src);
-        const int aa = __shfl(anc, src);
-        const uint32_t an = static_cast<uint32_t>(__shfl(static_cast<int>(anchor), src));
-        if (anc >= 0) {
-            val ^= av;
-            anc = aa;
-            if (aa < 0) anchor = an;
-        }
-    }
-    return val;
-}
-
 // One group of a segment walk (positions [s0, s1) of a narrow tree, one wave): Wd[q] becomes the XOR of
 // the Tt words on the path from q up to its first ancestor before s0, excluded, and anch[q] that
 // ancestor (kNoParent when the path reaches the root inside the segment).  rv/ra: the wave's ring of its
-// latest rmask + 1 positions (words and anchors) in LDS.
-constexpr int kSegUnroll = 4;  // windows per prefetched group of a segment walk
+// latest rmask + 1 positions (words and anchors) in LDS.  The group's windows settle together
+// (settle_windows) to words relative to each window's entry; only the last step, joining a window to
+// its ancestors' words in the ring, runs window after window, so the serial chain of the walk is one
+// LDS read and write per window.
+constexpr int kSegUnroll = 2;  // windows per prefetched group of a segment walk (settled in lockstep)
 using SegWindows = WindowGroupT<kSegUnroll>;
 
 __device__ __forceinline__ void segment_group(const SegWindows &g, uint32_t base, uint32_t s0, uint32_t s1,
                                               const uint64_t *Tt, uint64_t *Wd, uint32_t *anch, uint64_t *rv,
                                               uint32_t *ra, uint32_t rmask, int lane) {
-    uint64_t t[kSegUnroll];
+    uint64_t rel[kSegUnroll];
+    uint32_t anc[kSegUnroll];
+    int jmp[kSegUnroll];
 #pragma unroll
-    for (int u = 0; u < kSegUnroll; ++u) t[u] = Tt[g.co[u] >= 0 ? g.co[u] : 0];
+    for (int u = 0; u < kSegUnroll; ++u) rel[u] = Tt[g.co[u] >= 0 ? g.co[u] : 0];
+#pragma unroll
+    for (int u = 0; u < kSegUnroll; ++u) {
+        const uint32_t wb = base + u * kWave;
+        const bool v = wb + lane < s1;
+        const uint32_t p = v ? g.par[u] : kNoParent;
+        if (!v || g.co[u] < 0) rel[u] = 0;
+        const bool inwin = p != kNoParent && p >= s0 && p >= wb;
+        jmp[u] = inwin ? static_cast<int>(p - wb) : -1;
+        anc[u] = inwin ? kNoParent : p;
+    }
+    settle_windows<kSegUnroll>(rel, anc, jmp, lane);
 #pragma unroll
     for (int u = 0; u < kSegUnroll; ++u) {
         const uint32_t wb = base + u * kWave;
         if (wb >= s1) break;
         const bool v = wb + lane < s1;
-        const uint32_t p = v ? g.par[u] : kNoParent;
-        const uint64_t tv = (v && g.co[u] >= 0) ? t[u] : 0ull;
-        const bool outside = p == kNoParent || p < s0;  // the path leaves the segment at p (or ends at the root)
-        const bool before = !outside && p < wb;
-        const bool near = before && p + rmask + 1 >= wb + kWave;
-        const bool far = before && !near;
-        // a parent older than the ring: this wave's own earlier stores must be complete before it is read
+        // an ancestor inside the segment (before this window) contributes its relative word and passes
+        // on its segment anchor; one before the segment (or kNoParent) is the segment anchor itself
+        const uint32_t x = v ? anc[u] : kNoParent;
+        const bool inseg = x != kNoParent && x >= s0;
+        const bool near = inseg && x + rmask + 1 >= wb + kWave;
+        const bool far = inseg && !near;
+        // an ancestor older than the ring: this wave's own earlier stores must be complete before it is read
         if (__ballot(far)) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         }
         uint64_t pw = 0;
-        uint32_t an = p;
+        uint32_t an = x;
         if (near) {
-            pw = rv[p & rmask];
-            an = ra[p & rmask];
+            pw = rv[x & rmask];
+            an = ra[x & rmask];
         } else if (far) {
-            pw = Wd[p];
-            an = anch[p];
+            pw = Wd[x];
+            an = anch[x];
         }
-        uint64_t val = tv ^ pw;
-        const bool inwin = v && !outside && p >= wb;
-        if (__ballot(inwin)) val = settle_anchored(val, an, inwin, p, wb, lane);
+        const uint64_t val = rel[u] ^ pw;
         if (v) {
             const uint32_t q = wb + lane;
             Wd[q] = val;
