@@ -974,20 +974,23 @@ __device__ __forceinline__ void tree_words(const uint32_t *pp, const int32_t *pc
 constexpr int kSegUnroll = 2;  // windows per prefetched group of a segment walk (settled in lockstep)
 using SegWindows = WindowGroupT<kSegUnroll>;
 
-__device__ __forceinline__ void segment_group(const SegWindows &g, uint32_t base, uint32_t s0, uint32_t s1,
-                                              const uint64_t *Tt, uint64_t *Wd, uint32_t *anch, uint64_t *rv,
+__device__ __forceinline__ void gather_group(uint64_t (&t)[kSegUnroll], const SegWindows &g, const uint64_t *Tt) {
+#pragma unroll
+    for (int u = 0; u < kSegUnroll; ++u) t[u] = Tt[g.co[u] >= 0 ? g.co[u] : 0];
+}
+
+__device__ __forceinline__ void segment_group(const SegWindows &g, const uint64_t (&t)[kSegUnroll], uint32_t base,
+                                              uint32_t s0, uint32_t s1, uint64_t *Wd, uint32_t *anch, uint64_t *rv,
                                               uint32_t *ra, uint32_t rmask, int lane) {
     uint64_t rel[kSegUnroll];
     uint32_t anc[kSegUnroll];
     int jmp[kSegUnroll];
 #pragma unroll
-    for (int u = 0; u < kSegUnroll; ++u) rel[u] = Tt[g.co[u] >= 0 ? g.co[u] : 0];
-#pragma unroll
     for (int u = 0; u < kSegUnroll; ++u) {
         const uint32_t wb = base + u * kWave;
         const bool v = wb + lane < s1;
         const uint32_t p = v ? g.par[u] : kNoParent;
-        if (!v || g.co[u] < 0) rel[u] = 0;
+        rel[u] = (v && g.co[u] >= 0) ? t[u] : 0ull;
         const bool inwin = p != kNoParent && p >= s0 && p >= wb;
         jmp[u] = inwin ? static_cast<int>(p - wb) : -1;
         anc[u] = inwin ? kNoParent : p;
@@ -1043,15 +1046,22 @@ __device__ void tree_words_segmented(const uint32_t *pp, const int32_t *pc, uint
     uint32_t *ra = reinterpret_cast<uint32_t *>(ring_base + static_cast<size_t>(nw) * (rmask + 1)) +
                    static_cast<size_t>(wid) * (rmask + 1);
     if (s0 < s1) {
+        // three groups in flight: this one's words, the next one's Tt words, the one after's positions
         constexpr uint32_t G = kWave * kSegUnroll;
-        SegWindows A, B;
-        load_group(A, pp, pc, s1, s0 + lane, kWave);
-        for (uint32_t base = s0; base < s1; base += 2 * G) {
-            load_group(B, pp, pc, s1, base + G + lane, kWave);
-            segment_group(A, base, s0, s1, Tt, Wd, anch, rv, ra, rmask, lane);
-            if (base + G >= s1) break;
-            load_group(A, pp, pc, s1, base + 2 * G + lane, kWave);
-            segment_group(B, base + G, s0, s1, Tt, Wd, anch, rv, ra, rmask, lane);
+        SegWindows cur, nxt;
+        uint64_t tc[kSegUnroll], tn[kSegUnroll];
+        load_group(cur, pp, pc, s1, s0 + lane, kWave);
+        gather_group(tc, cur, Tt);
+        load_group(nxt, pp, pc, s1, s0 + G + lane, kWave);
+        for (uint32_t base = s0; base < s1; base += G) {
+            gather_group(tn, nxt, Tt);
+            SegWindows later;
+            load_group(later, pp, pc, s1, base + 2 * G + lane, kWave);
+            segment_group(cur, tc, base, s0, s1, Wd, anch, rv, ra, rmask, lane);
+            cur = nxt;
+            nxt = later;
+#pragma unroll
+            for (int u = 0; u < kSegUnroll; ++u) tc[u] = tn[u];
         }
     }
     __syncthreads();
