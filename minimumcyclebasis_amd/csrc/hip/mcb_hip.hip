@@ -88,6 +88,10 @@ constexpr int kLdsLimit = 64 * 1024;
 #define MCB_BIG_WINDOWS 32
 #endif
 constexpr int kUnroll = MCB_UNROLL;           // tree windows per pipelined load group
+#ifndef MCB_TW_UNROLL
+#define MCB_TW_UNROLL 2
+#endif
+constexpr int kTwUnroll = MCB_TW_UNROLL;      // windows per group of a one-wave tree walk
 constexpr int kBlockWaves = MCB_BLOCK_WAVES;  // waves per block of the parity/search kernel
 constexpr uint32_t kBigPositions = 64u * MCB_BIG_WINDOWS;  // larger trees are shared by a whole block
 
@@ -916,17 +920,15 @@ __device__ __forceinline__ void settle_windows(uint64_t (&rel)[kU], uint32_t (&a
     }
 }
 
-// Path words of kUnroll windows of one wave's tree: word[pos] = word[parent] ^ Tt[coordinate].
-// kRing: Wd is global scratch and ring (rmask + 1 words of LDS, a power of two of at least two
-// windows) keeps the most recent windows' words, so a parent in one of them is read from LDS.
-template <bool kRing>
-__device__ __forceinline__ void word_group(const WindowGroup &g, uint32_t base, uint32_t npos, const uint64_t *Tt,
-                                           uint64_t *Wd, uint64_t *ring, uint32_t rmask, int lane) {
-    uint64_t t[kUnroll];
+// Path words of U windows of one wave's tree: word[pos] = word[parent] ^ Tt[coordinate], t[u] being
+// window u's Tt words (gathered while the group before ran).  kRing: Wd is global scratch and ring
+// (rmask + 1 words of LDS, a power of two of at least two windows) keeps the most recent windows'
+// words, so a parent in one of them is read from LDS.
+template <bool kRing, int U>
+__device__ __forceinline__ void word_group(const WindowGroupT<U> &g, const uint64_t (&t)[U], uint32_t base,
+                                           uint32_t npos, uint64_t *Wd, uint64_t *ring, uint32_t rmask, int lane) {
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) t[u] = Tt[g.co[u] >= 0 ? g.co[u] : 0];
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
+    for (int u = 0; u < U; ++u) {
         const uint32_t wb = base + u * kWave;
         if (wb >= npos) break;
         const bool v = wb + lane < npos;
@@ -949,18 +951,33 @@ __device__ __forceinline__ void word_group(const WindowGroup &g, uint32_t base, 
     }
 }
 
+template <int U>
+__device__ __forceinline__ void gather_windows(uint64_t (&t)[U], const WindowGroupT<U> &g, const uint64_t *Tt) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) t[u] = Tt[g.co[u] >= 0 ? g.co[u] : 0];
+}
+
+// One wave's walk of a tree, kTwUnroll windows per group and three groups in flight: the current
+// group's words, the next group's Tt words, the positions of the group after.
 template <bool kRing = false>
 __device__ __forceinline__ void tree_words(const uint32_t *pp, const int32_t *pc, uint32_t npos, const uint64_t *Tt,
                                            uint64_t *Wd, int lane, uint64_t *ring = nullptr, uint32_t rmask = 0) {
-    constexpr uint32_t G = kWave * kUnroll;
-    WindowGroup A, B;
-    load_group(A, pp, pc, npos, lane, kWave);
-    for (uint32_t base = 0; base < npos; base += 2 * G) {
-        load_group(B, pp, pc, npos, base + G + lane, kWave);
-        word_group<kRing>(A, base, npos, Tt, Wd, ring, rmask, lane);
-        if (base + G >= npos) break;
-        load_group(A, pp, pc, npos, base + 2 * G + lane, kWave);
-        word_group<kRing>(B, base + G, npos, Tt, Wd, ring, rmask, lane);
+    constexpr int U = kTwUnroll;
+    constexpr uint32_t G = kWave * U;
+    WindowGroupT<U> cur, nxt;
+    uint64_t tc[U], tn[U];
+    load_group(cur, pp, pc, npos, lane, kWave);
+    gather_windows(tc, cur, Tt);
+    load_group(nxt, pp, pc, npos, G + lane, kWave);
+    for (uint32_t base = 0; base < npos; base += G) {
+        gather_windows(tn, nxt, Tt);
+        WindowGroupT<U> later;
+        load_group(later, pp, pc, npos, base + 2 * G + lane, kWave);
+        word_group<kRing, U>(cur, tc, base, npos, Wd, ring, rmask, lane);
+        cur = nxt;
+        nxt = later;
+#pragma unroll
+        for (int u = 0; u < U; ++u) tc[u] = tn[u];
     }
 }
 
@@ -973,11 +990,6 @@ __device__ __forceinline__ void tree_words(const uint32_t *pp, const int32_t *pc
 // LDS read and write per window.
 constexpr int kSegUnroll = 2;  // windows per prefetched group of a segment walk (settled in lockstep)
 using SegWindows = WindowGroupT<kSegUnroll>;
-
-__device__ __forceinline__ void gather_group(uint64_t (&t)[kSegUnroll], const SegWindows &g, const uint64_t *Tt) {
-#pragma unroll
-    for (int u = 0; u < kSegUnroll; ++u) t[u] = Tt[g.co[u] >= 0 ? g.co[u] : 0];
-}
 
 __device__ __forceinline__ void segment_group(const SegWindows &g, const uint64_t (&t)[kSegUnroll], uint32_t base,
                                               uint32_t s0, uint32_t s1, uint64_t *Wd, uint32_t *anch, uint64_t *rv,
@@ -1051,10 +1063,10 @@ __device__ void tree_words_segmented(const uint32_t *pp, const int32_t *pc, uint
         SegWindows cur, nxt;
         uint64_t tc[kSegUnroll], tn[kSegUnroll];
         load_group(cur, pp, pc, s1, s0 + lane, kWave);
-        gather_group(tc, cur, Tt);
+        gather_windows(tc, cur, Tt);
         load_group(nxt, pp, pc, s1, s0 + G + lane, kWave);
         for (uint32_t base = s0; base < s1; base += G) {
-            gather_group(tn, nxt, Tt);
+            gather_windows(tn, nxt, Tt);
             SegWindows later;
             load_group(later, pp, pc, s1, base + 2 * G + lane, kWave);
             segment_group(cur, tc, base, s0, s1, Wd, anch, rv, ra, rmask, lane);
