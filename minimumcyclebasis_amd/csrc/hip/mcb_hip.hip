@@ -1768,8 +1768,8 @@ __device__ __forceinline__ void merge_stream(int64_t s0, int members, int64_t sl
 }
 
 // Wave c: the bitmap words [c * chunk_words, (c + 1) * chunk_words) (chunk_words a multiple of 64, at
-// most 64 * kMaxRounds) -> slot c of the first level (and the block's finalisation when that is the only
-// level).
+// most 64 * kMaxRounds) -> slot c of the first level; with one level the block's finalisation, and with
+// one merge above it that merge and the finalisation (levels above that: k_block_cascade).
 __global__ __launch_bounds__(256) void k_block_basis(BlockArgs a) {
     __shared__ uint64_t s_words[4][kMaxRounds * kWave];
     __shared__ uint32_t s_offs[4][kMaxRounds * kWave];
@@ -1784,6 +1784,25 @@ __global__ __launch_bounds__(256) void k_block_basis(BlockArgs a) {
     if (a.levels == 1) {
         __threadfence();
         block_finalize(a, lane);
+    } else if (a.levels == 2) {
+        // One merge above the first level (a few dozen bases): the last wave to finish merges them and
+        // finalises the block, so the block takes no launch of its own for the merge.  Each wave
+        // publishes its base with an agent-scope release fence before it counts itself; the last
+        // arrival clears the counter for the next block and takes an acquire fence before it reads the
+        // others' bases.
+        const int64_t dst = a.lvl_slot[1];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        uint32_t before = 0;
+        if (lane == 0) before = atomicAdd(a.arrive + dst, 1u);
+        before = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(before), 0));
+        if (before + 1 != static_cast<uint32_t>(a.lvl_n[0])) return;
+        if (lane == 0) atomicExch(a.arrive + dst, 0u);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        // this wave's bitmap staging is free: its first 256 words hold the reduction tables
+        merge_stream(a.lvl_slot[0], static_cast<int>(a.lvl_n[0]), dst, a.kk, a.sum_cnt, a.sum_rank, a.sum_q,
+                     s_words[wv], s_offs[wv], lane);
+        __threadfence();
+        block_finalize(a, lane);
     }
 }
 
@@ -1792,9 +1811,9 @@ __global__ __launch_bounds__(256) void k_block_basis(BlockArgs a) {
 // each merge group to finish merges that group: each wave publishes its merged base with an
 // agent-scope release fence before it counts itself in the group's arrival counter, and the last
 // arrival clears the counter for the next block and takes an acquire fence before it reads the
-// others' bases.  The wave that completes the top level finalises the block.  (The first level
-// stays a launch of its own: fences in its hundreds of waves, each writing back and invalidating
-// its L2, cost more than the launch.)
+// others' bases.  The wave that completes the top level finalises the block.  (Used when the block
+// has more than one merge level; the first level stays a launch of its own then, since fences in its
+// hundreds of waves, each writing back and invalidating its L2, cost more than the launch.)
 __global__ __launch_bounds__(64) void k_block_cascade(BlockArgs a) {
     __shared__ uint32_t s_offs[kWave];
     __shared__ uint64_t s_tab[kMergeTab];
@@ -3290,10 +3309,10 @@ int launch_block(mcb_solver *s, int32_t e, int32_t kk) {
     if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_PARITY_SEARCH))) return rc;
     if ((rc = launch_on(s, true, k_block_q, dim3(s->q_blocks), dim3(s->q_waves * kWave), s->q_lds, a))) return rc;
     if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_SELECT))) return rc;
-    if ((rc = launch_on(s, s->levels == 1, k_block_basis, dim3(static_cast<unsigned>((s->lvl_n[0] + 3) / 4)),
+    if ((rc = launch_on(s, s->levels <= 2, k_block_basis, dim3(static_cast<unsigned>((s->lvl_n[0] + 3) / 4)),
                         dim3(256), 0, a)))
         return rc;
-    if (s->levels > 1 &&
+    if (s->levels > 2 &&
         (rc = launch_on(s, true, k_block_cascade, dim3(static_cast<unsigned>(s->lvl_n[1])), dim3(kWave), 0, a)))
         return rc;
     if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_CYCLE))) return rc;
