@@ -3885,15 +3885,17 @@ int setup_blocked(mcb_solver *s, const int64_t *tree_offset, const uint32_t *par
         return rc;
     s->bk = want > 1 ? want : kBlockK;
     use_plan(s, 0);
-    // greedy-basis levels: chunk_words bitmap words per first-level basis, group bases per merge.  The
-    // bitmap is sparse (a block's tree pass emits a few thousand of millions of ranks), so by default
-    // a first-level basis reads as many rounds of 64 words as keep the first level within one merge
-    // group of 64: one merge then finishes the block, with no arrival between workgroups.  A chosen
-    // merge group keeps one round (4096 ranks) per first-level basis.
+    // greedy-basis levels: chunk_words bitmap words per first-level basis, group bases per merge.  When
+    // at most kMaxRounds rounds of 64 bitmap words per first-level basis bring the first level within
+    // one merge group of 64 (Delaunay 100k: 11 rounds), a first-level basis reads that many: the last
+    // one to finish then merges them all and finalises the block in the same launch.  Otherwise (and
+    // with a chosen merge group) one round, 4096 ranks, per first-level basis: a first-level wave
+    // reading many rounds of a dense bitmap (ER 20 000) inserts batch after batch alone, which costs
+    // more than the merge levels it saves.
     int64_t n = (s->K + kChunk - 1) / kChunk, slots = 0;
     s->chunk_words = kWave;
-    if (s->opt.merge_group == 0 && n > 64) {
-        const int64_t rounds = std::min<int64_t>(kMaxRounds, (n + 63) / 64);
+    if (s->opt.merge_group == 0 && n > 64 && n <= 64 * kMaxRounds) {
+        const int64_t rounds = (n + 63) / 64;
         s->chunk_words = static_cast<int32_t>(rounds * kWave);
         n = (s->K + static_cast<int64_t>(kChunk) * rounds - 1) / (static_cast<int64_t>(kChunk) * rounds);
     }
