@@ -144,6 +144,10 @@ typedef struct {
                                  phase: every timing_every-th phase; batch: step) records events, counted
                                  from the last reset; 0 or 1 = every one.  An event pair costs the stream a
                                  few microseconds, so a sample keeps a timed loop close to an untimed one. */
+    int32_t segment_resolve;  /* blocked tree pass, narrow trees in global scratch walked a segment per wave:
+                                 0 = resolve only the segments' last positions in order and join the rest at
+                                 the candidates' ends, where every segment's anchors allow it (the default);
+                                 1 = resolve every segment in order.  Both give the same words. */
 } mcb_solver_options;
 
 void mcb_solver_options_init(mcb_solver_options *opt);

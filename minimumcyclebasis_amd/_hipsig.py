@@ -35,6 +35,7 @@ class SolverOptions(ctypes.Structure):
         ("tree_cap", i32),
         ("merge_group", i32),
         ("timing_every", i32),
+        ("segment_resolve", i32),
     ]
 
 

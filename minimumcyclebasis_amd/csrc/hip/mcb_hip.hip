@@ -772,6 +772,7 @@ struct BlockArgs {
     int64_t K;
     int32_t e, kk, N, W, n_big, n_small, big_words, small_words, group;
     int32_t chunk_words;  // bitmap words per first-level basis
+    int32_t seg_in_order;  // narrow scratch trees: resolve every segment in order
 };
 
 // One block of kk phases of one problem in that problem's own numbering: its support matrix
@@ -993,7 +994,8 @@ using SegWindows = WindowGroupT<kSegUnroll>;
 
 __device__ __forceinline__ void segment_group(const SegWindows &g, const uint64_t (&t)[kSegUnroll], uint32_t base,
                                               uint32_t s0, uint32_t s1, uint64_t *Wd, uint32_t *anch, uint64_t *rv,
-                                              uint32_t *ra, uint32_t rmask, int lane) {
+                                              uint32_t *ra, uint32_t rmask, uint32_t near_lo, uint32_t seg0_end,
+                                              bool &far_anchor, int lane) {
     uint64_t rel[kSegUnroll];
     uint32_t anc[kSegUnroll];
     int jmp[kSegUnroll];
@@ -1040,23 +1042,29 @@ __device__ __forceinline__ void segment_group(const SegWindows &g, const uint64_
             anch[q] = an;
             rv[q & rmask] = val;
             ra[q & rmask] = an;
+            // a segment anchor outside segment 0 and before the previous segment's ring
+            far_anchor |= an != kNoParent && an >= seg0_end && an < near_lo;
         }
     }
 }
 
 // A narrow tree in scratch walked by all nw waves of the workgroup: wave s walks the positions
-// [s·seg, (s+1)·seg) relative to the segment (segment_group); then the segments after the first are
-// resolved in order, every thread taking positions of one segment at a time: the word of q is its
-// relative word XOR the final word of its anchor, which lies in an earlier segment.  The serial chain
-// is one segment's windows plus nw − 1 parallel passes, instead of all the tree's windows.
-__device__ void tree_words_segmented(const uint32_t *pp, const int32_t *pc, uint32_t npos, const uint64_t *Tt,
+// [s·seg, (s+1)·seg) relative to the segment (segment_group).  When every segment's anchors lie in segment 0
+// or in the previous segment's last T = rmask + 1 positions (still in that segment's LDS ring), only the rings
+// are resolved, segment after segment, in LDS (their final words and kNoParent anchors go to scratch); every
+// other position keeps its relative word and anchor, whose word is now final, and the candidate run joins the
+// two at the ends it reads (run_words with anch): returns true.  Otherwise the segments after the first are
+// resolved in order, every thread taking positions of one segment at a time (the word of q is its relative
+// word XOR the final word of its anchor), and it returns false.
+__device__ bool tree_words_segmented(const uint32_t *pp, const int32_t *pc, uint32_t npos, const uint64_t *Tt,
                                      uint64_t *Wd, uint32_t *anch, uint64_t *ring_base, uint32_t rmask, int wid,
-                                     int nw, int lane) {
+                                     int nw, bool in_order, int lane) {
     const uint32_t seg = ((npos + nw - 1) / nw + kWave - 1) / kWave * kWave;
     const uint32_t s0 = static_cast<uint32_t>(wid) * seg, s1 = min(npos, s0 + seg);
-    uint64_t *rv = ring_base + static_cast<size_t>(wid) * (rmask + 1);
-    uint32_t *ra = reinterpret_cast<uint32_t *>(ring_base + static_cast<size_t>(nw) * (rmask + 1)) +
-                   static_cast<size_t>(wid) * (rmask + 1);
+    const uint32_t T = rmask + 1;
+    uint64_t *rv = ring_base + static_cast<size_t>(wid) * T;
+    uint32_t *ra = reinterpret_cast<uint32_t *>(ring_base + static_cast<size_t>(nw) * T) + static_cast<size_t>(wid) * T;
+    bool far_anchor = false;
     if (s0 < s1) {
         // three groups in flight: this one's words, the next one's Tt words, the one after's positions
         constexpr uint32_t G = kWave * kSegUnroll;
@@ -1065,18 +1073,39 @@ __device__ void tree_words_segmented(const uint32_t *pp, const int32_t *pc, uint
         load_group(cur, pp, pc, s1, s0 + lane, kWave);
         gather_windows(tc, cur, Tt);
         load_group(nxt, pp, pc, s1, s0 + G + lane, kWave);
+        const uint32_t near_lo = s0 > T ? s0 - T : 0u;
         for (uint32_t base = s0; base < s1; base += G) {
             gather_windows(tn, nxt, Tt);
             SegWindows later;
             load_group(later, pp, pc, s1, base + 2 * G + lane, kWave);
-            segment_group(cur, tc, base, s0, s1, Wd, anch, rv, ra, rmask, lane);
+            segment_group(cur, tc, base, s0, s1, Wd, anch, rv, ra, rmask, near_lo, seg, far_anchor, lane);
             cur = nxt;
             nxt = later;
 #pragma unroll
             for (int u = 0; u < kSegUnroll; ++u) tc[u] = tn[u];
         }
     }
-    __syncthreads();
+    if (!__syncthreads_or(far_anchor || seg < T || in_order)) {
+        for (uint32_t sg = 1; sg < static_cast<uint32_t>(nw); ++sg) {
+            const uint32_t a0 = sg * seg;
+            if (a0 >= npos) break;
+            const uint32_t a1 = min(npos, a0 + seg);
+            const uint32_t t0 = a1 - a0 > T ? a1 - T : a0;
+            uint64_t *rw = ring_base + static_cast<size_t>(sg) * T;
+            uint32_t *rn = reinterpret_cast<uint32_t *>(ring_base + static_cast<size_t>(nw) * T) + static_cast<size_t>(sg) * T;
+            const uint64_t *pw = ring_base + static_cast<size_t>(sg - 1) * T;  // the previous segment's ring, final
+            for (uint32_t q = t0 + threadIdx.x; q < a1; q += blockDim.x) {
+                const uint32_t x = rn[q & rmask];
+                uint64_t w = rw[q & rmask];
+                if (x != kNoParent) w ^= x >= seg ? pw[x & rmask] : Wd[x];
+                rw[q & rmask] = w;
+                Wd[q] = w;
+                anch[q] = kNoParent;
+            }
+            __syncthreads();
+        }
+        return true;
+    }
     for (int sg = 1; sg < nw; ++sg) {
         const uint32_t a0 = static_cast<uint32_t>(sg) * seg;
         if (a0 >= npos) break;
@@ -1104,6 +1133,7 @@ __device__ void tree_words_segmented(const uint32_t *pp, const int32_t *pc, uint
         }
         __syncthreads();
     }
+    return false;
 }
 
 // The same for a big tree shared by all waves of the workgroup: window w belongs to wave w mod nw
@@ -1206,7 +1236,7 @@ __device__ __forceinline__ int basis_reduce_insert(uint64_t &piv, uint64_t &P, u
 // on earlier candidates overall, so it cannot be in the block's greedy basis and is dropped.
 template <typename A>
 __device__ __forceinline__ void run_words(const A &a, const uint64_t *Wd, int64_t k_begin, int64_t k_end, int kk,
-                                          int lane) {
+                                          int lane, const uint32_t *anch = nullptr) {
     constexpr int kB = 4;
     uint64_t piv = 0, P = 0;
     int npiv = 0;
@@ -1228,6 +1258,11 @@ __device__ __forceinline__ void run_words(const A &a, const uint64_t *Wd, int64_
             uint64_t q = 0;
             if (k < k_end) {
                 q = Wd[pu[b]] ^ Wd[pv[b]];
+                if (anch) {  // words relative to a segment's entry: XOR in the final word of each end's anchor
+                    const uint32_t xu = anch[pu[b]], xv = anch[pv[b]];
+                    if (xu != kNoParent) q ^= Wd[xu];
+                    if (xv != kNoParent) q ^= Wd[xv];
+                }
                 if (c[b] >= 0) q ^= a.Tt[c[b]];
             }
             npiv += basis_reduce_insert(piv, P, q, lane, [&]() {
@@ -1237,6 +1272,10 @@ __device__ __forceinline__ void run_words(const A &a, const uint64_t *Wd, int64_
         }
     }
 }
+
+__device__ __forceinline__ bool seg_in_order_of(const BlockArgs &a) { return a.seg_in_order != 0; }
+template <typename A>
+__device__ __forceinline__ bool seg_in_order_of(const A &) { return false; }
 
 // A small item of at most one window of positions and one batch of candidates.
 __device__ __forceinline__ bool is_tiny(const ItemMeta &m) {
@@ -1353,17 +1392,20 @@ __device__ __forceinline__ void tree_pass(const A &a, const ItemMeta *big, uint3
         // path words, then each wave takes one contiguous run of the item's candidates (the two calls
         // keep the LDS and the scratch accesses in their own address spaces)
         auto one_tree = [&](uint64_t *Wd, uint32_t *rd, bool in_scratch) {
+            uint32_t *seg_anch = nullptr;  // set while a segmented walk's words stay relative
             if (nar) {
                 // narrow, deep tree: windows mostly wait on the window before them, so one wave walks
                 // it alone (no handoffs between waves); the others wait at the barrier.  A tree in
                 // scratch keeps its latest windows in the idle LDS tree region as well (ring), so
                 // that the walk does not wait on global memory for every window.
                 if (in_scratch && seg_mask) {
-                    // all waves, a segment each, then the segments resolved in order
-                    uint32_t *anch = reinterpret_cast<uint32_t *>(a.scratch + static_cast<int64_t>(gridDim.x) * a.huge_words) +
-                                     static_cast<int64_t>(blockIdx.x) * a.huge_words;
-                    tree_words_segmented(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wd, anch, Wbig, seg_mask,
-                                         wid, nw, lane);
+                    // all waves, a segment each, then the segments' rings resolved in order (the rest at the
+                    // candidates' ends) or, failing that, every segment in order
+                    seg_anch = reinterpret_cast<uint32_t *>(a.scratch + static_cast<int64_t>(gridDim.x) * a.huge_words) +
+                               static_cast<int64_t>(blockIdx.x) * a.huge_words;
+                    if (!tree_words_segmented(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wd, seg_anch, Wbig,
+                                              seg_mask, wid, nw, seg_in_order_of(a), lane))
+                        seg_anch = nullptr;
                 } else if (wid == 0) {
                     if (in_scratch && ring_mask) {
                         tree_words<true>(a.parent_pos + off, a.pos_coord + off, npos, a.Tt, Wd, lane, Wbig, ring_mask);
@@ -1377,7 +1419,7 @@ __device__ __forceinline__ void tree_pass(const A &a, const ItemMeta *big, uint3
             __syncthreads();
             const int64_t seg = (ce - cb + nw - 1) / nw;
             const int64_t s0 = cb + seg * wid, s1 = s0 + seg < ce ? s0 + seg : ce;
-            if (s0 < s1) run_words(a, Wd, s0, s1, kk_of(r), lane);
+            if (s0 < s1) run_words(a, Wd, s0, s1, kk_of(r), lane, seg_anch);
             __syncthreads();
         };
         if (npos <= static_cast<uint32_t>(a.big_words)) {
@@ -3236,6 +3278,7 @@ BlockArgs block_args(const mcb_solver *s, int32_t e, int32_t kk) {
         a.scratch = tp.scratch;
         a.huge_words = tp.huge_words;
     }
+    a.seg_in_order = s->opt.segment_resolve == 1 ? 1 : 0;
     a.big_list = s->q_big_list;
     a.small_list = s->q_small_list;
     a.wg_big = s->q_wg_big;
@@ -3978,6 +4021,8 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
     if (opt && (opt->merge_group < 0 || opt->merge_group == 1 || opt->merge_group > 64))
         return fail(MCB_E_INVALID, "merge_group must be 0 or 2..64");
     if (opt && opt->timing_every < 0) return fail(MCB_E_INVALID, "timing_every must be 0 or more");
+    if (opt && (opt->segment_resolve < 0 || opt->segment_resolve > 1))
+        return fail(MCB_E_INVALID, "segment_resolve must be 0 or 1");
     rc = ensure_device();
     if (rc) return rc;
     mcb_solver *s = new (std::nothrow) mcb_solver();
@@ -4660,6 +4705,8 @@ int mcb_batch_create(const mcb_solver_desc *descs, int32_t count, const mcb_solv
     if (opt && (opt->merge_group < 0 || opt->merge_group == 1 || opt->merge_group > 64))
         return fail(MCB_E_INVALID, "merge_group must be 0 or 2..64");
     if (opt && opt->timing_every < 0) return fail(MCB_E_INVALID, "timing_every must be 0 or more");
+    if (opt && (opt->segment_resolve < 0 || opt->segment_resolve > 1))
+        return fail(MCB_E_INVALID, "segment_resolve must be 0 or 1");
     int rc = ensure_device();
     if (rc) return rc;
     mcb_batch *b = new (std::nothrow) mcb_batch();

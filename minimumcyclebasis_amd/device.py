@@ -63,13 +63,15 @@ class Solver:
 
     def __init__(self, problem: Problem, store_cycles: bool = False, timing: int | bool = False,
                  waves_per_block: int = 0, max_blocks: int = 0, block_phases: int = 0, tree_cap: int = 0,
-                 merge_group: int = 0, timing_every: int = 0):
+                 merge_group: int = 0, timing_every: int = 0, segment_resolve: int = 0):
         """timing: 0/False none, 1/True every kernel, 2 the tree pass and the update only (HIP events).
         timing_every: with timing on, only every timing_every-th block of phases records events (0, 1 all).
         block_phases: phases per pass over the trees (0 automatic, 1 one kernel per phase, up to 64).
         waves_per_block: waves per workgroup of the blocked tree pass (4, 8, 16; 0 timed at create).
         tree_cap: largest tree kept in LDS by the tree pass (0 timed at create, -1 as large as fits).
-        merge_group: first-level bases merged per wave of the greedy basis (0 automatic, 2..64)."""
+        merge_group: first-level bases merged per wave of the greedy basis (0 automatic, 2..64).
+        segment_resolve: narrow trees in scratch: 0 rings first where the anchors allow, 1 every segment in
+        order (the same words either way)."""
         info = problem.info
         if info.single_cycle:
             raise ValueError("a single-cycle graph has no phase loop")
@@ -89,6 +91,7 @@ class Solver:
         o.block_phases = int(block_phases)
         o.tree_cap = int(tree_cap)
         o.merge_group = int(merge_group)
+        o.segment_resolve = int(segment_resolve)
         o.timing_every = int(timing_every)
         h = ctypes.c_void_p()
         _lib.check_hip(_lib.hip().mcb_solver_create(ctypes.byref(d), ctypes.byref(o), ctypes.byref(h)))

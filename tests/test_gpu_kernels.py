@@ -292,6 +292,39 @@ def test_first_level_rounds_agree_with_one_round(n):
     p.close()
 
 
+def test_segmented_walk_resolves_agree():
+    """Delaunay 10k with the tree pass's LDS trees capped at 768 positions: its 75 larger trees are narrow and
+    walked in global scratch, a segment per wave of a 4-wave workgroup.  Resolving only each segment's last
+    positions in order and joining the rest at the candidates' ends (the default), and resolving every segment
+    in order, give the selections of the solver whose trees all stay in LDS; sampled phases pass the
+    oracle's checks."""
+    from fullsize import check_loop, sample_phases
+    from minimumcyclebasis_amd import generators as G
+    from minimumcyclebasis_amd.host import Problem
+
+    p = Problem.from_edges(*G.delaunay(10000, seed=3), prune_trees=True)
+    assert (np.diff(p.array("tree_offset")) > 768).sum() >= 50
+    print(f"case delaunay10000 K={p.info.num_candidates}", flush=True)
+    ref = Solver(p, waves_per_block=4, tree_cap=-1)
+    ref.run()
+    ref.sync()
+    want = ref.selected()
+    ref.close()
+    for mode in (0, 1):
+        print(f"case delaunay10000 cap 768 segment_resolve={mode}", flush=True)
+        s = Solver(p, waves_per_block=4, tree_cap=768, segment_resolve=mode)
+        s.run()
+        s.sync()
+        assert np.array_equal(s.selected(), want), f"segment_resolve={mode}"
+        if mode == 0:
+            check_loop(p, s.selected(), s.support_vector_rows, sample_phases(p.info.num_non_tree, spread=8),
+                       seed=9, label="delaunay10000 segmented")
+        s.close()
+    with pytest.raises(RuntimeError):
+        Solver(p, segment_resolve=2)
+    p.close()
+
+
 def test_merge_group_rejected():
     p = problem_of(load_trace("grid16x16"), "reduced", keep_vertex_trees=True, prune_trees=True)
     for bad in (1, 65, -2):
