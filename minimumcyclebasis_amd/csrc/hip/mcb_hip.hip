@@ -689,7 +689,7 @@ constexpr int kQWaves = 16;      // waves per workgroup of the tree pass
 constexpr int kChunk = 4096;     // ranks per first-level greedy basis (one candidate-bitmap word per lane)
 constexpr int kMaxRounds = 16;   // bitmap rounds of 64 words per first-level basis of the blocked form
 constexpr int64_t kXorWorkgroups = 2048;  // grid of k_block_xor (eight workgroups per CU)
-constexpr int64_t kDotWorkgroups = 512;   // grid of k_block_dot (two eight-wave workgroups per CU)
+constexpr int64_t kDotWorkgroups = 1024;  // grid of k_block_dot (four four-wave workgroups per CU)
 // The update's byte counter is spread over this many slots (slot = workgroup index mod kByteSlots), so
 // that the waves of a launch do not queue on one address; the host sums them.
 constexpr int kByteSlots = 64;
@@ -2136,7 +2136,7 @@ __global__ __launch_bounds__(256) void k_block_pairs(BlockArgs a) {
 // their pair offsets [kSegWords + 1].
 constexpr int kSegWords = 1024;
 constexpr int kSegPairs = 2048;
-constexpr int kDotWaves = 8;      // waves per product workgroup
+constexpr int kDotWaves = 4;      // waves per product workgroup
 constexpr int kDotCols = kWave;   // later columns per product tile
 
 // Column j's products d (bit i: <C_{e+i}, S_j>) into its coefficients b (the recurrence over the block's
