@@ -992,10 +992,31 @@ __device__ __forceinline__ void tree_words(const uint32_t *pp, const int32_t *pc
 constexpr int kSegUnroll = 2;  // windows per prefetched group of a segment walk (settled in lockstep)
 using SegWindows = WindowGroupT<kSegUnroll>;
 
+// A group's words and anchors for scratch, stored at the start of the next group: after the loads that the next
+// wait is for, so that the wait (the vector memory counter retires in order) does not wait on these stores.
+struct SegPending {
+    uint64_t val[kSegUnroll];
+    uint32_t an[kSegUnroll];
+    uint32_t base;  // kNoParent: nothing pending
+};
+
+__device__ __forceinline__ void flush_pending(const SegPending &pd, uint32_t s1, uint64_t *Wd, uint32_t *anch, int lane) {
+    if (pd.base == kNoParent) return;
+#pragma unroll
+    for (int u = 0; u < kSegUnroll; ++u) {
+        const uint32_t q = pd.base + u * kWave + lane;
+        if (q < s1) {
+            Wd[q] = pd.val[u];
+            anch[q] = pd.an[u];
+        }
+    }
+}
+
 __device__ __forceinline__ void segment_group(const SegWindows &g, const uint64_t (&t)[kSegUnroll], uint32_t base,
                                               uint32_t s0, uint32_t s1, uint64_t *Wd, uint32_t *anch, uint64_t *rv,
                                               uint32_t *ra, uint32_t rmask, uint32_t near_lo, uint32_t seg0_end,
-                                              bool &far_anchor, int lane) {
+                                              bool &far_anchor, SegPending &pd, int lane) {
+    flush_pending(pd, s1, Wd, anch, lane);
     uint64_t rel[kSegUnroll];
     uint32_t anc[kSegUnroll];
     int jmp[kSegUnroll];
@@ -1036,16 +1057,17 @@ __device__ __forceinline__ void segment_group(const SegWindows &g, const uint64_
             an = anch[x];
         }
         const uint64_t val = rel[u] ^ pw;
+        pd.val[u] = val;
+        pd.an[u] = an;
         if (v) {
             const uint32_t q = wb + lane;
-            Wd[q] = val;
-            anch[q] = an;
             rv[q & rmask] = val;
             ra[q & rmask] = an;
             // a segment anchor outside segment 0 and before the previous segment's ring
             far_anchor |= an != kNoParent && an >= seg0_end && an < near_lo;
         }
     }
+    pd.base = base;
 }
 
 // A narrow tree in scratch walked by all nw waves of the workgroup: wave s walks the positions
@@ -1074,16 +1096,19 @@ __device__ bool tree_words_segmented(const uint32_t *pp, const int32_t *pc, uint
         gather_windows(tc, cur, Tt);
         load_group(nxt, pp, pc, s1, s0 + G + lane, kWave);
         const uint32_t near_lo = s0 > T ? s0 - T : 0u;
+        SegPending pd;
+        pd.base = kNoParent;
         for (uint32_t base = s0; base < s1; base += G) {
             gather_windows(tn, nxt, Tt);
             SegWindows later;
             load_group(later, pp, pc, s1, base + 2 * G + lane, kWave);
-            segment_group(cur, tc, base, s0, s1, Wd, anch, rv, ra, rmask, near_lo, seg, far_anchor, lane);
+            segment_group(cur, tc, base, s0, s1, Wd, anch, rv, ra, rmask, near_lo, seg, far_anchor, pd, lane);
             cur = nxt;
             nxt = later;
 #pragma unroll
             for (int u = 0; u < kSegUnroll; ++u) tc[u] = tn[u];
         }
+        flush_pending(pd, s1, Wd, anch, lane);
     }
     if (!__syncthreads_or(far_anchor || seg < T || in_order)) {
         for (uint32_t sg = 1; sg < static_cast<uint32_t>(nw); ++sg) {
