@@ -1277,6 +1277,9 @@ __device__ __forceinline__ void run_words(const A &a, const uint64_t *Wd, int64_
             rk[b] = a.rc_rank[at];
             c[b] = a.rc_coord[at];
         }
+        // every batch's words first, then the insertions and their emissions: the vector memory counter
+        // retires in order, so a load issued after an emission's store would wait for that store
+        uint64_t qb[kB];
 #pragma unroll
         for (int b = 0; b < kB; ++b) {
             const int64_t k = k0 + b * kWave + lane;
@@ -1290,6 +1293,11 @@ __device__ __forceinline__ void run_words(const A &a, const uint64_t *Wd, int64_
                 }
                 if (c[b] >= 0) q ^= a.Tt[c[b]];
             }
+            qb[b] = q;
+        }
+#pragma unroll
+        for (int b = 0; b < kB; ++b) {
+            const uint64_t q = qb[b];
             npiv += basis_reduce_insert(piv, P, q, lane, [&]() {
                 a.Qr[rk[b]] = q;
                 atomicOr(reinterpret_cast<unsigned long long *>(a.qbits + (rk[b] >> 6)), 1ull << (rk[b] & 63));
