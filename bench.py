@@ -293,6 +293,7 @@ def time_leg(form: str, graph, args, threads: int) -> dict:
         pre["host_trees_preprocess_s"] = round(host_trees_s, 3)
     return {"form": form, "prob": prob, "solver": solver, "host_s": host_s, "upload_s": upload_s, "loop_s": loop_s,
             "kt": kt, "pos": pos, "cand": cand, "upd": upd, "sel": sel, "shape": solver.launch_shape(),
+            "grids": solver.update_grids(),
             "fvs": fvs_info, "total_weight": int(prob.array("cand_weight")[sel].astype(np.int64).sum()),
             "pre": pre}
 
@@ -377,6 +378,7 @@ def run_single(args) -> int:
         "block_phases": shape["block_phases"],
         "tree_pass_waves": shape["waves_per_block"],
         "tree_pass_lds": shape["lds_bytes"],
+        "update_grids": [head["grids"]["dot_workgroups"], head["grids"]["xor_workgroups"]],
         "launches_per_step": per_step,
         "timed_launches": k_n,
         "kernel_ms_per_step": {k: round(v[0] / v[1] * per_step, 3) for k, v in kt.items() if v[1]},

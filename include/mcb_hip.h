@@ -148,6 +148,10 @@ typedef struct {
                                  0 = resolve only the segments' last positions in order and join the rest at
                                  the candidates' ends, where every segment's anchors allow it (the default);
                                  1 = resolve every segment in order.  Both give the same words. */
+    int32_t dot_workgroups;   /* blocked form's support vector update: workgroup cap of the dot product
+                                 launch; 0 = timed at create over the first blocks of phases among 512, 1024
+                                 and 2048, 1..65536 = that cap.  Every cap gives the same results. */
+    int32_t xor_workgroups;   /* the same for the XOR launch, timed among 1024, 2048 and 4096 */
 } mcb_solver_options;
 
 void mcb_solver_options_init(mcb_solver_options *opt);
@@ -229,6 +233,10 @@ int mcb_solver_work_counters(mcb_solver *s, int64_t *positions, int64_t *candida
  * number of phases per block in use. */
 int mcb_solver_launch_shape(const mcb_solver *s, int32_t *blocks, int32_t *waves_per_block, int32_t *lds_bytes,
                             int32_t *block_phases);
+
+/* Workgroup caps of the support vector update's two launches in use (set or timed at create): no
+ * counterpart in the reference, which has no such launches. */
+int mcb_solver_update_grids(const mcb_solver *s, int32_t *dot_workgroups, int32_t *xor_workgroups);
 
 void mcb_solver_destroy(mcb_solver *s);
 

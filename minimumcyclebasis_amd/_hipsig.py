@@ -36,6 +36,8 @@ class SolverOptions(ctypes.Structure):
         ("merge_group", i32),
         ("timing_every", i32),
         ("segment_resolve", i32),
+        ("dot_workgroups", i32),
+        ("xor_workgroups", i32),
     ]
 
 
@@ -86,6 +88,7 @@ SIGNATURES = {
     "mcb_solver_work_counters": (ctypes.c_int, [P, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i64)]),
     "mcb_solver_launch_shape": (ctypes.c_int, [P, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32),
                                                ctypes.POINTER(i32)]),
+    "mcb_solver_update_grids": (ctypes.c_int, [P, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
     "mcb_solver_destroy": (None, [P]),
     "mcb_batch_create": (ctypes.c_int, [P, i32, ctypes.POINTER(SolverOptions), ctypes.POINTER(P)]),
     "mcb_batch_reset": (ctypes.c_int, [P]),

@@ -3153,6 +3153,9 @@ struct mcb_solver {
     hipGraph_t graph = nullptr;     // the whole loop, captured by mcb_solver_run_graph
     hipGraphExec_t graph_exec = nullptr;
     int64_t graph_positions = 0, graph_candidates = 0;  // tree pass work of one replay
+    int32_t dot_wg = static_cast<int32_t>(kDotWorkgroups), xor_wg = static_cast<int32_t>(kXorWorkgroups);  // update grids
+    hipEvent_t *tune_ev = nullptr;  // while the update grids are timed: events around each update launch
+    size_t tune_n = 0, tune_cap = 0;
     uint64_t *Tt = nullptr, *Qr = nullptr, *qbits = nullptr, *sum_q = nullptr, *blk_q = nullptr, *blk_m = nullptr, *blk_g = nullptr;
     uint64_t *C_blk = nullptr, *Sf_blk = nullptr, *Ct = nullptr, *Bc = nullptr, *pair_c = nullptr;
     uint32_t *pw_list = nullptr, *pw_off = nullptr, *pw_cnt = nullptr, *pair_i = nullptr;
@@ -3374,6 +3377,14 @@ BlockArgs block_args(const mcb_solver *s, int32_t e, int32_t kk) {
 }
 
 // One block of kk phases starting at e: transpose, tree pass, greedy basis, cycle vectors, update.
+// An event on the stream while the update grids are timed (four per block: before and after the dot
+// launch, before and after the XOR launch).
+int tune_mark(mcb_solver *s) {
+    if (!s->tune_ev || s->tune_n >= s->tune_cap) return MCB_OK;
+    HIP_TRY(hipEventRecord(s->tune_ev[s->tune_n++], s->stream));
+    return MCB_OK;
+}
+
 int launch_block(mcb_solver *s, int32_t e, int32_t kk) {
     const BlockArgs a = block_args(s, e, kk);
     const int64_t wl = (static_cast<int64_t>(e) + kk + kWave - 1) / kWave;
@@ -3401,22 +3412,25 @@ int launch_block(mcb_solver *s, int32_t e, int32_t kk) {
         return rc;
     if ((rc = timed_end(s)) || (rc = timed_begin(s, MCB_K_UPDATE))) return rc;
     const int64_t later = s->N - e - kk;
+    if ((rc = tune_mark(s))) return rc;
     if (later > 0) {
         const size_t dlds = sizeof(uint64_t) * (kBlockK + kSegPairs) + sizeof(uint32_t) * (kSegPairs + 2 * kSegWords + 1);
         // a fixed grid: a short word list is staged once per workgroup and its waves take tiles in turn;
         // a long one is taken tile by tile by whole workgroups
         if ((rc = launch_on(s, false, k_block_dot,
                             dim3(static_cast<unsigned>(std::min<int64_t>((later + kDotCols - 1) / kDotCols,
-                                                                         kDotWorkgroups))),
+                                                                         s->dot_wg))),
                             dim3(kDotWaves * kWave), dlds, a)))
             return rc;
     }
+    if ((rc = tune_mark(s)) || (rc = tune_mark(s))) return rc;
     // (column tile, word tile) pairs over the block's own columns and the listed later ones, dealt to at
-    // most kXorWorkgroups workgroups (the list's length is known only on the device)
+    // most xor_wg workgroups (the list's length is known only on the device)
     const int64_t most = (1 + (std::max<int64_t>(later, 0) + 255) / 256) * ((wl + kTile - 1) / kTile);
-    if ((rc = launch_on(s, true, k_block_xor, dim3(static_cast<unsigned>(std::min<int64_t>(most, kXorWorkgroups))),
+    if ((rc = launch_on(s, true, k_block_xor, dim3(static_cast<unsigned>(std::min<int64_t>(most, s->xor_wg))),
                         dim3(256), sizeof(uint64_t) * 16 * 16 * kTile, a)))
         return rc;
+    if ((rc = tune_mark(s))) return rc;
     s->blk_positions += s->plans[s->plan].pos_cum.back();
     s->blk_candidates += s->plans[s->plan].cand_cum.back();
     return timed_end(s);
@@ -3937,6 +3951,71 @@ int pick_fastest(size_t nplans, hipStream_t stream, UseFn use, LaunchFn launch) 
     return MCB_OK;
 }
 
+// Grids of the support vector update: each of dot_workgroups and xor_workgroups that the options leave at
+// 0 is timed over the first (at most eight) blocks of phases, from the unit support vectors, among three
+// grid caps, and the fastest kept; the other keeps its option's value.  A run with the middle caps warms up
+// first and counts for nothing.  Problems of fewer than two blocks keep the defaults.  Every grid gives the
+// same results (both kernels stride over their tiles); only the time differs.  Leaves the solver reset.
+int tune_update_grids(mcb_solver *s) {
+    static const int32_t kDotTry[3] = {512, static_cast<int32_t>(kDotWorkgroups), 2048};
+    static const int32_t kXorTry[3] = {1024, static_cast<int32_t>(kXorWorkgroups), 4096};
+    const bool td = s->opt.dot_workgroups == 0, tx = s->opt.xor_workgroups == 0;
+    s->dot_wg = td ? kDotTry[1] : s->opt.dot_workgroups;
+    s->xor_wg = tx ? kXorTry[1] : s->opt.xor_workgroups;
+    const int32_t nb = s->bk > 1 ? std::min<int32_t>(8, s->N / s->bk) : 0;
+    if ((!td && !tx) || nb < 2) return MCB_OK;
+    std::vector<hipEvent_t> ev(4 * static_cast<size_t>(nb), nullptr);
+    int rc = MCB_OK;
+    for (auto &x : ev)
+        if (hipEventCreate(&x) != hipSuccess) {
+            rc = fail(MCB_E_DEVICE, "hipEventCreate");
+            break;
+        }
+    const int32_t timing = s->opt.timing;
+    s->opt.timing = 0;
+    double dt[3] = {0, 0, 0}, xt[3] = {0, 0, 0};
+    const int order[4] = {1, 0, 1, 2};
+    for (int r = 0; r < 4 && rc == MCB_OK; ++r) {
+        const int c = order[r];
+        if (td) s->dot_wg = kDotTry[c];
+        if (tx) s->xor_wg = kXorTry[c];
+        if ((rc = mcb_solver_reset(s))) break;
+        s->tune_ev = ev.data();
+        s->tune_n = 0;
+        s->tune_cap = ev.size();
+        rc = mcb_solver_run(s, nb * s->bk);
+        s->tune_ev = nullptr;
+        if (rc) break;
+        if (hipStreamSynchronize(s->stream) != hipSuccess || s->tune_n != ev.size()) {
+            rc = fail(MCB_E_DEVICE, "update grid timing");
+            break;
+        }
+        if (r == 0) continue;
+        for (int32_t b = 0; b < nb; ++b) {
+            float d = 0.0f, x = 0.0f;
+            if (hipEventElapsedTime(&d, ev[4 * b], ev[4 * b + 1]) != hipSuccess ||
+                hipEventElapsedTime(&x, ev[4 * b + 2], ev[4 * b + 3]) != hipSuccess) {
+                rc = fail(MCB_E_DEVICE, "update grid timing");
+                break;
+            }
+            dt[c] += d;
+            xt[c] += x;
+        }
+    }
+    s->opt.timing = timing;
+    for (auto &x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (rc) return rc;
+    int bd = 0, bx = 0;
+    for (int c = 1; c < 3; ++c) {
+        if (dt[c] < dt[bd]) bd = c;
+        if (xt[c] < xt[bx]) bx = c;
+    }
+    if (td) s->dot_wg = kDotTry[bd];
+    if (tx) s->xor_wg = kXorTry[bx];
+    return mcb_solver_reset(s);
+}
+
 // Buffers and launch shape of the blocked form.  Chooses the per-phase form instead when
 // block_phases asks for it or (block_phases 0) when no tree pass plan fits the LDS.
 int setup_blocked(mcb_solver *s, const int64_t *tree_offset, const uint32_t *parent_pos, const int64_t *rc_off) {
@@ -4056,6 +4135,9 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
     if (opt && opt->timing_every < 0) return fail(MCB_E_INVALID, "timing_every must be 0 or more");
     if (opt && (opt->segment_resolve < 0 || opt->segment_resolve > 1))
         return fail(MCB_E_INVALID, "segment_resolve must be 0 or 1");
+    if (opt && (opt->dot_workgroups < 0 || opt->dot_workgroups > 65536 || opt->xor_workgroups < 0 ||
+                opt->xor_workgroups > 65536))
+        return fail(MCB_E_INVALID, "dot_workgroups and xor_workgroups must be 0..65536");
     rc = ensure_device();
     if (rc) return rc;
     mcb_solver *s = new (std::nothrow) mcb_solver();
@@ -4195,6 +4277,7 @@ int mcb_solver_create(const mcb_solver_desc *d, const mcb_solver_options *opt, m
             });
         if (rc || (rc = mcb_solver_reset(s))) return cleanup(rc);
     }
+    if ((rc = tune_update_grids(s))) return cleanup(rc);
     *out = s;
     return MCB_OK;
 }
@@ -4437,6 +4520,13 @@ int mcb_solver_launch_shape(const mcb_solver *s, int32_t *blocks, int32_t *waves
     if (waves_per_block) *waves_per_block = blk ? s->q_waves : s->wpb;
     if (lds_bytes) *lds_bytes = blk ? s->q_lds : s->lds_bytes;
     if (block_phases) *block_phases = s->bk;
+    return MCB_OK;
+}
+
+int mcb_solver_update_grids(const mcb_solver *s, int32_t *dot_workgroups, int32_t *xor_workgroups) {
+    if (!s) return fail(MCB_E_INVALID, "invalid arguments");
+    if (dot_workgroups) *dot_workgroups = s->dot_wg;
+    if (xor_workgroups) *xor_workgroups = s->xor_wg;
     return MCB_OK;
 }
 

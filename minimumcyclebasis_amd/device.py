@@ -63,7 +63,8 @@ class Solver:
 
     def __init__(self, problem: Problem, store_cycles: bool = False, timing: int | bool = False,
                  waves_per_block: int = 0, max_blocks: int = 0, block_phases: int = 0, tree_cap: int = 0,
-                 merge_group: int = 0, timing_every: int = 0, segment_resolve: int = 0):
+                 merge_group: int = 0, timing_every: int = 0, segment_resolve: int = 0, dot_workgroups: int = 0,
+                 xor_workgroups: int = 0):
         """timing: 0/False none, 1/True every kernel, 2 the tree pass and the update only (HIP events).
         timing_every: with timing on, only every timing_every-th block of phases records events (0, 1 all).
         block_phases: phases per pass over the trees (0 automatic, 1 one kernel per phase, up to 64).
@@ -71,7 +72,9 @@ class Solver:
         tree_cap: largest tree kept in LDS by the tree pass (0 timed at create, -1 as large as fits).
         merge_group: first-level bases merged per wave of the greedy basis (0 automatic, 2..64).
         segment_resolve: narrow trees in scratch: 0 rings first where the anchors allow, 1 every segment in
-        order (the same words either way)."""
+        order (the same words either way).
+        dot_workgroups, xor_workgroups: workgroup caps of the support vector update's two launches (0 timed
+        at create over the first blocks, 1..65536 that cap; the same results either way)."""
         info = problem.info
         if info.single_cycle:
             raise ValueError("a single-cycle graph has no phase loop")
@@ -93,6 +96,8 @@ class Solver:
         o.merge_group = int(merge_group)
         o.segment_resolve = int(segment_resolve)
         o.timing_every = int(timing_every)
+        o.dot_workgroups = int(dot_workgroups)
+        o.xor_workgroups = int(xor_workgroups)
         h = ctypes.c_void_p()
         _lib.check_hip(_lib.hip().mcb_solver_create(ctypes.byref(d), ctypes.byref(o), ctypes.byref(h)))
         self._h = h
@@ -184,6 +189,12 @@ class Solver:
         v = [ctypes.c_int32() for _ in range(4)]
         _lib.check_hip(_lib.hip().mcb_solver_launch_shape(self._h, *[ctypes.byref(x) for x in v]))
         return dict(zip(("blocks", "waves_per_block", "lds_bytes", "block_phases"), (x.value for x in v)))
+
+    def update_grids(self) -> dict:
+        """Workgroup caps of the support vector update's launches in use (set or timed at create)."""
+        d, x = ctypes.c_int32(), ctypes.c_int32()
+        _lib.check_hip(_lib.hip().mcb_solver_update_grids(self._h, ctypes.byref(d), ctypes.byref(x)))
+        return {"dot_workgroups": d.value, "xor_workgroups": x.value}
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -325,6 +325,48 @@ def test_segmented_walk_resolves_agree():
     p.close()
 
 
+def test_update_grids_agree():
+    """Every workgroup cap of the support vector update's two launches gives the same selections and
+    support vectors: caps of one workgroup (every tile of a launch in turn), odd caps, the timed caps' range
+    and caps above any launch's tile count, against the solver with the caps timed at create (one of the
+    three timed values each); sampled phases pass the oracle's checks."""
+    from fullsize import check_loop, sample_phases
+    from minimumcyclebasis_amd import generators as G
+    from minimumcyclebasis_amd.host import Problem
+
+    small = problem_of(load_trace("grid16x16"), "reduced", keep_vertex_trees=True, prune_trees=True)
+    p = Problem.from_edges(*G.delaunay(10000, seed=5), prune_trees=True)
+    for prob, name, caps in ((small, "grid16x16", ((1, 1), (3, 5))),
+                             (p, "delaunay10000", ((7, 3), (512, 4096), (65536, 65536)))):
+        print(f"case {name} update grids timed", flush=True)
+        ref = Solver(prob)
+        g = ref.update_grids()
+        assert g["dot_workgroups"] in (512, 1024, 2048) and g["xor_workgroups"] in (1024, 2048, 4096), g
+        ref.run()
+        ref.sync()
+        want = ref.selected()
+        rows = np.unique(np.linspace(0, prob.info.num_non_tree - 1, 40).astype(np.int32))
+        want_s = ref.support_vector_rows(rows)
+        ref.close()
+        for d, x in caps:
+            print(f"case {name} dot_workgroups={d} xor_workgroups={x}", flush=True)
+            s = Solver(prob, dot_workgroups=d, xor_workgroups=x)
+            assert s.update_grids() == {"dot_workgroups": d, "xor_workgroups": x}
+            s.run()
+            s.sync()
+            assert np.array_equal(s.selected(), want), (name, d, x)
+            assert np.array_equal(s.support_vector_rows(rows), want_s), (name, d, x)
+            if (d, x) == (7, 3):
+                check_loop(prob, s.selected(), s.support_vector_rows,
+                           sample_phases(prob.info.num_non_tree, spread=8), seed=11, label=f"{name} grids {d},{x}")
+            s.close()
+    for bad in ((-1, 0), (0, 65537)):
+        with pytest.raises(RuntimeError):
+            Solver(small, dot_workgroups=bad[0], xor_workgroups=bad[1])
+    small.close()
+    p.close()
+
+
 def test_merge_group_rejected():
     p = problem_of(load_trace("grid16x16"), "reduced", keep_vertex_trees=True, prune_trees=True)
     for bad in (1, 65, -2):
