@@ -1599,24 +1599,22 @@ __device__ void finalize_core(int kk, const uint32_t *sum_cnt, const uint32_t *s
         // kk are zero)
         uint64_t col = transpose64(q, lane);
         uint64_t m = lane < kk ? (1ull << lane) : 0ull;
-        uint64_t taken = 0;   // wave-uniform
-        uint64_t winbit = 0;  // lane i: phase i's winner as a bit of the rank positions
-        bool missing = false; // a column without an untaken bit (then the block fails)
         // Branch-free steps (columns at or above kk are zero, so they never take an update): the
-        // chain from one step to the next is a lane read, the lowest untaken bit and a masked XOR.
-#pragma unroll 4
+        // chain from one step to the next is a lane read, the column's lowest bit and a masked XOR.
+        // Step b clears its winner's bit from every later column, so column b holds no earlier
+        // winner's bit by its own step: its lowest bit is the lowest untaken one, and column b at
+        // the end (no later step changes it) gives phase b's winner.  A column without a bit at its
+        // step (zero at the end) means the block fails.
         for (int b = 0; b < kk; ++b) {
             const uint64_t cb = rl64(col, b);
-            const uint64_t rbit = lowbit(cb & ~taken);
-            missing |= rbit == 0;
-            taken |= rbit;
-            winbit = lane == b ? rbit : winbit;
+            const uint64_t rbit = lowbit(cb);
             const uint64_t mb = rl64(m, b);
             const bool hit = lane > b && (col & rbit) != 0;
             m ^= hit ? mb : 0ull;
             col ^= hit ? cb : 0ull;
         }
-        ok = !missing;
+        ok = !__ballot(lane < kk && col == 0);
+        const uint64_t winbit = lane < kk ? lowbit(col) : 0ull;  // lane i: phase i's winner as a bit of the rank positions
         const int win = winbit ? __ffsll(static_cast<unsigned long long>(winbit)) - 1 : 0;
         if (ok) {
             const uint32_t qlo = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(q)), win));
