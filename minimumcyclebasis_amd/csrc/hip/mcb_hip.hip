@@ -1207,6 +1207,31 @@ __device__ __forceinline__ void tree_words_coop(const uint32_t *pp, const int32_
     }
 }
 
+// New pivots from a batch already reduced by the pivots (lane i: word v), one per step, without
+// branches inside the step: the lowest nonzero lane p gives its word, whose lowest bit c is a new
+// pivot column; lane c's pivot becomes that word, and every other pivot and every lane whose word
+// has bit c take it (so every pivot stays zero in every other pivot column, and lane p's word becomes
+// zero).  The chain from one ballot to the next is a lane read, two bit scans and masked XORs.
+// Returns whether this lane became a pivot; added counts the new pivots (wave-uniform).
+__device__ __forceinline__ bool pivot_steps(uint64_t &piv, uint64_t &P, uint64_t v, int lane, int &added) {
+    bool mine = false;
+    uint64_t nz = __ballot(v != 0);
+    while (nz) {
+        const int p = __builtin_ctzll(nz);
+        const uint64_t pvv = rl64(v, p);  // nonzero: lane p's word
+        const int c = __builtin_ctzll(pvv);
+        mine = mine || lane == p;
+        const bool vb = ((v >> c) & 1ull) != 0;
+        const bool pb = ((piv >> c) & 1ull) != 0;
+        piv = lane == c ? pvv : (pb ? piv ^ pvv : piv);
+        P |= 1ull << c;
+        ++added;
+        v ^= vb ? pvv : 0ull;
+        nz = __ballot(v != 0);
+    }
+    return mine;
+}
+
 // Greedy insertion of one batch (lane i: word q, ranks ascending with the lane; q = 0 for no entry)
 // into a wave basis kept in reduced echelon form: lane b holds the pivot of column b (P: the pivot
 // columns), and every pivot is zero in every other pivot column.  The batch is reduced by the
@@ -1222,31 +1247,16 @@ __device__ __forceinline__ int basis_reduce_insert(uint64_t &piv, uint64_t &P, u
     if (__ballot((v & P) != 0)) {
         uint64_t r = 0;
         for (uint64_t pm = P; pm; pm &= pm - 1) {
-            const uint64_t cb = lowbit(pm);
-            const uint64_t pc = rl64(piv, __ffsll(static_cast<unsigned long long>(cb)) - 1);
-            r ^= (v & cb) ? pc : 0ull;
+            const int c = __builtin_ctzll(pm);
+            const uint64_t pc = rl64(piv, c);
+            r ^= ((v >> c) & 1ull) ? pc : 0ull;
         }
         v ^= r;
     }
-    // One new pivot per step, without branches inside the step: the chain from one ballot to the
-    // next is a lane read, the pivot row's lowest bit and one masked XOR.  A lane becomes a pivot at
-    // most once (its own word is cleared then), so its emit() can wait until the loop ends.
-    const uint64_t me = 1ull << lane;
-    uint64_t mine = 0;
-    uint64_t nz = __ballot(v != 0);
+    // A lane becomes a pivot at most once (its own word is cleared then), so its emit() can wait
+    // until the loop ends.
     int added = 0;
-    while (nz) {
-        const uint64_t pbit = lowbit(nz);
-        const int p = __ffsll(static_cast<unsigned long long>(pbit)) - 1;
-        const uint64_t pvv = rl64(v, p);
-        const uint64_t cbit = lowbit(pvv);
-        mine |= me & pbit;
-        piv = (me & cbit) ? pvv : ((piv & cbit) ? piv ^ pvv : piv);
-        P |= cbit;
-        ++added;
-        v ^= (v & cbit) ? pvv : 0ull;
-        nz = __ballot(v != 0);
-    }
+    const bool mine = pivot_steps(piv, P, v, lane, added);
     P = static_cast<uint64_t>(uniform64(static_cast<int64_t>(P)));
     if (mine) emit();
     return added;
@@ -1550,21 +1560,9 @@ __device__ __forceinline__ void basis_insert_tab(uint64_t &piv, uint64_t &P, uin
         for (int g = 0; g < 16; ++g) r ^= tab[g * 16 + static_cast<int>((v >> (4 * g)) & 15ull)];
         v ^= r;
     }
-    const uint64_t me = 1ull << lane;
-    uint64_t mine = 0;
-    uint64_t nz = __ballot(v != 0);
-    if (nz) stale = true;
-    while (nz) {  // as in basis_reduce_insert
-        const uint64_t pbit = lowbit(nz);
-        const int p = __ffsll(static_cast<unsigned long long>(pbit)) - 1;
-        const uint64_t pvv = rl64(v, p);
-        const uint64_t cbit = lowbit(pvv);
-        mine |= me & pbit;
-        piv = (me & cbit) ? pvv : ((piv & cbit) ? piv ^ pvv : piv);
-        P |= cbit;
-        v ^= (v & cbit) ? pvv : 0ull;
-        nz = __ballot(v != 0);
-    }
+    if (__ballot(v != 0)) stale = true;
+    int added = 0;
+    const bool mine = pivot_steps(piv, P, v, lane, added);
     // each new pivot lane writes its own entry, after the new pivots of the lanes before it
     const uint64_t newp = __ballot(mine != 0);
     if (mine) {
