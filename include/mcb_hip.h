@@ -150,8 +150,9 @@ typedef struct {
                                  1 = resolve every segment in order.  Both give the same words. */
     int32_t dot_workgroups;   /* blocked form's support vector update: workgroup cap of the dot product
                                  launch; 0 = timed at create over the first blocks of phases among 512, 1024
-                                 and 2048, 1..65536 = that cap.  Every cap gives the same results. */
-    int32_t xor_workgroups;   /* the same for the XOR launch, timed among 1024, 2048 and 4096 */
+                                 and 2048 (1024 without timing below 256 blocks of phases), 1..65536 = that
+                                 cap.  Every cap gives the same results. */
+    int32_t xor_workgroups;   /* the same for the XOR launch, timed among 1024, 2048 and 4096 (2048) */
 } mcb_solver_options;
 
 void mcb_solver_options_init(mcb_solver_options *opt);

@@ -3950,8 +3950,9 @@ int pick_fastest(size_t nplans, hipStream_t stream, UseFn use, LaunchFn launch) 
 // Grids of the support vector update: each of dot_workgroups and xor_workgroups that the options leave at
 // 0 is timed over the first (at most eight) blocks of phases, from the unit support vectors, among three
 // grid caps, and the fastest kept; the other keeps its option's value.  A run with the middle caps warms up
-// first and counts for nothing.  Problems of fewer than two blocks keep the defaults.  Every grid gives the
-// same results (both kernels stride over their tiles); only the time differs.  Leaves the solver reset.
+// first and counts for nothing.  Problems of fewer than 256 blocks keep the middle caps: there the timing
+// would cost more than a loop could gain from it.  Every grid gives the same results (both kernels stride
+// over their tiles); only the time differs.  Leaves the solver reset.
 int tune_update_grids(mcb_solver *s) {
     static const int32_t kDotTry[3] = {512, static_cast<int32_t>(kDotWorkgroups), 2048};
     static const int32_t kXorTry[3] = {1024, static_cast<int32_t>(kXorWorkgroups), 4096};
@@ -3959,7 +3960,7 @@ int tune_update_grids(mcb_solver *s) {
     s->dot_wg = td ? kDotTry[1] : s->opt.dot_workgroups;
     s->xor_wg = tx ? kXorTry[1] : s->opt.xor_workgroups;
     const int32_t nb = s->bk > 1 ? std::min<int32_t>(8, s->N / s->bk) : 0;
-    if ((!td && !tx) || nb < 2) return MCB_OK;
+    if ((!td && !tx) || s->bk <= 1 || s->N / s->bk < 256) return MCB_OK;
     std::vector<hipEvent_t> ev(4 * static_cast<size_t>(nb), nullptr);
     int rc = MCB_OK;
     for (auto &x : ev)
